@@ -1,0 +1,169 @@
+/*
+ * C ABI of libnegf_hip.so: batched phonon-NEGF transport on MI355X (gfx950).
+ *
+ * Plain pointers and sizes only.  Every array argument is DEVICE memory
+ * (hipMalloc'd or a PyTorch-ROCm tensor); complex arrays are complex128
+ * interleaved (re, im), row-major.  Every call is stream-ordered on `stream`
+ * (a hipStream_t, NULL = default stream) and returns before the work finishes.
+ * Return value: NEGF_OK or an NEGF_ERR_* code; negf_last_error() (thread
+ * local) describes the last failure.
+ *
+ * One "instance" is one (omega, k_par, eta) point.  Blocks that depend only on
+ * k_par are stored once per "set" and instances point to their set through
+ * set_of[].  Per-instance outcomes are reported in int32 status arrays:
+ *   0 ok, 1 decimation did not converge within maxiter, 2 non-finite values,
+ *   3 exactly singular pivot in a lead inverse,
+ *   12 / 13 non-finite / singular in the device sweep.
+ *
+ * The entry points replace the Python-level hot path of the reference package
+ * negfpy (paths relative to its root); the ctypes binding is
+ * negfpy_amd/_lib.py.
+ */
+#ifndef NEGF_HIP_H
+#define NEGF_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NEGF_ABI_VERSION 1
+
+enum { NEGF_FP64 = 0, NEGF_MIXED = 1 };
+enum { NEGF_OK = 0, NEGF_ERR_ARG = 1, NEGF_ERR_HIP = 2 };
+
+/* ABI version (NEGF_ABI_VERSION). */
+int negf_version(void);
+
+/* Description of the last error on this thread. */
+const char* negf_last_error(void);
+
+/* Largest supported principal-layer block size for a precision. */
+int negf_max_block(int precision);
+
+/*
+ * Surface Green's function of semi-infinite leads by Lopez-Sancho decimation.
+ * Replaces surface_gf_sancho_rubio, src/negfpy/core/surface_gf.py:18-81
+ * (and the per-k loop of lead_surface_dos*, src/negfpy/core/dos.py:31-251).
+ *   lead    [n_sets][3][n][n]  d00, d01, d10 of each set (d10 already resolved)
+ *   set_of  [n_inst]           set index of each instance
+ *   omega, eta [n_inst]        z = (omega + i eta)^2
+ *   tol, maxiter               stop when max(|alpha|, |beta|) < tol
+ *   refine_steps               NEGF_MIXED only: FP64 residual steps after the FP32 decimation
+ *   g_out   [n_inst][n][n]     g_s = (z - eps_s)^-1
+ *   dos_out [n_inst] or NULL   (omega/pi) Re Tr[i (g - g^H)]
+ *   iters, status [n_inst]
+ */
+int negf_sancho_rubio_batched(void* stream, int precision, int n, int n_inst, const double* lead,
+                              const int32_t* set_of, const double* omega, const double* eta,
+                              double tol, int maxiter, int refine_steps, double* g_out,
+                              double* dos_out, int32_t* iters, int32_t* status);
+
+/*
+ * Contact self-energy Sigma = V g V^H scattered into an n_dev x n_dev end-layer
+ * block at positions idx[0..n_c).  Replaces _self_energy_contact and the
+ * contact embedding, src/negfpy/core/negf.py:72-97 and :245-248.
+ *   v [n_sets][n_c][n_lead], g [n_inst][n_lead][n_lead], sigma_out [n_inst][n_dev][n_dev]
+ *   status_in [n_inst] or NULL: instances with a nonzero status are skipped.
+ */
+int negf_self_energy_batched(void* stream, int n_lead, int n_c, int n_dev, int n_inst,
+                             const double* v, const int32_t* set_of, const double* g,
+                             const int32_t* idx, const int32_t* status_in, double* sigma_out);
+
+/*
+ * Block recursive Green's function sweep for G_{1N} of
+ * A = (omega + i eta_dev)^2 - D - Sigma_L - Sigma_R (D block tridiagonal, lower
+ * blocks = coupling^H).  Replaces the sparse assembly and SuperLU solve for the
+ * right-contact columns, src/negfpy/core/negf.py:53-69, :229-250, :350-354.
+ *   onsite [n_sets][n_layers][n][n], coupling [n_sets][n_layers-1][n][n]
+ *   sigma_l/sigma_r [n_inst][n][n] (from negf_self_energy_batched)
+ *   g1n_out [n_inst][n][n]; status [n_inst] (12/13 on device failure, or status_in)
+ */
+int negf_rgf_g1n_batched(void* stream, int precision, int n, int n_layers, int n_inst,
+                         const double* onsite, const double* coupling, const int32_t* set_of,
+                         const double* omega, const double* eta_dev, const double* sigma_l,
+                         const double* sigma_r, const int32_t* status_in, double* g1n_out,
+                         int32_t* status);
+
+/*
+ * T = Re Tr[Gamma_L G_1N Gamma_R G_1N^H], Gamma = i (Sigma - Sigma^H), in FP64.
+ * Replaces _broadening and the trace, src/negfpy/core/negf.py:115-116, :346-357.
+ * Instances with status_in != 0 get NaN.
+ */
+int negf_transmission_trace_batched(void* stream, int n, int n_inst, const double* sigma_l,
+                                    const double* sigma_r, const double* g1n,
+                                    const int32_t* status_in, double* t_out);
+
+/* Copy st_r into st_l wherever st_l is 0 (left-lead failure takes precedence). */
+int negf_merge_status(void* stream, int n_inst, int32_t* st_l, const int32_t* st_r);
+
+/* One batched transmission problem (see negf_transmission_batched). */
+typedef struct negf_transmission_problem {
+  int precision;    /* NEGF_FP64 or NEGF_MIXED */
+  int n;            /* device principal-layer block size */
+  int n_layers;     /* device layers */
+  int n_inst;       /* instances */
+  int n_lead_l;     /* left lead block size */
+  int n_lead_r;     /* right lead block size */
+  int n_c_l;        /* left contact size (rows of v_l) */
+  int n_c_r;        /* right contact size (rows of v_r) */
+  int share_leads;  /* 1: the right lead equals the left one, one decimation serves both */
+  int maxiter;      /* decimation iteration limit */
+  int refine_steps; /* NEGF_MIXED: FP64 residual steps per surface GF */
+  double tol;       /* decimation tolerance */
+  const double* lead_l;   /* [n_sets][3][n_lead_l][n_lead_l] */
+  const double* lead_r;   /* [n_sets][3][n_lead_r][n_lead_r] */
+  const double* v_l;      /* [n_sets][n_c_l][n_lead_l] device-to-lead coupling */
+  const double* v_r;      /* [n_sets][n_c_r][n_lead_r] */
+  const int32_t* idx_l;   /* [n_c_l] contact positions inside the first layer */
+  const int32_t* idx_r;   /* [n_c_r] contact positions inside the last layer */
+  const double* onsite;   /* [n_sets][n_layers][n][n] */
+  const double* coupling; /* [n_sets][n_layers-1][n][n] */
+  const int32_t* set_of;  /* [n_inst] */
+  const double* omega;    /* [n_inst] */
+  const double* eta_lead; /* [n_inst] */
+  const double* eta_dev;  /* [n_inst] */
+  double* t_out;          /* [n_inst] transmission */
+  double* dos_l;          /* [n_inst] left-lead surface DOS, or NULL */
+  double* dos_r;          /* [n_inst] right-lead surface DOS, or NULL (unused when shared) */
+  int32_t* iters_l;       /* [n_inst] decimation steps, left lead */
+  int32_t* iters_r;       /* [n_inst] decimation steps, right lead (unused when shared) */
+  int32_t* status;        /* [n_inst] */
+} negf_transmission_problem;
+
+/* Device workspace needed by negf_transmission_batched. */
+size_t negf_transmission_workspace_bytes(const negf_transmission_problem* p);
+
+/*
+ * Full (omega, k_par) transmission solve for a batch: both surface GFs, both
+ * self-energies, the device sweep and the trace.  Replaces transmission(),
+ * src/negfpy/core/negf.py:306-357, and the k loops of transmission_kavg*,
+ * :360-688, which call it once per point.
+ */
+int negf_transmission_batched(void* stream, const negf_transmission_problem* p, void* workspace,
+                              size_t workspace_bytes);
+
+/*
+ * Thermal conductance G(T_i) = (1/2pi) int k_B T(w) x^2 e^-x / (1 - e^-x)^2 dw,
+ * x = hbar w / (k_B T_i), trapezoid rule over the given grid (w in rad/s);
+ * 0 for T_i <= 0.  Replaces _thermal_conductance_from_spectrum,
+ * src/negfpy/workflows/ifc_bulk.py:408-438.
+ */
+int negf_thermal_conductance(void* stream, int n_w, const double* omega_rad_s, const double* trans,
+                             int n_t, const double* temps, double* out);
+
+/*
+ * Landauer heat current J = (1/2pi) int [k_B T_L phi(x_L) - k_B T_R phi(x_R)] T(w) dw,
+ * phi(x) = x / (e^x - 1).  Replaces heat_current_from_spectrum,
+ * src/negfpy/core/thermal_current.py:30-67.  out: one double.
+ */
+int negf_heat_current(void* stream, int n_w, const double* omega_rad_s, const double* trans,
+                      double temp_left, double temp_right, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEGF_HIP_H */

@@ -1,0 +1,254 @@
+"""Batched GPU engine: packs block sets into device tensors and drives the HIP
+library through its C ABI.
+
+A *set* holds everything that depends on k_par only (lead blocks, device
+blocks, device-to-lead couplings); an *instance* is one (omega, k_par, eta)
+point that refers to its set.  All per-set blocks are uploaded once and reused
+by every omega of that k_par.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from negfpy_amd import _lib
+
+PRECISIONS = {"fp64": _lib.FP64, "mixed": _lib.MIXED}
+DEFAULT_REFINE_STEPS = 2
+
+
+def precision_code(precision: str) -> int:
+    try:
+        return PRECISIONS[precision]
+    except KeyError:
+        raise ValueError(f"precision must be one of {sorted(PRECISIONS)}, got {precision!r}") from None
+
+
+def device() -> torch.device:
+    _lib.load()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def to_dev_c128(x) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return x.to(device=device(), dtype=torch.complex128).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.complex128)).to(device())
+
+
+def to_dev_f64(x) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return x.to(device=device(), dtype=torch.float64).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).to(device())
+
+
+def to_dev_i32(x) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return x.to(device=device(), dtype=torch.int32).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.int32)).to(device())
+
+
+# ---------------------------------------------------------------------------
+# surface Green's functions
+# ---------------------------------------------------------------------------
+
+@dataclass
+class SurfaceResult:
+    g: torch.Tensor        # (n_inst, n, n) complex128
+    dos: torch.Tensor      # (n_inst,) float64
+    iters: torch.Tensor    # (n_inst,) int32
+    status: torch.Tensor   # (n_inst,) int32
+
+
+def surface_gf_batch(leads, set_of, omega, eta, *, tol=1e-12, maxiter=200, precision="fp64",
+                     refine_steps=DEFAULT_REFINE_STEPS) -> SurfaceResult:
+    """Decimate every instance's lead.  ``leads`` is (n_sets, 3, n, n) = (d00, d01, d10)."""
+    lib = _lib.load()
+    leads = to_dev_c128(leads)
+    if leads.ndim != 4 or leads.shape[1] != 3 or leads.shape[2] != leads.shape[3]:
+        raise ValueError("leads must have shape (n_sets, 3, n, n)")
+    n = int(leads.shape[2])
+    set_of = to_dev_i32(set_of)
+    omega = to_dev_f64(omega)
+    eta = to_dev_f64(eta)
+    n_inst = int(omega.numel())
+    if set_of.numel() != n_inst or eta.numel() != n_inst:
+        raise ValueError("set_of, omega and eta must have one entry per instance")
+    dev = leads.device
+    g = torch.empty((n_inst, n, n), dtype=torch.complex128, device=dev)
+    dos = torch.empty((n_inst,), dtype=torch.float64, device=dev)
+    iters = torch.zeros((n_inst,), dtype=torch.int32, device=dev)
+    status = torch.zeros((n_inst,), dtype=torch.int32, device=dev)
+    _lib.check(lib.negf_sancho_rubio_batched(
+        _lib.stream(), precision_code(precision), n, n_inst, leads.data_ptr(), set_of.data_ptr(),
+        omega.data_ptr(), eta.data_ptr(), float(tol), int(maxiter), int(refine_steps),
+        g.data_ptr(), dos.data_ptr(), iters.data_ptr(), status.data_ptr()))
+    return SurfaceResult(g=g, dos=dos, iters=iters, status=status)
+
+
+# ---------------------------------------------------------------------------
+# transmission
+# ---------------------------------------------------------------------------
+
+@dataclass
+class TransportSets:
+    """Device tensors of one transport problem, one entry per k_par set."""
+
+    lead_l: torch.Tensor     # (S, 3, nl, nl)
+    lead_r: torch.Tensor     # (S, 3, nr, nr)
+    v_l: torch.Tensor        # (S, ncl, nl)
+    v_r: torch.Tensor        # (S, ncr, nr)
+    idx_l: torch.Tensor      # (ncl,) int32, positions in the first layer
+    idx_r: torch.Tensor      # (ncr,) int32, positions in the last layer
+    onsite: torch.Tensor     # (S, L, n, n)
+    coupling: torch.Tensor   # (S, max(L-1,1), n, n)
+    share_leads: bool
+
+    @property
+    def n(self) -> int:
+        return int(self.onsite.shape[2])
+
+    @property
+    def n_layers(self) -> int:
+        return int(self.onsite.shape[1])
+
+    @property
+    def n_sets(self) -> int:
+        return int(self.onsite.shape[0])
+
+    @classmethod
+    def from_arrays(cls, lead_l, lead_r, v_l, v_r, idx_l, idx_r, onsite, coupling,
+                    share_leads: bool) -> "TransportSets":
+        onsite = to_dev_c128(onsite)
+        s, nl_, n, _ = onsite.shape
+        if nl_ > 1:
+            coupling = to_dev_c128(coupling)
+        else:
+            coupling = torch.zeros((s, 1, n, n), dtype=torch.complex128, device=onsite.device)
+        lead_l = to_dev_c128(lead_l)
+        return cls(lead_l=lead_l, lead_r=lead_l if share_leads else to_dev_c128(lead_r),
+                   v_l=to_dev_c128(v_l), v_r=to_dev_c128(v_r), idx_l=to_dev_i32(idx_l),
+                   idx_r=to_dev_i32(idx_r), onsite=onsite, coupling=coupling,
+                   share_leads=bool(share_leads))
+
+
+@dataclass
+class TransmissionResult:
+    t: torch.Tensor        # (n_inst,) float64
+    status: torch.Tensor   # (n_inst,) int32
+    iters_l: torch.Tensor
+    iters_r: torch.Tensor
+    dos_l: torch.Tensor
+    dos_r: torch.Tensor
+
+
+class TransmissionSolver:
+    """Reusable batched solver: workspace is kept between calls of equal size."""
+
+    def __init__(self, sets: TransportSets, *, precision="fp64", tol=1e-12, maxiter=200,
+                 refine_steps=DEFAULT_REFINE_STEPS):
+        self.sets = sets
+        self.precision = precision_code(precision)
+        self.tol = float(tol)
+        self.maxiter = int(maxiter)
+        self.refine_steps = int(refine_steps)
+        self._ws = None
+        self._lib = _lib.load()
+
+    def _problem(self, n_inst, set_of, omega, eta_lead, eta_dev, out):
+        s = self.sets
+        p = _lib.TransmissionProblem()
+        p.precision = self.precision
+        p.n = s.n
+        p.n_layers = s.n_layers
+        p.n_inst = n_inst
+        p.n_lead_l = int(s.lead_l.shape[2])
+        p.n_lead_r = int(s.lead_r.shape[2])
+        p.n_c_l = int(s.v_l.shape[1])
+        p.n_c_r = int(s.v_r.shape[1])
+        p.share_leads = 1 if s.share_leads else 0
+        p.maxiter = self.maxiter
+        p.refine_steps = self.refine_steps
+        p.tol = self.tol
+        p.lead_l = s.lead_l.data_ptr()
+        p.lead_r = s.lead_r.data_ptr()
+        p.v_l = s.v_l.data_ptr()
+        p.v_r = s.v_r.data_ptr()
+        p.idx_l = s.idx_l.data_ptr()
+        p.idx_r = s.idx_r.data_ptr()
+        p.onsite = s.onsite.data_ptr()
+        p.coupling = s.coupling.data_ptr()
+        p.set_of = set_of.data_ptr()
+        p.omega = omega.data_ptr()
+        p.eta_lead = eta_lead.data_ptr()
+        p.eta_dev = eta_dev.data_ptr()
+        p.t_out = out.t.data_ptr()
+        p.dos_l = out.dos_l.data_ptr()
+        p.dos_r = out.dos_r.data_ptr()
+        p.iters_l = out.iters_l.data_ptr()
+        p.iters_r = out.iters_r.data_ptr()
+        p.status = out.status.data_ptr()
+        return p
+
+    def alloc_result(self, n_inst: int) -> TransmissionResult:
+        dev = self.sets.onsite.device
+        return TransmissionResult(
+            t=torch.empty((n_inst,), dtype=torch.float64, device=dev),
+            status=torch.zeros((n_inst,), dtype=torch.int32, device=dev),
+            iters_l=torch.zeros((n_inst,), dtype=torch.int32, device=dev),
+            iters_r=torch.zeros((n_inst,), dtype=torch.int32, device=dev),
+            dos_l=torch.zeros((n_inst,), dtype=torch.float64, device=dev),
+            dos_r=torch.zeros((n_inst,), dtype=torch.float64, device=dev))
+
+    def run(self, set_of, omega, eta_lead, eta_dev=None, out: TransmissionResult | None = None
+            ) -> TransmissionResult:
+        set_of = to_dev_i32(set_of)
+        omega = to_dev_f64(omega)
+        eta_lead = to_dev_f64(eta_lead)
+        eta_dev = eta_lead if eta_dev is None else to_dev_f64(eta_dev)
+        n_inst = int(omega.numel())
+        if not (set_of.numel() == eta_lead.numel() == eta_dev.numel() == n_inst):
+            raise ValueError("set_of, omega, eta_lead, eta_dev need one entry per instance")
+        if out is None:
+            out = self.alloc_result(n_inst)
+        if n_inst == 0:
+            return out
+        p = self._problem(n_inst, set_of, omega, eta_lead, eta_dev, out)
+        need = int(self._lib.negf_transmission_workspace_bytes(ctypes.byref(p)))
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = torch.empty((need,), dtype=torch.uint8, device=self.sets.onsite.device)
+        _lib.check(self._lib.negf_transmission_batched(
+            _lib.stream(), ctypes.byref(p), self._ws.data_ptr(), need))
+        if self.sets.share_leads:
+            out.iters_r.copy_(out.iters_l)
+            out.dos_r.copy_(out.dos_l)
+        return out
+
+
+# ---------------------------------------------------------------------------
+# omega integrals
+# ---------------------------------------------------------------------------
+
+def thermal_conductance(omega_rad_s, tvals, temps) -> np.ndarray:
+    lib = _lib.load()
+    w = to_dev_f64(omega_rad_s)
+    t = to_dev_f64(tvals)
+    tk = to_dev_f64(temps)
+    out = torch.empty((int(tk.numel()),), dtype=torch.float64, device=w.device)
+    _lib.check(lib.negf_thermal_conductance(_lib.stream(), int(w.numel()), w.data_ptr(),
+                                            t.data_ptr(), int(tk.numel()), tk.data_ptr(),
+                                            out.data_ptr()))
+    return out.cpu().numpy()
+
+
+def heat_current(omega_rad_s, tvals, temp_left, temp_right) -> float:
+    lib = _lib.load()
+    w = to_dev_f64(omega_rad_s)
+    t = to_dev_f64(tvals)
+    out = torch.empty((1,), dtype=torch.float64, device=w.device)
+    _lib.check(lib.negf_heat_current(_lib.stream(), int(w.numel()), w.data_ptr(), t.data_ptr(),
+                                     float(temp_left), float(temp_right), out.data_ptr()))
+    return float(out.cpu().item())

@@ -1,0 +1,508 @@
+"""Device Green's function and transmission on the GPU.
+
+Same entry points, arguments, defaults, conventions and exceptions as the
+reference ``src/negfpy/core/negf.py``:
+
+* energy variable z = (omega + i eta)^2 for the leads and
+  (omega + i eta_device)^2 for the device (negf.py:163-166, :245);
+* default couplings V_L = d10_L (or d01_L^H) and V_R = d01_R, Sigma = V g V^H
+  with both surface GFs from ``surface_gf(d00, d01, d10)`` (negf.py:192-201, :97);
+* ``omega_scale`` divides omega, eta and every block by s, s^2; the default
+  couplings are then divided by s^2 once more, exactly as negf.py:177-204 does;
+* default contacts are the first / last n_c device DOFs (negf.py:208-217);
+* T = Re Tr[Gamma_L G_LR Gamma_R G_LR^H] returned as a Python float (negf.py:356-357).
+
+Where the reference factors the whole device with SuperLU per point, this
+module packs the blocks of every k_par once into device tensors and solves all
+(omega, k_par) points of a call in one batched launch sequence (decimation,
+self-energies, block recursive Green's function sweep, FP64 trace).  Contacts
+that reach beyond the first/last principal layer are handled by merging
+layers into larger blocks.
+"""
+
+from __future__ import annotations
+
+from collections.abc import Callable
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _engine
+from ._status import transmission_value
+from .surface_gf import _prec, canonical_method, pack_lead
+from .types import Device1D, DeviceKSpace, KPar, LeadBlocks, LeadKSpace
+
+Array = np.ndarray
+LeadLike = LeadBlocks | LeadKSpace
+DeviceLike = Device1D | DeviceKSpace
+ContactIndices = slice | list[int] | tuple[int, ...] | Array | None
+DeviceToLeadLike = Array | Callable[[KPar], Array]
+
+
+def _resolve_lead_blocks(lead: LeadLike, kpar: KPar) -> LeadBlocks:
+    """negf.py:22-32."""
+    if isinstance(lead, LeadBlocks):
+        return lead
+    blocks = lead.blocks(kpar=kpar)
+    if len(blocks) == 2:
+        return LeadBlocks(d00=blocks[0], d01=blocks[1])
+    if len(blocks) == 3:
+        return LeadBlocks(d00=blocks[0], d01=blocks[1], d10=blocks[2])
+    raise ValueError("LeadKSpace blocks_builder must return (d00,d01) or (d00,d01,d10).")
+
+
+def _resolve_device(device: DeviceLike, kpar: KPar) -> Device1D:
+    if isinstance(device, Device1D):
+        return device
+    return device.device(kpar=kpar)
+
+
+def _resolve_coupling(coupling, kpar):
+    if coupling is None:
+        return None
+    if callable(coupling):
+        return np.asarray(coupling(kpar), dtype=np.complex128)
+    return np.asarray(coupling, dtype=np.complex128)
+
+
+def _normalize_contact_indices(indices: ContactIndices, *, dim: int, default: Array) -> Array:
+    """negf.py:119-138."""
+    if indices is None:
+        idx = np.asarray(default, dtype=int).ravel()
+    elif isinstance(indices, slice):
+        idx = np.arange(dim, dtype=int)[indices]
+    else:
+        idx = np.asarray(indices, dtype=int).ravel()
+    if idx.size == 0:
+        raise ValueError("Contact indices must be non-empty.")
+    if np.any(idx < 0) or np.any(idx >= dim):
+        raise ValueError("Contact indices out of device range.")
+    if np.unique(idx).size != idx.size:
+        raise ValueError("Contact indices must be unique.")
+    return idx
+
+
+@dataclass
+class KProblem:
+    """Everything of one k_par point, resolved, scaled and laid out for the kernels."""
+
+    lead_l: Array      # (3, nl, nl)
+    lead_r: Array      # (3, nr, nr)
+    v_l: Array         # (ncl, nl)
+    v_r: Array         # (ncr, nr)
+    pos_l: Array       # contact positions inside the first (merged) layer
+    pos_r: Array       # contact positions inside the last (merged) layer
+    onsite: Array      # (L', n', n')
+    coupling: Array    # (L'-1, n', n')
+    idx_l: Array       # contact indices in the original device numbering
+    idx_r: Array
+    scale: float       # omega_scale or 1
+    dim: int
+
+    def signature(self):
+        return (self.onsite.shape, self.lead_l.shape, self.lead_r.shape, self.v_l.shape,
+                self.v_r.shape, tuple(self.pos_l.tolist()), tuple(self.pos_r.tolist()))
+
+
+def _merge_layers(onsite, coupling, m):
+    """Group m consecutive layers into one block; pad the tail with decoupled zero DOFs."""
+    nl = len(onsite)
+    n = onsite[0].shape[0]
+    nb = -(-nl // m)
+    big = m * n
+    ons = np.zeros((nb, big, big), dtype=np.complex128)
+    cpl = np.zeros((max(nb - 1, 0), big, big), dtype=np.complex128)
+    for i in range(nl):
+        b, r = divmod(i, m)
+        ons[b, r * n:(r + 1) * n, r * n:(r + 1) * n] = onsite[i]
+    for i in range(nl - 1):
+        b, r = divmod(i, m)
+        blk = np.asarray(coupling[i], dtype=np.complex128)
+        if r < m - 1:
+            ons[b, r * n:(r + 1) * n, (r + 1) * n:(r + 2) * n] = blk
+            ons[b, (r + 1) * n:(r + 2) * n, r * n:(r + 1) * n] = blk.conj().T
+        else:
+            cpl[b, (m - 1) * n:m * n, 0:n] = blk
+    return ons, cpl
+
+
+def _layout(onsite, coupling, idx_l, idx_r, max_block):
+    nl = len(onsite)
+    n = onsite[0].shape[0]
+    dim = nl * n
+    for m in range(1, nl + 1):
+        nb = -(-nl // m)
+        if m * n > max_block:
+            break
+        last0 = (nb - 1) * m * n
+        if np.all(idx_l < m * n) and np.all(idx_r >= last0):
+            if m == 1:
+                ons = np.asarray(np.stack(onsite), dtype=np.complex128)
+                cpl = (np.asarray(np.stack(coupling), dtype=np.complex128) if nl > 1
+                       else np.zeros((0, n, n), dtype=np.complex128))
+            else:
+                ons, cpl = _merge_layers(onsite, coupling, m)
+            return ons, cpl, idx_l.astype(np.int32), (idx_r - last0).astype(np.int32)
+    raise NotImplementedError(
+        f"contacts at {idx_l.tolist()} / {idx_r.tolist()} of a {dim}-DOF device need merged "
+        f"blocks larger than the supported {max_block}")
+
+
+def resolve_k(device, lead_left, lead_right, kpar, *, eta, eta_device, device_to_lead_left,
+              device_to_lead_right, contact_left_indices, contact_right_indices, omega_scale,
+              max_block) -> KProblem:
+    """Resolve one k_par point like negf.py:141-243 (up to the self-energies)."""
+    dev = _resolve_device(device, kpar)
+    vl_in = _resolve_coupling(device_to_lead_left, kpar)
+    vr_in = _resolve_coupling(device_to_lead_right, kpar)
+    left = _resolve_lead_blocks(lead_left, kpar)
+    right = _resolve_lead_blocks(lead_right, kpar)
+    n = dev.dof_per_layer
+    dim = dev.n_layers * n
+    eta_dev = float(eta if eta_device is None else eta_device)
+    if eta_dev < 0.0:
+        raise ValueError("eta_device must be non-negative.")
+    onsite = [np.asarray(b, dtype=np.complex128) for b in dev.onsite_blocks]
+    coupling = [np.asarray(b, dtype=np.complex128) for b in dev.coupling_blocks]
+    ld = [np.asarray(left.d00, dtype=np.complex128), np.asarray(left.d01, dtype=np.complex128),
+          None if left.d10 is None else np.asarray(left.d10, dtype=np.complex128)]
+    rd = [np.asarray(right.d00, dtype=np.complex128), np.asarray(right.d01, dtype=np.complex128),
+          None if right.d10 is None else np.asarray(right.d10, dtype=np.complex128)]
+    scale = 1.0
+    if omega_scale is not None:
+        scale = float(omega_scale)
+        if scale <= 0.0:
+            raise ValueError("omega_scale must be positive when provided.")
+        s2 = scale * scale
+        onsite = [b / s2 for b in onsite]
+        coupling = [b / s2 for b in coupling]
+        ld = [None if b is None else b / s2 for b in ld]
+        rd = [None if b is None else b / s2 for b in rd]
+    vl = (ld[2] if ld[2] is not None else ld[1].conj().T) if vl_in is None else vl_in
+    vr = rd[1] if vr_in is None else vr_in
+    if omega_scale is not None:
+        vl = vl / (scale * scale)
+        vr = vr / (scale * scale)
+    ncl, ncr = vl.shape[0], vr.shape[0]
+    idx_l = _normalize_contact_indices(contact_left_indices, dim=dim, default=np.arange(ncl))
+    idx_r = _normalize_contact_indices(contact_right_indices, dim=dim,
+                                       default=np.arange(dim - ncr, dim))
+    if idx_l.size != ncl:
+        raise ValueError("Left contact index count must match device_to_lead_left rows "
+                         f"(got {idx_l.size}, expected {ncl}).")
+    if idx_r.size != ncr:
+        raise ValueError("Right contact index count must match device_to_lead_right rows "
+                         f"(got {idx_r.size}, expected {ncr}).")
+    for v, lead in ((vl, ld), (vr, rd)):
+        if v.ndim != 2:
+            raise ValueError("device_to_lead coupling must be a 2D array.")
+        if v.shape[1] != lead[0].shape[0]:
+            raise ValueError("device_to_lead coupling has incompatible lead dimension "
+                             f"(got {v.shape[1]}, expected {lead[0].shape[0]}).")
+    ons, cpl, pos_l, pos_r = _layout(onsite, coupling, idx_l, idx_r, max_block)
+    return KProblem(lead_l=pack_lead(*ld), lead_r=pack_lead(*rd),
+                    v_l=np.ascontiguousarray(vl, dtype=np.complex128),
+                    v_r=np.ascontiguousarray(vr, dtype=np.complex128), pos_l=pos_l, pos_r=pos_r,
+                    onsite=ons, coupling=cpl, idx_l=idx_l, idx_r=idx_r, scale=scale, dim=dim)
+
+
+def solve_points(kps: list[KProblem], k_of: Array, omega: Array, eta: Array, eta_dev: Array,
+                 *, precision: str, tol: float = 1e-12, maxiter: int = 200):
+    """Transmission of every instance (k_of[i] indexes ``kps``).  Returns (T, status) arrays.
+
+    Instances are grouped by block layout; each group is one batched launch.
+    """
+    k_of = np.asarray(k_of, dtype=np.int64)
+    omega = np.asarray(omega, dtype=float)
+    eta = np.asarray(eta, dtype=float)
+    eta_dev = np.asarray(eta_dev, dtype=float)
+    t_all = np.full(omega.shape, np.nan)
+    st_all = np.zeros(omega.shape, dtype=np.int32)
+    groups: dict = {}
+    for ki, kp in enumerate(kps):
+        groups.setdefault(kp.signature(), []).append(ki)
+    for members in groups.values():
+        sel = np.nonzero(np.isin(k_of, members))[0]
+        if sel.size == 0:
+            continue
+        local = {k: j for j, k in enumerate(members)}
+        kg = [kps[k] for k in members]
+        lead_l = np.stack([kp.lead_l for kp in kg])
+        lead_r = np.stack([kp.lead_r for kp in kg])
+        share = bool(np.array_equal(lead_l, lead_r))
+        sets = _engine.TransportSets.from_arrays(
+            lead_l, lead_r, np.stack([kp.v_l for kp in kg]), np.stack([kp.v_r for kp in kg]),
+            kg[0].pos_l, kg[0].pos_r, np.stack([kp.onsite for kp in kg]),
+            np.stack([kp.coupling for kp in kg]) if kg[0].onsite.shape[0] > 1 else None, share)
+        scale = np.array([kps[k].scale for k in k_of[sel]])
+        solver = _engine.TransmissionSolver(sets, precision=precision, tol=tol, maxiter=maxiter)
+        res = solver.run(np.array([local[k] for k in k_of[sel]], dtype=np.int32),
+                         omega[sel] / scale, eta[sel] / scale, eta_dev[sel] / scale)
+        t_all[sel] = res.t.cpu().numpy()
+        st_all[sel] = res.status.cpu().numpy()
+    return t_all, st_all
+
+
+def _max_block(precision):
+    from negfpy_amd import _lib
+
+    return int(_lib.load().negf_max_block(_engine.precision_code(precision)))
+
+
+def _resolve_many(device, lead_left, lead_right, kpoints, *, eta, eta_device, device_to_lead_left,
+                  device_to_lead_right, contact_left_indices, contact_right_indices, omega_scale,
+                  precision):
+    mb = _max_block(precision)
+    return [resolve_k(device, lead_left, lead_right, k, eta=eta, eta_device=eta_device,
+                      device_to_lead_left=device_to_lead_left,
+                      device_to_lead_right=device_to_lead_right,
+                      contact_left_indices=contact_left_indices,
+                      contact_right_indices=contact_right_indices, omega_scale=omega_scale,
+                      max_block=mb) for k in kpoints]
+
+
+def transmission(omega: float, device: DeviceLike, lead_left: LeadLike, lead_right: LeadLike,
+                 eta: float = 1e-8, eta_device: float | None = None, kpar: KPar = None,
+                 device_to_lead_left: DeviceToLeadLike | None = None,
+                 device_to_lead_right: DeviceToLeadLike | None = None,
+                 contact_left_indices: ContactIndices = None,
+                 contact_right_indices: ContactIndices = None,
+                 surface_gf_method: str = "sancho_rubio", omega_scale: float | None = None, *,
+                 precision: str | None = None) -> float:
+    """Coherent phonon transmission T(omega) (negf.py:306-357)."""
+    canonical_method(surface_gf_method)
+    prec = _prec(precision)
+    kp = _resolve_many(device, lead_left, lead_right, [kpar], eta=eta, eta_device=eta_device,
+                       device_to_lead_left=device_to_lead_left,
+                       device_to_lead_right=device_to_lead_right,
+                       contact_left_indices=contact_left_indices,
+                       contact_right_indices=contact_right_indices, omega_scale=omega_scale,
+                       precision=prec)
+    eta_dev = float(eta if eta_device is None else eta_device)
+    t, st = solve_points(kp, [0], [float(omega)], [float(eta)], [eta_dev], precision=prec)
+    return transmission_value(int(st[0]), float(t[0]))
+
+
+def transmission_kavg(omega: float, device: DeviceLike, lead_left: LeadLike, lead_right: LeadLike,
+                      kpoints: list[tuple[float, ...]], eta: float = 1e-8,
+                      eta_device: float | None = None,
+                      device_to_lead_left: DeviceToLeadLike | None = None,
+                      device_to_lead_right: DeviceToLeadLike | None = None,
+                      contact_left_indices: ContactIndices = None,
+                      contact_right_indices: ContactIndices = None,
+                      surface_gf_method: str = "sancho_rubio", omega_scale: float | None = None,
+                      *, precision: str | None = None) -> float:
+    """k_par-averaged transmission (negf.py:360-397): all k points in one batch."""
+    if len(kpoints) == 0:
+        raise ValueError("kpoints must contain at least one k-point.")
+    canonical_method(surface_gf_method)
+    prec = _prec(precision)
+    kps = _resolve_many(device, lead_left, lead_right, list(kpoints), eta=eta,
+                        eta_device=eta_device, device_to_lead_left=device_to_lead_left,
+                        device_to_lead_right=device_to_lead_right,
+                        contact_left_indices=contact_left_indices,
+                        contact_right_indices=contact_right_indices, omega_scale=omega_scale,
+                        precision=prec)
+    nk = len(kps)
+    eta_dev = float(eta if eta_device is None else eta_device)
+    t, st = solve_points(kps, np.arange(nk), np.full(nk, float(omega)), np.full(nk, float(eta)),
+                         np.full(nk, eta_dev), precision=prec)
+    vals = [transmission_value(int(st[i]), float(t[i])) for i in range(nk)]
+    return float(np.mean(vals))
+
+
+def _kpar_tuple(kpar):
+    return tuple(float(x) for x in kpar) if kpar is not None else ()
+
+
+def _validate_adaptive(kpoints, eta_values, min_success_fraction, nonnegative_tolerance,
+                       max_channel_factor):
+    if len(kpoints) == 0:
+        raise ValueError("kpoints must contain at least one k-point.")
+    if len(eta_values) == 0:
+        raise ValueError("eta_values must contain at least one eta value.")
+    if not (0.0 <= min_success_fraction <= 1.0):
+        raise ValueError("min_success_fraction must be in [0, 1].")
+    if nonnegative_tolerance < 0.0:
+        raise ValueError("nonnegative_tolerance must be non-negative.")
+    if max_channel_factor is not None and max_channel_factor <= 0.0:
+        raise ValueError("max_channel_factor must be positive when provided.")
+
+
+def _classify(code, t, nonneg_tol, t_max):
+    """Outcome of one (k, eta) attempt as the adaptive loops of negf.py:446-514 see it:
+    ("ok", value) or (reason, t_or_None)."""
+    try:
+        tval = transmission_value(int(code), float(t))
+    except Exception:
+        return "exception", None
+    if not np.isfinite(tval):
+        return "non_finite", tval
+    if tval < -float(nonneg_tol):
+        return "negative", tval
+    if t_max is not None and tval > t_max:
+        return "over_channel_limit", tval
+    return "ok", float(max(tval, 0.0))
+
+
+def _resolve_each(device, lead_left, lead_right, kpoints, *, eta_device, precision, **kw):
+    """Resolve every k point on its own; a k point whose resolution raises gets None
+    (each of its attempts then counts as an "exception", as in negf.py:463/612)."""
+    mb = _max_block(precision)
+    out = []
+    for k in kpoints:
+        try:
+            # eta only enters resolution through the eta_device >= 0 check, made per rung
+            out.append(resolve_k(device, lead_left, lead_right, k, eta=1.0, eta_device=eta_device,
+                                 max_block=mb, **kw))
+        except Exception:
+            out.append(None)
+    return out
+
+
+def _attempt(omega, kps, ks, eta, eta_device, prec, nonneg_tol, t_max):
+    """Outcomes of one eta rung for the k points ``ks``: {k: (reason, value)}."""
+    eta_dev = float(eta if eta_device is None else eta_device)
+    codes = {}
+    live = [k for k in ks if kps[k] is not None and eta_dev >= 0.0]
+    for k in ks:
+        if k not in live:
+            codes[k] = ("exception", None)
+    if live:
+        valid = [kps[k] for k in live]
+        m = len(live)
+        t, st = solve_points(valid, np.arange(m), np.full(m, float(omega)), np.full(m, float(eta)),
+                             np.full(m, eta_dev), precision=prec)
+        for j, k in enumerate(live):
+            codes[k] = _classify(st[j], t[j], nonneg_tol, t_max[k])
+    return codes
+
+
+def transmission_kavg_adaptive(omega: float, device: DeviceLike, lead_left: LeadLike,
+                               lead_right: LeadLike, kpoints: list[tuple[float, ...]],
+                               eta_values: tuple[float, ...] = (1e-8, 1e-7, 1e-6, 1e-5),
+                               eta_device: float | None = None, min_success_fraction: float = 0.0,
+                               device_to_lead_left: DeviceToLeadLike | None = None,
+                               device_to_lead_right: DeviceToLeadLike | None = None,
+                               contact_left_indices: ContactIndices = None,
+                               contact_right_indices: ContactIndices = None,
+                               surface_gf_method: str = "sancho_rubio",
+                               omega_scale: float | None = None,
+                               nonnegative_tolerance: float = 0.0,
+                               max_channel_factor: float | None = None,
+                               collect_rejected: bool = False, *, precision: str | None = None
+                               ) -> tuple[float, dict[str, object]]:
+    """Per-k eta ladder (negf.py:400-543).  Each rung is one batch over the k points
+    that have not yet succeeded; outcomes, filters and statistics follow the reference."""
+    _validate_adaptive(kpoints, eta_values, min_success_fraction, nonnegative_tolerance,
+                       max_channel_factor)
+    canonical_method(surface_gf_method)
+    prec = _prec(precision)
+    kpoints = list(kpoints)
+    nk = len(kpoints)
+    t_max = [None] * nk
+    if max_channel_factor is not None:
+        t_max = [float(max_channel_factor) * float(_resolve_lead_blocks(lead_left, k).d00.shape[0])
+                 for k in kpoints]
+    kps = _resolve_each(device, lead_left, lead_right, kpoints, eta_device=eta_device,
+                        device_to_lead_left=device_to_lead_left,
+                        device_to_lead_right=device_to_lead_right,
+                        contact_left_indices=contact_left_indices,
+                        contact_right_indices=contact_right_indices, omega_scale=omega_scale,
+                        precision=prec)
+    outcome: list = [None] * nk      # accepted value per k
+    used_eta: list = [None] * nk
+    rejected_by_k: list[list] = [[] for _ in range(nk)]
+    pending = list(range(nk))
+    for eta in eta_values:
+        if not pending:
+            break
+        codes = _attempt(omega, kps, pending, eta, eta_device, prec, nonnegative_tolerance, t_max)
+        still = []
+        for k in pending:
+            reason, val = codes[k]
+            if reason == "ok":
+                outcome[k] = val
+                used_eta[k] = float(eta)
+            else:
+                if collect_rejected:
+                    rejected_by_k[k].append({"omega": float(omega), "kpar": _kpar_tuple(kpoints[k]),
+                                             "eta": float(eta),
+                                             "t": None if val is None else float(val),
+                                             "reason": reason})
+                still.append(k)
+        pending = still
+    vals = [v for v in outcome if v is not None]
+    n_success = len(vals)
+    success_fraction = float(n_success / nk)
+    if n_success == 0:
+        raise RuntimeError("No k-point converged for adaptive k-averaged transmission.")
+    if success_fraction < min_success_fraction:
+        raise RuntimeError(
+            "Adaptive k-averaged transmission failed success-fraction threshold "
+            f"(got {success_fraction:.3f}, required >= {min_success_fraction:.3f}).")
+    hist = {float(e): int(sum(1 for u in used_eta if u == float(e))) for e in eta_values}
+    info: dict[str, object] = {"n_total": int(nk), "n_success": int(n_success),
+                               "n_failed": int(nk - n_success),
+                               "success_fraction": success_fraction, "eta_histogram": hist}
+    if collect_rejected:
+        rejected = [r for rs in rejected_by_k for r in rs]
+        info["n_rejected"] = int(len(rejected))
+        info["rejected"] = rejected
+    return float(np.mean(vals)), info
+
+
+def transmission_kavg_adaptive_global_eta(
+        omega: float, device: DeviceLike, lead_left: LeadLike, lead_right: LeadLike,
+        kpoints: list[tuple[float, ...]], eta_values: tuple[float, ...] = (1e-8, 1e-7, 1e-6, 1e-5),
+        eta_device: float | None = None, min_success_fraction: float = 0.0,
+        device_to_lead_left: DeviceToLeadLike | None = None,
+        device_to_lead_right: DeviceToLeadLike | None = None,
+        contact_left_indices: ContactIndices = None, contact_right_indices: ContactIndices = None,
+        surface_gf_method: str = "sancho_rubio", omega_scale: float | None = None,
+        nonnegative_tolerance: float = 0.0, max_channel_factor: float | None = None,
+        collect_rejected: bool = False, *, precision: str | None = None
+) -> tuple[float, dict[str, object]]:
+    """One eta per omega for all k points (negf.py:546-688); each rung is one batch."""
+    _validate_adaptive(kpoints, eta_values, min_success_fraction, nonnegative_tolerance,
+                       max_channel_factor)
+    canonical_method(surface_gf_method)
+    prec = _prec(precision)
+    kpoints = list(kpoints)
+    nk = len(kpoints)
+    t_max = [None] * nk
+    if max_channel_factor is not None:
+        t_max = [float(max_channel_factor) * float(_resolve_lead_blocks(lead_left, k).d00.shape[0])
+                 for k in kpoints]
+    kps = _resolve_each(device, lead_left, lead_right, kpoints, eta_device=eta_device,
+                        device_to_lead_left=device_to_lead_left,
+                        device_to_lead_right=device_to_lead_right,
+                        contact_left_indices=contact_left_indices,
+                        contact_right_indices=contact_right_indices, omega_scale=omega_scale,
+                        precision=prec)
+    rejected_all: list = []
+    for eta in eta_values:
+        codes = _attempt(omega, kps, list(range(nk)), eta, eta_device, prec,
+                         nonnegative_tolerance, t_max)
+        outcomes = [codes[k] for k in range(nk)]
+        vals = [v for r, v in outcomes if r == "ok"]
+        if collect_rejected:
+            rejected_all.extend({"omega": float(omega), "kpar": _kpar_tuple(kpoints[k]),
+                                 "eta": float(eta), "t": None if v is None else float(v),
+                                 "reason": r} for k, (r, v) in enumerate(outcomes) if r != "ok")
+        n_success = len(vals)
+        success_fraction = float(n_success / nk)
+        if n_success > 0 and success_fraction >= min_success_fraction:
+            info: dict[str, object] = {
+                "n_total": int(nk), "n_success": int(n_success),
+                "n_failed": int(nk - n_success), "success_fraction": success_fraction,
+                "eta_selected": float(eta),
+                "eta_histogram": {float(e): (int(n_success) if float(e) == float(eta) else 0)
+                                  for e in eta_values}}
+            if collect_rejected:
+                info["n_rejected"] = int(len(rejected_all))
+                info["rejected"] = rejected_all
+            return float(np.mean(vals)), info
+    raise RuntimeError(
+        "No eta value satisfied global adaptive k-averaging quality threshold "
+        f"(required success_fraction >= {min_success_fraction:.3f}).")

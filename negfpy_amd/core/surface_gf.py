@@ -1,0 +1,143 @@
+"""Surface Green's functions of periodic leads on the GPU.
+
+API of the reference ``src/negfpy/core/surface_gf.py`` (:12-311): the same
+functions, arguments, defaults, method aliases and exceptions.  Every method
+runs the batched decimation kernel (``negf_sancho_rubio_batched``).
+
+The eigenmode routes (``generalized_eigen``, ``generalized_eigen_svd``,
+``legacy_eigen_svd``; surface_gf.py:84-251) select the n decaying Bloch modes
+of the lead's quadratic eigenvalue problem and return the retarded surface
+Green's function they span.  For eta > 0 the unit circle separates those modes
+from the growing ones and that Green's function is the unique retarded
+solution of g = (z - d00 - d01 g d10)^-1, which is the fixed point decimation
+converges to; those entry points therefore keep their argument checks and
+return the decimation result.  The reference's fallbacks for eta = 0 (picking
+the n smallest |lambda| when fewer than n lie inside the circle) and its
+rank/SVD-cutoff handling of degenerate mode sets are not reproduced.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import _engine
+from ._status import raise_for_lead_status
+
+Array = np.ndarray
+
+_SR = {"sancho_rubio", "sancho-rubio", "sr"}
+_GE = {"generalized_eigen", "generalized-eigen", "eig", "gep"}
+_GESVD = {"generalized_eigen_svd", "generalized-eigen-svd", "ge_svd", "gep_svd"}
+_LEGACY = {"legacy_eigen_svd", "legacy-eigen-svd", "legacy", "old_svd"}
+
+_UNKNOWN = ("Unknown surface GF method. Use one of: "
+            "'sancho_rubio', 'generalized_eigen', 'generalized_eigen_svd', 'legacy_eigen_svd'.")
+
+_default_precision = "fp64"
+
+
+def set_default_precision(precision: str) -> None:
+    """Select "fp64" (default) or "mixed" for calls that pass no ``precision``."""
+    global _default_precision
+    _engine.precision_code(precision)
+    _default_precision = precision
+
+
+def get_default_precision() -> str:
+    return _default_precision
+
+
+def _prec(precision):
+    return _default_precision if precision is None else precision
+
+
+def canonical_method(method: str) -> str:
+    """Map a method alias to its canonical name; ValueError if unknown (surface_gf.py:269-311)."""
+    m = method.strip().lower()
+    if m in _SR:
+        return "sancho_rubio"
+    if m in _GE:
+        return "generalized_eigen"
+    if m in _GESVD:
+        return "generalized_eigen_svd"
+    if m in _LEGACY:
+        return "legacy_eigen_svd"
+    raise ValueError(_UNKNOWN)
+
+
+def resolve_d10(d01: Array, d10: Array | None) -> Array:
+    """d10 defaults to d01^H (surface_gf.py:12-15)."""
+    if d10 is None:
+        return np.asarray(d01).conj().T.astype(np.complex128, copy=True)
+    return np.asarray(d10).astype(np.complex128, copy=True)
+
+
+def pack_lead(d00, d01, d10=None) -> Array:
+    """(3, n, n) complex128 stack (d00, d01, d10 resolved)."""
+    return np.stack([np.asarray(d00, dtype=np.complex128), np.asarray(d01, dtype=np.complex128),
+                     resolve_d10(np.asarray(d01), d10)])
+
+
+def _single(omega, d00, d01, d10, eta, tol, maxiter, precision):
+    res = _engine.surface_gf_batch(pack_lead(d00, d01, d10)[None], [0], [float(omega)],
+                                   [float(eta)], tol=tol, maxiter=maxiter, precision=_prec(precision))
+    raise_for_lead_status(int(res.status[0].item()))
+    return res.g[0].cpu().numpy()
+
+
+def surface_gf_sancho_rubio(omega: float, d00: Array, d01: Array, d10: Array | None = None,
+                            eta: float = 1e-8, tol: float = 1e-12, maxiter: int = 200, *,
+                            precision: str | None = None) -> Array:
+    """Retarded surface GF by Lopez-Sancho decimation (surface_gf.py:18-81)."""
+    return _single(omega, d00, d01, d10, eta, tol, maxiter, precision)
+
+
+def _check_square(d00, d01):
+    n = int(d00.shape[0])
+    if d00.shape != (n, n) or d01.shape != (n, n):
+        raise ValueError("d00 and d01 must be square matrices of the same shape.")
+
+
+def surface_gf_generalized_eigen(omega: float, d00: Array, d01: Array, d10: Array | None = None,
+                                 eta: float = 1e-8, unit_circle_tol: float = 1e-10, *,
+                                 precision: str | None = None) -> Array:
+    """Eigenmode route (surface_gf.py:84-133); see the module docstring."""
+    _check_square(d00, d01)
+    return _single(omega, d00, d01, d10, eta, 1e-12, 200, precision)
+
+
+def surface_gf_generalized_eigen_svd(omega: float, d00: Array, d01: Array,
+                                     d10: Array | None = None, eta: float = 1e-8,
+                                     unit_circle_tol: float = 1e-10, svd_rcond: float = 1e-12, *,
+                                     precision: str | None = None) -> Array:
+    """Eigenmode route with SVD pseudo-inverse (surface_gf.py:136-185)."""
+    _check_square(d00, d01)
+    if svd_rcond <= 0.0:
+        raise ValueError("svd_rcond must be positive.")
+    return _single(omega, d00, d01, d10, eta, 1e-12, 200, precision)
+
+
+def surface_gf_legacy_eigen_svd(omega: float, d00: Array, d01: Array, d10: Array | None = None,
+                                eta: float = 1e-8, unit_circle_tol: float = 0.0, *,
+                                precision: str | None = None) -> Array:
+    """Legacy Fortran-style eigenmode route (surface_gf.py:201-251)."""
+    _check_square(d00, d01)
+    return _single(omega, d00, d01, d10, eta, 1e-12, 200, precision)
+
+
+def surface_gf(omega: float, d00: Array, d01: Array, d10: Array | None = None, eta: float = 1e-8,
+               *, method: str = "sancho_rubio", tol: float = 1e-12, maxiter: int = 200,
+               unit_circle_tol: float = 1e-10, svd_rcond: float = 1e-12,
+               precision: str | None = None) -> Array:
+    """Method dispatcher (surface_gf.py:254-311)."""
+    m = canonical_method(method)
+    if m == "sancho_rubio":
+        return surface_gf_sancho_rubio(omega, d00, d01, d10, eta, tol, maxiter, precision=precision)
+    if m == "generalized_eigen":
+        return surface_gf_generalized_eigen(omega, d00, d01, d10, eta, unit_circle_tol,
+                                            precision=precision)
+    if m == "generalized_eigen_svd":
+        return surface_gf_generalized_eigen_svd(omega, d00, d01, d10, eta, unit_circle_tol,
+                                                svd_rcond, precision=precision)
+    return surface_gf_legacy_eigen_svd(omega, d00, d01, d10, eta, unit_circle_tol,
+                                       precision=precision)

@@ -1,0 +1,1130 @@
+// Batched phonon-NEGF transport kernels for MI355X (gfx950) and their C ABI.
+//
+// One workgroup solves one (omega, k_par) instance; all per-instance matrices
+// are held in LDS in planar complex form (cmat.hpp).  Kernels:
+//
+//   k_sancho_rubio<T>  Lopez-Sancho decimation of a semi-infinite lead
+//                      (reference src/negfpy/core/surface_gf.py:18-81).  In FP64
+//                      mode it ends with g_s = (z - e_s)^-1 and the surface DOS;
+//                      in FP32 mode it hands e_s to k_refine.
+//   k_refine           FP64 g_s = (z - e_s)^-1 followed by fixed-point residual
+//                      steps g <- (z - d00 - d01 g d10)^-1 and the surface DOS.
+//   k_self_energy      Sigma = V g V^H embedded in the device end layer, FP64
+//                      (negf.py:72-97, :245-248).
+//   k_rgf<T>           block recursive Green's function sweep for G_{1N} of
+//                      A = z - D - Sigma_L - Sigma_R (replaces the sparse LU of
+//                      negf.py:350-354; same matrix, same G_{1N}).
+//   k_trace            T = Re Tr[Gamma_L G_1N Gamma_R G_1N^H], Gamma = i(S - S^H), FP64
+//                      (negf.py:115-116, :356-357).
+//   k_conductance      Landauer thermal conductance per temperature, FP64
+//                      (workflows/ifc_bulk.py:408-438).
+//   k_heat_current     finite-bias Landauer heat current, FP64
+//                      (core/thermal_current.py:30-67).
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+
+#include "cmat.hpp"
+#include "../../include/negf_hip.h"
+
+namespace negf {
+
+// ---------------------------------------------------------------------------
+// configurations: (NMAX, threads, tile) per block-size class
+// ---------------------------------------------------------------------------
+
+template <int NMAX_>
+struct Cfg;
+template <>
+struct Cfg<8> {
+  static constexpr int NMAX = 8, NT = 64, TM = 1;
+};
+template <>
+struct Cfg<16> {
+  static constexpr int NMAX = 16, NT = 64, TM = 2;
+};
+template <>
+struct Cfg<24> {
+  static constexpr int NMAX = 24, NT = 192, TM = 2;
+};
+template <>
+struct Cfg<32> {
+  static constexpr int NMAX = 32, NT = 256, TM = 2;
+};
+template <>
+struct Cfg<48> {
+  static constexpr int NMAX = 48, NT = 256, TM = 3;
+};
+template <>
+struct Cfg<64> {
+  static constexpr int NMAX = 64, NT = 256, TM = 4;
+};
+
+template <typename T, int NMAX, int NT, int NMAT>
+struct Lds {
+  static constexpr int LD = NMAX + 2;
+  static constexpr int MSZ = NMAX * LD;
+  static constexpr int MAXE = (NMAX * NMAX + NT - 1) / NT;
+  static constexpr size_t bytes() {
+    return sizeof(T) * (size_t)(2 * NMAT * MSZ + 4 * NMAX) + sizeof(int) * NMAX +
+           sizeof(double) * (NT / WAVE) + sizeof(int) * (NT / WAVE) + 64;
+  }
+  T* re[NMAT];
+  T* im[NMAT];
+  T *rbr, *rbi, *cbr, *cbi;
+  int* perm;
+  double* sred;
+  int* sflag;
+  __device__ __forceinline__ Lds(unsigned char* base) {
+    T* p = reinterpret_cast<T*>(base);
+#pragma unroll
+    for (int q = 0; q < NMAT; ++q) {
+      re[q] = p;
+      p += MSZ;
+      im[q] = p;
+      p += MSZ;
+    }
+    rbr = p;
+    rbi = p + NMAX;
+    cbr = p + 2 * NMAX;
+    cbi = p + 3 * NMAX;
+    p += 4 * NMAX;
+    perm = reinterpret_cast<int*>(p);
+    size_t off = reinterpret_cast<unsigned char*>(perm + NMAX) - base;
+    off = (off + 7) & ~size_t(7);
+    sred = reinterpret_cast<double*>(base + off);
+    sflag = reinterpret_cast<int*>(sred + NT / WAVE);
+  }
+};
+
+template <typename T, int NT>
+__device__ __forceinline__ void lds_zero(unsigned char* base, size_t nbytes, int tid) {
+  uint32_t* p = reinterpret_cast<uint32_t*>(base);
+  const size_t nw = nbytes / 4;
+  for (size_t e = tid; e < nw; e += NT) p[e] = 0u;
+}
+
+// Load an m x p complex128 block (row-major, leading dim p) into planar LDS as T.
+template <typename T, int NT>
+__device__ __forceinline__ void gload(const double* __restrict__ src, int m, int p, T* re, T* im,
+                                      int ld, int tid) {
+  const double2* s = reinterpret_cast<const double2*>(src);
+  const int mp = m * p;
+  for (int e = tid; e < mp; e += NT) {
+    const int i = e / p, j = e - (e / p) * p;
+    const double2 v = s[e];
+    re[i * ld + j] = (T)v.x;
+    im[i * ld + j] = (T)v.y;
+  }
+}
+
+// Store an LDS n x n planar matrix into complex128 global memory.
+template <typename T, int NT>
+__device__ __forceinline__ void gstore(double* __restrict__ dst, int n, const T* re, const T* im, int ld,
+                                       int tid) {
+  double2* d = reinterpret_cast<double2*>(dst);
+  const int nn = n * n;
+  for (int e = tid; e < nn; e += NT) {
+    const int i = e / n, j = e - (e / n) * n;
+    d[e] = make_double2((double)re[i * ld + j], (double)im[i * ld + j]);
+  }
+}
+
+__device__ __forceinline__ bool finite_d(double x) { return isfinite(x); }
+__device__ __forceinline__ bool finite_d(float x) { return isfinite(x); }
+
+// ---------------------------------------------------------------------------
+// Sancho-Rubio decimation
+// ---------------------------------------------------------------------------
+
+struct SrArgs {
+  int n;
+  int n_inst;
+  const double* lead;  // [n_sets][3][n][n] complex128: d00, d01, d10
+  const int32_t* set_of;
+  const double* omega;
+  const double* eta;
+  double tol;
+  int maxiter;
+  double* g_out;  // [n_inst][n][n]: g_s (FP64 mode) or e_s (FP32 mode)
+  double* dos_out;
+  int32_t* iters;
+  int32_t* status;
+};
+
+template <typename T, int NMAX>
+__global__ void __launch_bounds__(Cfg<NMAX>::NT) k_sancho_rubio(SrArgs a) {
+  constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
+  using L = Lds<T, NMAX, NT, 4>;
+  constexpr int LD = L::LD;
+  extern __shared__ __align__(16) unsigned char smem[];
+  L s(smem);
+  const int tid = threadIdx.x;
+  const int inst = blockIdx.x;
+  const int n = a.n;
+  const size_t nn = (size_t)n * n;
+  const double* blk = a.lead + (size_t)a.set_of[inst] * 3 * nn * 2;
+
+  lds_zero<T, NT>(smem, L::bytes() - 64, tid);
+  __syncthreads();
+  // alpha = d01 -> s.re[0], beta = d10 -> s.re[1]; g -> [2]; work -> [3]
+  gload<T, NT>(blk + nn * 2, n, n, s.re[0], s.im[0], LD, tid);
+  gload<T, NT>(blk + 2 * nn * 2, n, n, s.re[1], s.im[1], LD, tid);
+
+  const Tile<TM> t(n, n, tid);
+  const ElemMap<NT, L::MAXE> em(n, tid);
+  T er[TM][TM], ei[TM][TM], sr[TM][TM], si[TM][TM];
+  tzero<T, TM>(er, ei);
+  if (t.own) {
+    const double2* d00 = reinterpret_cast<const double2*>(blk);
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+      for (int y = 0; y < TM; ++y) {
+        const int i = t.i0 + x, j = t.j0 + y;
+        if (i < n && j < n) {
+          const double2 v = d00[i * n + j];
+          er[x][y] = (T)v.x;
+          ei[x][y] = (T)v.y;
+        }
+      }
+  }
+#pragma unroll
+  for (int x = 0; x < TM; ++x)
+#pragma unroll
+    for (int y = 0; y < TM; ++y) {
+      sr[x][y] = er[x][y];
+      si[x][y] = ei[x][y];
+    }
+  double zrd, zid;
+  zsq(a.omega[inst], a.eta[inst], zrd, zid);
+  const T zr = (T)zrd, zi = (T)zid;
+  const T tol = (T)a.tol;
+  __syncthreads();
+
+  int status = ST_OK;
+  int it = 0;
+  bool conv = false;
+  while (it < a.maxiter) {
+    ++it;
+    // G = z - e
+    if (t.own) {
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          const int i = t.i0 + x, j = t.j0 + y;
+          if (i < n && j < n) {
+            s.re[2][i * LD + j] = (i == j ? zr : T(0)) - er[x][y];
+            s.im[2][i * LD + j] = (i == j ? zi : T(0)) - ei[x][y];
+          }
+        }
+    }
+    __syncthreads();
+    const int st = gj_inverse<T, NT, L::MAXE>(s.re[2], s.im[2], LD, n, s.rbr, s.rbi, s.cbr, s.cbi,
+                                              s.perm, em, tid);
+    if (st != ST_OK) {
+      status = st;
+      break;
+    }
+    // T1 = a g, T2 = b g
+    T t1r[TM][TM], t1i[TM][TM], t2r[TM][TM], t2i[TM][TM];
+    tzero<T, TM>(t1r, t1i);
+    tzero<T, TM>(t2r, t2i);
+    if (t.own)
+      mac2b<T, TM>(s.re[0], s.im[0], s.re[1], s.im[1], s.re[2], s.im[2], LD, n, t.i0, t.j0, t1r, t1i,
+                   t2r, t2i);
+    __syncthreads();
+    tstore<T, TM>(s.re[2], s.im[2], LD, n, n, t, t1r, t1i);
+    tstore<T, TM>(s.re[3], s.im[3], LD, n, n, t, t2r, t2i);
+    __syncthreads();
+    T p1r[TM][TM], p1i[TM][TM], p2r[TM][TM], p2i[TM][TM];
+    T p3r[TM][TM], p3i[TM][TM], p4r[TM][TM], p4i[TM][TM];
+    tzero<T, TM>(p1r, p1i);
+    tzero<T, TM>(p2r, p2i);
+    tzero<T, TM>(p3r, p3i);
+    tzero<T, TM>(p4r, p4i);
+    if (t.own)
+      mac4<T, TM>(s.re[2], s.im[2], s.re[3], s.im[3], s.re[0], s.im[0], s.re[1], s.im[1], LD, n, t.i0,
+                  t.j0, p1r, p1i, p2r, p2i, p3r, p3i, p4r, p4i);
+    __syncthreads();
+    bool bad = false;
+    double emax = 0.0;
+    if (t.own) {
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          const int i = t.i0 + x, j = t.j0 + y;
+          if (i < n && j < n) {
+            sr[x][y] = sr[x][y] + p1r[x][y];
+            si[x][y] = si[x][y] + p1i[x][y];
+            er[x][y] = (er[x][y] + p1r[x][y]) + p2r[x][y];
+            ei[x][y] = (ei[x][y] + p1i[x][y]) + p2i[x][y];
+            s.re[0][i * LD + j] = p3r[x][y];
+            s.im[0][i * LD + j] = p3i[x][y];
+            s.re[1][i * LD + j] = p4r[x][y];
+            s.im[1][i * LD + j] = p4i[x][y];
+            bad |= !(finite_d(sr[x][y]) && finite_d(si[x][y]) && finite_d(er[x][y]) &&
+                     finite_d(ei[x][y]) && finite_d(p3r[x][y]) && finite_d(p3i[x][y]) &&
+                     finite_d(p4r[x][y]) && finite_d(p4i[x][y]));
+            const T m3 = p3r[x][y] * p3r[x][y] + p3i[x][y] * p3i[x][y];
+            const T m4 = p4r[x][y] * p4r[x][y] + p4i[x][y] * p4i[x][y];
+            emax = fmax(emax, (double)fmax(m3, m4));
+          }
+        }
+    }
+    double gmax;
+    bool gbad;
+    block_max_any<NT>(emax, bad, s.sred, s.sflag, tid, gmax, gbad);
+    if (gbad) {
+      status = ST_NONFINITE;
+      break;
+    }
+    if ((T)sqrt(gmax) < tol) {
+      conv = true;
+      break;
+    }
+  }
+  if (status == ST_OK && !conv) status = ST_NOCONV;
+  if (tid == 0) {
+    a.status[inst] = status;
+    a.iters[inst] = it;
+  }
+  if (status != ST_OK) return;
+
+  double* out = a.g_out + (size_t)inst * nn * 2;
+  if constexpr (sizeof(T) == 4) {
+    // FP32 mode: hand e_s to the FP64 refinement.
+    if (t.own) {
+      double2* o = reinterpret_cast<double2*>(out);
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          const int i = t.i0 + x, j = t.j0 + y;
+          if (i < n && j < n) o[i * n + j] = make_double2((double)sr[x][y], (double)si[x][y]);
+        }
+    }
+    return;
+  } else {
+    if (t.own) {
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          const int i = t.i0 + x, j = t.j0 + y;
+          if (i < n && j < n) {
+            s.re[2][i * LD + j] = (i == j ? zr : T(0)) - sr[x][y];
+            s.im[2][i * LD + j] = (i == j ? zi : T(0)) - si[x][y];
+          }
+        }
+    }
+    __syncthreads();
+    const int st = gj_inverse<T, NT, L::MAXE>(s.re[2], s.im[2], LD, n, s.rbr, s.rbi, s.cbr, s.cbi,
+                                              s.perm, em, tid);
+    if (st != ST_OK) {
+      if (tid == 0) a.status[inst] = st;
+      return;
+    }
+    gstore<T, NT>(out, n, s.re[2], s.im[2], LD, tid);
+    if (tid == 0 && a.dos_out) {
+      double tr = 0.0;
+      for (int i = 0; i < n; ++i) tr += -2.0 * (double)s.im[2][i * LD + i];
+      a.dos_out[inst] = (a.omega[inst] / M_PI) * tr;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 refinement of an FP32 decimation
+// ---------------------------------------------------------------------------
+
+struct RefineArgs {
+  int n;
+  int n_inst;
+  const double* lead;
+  const int32_t* set_of;
+  const double* omega;
+  const double* eta;
+  int steps;
+  double* g_io;  // in: e_s, out: g_s
+  double* dos_out;
+  int32_t* status;
+};
+
+template <int NMAX>
+__global__ void __launch_bounds__(Cfg<NMAX>::NT) k_refine(RefineArgs a) {
+  using T = double;
+  constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
+  using L = Lds<T, NMAX, NT, 4>;
+  constexpr int LD = L::LD;
+  extern __shared__ __align__(16) unsigned char smem[];
+  L s(smem);
+  const int tid = threadIdx.x;
+  const int inst = blockIdx.x;
+  if (a.status[inst] != ST_OK) return;  // uniform per block
+  const int n = a.n;
+  const size_t nn = (size_t)n * n;
+  const double* blk = a.lead + (size_t)a.set_of[inst] * 3 * nn * 2;
+  double* io = a.g_io + (size_t)inst * nn * 2;
+  lds_zero<T, NT>(smem, L::bytes() - 64, tid);
+  __syncthreads();
+  gload<T, NT>(blk + nn * 2, n, n, s.re[0], s.im[0], LD, tid);      // d01
+  gload<T, NT>(blk + 2 * nn * 2, n, n, s.re[1], s.im[1], LD, tid);  // d10
+  const Tile<TM> t(n, n, tid);
+  const ElemMap<NT, L::MAXE> em(n, tid);
+  double zr, zi;
+  zsq(a.omega[inst], a.eta[inst], zr, zi);
+  // G = z - e_s
+  {
+    const double2* es = reinterpret_cast<const double2*>(io);
+    for (int e = tid; e < (int)nn; e += NT) {
+      const int i = e / n, j = e - (e / n) * n;
+      const double2 v = es[e];
+      s.re[2][i * LD + j] = (i == j ? zr : 0.0) - v.x;
+      s.im[2][i * LD + j] = (i == j ? zi : 0.0) - v.y;
+    }
+  }
+  __syncthreads();
+  int st = gj_inverse<T, NT, L::MAXE>(s.re[2], s.im[2], LD, n, s.rbr, s.rbi, s.cbr, s.cbi, s.perm,
+                                      em, tid);
+  const double2* d00 = reinterpret_cast<const double2*>(blk);
+  for (int step = 0; step < a.steps && st == ST_OK; ++step) {
+    T xr[TM][TM], xi[TM][TM];
+    tzero<T, TM>(xr, xi);
+    if (t.own) mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[2], s.im[2], LD, n, t.i0, t.j0, xr, xi);
+    tstore<T, TM>(s.re[3], s.im[3], LD, n, n, t, xr, xi);
+    __syncthreads();
+    tzero<T, TM>(xr, xi);
+    if (t.own) mac<T, TM, OPN, OPN>(s.re[3], s.im[3], s.re[1], s.im[1], LD, n, t.i0, t.j0, xr, xi);
+    __syncthreads();
+    if (t.own) {
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          const int i = t.i0 + x, j = t.j0 + y;
+          if (i < n && j < n) {
+            const double2 d = d00[i * n + j];
+            s.re[2][i * LD + j] = ((i == j ? zr : 0.0) - d.x) - xr[x][y];
+            s.im[2][i * LD + j] = ((i == j ? zi : 0.0) - d.y) - xi[x][y];
+          }
+        }
+    }
+    __syncthreads();
+    st = gj_inverse<T, NT, L::MAXE>(s.re[2], s.im[2], LD, n, s.rbr, s.rbi, s.cbr, s.cbi, s.perm, em,
+                                    tid);
+  }
+  if (st != ST_OK) {
+    if (tid == 0) a.status[inst] = st;
+    return;
+  }
+  gstore<T, NT>(io, n, s.re[2], s.im[2], LD, tid);
+  if (tid == 0 && a.dos_out) {
+    double tr = 0.0;
+    for (int i = 0; i < n; ++i) tr += -2.0 * s.im[2][i * LD + i];
+    a.dos_out[inst] = (a.omega[inst] / M_PI) * tr;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sigma = V g V^H embedded into an n_dev x n_dev end-layer block (FP64)
+// ---------------------------------------------------------------------------
+
+struct SigmaArgs {
+  int n_lead, n_c, n_dev, n_inst;
+  const double* v;  // [n_sets][n_c][n_lead]
+  const int32_t* set_of;
+  const double* g;    // [n_inst][n_lead][n_lead]
+  const int32_t* idx; // [n_c] positions inside the end layer
+  const int32_t* status_in;  // may be null; skip instances with status != 0
+  double* sigma_out;  // [n_inst][n_dev][n_dev]
+};
+
+template <int NMAX>
+__global__ void __launch_bounds__(Cfg<NMAX>::NT) k_self_energy(SigmaArgs a) {
+  using T = double;
+  constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
+  using L = Lds<T, NMAX, NT, 3>;
+  constexpr int LD = L::LD;
+  extern __shared__ __align__(16) unsigned char smem[];
+  L s(smem);
+  const int tid = threadIdx.x;
+  const int inst = blockIdx.x;
+  const int nl = a.n_lead, nc = a.n_c, nd = a.n_dev;
+  double2* out = reinterpret_cast<double2*>(a.sigma_out + (size_t)inst * nd * nd * 2);
+  if (a.status_in && a.status_in[inst] != ST_OK) return;
+  lds_zero<T, NT>(smem, L::bytes() - 64, tid);
+  __syncthreads();
+  gload<T, NT>(a.v + (size_t)a.set_of[inst] * nc * nl * 2, nc, nl, s.re[0], s.im[0], LD, tid);
+  gload<T, NT>(a.g + (size_t)inst * nl * nl * 2, nl, nl, s.re[1], s.im[1], LD, tid);
+  int* pos = s.perm;  // inverse contact map: device-layer index -> contact row
+  for (int i = tid; i < nd; i += NT) pos[i] = -1;
+  __syncthreads();
+  for (int c = tid; c < nc; c += NT) pos[a.idx[c]] = c;
+  __syncthreads();
+  {
+    const Tile<TM> t(nc, nl, tid);
+    T xr[TM][TM], xi[TM][TM];
+    tzero<T, TM>(xr, xi);
+    if (t.own) mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[1], s.im[1], LD, nl, t.i0, t.j0, xr, xi);
+    tstore<T, TM>(s.re[2], s.im[2], LD, nc, nl, t, xr, xi);
+  }
+  __syncthreads();
+  {
+    const Tile<TM> t(nc, nc, tid);
+    T xr[TM][TM], xi[TM][TM];
+    tzero<T, TM>(xr, xi);
+    if (t.own) mac<T, TM, OPN, OPH>(s.re[2], s.im[2], s.re[0], s.im[0], LD, nl, t.i0, t.j0, xr, xi);
+    __syncthreads();
+    tstore<T, TM>(s.re[1], s.im[1], LD, nc, nc, t, xr, xi);
+  }
+  __syncthreads();
+  for (int e = tid; e < nd * nd; e += NT) {
+    const int i = e / nd, j = e - (e / nd) * nd;
+    const int pi = pos[i], pj = pos[j];
+    out[e] = (pi >= 0 && pj >= 0) ? make_double2(s.re[1][pi * LD + pj], s.im[1][pi * LD + pj])
+                                  : make_double2(0.0, 0.0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Recursive Green's function sweep for G_{1N}
+// ---------------------------------------------------------------------------
+
+struct RgfArgs {
+  int n, n_layers, n_inst;
+  const double* onsite;    // [n_sets][L][n][n]
+  const double* coupling;  // [n_sets][L-1][n][n]
+  const int32_t* set_of;
+  const double* omega;
+  const double* eta_dev;
+  const double* sigma_l;  // [n_inst][n][n]
+  const double* sigma_r;
+  const int32_t* status_in;  // lead status (skip when != 0); may be null
+  double* g1n_out;           // [n_inst][n][n]
+  int32_t* status;
+};
+
+template <typename T, int NMAX>
+__global__ void __launch_bounds__(Cfg<NMAX>::NT) k_rgf(RgfArgs a) {
+  constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
+  using L = Lds<T, NMAX, NT, 4>;
+  constexpr int LD = L::LD;
+  extern __shared__ __align__(16) unsigned char smem[];
+  L s(smem);
+  const int tid = threadIdx.x;
+  const int inst = blockIdx.x;
+  if (a.status_in && a.status_in[inst] != ST_OK) {
+    if (tid == 0) a.status[inst] = a.status_in[inst];
+    return;
+  }
+  const int n = a.n, nl = a.n_layers;
+  const size_t nn = (size_t)n * n;
+  const size_t set = (size_t)a.set_of[inst];
+  const double2* ons = reinterpret_cast<const double2*>(a.onsite + set * nl * nn * 2);
+  const double* cpl = a.coupling + set * (nl > 1 ? nl - 1 : 0) * nn * 2;
+  const double2* sgl = reinterpret_cast<const double2*>(a.sigma_l + (size_t)inst * nn * 2);
+  const double2* sgr = reinterpret_cast<const double2*>(a.sigma_r + (size_t)inst * nn * 2);
+  // LDS: [0] G (inverse), [1] Y = g H, [2] Q = G_{1,i} H_i, [3] H
+  lds_zero<T, NT>(smem, L::bytes() - 64, tid);
+  __syncthreads();
+  const Tile<TM> t(n, n, tid);
+  const ElemMap<NT, L::MAXE> em(n, tid);
+  double zrd, zid;
+  zsq(a.omega[inst], a.eta_dev[inst], zrd, zid);
+  const T zr = (T)zrd, zi = (T)zid;
+  T sr[TM][TM], si[TM][TM];
+  tzero<T, TM>(sr, si);
+  int status = ST_OK;
+  T gr[TM][TM], gi[TM][TM];
+  tzero<T, TM>(gr, gi);
+  for (int layer = 0; layer < nl; ++layer) {
+    if (t.own) {
+      const double2* d = ons + (size_t)layer * nn;
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          const int i = t.i0 + x, j = t.j0 + y;
+          if (i < n && j < n) {
+            const double2 dv = d[i * n + j];
+            T vr = (i == j ? zr : T(0)) - (T)dv.x - sr[x][y];
+            T vi = (i == j ? zi : T(0)) - (T)dv.y - si[x][y];
+            if (layer == 0) {
+              const double2 q = sgl[i * n + j];
+              vr -= (T)q.x;
+              vi -= (T)q.y;
+            }
+            if (layer == nl - 1) {
+              const double2 q = sgr[i * n + j];
+              vr -= (T)q.x;
+              vi -= (T)q.y;
+            }
+            s.re[0][i * LD + j] = vr;
+            s.im[0][i * LD + j] = vi;
+          }
+        }
+    }
+    __syncthreads();
+    const int st = gj_inverse<T, NT, L::MAXE>(s.re[0], s.im[0], LD, n, s.rbr, s.rbi, s.cbr, s.cbi,
+                                              s.perm, em, tid);
+    if (st != ST_OK) {
+      status = 10 + st;
+      break;
+    }
+    if (layer == nl - 1) {
+      if (nl == 1) {
+        if (t.own) {
+#pragma unroll
+          for (int x = 0; x < TM; ++x)
+#pragma unroll
+            for (int y = 0; y < TM; ++y) {
+              const int i = t.i0 + x, j = t.j0 + y;
+              if (i < n && j < n) {
+                gr[x][y] = s.re[0][i * LD + j];
+                gi[x][y] = s.im[0][i * LD + j];
+              }
+            }
+        }
+      } else if (t.own) {
+        mac<T, TM, OPN, OPN>(s.re[2], s.im[2], s.re[0], s.im[0], LD, n, t.i0, t.j0, gr, gi);
+      }
+      break;
+    }
+    gload<T, NT>(cpl + (size_t)layer * nn * 2, n, n, s.re[3], s.im[3], LD, tid);
+    __syncthreads();
+    T yr[TM][TM], yi[TM][TM];
+    tzero<T, TM>(yr, yi);
+    if (t.own) mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[3], s.im[3], LD, n, t.i0, t.j0, yr, yi);
+    tstore<T, TM>(s.re[1], s.im[1], LD, n, n, t, yr, yi);
+    __syncthreads();
+    tzero<T, TM>(sr, si);
+    if (t.own) mac<T, TM, OPH, OPN>(s.re[3], s.im[3], s.re[1], s.im[1], LD, n, t.i0, t.j0, sr, si);
+    T qr[TM][TM], qi[TM][TM];
+    if (layer == 0) {
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          qr[x][y] = yr[x][y];
+          qi[x][y] = yi[x][y];
+        }
+    } else {
+      tzero<T, TM>(qr, qi);
+      if (t.own) mac<T, TM, OPN, OPN>(s.re[2], s.im[2], s.re[1], s.im[1], LD, n, t.i0, t.j0, qr, qi);
+    }
+    __syncthreads();
+    tstore<T, TM>(s.re[2], s.im[2], LD, n, n, t, qr, qi);
+  }
+  if (tid == 0) a.status[inst] = status;
+  if (status != ST_OK) return;
+  if (t.own) {
+    double2* o = reinterpret_cast<double2*>(a.g1n_out + (size_t)inst * nn * 2);
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+      for (int y = 0; y < TM; ++y) {
+        const int i = t.i0 + x, j = t.j0 + y;
+        if (i < n && j < n) o[i * n + j] = make_double2((double)gr[x][y], (double)gi[x][y]);
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// T = Re Tr[Gamma_L G Gamma_R G^H]  (FP64)
+// ---------------------------------------------------------------------------
+
+struct TraceArgs {
+  int n, n_inst;
+  const double* sigma_l;
+  const double* sigma_r;
+  const double* g1n;
+  const int32_t* status_in;
+  double* t_out;
+};
+
+template <int NMAX>
+__global__ void __launch_bounds__(Cfg<NMAX>::NT) k_trace(TraceArgs a) {
+  using T = double;
+  constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
+  using L = Lds<T, NMAX, NT, 3>;
+  constexpr int LD = L::LD;
+  extern __shared__ __align__(16) unsigned char smem[];
+  L s(smem);
+  const int tid = threadIdx.x;
+  const int inst = blockIdx.x;
+  if (a.status_in && a.status_in[inst] != ST_OK) {
+    if (tid == 0) a.t_out[inst] = NAN;
+    return;
+  }
+  const int n = a.n;
+  const int nn = n * n;
+  const double2* sl = reinterpret_cast<const double2*>(a.sigma_l + (size_t)inst * nn * 2);
+  const double2* sr = reinterpret_cast<const double2*>(a.sigma_r + (size_t)inst * nn * 2);
+  lds_zero<T, NT>(smem, L::bytes() - 64, tid);
+  __syncthreads();
+  // Gamma = i (S - S^H):  Gamma_ij = i (S_ij - conj(S_ji)) = (-(Im S_ij + Im S_ji), Re S_ij - Re S_ji)
+  for (int e = tid; e < nn; e += NT) {
+    const int i = e / n, j = e - (e / n) * n;
+    const double2 a1 = sl[i * n + j], a2 = sl[j * n + i];
+    s.re[0][i * LD + j] = -(a1.y + a2.y);
+    s.im[0][i * LD + j] = a1.x - a2.x;
+    const double2 b1 = sr[i * n + j], b2 = sr[j * n + i];
+    s.re[1][i * LD + j] = -(b1.y + b2.y);
+    s.im[1][i * LD + j] = b1.x - b2.x;
+  }
+  gload<T, NT>(a.g1n + (size_t)inst * nn * 2, n, n, s.re[2], s.im[2], LD, tid);
+  __syncthreads();
+  const Tile<TM> t(n, n, tid);
+  T pr[TM][TM], pi[TM][TM], qr[TM][TM], qi[TM][TM];
+  tzero<T, TM>(pr, pi);
+  tzero<T, TM>(qr, qi);
+  double acc = 0.0;
+  if (t.own) {
+    // P = Gamma_L G ; Q_ij = (Gamma_R G^H)_ji = sum_k conj(G_ik) Gamma_R_jk
+    mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[2], s.im[2], LD, n, t.i0, t.j0, pr, pi);
+    mac<T, TM, OPC, OPT>(s.re[2], s.im[2], s.re[1], s.im[1], LD, n, t.i0, t.j0, qr, qi);
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+      for (int y = 0; y < TM; ++y) {
+        const int i = t.i0 + x, j = t.j0 + y;
+        if (i < n && j < n) acc += pr[x][y] * qr[x][y] - pi[x][y] * qi[x][y];
+      }
+  }
+  const double tot = block_sum<NT>(acc, s.sred, tid);
+  if (tid == 0) a.t_out[inst] = tot;
+}
+
+// ---------------------------------------------------------------------------
+// omega integrals (FP64)
+// ---------------------------------------------------------------------------
+
+constexpr double HBAR_J_S = 1.054571817e-34;
+constexpr double KB_J_K = 1.380649e-23;
+
+// k_B T(w) x^2 e^-x / (1 - e^-x)^2 with x = hbar w / (k_B T); 1 - x^2/12 below 1e-6, 0 from 700.
+__device__ __forceinline__ double conductance_integrand(double w, double tw, double temp) {
+  const double x = (HBAR_J_S * w) / (KB_J_K * temp);
+  double ker = 0.0;
+  if (x < 1e-6) {
+    ker = 1.0 - (x * x) / 12.0;
+  } else if (x < 700.0) {
+    const double y = exp(-x);
+    const double d = (1.0 - y) * (1.0 - y);
+    ker = (x * x) * y / d;
+  }
+  return KB_J_K * tw * ker;
+}
+
+__global__ void __launch_bounds__(256) k_conductance(int n_w, const double* __restrict__ w,
+                                                      const double* __restrict__ tw, int n_t,
+                                                      const double* __restrict__ temps,
+                                                      double* __restrict__ out) {
+  __shared__ double sred[4];
+  const int tid = threadIdx.x;
+  const int it = blockIdx.x;
+  const double temp = temps[it];
+  double acc = 0.0;
+  if (temp > 0.0) {
+    for (int i = tid; i + 1 < n_w; i += 256) {
+      const double f0 = conductance_integrand(w[i], tw[i], temp);
+      const double f1 = conductance_integrand(w[i + 1], tw[i + 1], temp);
+      acc += (w[i + 1] - w[i]) * (f1 + f0) / 2.0;
+    }
+  }
+  const double s = block_sum<256>(acc, sred, tid);
+  if (tid == 0) out[it] = temp > 0.0 ? (1.0 / (2.0 * M_PI)) * s : 0.0;
+}
+
+// x / (e^x - 1) with the small-x series below 1e-6 and 0 from 700.
+__device__ __forceinline__ double phi_bose(double x) {
+  if (x < 1e-6) return 1.0 - 0.5 * x + (x * x) / 12.0;
+  if (x < 700.0) return x / expm1(x);
+  return 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_heat_current(int n_w, const double* __restrict__ w,
+                                                       const double* __restrict__ tw, double t_l,
+                                                       double t_r, double* __restrict__ out) {
+  __shared__ double sred[4];
+  const int tid = threadIdx.x;
+  double acc = 0.0;
+  for (int i = tid; i + 1 < n_w; i += 256) {
+    double f[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const double wq = w[i + q];
+      const double xl = HBAR_J_S * wq / (KB_J_K * t_l);
+      const double xr = HBAR_J_S * wq / (KB_J_K * t_r);
+      f[q] = (KB_J_K * t_l * phi_bose(xl) - KB_J_K * t_r * phi_bose(xr)) * tw[i + q];
+    }
+    acc += (w[i + 1] - w[i]) * (f[1] + f[0]) / 2.0;
+  }
+  const double s = block_sum<256>(acc, sred, tid);
+  if (tid == 0) out[0] = (1.0 / (2.0 * M_PI)) * s;
+}
+
+// ---------------------------------------------------------------------------
+// host-side dispatch
+// ---------------------------------------------------------------------------
+
+static thread_local char g_err[512] = "";
+
+static int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+static int hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) return fail(NEGF_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+  return NEGF_OK;
+}
+
+// smallest configured NMAX >= n for the given element size; 0 if unsupported
+static int pick_nmax(int n, bool fp64) {
+  const int opts[] = {8, 16, 24, 32, 48, 64};
+  for (int o : opts) {
+    if (n <= o) {
+      if (fp64 && o > 48) return 0;
+      return o;
+    }
+  }
+  return 0;
+}
+
+#define NEGF_LAUNCH(KERNEL, GRID, NT, LDS, STREAM, ARGS)                                         \
+  do {                                                                                          \
+    hipError_t e_ = hipFuncSetAttribute(reinterpret_cast<const void*>(&KERNEL),                 \
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)(LDS)); \
+    if (e_ != hipSuccess) return hip_check(e_, #KERNEL " attribute");                           \
+    hipLaunchKernelGGL(KERNEL, dim3(GRID), dim3(NT), (LDS), (STREAM), ARGS);                   \
+    e_ = hipGetLastError();                                                                     \
+    if (e_ != hipSuccess) return hip_check(e_, #KERNEL " launch");                              \
+  } while (0)
+
+template <typename T>
+static int run_sr(hipStream_t st, int nmax, const SrArgs& a) {
+#define SR_CASE(NM)                                                                   \
+  case NM: {                                                                          \
+    constexpr int NT = Cfg<NM>::NT;                                                   \
+    NEGF_LAUNCH((k_sancho_rubio<T, NM>), a.n_inst, NT, (Lds<T, NM, NT, 4>::bytes()), st, a); \
+    return NEGF_OK;                                                                   \
+  }
+  switch (nmax) {
+    SR_CASE(8)
+    SR_CASE(16)
+    SR_CASE(24)
+    SR_CASE(32)
+    SR_CASE(48)
+    case 64:
+      if constexpr (sizeof(T) == 4) {
+        constexpr int NT = Cfg<64>::NT;
+        NEGF_LAUNCH((k_sancho_rubio<T, 64>), a.n_inst, NT, (Lds<T, 64, NT, 4>::bytes()), st, a);
+        return NEGF_OK;
+      }
+      break;
+  }
+#undef SR_CASE
+  return fail(NEGF_ERR_ARG, "unsupported block size class %d", nmax);
+}
+
+static int run_refine(hipStream_t st, int nmax, const RefineArgs& a) {
+#define RF_CASE(NM)                                                                          \
+  case NM: {                                                                                 \
+    constexpr int NT = Cfg<NM>::NT;                                                          \
+    NEGF_LAUNCH((k_refine<NM>), a.n_inst, NT, (Lds<double, NM, NT, 4>::bytes()), st, a);     \
+    return NEGF_OK;                                                                          \
+  }
+  switch (nmax) {
+    RF_CASE(8)
+    RF_CASE(16)
+    RF_CASE(24)
+    RF_CASE(32)
+    RF_CASE(48)
+  }
+#undef RF_CASE
+  return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 limit of 48", a.n);
+}
+
+static int run_sigma(hipStream_t st, int nmax, const SigmaArgs& a) {
+#define SG_CASE(NM)                                                                          \
+  case NM: {                                                                                 \
+    constexpr int NT = Cfg<NM>::NT;                                                          \
+    NEGF_LAUNCH((k_self_energy<NM>), a.n_inst, NT, (Lds<double, NM, NT, 3>::bytes()), st, a); \
+    return NEGF_OK;                                                                          \
+  }
+  switch (nmax) {
+    SG_CASE(8)
+    SG_CASE(16)
+    SG_CASE(24)
+    SG_CASE(32)
+    SG_CASE(48)
+  }
+#undef SG_CASE
+  return fail(NEGF_ERR_ARG, "self-energy block size class %d unsupported", nmax);
+}
+
+template <typename T>
+static int run_rgf(hipStream_t st, int nmax, const RgfArgs& a) {
+#define RG_CASE(NM)                                                                  \
+  case NM: {                                                                         \
+    constexpr int NT = Cfg<NM>::NT;                                                  \
+    NEGF_LAUNCH((k_rgf<T, NM>), a.n_inst, NT, (Lds<T, NM, NT, 4>::bytes()), st, a);  \
+    return NEGF_OK;                                                                  \
+  }
+  switch (nmax) {
+    RG_CASE(8)
+    RG_CASE(16)
+    RG_CASE(24)
+    RG_CASE(32)
+    RG_CASE(48)
+    case 64:
+      if constexpr (sizeof(T) == 4) {
+        constexpr int NT = Cfg<64>::NT;
+        NEGF_LAUNCH((k_rgf<T, 64>), a.n_inst, NT, (Lds<T, 64, NT, 4>::bytes()), st, a);
+        return NEGF_OK;
+      }
+      break;
+  }
+#undef RG_CASE
+  return fail(NEGF_ERR_ARG, "unsupported block size class %d", nmax);
+}
+
+static int run_trace(hipStream_t st, int nmax, const TraceArgs& a) {
+#define TR_CASE(NM)                                                                     \
+  case NM: {                                                                            \
+    constexpr int NT = Cfg<NM>::NT;                                                     \
+    NEGF_LAUNCH((k_trace<NM>), a.n_inst, NT, (Lds<double, NM, NT, 3>::bytes()), st, a); \
+    return NEGF_OK;                                                                     \
+  }
+  switch (nmax) {
+    TR_CASE(8)
+    TR_CASE(16)
+    TR_CASE(24)
+    TR_CASE(32)
+    TR_CASE(48)
+  }
+#undef TR_CASE
+  return fail(NEGF_ERR_ARG, "trace block size class %d unsupported", nmax);
+}
+
+}  // namespace negf
+
+using namespace negf;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int negf_version(void) { return NEGF_ABI_VERSION; }
+
+const char* negf_last_error(void) { return g_err; }
+
+int negf_max_block(int precision) { return precision == NEGF_FP64 ? 48 : 64; }
+
+int negf_sancho_rubio_batched(void* stream, int precision, int n, int n_inst, const double* lead,
+                              const int32_t* set_of, const double* omega, const double* eta,
+                              double tol, int maxiter, int refine_steps, double* g_out,
+                              double* dos_out, int32_t* iters, int32_t* status) {
+  if (n <= 0 || n_inst < 0 || maxiter < 0 || refine_steps < 0)
+    return fail(NEGF_ERR_ARG, "invalid sizes (n=%d n_inst=%d maxiter=%d)", n, n_inst, maxiter);
+  if (!lead || !set_of || !omega || !eta || !g_out || !iters || !status)
+    return fail(NEGF_ERR_ARG, "null pointer argument");
+  if (n_inst == 0) return NEGF_OK;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const bool fp64 = precision == NEGF_FP64;
+  const int nmax = pick_nmax(n, fp64);
+  if (!nmax) return fail(NEGF_ERR_ARG, "block size %d exceeds the supported maximum", n);
+  SrArgs a{n, n_inst, lead, set_of, omega, eta, tol, maxiter, g_out, dos_out, iters, status};
+  int rc;
+  if (fp64) {
+    rc = run_sr<double>(st, nmax, a);
+  } else if (precision == NEGF_MIXED) {
+    rc = run_sr<float>(st, nmax, a);
+    if (rc == NEGF_OK) {
+      const int nm64 = pick_nmax(n, true);
+      if (!nm64) return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 refinement limit", n);
+      RefineArgs r{n, n_inst, lead, set_of, omega, eta, refine_steps, g_out, dos_out, status};
+      rc = run_refine(st, nm64, r);
+    }
+  } else {
+    return fail(NEGF_ERR_ARG, "unknown precision %d", precision);
+  }
+  return rc;
+}
+
+int negf_self_energy_batched(void* stream, int n_lead, int n_c, int n_dev, int n_inst,
+                             const double* v, const int32_t* set_of, const double* g,
+                             const int32_t* idx, const int32_t* status_in, double* sigma_out) {
+  if (n_lead <= 0 || n_c <= 0 || n_dev <= 0 || n_c > n_dev || n_inst < 0)
+    return fail(NEGF_ERR_ARG, "invalid self-energy sizes (lead %d, contact %d, layer %d)", n_lead,
+                n_c, n_dev);
+  if (!v || !set_of || !g || !idx || !sigma_out) return fail(NEGF_ERR_ARG, "null pointer argument");
+  if (n_inst == 0) return NEGF_OK;
+  int m = n_lead > n_dev ? n_lead : n_dev;
+  const int nmax = pick_nmax(m, true);
+  if (!nmax) return fail(NEGF_ERR_ARG, "self-energy block %d exceeds 48", m);
+  SigmaArgs a{n_lead, n_c, n_dev, n_inst, v, set_of, g, idx, status_in, sigma_out};
+  return run_sigma(reinterpret_cast<hipStream_t>(stream), nmax, a);
+}
+
+int negf_rgf_g1n_batched(void* stream, int precision, int n, int n_layers, int n_inst,
+                         const double* onsite, const double* coupling, const int32_t* set_of,
+                         const double* omega, const double* eta_dev, const double* sigma_l,
+                         const double* sigma_r, const int32_t* status_in, double* g1n_out,
+                         int32_t* status) {
+  if (n <= 0 || n_layers <= 0 || n_inst < 0)
+    return fail(NEGF_ERR_ARG, "invalid RGF sizes (n=%d layers=%d)", n, n_layers);
+  if (!onsite || (n_layers > 1 && !coupling) || !set_of || !omega || !eta_dev || !sigma_l ||
+      !sigma_r || !g1n_out || !status)
+    return fail(NEGF_ERR_ARG, "null pointer argument");
+  if (n_inst == 0) return NEGF_OK;
+  const bool fp64 = precision == NEGF_FP64;
+  const int nmax = pick_nmax(n, fp64);
+  if (!nmax) return fail(NEGF_ERR_ARG, "block size %d exceeds the supported maximum", n);
+  RgfArgs a{n, n_layers, n_inst, onsite, coupling, set_of, omega, eta_dev,
+            sigma_l, sigma_r, status_in, g1n_out, status};
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (fp64) return run_rgf<double>(st, nmax, a);
+  if (precision == NEGF_MIXED) return run_rgf<float>(st, nmax, a);
+  return fail(NEGF_ERR_ARG, "unknown precision %d", precision);
+}
+
+int negf_transmission_trace_batched(void* stream, int n, int n_inst, const double* sigma_l,
+                                    const double* sigma_r, const double* g1n,
+                                    const int32_t* status_in, double* t_out) {
+  if (n <= 0 || n_inst < 0) return fail(NEGF_ERR_ARG, "invalid trace sizes");
+  if (!sigma_l || !sigma_r || !g1n || !t_out) return fail(NEGF_ERR_ARG, "null pointer argument");
+  if (n_inst == 0) return NEGF_OK;
+  const int nmax = pick_nmax(n, true);
+  if (!nmax) return fail(NEGF_ERR_ARG, "trace block %d exceeds 48", n);
+  TraceArgs a{n, n_inst, sigma_l, sigma_r, g1n, status_in, t_out};
+  return run_trace(reinterpret_cast<hipStream_t>(stream), nmax, a);
+}
+
+size_t negf_transmission_workspace_bytes(const negf_transmission_problem* p) {
+  if (!p) return 0;
+  const size_t ni = (size_t)p->n_inst;
+  const size_t nll = (size_t)p->n_lead_l * p->n_lead_l, nlr = (size_t)p->n_lead_r * p->n_lead_r;
+  const size_t nd = (size_t)p->n * p->n;
+  size_t b = 0;
+  b += ni * nll * 16;                              // g_L
+  b += p->share_leads ? 0 : ni * nlr * 16;         // g_R
+  b += 2 * ni * nd * 16;                           // Sigma_L, Sigma_R
+  b += ni * nd * 16;                               // G_1N
+  b += 2 * ni * sizeof(int32_t);                   // lead status L/R
+  return b + 256;
+}
+
+int negf_transmission_batched(void* stream, const negf_transmission_problem* p, void* workspace,
+                              size_t workspace_bytes) {
+  if (!p) return fail(NEGF_ERR_ARG, "null problem");
+  if (p->n_inst == 0) return NEGF_OK;
+  const size_t need = negf_transmission_workspace_bytes(p);
+  if (!workspace || workspace_bytes < need)
+    return fail(NEGF_ERR_ARG, "workspace too small (%zu < %zu bytes)", workspace_bytes, need);
+  const size_t ni = (size_t)p->n_inst;
+  unsigned char* w = reinterpret_cast<unsigned char*>(workspace);
+  double* g_l = reinterpret_cast<double*>(w);
+  w += ni * p->n_lead_l * p->n_lead_l * 16;
+  double* g_r = g_l;
+  if (!p->share_leads) {
+    g_r = reinterpret_cast<double*>(w);
+    w += ni * p->n_lead_r * p->n_lead_r * 16;
+  }
+  const size_t nd = (size_t)p->n * p->n;
+  double* sig_l = reinterpret_cast<double*>(w);
+  w += ni * nd * 16;
+  double* sig_r = reinterpret_cast<double*>(w);
+  w += ni * nd * 16;
+  double* g1n = reinterpret_cast<double*>(w);
+  w += ni * nd * 16;
+  int32_t* st_l = reinterpret_cast<int32_t*>(w);
+  w += ni * sizeof(int32_t);
+  int32_t* st_r = reinterpret_cast<int32_t*>(w);
+
+  int rc = negf_sancho_rubio_batched(stream, p->precision, p->n_lead_l, p->n_inst, p->lead_l,
+                                     p->set_of, p->omega, p->eta_lead, p->tol, p->maxiter,
+                                     p->refine_steps, g_l, p->dos_l, p->iters_l, st_l);
+  if (rc) return rc;
+  if (!p->share_leads) {
+    rc = negf_sancho_rubio_batched(stream, p->precision, p->n_lead_r, p->n_inst, p->lead_r,
+                                   p->set_of, p->omega, p->eta_lead, p->tol, p->maxiter,
+                                   p->refine_steps, g_r, p->dos_r, p->iters_r, st_r);
+    if (rc) return rc;
+  } else {
+    hipError_t e = hipMemcpyAsync(st_r, st_l, ni * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                                  reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_check(e, "status copy");
+  }
+  rc = negf_self_energy_batched(stream, p->n_lead_l, p->n_c_l, p->n, p->n_inst, p->v_l, p->set_of,
+                                g_l, p->idx_l, st_l, sig_l);
+  if (rc) return rc;
+  rc = negf_self_energy_batched(stream, p->n_lead_r, p->n_c_r, p->n, p->n_inst, p->v_r, p->set_of,
+                                g_r, p->idx_r, st_r, sig_r);
+  if (rc) return rc;
+  // merge lead status into st_l (left failure wins, as the reference evaluates the left lead first)
+  rc = negf_merge_status(stream, p->n_inst, st_l, st_r);
+  if (rc) return rc;
+  rc = negf_rgf_g1n_batched(stream, p->precision, p->n, p->n_layers, p->n_inst, p->onsite,
+                            p->coupling, p->set_of, p->omega, p->eta_dev, sig_l, sig_r, st_l, g1n,
+                            p->status);
+  if (rc) return rc;
+  return negf_transmission_trace_batched(stream, p->n, p->n_inst, sig_l, sig_r, g1n, p->status,
+                                         p->t_out);
+}
+
+}  // extern "C"
+
+namespace negf {
+__global__ void k_merge_status(int n, int32_t* a, const int32_t* b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && a[i] == ST_OK) a[i] = b[i];
+}
+}  // namespace negf
+
+extern "C" int negf_merge_status(void* stream, int n_inst, int32_t* st_l, const int32_t* st_r) {
+  if (n_inst <= 0) return NEGF_OK;
+  hipLaunchKernelGGL(negf::k_merge_status, dim3((n_inst + 255) / 256), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), n_inst, st_l, st_r);
+  return hip_check(hipGetLastError(), "merge status");
+}
+
+extern "C" int negf_thermal_conductance(void* stream, int n_w, const double* omega_rad_s,
+                                        const double* trans, int n_t, const double* temps,
+                                        double* out) {
+  if (n_w < 2 || n_t < 0) return fail(NEGF_ERR_ARG, "need >= 2 frequencies");
+  if (!omega_rad_s || !trans || !temps || !out) return fail(NEGF_ERR_ARG, "null pointer argument");
+  if (n_t == 0) return NEGF_OK;
+  hipLaunchKernelGGL(negf::k_conductance, dim3(n_t), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), n_w, omega_rad_s, trans, n_t, temps,
+                     out);
+  return hip_check(hipGetLastError(), "conductance launch");
+}
+
+extern "C" int negf_heat_current(void* stream, int n_w, const double* omega_rad_s,
+                                 const double* trans, double temp_left, double temp_right,
+                                 double* out) {
+  if (n_w < 2) return fail(NEGF_ERR_ARG, "need >= 2 frequencies");
+  if (!(temp_left > 0.0) || !(temp_right > 0.0))
+    return fail(NEGF_ERR_ARG, "temperatures must be positive");
+  if (!omega_rad_s || !trans || !out) return fail(NEGF_ERR_ARG, "null pointer argument");
+  hipLaunchKernelGGL(negf::k_heat_current, dim3(1), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), n_w, omega_rad_s, trans, temp_left,
+                     temp_right, out);
+  return hip_check(hipGetLastError(), "heat current launch");
+}

@@ -1,0 +1,113 @@
+"""Seeded synthetic transport problems used by the benchmark and the tests.
+
+Force constants are sums of pair spring tensors: each interacting atom pair
+(a, b) gets a random positive semi-definite 3x3 tensor K_ab, Phi_ab = -K_ab off
+the diagonal and the acoustic sum rule Phi_aa = sum_b K_ab on it.  That keeps
+D(k) = M^-1/2 Phi(k) M^-1/2 Hermitian and positive semi-definite with an
+acoustic branch at k = 0, as a physical crystal has.  Units: force constants in
+eV/A^2, masses in amu, omega in sqrt(eV/A^2/amu) (OMEGA_UNIT_RAD_S rad/s).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from negfpy_amd.core.types import Device1D, LeadBlocks
+from negfpy_amd.models.multidof import MultiDofCellParams, multidof_device, multidof_lead_blocks
+
+EV_J = 1.602176634e-19
+AMU_KG = 1.66053906660e-27
+ANGSTROM_M = 1e-10
+OMEGA_UNIT_RAD_S = float(np.sqrt(EV_J / (ANGSTROM_M ** 2 * AMU_KG)))
+
+
+def _spring(rng, fc_max):
+    a = rng.uniform(-1.0, 1.0, (3, 3))
+    k = a @ a.T
+    return fc_max * k / np.abs(k).max()
+
+
+def ribbon_params(seed: int = 0, n_atoms: int = 8, fc_max: float = 5.0,
+                  mass_range=(10.0, 30.0)) -> MultiDofCellParams:
+    """Quasi-1D ribbon cell: every atom pair inside the cell and to the next cell is
+    coupled by a random spring tensor with entries in [-fc_max, fc_max]."""
+    rng = np.random.default_rng(seed)
+    masses = rng.uniform(mass_range[0], mass_range[1], n_atoms)
+    nd = 3 * n_atoms
+    fc00 = np.zeros((nd, nd))
+    fc01 = np.zeros((nd, nd))
+    for a in range(n_atoms):
+        for b in range(a + 1, n_atoms):
+            k = _spring(rng, fc_max)
+            fc00[3 * a:3 * a + 3, 3 * b:3 * b + 3] = -k
+            fc00[3 * b:3 * b + 3, 3 * a:3 * a + 3] = -k
+            fc00[3 * a:3 * a + 3, 3 * a:3 * a + 3] += k
+            fc00[3 * b:3 * b + 3, 3 * b:3 * b + 3] += k
+    for a in range(n_atoms):
+        for b in range(n_atoms):
+            k = _spring(rng, fc_max)
+            fc01[3 * a:3 * a + 3, 3 * b:3 * b + 3] = -k
+            fc00[3 * a:3 * a + 3, 3 * a:3 * a + 3] += k
+            fc00[3 * b:3 * b + 3, 3 * b:3 * b + 3] += k
+    return MultiDofCellParams(masses=masses, fc00=fc00, fc01=fc01, dof_per_atom=3)
+
+
+def omega_max(lead: LeadBlocks, nk: int = 64) -> float:
+    """Top of the lead's bulk band: max over kx of sqrt(eig D(kx))."""
+    d00 = np.asarray(lead.d00, dtype=np.complex128)
+    d01 = np.asarray(lead.d01, dtype=np.complex128)
+    d10 = d01.conj().T if lead.d10 is None else np.asarray(lead.d10, dtype=np.complex128)
+    top = 0.0
+    for kx in np.linspace(0.0, np.pi, nk):
+        dk = d00 + d01 * np.exp(1j * kx) + d10 * np.exp(-1j * kx)
+        top = max(top, float(np.linalg.eigvalsh(0.5 * (dk + dk.conj().T)).max()))
+    return float(np.sqrt(max(top, 0.0)))
+
+
+@dataclass
+class RibbonProblem:
+    """Quasi-1D multi-DOF ribbon: n = 3 * n_atoms DOF per layer, mass-disordered device."""
+
+    lead: LeadBlocks
+    device: Device1D
+    omegas: np.ndarray
+    eta: float
+    params: MultiDofCellParams
+    wmax: float
+
+
+def ribbon_problem(seed: int = 0, n_atoms: int = 8, n_layers: int = 32, n_omega: int = 4096,
+                   eta: float = 1e-6, span: float = 1.1) -> RibbonProblem:
+    """The 24-DOF, 32-layer, 4096-omega ribbon workload (for n_atoms = 8).
+
+    Lead cells use the seeded masses; every device layer draws its own masses
+    from the same range, so the device scatters.  omega_i = span * w_max (i + 1/2) / n_omega.
+    """
+    params = ribbon_params(seed, n_atoms)
+    rng = np.random.default_rng(seed + 1)
+    layer_masses = rng.uniform(10.0, 30.0, (n_layers, n_atoms))
+    lead = multidof_lead_blocks(params)
+    device = multidof_device(n_layers, params, layer_masses=layer_masses)
+    wmax = omega_max(lead)
+    omegas = span * wmax * (np.arange(n_omega) + 0.5) / n_omega
+    return RibbonProblem(lead=lead, device=device, omegas=omegas, eta=eta, params=params,
+                         wmax=wmax)
+
+
+def random_lead(seed: int, n: int, scale: float = 1.0):
+    """Small random Hermitian lead (d00, d01) with a real band, for parity tests."""
+    n_atoms = max(1, n // 3)
+    if 3 * n_atoms == n:
+        p = ribbon_params(seed, n_atoms, fc_max=scale)
+        lb = multidof_lead_blocks(p)
+        return np.asarray(lb.d00), np.asarray(lb.d01)
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n))
+    d01 = -scale * (a @ a.conj().T) / n
+    b = rng.uniform(-1, 1, (n, n))
+    d00 = scale * (b @ b.T) / n
+    d00 = d00 - d01 - d01.conj().T
+    d00 = 0.5 * (d00 + d00.conj().T)
+    return d00, d01

@@ -1,0 +1,187 @@
+"""Pin the FP64 CPU oracle before it is trusted as the checker.
+
+Each check is chosen so that a plausible slip (dropped decimation term, wrong
+sign or transpose, wrong branch, wrong integration weight) fails it:
+
+* closed-form surface GF and surface DOS of the monatomic chain;
+* the Riccati fixed point g = (z - d00 - d01 g d10)^-1 on random multi-DOF leads;
+* agreement of decimation with three independent eigenmode routes;
+* three independent device solvers (sparse LU, dense inverse, block RGF);
+* T = channel count for identity-block leads, T = 1 / 0 in / out of the chain band;
+* omega_scale invariance with explicit couplings, and the reference's double
+  division of default couplings (negf.py:177-204);
+* thermal conductance against the reference's own study files T(w) -> k(T)
+  and against the quantum of thermal conductance;
+* the heat current against its closed form for a unit step T(w).
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from negfpy_amd.core.types import LeadBlocks
+from negfpy_amd.workloads import omega_max, random_lead, ribbon_problem
+from oracle import negf_oracle as o
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "study_kappa")
+
+
+def chain_g_closed_form(w, eta, k=1.0, m=1.0):
+    z = (w + 1j * eta) ** 2
+    a, b = 2 * k / m, k / m
+    r = np.sqrt((z - a) ** 2 - 4 * b * b + 0j)
+    roots = [((z - a) + r) / (2 * b * b), ((z - a) - r) / (2 * b * b)]
+    # retarded surface GF: the root whose Bloch factor b*g decays (|b g| < 1)
+    return min(roots, key=lambda g: abs(b * g))
+
+
+@pytest.mark.parametrize("w", [0.05, 0.3, 1.0, 1.7, 1.99, 2.05, 2.5, 3.3])
+@pytest.mark.parametrize("eta", [1e-6, 1e-8])
+def test_chain_surface_gf_closed_form(w, eta):
+    g = o.sancho_rubio(w, np.array([[2.0]]), np.array([[-1.0]]), None, eta)
+    ref = chain_g_closed_form(w, eta)
+    assert abs(g[0, 0] - ref) / abs(ref) < 1e-11
+    assert g[0, 0].imag <= 0.0
+
+
+@pytest.mark.parametrize("w", [0.2, 0.9, 1.5, 1.9])
+def test_chain_surface_dos_closed_form(w):
+    g = o.sancho_rubio(w, np.array([[2.0]]), np.array([[-1.0]]), None, 1e-10)
+    dos = o.surface_dos(w, g)
+    assert abs(dos - w * w * np.sqrt(4 - w * w) / np.pi) < 1e-7
+
+
+@pytest.mark.parametrize("n", [3, 5, 12, 24])
+@pytest.mark.parametrize("eta", [1e-6, 1e-8])
+def test_decimation_solves_riccati_and_matches_eigen_routes(n, eta):
+    d00, d01 = random_lead(n, n)
+    wmax = omega_max(LeadBlocks(d00=d00, d01=d01))
+    # FP64 decimation is conditioned like 1/eta near band features: at eta = 1e-8 its
+    # Riccati residual reaches ~1e-7 at a van Hove point of the 3-DOF lead.
+    bound = {1e-6: 1e-10, 1e-8: 1e-6}[eta]
+    for w in np.linspace(0.05, 1.1, 6) * wmax:
+        g = o.sancho_rubio(w, d00, d01, None, eta)
+        assert o.riccati_residual(w, d00, d01, None, eta, g) < bound
+        nrm = np.linalg.norm(g)
+        # the legacy route inverts twice through an SVD pseudo-inverse: looser agreement
+        for route, tol in ((o.generalized_eigen, bound * 10), (o.generalized_eigen_svd, bound * 10),
+                           (o.legacy_eigen_svd, max(bound * 10, 1e-7))):
+            gr = route(w, d00, d01, None, eta)
+            assert np.linalg.norm(g - gr) / nrm < tol, route.__name__
+
+
+def test_decimation_error_paths():
+    with pytest.raises(RuntimeError, match="did not converge"):
+        o.sancho_rubio(1.0, np.array([[2.0]]), np.array([[-1.0]]), None, 1e-8, maxiter=1)
+    with pytest.raises(RuntimeError):
+        o.sancho_rubio(1.0, np.array([[np.nan]]), np.array([[-1.0]]), None, 1e-8)
+
+
+def _random_device(seed, n, n_layers):
+    d00, d01 = random_lead(seed, n)
+    rng = np.random.default_rng(seed + 100)
+    ons = []
+    for _ in range(n_layers):
+        p = rng.uniform(-0.2, 0.2, (n, n))
+        ons.append(d00 + 0.5 * (p + p.T))
+    cps = [d01 * rng.uniform(0.8, 1.2) for _ in range(n_layers - 1)]
+    return d00, d01, ons, cps
+
+
+@pytest.mark.parametrize("n,nl", [(1, 7), (3, 5), (6, 9), (12, 4)])
+def test_three_device_solvers_agree(n, nl):
+    d00, d01, ons, cps = _random_device(3, n, nl)
+    lead = (d00, d01, None)
+    wmax = omega_max(LeadBlocks(d00=d00, d01=d01))
+    for w in np.linspace(0.1, 1.05, 5) * wmax:
+        ts = o.transmission_sparse(w, ons, cps, lead, lead, eta=1e-7)
+        td = o.transmission_dense(w, ons, cps, lead, lead, eta=1e-7)
+        tr = o.transmission_rgf(w, ons, cps, lead, eta=1e-7)
+        scale = max(abs(ts), 1e-6)
+        assert abs(ts - td) / scale < 1e-9
+        assert abs(ts - tr) / scale < 1e-9
+        assert ts > -1e-10
+
+
+def test_chain_transmission_band():
+    ons = [np.array([[2.0]])] * 20
+    cps = [np.array([[-1.0]])] * 19
+    lead = (np.array([[2.0]]), np.array([[-1.0]]), None)
+    # eta_device = 1e-8 absorbs ~1e-6 over 20 layers; out of band T vanishes
+    for w in np.linspace(0.1, 1.9, 7):
+        assert abs(o.transmission_sparse(w, ons, cps, lead, lead, eta=1e-8) - 1.0) < 5e-6
+    for w in np.linspace(2.1, 3.0, 4):
+        assert abs(o.transmission_sparse(w, ons, cps, lead, lead, eta=1e-8)) < 1e-6
+
+
+def test_identity_blocks_give_channel_count():
+    n = 6
+    d00, d01 = 2.0 * np.eye(n), -np.eye(n)
+    ons = [d00] * 10
+    cps = [d01] * 9
+    t = o.transmission_sparse(1.0, ons, cps, (d00, d01, None), (d00, d01, None), eta=1e-8)
+    assert abs(t - n) < 1e-5
+
+
+def test_omega_scale_conventions():
+    d00, d01, ons, cps = _random_device(5, 4, 5)
+    lead = (d00, d01, None)
+    w = 0.7 * omega_max(LeadBlocks(d00=d00, d01=d01))
+    base = o.transmission_sparse(w, ons, cps, lead, lead, eta=1e-7)
+    # explicit couplings: omega_scale is an exact change of units
+    t_exp = o.transmission_sparse(w, ons, cps, lead, lead, eta=1e-7, v_left=d01.conj().T,
+                                  v_right=d01, omega_scale=3.0)
+    assert abs(t_exp - base) / abs(base) < 1e-9
+    # default couplings are divided by s^2 twice (negf.py:177-204)
+    s = 3.0
+    t_def = o.transmission_sparse(w, ons, cps, lead, lead, eta=1e-7, omega_scale=s)
+    t_equiv = o.transmission_sparse(w, ons, cps, lead, lead, eta=1e-7, v_left=d01.conj().T / s**2,
+                                    v_right=d01 / s**2)
+    assert abs(t_def - t_equiv) / abs(t_equiv) < 1e-9
+    assert abs(t_def - base) / abs(base) > 1e-3
+
+
+@pytest.mark.parametrize("name", ["graphene", "silicon", "tan"])
+def test_thermal_conductance_matches_reference_study_files(name):
+    tr = np.loadtxt(os.path.join(GOLDEN, f"{name}_transmission.tsv"), comments="#")
+    kp = np.loadtxt(os.path.join(GOLDEN, f"{name}_thermal_conductance.tsv"), comments="#")
+    g = o.thermal_conductance(tr[:, 0], tr[:, 3], kp[:, 0])
+    pos = kp[:, 1] > 0
+    assert np.all(g[~pos] == 0.0)
+    assert np.max(np.abs(g[pos] - kp[pos, 1]) / kp[pos, 1]) < 1e-11  # files print 13 digits
+    gpa = o.thermal_conductance(tr[:, 0], tr[:, 4], kp[:, 0])
+    assert np.max(np.abs(gpa[pos] - kp[pos, 2]) / kp[pos, 2]) < 1e-11
+
+
+def test_thermal_conductance_quantum():
+    # unit transmission on a fine grid up to x ~ 60: G -> pi^2 k_B^2 T / (3 h)
+    temp = 300.0
+    wmax = 60.0 * o.KB * temp / o.HBAR
+    w_rad = np.linspace(0.0, wmax, 400001)
+    w_qe = w_rad / float(o.qe_omega_to_rad_s(1.0))
+    g = o.thermal_conductance(w_qe, np.ones_like(w_qe), [temp])[0]
+    g0 = np.pi ** 2 * o.KB ** 2 * temp / (3.0 * 2.0 * np.pi * o.HBAR)
+    assert abs(g - g0) / g0 < 1e-8
+
+
+def test_heat_current_unit_step_closed_form():
+    tl, tr_ = 310.0, 290.0
+    wmax = 80.0 * o.KB * tl / o.HBAR
+    w = np.linspace(0.0, wmax, 400001)
+    j = o.heat_current_from_spectrum(w, np.ones_like(w), tl, tr_)
+    # int_0^inf x/(e^x - 1) dx = pi^2/6  =>  J = (pi^2/6)(k_B^2/hbar)(T_L^2 - T_R^2)/(2 pi)
+    j0 = (np.pi ** 2 / 6.0) * (o.KB ** 2 / o.HBAR) * (tl ** 2 - tr_ ** 2) / (2.0 * np.pi)
+    assert abs(j - j0) / j0 < 1e-8
+    with pytest.raises(ValueError):
+        o.heat_current_from_spectrum([0.0, 1.0], [1.0, 1.0], -1.0, 300.0)
+
+
+def test_ribbon_workload_oracle_consistency():
+    p = ribbon_problem(n_layers=6, n_omega=64)
+    lead = (p.lead.d00, p.lead.d01, None)
+    for w in p.omegas[[3, 20, 40]]:
+        ts = o.transmission_sparse(w, p.device.onsite_blocks, p.device.coupling_blocks, lead, lead,
+                                   eta=p.eta)
+        tr = o.transmission_rgf(w, p.device.onsite_blocks, p.device.coupling_blocks, lead, eta=p.eta)
+        assert abs(ts - tr) <= 1e-9 * max(abs(ts), 1e-3)
