@@ -1,0 +1,224 @@
+"""Benchmark: (omega, k) transmission solves per second on one MI355X.
+
+Workload: the quasi-1D multi-DOF ribbon (8 atoms x 3 DOF = 24 DOF per principal
+layer, 32 device layers, 4096 omega on (0, 1.1 w_max), eta = 1e-6, seeded
+masses in [10, 30] amu and pair force constants in [-5, 5] eV/A^2 with the
+acoustic sum rule).  One step is one full transmission solve of all 4096
+(omega, k) instances: lead decimation, self-energies, recursive Green's function
+sweep and FP64 trace, with all inputs resident in HBM.
+
+The headline value is the mixed path (FP32 decimation and sweep, FP64
+refinement and trace); the FP64 path is timed in the same run, and the mixed
+error against it is reported.  The CPU baseline is the FP64 NumPy/SciPy oracle
+(decimation + SuperLU, the reference algorithm) on a bounded sample of the same
+instances, over a fork pool of the host's cores with one BLAS thread each.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "(ω,k) transmission solves/s, mixed vs FP64 GPU path at ≤1e-6 rel err in T"
+PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}  # MI355X dense matrix peaks (FP64, FP32)
+
+
+def _cpu_worker(args):
+    idx, = args
+    from threadpoolctl import threadpool_limits
+
+    from oracle import negf_oracle as o
+
+    p = _CPU_PROBLEM
+    lead = (p.lead.d00, p.lead.d01, None)
+    out = []
+    with threadpool_limits(limits=1):
+        for i in idx:
+            out.append(o.transmission_sparse(float(p.omegas[i]), p.device.onsite_blocks,
+                                             p.device.coupling_blocks, lead, lead, eta=p.eta))
+    return out
+
+
+_CPU_PROBLEM = None
+
+
+def cpu_baseline(problem, n_sample: int):
+    """Time the oracle on n_sample evenly spaced instances over a fork pool."""
+    import multiprocessing as mp
+
+    global _CPU_PROBLEM
+    _CPU_PROBLEM = problem
+    workers = max(1, min(os.cpu_count() or 1, 16))
+    sel = np.linspace(0, problem.omegas.size - 1, n_sample).astype(int)
+    chunks = [sel[i::workers] for i in range(workers)]
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(workers) as pool:
+        parts = pool.map(_cpu_worker, [(c,) for c in chunks])
+    wall = time.perf_counter() - t0
+    vals = np.empty(n_sample)
+    for c, part in zip(chunks, parts):
+        for i, v in zip(c, part):
+            vals[np.searchsorted(sel, i)] = v
+    return sel, vals, wall, workers
+
+
+def flops_per_instance(n, n_layers, iters, refine_steps, mixed):
+    """Algorithmic real flops of each kernel (complex multiply-add = 8 flops).
+
+    decimation step: one Gauss-Jordan inverse (n^3 cMAC) + 6 products (6 n^3 cMAC)
+    RGF sweep: n_layers inverses + 3 (n_layers - 1) products
+    """
+    n3 = float(n) ** 3
+    sr = 56.0 * n3 * iters + (0.0 if mixed else 8.0 * n3)
+    refine = (8.0 + 24.0 * refine_steps) * n3 if mixed else 0.0
+    rgf = 8.0 * n3 * n_layers + 24.0 * n3 * (n_layers - 1)
+    return {"sancho_rubio": sr, "refine": refine, "rgf": rgf, "self_energy": 16.0 * n3,
+            "trace": 16.0 * n3}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X; "
+              "the multi-GPU path (sharding k points, one all-reduce of T(omega)) is not run.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from negfpy_amd.workloads import ribbon_problem
+
+    problem = ribbon_problem(seed=0, n_atoms=8, n_layers=32, n_omega=4096, eta=1e-6)
+    n = problem.lead.d00.shape[0]
+    n_layers = problem.device.n_layers
+    n_inst = problem.omegas.size
+
+    cpu = None
+    if not args.no_cpu:
+        sel, cpu_vals, wall, workers = cpu_baseline(problem, args.cpu_sample)
+        cpu = {"sel": sel, "vals": cpu_vals, "wall": wall, "workers": workers}
+        print(f"cpu baseline: {sel.size} solves in {wall:.2f} s on {workers} processes",
+              file=sys.stderr, flush=True)
+
+    import torch
+
+    from negfpy_amd.core import _engine
+    from negfpy_amd.core.negf import resolve_k
+
+    torch.cuda.set_device(0)
+    kp = resolve_k(problem.device, problem.lead, problem.lead, None, eta=problem.eta,
+                   eta_device=None, device_to_lead_left=None, device_to_lead_right=None,
+                   contact_left_indices=None, contact_right_indices=None, omega_scale=None,
+                   max_block=64)
+    sets = _engine.TransportSets.from_arrays(kp.lead_l[None], kp.lead_r[None], kp.v_l[None],
+                                             kp.v_r[None], kp.pos_l, kp.pos_r, kp.onsite[None],
+                                             kp.coupling[None], True)
+    set_of = torch.zeros(n_inst, dtype=torch.int32, device="cuda")
+    omega = torch.tensor(problem.omegas, dtype=torch.float64, device="cuda")
+    eta = torch.full((n_inst,), problem.eta, dtype=torch.float64, device="cuda")
+
+    results = {}
+    for prec in ("fp64", "mixed"):
+        solver = _engine.TransmissionSolver(sets, precision=prec)
+        out = solver.alloc_result(n_inst)
+        for _ in range(args.warmup):
+            solver.run(set_of, omega, eta, out=out)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            solver.run(set_of, omega, eta, out=out)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        # per-kernel times: the same kernels launched stage by stage between HIP events
+        ev: dict = {}
+        for _ in range(max(3, min(args.steps, 10))):
+            solver.run_staged(set_of, omega, eta, events=ev)
+        torch.cuda.synchronize()
+        stage_ms = {k: float(np.mean([a.elapsed_time(b) for a, b in v])) for k, v in ev.items()}
+        status = out.status.cpu().numpy()
+        results[prec] = {"t": out.t.cpu().numpy(), "iters": out.iters_l.cpu().numpy(),
+                         "status": status, "s_per_step": dt, "stage_ms": stage_ms}
+        print(f"{prec}: {dt * 1e3:.3f} ms/step, {n_inst / dt:.1f} solves/s, stages {stage_ms}",
+              file=sys.stderr, flush=True)
+
+    t64 = results["fp64"]["t"]
+    t32 = results["mixed"]["t"]
+    ok = (results["fp64"]["status"] == 0) & (results["mixed"]["status"] == 0)
+    rel = np.abs(t32 - t64) / np.maximum(np.abs(t64), 1e-3)
+    rel = rel[ok]
+
+    # roofline of the dominant kernel of the headline (mixed) path
+    mix = results["mixed"]
+    mean_iters = float(np.mean(mix["iters"]))
+    fl = flops_per_instance(n, n_layers, mean_iters, 2, True)
+    stage_to_kernel = {"sancho_rubio_l": "sancho_rubio", "refine_l": "refine", "rgf": "rgf",
+                       "self_energy_l": "self_energy", "trace": "trace"}
+    dom_stage = max((s for s in mix["stage_ms"] if s in stage_to_kernel),
+                    key=lambda s: mix["stage_ms"][s])
+    dom = stage_to_kernel[dom_stage]
+    peak = PEAK_TFLOPS["fp64"] if dom in ("refine", "self_energy", "trace") else PEAK_TFLOPS["mixed"]
+    achieved = fl[dom] * n_inst / (mix["stage_ms"][dom_stage] * 1e-3) / 1e12
+    fp64_fl = flops_per_instance(n, n_layers, float(np.mean(results["fp64"]["iters"])), 0, False)
+    f64_dom = max(("sancho_rubio_l", "rgf"), key=lambda s: results["fp64"]["stage_ms"][s])
+    f64_ach = (fp64_fl[stage_to_kernel[f64_dom]] * n_inst
+               / (results["fp64"]["stage_ms"][f64_dom] * 1e-3) / 1e12)
+
+    value = n_inst / mix["s_per_step"]
+    fp64_value = n_inst / results["fp64"]["s_per_step"]
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mix["s_per_step"] * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex64 (decimation, sweep) + fp64 (refinement, trace)",
+        "data": "synthetic",
+        "config": {"workload": "multi-DOF ribbon: n=24 DOF/layer, 32 layers, 4096 omega, eta=1e-6",
+                   "instances": n_inst, "block": n, "layers": n_layers},
+        "fp64_value": round(fp64_value, 1),
+        "fp64_ms_per_step": round(results["fp64"]["s_per_step"] * 1e3, 4),
+        "mixed_over_fp64": round(value / fp64_value, 3),
+        "mixed_rel_err_vs_fp64": {"max": float(rel.max()), "median": float(np.median(rel)),
+                                  "frac_above_1e-6": float(np.mean(rel > 1e-6)),
+                                  "floor": "relative to max(|T|, 1e-3)"},
+        "mean_decimation_steps": mean_iters,
+        "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3),
+                     "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 5),
+                     "traffic": None,
+                     "fp64_path": {"kernel": stage_to_kernel[f64_dom], "achieved": round(f64_ach, 3),
+                                   "peak": PEAK_TFLOPS["fp64"],
+                                   "frac": round(f64_ach / PEAK_TFLOPS["fp64"], 5)}},
+        "stage_ms": {"mixed": mix["stage_ms"], "fp64": results["fp64"]["stage_ms"]},
+    }
+    if cpu is not None:
+        sel = cpu["sel"]
+        err64 = np.abs(t64[sel] - cpu["vals"]) / np.maximum(np.abs(cpu["vals"]), 1e-3)
+        line["fp64_rel_err_vs_oracle_sample"] = float(err64.max())
+        line["cpu_baseline"] = {
+            "value": round(sel.size / cpu["wall"], 2), "unit": "solves/s",
+            "cores": cpu["workers"], "kind": "port",
+            "sample": f"{sel.size} of the {n_inst} instances (evenly spaced omega), FP64 NumPy/"
+                      f"SciPy oracle: decimation + SuperLU, 1 BLAS thread per process"}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

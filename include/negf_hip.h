@@ -51,7 +51,9 @@ int negf_max_block(int precision);
  *   set_of  [n_inst]           set index of each instance
  *   omega, eta [n_inst]        z = (omega + i eta)^2
  *   tol, maxiter               stop when max(|alpha|, |beta|) < tol
- *   refine_steps               NEGF_MIXED only: FP64 residual steps after the FP32 decimation
+ *   refine_steps               NEGF_MIXED only: FP64 residual steps after the FP32 decimation;
+ *                              < 0 leaves the FP32 decimation's eps_s in g_out (complex128)
+ *                              for a separate negf_surface_refine_batched call
  *   g_out   [n_inst][n][n]     g_s = (z - eps_s)^-1
  *   dos_out [n_inst] or NULL   (omega/pi) Re Tr[i (g - g^H)]
  *   iters, status [n_inst]
@@ -60,6 +62,16 @@ int negf_sancho_rubio_batched(void* stream, int precision, int n, int n_inst, co
                               const int32_t* set_of, const double* omega, const double* eta,
                               double tol, int maxiter, int refine_steps, double* g_out,
                               double* dos_out, int32_t* iters, int32_t* status);
+
+/*
+ * FP64 completion of a mixed-precision decimation: g = (z - eps_s)^-1 from the
+ * eps_s in g_io, then `steps` residual steps g <- (z - d00 - d01 g d10)^-1 (the
+ * fixed-point form of surface_gf.py:81 at convergence), the surface DOS, and g
+ * written back to g_io.  Instances with status != 0 are skipped.
+ */
+int negf_surface_refine_batched(void* stream, int n, int n_inst, const double* lead,
+                                const int32_t* set_of, const double* omega, const double* eta,
+                                int steps, double* g_io, double* dos_out, int32_t* status);
 
 /*
  * Contact self-energy Sigma = V g V^H scattered into an n_dev x n_dev end-layer

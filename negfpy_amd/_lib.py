@@ -59,6 +59,8 @@ _SIGS = {
     "negf_sancho_rubio_batched": (
         [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_dbl, _c_int, _c_int, _vp, _vp, _vp, _vp],
         _c_int),
+    "negf_surface_refine_batched": (
+        [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_int, _vp, _vp, _vp], _c_int),
     "negf_self_energy_batched": (
         [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
     "negf_rgf_g1n_batched": (

@@ -228,6 +228,84 @@ class TransmissionSolver:
         return out
 
 
+    def run_staged(self, set_of, omega, eta_lead, eta_dev=None, events=None
+                   ) -> TransmissionResult:
+        """Same computation as :meth:`run`, one C-ABI call per stage.
+
+        ``events`` (a dict) receives a pair of torch.cuda.Event per stage, recorded
+        on the launch stream around the stage's single kernel, so callers can time
+        every kernel separately.
+        """
+        lib = self._lib
+        s = self.sets
+        set_of = to_dev_i32(set_of)
+        omega = to_dev_f64(omega)
+        eta_lead = to_dev_f64(eta_lead)
+        eta_dev = eta_lead if eta_dev is None else to_dev_f64(eta_dev)
+        n_inst = int(omega.numel())
+        out = self.alloc_result(n_inst)
+        dev = s.onsite.device
+        nl, nr, n = int(s.lead_l.shape[2]), int(s.lead_r.shape[2]), s.n
+        c128 = torch.complex128
+        g_l = torch.empty((n_inst, nl, nl), dtype=c128, device=dev)
+        g_r = g_l if s.share_leads else torch.empty((n_inst, nr, nr), dtype=c128, device=dev)
+        sig_l = torch.empty((n_inst, n, n), dtype=c128, device=dev)
+        sig_r = torch.empty((n_inst, n, n), dtype=c128, device=dev)
+        g1n = torch.empty((n_inst, n, n), dtype=c128, device=dev)
+        st_l = torch.zeros((n_inst,), dtype=torch.int32, device=dev)
+        st_r = st_l.clone() if not s.share_leads else torch.zeros_like(st_l)
+        st = _lib.stream()
+
+        def stage(name, fn):
+            if events is not None:
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record()
+                _lib.check(fn())
+                b.record()
+                events.setdefault(name, []).append((a, b))
+            else:
+                _lib.check(fn())
+
+        mixed = self.precision == _lib.MIXED
+        sides = [("l", s.lead_l, g_l, out.dos_l, out.iters_l, st_l)]
+        if not s.share_leads:
+            sides.append(("r", s.lead_r, g_r, out.dos_r, out.iters_r, st_r))
+        for side, lead, g, dos, iters, stt in sides:
+            nn = int(lead.shape[2])
+            stage(f"sancho_rubio_{side}", lambda lead=lead, g=g, dos=dos, iters=iters, stt=stt, nn=nn:
+                  lib.negf_sancho_rubio_batched(
+                      st, self.precision, nn, n_inst, lead.data_ptr(), set_of.data_ptr(),
+                      omega.data_ptr(), eta_lead.data_ptr(), self.tol, self.maxiter,
+                      -1 if mixed else self.refine_steps, g.data_ptr(), dos.data_ptr(),
+                      iters.data_ptr(), stt.data_ptr()))
+            if mixed:
+                stage(f"refine_{side}", lambda lead=lead, g=g, dos=dos, stt=stt, nn=nn:
+                      lib.negf_surface_refine_batched(
+                          st, nn, n_inst, lead.data_ptr(), set_of.data_ptr(), omega.data_ptr(),
+                          eta_lead.data_ptr(), self.refine_steps, g.data_ptr(), dos.data_ptr(),
+                          stt.data_ptr()))
+        if s.share_leads:
+            st_r.copy_(st_l)
+        stage("self_energy_l", lambda: lib.negf_self_energy_batched(
+            st, nl, int(s.v_l.shape[1]), n, n_inst, s.v_l.data_ptr(), set_of.data_ptr(),
+            g_l.data_ptr(), s.idx_l.data_ptr(), st_l.data_ptr(), sig_l.data_ptr()))
+        stage("self_energy_r", lambda: lib.negf_self_energy_batched(
+            st, nr, int(s.v_r.shape[1]), n, n_inst, s.v_r.data_ptr(), set_of.data_ptr(),
+            g_r.data_ptr(), s.idx_r.data_ptr(), st_r.data_ptr(), sig_r.data_ptr()))
+        _lib.check(lib.negf_merge_status(st, n_inst, st_l.data_ptr(), st_r.data_ptr()))
+        stage("rgf", lambda: lib.negf_rgf_g1n_batched(
+            st, self.precision, n, s.n_layers, n_inst, s.onsite.data_ptr(), s.coupling.data_ptr(),
+            set_of.data_ptr(), omega.data_ptr(), eta_dev.data_ptr(), sig_l.data_ptr(),
+            sig_r.data_ptr(), st_l.data_ptr(), g1n.data_ptr(), out.status.data_ptr()))
+        stage("trace", lambda: lib.negf_transmission_trace_batched(
+            st, n, n_inst, sig_l.data_ptr(), sig_r.data_ptr(), g1n.data_ptr(),
+            out.status.data_ptr(), out.t.data_ptr()))
+        if s.share_leads:
+            out.iters_r.copy_(out.iters_l)
+            out.dos_r.copy_(out.dos_l)
+        return out
+
+
 # ---------------------------------------------------------------------------
 # omega integrals
 # ---------------------------------------------------------------------------

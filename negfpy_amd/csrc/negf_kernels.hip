@@ -940,7 +940,7 @@ int negf_sancho_rubio_batched(void* stream, int precision, int n, int n_inst, co
                               const int32_t* set_of, const double* omega, const double* eta,
                               double tol, int maxiter, int refine_steps, double* g_out,
                               double* dos_out, int32_t* iters, int32_t* status) {
-  if (n <= 0 || n_inst < 0 || maxiter < 0 || refine_steps < 0)
+  if (n <= 0 || n_inst < 0 || maxiter < 0)
     return fail(NEGF_ERR_ARG, "invalid sizes (n=%d n_inst=%d maxiter=%d)", n, n_inst, maxiter);
   if (!lead || !set_of || !omega || !eta || !g_out || !iters || !status)
     return fail(NEGF_ERR_ARG, "null pointer argument");
@@ -954,17 +954,29 @@ int negf_sancho_rubio_batched(void* stream, int precision, int n, int n_inst, co
   if (fp64) {
     rc = run_sr<double>(st, nmax, a);
   } else if (precision == NEGF_MIXED) {
+    if (!pick_nmax(n, true))
+      return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 refinement limit", n);
     rc = run_sr<float>(st, nmax, a);
-    if (rc == NEGF_OK) {
-      const int nm64 = pick_nmax(n, true);
-      if (!nm64) return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 refinement limit", n);
-      RefineArgs r{n, n_inst, lead, set_of, omega, eta, refine_steps, g_out, dos_out, status};
-      rc = run_refine(st, nm64, r);
-    }
+    if (rc == NEGF_OK && refine_steps >= 0)
+      rc = negf_surface_refine_batched(stream, n, n_inst, lead, set_of, omega, eta, refine_steps,
+                                       g_out, dos_out, status);
   } else {
     return fail(NEGF_ERR_ARG, "unknown precision %d", precision);
   }
   return rc;
+}
+
+int negf_surface_refine_batched(void* stream, int n, int n_inst, const double* lead,
+                                const int32_t* set_of, const double* omega, const double* eta,
+                                int steps, double* g_io, double* dos_out, int32_t* status) {
+  if (n <= 0 || n_inst < 0 || steps < 0) return fail(NEGF_ERR_ARG, "invalid refinement sizes");
+  if (!lead || !set_of || !omega || !eta || !g_io || !status)
+    return fail(NEGF_ERR_ARG, "null pointer argument");
+  if (n_inst == 0) return NEGF_OK;
+  const int nm64 = pick_nmax(n, true);
+  if (!nm64) return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 refinement limit", n);
+  RefineArgs r{n, n_inst, lead, set_of, omega, eta, steps, g_io, dos_out, status};
+  return run_refine(reinterpret_cast<hipStream_t>(stream), nm64, r);
 }
 
 int negf_self_energy_batched(void* stream, int n_lead, int n_c, int n_dev, int n_inst,
