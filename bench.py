@@ -29,6 +29,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "(ω,k) transmission solves/s, mixed vs FP64 GPU path at ≤1e-6 rel err in T"
 PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}  # MI355X dense matrix peaks (FP64, FP32)
+# HBM bytes per launch of the dominant kernels, from rocprofv3 FETCH_SIZE + WRITE_SIZE
+# (profiles/); None until measured for the current kernels
+TRAFFIC: dict = {}
 
 
 def _cpu_worker(args):
@@ -160,14 +163,13 @@ def main():
     rel = np.abs(t32 - t64) / np.maximum(np.abs(t64), 1e-3)
     rel = rel[ok]
 
-    # roofline of the dominant kernel of the headline (mixed) path
     mix = results["mixed"]
     mean_iters = float(np.mean(mix["iters"]))
     fl = flops_per_instance(n, n_layers, mean_iters, 2, True)
     stage_to_kernel = {"sancho_rubio_l": "sancho_rubio", "refine_l": "refine", "rgf": "rgf",
                        "self_energy_l": "self_energy", "trace": "trace"}
-    dom_stage = max((s for s in mix["stage_ms"] if s in stage_to_kernel),
-                    key=lambda s: mix["stage_ms"][s])
+    dom_stage = max((k for k in mix["stage_ms"] if k in stage_to_kernel),
+                    key=lambda k: mix["stage_ms"][k])
     dom = stage_to_kernel[dom_stage]
     peak = PEAK_TFLOPS["fp64"] if dom in ("refine", "self_energy", "trace") else PEAK_TFLOPS["mixed"]
     achieved = fl[dom] * n_inst / (mix["stage_ms"][dom_stage] * 1e-3) / 1e12
@@ -176,8 +178,13 @@ def main():
     f64_ach = (fp64_fl[stage_to_kernel[f64_dom]] * n_inst
                / (results["fp64"]["stage_ms"][f64_dom] * 1e-3) / 1e12)
 
-    value = n_inst / mix["s_per_step"]
+    mixed_value = n_inst / mix["s_per_step"]
     fp64_value = n_inst / results["fp64"]["s_per_step"]
+    # the metric counts mixed-precision solves only at <= 1e-6 relative error in T;
+    # when the mixed path misses that bar on this workload the headline is the FP64 path
+    mixed_ok = bool(rel.size and rel.max() <= 1e-6)
+    head = "mixed" if mixed_ok else "fp64"
+    value = mixed_value if mixed_ok else fp64_value
     line = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -185,27 +192,38 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(mix["s_per_step"] * 1e3, 4),
+        "ms_per_step": round(results[head]["s_per_step"] * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "complex64 (decimation, sweep) + fp64 (refinement, trace)",
+        "dtype": ("complex64 (decimation, sweep) + fp64 (refinement, trace)" if mixed_ok
+                  else "complex128"),
+        "headline_path": head,
+        "mixed_meets_1e-6": mixed_ok,
+        "mixed_value": round(mixed_value, 1),
         "data": "synthetic",
         "config": {"workload": "multi-DOF ribbon: n=24 DOF/layer, 32 layers, 4096 omega, eta=1e-6",
                    "instances": n_inst, "block": n, "layers": n_layers},
         "fp64_value": round(fp64_value, 1),
         "fp64_ms_per_step": round(results["fp64"]["s_per_step"] * 1e3, 4),
-        "mixed_over_fp64": round(value / fp64_value, 3),
+        "mixed_over_fp64": round(mixed_value / fp64_value, 3),
         "mixed_rel_err_vs_fp64": {"max": float(rel.max()), "median": float(np.median(rel)),
                                   "frac_above_1e-6": float(np.mean(rel > 1e-6)),
                                   "floor": "relative to max(|T|, 1e-3)"},
         "mean_decimation_steps": mean_iters,
-        "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3),
-                     "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 5),
-                     "traffic": None,
-                     "fp64_path": {"kernel": stage_to_kernel[f64_dom], "achieved": round(f64_ach, 3),
-                                   "peak": PEAK_TFLOPS["fp64"],
-                                   "frac": round(f64_ach / PEAK_TFLOPS["fp64"], 5)}},
+        "roofline": (
+            {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": peak,
+             "unit": "TFLOP/s", "frac": round(achieved / peak, 5), "traffic": TRAFFIC.get(dom),
+             "fp64_path": {"kernel": stage_to_kernel[f64_dom], "achieved": round(f64_ach, 3),
+                           "peak": PEAK_TFLOPS["fp64"],
+                           "frac": round(f64_ach / PEAK_TFLOPS["fp64"], 5)}}
+            if mixed_ok else
+            {"bound": "mfma", "kernel": stage_to_kernel[f64_dom], "achieved": round(f64_ach, 3),
+             "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+             "frac": round(f64_ach / PEAK_TFLOPS["fp64"], 5),
+             "traffic": TRAFFIC.get(stage_to_kernel[f64_dom]),
+             "mixed_path": {"kernel": dom, "achieved": round(achieved, 3), "peak": peak,
+                            "frac": round(achieved / peak, 5)}}),
         "stage_ms": {"mixed": mix["stage_ms"], "fp64": results["fp64"]["stage_ms"]},
     }
     if cpu is not None:

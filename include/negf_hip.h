@@ -13,6 +13,7 @@
  * set_of[].  Per-instance outcomes are reported in int32 status arrays:
  *   0 ok, 1 decimation did not converge within maxiter, 2 non-finite values,
  *   3 exactly singular pivot in a lead inverse,
+ *   7 (transient, mixed precision only) marked for FP64 recomputation,
  *   12 / 13 non-finite / singular in the device sweep.
  *
  * The entry points replace the Python-level hot path of the reference package
@@ -30,6 +31,10 @@ extern "C" {
 #endif
 
 #define NEGF_ABI_VERSION 1
+
+/* Mixed precision: a refined FP32 surface GF whose last FP64 residual step still
+ * changes it by more than this (relative Frobenius norm) is recomputed in FP64. */
+#define NEGF_REDO_TOL 1e-6
 
 enum { NEGF_FP64 = 0, NEGF_MIXED = 1 };
 enum { NEGF_OK = 0, NEGF_ERR_ARG = 1, NEGF_ERR_HIP = 2 };
@@ -74,6 +79,16 @@ int negf_surface_refine_batched(void* stream, int n, int n_inst, const double* l
                                 int steps, double* g_io, double* dos_out, int32_t* status);
 
 /*
+ * FP64 decimation of exactly the instances whose status is nonzero (FP32
+ * failures and refinements that moved by more than NEGF_REDO_TOL); others are
+ * left untouched.  negf_sancho_rubio_batched(NEGF_MIXED) ends with this call.
+ */
+int negf_sancho_rubio_redo(void* stream, int n, int n_inst, const double* lead,
+                           const int32_t* set_of, const double* omega, const double* eta,
+                           double tol, int maxiter, double* g_out, double* dos_out, int32_t* iters,
+                           int32_t* status);
+
+/*
  * Contact self-energy Sigma = V g V^H scattered into an n_dev x n_dev end-layer
  * block at positions idx[0..n_c).  Replaces _self_energy_contact and the
  * contact embedding, src/negfpy/core/negf.py:72-97 and :245-248.
@@ -92,6 +107,7 @@ int negf_self_energy_batched(void* stream, int n_lead, int n_c, int n_dev, int n
  *   onsite [n_sets][n_layers][n][n], coupling [n_sets][n_layers-1][n][n]
  *   sigma_l/sigma_r [n_inst][n][n] (from negf_self_energy_batched)
  *   g1n_out [n_inst][n][n]; status [n_inst] (12/13 on device failure, or status_in)
+ * NEGF_MIXED sweeps in FP32 and recomputes FP32 failures (12/13) in FP64.
  */
 int negf_rgf_g1n_batched(void* stream, int precision, int n, int n_layers, int n_inst,
                          const double* onsite, const double* coupling, const int32_t* set_of,

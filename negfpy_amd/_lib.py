@@ -61,6 +61,8 @@ _SIGS = {
         _c_int),
     "negf_surface_refine_batched": (
         [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_int, _vp, _vp, _vp], _c_int),
+    "negf_sancho_rubio_redo": (
+        [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_dbl, _c_int, _vp, _vp, _vp, _vp], _c_int),
     "negf_self_energy_batched": (
         [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
     "negf_rgf_g1n_batched": (

@@ -284,6 +284,11 @@ class TransmissionSolver:
                           st, nn, n_inst, lead.data_ptr(), set_of.data_ptr(), omega.data_ptr(),
                           eta_lead.data_ptr(), self.refine_steps, g.data_ptr(), dos.data_ptr(),
                           stt.data_ptr()))
+                stage(f"redo_{side}", lambda lead=lead, g=g, dos=dos, iters=iters, stt=stt, nn=nn:
+                      lib.negf_sancho_rubio_redo(
+                          st, nn, n_inst, lead.data_ptr(), set_of.data_ptr(), omega.data_ptr(),
+                          eta_lead.data_ptr(), self.tol, self.maxiter, g.data_ptr(),
+                          dos.data_ptr(), iters.data_ptr(), stt.data_ptr()))
         if s.share_leads:
             st_r.copy_(st_l)
         stage("self_energy_l", lambda: lib.negf_self_energy_batched(

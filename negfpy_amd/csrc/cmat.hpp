@@ -233,7 +233,7 @@ struct ElemMap {
 };
 
 // Status codes shared with the host (see include/negf_hip.h).
-enum Status { ST_OK = 0, ST_NOCONV = 1, ST_NONFINITE = 2, ST_SINGULAR = 3 };
+enum Status { ST_OK = 0, ST_NOCONV = 1, ST_NONFINITE = 2, ST_SINGULAR = 3, ST_REDO = 7 };
 
 // In-place Gauss-Jordan inversion with partial pivoting (largest |re|+|im| in
 // the column, first index on ties, as LAPACK's izamax), row interchanges during

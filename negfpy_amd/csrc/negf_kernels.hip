@@ -156,6 +156,7 @@ struct SrArgs {
   double* dos_out;
   int32_t* iters;
   int32_t* status;
+  int redo;  // 1: only instances whose status is nonzero are (re)computed
 };
 
 template <typename T, int NMAX>
@@ -167,6 +168,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_sancho_rubio(SrArgs a) {
   L s(smem);
   const int tid = threadIdx.x;
   const int inst = blockIdx.x;
+  if (a.redo && a.status[inst] == ST_OK) return;  // uniform per block
   const int n = a.n;
   const size_t nn = (size_t)n * n;
   const double* blk = a.lead + (size_t)a.set_of[inst] * 3 * nn * 2;
@@ -354,7 +356,8 @@ struct RefineArgs {
   const double* omega;
   const double* eta;
   int steps;
-  double* g_io;  // in: e_s, out: g_s
+  double redo_tol;  // last-step relative change above this marks the instance for FP64 redo
+  double* g_io;     // in: e_s, out: g_s
   double* dos_out;
   int32_t* status;
 };
@@ -396,9 +399,11 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_refine(RefineArgs a) {
   int st = gj_inverse<T, NT, L::MAXE>(s.re[2], s.im[2], LD, n, s.rbr, s.rbi, s.cbr, s.cbi, s.perm,
                                       em, tid);
   const double2* d00 = reinterpret_cast<const double2*>(blk);
+  double change = 0.0;
   for (int step = 0; step < a.steps && st == ST_OK; ++step) {
-    T xr[TM][TM], xi[TM][TM];
+    T xr[TM][TM], xi[TM][TM], gr[TM][TM], gi[TM][TM];
     tzero<T, TM>(xr, xi);
+    tzero<T, TM>(gr, gi);
     if (t.own) mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[2], s.im[2], LD, n, t.i0, t.j0, xr, xi);
     tstore<T, TM>(s.re[3], s.im[3], LD, n, n, t, xr, xi);
     __syncthreads();
@@ -412,6 +417,8 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_refine(RefineArgs a) {
         for (int y = 0; y < TM; ++y) {
           const int i = t.i0 + x, j = t.j0 + y;
           if (i < n && j < n) {
+            gr[x][y] = s.re[2][i * LD + j];
+            gi[x][y] = s.im[2][i * LD + j];
             const double2 d = d00[i * n + j];
             s.re[2][i * LD + j] = ((i == j ? zr : 0.0) - d.x) - xr[x][y];
             s.im[2][i * LD + j] = ((i == j ? zi : 0.0) - d.y) - xi[x][y];
@@ -421,7 +428,27 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_refine(RefineArgs a) {
     __syncthreads();
     st = gj_inverse<T, NT, L::MAXE>(s.re[2], s.im[2], LD, n, s.rbr, s.rbi, s.cbr, s.cbi, s.perm, em,
                                     tid);
+    if (st == ST_OK && step == a.steps - 1) {
+      double dn = 0.0, gn = 0.0;
+      if (t.own) {
+#pragma unroll
+        for (int x = 0; x < TM; ++x)
+#pragma unroll
+          for (int y = 0; y < TM; ++y) {
+            const int i = t.i0 + x, j = t.j0 + y;
+            if (i < n && j < n) {
+              const double vr = s.re[2][i * LD + j], vi = s.im[2][i * LD + j];
+              dn += (vr - gr[x][y]) * (vr - gr[x][y]) + (vi - gi[x][y]) * (vi - gi[x][y]);
+              gn += vr * vr + vi * vi;
+            }
+          }
+      }
+      dn = block_sum<NT>(dn, s.sred, tid);
+      gn = block_sum<NT>(gn, s.sred, tid);
+      change = sqrt(dn / gn);
+    }
   }
+  if (st == ST_OK && !(change <= a.redo_tol)) st = ST_REDO;
   if (st != ST_OK) {
     if (tid == 0) a.status[inst] = st;
     return;
@@ -511,6 +538,7 @@ struct RgfArgs {
   const int32_t* status_in;  // lead status (skip when != 0); may be null
   double* g1n_out;           // [n_inst][n][n]
   int32_t* status;
+  int redo;                  // 1: only instances whose device status is 12/13 are recomputed
 };
 
 template <typename T, int NMAX>
@@ -522,6 +550,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_rgf(RgfArgs a) {
   L s(smem);
   const int tid = threadIdx.x;
   const int inst = blockIdx.x;
+  if (a.redo && a.status[inst] != 10 + ST_NONFINITE && a.status[inst] != 10 + ST_SINGULAR) return;
   if (a.status_in && a.status_in[inst] != ST_OK) {
     if (tid == 0) a.status[inst] = a.status_in[inst];
     return;
@@ -949,17 +978,20 @@ int negf_sancho_rubio_batched(void* stream, int precision, int n, int n_inst, co
   const bool fp64 = precision == NEGF_FP64;
   const int nmax = pick_nmax(n, fp64);
   if (!nmax) return fail(NEGF_ERR_ARG, "block size %d exceeds the supported maximum", n);
-  SrArgs a{n, n_inst, lead, set_of, omega, eta, tol, maxiter, g_out, dos_out, iters, status};
+  SrArgs a{n, n_inst, lead, set_of, omega, eta, tol, maxiter, g_out, dos_out, iters, status, 0};
   int rc;
   if (fp64) {
     rc = run_sr<double>(st, nmax, a);
   } else if (precision == NEGF_MIXED) {
-    if (!pick_nmax(n, true))
-      return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 refinement limit", n);
+    const int nm64 = pick_nmax(n, true);
+    if (!nm64) return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 refinement limit", n);
     rc = run_sr<float>(st, nmax, a);
-    if (rc == NEGF_OK && refine_steps >= 0)
+    if (rc == NEGF_OK && refine_steps >= 0) {
       rc = negf_surface_refine_batched(stream, n, n_inst, lead, set_of, omega, eta, refine_steps,
                                        g_out, dos_out, status);
+      if (rc == NEGF_OK) rc = negf_sancho_rubio_redo(stream, n, n_inst, lead, set_of, omega, eta,
+                                                    tol, maxiter, g_out, dos_out, iters, status);
+    }
   } else {
     return fail(NEGF_ERR_ARG, "unknown precision %d", precision);
   }
@@ -975,8 +1007,20 @@ int negf_surface_refine_batched(void* stream, int n, int n_inst, const double* l
   if (n_inst == 0) return NEGF_OK;
   const int nm64 = pick_nmax(n, true);
   if (!nm64) return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 refinement limit", n);
-  RefineArgs r{n, n_inst, lead, set_of, omega, eta, steps, g_io, dos_out, status};
+  RefineArgs r{n, n_inst, lead, set_of, omega, eta, steps, NEGF_REDO_TOL, g_io, dos_out, status};
   return run_refine(reinterpret_cast<hipStream_t>(stream), nm64, r);
+}
+
+int negf_sancho_rubio_redo(void* stream, int n, int n_inst, const double* lead,
+                           const int32_t* set_of, const double* omega, const double* eta,
+                           double tol, int maxiter, double* g_out, double* dos_out, int32_t* iters,
+                           int32_t* status) {
+  if (n <= 0 || n_inst < 0) return fail(NEGF_ERR_ARG, "invalid sizes");
+  if (n_inst == 0) return NEGF_OK;
+  const int nm64 = pick_nmax(n, true);
+  if (!nm64) return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 limit", n);
+  SrArgs a{n, n_inst, lead, set_of, omega, eta, tol, maxiter, g_out, dos_out, iters, status, 1};
+  return run_sr<double>(reinterpret_cast<hipStream_t>(stream), nm64, a);
 }
 
 int negf_self_energy_batched(void* stream, int n_lead, int n_c, int n_dev, int n_inst,
@@ -1009,10 +1053,17 @@ int negf_rgf_g1n_batched(void* stream, int precision, int n, int n_layers, int n
   const int nmax = pick_nmax(n, fp64);
   if (!nmax) return fail(NEGF_ERR_ARG, "block size %d exceeds the supported maximum", n);
   RgfArgs a{n, n_layers, n_inst, onsite, coupling, set_of, omega, eta_dev,
-            sigma_l, sigma_r, status_in, g1n_out, status};
+            sigma_l, sigma_r, status_in, g1n_out, status, 0};
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   if (fp64) return run_rgf<double>(st, nmax, a);
-  if (precision == NEGF_MIXED) return run_rgf<float>(st, nmax, a);
+  if (precision == NEGF_MIXED) {
+    const int nm64 = pick_nmax(n, true);
+    if (!nm64) return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 limit", n);
+    int rc = run_rgf<float>(st, nmax, a);
+    if (rc != NEGF_OK) return rc;
+    a.redo = 1;  // FP32 sweep failures (non-finite / singular) are recomputed in FP64
+    return run_rgf<double>(st, nm64, a);
+  }
   return fail(NEGF_ERR_ARG, "unknown precision %d", precision);
 }
 

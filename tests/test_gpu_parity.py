@@ -1,9 +1,11 @@
 """GPU parity: the HIP path (through the C ABI) against the FP64 CPU oracle.
 
-Bar: FP64 mode within 1e-10 relative (T scaled by max(T, 1e-3)) wherever the
-decimation itself is well conditioned (eta = 1e-6 .. 1e-7); the reference's
-own algorithm is conditioned like 1/eta near band features, so the eta = 1e-8
-cases carry the looser bound that FP64 decimation itself reaches there.
+Bar for FP64 mode: |T_gpu - T_oracle| <= max(1e-10, 50 s) * max(|T|, 1e-3),
+where s is the oracle's own rounding-level spread at that point: the largest
+change of T_oracle when every input block is perturbed by about one ulp
+(three seeded draws).  At most points s ~ 1e-14..1e-12 and the 1e-10 bar
+applies; near band features decimation is conditioned like 1/eta and s reaches
+1e-8, which no FP64 evaluation order can beat.
 """
 
 import numpy as np
@@ -44,6 +46,33 @@ T_TOL = 1e-10
 
 def rel(a, b, floor=1e-3):
     return np.max(np.abs(np.asarray(a) - np.asarray(b)) / np.maximum(np.abs(b), floor))
+
+
+def _ulp_perturb(x, rng):
+    if x is None:
+        return None
+    x = np.asarray(x, dtype=np.complex128)
+    return x * (1.0 + 2.2e-16 * rng.standard_normal(x.shape))
+
+
+def oracle_t(w, ons, cps, ll, lr, **kw):
+    """(T, spread): oracle transmission and its rounding-level spread."""
+    t = o.transmission_sparse(w, ons, cps, ll, lr, **kw)
+    rng = np.random.default_rng(1234)
+    spread = 0.0
+    for _ in range(3):
+        pl = tuple(_ulp_perturb(b, rng) for b in ll)
+        pr = tuple(_ulp_perturb(b, rng) for b in lr)
+        po = [_ulp_perturb(b, rng) for b in ons]
+        pc = [_ulp_perturb(b, rng) for b in cps]
+        spread = max(spread, abs(o.transmission_sparse(w, po, pc, pl, pr, **kw) - t))
+    return t, spread
+
+
+def assert_parity(t_gpu, refs):
+    for tg, (tr, sp) in zip(np.atleast_1d(t_gpu), refs):
+        bound = max(T_TOL * max(abs(tr), 1e-3), 50.0 * sp)
+        assert abs(tg - tr) <= bound, (tg, tr, sp)
 
 
 @pytest.mark.parametrize("n", [1, 3, 6, 12, 16, 24, 30, 48])
@@ -87,9 +116,9 @@ def test_transmission_multidof_matches_oracle(gpu, n_atoms, n_layers):
     p = _ribbon_small(n_atoms, n_layers)
     t = transmission_spectrum(p.omegas, p.device, p.lead, p.lead, eta=p.eta)
     lead = (p.lead.d00, p.lead.d01, None)
-    tr = np.array([o.transmission_sparse(w, p.device.onsite_blocks, p.device.coupling_blocks,
-                                         lead, lead, eta=p.eta) for w in p.omegas])
-    assert rel(t, tr) < T_TOL
+    refs = [oracle_t(w, p.device.onsite_blocks, p.device.coupling_blocks, lead, lead, eta=p.eta)
+            for w in p.omegas]
+    assert_parity(t, refs)
 
 
 def test_transmission_single_point_api(gpu):
@@ -97,10 +126,10 @@ def test_transmission_single_point_api(gpu):
     lead = (p.lead.d00, p.lead.d01, None)
     for w in p.omegas[::6]:
         t = transmission(w, p.device, p.lead, p.lead, eta=1e-7, eta_device=3e-7)
-        tr = o.transmission_sparse(w, p.device.onsite_blocks, p.device.coupling_blocks, lead, lead,
-                                   eta=1e-7, eta_device=3e-7)
+        ref = oracle_t(w, p.device.onsite_blocks, p.device.coupling_blocks, lead, lead,
+                       eta=1e-7, eta_device=3e-7)
         assert isinstance(t, float)
-        assert rel(t, tr) < T_TOL
+        assert_parity(t, [ref])
 
 
 def test_transmission_different_leads(gpu):
@@ -111,9 +140,8 @@ def test_transmission_different_leads(gpu):
     lr = (pr.lead.d00, pr.lead.d01, None)
     for w in pl.omegas[::5]:
         t = transmission(w, pl.device, pl.lead, pr.lead, eta=1e-6)
-        tr = o.transmission_sparse(w, pl.device.onsite_blocks, pl.device.coupling_blocks, ll, lr,
-                                   eta=1e-6)
-        assert rel(t, tr) < T_TOL
+        assert_parity(t, [oracle_t(w, pl.device.onsite_blocks, pl.device.coupling_blocks, ll, lr,
+                                   eta=1e-6)])
 
 
 def test_contacts_and_couplings(gpu):
@@ -136,8 +164,7 @@ def test_contacts_and_couplings(gpu):
                "idx_left": kw.get("contact_left_indices"),
                "idx_right": kw.get("contact_right_indices")}
         t = transmission(w, dev, lead, lead, eta=1e-6, **kw)
-        tr = o.transmission_sparse(w, ons, [d01] * 5, ld, ld, eta=1e-6, **okw)
-        assert rel(t, tr) < T_TOL, kw
+        assert_parity(t, [oracle_t(w, ons, [d01] * 5, ld, ld, eta=1e-6, **okw)])
 
 
 def test_omega_scale_parity(gpu):
@@ -145,9 +172,8 @@ def test_omega_scale_parity(gpu):
     lead = (p.lead.d00, p.lead.d01, None)
     for w in p.omegas[::8]:
         t = transmission(w, p.device, p.lead, p.lead, eta=1e-6, omega_scale=2.5)
-        tr = o.transmission_sparse(w, p.device.onsite_blocks, p.device.coupling_blocks, lead, lead,
-                                   eta=1e-6, omega_scale=2.5)
-        assert rel(t, tr) < T_TOL
+        assert_parity(t, [oracle_t(w, p.device.onsite_blocks, p.device.coupling_blocks, lead,
+                                   lead, eta=1e-6, omega_scale=2.5)])
 
 
 def test_kspace_lattice_matches_oracle(gpu):
@@ -219,9 +245,9 @@ def test_ribbon_workload_parity(gpu):
     t = transmission_spectrum(p.omegas[sel], p.device, p.lead, p.lead, eta=p.eta)
     lead = (p.lead.d00, p.lead.d01, None)
     chk = sel[::8]
-    tr = np.array([o.transmission_sparse(w, p.device.onsite_blocks, p.device.coupling_blocks, lead,
-                                         lead, eta=p.eta) for w in p.omegas[chk]])
-    assert rel(t[::8], tr) < T_TOL
+    refs = [oracle_t(w, p.device.onsite_blocks, p.device.coupling_blocks, lead, lead, eta=p.eta)
+            for w in p.omegas[chk]]
+    assert_parity(t[::8], refs)
     assert np.all(t > -1e-9)
 
 
@@ -234,5 +260,7 @@ def test_mixed_precision_error_bound(gpu):
                                 precision="mixed")
     err = np.abs(t32 - t64) / np.maximum(np.abs(t64), 1e-3)
     print(f"mixed vs fp64: max {err.max():.3e} median {np.median(err):.3e}", flush=True)
-    assert np.median(err) < 1e-3
+    # FP32 sweeps through disordered layers lose digits (documented); failures of the
+    # FP32 kernels are recomputed in FP64, so every instance returns a finite T
     assert np.all(np.isfinite(t32))
+    assert np.median(err) < 1e-4
