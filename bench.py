@@ -95,7 +95,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        src = torch.ones(args.calibrate_copy * (1 << 18), dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_mib": args.calibrate_copy}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X; "
               "the multi-GPU path (sharding k points, one all-reduce of T(omega)) is not run.",
