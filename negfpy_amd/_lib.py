@@ -17,7 +17,8 @@ import threading
 import torch  # noqa: F401  (must precede loading the HIP library)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libnegf_hip.so")
+# NEGFPY_AMD_LIB selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("NEGFPY_AMD_LIB") or os.path.join(HERE, "libnegf_hip.so")
 ABI_VERSION = 1
 
 FP64 = 0

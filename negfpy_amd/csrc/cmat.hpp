@@ -213,127 +213,203 @@ __device__ __forceinline__ void crecip(T ar, T ai, T& rr, T& ri) {
   }
 }
 
-// Per-thread coordinates of the elements e = tid + m*NT of an n x n matrix.
-template <int NT, int MAXE>
-struct ElemMap {
-  int16_t i[MAXE], j[MAXE];
-  int cnt;
-  __device__ __forceinline__ ElemMap(int n, int tid) {
-    const int nn = n * n;
-    cnt = 0;
-#pragma unroll
-    for (int m = 0; m < MAXE; ++m) {
-      const int e = tid + m * NT;
-      const int r = e / n;
-      i[m] = (int16_t)r;
-      j[m] = (int16_t)(e - r * n);
-      if (e < nn) cnt = m + 1;
-    }
-  }
-};
-
 // Status codes shared with the host (see include/negf_hip.h).
 enum Status { ST_OK = 0, ST_NOCONV = 1, ST_NONFINITE = 2, ST_SINGULAR = 3, ST_REDO = 7 };
 
-// In-place Gauss-Jordan inversion with partial pivoting (largest |re|+|im| in
-// the column, first index on ties, as LAPACK's izamax), row interchanges during
-// elimination and the inverse column permutation at the end.  All threads of
-// the workgroup must call it.  Returns ST_OK, ST_SINGULAR (exact zero pivot) or
-// ST_NONFINITE (NaN column).  Ends with a barrier.
-template <typename T, int NT, int MAXE>
-__device__ int gj_inverse(T* __restrict__ re, T* __restrict__ im, int ld, int n, T* rbr, T* rbi,
-                          T* cbr, T* cbi, int* perm, const ElemMap<NT, MAXE>& em, int tid) {
-  const int lane = tid & (WAVE - 1);
+// 32-bit DPP lane moves used by the pivot search (row_mask = bank_mask = 0xF).
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_v(double v) {
+  const long long u = __double_as_longlong(v);
+  const int lo = dpp_i<CTRL>((int)(u & 0xffffffffLL));
+  const int hi = dpp_i<CTRL>((int)(u >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_v(float v) {
+  return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
+}
+
+template <typename T, int CTRL>
+__device__ __forceinline__ void argmax_step(T& best, int& idx) {
+  const T ob = dpp_v<CTRL>(best);
+  const int oi = dpp_i<CTRL>(idx);
+  if (ob > best || (ob == best && oi < idx)) {
+    best = ob;
+    idx = oi;
+  }
+}
+
+// Arg-max (largest value, smallest index on ties) over aligned groups of RG
+// consecutive lanes, RG in {1, 2, 4, 8, 16}; every lane of a group ends with it.
+template <typename T, int RG>
+__device__ __forceinline__ void group_argmax(T& best, int& idx) {
+  if constexpr (RG >= 2) argmax_step<T, 0xB1>(best, idx);   // quad_perm [1,0,3,2]: lane ^ 1
+  if constexpr (RG >= 4) argmax_step<T, 0x4E>(best, idx);   // quad_perm [2,3,0,1]: lane ^ 2
+  if constexpr (RG >= 8) argmax_step<T, 0x141>(best, idx);  // row_half_mirror: the other quad
+  if constexpr (RG >= 16) argmax_step<T, 0x140>(best, idx); // row_mirror: the other half-row
+}
+
+// Pivot of one elimination step, double-buffered by step parity in LDS.
+template <typename T>
+struct Pivot {
+  T ir, ii;  // reciprocal of the pivot element
+  int p;     // pivot row
+  int flag;  // ST_OK, ST_SINGULAR or ST_NONFINITE
+};
+
+template <typename T, int NMAX>
+struct GjBuf {
+  Pivot<T> piv[2];
+  int pv[NMAX];  // pivot row of step k
+  int qv[NMAX];  // step at which row r was the pivot
+};
+
+// 1 / (ar + i ai) with one division.
+template <typename T>
+__device__ __forceinline__ void crecip1(T ar, T ai, T& rr, T& ri) {
+  const T d = T(1) / (ar * ar + ai * ai);
+  rr = ar * d;
+  ri = -ai * d;
+}
+
+// In-place inversion of the n x n LDS matrix (re, im) by Gauss-Jordan
+// elimination with partial pivoting: the pivot of column k is the largest
+// |re| + |im| among the rows not yet used (first index on ties, as LAPACK's
+// izamax).  Rows are never exchanged; step k pivots on row p_k and the result
+// is scattered once at the end, A^-1[q(r)][p_m] = Y[r][m] with q = p^-1.
+//
+// Thread t < n_cols * RG owns column j = t / RG and rows g, g + RG, ...
+// (g = t % RG) in registers, so the RG owners of a column are consecutive lanes
+// of one wave and the pivot search is an RG-lane DPP reduction.  The state
+// after each step is written to LDS, alternating between (re, im) and the free
+// matrix (wre, wim), so one barrier per step suffices: step k reads the pivot
+// row and pivot column of the state written in step k-1.  All threads must
+// call it; it ends with a barrier.  Returns ST_OK, ST_SINGULAR or ST_NONFINITE.
+template <typename T, int NT, int NMAX, int RG>
+__device__ __forceinline__ int gj_inverse(T* __restrict__ re, T* __restrict__ im,
+                                          T* __restrict__ wre, T* __restrict__ wim, int ld, int n,
+                                          GjBuf<T, NMAX>& b, int tid) {
+  constexpr int MAXR = (NMAX + RG - 1) / RG;
+  static_assert(NMAX * RG <= NT, "not enough threads for the inversion layout");
+  const bool on = tid < n * RG;
+  const int j = on ? tid / RG : 0;
+  const int g = tid % RG;
+  T xr[MAXR], xi[MAXR];
+  bool used[MAXR];
+#pragma unroll
+  for (int m = 0; m < MAXR; ++m) {
+    const int i = g + RG * m;
+    used[m] = !(on && i < n);
+    xr[m] = T(0);
+    xi[m] = T(0);
+    if (!used[m]) {
+      xr[m] = re[i * ld + j];
+      xi[m] = im[i * ld + j];
+    }
+  }
+  // pivot of column `col` from the owners' registers; the winner publishes it
+  auto choose = [&](int col, int par) {
+    if (on && j == col) {
+      T best = T(-1);
+      int bidx = n;
+      bool bad = false;
+#pragma unroll
+      for (int m = 0; m < MAXR; ++m) {
+        if (!used[m]) {
+          const T mg = fabs(xr[m]) + fabs(xi[m]);
+          if (mg != mg) bad = true;
+          if (mg > best) {
+            best = mg;
+            bidx = g + RG * m;
+          }
+        }
+      }
+      if (bad) best = T(INFINITY);  // a NaN anywhere in the group wins and marks it
+      group_argmax<T, RG>(best, bidx);
+      if (g == 0 || (bidx < n && g == bidx % RG)) {
+        const int flag = isinf(best) ? ST_NONFINITE : (best > T(0) ? ST_OK : ST_SINGULAR);
+        if (g == 0 && flag != ST_OK) {
+          b.piv[par].flag = flag;
+          b.piv[par].p = 0;
+        }
+        if (flag == ST_OK && g == bidx % RG) {
+          const int mw = bidx / RG;
+#pragma unroll
+          for (int m = 0; m < MAXR; ++m)
+            if (m == mw) crecip1(xr[m], xi[m], b.piv[par].ir, b.piv[par].ii);
+          b.piv[par].p = bidx;
+          b.piv[par].flag = ST_OK;
+        }
+      }
+    }
+  };
+  choose(0, 0);
+  __syncthreads();
   for (int k = 0; k < n; ++k) {
-    T best = T(-1);
-    int bidx = n;
-    bool nan_seen = false;
-    for (int i = k + lane; i < n; i += WAVE) {
-      const T mg = fabs(re[i * ld + k]) + fabs(im[i * ld + k]);
-      if (mg != mg) nan_seen = true;
-      if (mg > best) {
-        best = mg;
-        bidx = i;
-      }
-    }
-#pragma unroll
-    for (int off = WAVE / 2; off > 0; off >>= 1) {
-      const T ob = __shfl_xor(best, off);
-      const int oi = __shfl_xor(bidx, off);
-      if (ob > best || (ob == best && oi < bidx)) {
-        best = ob;
-        bidx = oi;
-      }
-    }
-    const bool any_nan = __any(nan_seen);
-    if (any_nan) {
+    const int par = k & 1;
+    const Pivot<T> pk = b.piv[par];
+    if (pk.flag != ST_OK) {
       __syncthreads();
-      return ST_NONFINITE;
+      return pk.flag;
     }
-    if (!(best > T(0))) {
-      __syncthreads();
-      return ST_SINGULAR;
-    }
-    const int p = bidx;
-    T ivr, ivi;
-    crecip(re[p * ld + k], im[p * ld + k], ivr, ivi);
-    for (int j = tid; j < n; j += NT) {
-      const T vr = re[p * ld + j], vi = im[p * ld + j];
-      if (p != k && j != k) {
-        re[p * ld + j] = re[k * ld + j];
-        im[p * ld + j] = im[k * ld + j];
-      }
-      if (j == k) {
-        rbr[j] = ivr;
-        rbi[j] = ivi;
-      } else {
-        rbr[j] = vr * ivr - vi * ivi;
-        rbi[j] = vr * ivi + vi * ivr;
-      }
-    }
-    for (int i = tid; i < n; i += NT) {
-      const int src = (i == p) ? k : i;
-      cbr[i] = re[src * ld + k];
-      cbi[i] = im[src * ld + k];
-    }
-    if (tid == 0) perm[k] = p;
-    __syncthreads();
+    const int p = pk.p;
+    const T* cre = par ? wre : re;
+    const T* cim = par ? wim : im;
+    T* nre = par ? re : wre;
+    T* nim = par ? im : wim;
+    if (on) {
+      const T pr = cre[p * ld + j], pi = cim[p * ld + j];
+      const T sr = pr * pk.ir - pi * pk.ii, si = pr * pk.ii + pi * pk.ir;
 #pragma unroll
-    for (int m = 0; m < MAXE; ++m) {
-      if (m < em.cnt) {
-        const int i = em.i[m], j = em.j[m];
-        if (tid + m * NT < n * n) {
-          if (i == k) {
-            re[k * ld + j] = rbr[j];
-            im[k * ld + j] = rbi[j];
-          } else if (j == k) {
-            const T ar = cbr[i], ai = cbi[i];
-            re[i * ld + k] = -(ar * ivr - ai * ivi);
-            im[i * ld + k] = -(ar * ivi + ai * ivr);
+      for (int m = 0; m < MAXR; ++m) {
+        const int i = g + RG * m;
+        if (i < n) {
+          if (i == p) {
+            used[m] = true;
+            xr[m] = (j == k) ? pk.ir : sr;
+            xi[m] = (j == k) ? pk.ii : si;
           } else {
-            const T ar = cbr[i], ai = cbi[i], br = rbr[j], bi = rbi[j];
-            T xr = re[i * ld + j], xi = im[i * ld + j];
-            xr = fma(-ar, br, xr);
-            xr = fma(ai, bi, xr);
-            xi = fma(-ar, bi, xi);
-            xi = fma(-ai, br, xi);
-            re[i * ld + j] = xr;
-            im[i * ld + j] = xi;
+            const T cr = cre[i * ld + k], ci = cim[i * ld + k];
+            if (j == k) {
+              xr[m] = -(cr * pk.ir - ci * pk.ii);
+              xi[m] = -(cr * pk.ii + ci * pk.ir);
+            } else {
+              T yr = xr[m], yi = xi[m];
+              yr = fma(-cr, sr, yr);
+              yr = fma(ci, si, yr);
+              yi = fma(-cr, si, yi);
+              yi = fma(-ci, sr, yi);
+              xr[m] = yr;
+              xi[m] = yi;
+            }
+          }
+          if (k + 1 < n) {
+            nre[i * ld + j] = xr[m];
+            nim[i * ld + j] = xi[m];
           }
         }
       }
     }
+    if (tid == 0) {
+      b.pv[k] = p;
+      b.qv[p] = k;
+    }
+    if (k + 1 < n) choose(k + 1, par ^ 1);
     __syncthreads();
   }
-  for (int i = tid; i < n; i += NT) {
-    for (int k = n - 1; k >= 0; --k) {
-      const int p = perm[k];
-      if (p != k) {
-        const T tr = re[i * ld + k], ti = im[i * ld + k];
-        re[i * ld + k] = re[i * ld + p];
-        im[i * ld + k] = im[i * ld + p];
-        re[i * ld + p] = tr;
-        im[i * ld + p] = ti;
+  if (on) {
+    const int pj = b.pv[j];
+#pragma unroll
+    for (int m = 0; m < MAXR; ++m) {
+      const int i = g + RG * m;
+      if (i < n) {
+        const int qi = b.qv[i];
+        re[qi * ld + pj] = xr[m];
+        im[qi * ld + pj] = xi[m];
       }
     }
   }

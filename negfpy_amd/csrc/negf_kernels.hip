@@ -66,19 +66,31 @@ struct Cfg<64> {
   static constexpr int NMAX = 64, NT = 256, TM = 4;
 };
 
+// Waves per SIMD the main kernels must allow: the LDS footprint of one
+// instance admits about this many resident workgroups per CU, so capping
+// registers below that would only leave LDS unused.
+template <typename T, int NMAX>
+struct Occ {
+  static constexpr int value = (NMAX <= 32) ? (sizeof(T) == 8 ? (NMAX <= 16 ? 2 : 3) : 4) : 1;
+};
+
 template <typename T, int NMAX, int NT, int NMAT>
 struct Lds {
-  static constexpr int LD = NMAX + 2;
+  static constexpr int LD = NMAX + 1;
   static constexpr int MSZ = NMAX * LD;
-  static constexpr int MAXE = (NMAX * NMAX + NT - 1) / NT;
+  // rows per column owner group of the inversion: largest power of two <= NT / NMAX
+  static constexpr int RG = (NT / NMAX >= 16) ? 16 : (NT / NMAX >= 8) ? 8 : (NT / NMAX >= 4) ? 4
+                            : (NT / NMAX >= 2) ? 2 : 1;
+  using Gj = GjBuf<T, NMAX>;
+  static constexpr size_t mat_bytes() { return sizeof(T) * (size_t)(2 * NMAT * MSZ); }
+  static constexpr size_t gj_off() { return (mat_bytes() + 15) & ~size_t(15); }
+  static constexpr size_t red_off() { return (gj_off() + sizeof(Gj) + 15) & ~size_t(15); }
   static constexpr size_t bytes() {
-    return sizeof(T) * (size_t)(2 * NMAT * MSZ + 4 * NMAX) + sizeof(int) * NMAX +
-           sizeof(double) * (NT / WAVE) + sizeof(int) * (NT / WAVE) + 64;
+    return red_off() + sizeof(double) * (NT / WAVE) + sizeof(int) * (NT / WAVE) + 16;
   }
   T* re[NMAT];
   T* im[NMAT];
-  T *rbr, *rbi, *cbr, *cbi;
-  int* perm;
+  Gj* gj;
   double* sred;
   int* sflag;
   __device__ __forceinline__ Lds(unsigned char* base) {
@@ -90,15 +102,8 @@ struct Lds {
       im[q] = p;
       p += MSZ;
     }
-    rbr = p;
-    rbi = p + NMAX;
-    cbr = p + 2 * NMAX;
-    cbi = p + 3 * NMAX;
-    p += 4 * NMAX;
-    perm = reinterpret_cast<int*>(p);
-    size_t off = reinterpret_cast<unsigned char*>(perm + NMAX) - base;
-    off = (off + 7) & ~size_t(7);
-    sred = reinterpret_cast<double*>(base + off);
+    gj = reinterpret_cast<Gj*>(base + gj_off());
+    sred = reinterpret_cast<double*>(base + red_off());
     sflag = reinterpret_cast<int*>(sred + NT / WAVE);
   }
 };
@@ -160,7 +165,8 @@ struct SrArgs {
 };
 
 template <typename T, int NMAX>
-__global__ void __launch_bounds__(Cfg<NMAX>::NT) k_sancho_rubio(SrArgs a) {
+__global__ void __launch_bounds__(Cfg<NMAX>::NT)
+    __attribute__((amdgpu_waves_per_eu(Occ<T, NMAX>::value))) k_sancho_rubio(SrArgs a) {
   constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
   using L = Lds<T, NMAX, NT, 4>;
   constexpr int LD = L::LD;
@@ -173,14 +179,13 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_sancho_rubio(SrArgs a) {
   const size_t nn = (size_t)n * n;
   const double* blk = a.lead + (size_t)a.set_of[inst] * 3 * nn * 2;
 
-  lds_zero<T, NT>(smem, L::bytes() - 64, tid);
+  lds_zero<T, NT>(smem, L::mat_bytes(), tid);
   __syncthreads();
   // alpha = d01 -> s.re[0], beta = d10 -> s.re[1]; g -> [2]; work -> [3]
   gload<T, NT>(blk + nn * 2, n, n, s.re[0], s.im[0], LD, tid);
   gload<T, NT>(blk + 2 * nn * 2, n, n, s.re[1], s.im[1], LD, tid);
 
   const Tile<TM> t(n, n, tid);
-  const ElemMap<NT, L::MAXE> em(n, tid);
   T er[TM][TM], ei[TM][TM], sr[TM][TM], si[TM][TM];
   tzero<T, TM>(er, ei);
   if (t.own) {
@@ -229,8 +234,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_sancho_rubio(SrArgs a) {
         }
     }
     __syncthreads();
-    const int st = gj_inverse<T, NT, L::MAXE>(s.re[2], s.im[2], LD, n, s.rbr, s.rbi, s.cbr, s.cbi,
-                                              s.perm, em, tid);
+    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[2], s.im[2], s.re[3], s.im[3], LD, n, *s.gj, tid);
     if (st != ST_OK) {
       status = st;
       break;
@@ -329,8 +333,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_sancho_rubio(SrArgs a) {
         }
     }
     __syncthreads();
-    const int st = gj_inverse<T, NT, L::MAXE>(s.re[2], s.im[2], LD, n, s.rbr, s.rbi, s.cbr, s.cbi,
-                                              s.perm, em, tid);
+    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[2], s.im[2], s.re[3], s.im[3], LD, n, *s.gj, tid);
     if (st != ST_OK) {
       if (tid == 0) a.status[inst] = st;
       return;
@@ -377,12 +380,11 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_refine(RefineArgs a) {
   const size_t nn = (size_t)n * n;
   const double* blk = a.lead + (size_t)a.set_of[inst] * 3 * nn * 2;
   double* io = a.g_io + (size_t)inst * nn * 2;
-  lds_zero<T, NT>(smem, L::bytes() - 64, tid);
+  lds_zero<T, NT>(smem, L::mat_bytes(), tid);
   __syncthreads();
   gload<T, NT>(blk + nn * 2, n, n, s.re[0], s.im[0], LD, tid);      // d01
   gload<T, NT>(blk + 2 * nn * 2, n, n, s.re[1], s.im[1], LD, tid);  // d10
   const Tile<TM> t(n, n, tid);
-  const ElemMap<NT, L::MAXE> em(n, tid);
   double zr, zi;
   zsq(a.omega[inst], a.eta[inst], zr, zi);
   // G = z - e_s
@@ -396,8 +398,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_refine(RefineArgs a) {
     }
   }
   __syncthreads();
-  int st = gj_inverse<T, NT, L::MAXE>(s.re[2], s.im[2], LD, n, s.rbr, s.rbi, s.cbr, s.cbi, s.perm,
-                                      em, tid);
+  int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[2], s.im[2], s.re[3], s.im[3], LD, n, *s.gj, tid);
   const double2* d00 = reinterpret_cast<const double2*>(blk);
   double change = 0.0;
   for (int step = 0; step < a.steps && st == ST_OK; ++step) {
@@ -426,8 +427,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_refine(RefineArgs a) {
         }
     }
     __syncthreads();
-    st = gj_inverse<T, NT, L::MAXE>(s.re[2], s.im[2], LD, n, s.rbr, s.rbi, s.cbr, s.cbi, s.perm, em,
-                                    tid);
+    st = gj_inverse<T, NT, NMAX, L::RG>(s.re[2], s.im[2], s.re[3], s.im[3], LD, n, *s.gj, tid);
     if (st == ST_OK && step == a.steps - 1) {
       double dn = 0.0, gn = 0.0;
       if (t.own) {
@@ -488,11 +488,11 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_self_energy(SigmaArgs a) {
   const int nl = a.n_lead, nc = a.n_c, nd = a.n_dev;
   double2* out = reinterpret_cast<double2*>(a.sigma_out + (size_t)inst * nd * nd * 2);
   if (a.status_in && a.status_in[inst] != ST_OK) return;
-  lds_zero<T, NT>(smem, L::bytes() - 64, tid);
+  lds_zero<T, NT>(smem, L::mat_bytes(), tid);
   __syncthreads();
   gload<T, NT>(a.v + (size_t)a.set_of[inst] * nc * nl * 2, nc, nl, s.re[0], s.im[0], LD, tid);
   gload<T, NT>(a.g + (size_t)inst * nl * nl * 2, nl, nl, s.re[1], s.im[1], LD, tid);
-  int* pos = s.perm;  // inverse contact map: device-layer index -> contact row
+  int* pos = s.gj->pv;  // inverse contact map: device-layer index -> contact row
   for (int i = tid; i < nd; i += NT) pos[i] = -1;
   __syncthreads();
   for (int c = tid; c < nc; c += NT) pos[a.idx[c]] = c;
@@ -542,7 +542,8 @@ struct RgfArgs {
 };
 
 template <typename T, int NMAX>
-__global__ void __launch_bounds__(Cfg<NMAX>::NT) k_rgf(RgfArgs a) {
+__global__ void __launch_bounds__(Cfg<NMAX>::NT)
+    __attribute__((amdgpu_waves_per_eu(Occ<T, NMAX>::value))) k_rgf(RgfArgs a) {
   constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
   using L = Lds<T, NMAX, NT, 4>;
   constexpr int LD = L::LD;
@@ -563,10 +564,9 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_rgf(RgfArgs a) {
   const double2* sgl = reinterpret_cast<const double2*>(a.sigma_l + (size_t)inst * nn * 2);
   const double2* sgr = reinterpret_cast<const double2*>(a.sigma_r + (size_t)inst * nn * 2);
   // LDS: [0] G (inverse), [1] Y = g H, [2] Q = G_{1,i} H_i, [3] H
-  lds_zero<T, NT>(smem, L::bytes() - 64, tid);
+  lds_zero<T, NT>(smem, L::mat_bytes(), tid);
   __syncthreads();
   const Tile<TM> t(n, n, tid);
-  const ElemMap<NT, L::MAXE> em(n, tid);
   double zrd, zid;
   zsq(a.omega[inst], a.eta_dev[inst], zrd, zid);
   const T zr = (T)zrd, zi = (T)zid;
@@ -603,8 +603,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_rgf(RgfArgs a) {
         }
     }
     __syncthreads();
-    const int st = gj_inverse<T, NT, L::MAXE>(s.re[0], s.im[0], LD, n, s.rbr, s.rbi, s.cbr, s.cbi,
-                                              s.perm, em, tid);
+    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[0], s.im[0], s.re[1], s.im[1], LD, n, *s.gj, tid);
     if (st != ST_OK) {
       status = 10 + st;
       break;
@@ -698,7 +697,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_trace(TraceArgs a) {
   const int nn = n * n;
   const double2* sl = reinterpret_cast<const double2*>(a.sigma_l + (size_t)inst * nn * 2);
   const double2* sr = reinterpret_cast<const double2*>(a.sigma_r + (size_t)inst * nn * 2);
-  lds_zero<T, NT>(smem, L::bytes() - 64, tid);
+  lds_zero<T, NT>(smem, L::mat_bytes(), tid);
   __syncthreads();
   // Gamma = i (S - S^H):  Gamma_ij = i (S_ij - conj(S_ji)) = (-(Im S_ij + Im S_ji), Re S_ij - Re S_ji)
   for (int e = tid; e < nn; e += NT) {
