@@ -268,10 +268,21 @@ struct GjBuf {
   int qv[NMAX];  // step at which row r was the pivot
 };
 
-// 1 / (ar + i ai) with one division.
+// 1 / (ar + i ai) = conj(a) / |a|^2; the real reciprocal is the hardware
+// estimate refined by one Newton step (error well below one ulp of the result).
+__device__ __forceinline__ double rcp_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  const double e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ float rcp_nr(float d) {
+  float r = __builtin_amdgcn_rcpf(d);
+  const float e = fmaf(-d, r, 1.0f);
+  return fmaf(r, e, r);
+}
 template <typename T>
 __device__ __forceinline__ void crecip1(T ar, T ai, T& rr, T& ri) {
-  const T d = T(1) / (ar * ar + ai * ai);
+  const T d = rcp_nr(ar * ar + ai * ai);
   rr = ar * d;
   ri = -ai * d;
 }
@@ -351,16 +362,26 @@ __device__ __forceinline__ int gj_inverse(T* __restrict__ re, T* __restrict__ im
   __syncthreads();
   for (int k = 0; k < n; ++k) {
     const int par = k & 1;
+    const T* cre = par ? wre : re;
+    const T* cim = par ? wim : im;
+    T* nre = par ? re : wre;
+    T* nim = par ? im : wim;
+    // column k of the current state does not depend on the pivot: load it
+    // together with the pivot record, then the pivot row entry, and only then
+    // store (the loads must not wait behind stores to the other buffer)
+    T cr[MAXR], ci[MAXR];
+#pragma unroll
+    for (int m = 0; m < MAXR; ++m) {
+      const int i = g + RG * m;
+      cr[m] = (on && i < n) ? cre[i * ld + k] : T(0);
+      ci[m] = (on && i < n) ? cim[i * ld + k] : T(0);
+    }
     const Pivot<T> pk = b.piv[par];
     if (pk.flag != ST_OK) {
       __syncthreads();
       return pk.flag;
     }
     const int p = pk.p;
-    const T* cre = par ? wre : re;
-    const T* cim = par ? wim : im;
-    T* nre = par ? re : wre;
-    T* nim = par ? im : wim;
     if (on) {
       const T pr = cre[p * ld + j], pi = cim[p * ld + j];
       const T sr = pr * pk.ir - pi * pk.ii, si = pr * pk.ii + pi * pk.ir;
@@ -372,22 +393,25 @@ __device__ __forceinline__ int gj_inverse(T* __restrict__ re, T* __restrict__ im
             used[m] = true;
             xr[m] = (j == k) ? pk.ir : sr;
             xi[m] = (j == k) ? pk.ii : si;
+          } else if (j == k) {
+            xr[m] = -(cr[m] * pk.ir - ci[m] * pk.ii);
+            xi[m] = -(cr[m] * pk.ii + ci[m] * pk.ir);
           } else {
-            const T cr = cre[i * ld + k], ci = cim[i * ld + k];
-            if (j == k) {
-              xr[m] = -(cr * pk.ir - ci * pk.ii);
-              xi[m] = -(cr * pk.ii + ci * pk.ir);
-            } else {
-              T yr = xr[m], yi = xi[m];
-              yr = fma(-cr, sr, yr);
-              yr = fma(ci, si, yr);
-              yi = fma(-cr, si, yi);
-              yi = fma(-ci, sr, yi);
-              xr[m] = yr;
-              xi[m] = yi;
-            }
+            T yr = xr[m], yi = xi[m];
+            yr = fma(-cr[m], sr, yr);
+            yr = fma(ci[m], si, yr);
+            yi = fma(-cr[m], si, yi);
+            yi = fma(-ci[m], sr, yi);
+            xr[m] = yr;
+            xi[m] = yi;
           }
-          if (k + 1 < n) {
+        }
+      }
+      if (k + 1 < n) {
+#pragma unroll
+        for (int m = 0; m < MAXR; ++m) {
+          const int i = g + RG * m;
+          if (i < n) {
             nre[i * ld + j] = xr[m];
             nim[i * ld + j] = xi[m];
           }
