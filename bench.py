@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--paths", default="fp64,mixed",
+                    help="comma list of precision paths to run (profiling one path alone)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -145,8 +147,9 @@ def main():
     omega = torch.tensor(problem.omegas, dtype=torch.float64, device="cuda")
     eta = torch.full((n_inst,), problem.eta, dtype=torch.float64, device="cuda")
 
+    paths = [q for q in args.paths.split(",") if q]
     results = {}
-    for prec in ("fp64", "mixed"):
+    for prec in paths:
         solver = _engine.TransmissionSolver(sets, precision=prec)
         out = solver.alloc_result(n_inst)
         for _ in range(args.warmup):
@@ -169,6 +172,11 @@ def main():
         print(f"{prec}: {dt * 1e3:.3f} ms/step, {n_inst / dt:.1f} solves/s, stages {stage_ms}",
               file=sys.stderr, flush=True)
 
+    if set(paths) != {"fp64", "mixed"}:
+        print(json.dumps({"paths": paths, "ms_per_step": {k: v["s_per_step"] * 1e3
+                                                          for k, v in results.items()},
+                          "stage_ms": {k: v["stage_ms"] for k, v in results.items()}}))
+        return
     t64 = results["fp64"]["t"]
     t32 = results["mixed"]["t"]
     ok = (results["fp64"]["status"] == 0) & (results["mixed"]["status"] == 0)
