@@ -116,6 +116,18 @@ int negf_rgf_g1n_batched(void* stream, int precision, int n, int n_layers, int n
                          int32_t* status);
 
 /*
+ * Full device Green's function G = A^-1 of A = (omega + i eta_dev)^2 - D - Sigma_L - Sigma_R,
+ * FP64, every block: a left-connected sweep followed by a backward pass over all
+ * L^2 blocks.  Replaces device_green_function's dense solve,
+ * src/negfpy/core/negf.py:253-303.
+ *   g_out [n_inst][L n][L n]; status [n_inst] (12/13 on device failure)
+ */
+int negf_device_green_batched(void* stream, int n, int n_layers, int n_inst, const double* onsite,
+                              const double* coupling, const int32_t* set_of, const double* omega,
+                              const double* eta_dev, const double* sigma_l, const double* sigma_r,
+                              double* g_out, int32_t* status);
+
+/*
  * T = Re Tr[Gamma_L G_1N Gamma_R G_1N^H], Gamma = i (Sigma - Sigma^H), in FP64.
  * Replaces _broadening and the trace, src/negfpy/core/negf.py:115-116, :346-357.
  * Instances with status_in != 0 get NaN.

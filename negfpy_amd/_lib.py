@@ -69,6 +69,8 @@ _SIGS = {
     "negf_rgf_g1n_batched": (
         [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
         _c_int),
+    "negf_device_green_batched": (
+        [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
     "negf_transmission_trace_batched": ([_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp], _c_int),
     "negf_merge_status": ([_vp, _c_int, _vp, _vp], _c_int),
     "negf_transmission_workspace_bytes": ([ctypes.POINTER(TransmissionProblem)], ctypes.c_size_t),

@@ -9,6 +9,7 @@ from .dos import (
     leads_surface_dos_spectrum,
 )
 from .negf import (
+    device_green_function,
     transmission,
     transmission_kavg,
     transmission_kavg_adaptive,
@@ -45,6 +46,7 @@ __all__ = [
     "lead_surface_dos_kavg_adaptive",
     "lead_surface_dos_spectrum",
     "leads_surface_dos_spectrum",
+    "device_green_function",
     "surface_gf",
     "surface_gf_generalized_eigen",
     "surface_gf_generalized_eigen_svd",

@@ -311,6 +311,51 @@ class TransmissionSolver:
         return out
 
 
+def device_green(sets: TransportSets, set_of, omega, eta_lead, eta_dev, *, tol=1e-12,
+                 maxiter=200):
+    """Full device Green's function of every instance (FP64).
+
+    Returns (G [n_inst, Ln, Ln], Sigma_L, Sigma_R [n_inst, n, n] end-layer blocks,
+    lead status, device status), all torch tensors on the GPU.
+    """
+    lib = _lib.load()
+    set_of = to_dev_i32(set_of)
+    omega = to_dev_f64(omega)
+    eta_lead = to_dev_f64(eta_lead)
+    eta_dev = to_dev_f64(eta_dev)
+    n_inst = int(omega.numel())
+    dev = sets.onsite.device
+    n, nl = sets.n, sets.n_layers
+    c128 = torch.complex128
+    st = _lib.stream()
+    sig = []
+    stats = []
+    for lead, v, idx in ((sets.lead_l, sets.v_l, sets.idx_l), (sets.lead_r, sets.v_r, sets.idx_r)):
+        nld = int(lead.shape[2])
+        g = torch.empty((n_inst, nld, nld), dtype=c128, device=dev)
+        dos = torch.empty((n_inst,), dtype=torch.float64, device=dev)
+        it = torch.zeros((n_inst,), dtype=torch.int32, device=dev)
+        stt = torch.zeros((n_inst,), dtype=torch.int32, device=dev)
+        _lib.check(lib.negf_sancho_rubio_batched(
+            st, _lib.FP64, nld, n_inst, lead.data_ptr(), set_of.data_ptr(), omega.data_ptr(),
+            eta_lead.data_ptr(), float(tol), int(maxiter), 0, g.data_ptr(), dos.data_ptr(),
+            it.data_ptr(), stt.data_ptr()))
+        sg = torch.zeros((n_inst, n, n), dtype=c128, device=dev)
+        _lib.check(lib.negf_self_energy_batched(
+            st, nld, int(v.shape[1]), n, n_inst, v.data_ptr(), set_of.data_ptr(), g.data_ptr(),
+            idx.data_ptr(), stt.data_ptr(), sg.data_ptr()))
+        sig.append(sg)
+        stats.append(stt)
+    lead_status = torch.where(stats[0] != 0, stats[0], stats[1])
+    gfull = torch.zeros((n_inst, n * nl, n * nl), dtype=c128, device=dev)
+    dst = torch.zeros((n_inst,), dtype=torch.int32, device=dev)
+    _lib.check(lib.negf_device_green_batched(
+        st, n, nl, n_inst, sets.onsite.data_ptr(), sets.coupling.data_ptr(), set_of.data_ptr(),
+        omega.data_ptr(), eta_dev.data_ptr(), sig[0].data_ptr(), sig[1].data_ptr(),
+        gfull.data_ptr(), dst.data_ptr()))
+    return gfull, sig[0], sig[1], lead_status, dst
+
+
 # ---------------------------------------------------------------------------
 # omega integrals
 # ---------------------------------------------------------------------------

@@ -261,6 +261,49 @@ def _resolve_many(device, lead_left, lead_right, kpoints, *, eta, eta_device, de
                       max_block=mb) for k in kpoints]
 
 
+def device_green_function(omega: float, device: DeviceLike, lead_left: LeadLike,
+                          lead_right: LeadLike, eta: float = 1e-8, eta_device: float | None = None,
+                          kpar: KPar = None, device_to_lead_left: DeviceToLeadLike | None = None,
+                          device_to_lead_right: DeviceToLeadLike | None = None,
+                          contact_left_indices: ContactIndices = None,
+                          contact_right_indices: ContactIndices = None,
+                          surface_gf_method: str = "sancho_rubio",
+                          omega_scale: float | None = None) -> tuple[Array, Array, Array]:
+    """(G, Sigma_L, Sigma_R) of the finite device, each dim x dim (negf.py:253-303).
+
+    G is every block of A^-1 from the GPU block sweep (FP64); the self-energies
+    are embedded at the contact indices like _embed_self_energies (negf.py:100-112).
+    """
+    from ._status import raise_for_lead_status
+
+    canonical_method(surface_gf_method)
+    kp = resolve_k(device, lead_left, lead_right, kpar, eta=eta, eta_device=eta_device,
+                   device_to_lead_left=device_to_lead_left,
+                   device_to_lead_right=device_to_lead_right,
+                   contact_left_indices=contact_left_indices,
+                   contact_right_indices=contact_right_indices, omega_scale=omega_scale,
+                   max_block=_max_block("fp64"))
+    sets = _engine.TransportSets.from_arrays(
+        kp.lead_l[None], kp.lead_r[None], kp.v_l[None], kp.v_r[None], kp.pos_l, kp.pos_r,
+        kp.onsite[None], kp.coupling[None] if kp.onsite.shape[0] > 1 else None, False)
+    eta_dev = float(eta if eta_device is None else eta_device)
+    g, sl, sr, lst, dst = _engine.device_green(sets, [0], [float(omega) / kp.scale],
+                                               [float(eta) / kp.scale], [eta_dev / kp.scale])
+    raise_for_lead_status(int(lst[0].item()))
+    code = int(dst[0].item())
+    if code == 13:
+        raise RuntimeError("Factor is exactly singular")
+    dim = kp.dim
+    gm = g[0].cpu().numpy()[:dim, :dim]
+    if code == 12:
+        gm = np.full((dim, dim), np.nan, dtype=np.complex128)
+    sig_l = np.zeros((dim, dim), dtype=np.complex128)
+    sig_r = np.zeros((dim, dim), dtype=np.complex128)
+    sig_l[np.ix_(kp.idx_l, kp.idx_l)] = sl[0].cpu().numpy()[np.ix_(kp.pos_l, kp.pos_l)]
+    sig_r[np.ix_(kp.idx_r, kp.idx_r)] = sr[0].cpu().numpy()[np.ix_(kp.pos_r, kp.pos_r)]
+    return gm, sig_l, sig_r
+
+
 def transmission(omega: float, device: DeviceLike, lead_left: LeadLike, lead_right: LeadLike,
                  eta: float = 1e-8, eta_device: float | None = None, kpar: KPar = None,
                  device_to_lead_left: DeviceToLeadLike | None = None,

@@ -667,6 +667,199 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
 }
 
 // ---------------------------------------------------------------------------
+// Full device Green's function G = A^-1 (FP64), all L^2 blocks
+// ---------------------------------------------------------------------------
+
+struct DgfArgs {
+  int n, n_layers, n_inst;
+  const double* onsite;
+  const double* coupling;
+  const int32_t* set_of;
+  const double* omega;
+  const double* eta_dev;
+  const double* sigma_l;
+  const double* sigma_r;
+  double* g_out;  // [n_inst][L n][L n]
+  int32_t* status;
+};
+
+// LDS <-> global block moves for an n x n block with global leading dim `gld` (complex).
+template <typename T, int NT>
+__device__ __forceinline__ void bload(const double* __restrict__ g, int gld, int n, T* re, T* im,
+                                      int ld, int tid) {
+  const double2* s = reinterpret_cast<const double2*>(g);
+  for (int e = tid; e < n * n; e += NT) {
+    const int i = e / n, j = e - (e / n) * n;
+    const double2 v = s[(size_t)i * gld + j];
+    re[i * ld + j] = (T)v.x;
+    im[i * ld + j] = (T)v.y;
+  }
+}
+
+template <typename T, int TM>
+__device__ __forceinline__ void tstore_glb(double* __restrict__ g, int gld, int n, const Tile<TM>& t,
+                                           const T (&cr)[TM][TM], const T (&ci)[TM][TM]) {
+  if (!t.own) return;
+  double2* d = reinterpret_cast<double2*>(g);
+#pragma unroll
+  for (int x = 0; x < TM; ++x)
+#pragma unroll
+    for (int y = 0; y < TM; ++y) {
+      const int i = t.i0 + x, j = t.j0 + y;
+      if (i < n && j < n) d[(size_t)i * gld + j] = make_double2((double)cr[x][y], (double)ci[x][y]);
+    }
+}
+
+template <int NMAX>
+__global__ void __launch_bounds__(Cfg<NMAX>::NT) k_device_green(DgfArgs a) {
+  using T = double;
+  constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
+  using L = Lds<T, NMAX, NT, 4>;
+  constexpr int LD = L::LD;
+  extern __shared__ __align__(16) unsigned char smem[];
+  L s(smem);
+  const int tid = threadIdx.x;
+  const int inst = blockIdx.x;
+  const int n = a.n, nl = a.n_layers, dim = n * nl;
+  const size_t nn = (size_t)n * n;
+  const size_t set = (size_t)a.set_of[inst];
+  const double2* ons = reinterpret_cast<const double2*>(a.onsite + set * nl * nn * 2);
+  const double* cpl = a.coupling + set * (nl > 1 ? nl - 1 : 0) * nn * 2;
+  const double2* sgl = reinterpret_cast<const double2*>(a.sigma_l + (size_t)inst * nn * 2);
+  const double2* sgr = reinterpret_cast<const double2*>(a.sigma_r + (size_t)inst * nn * 2);
+  double* G = a.g_out + (size_t)inst * dim * dim * 2;
+  auto blk = [&](int bi, int bj) { return G + ((size_t)bi * n * dim + (size_t)bj * n) * 2; };
+  lds_zero<T, NT>(smem, L::mat_bytes(), tid);
+  __syncthreads();
+  const Tile<TM> t(n, n, tid);
+  double zr, zi;
+  zsq(a.omega[inst], a.eta_dev[inst], zr, zi);
+  T sr[TM][TM], si[TM][TM];
+  tzero<T, TM>(sr, si);
+  int status = ST_OK;
+  // forward sweep: left-connected g_i stored in the diagonal blocks of G
+  for (int layer = 0; layer < nl && status == ST_OK; ++layer) {
+    if (t.own) {
+      const double2* d = ons + (size_t)layer * nn;
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          const int i = t.i0 + x, j = t.j0 + y;
+          if (i < n && j < n) {
+            const double2 dv = d[i * n + j];
+            T vr = (i == j ? zr : 0.0) - dv.x - sr[x][y];
+            T vi = (i == j ? zi : 0.0) - dv.y - si[x][y];
+            if (layer == 0) {
+              vr -= sgl[i * n + j].x;
+              vi -= sgl[i * n + j].y;
+            }
+            if (layer == nl - 1) {
+              vr -= sgr[i * n + j].x;
+              vi -= sgr[i * n + j].y;
+            }
+            s.re[0][i * LD + j] = vr;
+            s.im[0][i * LD + j] = vi;
+          }
+        }
+    }
+    __syncthreads();
+    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[0], s.im[0], s.re[1], s.im[1], LD, n, *s.gj,
+                                                  tid);
+    if (st != ST_OK) {
+      status = 10 + st;
+      break;
+    }
+    {
+      T gr[TM][TM], gi[TM][TM];
+      tzero<T, TM>(gr, gi);
+      if (t.own) {
+#pragma unroll
+        for (int x = 0; x < TM; ++x)
+#pragma unroll
+          for (int y = 0; y < TM; ++y) {
+            const int i = t.i0 + x, j = t.j0 + y;
+            if (i < n && j < n) {
+              gr[x][y] = s.re[0][i * LD + j];
+              gi[x][y] = s.im[0][i * LD + j];
+            }
+          }
+      }
+      tstore_glb<T, TM>(blk(layer, layer), dim, n, t, gr, gi);
+    }
+    if (layer == nl - 1) break;
+    gload<T, NT>(cpl + (size_t)layer * nn * 2, n, n, s.re[2], s.im[2], LD, tid);
+    __syncthreads();
+    T yr[TM][TM], yi[TM][TM];
+    tzero<T, TM>(yr, yi);
+    if (t.own) mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[2], s.im[2], LD, n, t.i0, t.j0, yr, yi);
+    tstore<T, TM>(s.re[3], s.im[3], LD, n, n, t, yr, yi);
+    __syncthreads();
+    tzero<T, TM>(sr, si);
+    if (t.own) mac<T, TM, OPH, OPN>(s.re[2], s.im[2], s.re[3], s.im[3], LD, n, t.i0, t.j0, sr, si);
+    __syncthreads();
+  }
+  if (tid == 0) a.status[inst] = status;
+  if (status != ST_OK) return;
+  __threadfence_block();
+  __syncthreads();
+  // backward pass: rows/columns i from the finished blocks of i + 1
+  for (int i = nl - 2; i >= 0; --i) {
+    bload<T, NT>(blk(i, i), dim, n, s.re[0], s.im[0], LD, tid);  // g_i
+    gload<T, NT>(cpl + (size_t)i * nn * 2, n, n, s.re[2], s.im[2], LD, tid);  // H_i
+    __syncthreads();
+    T ur[TM][TM], ui[TM][TM];
+    tzero<T, TM>(ur, ui);
+    if (t.own) mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[2], s.im[2], LD, n, t.i0, t.j0, ur, ui);
+    T vr[TM][TM], vi[TM][TM];
+    tzero<T, TM>(vr, vi);
+    if (t.own) mac<T, TM, OPH, OPN>(s.re[2], s.im[2], s.re[0], s.im[0], LD, n, t.i0, t.j0, vr, vi);
+    tstore<T, TM>(s.re[1], s.im[1], LD, n, n, t, ur, ui);  // U = g_i H_i
+    tstore<T, TM>(s.re[3], s.im[3], LD, n, n, t, vr, vi);  // V = H_i^H g_i
+    __syncthreads();
+    for (int j = i + 1; j < nl; ++j) {
+      // G_{i,j} = U G_{i+1,j}
+      bload<T, NT>(blk(i + 1, j), dim, n, s.re[2], s.im[2], LD, tid);
+      __syncthreads();
+      T cr[TM][TM], ci[TM][TM];
+      tzero<T, TM>(cr, ci);
+      if (t.own) mac<T, TM, OPN, OPN>(s.re[1], s.im[1], s.re[2], s.im[2], LD, n, t.i0, t.j0, cr, ci);
+      tstore_glb<T, TM>(blk(i, j), dim, n, t, cr, ci);
+      __syncthreads();
+      // G_{j,i} = G_{j,i+1} V
+      bload<T, NT>(blk(j, i + 1), dim, n, s.re[2], s.im[2], LD, tid);
+      __syncthreads();
+      tzero<T, TM>(cr, ci);
+      if (t.own) mac<T, TM, OPN, OPN>(s.re[2], s.im[2], s.re[3], s.im[3], LD, n, t.i0, t.j0, cr, ci);
+      tstore_glb<T, TM>(blk(j, i), dim, n, t, cr, ci);
+      __threadfence_block();
+      __syncthreads();
+    }
+    // G_{i,i} = g_i + U G_{i+1,i}
+    bload<T, NT>(blk(i + 1, i), dim, n, s.re[2], s.im[2], LD, tid);
+    __syncthreads();
+    T cr[TM][TM], ci[TM][TM];
+    tzero<T, TM>(cr, ci);
+    if (t.own) {
+      mac<T, TM, OPN, OPN>(s.re[1], s.im[1], s.re[2], s.im[2], LD, n, t.i0, t.j0, cr, ci);
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          const int r = t.i0 + x, c = t.j0 + y;
+          if (r < n && c < n) {
+            cr[x][y] += s.re[0][r * LD + c];
+            ci[x][y] += s.im[0][r * LD + c];
+          }
+        }
+    }
+    tstore_glb<T, TM>(blk(i, i), dim, n, t, cr, ci);
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // T = Re Tr[Gamma_L G Gamma_R G^H]  (FP64)
 // ---------------------------------------------------------------------------
 
@@ -948,6 +1141,24 @@ static int run_trace(hipStream_t st, int nmax, const TraceArgs& a) {
   return fail(NEGF_ERR_ARG, "trace block size class %d unsupported", nmax);
 }
 
+static int run_dgf(hipStream_t st, int nmax, const DgfArgs& a) {
+#define DG_CASE(NM)                                                                           \
+  case NM: {                                                                                  \
+    constexpr int NT = Cfg<NM>::NT;                                                           \
+    NEGF_LAUNCH((k_device_green<NM>), a.n_inst, NT, (Lds<double, NM, NT, 4>::bytes()), st, a); \
+    return NEGF_OK;                                                                           \
+  }
+  switch (nmax) {
+    DG_CASE(8)
+    DG_CASE(16)
+    DG_CASE(24)
+    DG_CASE(32)
+    DG_CASE(48)
+  }
+#undef DG_CASE
+  return fail(NEGF_ERR_ARG, "block size class %d unsupported", nmax);
+}
+
 }  // namespace negf
 
 using namespace negf;
@@ -1064,6 +1275,23 @@ int negf_rgf_g1n_batched(void* stream, int precision, int n, int n_layers, int n
     return run_rgf<double>(st, nm64, a);
   }
   return fail(NEGF_ERR_ARG, "unknown precision %d", precision);
+}
+
+int negf_device_green_batched(void* stream, int n, int n_layers, int n_inst, const double* onsite,
+                              const double* coupling, const int32_t* set_of, const double* omega,
+                              const double* eta_dev, const double* sigma_l, const double* sigma_r,
+                              double* g_out, int32_t* status) {
+  if (n <= 0 || n_layers <= 0 || n_inst < 0)
+    return fail(NEGF_ERR_ARG, "invalid sizes (n=%d layers=%d)", n, n_layers);
+  if (!onsite || (n_layers > 1 && !coupling) || !set_of || !omega || !eta_dev || !sigma_l ||
+      !sigma_r || !g_out || !status)
+    return fail(NEGF_ERR_ARG, "null pointer argument");
+  if (n_inst == 0) return NEGF_OK;
+  const int nmax = pick_nmax(n, true);
+  if (!nmax) return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 limit of 48", n);
+  DgfArgs a{n, n_layers, n_inst, onsite, coupling, set_of, omega, eta_dev, sigma_l, sigma_r, g_out,
+            status};
+  return run_dgf(reinterpret_cast<hipStream_t>(stream), nmax, a);
 }
 
 int negf_transmission_trace_batched(void* stream, int n, int n_inst, const double* sigma_l,

@@ -264,3 +264,35 @@ def test_mixed_precision_error_bound(gpu):
     # FP32 kernels are recomputed in FP64, so every instance returns a finite T
     assert np.all(np.isfinite(t32))
     assert np.median(err) < 1e-4
+
+
+@pytest.mark.parametrize("case", ["chain", "multidof", "contacts", "merged"])
+def test_device_green_function_matches_oracle(gpu, case):
+    from negfpy_amd.core import device_green_function
+
+    if case == "chain":
+        params = ChainParams(mass=1.0, spring=1.0)
+        lb = lead_blocks(params)
+        dev = device_perfect_chain(9, params)
+        ons, cps, ll = dev.onsite_blocks, dev.coupling_blocks, (lb.d00, lb.d01, None)
+        kw, okw, w = {}, {}, 1.1
+    else:
+        p = _ribbon_small(2, 5)
+        lb, dev = p.lead, p.device
+        ons, cps, ll = dev.onsite_blocks, dev.coupling_blocks, (lb.d00, lb.d01, None)
+        w = float(p.omegas[7])
+        kw, okw = {}, {}
+        if case == "contacts":
+            v = np.asarray(lb.d01).conj().T[:3]
+            kw = {"device_to_lead_left": v, "contact_left_indices": [0, 2, 4]}
+            okw = {"v_left": v, "idx_left": [0, 2, 4]}
+        if case == "merged":
+            v = np.asarray(lb.d01).conj().T[:3]
+            kw = {"device_to_lead_left": v, "contact_left_indices": [0, 5, 8]}
+            okw = {"v_left": v, "idx_left": [0, 5, 8]}
+    g, sl, sr = device_green_function(w, dev, lb, lb, eta=1e-6, **kw)
+    gr, slr, srr = o.device_green_dense(w, ons, cps, ll, ll, eta=1e-6, **okw)
+    assert g.shape == gr.shape and sl.shape == slr.shape
+    assert np.linalg.norm(g - gr) <= 1e-9 * np.linalg.norm(gr)
+    assert np.linalg.norm(sl - slr) <= 1e-10 * np.linalg.norm(slr)
+    assert np.linalg.norm(sr - srr) <= 1e-10 * np.linalg.norm(srr)
