@@ -35,6 +35,13 @@ Pinning status (what checks each function; see ``tests/test_oracle_*.py``):
 * ``heat_current_from_spectrum``, ``thermal_conductance`` -- reference study
                                  T(w) -> k(T) TSV pairs; step-function closed
                                  form.
+
+Every function above is also checked against vectors the reference package
+itself wrote on seeded inputs (``tests/golden/reference_vectors.npz``, made by
+``tests/golden/make_reference_vectors.py``; ``tests/test_reference_vectors.py``):
+random multi-DOF transmission with every surface-GF method, omega_scale,
+contacts and two leads, surface GFs (including the eigen routes at eta = 0),
+DOS, the device Green's function and the heat current.
 """
 
 from __future__ import annotations

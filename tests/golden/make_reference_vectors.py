@@ -1,0 +1,224 @@
+"""Write the reference-made golden vectors (run in the development container only).
+
+This script imports the reference package ``negfpy`` from its source tree (the
+first argument, default ``/root/reference/src``) and records its outputs on
+seeded inputs for the conventions that neither closed forms nor the agreement
+of independent solvers fix:
+
+* ``transmission`` on random multi-DOF leads/devices (Sancho-Rubio truncation at
+  tol = 1e-12 and the SuperLU device solve), with default couplings, explicit
+  device-to-lead couplings with contact indices, different leads, the
+  ``omega_scale`` path (which also rescales the default couplings), and the
+  eigenmode surface-GF routes;
+* ``surface_gf`` for every method, including the eigen routes' fallback at
+  eta = 0 inside the band (``surface_gf.py:123-124``);
+* ``lead_surface_dos`` on a multi-DOF lead, with and without
+  ``normalize_per_mode``;
+* ``device_green_function`` (``negf.py:253-303``);
+* ``transmission_kavg_adaptive`` / ``..._global_eta`` decisions and statistics;
+* the k-space builders used by the reference's transport tests;
+* ``transmission`` at the 24-DOF, 32-layer ribbon shape (a few omega points).
+
+Outputs (data only, no code): ``reference_vectors.npz`` and
+``reference_adaptive.json`` next to this file.  Inputs that are not stored
+verbatim are rebuilt by ``negfpy_amd.workloads`` / ``negfpy_amd.models``; the
+npz stores a checksum of each so a drift in those generators shows up.
+"""
+
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+
+
+def _checksum(*arrays) -> str:
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(np.asarray(a, dtype=np.complex128)).tobytes())
+    return h.hexdigest()
+
+
+def _small_multidof(seed, n_atoms, masses=None):
+    """Random PSD-ish spring tensors (same generator as the ribbon workload)."""
+    from negfpy_amd.workloads import ribbon_params
+
+    p = ribbon_params(seed, n_atoms)
+    return p.masses if masses is None else np.asarray(masses, float), p.fc00, p.fc01
+
+
+def main(ref_src: str) -> None:
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, ref_src)
+    import negfpy.core as rc
+    import negfpy.models as rm
+
+    out: dict[str, np.ndarray] = {}
+
+    # --- random multi-DOF transport (n = 6, L = 5) ---------------------------
+    masses, fc00, fc01 = _small_multidof(11, 2)
+    rng = np.random.default_rng(12)
+    layer_masses = rng.uniform(10.0, 30.0, (5, 2))
+    params = rm.MultiDofCellParams(masses=masses, fc00=fc00, fc01=fc01, dof_per_atom=3)
+    lead = rm.multidof_lead_blocks(params)
+    dev = rm.multidof_device(5, params, layer_masses=layer_masses)
+    out["md_masses"] = masses
+    out["md_fc00"] = fc00
+    out["md_fc01"] = fc01
+    out["md_layer_masses"] = layer_masses
+    out["md_d00"] = lead.d00
+    out["md_d01"] = lead.d01
+    out["md_onsite"] = np.stack(dev.onsite_blocks)
+    out["md_coupling"] = np.stack(dev.coupling_blocks)
+    w_edge = float(np.sqrt(np.max(np.linalg.eigvalsh(lead.d00 + lead.d01 + lead.d01.conj().T))))
+    omegas = np.linspace(0.02, 1.15, 14) * w_edge
+    out["md_omegas"] = omegas
+
+    out["md_t_sr"] = np.array([rc.transmission(w, dev, lead, lead, eta=1e-6) for w in omegas])
+    out["md_t_eta8"] = np.array([rc.transmission(w, dev, lead, lead, eta=1e-8, eta_device=1e-9)
+                                 for w in omegas])
+    out["md_t_scaled"] = np.array([rc.transmission(w, dev, lead, lead, eta=1e-6, omega_scale=1.7)
+                                   for w in omegas])
+    for meth in ("generalized_eigen", "generalized_eigen_svd", "legacy_eigen_svd"):
+        out[f"md_t_{meth}"] = np.array([rc.transmission(w, dev, lead, lead, eta=1e-6,
+                                                        surface_gf_method=meth)
+                                        for w in omegas])
+
+    # explicit device-to-lead couplings onto three contact DOFs
+    vl = rng.normal(size=(3, 6)) * 0.3
+    vr = rng.normal(size=(3, 6)) * 0.3
+    out["md_vl"] = vl
+    out["md_vr"] = vr
+    out["md_t_contacts"] = np.array([
+        rc.transmission(w, dev, lead, lead, eta=1e-6, device_to_lead_left=vl,
+                        device_to_lead_right=vr, contact_left_indices=[0, 2, 4],
+                        contact_right_indices=[1, 3, 5])
+        for w in omegas])
+
+    # different leads (right lead heavier)
+    params_r = rm.MultiDofCellParams(masses=masses * 1.6, fc00=fc00, fc01=fc01, dof_per_atom=3)
+    lead_r = rm.multidof_lead_blocks(params_r)
+    out["md_d00_r"] = lead_r.d00
+    out["md_d01_r"] = lead_r.d01
+    out["md_t_two_leads"] = np.array([rc.transmission(w, dev, lead, lead_r, eta=1e-6)
+                                      for w in omegas])
+
+    # device Green's function with contact subsets
+    g, sl, sr = rc.device_green_function(0.45 * w_edge, dev, lead, lead, eta=1e-6,
+                                         device_to_lead_left=vl, device_to_lead_right=vr,
+                                         contact_left_indices=[0, 2, 4],
+                                         contact_right_indices=[25, 27, 29])
+    out["dg_omega"] = np.array(0.45 * w_edge)
+    out["dg_g"] = g
+    out["dg_sigma_l"] = sl
+    out["dg_sigma_r"] = sr
+
+    # --- surface GF, every method ---------------------------------------------
+    sg_omegas = np.array([0.1, 0.5, 0.9, 1.3]) * w_edge
+    for meth in ("sancho_rubio", "generalized_eigen", "generalized_eigen_svd", "legacy_eigen_svd"):
+        out[f"sg_{meth}"] = np.stack([rc.surface_gf(w, lead.d00, lead.d01, eta=1e-7, method=meth)
+                                      for w in sg_omegas])
+    out["sg_omegas"] = sg_omegas
+    # eta = 0 inside the band: propagating modes sit on |lambda| = 1 and the eigen
+    # routes fall back to the n smallest |lambda| (surface_gf.py:123-124)
+    for meth in ("generalized_eigen", "generalized_eigen_svd", "legacy_eigen_svd"):
+        out[f"sg0_{meth}"] = rc.surface_gf(0.5 * w_edge, lead.d00, lead.d01, eta=0.0, method=meth)
+
+    # --- surface DOS, multi-DOF ------------------------------------------------
+    out["dos"] = np.array([rc.lead_surface_dos(w, lead, eta=1e-7) for w in omegas])
+    out["dos_norm"] = np.array([rc.lead_surface_dos(w, lead, eta=1e-7, normalize_per_mode=True)
+                                for w in omegas])
+
+    # --- heat current (np.trapz semantics) ----------------------------------
+    hw = np.linspace(1e11, 8e13, 57)
+    ht = np.abs(np.sin(hw / 2e13))
+    out["hc_omegas"] = hw
+    out["hc_t"] = ht
+    out["hc_j"] = np.array([rc.heat_current_from_spectrum(hw, ht, 310.0, 290.0),
+                            rc.heat_current_from_spectrum(hw, ht, 5.0, 1.0)])
+
+    # --- k-space builders ------------------------------------------------------
+    sq = rm.SquareLatticeParams(mass=1.3, spring_x=0.9, spring_y=0.6)
+    cu = rm.CubicLatticeParams(mass=0.8, spring_x=1.1, spring_y=0.5, spring_z=0.7,
+                               onsite_pinning=0.05)
+    kys = np.array([-2.0, -0.3, 0.0, 0.7, 3.1])
+    sq_lead = rm.square_lattice_lead(sq)
+    sq_dev = rm.square_lattice_device(4, sq, defect_index=2, defect_mass=2.0)
+    cu_lead = rm.cubic_lattice_lead(cu)
+    cu_dev = rm.cubic_lattice_device(3, cu, defect_index=1, defect_mass=0.5)
+    out["kb_ky"] = kys
+    out["kb_sq_lead"] = np.stack([np.stack(sq_lead.blocks_builder((k,))[:2]) for k in kys])
+    out["kb_sq_dev_onsite"] = np.stack([np.stack(sq_dev.device_builder((k,)).onsite_blocks)
+                                        for k in kys])
+    out["kb_cu_lead"] = np.stack([np.stack(cu_lead.blocks_builder((k, 0.4 * k))[:2]) for k in kys])
+    out["kb_cu_dev_onsite"] = np.stack([np.stack(cu_dev.device_builder((k, 0.4 * k)).onsite_blocks)
+                                        for k in kys])
+
+    # --- square-lattice k-resolved transmission ------------------------------
+    kpts = [(float(k),) for k in np.linspace(-np.pi, np.pi, 7, endpoint=False)]
+    out["sq_kpts"] = np.array([k[0] for k in kpts])
+    out["sq_t"] = np.array([[rc.transmission(w, sq_dev, sq_lead, sq_lead, eta=1e-7, kpar=k)
+                             for k in kpts] for w in (0.4, 1.1, 1.6)])
+
+    # --- adaptive drivers -------------------------------------------------------
+    adaptive = {}
+    chain = rm.ChainParams(mass=1.0, spring=1.0)
+    clead = rm.lead_blocks(chain)
+    cdev = rm.device_perfect_chain(8, chain)
+    t, info = rc.transmission_kavg_adaptive(0.9, sq_dev, sq_lead, sq_lead, kpoints=kpts,
+                                            eta_values=(1e-8, 1e-6),
+                                            nonnegative_tolerance=1e-6, collect_rejected=True)
+    adaptive["sq_default"] = {"omega": 0.9, "t": t, "info": info}
+    t, info = rc.transmission_kavg_adaptive(1.1, sq_dev, sq_lead, sq_lead, kpoints=kpts,
+                                            eta_values=(1e-8, 1e-7), max_channel_factor=0.5,
+                                            nonnegative_tolerance=1e-6, collect_rejected=True)
+    adaptive["sq_channel_limit"] = {"omega": 1.1, "t": t, "info": info}
+    t, info = rc.transmission_kavg_adaptive_global_eta(
+        1.1, sq_dev, sq_lead, sq_lead, kpoints=kpts, eta_values=(1e-8, 1e-7),
+        min_success_fraction=0.2, max_channel_factor=0.5, nonnegative_tolerance=1e-6,
+        collect_rejected=True)
+    adaptive["sq_global_eta"] = {"omega": 1.1, "t": t, "info": info}
+    t, info = rc.transmission_kavg_adaptive(1.0, cdev, clead, clead, kpoints=[(), ()])
+    adaptive["chain_plain"] = {"omega": 1.0, "t": t, "info": info}
+
+    # --- ribbon, 24 DOF x 32 layers -----------------------------------------
+    from negfpy_amd.workloads import ribbon_problem
+
+    rp = ribbon_problem(seed=0, n_omega=4096)
+    rparams = rm.MultiDofCellParams(masses=rp.params.masses, fc00=rp.params.fc00,
+                                    fc01=rp.params.fc01, dof_per_atom=3)
+    rlead = rm.multidof_lead_blocks(rparams)
+    rng0 = np.random.default_rng(1)
+    rdev = rm.multidof_device(32, rparams, layer_masses=rng0.uniform(10.0, 30.0, (32, 8)))
+    sel = np.array([5, 700, 1400, 2100, 2800, 3500, 3900, 4090])
+    out["rb_index"] = sel
+    out["rb_checksum"] = np.frombuffer(bytes.fromhex(_checksum(
+        rlead.d00, rlead.d01, *rdev.onsite_blocks, *rdev.coupling_blocks, rp.omegas)), np.uint8)
+    out["rb_t"] = np.array([rc.transmission(rp.omegas[i], rdev, rlead, rlead, eta=rp.eta)
+                            for i in sel])
+
+    np.savez_compressed(os.path.join(HERE, "reference_vectors.npz"), **out)
+
+    def _jsonable(o):
+        if isinstance(o, dict):
+            return {repr(float(k)) if isinstance(k, float) else str(k): _jsonable(v)
+                    for k, v in o.items()}
+        if isinstance(o, (list, tuple)):
+            return [_jsonable(v) for v in o]
+        if isinstance(o, (np.floating, np.integer)):
+            return o.item()
+        return o
+
+    with open(os.path.join(HERE, "reference_adaptive.json"), "w") as f:
+        json.dump(_jsonable(adaptive), f, indent=1, sort_keys=True)
+        f.write("\n")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/src")

@@ -1,12 +1,5 @@
-"""GPU parity: the HIP path (through the C ABI) against the FP64 CPU oracle.
-
-Bar for FP64 mode: |T_gpu - T_oracle| <= max(1e-10, 50 s) * max(|T|, 1e-3),
-where s is the oracle's own rounding-level spread at that point: the largest
-change of T_oracle when every input block is perturbed by about one ulp
-(three seeded draws).  At most points s ~ 1e-14..1e-12 and the 1e-10 bar
-applies; near band features decimation is conditioned like 1/eta and s reaches
-1e-8, which no FP64 evaluation order can beat.
-"""
+"""GPU parity: the HIP path (through the C ABI) against the FP64 CPU oracle, with the
+spread-aware bar of ``parity_util``."""
 
 import numpy as np
 import pytest
@@ -38,41 +31,9 @@ from negfpy_amd.models import (
 )
 from negfpy_amd.workloads import omega_max, random_lead, ribbon_problem
 from oracle import negf_oracle as o
+from parity_util import T_TOL, assert_parity, oracle_t, rel
 
 pytestmark = pytest.mark.gpu
-
-T_TOL = 1e-10
-
-
-def rel(a, b, floor=1e-3):
-    return np.max(np.abs(np.asarray(a) - np.asarray(b)) / np.maximum(np.abs(b), floor))
-
-
-def _ulp_perturb(x, rng):
-    if x is None:
-        return None
-    x = np.asarray(x, dtype=np.complex128)
-    return x * (1.0 + 2.2e-16 * rng.standard_normal(x.shape))
-
-
-def oracle_t(w, ons, cps, ll, lr, **kw):
-    """(T, spread): oracle transmission and its rounding-level spread."""
-    t = o.transmission_sparse(w, ons, cps, ll, lr, **kw)
-    rng = np.random.default_rng(1234)
-    spread = 0.0
-    for _ in range(3):
-        pl = tuple(_ulp_perturb(b, rng) for b in ll)
-        pr = tuple(_ulp_perturb(b, rng) for b in lr)
-        po = [_ulp_perturb(b, rng) for b in ons]
-        pc = [_ulp_perturb(b, rng) for b in cps]
-        spread = max(spread, abs(o.transmission_sparse(w, po, pc, pl, pr, **kw) - t))
-    return t, spread
-
-
-def assert_parity(t_gpu, refs):
-    for tg, (tr, sp) in zip(np.atleast_1d(t_gpu), refs):
-        bound = max(T_TOL * max(abs(tr), 1e-3), 50.0 * sp)
-        assert abs(tg - tr) <= bound, (tg, tr, sp)
 
 
 @pytest.mark.parametrize("n", [1, 3, 6, 12, 16, 24, 30, 48])
