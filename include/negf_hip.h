@@ -128,6 +128,28 @@ int negf_device_green_batched(void* stream, int n, int n_layers, int n_inst, con
                               double* g_out, int32_t* status);
 
 /*
+ * Adjoint of the transmission (FP64): T and, for every instance, the gradient of
+ * w * T with respect to every onsite block D_ii and coupling block C_i of the
+ * device (C_i enters D above the diagonal, C_i^H below), as dT/dRe + i dT/dIm:
+ *   grad_onsite_i   = 2 w (G_iN W G_1i)^H,
+ *   grad_coupling_i = 2 w ((G_{i+1,N} W G_1i)^H + G_iN W G_{1,i+1}),
+ * W = Gamma_R G_1N^H Gamma_L, from a right-connected sweep (row 1 of G) and a
+ * left-connected sweep (column N).  The reference has no gradients; this is the
+ * reverse mode of its transmission, src/negfpy/core/negf.py:306-357, with respect
+ * to the device blocks it assembles at :53-69.
+ *   weight [n_inst] (dL/dT, null: 1); workspace >= negf_adjoint_workspace_bytes;
+ *   t_out [n_inst]; grad_onsite [n_inst][L][n][n]; grad_coupling [n_inst][L-1][n][n];
+ *   status [n_inst] (lead status passed through, 12/13 on device failure)
+ */
+size_t negf_adjoint_workspace_bytes(int n, int n_layers, int n_inst);
+int negf_adjoint_batched(void* stream, int n, int n_layers, int n_inst, const double* onsite,
+                         const double* coupling, const int32_t* set_of, const double* omega,
+                         const double* eta_dev, const double* sigma_l, const double* sigma_r,
+                         const int32_t* status_in, const double* weight, void* workspace,
+                         size_t workspace_bytes, double* t_out, double* grad_onsite,
+                         double* grad_coupling, int32_t* status);
+
+/*
  * T = Re Tr[Gamma_L G_1N Gamma_R G_1N^H], Gamma = i (Sigma - Sigma^H), in FP64.
  * Replaces _broadening and the trace, src/negfpy/core/negf.py:115-116, :346-357.
  * Instances with status_in != 0 get NaN.
@@ -193,6 +215,14 @@ int negf_transmission_batched(void* stream, const negf_transmission_problem* p, 
  */
 int negf_thermal_conductance(void* stream, int n_w, const double* omega_rad_s, const double* trans,
                              int n_t, const double* temps, double* out);
+
+/*
+ * d G(T_t) / d T(w_i) for the conductance above: weights_out [n_t][n_w] with
+ * G(T_t) = sum_i weights_out[t][i] T(w_i).  Chain rule of
+ * src/negfpy/workflows/ifc_bulk.py:408-438 for gradients of kappa.
+ */
+int negf_conductance_weights(void* stream, int n_w, const double* omega_rad_s, int n_t,
+                             const double* temps, double* weights_out);
 
 /*
  * Landauer heat current J = (1/2pi) int [k_B T_L phi(x_L) - k_B T_R phi(x_R)] T(w) dw,

@@ -71,6 +71,10 @@ _SIGS = {
         _c_int),
     "negf_device_green_batched": (
         [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
+    "negf_adjoint_workspace_bytes": ([_c_int, _c_int, _c_int], ctypes.c_size_t),
+    "negf_adjoint_batched": (
+        [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+         ctypes.c_size_t, _vp, _vp, _vp, _vp], _c_int),
     "negf_transmission_trace_batched": ([_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp], _c_int),
     "negf_merge_status": ([_vp, _c_int, _vp, _vp], _c_int),
     "negf_transmission_workspace_bytes": ([ctypes.POINTER(TransmissionProblem)], ctypes.c_size_t),
@@ -78,6 +82,7 @@ _SIGS = {
         [_vp, ctypes.POINTER(TransmissionProblem), _vp, ctypes.c_size_t], _c_int),
     "negf_thermal_conductance": ([_vp, _c_int, _vp, _vp, _c_int, _vp, _vp], _c_int),
     "negf_heat_current": ([_vp, _c_int, _vp, _vp, _c_dbl, _c_dbl, _vp], _c_int),
+    "negf_conductance_weights": ([_vp, _c_int, _vp, _c_int, _vp, _vp], _c_int),
 }
 
 EXPORTS = tuple(_SIGS)

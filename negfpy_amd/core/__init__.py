@@ -1,6 +1,7 @@
 """GPU phonon-NEGF transport core with the API of the reference ``negfpy.core``
 (``src/negfpy/core/__init__.py:1-78``, transport part)."""
 
+from .adjoint import thermal_conductance_torch, transmission_torch
 from .dos import (
     lead_surface_dos,
     lead_surface_dos_kavg,
@@ -37,6 +38,8 @@ from .thermal_current import (
 from .types import Device1D, DeviceKSpace, LeadBlocks, LeadKSpace
 
 __all__ = [
+    "transmission_torch",
+    "thermal_conductance_torch",
     "Device1D",
     "DeviceKSpace",
     "LeadBlocks",

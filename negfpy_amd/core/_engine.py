@@ -311,21 +311,15 @@ class TransmissionSolver:
         return out
 
 
-def device_green(sets: TransportSets, set_of, omega, eta_lead, eta_dev, *, tol=1e-12,
-                 maxiter=200):
-    """Full device Green's function of every instance (FP64).
+def lead_self_energies(sets: TransportSets, set_of, omega, eta_lead, *, tol=1e-12, maxiter=200):
+    """FP64 decimation of both leads and Sigma = V g V^H embedded in the end layers.
 
-    Returns (G [n_inst, Ln, Ln], Sigma_L, Sigma_R [n_inst, n, n] end-layer blocks,
-    lead status, device status), all torch tensors on the GPU.
+    Returns (Sigma_L, Sigma_R [n_inst, n, n], lead status [n_inst]) on the GPU.
     """
     lib = _lib.load()
-    set_of = to_dev_i32(set_of)
-    omega = to_dev_f64(omega)
-    eta_lead = to_dev_f64(eta_lead)
-    eta_dev = to_dev_f64(eta_dev)
     n_inst = int(omega.numel())
     dev = sets.onsite.device
-    n, nl = sets.n, sets.n_layers
+    n = sets.n
     c128 = torch.complex128
     st = _lib.stream()
     sig = []
@@ -347,13 +341,70 @@ def device_green(sets: TransportSets, set_of, omega, eta_lead, eta_dev, *, tol=1
         sig.append(sg)
         stats.append(stt)
     lead_status = torch.where(stats[0] != 0, stats[0], stats[1])
-    gfull = torch.zeros((n_inst, n * nl, n * nl), dtype=c128, device=dev)
+    return sig[0], sig[1], lead_status
+
+
+def device_green(sets: TransportSets, set_of, omega, eta_lead, eta_dev, *, tol=1e-12,
+                 maxiter=200):
+    """Full device Green's function of every instance (FP64).
+
+    Returns (G [n_inst, Ln, Ln], Sigma_L, Sigma_R [n_inst, n, n] end-layer blocks,
+    lead status, device status), all torch tensors on the GPU.
+    """
+    lib = _lib.load()
+    set_of = to_dev_i32(set_of)
+    omega = to_dev_f64(omega)
+    eta_lead = to_dev_f64(eta_lead)
+    eta_dev = to_dev_f64(eta_dev)
+    n_inst = int(omega.numel())
+    n, nl = sets.n, sets.n_layers
+    sig_l, sig_r, lead_status = lead_self_energies(sets, set_of, omega, eta_lead, tol=tol,
+                                                   maxiter=maxiter)
+    dev = sets.onsite.device
+    gfull = torch.zeros((n_inst, n * nl, n * nl), dtype=torch.complex128, device=dev)
     dst = torch.zeros((n_inst,), dtype=torch.int32, device=dev)
     _lib.check(lib.negf_device_green_batched(
-        st, n, nl, n_inst, sets.onsite.data_ptr(), sets.coupling.data_ptr(), set_of.data_ptr(),
-        omega.data_ptr(), eta_dev.data_ptr(), sig[0].data_ptr(), sig[1].data_ptr(),
-        gfull.data_ptr(), dst.data_ptr()))
-    return gfull, sig[0], sig[1], lead_status, dst
+        _lib.stream(), n, nl, n_inst, sets.onsite.data_ptr(), sets.coupling.data_ptr(),
+        set_of.data_ptr(), omega.data_ptr(), eta_dev.data_ptr(), sig_l.data_ptr(),
+        sig_r.data_ptr(), gfull.data_ptr(), dst.data_ptr()))
+    return gfull, sig_l, sig_r, lead_status, dst
+
+
+@dataclass
+class AdjointResult:
+    t: torch.Tensor              # (n_inst,) float64
+    grad_onsite: torch.Tensor    # (n_inst, L, n, n) complex128, weight * dT/dD_ii
+    grad_coupling: torch.Tensor  # (n_inst, max(L-1, 1), n, n) complex128, weight * dT/dC_i
+    status: torch.Tensor         # (n_inst,) int32
+
+
+def adjoint(sets: TransportSets, set_of, omega, eta_lead, eta_dev, weight=None, *, tol=1e-12,
+            maxiter=200) -> AdjointResult:
+    """T and weight * dT/d(device blocks) of every instance (FP64 adjoint kernel)."""
+    lib = _lib.load()
+    set_of = to_dev_i32(set_of)
+    omega = to_dev_f64(omega)
+    eta_lead = to_dev_f64(eta_lead)
+    eta_dev = to_dev_f64(eta_dev)
+    n_inst = int(omega.numel())
+    n, nl = sets.n, sets.n_layers
+    sig_l, sig_r, lead_status = lead_self_energies(sets, set_of, omega, eta_lead, tol=tol,
+                                                   maxiter=maxiter)
+    dev = sets.onsite.device
+    w = None if weight is None else to_dev_f64(weight)
+    ws_bytes = int(lib.negf_adjoint_workspace_bytes(n, nl, n_inst))
+    ws = torch.empty((max(ws_bytes, 1),), dtype=torch.uint8, device=dev)
+    t = torch.empty((n_inst,), dtype=torch.float64, device=dev)
+    g_ons = torch.zeros((n_inst, nl, n, n), dtype=torch.complex128, device=dev)
+    g_cpl = torch.zeros((n_inst, max(nl - 1, 1), n, n), dtype=torch.complex128, device=dev)
+    status = torch.zeros((n_inst,), dtype=torch.int32, device=dev)
+    _lib.check(lib.negf_adjoint_batched(
+        _lib.stream(), n, nl, n_inst, sets.onsite.data_ptr(), sets.coupling.data_ptr(),
+        set_of.data_ptr(), omega.data_ptr(), eta_dev.data_ptr(), sig_l.data_ptr(),
+        sig_r.data_ptr(), lead_status.data_ptr(), None if w is None else w.data_ptr(),
+        ws.data_ptr(), ws_bytes, t.data_ptr(), g_ons.data_ptr(), g_cpl.data_ptr(),
+        status.data_ptr()))
+    return AdjointResult(t=t, grad_onsite=g_ons, grad_coupling=g_cpl, status=status)
 
 
 # ---------------------------------------------------------------------------
@@ -380,3 +431,25 @@ def heat_current(omega_rad_s, tvals, temp_left, temp_right) -> float:
     _lib.check(lib.negf_heat_current(_lib.stream(), int(w.numel()), w.data_ptr(), t.data_ptr(),
                                      float(temp_left), float(temp_right), out.data_ptr()))
     return float(out.cpu().item())
+
+
+def thermal_conductance_tensor(omega_rad_s: torch.Tensor, tvals: torch.Tensor,
+                               temps: torch.Tensor) -> torch.Tensor:
+    """kappa(T_t) of a GPU spectrum, as a GPU tensor (the k_conductance kernel)."""
+    lib = _lib.load()
+    w, t, tk = to_dev_f64(omega_rad_s), to_dev_f64(tvals), to_dev_f64(temps)
+    out = torch.empty((int(tk.numel()),), dtype=torch.float64, device=w.device)
+    _lib.check(lib.negf_thermal_conductance(_lib.stream(), int(w.numel()), w.data_ptr(),
+                                            t.data_ptr(), int(tk.numel()), tk.data_ptr(),
+                                            out.data_ptr()))
+    return out
+
+
+def conductance_weights(omega_rad_s: torch.Tensor, temps: torch.Tensor) -> torch.Tensor:
+    """(n_t, n_w) weights with kappa(T_t) = sum_i weights[t, i] T(w_i)."""
+    lib = _lib.load()
+    w, tk = to_dev_f64(omega_rad_s), to_dev_f64(temps)
+    out = torch.empty((int(tk.numel()), int(w.numel())), dtype=torch.float64, device=w.device)
+    _lib.check(lib.negf_conductance_weights(_lib.stream(), int(w.numel()), w.data_ptr(),
+                                            int(tk.numel()), tk.data_ptr(), out.data_ptr()))
+    return out

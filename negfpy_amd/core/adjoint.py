@@ -1,0 +1,182 @@
+"""Differentiable transmission and thermal conductance.
+
+The reference computes T(omega) and kappa(T) without gradients
+(``src/negfpy/core/negf.py:306-357``, ``src/negfpy/workflows/ifc_bulk.py:408-438``).
+Here both are ``torch.autograd.Function``s:
+
+* ``transmission_torch`` -- T(omega) for a device given as torch tensors of onsite
+  and coupling blocks.  Forward: the batched FP64 transmission kernels.  Backward:
+  the adjoint kernel ``k_adjoint`` (one right- and one left-connected sweep per
+  (omega) instance), weighted by the incoming dL/dT and summed over omega on the
+  GPU.  Leads and device-to-lead couplings are constants.
+* ``thermal_conductance_torch`` -- kappa(T) of a GPU spectrum (the k_conductance
+  kernel); backward multiplies by the trapezoid/Bose weights of
+  ``k_conductance_weights``.
+
+Chaining them gives d kappa / d(force constants) through any torch expression
+that builds the blocks from force constants and masses.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _engine
+from ._status import transmission_value
+from .negf import _max_block, resolve_k
+from .types import Device1D
+
+__all__ = ["transmission_torch", "thermal_conductance_torch"]
+
+# Device memory one backward launch may use for adjoint workspace and per-instance
+# gradients; larger omega batches are processed in chunks of this size.
+ADJOINT_CHUNK_BYTES = 2 << 30
+
+
+def _merge_layers_torch(onsite: torch.Tensor, coupling: torch.Tensor, m: int):
+    """Group m consecutive layers into one block (differentiable), zero-padding the tail;
+    the same layout as the product's block merging for contacts deep in the device."""
+    nl, n = int(onsite.shape[0]), int(onsite.shape[1])
+    nb = -(-nl // m)
+    big = m * n
+    ons = onsite.new_zeros((nb, big, big))
+    cpl = onsite.new_zeros((max(nb - 1, 0), big, big))
+    for i in range(nl):
+        b, r = divmod(i, m)
+        ons[b, r * n:(r + 1) * n, r * n:(r + 1) * n] = onsite[i]
+    for i in range(nl - 1):
+        b, r = divmod(i, m)
+        blk = coupling[i]
+        if r < m - 1:
+            ons[b, r * n:(r + 1) * n, (r + 1) * n:(r + 2) * n] = blk
+            ons[b, (r + 1) * n:(r + 2) * n, r * n:(r + 1) * n] = blk.conj().transpose(0, 1)
+        else:
+            cpl[b, (m - 1) * n:m * n, 0:n] = blk
+    return ons, cpl
+
+
+class _Static:
+    """Everything of one transmission problem except the device blocks."""
+
+    def __init__(self, kp, omega, eta, eta_dev):
+        s = kp.scale
+        self.kp = kp
+        self.omega = _engine.to_dev_f64(np.asarray(omega, dtype=float) / s)
+        self.eta = _engine.to_dev_f64(np.full(self.omega.numel(), float(eta) / s))
+        self.eta_dev = _engine.to_dev_f64(np.full(self.omega.numel(), float(eta_dev) / s))
+        self.set_of = torch.zeros(self.omega.numel(), dtype=torch.int32,
+                                  device=self.omega.device)
+
+    def sets(self, ons: torch.Tensor, cpl: torch.Tensor) -> _engine.TransportSets:
+        kp = self.kp
+        share = bool(np.array_equal(kp.lead_l, kp.lead_r))
+        return _engine.TransportSets.from_arrays(
+            kp.lead_l[None], kp.lead_r[None], kp.v_l[None], kp.v_r[None], kp.pos_l, kp.pos_r,
+            ons.detach()[None], cpl.detach()[None] if ons.shape[0] > 1 else None, share)
+
+
+class _TransmissionFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, ons, cpl, static):
+        sets = static.sets(ons, cpl)
+        solver = _engine.TransmissionSolver(sets, precision="fp64")
+        res = solver.run(static.set_of, static.omega, static.eta, static.eta_dev)
+        st = res.status.cpu().numpy()
+        t = res.t
+        if np.any(st != 0):
+            tc = t.cpu().numpy()
+            vals = [transmission_value(int(c), float(v)) for c, v in zip(st, tc)]
+            t = torch.as_tensor(np.asarray(vals), dtype=torch.float64, device=t.device)
+        ctx.static = static
+        ctx.save_for_backward(ons, cpl)
+        return t
+
+    @staticmethod
+    def backward(ctx, grad_t):
+        ons, cpl = ctx.saved_tensors
+        static = ctx.static
+        sets = static.sets(ons, cpl)
+        weight = grad_t.detach().to(torch.float64).contiguous()
+        nl, n = int(ons.shape[0]), int(ons.shape[1])
+        n_inst = int(static.omega.numel())
+        # per instance: (2L - 1) workspace blocks and as many gradient blocks
+        per_inst = 2 * (2 * nl - 1) * n * n * 16
+        chunk = max(1, min(n_inst, ADJOINT_CHUNK_BYTES // per_inst))
+        g_ons = torch.zeros_like(ons)
+        g_cpl = torch.zeros((max(nl - 1, 1), n, n), dtype=ons.dtype, device=ons.device)
+        for a in range(0, n_inst, chunk):
+            b = min(n_inst, a + chunk)
+            res = _engine.adjoint(sets, static.set_of[a:b], static.omega[a:b], static.eta[a:b],
+                                  static.eta_dev[a:b], weight=weight[a:b])
+            g_ons += res.grad_onsite.sum(dim=0)
+            g_cpl += res.grad_coupling.sum(dim=0)
+        return g_ons, (g_cpl[:nl - 1] if nl > 1 else None), None
+
+
+def transmission_torch(omega, onsite, coupling, lead_left, lead_right, *, eta: float = 1e-8,
+                       eta_device: float | None = None, device_to_lead_left=None,
+                       device_to_lead_right=None, contact_left_indices=None,
+                       contact_right_indices=None, omega_scale: float | None = None
+                       ) -> torch.Tensor:
+    """T(omega) for every omega of ``omega`` as a differentiable GPU tensor.
+
+    ``onsite`` is (L, n, n) and ``coupling`` (L-1, n, n), real or complex torch tensors
+    (or arrays); gradients flow to both.  Arguments and conventions otherwise follow
+    ``transmission`` (``negf.py:306-357``), with the leads and device-to-lead couplings
+    held fixed.
+    """
+    c128 = torch.complex128
+    dev = _engine.device()
+    ons = onsite if isinstance(onsite, torch.Tensor) else torch.as_tensor(np.asarray(onsite))
+    cpl = coupling if isinstance(coupling, torch.Tensor) else torch.as_tensor(np.asarray(coupling))
+    ons = ons.to(device=dev, dtype=c128)
+    cpl = cpl.to(device=dev, dtype=c128)
+    if ons.ndim != 3 or ons.shape[1] != ons.shape[2]:
+        raise ValueError("onsite must have shape (n_layers, n, n).")
+    nl, n = int(ons.shape[0]), int(ons.shape[1])
+    if cpl.ndim != 3 or int(cpl.shape[0]) != nl - 1 or (nl > 1 and tuple(cpl.shape[1:]) != (n, n)):
+        raise ValueError("coupling must have shape (n_layers - 1, n, n).")
+    w = np.atleast_1d(np.asarray(omega, dtype=float))
+    host = Device1D(onsite_blocks=list(ons.detach().cpu().numpy()),
+                    coupling_blocks=list(cpl.detach().cpu().numpy()))
+    kp = resolve_k(host, lead_left, lead_right, None, eta=eta, eta_device=eta_device,
+                   device_to_lead_left=device_to_lead_left,
+                   device_to_lead_right=device_to_lead_right,
+                   contact_left_indices=contact_left_indices,
+                   contact_right_indices=contact_right_indices, omega_scale=omega_scale,
+                   max_block=_max_block("fp64"))
+    if omega_scale is not None:
+        s2 = float(omega_scale) ** 2
+        ons = ons / s2
+        cpl = cpl / s2
+    m = int(kp.onsite.shape[1]) // n
+    if m > 1:
+        ons, cpl = _merge_layers_torch(ons, cpl, m)
+    eta_dev = float(eta if eta_device is None else eta_device)
+    return _TransmissionFn.apply(ons, cpl, _Static(kp, w, eta, eta_dev))
+
+
+class _ConductanceFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, tvals, omega_rad_s, temps):
+        ctx.save_for_backward(omega_rad_s, temps)
+        return _engine.thermal_conductance_tensor(omega_rad_s, tvals, temps)
+
+    @staticmethod
+    def backward(ctx, grad_k):
+        omega_rad_s, temps = ctx.saved_tensors
+        wts = _engine.conductance_weights(omega_rad_s, temps)
+        return grad_k.to(torch.float64) @ wts, None, None
+
+
+def thermal_conductance_torch(omega_rad_s, tvals: torch.Tensor, temperatures) -> torch.Tensor:
+    """kappa(T) = (1/2pi) int k_B T(w) x^2 e^-x / (1 - e^-x)^2 dw (w in rad/s), as a
+    differentiable function of the spectrum ``tvals`` (``ifc_bulk.py:408-438``)."""
+    dev = _engine.device()
+    w = torch.as_tensor(np.asarray(omega_rad_s, dtype=float), device=dev)
+    tk = torch.as_tensor(np.atleast_1d(np.asarray(temperatures, dtype=float)), device=dev)
+    t = tvals.to(device=dev, dtype=torch.float64)
+    if t.shape != w.shape:
+        raise ValueError("tvals must have one entry per omega.")
+    return _ConductanceFn.apply(t, w, tk)

@@ -860,6 +860,302 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_device_green(DgfArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Adjoint: T and dT/dD for every device block (FP64)
+// ---------------------------------------------------------------------------
+//
+// With A = z - D - Sigma_L - Sigma_R, G = A^-1 and W = Gamma_R G_1N^H Gamma_L,
+// dT = 2 Re Tr[W dG_1N] and dG_1N = sum_ij G_1i dD_ij G_jN, so
+//   dT/dD_ij (as d/dRe + i d/dIm) = 2 (G_jN W G_1i)^H.
+// Row 1 of G comes from a right-connected sweep (v_0 = G_11, v_i = v_{i-1} R_i,
+// R_i = C_{i-1} gR_i), column N from the left-connected sweep (u_{N-1} = G_NN,
+// u_i = Y_i u_{i+1}, Y_i = gL_i C_i).  Onsite block i gets 2 w (u_i P_i)^H and
+// coupling C_i (which enters D as C_i above and C_i^H below the diagonal) gets
+// 2 w ((u_{i+1} P_i)^H + u_i P_{i+1}), with P_i = W v_i and w the caller's
+// weight of instance T (its dL/dT).
+
+struct AdjArgs {
+  int n, n_layers, n_inst;
+  const double* onsite;
+  const double* coupling;
+  const int32_t* set_of;
+  const double* omega;
+  const double* eta_dev;
+  const double* sigma_l;
+  const double* sigma_r;
+  const int32_t* status_in;  // lead status; may be null
+  const double* weight;      // [n_inst] dL/dT; may be null (weight 1)
+  double* work;              // [n_inst][2L-1][n][n] complex
+  double* t_out;             // [n_inst]
+  double* grad_onsite;       // [n_inst][L][n][n] complex
+  double* grad_coupling;     // [n_inst][L-1][n][n] complex
+  int32_t* status;
+};
+
+template <int NMAX>
+__global__ void __launch_bounds__(Cfg<NMAX>::NT) k_adjoint(AdjArgs a) {
+  using T = double;
+  constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
+  using L = Lds<T, NMAX, NT, 4>;
+  constexpr int LD = L::LD;
+  extern __shared__ __align__(16) unsigned char smem[];
+  L s(smem);
+  const int tid = threadIdx.x;
+  const int inst = blockIdx.x;
+  if (a.status_in && a.status_in[inst] != ST_OK) {
+    if (tid == 0) {
+      a.status[inst] = a.status_in[inst];
+      a.t_out[inst] = NAN;
+    }
+    return;
+  }
+  const int n = a.n, nl = a.n_layers;
+  const size_t nn = (size_t)n * n;
+  const size_t set = (size_t)a.set_of[inst];
+  const double2* ons = reinterpret_cast<const double2*>(a.onsite + set * nl * nn * 2);
+  const double* cpl = a.coupling + set * (nl > 1 ? nl - 1 : 0) * nn * 2;
+  const double2* sgl = reinterpret_cast<const double2*>(a.sigma_l + (size_t)inst * nn * 2);
+  const double2* sgr = reinterpret_cast<const double2*>(a.sigma_r + (size_t)inst * nn * 2);
+  double* vw = a.work + (size_t)inst * (2 * nl - 1) * nn * 2;  // R_i, then v_i, then P_i
+  double* yw = vw + (size_t)nl * nn * 2;                         // Y_i
+  double* go = a.grad_onsite + (size_t)inst * nl * nn * 2;
+  double* gc = a.grad_coupling + (size_t)inst * (nl > 1 ? nl - 1 : 0) * nn * 2;
+  auto blk = [&](double* base, int i) { return base + (size_t)i * nn * 2; };
+  const double w2 = 2.0 * (a.weight ? a.weight[inst] : 1.0);
+  lds_zero<T, NT>(smem, L::mat_bytes(), tid);
+  __syncthreads();
+  const Tile<TM> t(n, n, tid);
+  double zr, zi;
+  zsq(a.omega[inst], a.eta_dev[inst], zr, zi);
+  // A_ii - S into LDS matrix 0
+  auto build = [&](int layer, const T (&sr)[TM][TM], const T (&si)[TM][TM]) {
+    if (!t.own) return;
+    const double2* d = ons + (size_t)layer * nn;
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+      for (int y = 0; y < TM; ++y) {
+        const int i = t.i0 + x, j = t.j0 + y;
+        if (i < n && j < n) {
+          const double2 dv = d[i * n + j];
+          T vr = (i == j ? zr : 0.0) - dv.x - sr[x][y];
+          T vi = (i == j ? zi : 0.0) - dv.y - si[x][y];
+          if (layer == 0) {
+            vr -= sgl[i * n + j].x;
+            vi -= sgl[i * n + j].y;
+          }
+          if (layer == nl - 1) {
+            vr -= sgr[i * n + j].x;
+            vi -= sgr[i * n + j].y;
+          }
+          s.re[0][i * LD + j] = vr;
+          s.im[0][i * LD + j] = vi;
+        }
+      }
+  };
+  int status = ST_OK;
+  T sr[TM][TM], si[TM][TM];
+  T cr[TM][TM], ci[TM][TM];
+
+  // 1. right-connected sweep: gR_i, R_i = C_{i-1} gR_i (stored), ends with gR_0 = G_11
+  tzero<T, TM>(sr, si);
+  for (int i = nl - 1; i >= 0; --i) {
+    build(i, sr, si);
+    __syncthreads();
+    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[0], s.im[0], s.re[1], s.im[1], LD, n, *s.gj,
+                                                  tid);
+    if (st != ST_OK) {
+      status = 10 + st;
+      break;
+    }
+    if (i == 0) break;
+    gload<T, NT>(cpl + (size_t)(i - 1) * nn * 2, n, n, s.re[2], s.im[2], LD, tid);
+    __syncthreads();
+    tzero<T, TM>(cr, ci);
+    if (t.own) mac<T, TM, OPN, OPN>(s.re[2], s.im[2], s.re[0], s.im[0], LD, n, t.i0, t.j0, cr, ci);
+    tstore<T, TM>(s.re[3], s.im[3], LD, n, n, t, cr, ci);
+    tstore_glb<T, TM>(blk(vw, i), n, n, t, cr, ci);
+    __syncthreads();
+    tzero<T, TM>(sr, si);
+    if (t.own) mac<T, TM, OPN, OPH>(s.re[3], s.im[3], s.re[2], s.im[2], LD, n, t.i0, t.j0, sr, si);
+    __syncthreads();
+  }
+  if (status != ST_OK) {
+    if (tid == 0) {
+      a.status[inst] = status;
+      a.t_out[inst] = NAN;
+    }
+    return;
+  }
+  // 2. row 1 of G: v_0 = G_11 (matrix 0), v_i = v_{i-1} R_i
+  tzero<T, TM>(cr, ci);
+  if (t.own) {
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+      for (int y = 0; y < TM; ++y) {
+        const int r = t.i0 + x, c = t.j0 + y;
+        if (r < n && c < n) {
+          cr[x][y] = s.re[0][r * LD + c];
+          ci[x][y] = s.im[0][r * LD + c];
+        }
+      }
+  }
+  tstore_glb<T, TM>(blk(vw, 0), n, n, t, cr, ci);
+  __threadfence_block();
+  __syncthreads();
+  for (int i = 1; i < nl; ++i) {
+    bload<T, NT>(blk(vw, i), n, n, s.re[1], s.im[1], LD, tid);
+    __syncthreads();
+    tzero<T, TM>(cr, ci);
+    if (t.own) mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[1], s.im[1], LD, n, t.i0, t.j0, cr, ci);
+    __syncthreads();
+    tstore<T, TM>(s.re[0], s.im[0], LD, n, n, t, cr, ci);
+    tstore_glb<T, TM>(blk(vw, i), n, n, t, cr, ci);
+    __threadfence_block();
+    __syncthreads();
+  }
+  // 3. Gamma_L (matrix 1), Gamma_R (matrix 2); W = Gamma_R G_1N^H Gamma_L; T = Re Tr[G_1N W]
+  for (int e = tid; e < (int)nn; e += NT) {
+    const int i = e / n, j = e - (e / n) * n;
+    const double2 a1 = sgl[i * n + j], a2 = sgl[j * n + i];
+    s.re[1][i * LD + j] = -(a1.y + a2.y);
+    s.im[1][i * LD + j] = a1.x - a2.x;
+    const double2 b1 = sgr[i * n + j], b2 = sgr[j * n + i];
+    s.re[2][i * LD + j] = -(b1.y + b2.y);
+    s.im[2][i * LD + j] = b1.x - b2.x;
+  }
+  __syncthreads();
+  tzero<T, TM>(cr, ci);
+  if (t.own) mac<T, TM, OPH, OPN>(s.re[0], s.im[0], s.re[1], s.im[1], LD, n, t.i0, t.j0, cr, ci);
+  tstore<T, TM>(s.re[3], s.im[3], LD, n, n, t, cr, ci);
+  __syncthreads();
+  tzero<T, TM>(cr, ci);
+  if (t.own) mac<T, TM, OPN, OPN>(s.re[2], s.im[2], s.re[3], s.im[3], LD, n, t.i0, t.j0, cr, ci);
+  double acc = 0.0;
+  if (t.own) {
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+      for (int y = 0; y < TM; ++y) {
+        const int r = t.i0 + x, c = t.j0 + y;
+        if (r < n && c < n) acc += cr[x][y] * s.re[0][c * LD + r] - ci[x][y] * s.im[0][c * LD + r];
+      }
+  }
+  const double tval = block_sum<NT>(acc, s.sred, tid);
+  if (tid == 0) a.t_out[inst] = tval;
+  tstore<T, TM>(s.re[1], s.im[1], LD, n, n, t, cr, ci);  // W
+  __syncthreads();
+  // 4. P_i = W v_i, in place
+  for (int i = 0; i < nl; ++i) {
+    bload<T, NT>(blk(vw, i), n, n, s.re[2], s.im[2], LD, tid);
+    __syncthreads();
+    tzero<T, TM>(cr, ci);
+    if (t.own) mac<T, TM, OPN, OPN>(s.re[1], s.im[1], s.re[2], s.im[2], LD, n, t.i0, t.j0, cr, ci);
+    __syncthreads();
+    tstore_glb<T, TM>(blk(vw, i), n, n, t, cr, ci);
+  }
+  __threadfence_block();
+  __syncthreads();
+  // 5. left-connected sweep: gL_i, Y_i = gL_i C_i (stored), ends with gL_{N-1} = G_NN
+  tzero<T, TM>(sr, si);
+  for (int i = 0; i < nl; ++i) {
+    build(i, sr, si);
+    __syncthreads();
+    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[0], s.im[0], s.re[1], s.im[1], LD, n, *s.gj,
+                                                  tid);
+    if (st != ST_OK) {
+      status = 10 + st;
+      break;
+    }
+    if (i == nl - 1) break;
+    gload<T, NT>(cpl + (size_t)i * nn * 2, n, n, s.re[2], s.im[2], LD, tid);
+    __syncthreads();
+    tzero<T, TM>(cr, ci);
+    if (t.own) mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[2], s.im[2], LD, n, t.i0, t.j0, cr, ci);
+    tstore<T, TM>(s.re[3], s.im[3], LD, n, n, t, cr, ci);
+    tstore_glb<T, TM>(blk(yw, i), n, n, t, cr, ci);
+    __syncthreads();
+    tzero<T, TM>(sr, si);
+    if (t.own) mac<T, TM, OPH, OPN>(s.re[2], s.im[2], s.re[3], s.im[3], LD, n, t.i0, t.j0, sr, si);
+    __syncthreads();
+  }
+  if (tid == 0) a.status[inst] = status;
+  if (status != ST_OK) return;
+  __threadfence_block();
+  // 6. gradients, i = N-1 .. 0, with u_{i+1} and P_{i+1} kept in LDS
+  T* u1r = s.re[0];  // u_{N-1} = G_NN
+  T* u1i = s.im[0];
+  T* p1r = s.re[1];
+  T* p1i = s.im[1];
+  T* f0r = s.re[2];
+  T* f0i = s.im[2];
+  T* f1r = s.re[3];
+  T* f1i = s.im[3];
+  __syncthreads();
+  bload<T, NT>(blk(vw, nl - 1), n, n, p1r, p1i, LD, tid);
+  __syncthreads();
+  tzero<T, TM>(cr, ci);
+  if (t.own) mac<T, TM, OPH, OPH>(p1r, p1i, u1r, u1i, LD, n, t.i0, t.j0, cr, ci);
+#pragma unroll
+  for (int x = 0; x < TM; ++x)
+#pragma unroll
+    for (int y = 0; y < TM; ++y) {
+      cr[x][y] *= w2;
+      ci[x][y] *= w2;
+    }
+  tstore_glb<T, TM>(blk(go, nl - 1), n, n, t, cr, ci);
+  for (int i = nl - 2; i >= 0; --i) {
+    bload<T, NT>(blk(yw, i), n, n, f0r, f0i, LD, tid);  // Y_i
+    bload<T, NT>(blk(vw, i), n, n, f1r, f1i, LD, tid);  // P_i
+    __syncthreads();
+    tzero<T, TM>(sr, si);  // u_i = Y_i u_{i+1}
+    if (t.own) mac<T, TM, OPN, OPN>(f0r, f0i, u1r, u1i, LD, n, t.i0, t.j0, sr, si);
+    __syncthreads();
+    tstore<T, TM>(f0r, f0i, LD, n, n, t, sr, si);  // f0 = u_i
+    __syncthreads();
+    // coupling i: (u_{i+1} P_i)^H + u_i P_{i+1} = P_i^H u_{i+1}^H + u_i P_{i+1}
+    tzero<T, TM>(cr, ci);
+    if (t.own) {
+      mac<T, TM, OPH, OPH>(f1r, f1i, u1r, u1i, LD, n, t.i0, t.j0, cr, ci);
+      mac<T, TM, OPN, OPN>(f0r, f0i, p1r, p1i, LD, n, t.i0, t.j0, cr, ci);
+    }
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+      for (int y = 0; y < TM; ++y) {
+        cr[x][y] *= w2;
+        ci[x][y] *= w2;
+      }
+    tstore_glb<T, TM>(blk(gc, i), n, n, t, cr, ci);
+    // onsite i: (u_i P_i)^H = P_i^H u_i^H
+    tzero<T, TM>(cr, ci);
+    if (t.own) mac<T, TM, OPH, OPH>(f1r, f1i, f0r, f0i, LD, n, t.i0, t.j0, cr, ci);
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+      for (int y = 0; y < TM; ++y) {
+        cr[x][y] *= w2;
+        ci[x][y] *= w2;
+      }
+    tstore_glb<T, TM>(blk(go, i), n, n, t, cr, ci);
+    __syncthreads();
+    // u_{i+1} <- u_i (f0), P_{i+1} <- P_i (f1); the old ones become free
+    T* xr = u1r;
+    T* xi = u1i;
+    u1r = f0r;
+    u1i = f0i;
+    f0r = xr;
+    f0i = xi;
+    xr = p1r;
+    xi = p1i;
+    p1r = f1r;
+    p1i = f1i;
+    f1r = xr;
+    f1i = xi;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // T = Re Tr[Gamma_L G Gamma_R G^H]  (FP64)
 // ---------------------------------------------------------------------------
 
@@ -964,6 +1260,26 @@ __global__ void __launch_bounds__(256) k_conductance(int n_w, const double* __re
   }
   const double s = block_sum<256>(acc, sred, tid);
   if (tid == 0) out[it] = temp > 0.0 ? (1.0 / (2.0 * M_PI)) * s : 0.0;
+}
+
+// d kappa(T_t) / d T(w_i): the trapezoid weights of k_conductance times its integrand
+// per unit transmission, so that kappa = weights @ T.
+__global__ void __launch_bounds__(256) k_conductance_weights(int n_w, const double* __restrict__ w,
+                                                              int n_t,
+                                                              const double* __restrict__ temps,
+                                                              double* __restrict__ out) {
+  const int it = blockIdx.y;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_w) return;
+  const double temp = temps[it];
+  double v = 0.0;
+  if (temp > 0.0 && n_w > 1) {
+    double h = 0.0;
+    if (i + 1 < n_w) h += w[i + 1] - w[i];
+    if (i > 0) h += w[i] - w[i - 1];
+    v = (1.0 / (2.0 * M_PI)) * conductance_integrand(w[i], 1.0, temp) * h / 2.0;
+  }
+  out[(size_t)it * n_w + i] = v;
 }
 
 // x / (e^x - 1) with the small-x series below 1e-6 and 0 from 700.
@@ -1159,6 +1475,24 @@ static int run_dgf(hipStream_t st, int nmax, const DgfArgs& a) {
   return fail(NEGF_ERR_ARG, "block size class %d unsupported", nmax);
 }
 
+static int run_adj(hipStream_t st, int nmax, const AdjArgs& a) {
+#define ADJ_CASE(NM)                                                                      \
+  case NM: {                                                                              \
+    constexpr int NT = Cfg<NM>::NT;                                                       \
+    NEGF_LAUNCH((k_adjoint<NM>), a.n_inst, NT, (Lds<double, NM, NT, 4>::bytes()), st, a); \
+    return NEGF_OK;                                                                       \
+  }
+  switch (nmax) {
+    ADJ_CASE(8)
+    ADJ_CASE(16)
+    ADJ_CASE(24)
+    ADJ_CASE(32)
+    ADJ_CASE(48)
+  }
+#undef ADJ_CASE
+  return fail(NEGF_ERR_ARG, "block size class %d unsupported", nmax);
+}
+
 }  // namespace negf
 
 using namespace negf;
@@ -1292,6 +1626,47 @@ int negf_device_green_batched(void* stream, int n, int n_layers, int n_inst, con
   DgfArgs a{n, n_layers, n_inst, onsite, coupling, set_of, omega, eta_dev, sigma_l, sigma_r, g_out,
             status};
   return run_dgf(reinterpret_cast<hipStream_t>(stream), nmax, a);
+}
+
+size_t negf_adjoint_workspace_bytes(int n, int n_layers, int n_inst) {
+  if (n <= 0 || n_layers <= 0 || n_inst <= 0) return 0;
+  return (size_t)n_inst * (size_t)(2 * n_layers - 1) * (size_t)n * (size_t)n * 16u;
+}
+
+int negf_adjoint_batched(void* stream, int n, int n_layers, int n_inst, const double* onsite,
+                         const double* coupling, const int32_t* set_of, const double* omega,
+                         const double* eta_dev, const double* sigma_l, const double* sigma_r,
+                         const int32_t* status_in, const double* weight, void* workspace,
+                         size_t workspace_bytes, double* t_out, double* grad_onsite,
+                         double* grad_coupling, int32_t* status) {
+  if (n <= 0 || n_layers <= 0 || n_inst < 0)
+    return fail(NEGF_ERR_ARG, "invalid sizes (n=%d layers=%d)", n, n_layers);
+  if (!onsite || (n_layers > 1 && (!coupling || !grad_coupling)) || !set_of || !omega ||
+      !eta_dev || !sigma_l || !sigma_r || !t_out || !grad_onsite || !status)
+    return fail(NEGF_ERR_ARG, "null pointer argument");
+  if (n_inst == 0) return NEGF_OK;
+  const size_t need = negf_adjoint_workspace_bytes(n, n_layers, n_inst);
+  if (!workspace || workspace_bytes < need)
+    return fail(NEGF_ERR_ARG, "workspace of %zu bytes is smaller than the %zu needed",
+                workspace_bytes, need);
+  const int nmax = pick_nmax(n, true);
+  if (!nmax) return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 limit of 48", n);
+  AdjArgs a{n, n_layers, n_inst, onsite, coupling, set_of, omega, eta_dev, sigma_l, sigma_r,
+            status_in, weight, reinterpret_cast<double*>(workspace), t_out, grad_onsite,
+            grad_coupling, status};
+  return run_adj(reinterpret_cast<hipStream_t>(stream), nmax, a);
+}
+
+int negf_conductance_weights(void* stream, int n_w, const double* omega_rad_s, int n_t,
+                             const double* temps, double* weights_out) {
+  if (n_w < 0 || n_t < 0) return fail(NEGF_ERR_ARG, "invalid sizes (n_w=%d n_t=%d)", n_w, n_t);
+  if (n_w == 0 || n_t == 0) return NEGF_OK;
+  if (!omega_rad_s || !temps || !weights_out) return fail(NEGF_ERR_ARG, "null pointer argument");
+  if (n_t > 65535) return fail(NEGF_ERR_ARG, "at most 65535 temperatures per call");
+  hipLaunchKernelGGL(k_conductance_weights, dim3((n_w + 255) / 256, n_t), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), n_w, omega_rad_s, n_t, temps,
+                     weights_out);
+  return hip_check(hipGetLastError(), "k_conductance_weights launch");
 }
 
 int negf_transmission_trace_batched(void* stream, int n, int n_inst, const double* sigma_l,
