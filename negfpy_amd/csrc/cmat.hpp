@@ -1,9 +1,8 @@
 // Workgroup-cooperative complex matrix primitives for gfx950 (CDNA4).
 //
 // Every matrix of one (omega, k) instance lives in LDS in planar form: a real
-// plane and an imaginary plane, each NMAX rows of LD = NMAX + 2 entries.  The
-// odd-pair padding keeps column walks (pivot search, A-operand reads) spread
-// over the LDS banks.  A thread "owns" a TM x TM output tile of every product
+// plane and an imaginary plane, each NMAX rows of LD = NMAX + 1 entries.  The
+// odd padding keeps column walks (A-operand reads) spread over the LDS banks.  A thread "owns" a TM x TM output tile of every product
 // and keeps tiles of state matrices (onsite energies, Schur accumulators) in
 // VGPRs, so only operands that other threads must read go through LDS.
 //
@@ -264,9 +263,24 @@ struct Pivot {
 template <typename T, int NMAX>
 struct GjBuf {
   Pivot<T> piv[2];
-  int pv[NMAX];  // pivot row of step k
-  int qv[NMAX];  // step at which row r was the pivot
+  int pv[NMAX];       // pivot row of step k
+  int qv[NMAX];       // step at which row r was the pivot
+  T colr[2][NMAX];    // column k of the current state, by step parity
+  T coli[2][NMAX];
 };
+
+// Lane shuffle within a wave (ds_bpermute), 32- or 64-bit.
+template <typename T>
+__device__ __forceinline__ T lane_shfl(T v, int src_lane) {
+  if constexpr (sizeof(T) == 8) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_bpermute(src_lane << 2, (int)(u & 0xffffffffLL));
+    const int hi = __builtin_amdgcn_ds_bpermute(src_lane << 2, (int)(u >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  } else {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src_lane << 2, __float_as_int(v)));
+  }
+}
 
 // 1 / (ar + i ai) = conj(a) / |a|^2; the real reciprocal is the hardware
 // estimate refined by one Newton step (error well below one ulp of the result).
@@ -293,53 +307,62 @@ __device__ __forceinline__ void crecip1(T ar, T ai, T& rr, T& ri) {
 // izamax).  Rows are never exchanged; step k pivots on row p_k and the result
 // is scattered once at the end, A^-1[q(r)][p_m] = Y[r][m] with q = p^-1.
 //
-// Thread t < n_cols * RG owns column j = t / RG and rows g, g + RG, ...
-// (g = t % RG) in registers, so the RG owners of a column are consecutive lanes
-// of one wave and the pivot search is an RG-lane DPP reduction.  The state
-// after each step is written to LDS, alternating between (re, im) and the free
-// matrix (wre, wim), so one barrier per step suffices: step k reads the pivot
-// row and pivot column of the state written in step k-1.  All threads must
-// call it; it ends with a barrier.  Returns ST_OK, ST_SINGULAR or ST_NONFINITE.
+// Thread t < NMAX * RG owns column j = t / RG and rows g, g + RG, ... (g = t % RG)
+// in registers for the whole elimination; the matrix is read from LDS once and
+// written back once.  A step needs only two things from other lanes: column k
+// of the current state, which its owners publish into a parity-buffered column
+// buffer, and the pivot row's entry of the lane's own column, which the lane
+// g = p % RG of the same (wave-contained) column group holds and hands over by a
+// lane shuffle.  The pivot search is an RG-lane DPP arg-max among the owners of
+// the next column, so one barrier per step suffices.  Requires NMAX % RG == 0:
+// every lane then works on rows and columns < NMAX of the padded matrix, and the
+// padding (rows/columns >= n, marked used) never feeds a live entry, so the
+// update is branch-free: new = x' - c s with c = -1 and x' = 0 on the pivot row,
+// x' = 0 in column k, and s the scaled pivot row (s_k = 1 / pivot).  All threads
+// must call it; it ends with a barrier.  Returns ST_OK, ST_SINGULAR or
+// ST_NONFINITE.
 template <typename T, int NT, int NMAX, int RG>
-__device__ __forceinline__ int gj_inverse(T* __restrict__ re, T* __restrict__ im,
-                                          T* __restrict__ wre, T* __restrict__ wim, int ld, int n,
+__device__ __forceinline__ int gj_inverse(T* __restrict__ re, T* __restrict__ im, int ld, int n,
                                           GjBuf<T, NMAX>& b, int tid) {
-  constexpr int MAXR = (NMAX + RG - 1) / RG;
+  static_assert(NMAX % RG == 0, "row groups must tile NMAX");
+  static_assert(WAVE % RG == 0, "a column group must sit inside one wave");
+  constexpr int MAXR = NMAX / RG;
   static_assert(NMAX * RG <= NT, "not enough threads for the inversion layout");
-  const bool on = tid < n * RG;
+  static_assert(MAXR <= 32, "row mask is 32 bits");
+  const bool on = tid < NMAX * RG;
   const int j = on ? tid / RG : 0;
   const int g = tid % RG;
+  const int gbase = (tid & (WAVE - 1)) & ~(RG - 1);
   T xr[MAXR], xi[MAXR];
-  bool used[MAXR];
+  unsigned used = 0;  // bit m: row g + RG m is a padding row or has been a pivot
 #pragma unroll
   for (int m = 0; m < MAXR; ++m) {
     const int i = g + RG * m;
-    used[m] = !(on && i < n);
-    xr[m] = T(0);
-    xi[m] = T(0);
-    if (!used[m]) {
-      xr[m] = re[i * ld + j];
-      xi[m] = im[i * ld + j];
-    }
+    if (i >= n) used |= 1u << m;
+    xr[m] = re[i * ld + j];
+    xi[m] = im[i * ld + j];
   }
-  // pivot of column `col` from the owners' registers; the winner publishes it
-  auto choose = [&](int col, int par) {
+  // the owners of column `col` publish it and choose its pivot
+  auto publish = [&](int col, int par) {
     if (on && j == col) {
+#pragma unroll
+      for (int m = 0; m < MAXR; ++m) {
+        b.colr[par][g + RG * m] = xr[m];
+        b.coli[par][g + RG * m] = xi[m];
+      }
       T best = T(-1);
       int bidx = n;
       bool bad = false;
 #pragma unroll
       for (int m = 0; m < MAXR; ++m) {
-        if (!used[m]) {
-          const T mg = fabs(xr[m]) + fabs(xi[m]);
-          if (mg != mg) bad = true;
-          if (mg > best) {
-            best = mg;
-            bidx = g + RG * m;
-          }
-        }
+        const T mg = fabs(xr[m]) + fabs(xi[m]);
+        const bool live = !((used >> m) & 1u);
+        bad = bad || (live && mg != mg);
+        const bool better = live && mg > best;
+        best = better ? mg : best;
+        bidx = better ? g + RG * m : bidx;
       }
-      if (bad) best = T(INFINITY);  // a NaN anywhere in the group wins and marks it
+      if (bad) best = T(INFINITY);  // a NaN anywhere in the column wins and marks it
       group_argmax<T, RG>(best, bidx);
       if (g == 0 || (bidx < n && g == bidx % RG)) {
         const int flag = isinf(best) ? ST_NONFINITE : (best > T(0) ? ST_OK : ST_SINGULAR);
@@ -349,83 +372,74 @@ __device__ __forceinline__ int gj_inverse(T* __restrict__ re, T* __restrict__ im
         }
         if (flag == ST_OK && g == bidx % RG) {
           const int mw = bidx / RG;
+          T pr = xr[0], pi = xi[0];
 #pragma unroll
-          for (int m = 0; m < MAXR; ++m)
-            if (m == mw) crecip1(xr[m], xi[m], b.piv[par].ir, b.piv[par].ii);
+          for (int m = 1; m < MAXR; ++m) {
+            pr = (m == mw) ? xr[m] : pr;
+            pi = (m == mw) ? xi[m] : pi;
+          }
+          crecip1(pr, pi, b.piv[par].ir, b.piv[par].ii);
           b.piv[par].p = bidx;
           b.piv[par].flag = ST_OK;
         }
       }
     }
   };
-  choose(0, 0);
+  publish(0, 0);
   __syncthreads();
   for (int k = 0; k < n; ++k) {
     const int par = k & 1;
-    const T* cre = par ? wre : re;
-    const T* cim = par ? wim : im;
-    T* nre = par ? re : wre;
-    T* nim = par ? im : wim;
-    // column k of the current state does not depend on the pivot: load it
-    // together with the pivot record, then the pivot row entry, and only then
-    // store (the loads must not wait behind stores to the other buffer)
     T cr[MAXR], ci[MAXR];
 #pragma unroll
     for (int m = 0; m < MAXR; ++m) {
-      const int i = g + RG * m;
-      cr[m] = (on && i < n) ? cre[i * ld + k] : T(0);
-      ci[m] = (on && i < n) ? cim[i * ld + k] : T(0);
+      cr[m] = b.colr[par][g + RG * m];
+      ci[m] = b.coli[par][g + RG * m];
     }
     const Pivot<T> pk = b.piv[par];
-    if (pk.flag != ST_OK) {
+    const int flag = __builtin_amdgcn_readfirstlane(pk.flag);
+    if (flag != ST_OK) {
       __syncthreads();
-      return pk.flag;
+      return flag;
     }
-    const int p = pk.p;
-    if (on) {
-      const T pr = cre[p * ld + j], pi = cim[p * ld + j];
-      const T sr = pr * pk.ir - pi * pk.ii, si = pr * pk.ii + pi * pk.ir;
+    const int p = __builtin_amdgcn_readfirstlane(pk.p);
+    // pivot row entry of this lane's column, from the group lane holding row p
+    const int mp = p / RG;
+    T vr = xr[0], vi = xi[0];
 #pragma unroll
-      for (int m = 0; m < MAXR; ++m) {
-        const int i = g + RG * m;
-        if (i < n) {
-          if (i == p) {
-            used[m] = true;
-            xr[m] = (j == k) ? pk.ir : sr;
-            xi[m] = (j == k) ? pk.ii : si;
-          } else if (j == k) {
-            xr[m] = -(cr[m] * pk.ir - ci[m] * pk.ii);
-            xi[m] = -(cr[m] * pk.ii + ci[m] * pk.ir);
-          } else {
-            T yr = xr[m], yi = xi[m];
-            yr = fma(-cr[m], sr, yr);
-            yr = fma(ci[m], si, yr);
-            yi = fma(-cr[m], si, yi);
-            yi = fma(-ci[m], sr, yi);
-            xr[m] = yr;
-            xi[m] = yi;
-          }
-        }
-      }
-      if (k + 1 < n) {
+    for (int m = 1; m < MAXR; ++m) {
+      vr = (m == mp) ? xr[m] : vr;
+      vi = (m == mp) ? xi[m] : vi;
+    }
+    const int src = gbase + (p % RG);
+    const T pr = lane_shfl(vr, src);
+    const T pi = lane_shfl(vi, src);
+    const bool jk = (j == k);
+    const T sr = jk ? pk.ir : pr * pk.ir - pi * pk.ii;
+    const T si = jk ? pk.ii : pr * pk.ii + pi * pk.ir;
 #pragma unroll
-        for (int m = 0; m < MAXR; ++m) {
-          const int i = g + RG * m;
-          if (i < n) {
-            nre[i * ld + j] = xr[m];
-            nim[i * ld + j] = xi[m];
-          }
-        }
-      }
+    for (int m = 0; m < MAXR; ++m) {
+      const int i = g + RG * m;
+      const bool isp = (i == p);
+      const T c_r = isp ? T(-1) : cr[m];
+      const T c_i = isp ? T(0) : ci[m];
+      T yr = (isp || jk) ? T(0) : xr[m];
+      T yi = (isp || jk) ? T(0) : xi[m];
+      yr = fma(-c_r, sr, yr);
+      yr = fma(c_i, si, yr);
+      yi = fma(-c_r, si, yi);
+      yi = fma(-c_i, sr, yi);
+      xr[m] = yr;
+      xi[m] = yi;
+      used |= isp ? (1u << m) : 0u;
     }
     if (tid == 0) {
       b.pv[k] = p;
       b.qv[p] = k;
     }
-    if (k + 1 < n) choose(k + 1, par ^ 1);
+    if (k + 1 < n) publish(k + 1, par ^ 1);
     __syncthreads();
   }
-  if (on) {
+  if (on && j < n) {
     const int pj = b.pv[j];
 #pragma unroll
     for (int m = 0; m < MAXR; ++m) {

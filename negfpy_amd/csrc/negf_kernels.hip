@@ -59,11 +59,11 @@ struct Cfg<32> {
 };
 template <>
 struct Cfg<48> {
-  static constexpr int NMAX = 48, NT = 256, TM = 3;
+  static constexpr int NMAX = 48, NT = 768, TM = 2;
 };
 template <>
 struct Cfg<64> {
-  static constexpr int NMAX = 64, NT = 256, TM = 4;
+  static constexpr int NMAX = 64, NT = 1024, TM = 2;
 };
 
 // Waves per SIMD the main kernels must allow: the LDS footprint of one
@@ -72,6 +72,12 @@ struct Cfg<64> {
 template <typename T, int NMAX>
 struct Occ {
   static constexpr int value = (NMAX <= 32) ? (sizeof(T) == 8 ? (NMAX <= 16 ? 2 : 3) : 4) : 1;
+};
+// The recursive sweep keeps three matrices per instance instead of four: at
+// n = 24 in FP64 five workgroups (15 waves) fit a CU.
+template <typename T, int NMAX>
+struct OccRgf {
+  static constexpr int value = (sizeof(T) == 8 && NMAX == 24) ? 4 : Occ<T, NMAX>::value;
 };
 
 template <typename T, int NMAX, int NT, int NMAT>
@@ -234,7 +240,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
         }
     }
     __syncthreads();
-    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[2], s.im[2], s.re[3], s.im[3], LD, n, *s.gj, tid);
+    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[2], s.im[2], LD, n, *s.gj, tid);
     if (st != ST_OK) {
       status = st;
       break;
@@ -333,7 +339,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
         }
     }
     __syncthreads();
-    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[2], s.im[2], s.re[3], s.im[3], LD, n, *s.gj, tid);
+    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[2], s.im[2], LD, n, *s.gj, tid);
     if (st != ST_OK) {
       if (tid == 0) a.status[inst] = st;
       return;
@@ -398,7 +404,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_refine(RefineArgs a) {
     }
   }
   __syncthreads();
-  int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[2], s.im[2], s.re[3], s.im[3], LD, n, *s.gj, tid);
+  int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[2], s.im[2], LD, n, *s.gj, tid);
   const double2* d00 = reinterpret_cast<const double2*>(blk);
   double change = 0.0;
   for (int step = 0; step < a.steps && st == ST_OK; ++step) {
@@ -427,7 +433,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_refine(RefineArgs a) {
         }
     }
     __syncthreads();
-    st = gj_inverse<T, NT, NMAX, L::RG>(s.re[2], s.im[2], s.re[3], s.im[3], LD, n, *s.gj, tid);
+    st = gj_inverse<T, NT, NMAX, L::RG>(s.re[2], s.im[2], LD, n, *s.gj, tid);
     if (st == ST_OK && step == a.steps - 1) {
       double dn = 0.0, gn = 0.0;
       if (t.own) {
@@ -543,9 +549,9 @@ struct RgfArgs {
 
 template <typename T, int NMAX>
 __global__ void __launch_bounds__(Cfg<NMAX>::NT)
-    __attribute__((amdgpu_waves_per_eu(Occ<T, NMAX>::value))) k_rgf(RgfArgs a) {
+    __attribute__((amdgpu_waves_per_eu(OccRgf<T, NMAX>::value))) k_rgf(RgfArgs a) {
   constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
-  using L = Lds<T, NMAX, NT, 4>;
+  using L = Lds<T, NMAX, NT, 3>;
   constexpr int LD = L::LD;
   extern __shared__ __align__(16) unsigned char smem[];
   L s(smem);
@@ -563,7 +569,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
   const double* cpl = a.coupling + set * (nl > 1 ? nl - 1 : 0) * nn * 2;
   const double2* sgl = reinterpret_cast<const double2*>(a.sigma_l + (size_t)inst * nn * 2);
   const double2* sgr = reinterpret_cast<const double2*>(a.sigma_r + (size_t)inst * nn * 2);
-  // LDS: [0] G (inverse), [1] Y = g H, [2] Q = G_{1,i} H_i, [3] H
+  // LDS: [0] g (inverted in place), then Y = g H; [1] H; [2] Q = G_{1,i} H_i
   lds_zero<T, NT>(smem, L::mat_bytes(), tid);
   __syncthreads();
   const Tile<TM> t(n, n, tid);
@@ -603,7 +609,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
         }
     }
     __syncthreads();
-    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[0], s.im[0], s.re[1], s.im[1], LD, n, *s.gj, tid);
+    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[0], s.im[0], LD, n, *s.gj, tid);
     if (st != ST_OK) {
       status = 10 + st;
       break;
@@ -627,15 +633,16 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
       }
       break;
     }
-    gload<T, NT>(cpl + (size_t)layer * nn * 2, n, n, s.re[3], s.im[3], LD, tid);
+    gload<T, NT>(cpl + (size_t)layer * nn * 2, n, n, s.re[1], s.im[1], LD, tid);
     __syncthreads();
     T yr[TM][TM], yi[TM][TM];
     tzero<T, TM>(yr, yi);
-    if (t.own) mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[3], s.im[3], LD, n, t.i0, t.j0, yr, yi);
-    tstore<T, TM>(s.re[1], s.im[1], LD, n, n, t, yr, yi);
+    if (t.own) mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[1], s.im[1], LD, n, t.i0, t.j0, yr, yi);
+    __syncthreads();  // every read of g is done: Y replaces it
+    tstore<T, TM>(s.re[0], s.im[0], LD, n, n, t, yr, yi);
     __syncthreads();
     tzero<T, TM>(sr, si);
-    if (t.own) mac<T, TM, OPH, OPN>(s.re[3], s.im[3], s.re[1], s.im[1], LD, n, t.i0, t.j0, sr, si);
+    if (t.own) mac<T, TM, OPH, OPN>(s.re[1], s.im[1], s.re[0], s.im[0], LD, n, t.i0, t.j0, sr, si);
     T qr[TM][TM], qi[TM][TM];
     if (layer == 0) {
 #pragma unroll
@@ -647,7 +654,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
         }
     } else {
       tzero<T, TM>(qr, qi);
-      if (t.own) mac<T, TM, OPN, OPN>(s.re[2], s.im[2], s.re[1], s.im[1], LD, n, t.i0, t.j0, qr, qi);
+      if (t.own) mac<T, TM, OPN, OPN>(s.re[2], s.im[2], s.re[0], s.im[0], LD, n, t.i0, t.j0, qr, qi);
     }
     __syncthreads();
     tstore<T, TM>(s.re[2], s.im[2], LD, n, n, t, qr, qi);
@@ -764,8 +771,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_device_green(DgfArgs a) {
         }
     }
     __syncthreads();
-    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[0], s.im[0], s.re[1], s.im[1], LD, n, *s.gj,
-                                                  tid);
+    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[0], s.im[0], LD, n, *s.gj, tid);
     if (st != ST_OK) {
       status = 10 + st;
       break;
@@ -961,8 +967,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_adjoint(AdjArgs a) {
   for (int i = nl - 1; i >= 0; --i) {
     build(i, sr, si);
     __syncthreads();
-    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[0], s.im[0], s.re[1], s.im[1], LD, n, *s.gj,
-                                                  tid);
+    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[0], s.im[0], LD, n, *s.gj, tid);
     if (st != ST_OK) {
       status = 10 + st;
       break;
@@ -1061,8 +1066,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_adjoint(AdjArgs a) {
   for (int i = 0; i < nl; ++i) {
     build(i, sr, si);
     __syncthreads();
-    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[0], s.im[0], s.re[1], s.im[1], LD, n, *s.gj,
-                                                  tid);
+    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[0], s.im[0], LD, n, *s.gj, tid);
     if (st != ST_OK) {
       status = 10 + st;
       break;
@@ -1418,7 +1422,7 @@ static int run_rgf(hipStream_t st, int nmax, const RgfArgs& a) {
 #define RG_CASE(NM)                                                                  \
   case NM: {                                                                         \
     constexpr int NT = Cfg<NM>::NT;                                                  \
-    NEGF_LAUNCH((k_rgf<T, NM>), a.n_inst, NT, (Lds<T, NM, NT, 4>::bytes()), st, a);  \
+    NEGF_LAUNCH((k_rgf<T, NM>), a.n_inst, NT, (Lds<T, NM, NT, 3>::bytes()), st, a);  \
     return NEGF_OK;                                                                  \
   }
   switch (nmax) {
@@ -1430,7 +1434,7 @@ static int run_rgf(hipStream_t st, int nmax, const RgfArgs& a) {
     case 64:
       if constexpr (sizeof(T) == 4) {
         constexpr int NT = Cfg<64>::NT;
-        NEGF_LAUNCH((k_rgf<T, 64>), a.n_inst, NT, (Lds<T, 64, NT, 4>::bytes()), st, a);
+        NEGF_LAUNCH((k_rgf<T, 64>), a.n_inst, NT, (Lds<T, 64, NT, 3>::bytes()), st, a);
         return NEGF_OK;
       }
       break;
