@@ -142,39 +142,33 @@ __device__ __forceinline__ void mac2b(const T* a1r, const T* a1i, const T* a2r, 
   }
 }
 
-// The four decimation products of one Sancho-Rubio step, sharing operands:
-//   P1 = T1 b, P2 = T2 a, P3 = T1 a, P4 = T2 b   with T1 = a g, T2 = b g.
+// Two products sharing the A operand: C1 += A B1, C2 += A B2.
 template <typename T, int TM>
-__device__ __forceinline__ void mac4(const T* t1r, const T* t1i, const T* t2r, const T* t2i,
-                                     const T* ar_, const T* ai_, const T* br_, const T* bi_, int ld,
-                                     int k, int i0, int j0, T (&p1r)[TM][TM], T (&p1i)[TM][TM],
-                                     T (&p2r)[TM][TM], T (&p2i)[TM][TM], T (&p3r)[TM][TM],
-                                     T (&p3i)[TM][TM], T (&p4r)[TM][TM], T (&p4i)[TM][TM]) {
+__device__ __forceinline__ void mac2a(const T* are, const T* aim, const T* b1r, const T* b1i,
+                                      const T* b2r, const T* b2i, int ld, int k, int i0, int j0,
+                                      T (&c1r)[TM][TM], T (&c1i)[TM][TM], T (&c2r)[TM][TM],
+                                      T (&c2i)[TM][TM]) {
 #pragma unroll 2
   for (int kk = 0; kk < k; ++kk) {
-    T xr[TM], xi[TM], yr[TM], yi[TM], ur[TM], ui[TM], vr[TM], vi[TM];
+    T xr[TM], xi[TM], ur[TM], ui[TM], vr[TM], vi[TM];
 #pragma unroll
     for (int a = 0; a < TM; ++a) {
-      xr[a] = t1r[(i0 + a) * ld + kk];
-      xi[a] = t1i[(i0 + a) * ld + kk];
-      yr[a] = t2r[(i0 + a) * ld + kk];
-      yi[a] = t2i[(i0 + a) * ld + kk];
+      xr[a] = are[(i0 + a) * ld + kk];
+      xi[a] = aim[(i0 + a) * ld + kk];
     }
 #pragma unroll
     for (int b = 0; b < TM; ++b) {
-      ur[b] = ar_[kk * ld + j0 + b];
-      ui[b] = ai_[kk * ld + j0 + b];
-      vr[b] = br_[kk * ld + j0 + b];
-      vi[b] = bi_[kk * ld + j0 + b];
+      ur[b] = b1r[kk * ld + j0 + b];
+      ui[b] = b1i[kk * ld + j0 + b];
+      vr[b] = b2r[kk * ld + j0 + b];
+      vi[b] = b2i[kk * ld + j0 + b];
     }
 #pragma unroll
     for (int a = 0; a < TM; ++a)
 #pragma unroll
       for (int b = 0; b < TM; ++b) {
-        cmac(p1r[a][b], p1i[a][b], xr[a], xi[a], vr[b], vi[b]);
-        cmac(p2r[a][b], p2i[a][b], yr[a], yi[a], ur[b], ui[b]);
-        cmac(p3r[a][b], p3i[a][b], xr[a], xi[a], ur[b], ui[b]);
-        cmac(p4r[a][b], p4i[a][b], yr[a], yi[a], vr[b], vi[b]);
+        cmac(c1r[a][b], c1i[a][b], xr[a], xi[a], ur[b], ui[b]);
+        cmac(c2r[a][b], c2i[a][b], xr[a], xi[a], vr[b], vi[b]);
       }
   }
 }

@@ -73,10 +73,10 @@ template <typename T, int NMAX>
 struct Occ {
   static constexpr int value = (NMAX <= 32) ? (sizeof(T) == 8 ? (NMAX <= 16 ? 2 : 3) : 4) : 1;
 };
-// The recursive sweep keeps three matrices per instance instead of four: at
+// The decimation and the recursive sweep keep three matrices per instance: at
 // n = 24 in FP64 five workgroups (15 waves) fit a CU.
 template <typename T, int NMAX>
-struct OccRgf {
+struct Occ3 {
   static constexpr int value = (sizeof(T) == 8 && NMAX == 24) ? 4 : Occ<T, NMAX>::value;
 };
 
@@ -172,9 +172,9 @@ struct SrArgs {
 
 template <typename T, int NMAX>
 __global__ void __launch_bounds__(Cfg<NMAX>::NT)
-    __attribute__((amdgpu_waves_per_eu(Occ<T, NMAX>::value))) k_sancho_rubio(SrArgs a) {
+    __attribute__((amdgpu_waves_per_eu(Occ3<T, NMAX>::value))) k_sancho_rubio(SrArgs a) {
   constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
-  using L = Lds<T, NMAX, NT, 4>;
+  using L = Lds<T, NMAX, NT, 3>;
   constexpr int LD = L::LD;
   extern __shared__ __align__(16) unsigned char smem[];
   L s(smem);
@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
 
   lds_zero<T, NT>(smem, L::mat_bytes(), tid);
   __syncthreads();
-  // alpha = d01 -> s.re[0], beta = d10 -> s.re[1]; g -> [2]; work -> [3]
+  // alpha = d01 -> s.re[0], beta = d10 -> s.re[1]; g, then T1 = alpha g, then T2 = beta g -> [2]
   gload<T, NT>(blk + nn * 2, n, n, s.re[0], s.im[0], LD, tid);
   gload<T, NT>(blk + 2 * nn * 2, n, n, s.re[1], s.im[1], LD, tid);
 
@@ -254,17 +254,33 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
                    t2r, t2i);
     __syncthreads();
     tstore<T, TM>(s.re[2], s.im[2], LD, n, n, t, t1r, t1i);
-    tstore<T, TM>(s.re[3], s.im[3], LD, n, n, t, t2r, t2i);
     __syncthreads();
-    T p1r[TM][TM], p1i[TM][TM], p2r[TM][TM], p2i[TM][TM];
-    T p3r[TM][TM], p3i[TM][TM], p4r[TM][TM], p4i[TM][TM];
+    // P1 = T1 b (into e_s and e at once), P3 = T1 a (the next alpha)
+    T p1r[TM][TM], p1i[TM][TM], p3r[TM][TM], p3i[TM][TM];
     tzero<T, TM>(p1r, p1i);
-    tzero<T, TM>(p2r, p2i);
     tzero<T, TM>(p3r, p3i);
+    if (t.own)
+      mac2a<T, TM>(s.re[2], s.im[2], s.re[1], s.im[1], s.re[0], s.im[0], LD, n, t.i0, t.j0, p1r, p1i,
+                   p3r, p3i);
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+      for (int y = 0; y < TM; ++y) {
+        sr[x][y] = sr[x][y] + p1r[x][y];
+        si[x][y] = si[x][y] + p1i[x][y];
+        er[x][y] = er[x][y] + p1r[x][y];
+        ei[x][y] = ei[x][y] + p1i[x][y];
+      }
+    __syncthreads();
+    tstore<T, TM>(s.re[2], s.im[2], LD, n, n, t, t2r, t2i);
+    __syncthreads();
+    // P2 = T2 a (into e), P4 = T2 b (the next beta)
+    T p2r[TM][TM], p2i[TM][TM], p4r[TM][TM], p4i[TM][TM];
+    tzero<T, TM>(p2r, p2i);
     tzero<T, TM>(p4r, p4i);
     if (t.own)
-      mac4<T, TM>(s.re[2], s.im[2], s.re[3], s.im[3], s.re[0], s.im[0], s.re[1], s.im[1], LD, n, t.i0,
-                  t.j0, p1r, p1i, p2r, p2i, p3r, p3i, p4r, p4i);
+      mac2a<T, TM>(s.re[2], s.im[2], s.re[0], s.im[0], s.re[1], s.im[1], LD, n, t.i0, t.j0, p2r, p2i,
+                   p4r, p4i);
     __syncthreads();
     bool bad = false;
     double emax = 0.0;
@@ -275,10 +291,8 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
         for (int y = 0; y < TM; ++y) {
           const int i = t.i0 + x, j = t.j0 + y;
           if (i < n && j < n) {
-            sr[x][y] = sr[x][y] + p1r[x][y];
-            si[x][y] = si[x][y] + p1i[x][y];
-            er[x][y] = (er[x][y] + p1r[x][y]) + p2r[x][y];
-            ei[x][y] = (ei[x][y] + p1i[x][y]) + p2i[x][y];
+            er[x][y] = er[x][y] + p2r[x][y];
+            ei[x][y] = ei[x][y] + p2i[x][y];
             s.re[0][i * LD + j] = p3r[x][y];
             s.im[0][i * LD + j] = p3i[x][y];
             s.re[1][i * LD + j] = p4r[x][y];
@@ -549,7 +563,7 @@ struct RgfArgs {
 
 template <typename T, int NMAX>
 __global__ void __launch_bounds__(Cfg<NMAX>::NT)
-    __attribute__((amdgpu_waves_per_eu(OccRgf<T, NMAX>::value))) k_rgf(RgfArgs a) {
+    __attribute__((amdgpu_waves_per_eu(Occ3<T, NMAX>::value))) k_rgf(RgfArgs a) {
   constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
   using L = Lds<T, NMAX, NT, 3>;
   constexpr int LD = L::LD;
@@ -1360,7 +1374,7 @@ static int run_sr(hipStream_t st, int nmax, const SrArgs& a) {
 #define SR_CASE(NM)                                                                   \
   case NM: {                                                                          \
     constexpr int NT = Cfg<NM>::NT;                                                   \
-    NEGF_LAUNCH((k_sancho_rubio<T, NM>), a.n_inst, NT, (Lds<T, NM, NT, 4>::bytes()), st, a); \
+    NEGF_LAUNCH((k_sancho_rubio<T, NM>), a.n_inst, NT, (Lds<T, NM, NT, 3>::bytes()), st, a); \
     return NEGF_OK;                                                                   \
   }
   switch (nmax) {
@@ -1372,7 +1386,7 @@ static int run_sr(hipStream_t st, int nmax, const SrArgs& a) {
     case 64:
       if constexpr (sizeof(T) == 4) {
         constexpr int NT = Cfg<64>::NT;
-        NEGF_LAUNCH((k_sancho_rubio<T, 64>), a.n_inst, NT, (Lds<T, 64, NT, 4>::bytes()), st, a);
+        NEGF_LAUNCH((k_sancho_rubio<T, 64>), a.n_inst, NT, (Lds<T, 64, NT, 3>::bytes()), st, a);
         return NEGF_OK;
       }
       break;
