@@ -73,8 +73,9 @@ template <typename T, int NMAX>
 struct Occ {
   static constexpr int value = (NMAX <= 32) ? (sizeof(T) == 8 ? (NMAX <= 16 ? 2 : 3) : 4) : 1;
 };
-// The decimation and the recursive sweep keep three matrices per instance: at
-// n = 24 in FP64 five workgroups (15 waves) fit a CU.
+// The recursive sweep keeps three matrices per instance and fits 128 VGPRs: at
+// n = 24 in FP64 five workgroups (15 waves) fit a CU.  (The decimation also keeps
+// three matrices but needs more registers than 128 without spilling.)
 template <typename T, int NMAX>
 struct Occ3 {
   static constexpr int value = (sizeof(T) == 8 && NMAX == 24) ? 4 : Occ<T, NMAX>::value;
@@ -172,7 +173,7 @@ struct SrArgs {
 
 template <typename T, int NMAX>
 __global__ void __launch_bounds__(Cfg<NMAX>::NT)
-    __attribute__((amdgpu_waves_per_eu(Occ3<T, NMAX>::value))) k_sancho_rubio(SrArgs a) {
+    __attribute__((amdgpu_waves_per_eu(Occ<T, NMAX>::value))) k_sancho_rubio(SrArgs a) {
   constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
   using L = Lds<T, NMAX, NT, 3>;
   constexpr int LD = L::LD;
@@ -593,8 +594,6 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
   T sr[TM][TM], si[TM][TM];
   tzero<T, TM>(sr, si);
   int status = ST_OK;
-  T gr[TM][TM], gi[TM][TM];
-  tzero<T, TM>(gr, gi);
   for (int layer = 0; layer < nl; ++layer) {
     if (t.own) {
       const double2* d = ons + (size_t)layer * nn;
@@ -605,22 +604,29 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
           const int i = t.i0 + x, j = t.j0 + y;
           if (i < n && j < n) {
             const double2 dv = d[i * n + j];
-            T vr = (i == j ? zr : T(0)) - (T)dv.x - sr[x][y];
-            T vi = (i == j ? zi : T(0)) - (T)dv.y - si[x][y];
-            if (layer == 0) {
-              const double2 q = sgl[i * n + j];
-              vr -= (T)q.x;
-              vi -= (T)q.y;
-            }
-            if (layer == nl - 1) {
-              const double2 q = sgr[i * n + j];
-              vr -= (T)q.x;
-              vi -= (T)q.y;
-            }
-            s.re[0][i * LD + j] = vr;
-            s.im[0][i * LD + j] = vi;
+            s.re[0][i * LD + j] = (i == j ? zr : T(0)) - (T)dv.x - sr[x][y];
+            s.im[0][i * LD + j] = (i == j ? zi : T(0)) - (T)dv.y - si[x][y];
           }
         }
+    }
+    // the contact self-energies of the end layers, in a coalesced pass (same
+    // order of operations: ((z - D - S) - Sigma_L) - Sigma_R)
+    if (layer == 0 || layer == nl - 1) {
+      __syncthreads();
+      for (int e = tid; e < (int)nn; e += NT) {
+        const int i = e / n, j = e - (e / n) * n;
+        T vr = s.re[0][i * LD + j], vi = s.im[0][i * LD + j];
+        if (layer == 0) {
+          vr -= (T)sgl[e].x;
+          vi -= (T)sgl[e].y;
+        }
+        if (layer == nl - 1) {
+          vr -= (T)sgr[e].x;
+          vi -= (T)sgr[e].y;
+        }
+        s.re[0][i * LD + j] = vr;
+        s.im[0][i * LD + j] = vi;
+      }
     }
     __syncthreads();
     const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[0], s.im[0], LD, n, *s.gj, tid);
@@ -629,6 +635,9 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
       break;
     }
     if (layer == nl - 1) {
+      // G_1N = Q_{N-1} g_N (g_1 itself for one layer), straight to the output
+      T gr[TM][TM], gi[TM][TM];
+      tzero<T, TM>(gr, gi);
       if (nl == 1) {
         if (t.own) {
 #pragma unroll
@@ -645,6 +654,11 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
       } else if (t.own) {
         mac<T, TM, OPN, OPN>(s.re[2], s.im[2], s.re[0], s.im[0], LD, n, t.i0, t.j0, gr, gi);
       }
+      // through LDS, so the output is written coalesced and its addresses are
+      // formed only here
+      tstore<T, TM>(s.re[1], s.im[1], LD, n, n, t, gr, gi);
+      __syncthreads();
+      gstore<T, NT>(a.g1n_out + (size_t)inst * nn * 2, n, s.re[1], s.im[1], LD, tid);
       break;
     }
     gload<T, NT>(cpl + (size_t)layer * nn * 2, n, n, s.re[1], s.im[1], LD, tid);
@@ -652,39 +666,23 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
     T yr[TM][TM], yi[TM][TM];
     tzero<T, TM>(yr, yi);
     if (t.own) mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[1], s.im[1], LD, n, t.i0, t.j0, yr, yi);
-    __syncthreads();  // every read of g is done: Y replaces it
+    __syncthreads();  // every read of g is done: Y replaces it (and starts Q at layer 0)
     tstore<T, TM>(s.re[0], s.im[0], LD, n, n, t, yr, yi);
+    if (layer == 0) tstore<T, TM>(s.re[2], s.im[2], LD, n, n, t, yr, yi);
     __syncthreads();
     tzero<T, TM>(sr, si);
     if (t.own) mac<T, TM, OPH, OPN>(s.re[1], s.im[1], s.re[0], s.im[0], LD, n, t.i0, t.j0, sr, si);
-    T qr[TM][TM], qi[TM][TM];
-    if (layer == 0) {
-#pragma unroll
-      for (int x = 0; x < TM; ++x)
-#pragma unroll
-        for (int y = 0; y < TM; ++y) {
-          qr[x][y] = yr[x][y];
-          qi[x][y] = yi[x][y];
-        }
-    } else {
+    if (layer > 0) {
+      T qr[TM][TM], qi[TM][TM];
       tzero<T, TM>(qr, qi);
       if (t.own) mac<T, TM, OPN, OPN>(s.re[2], s.im[2], s.re[0], s.im[0], LD, n, t.i0, t.j0, qr, qi);
+      __syncthreads();
+      tstore<T, TM>(s.re[2], s.im[2], LD, n, n, t, qr, qi);
+    } else {
+      __syncthreads();
     }
-    __syncthreads();
-    tstore<T, TM>(s.re[2], s.im[2], LD, n, n, t, qr, qi);
   }
   if (tid == 0) a.status[inst] = status;
-  if (status != ST_OK) return;
-  if (t.own) {
-    double2* o = reinterpret_cast<double2*>(a.g1n_out + (size_t)inst * nn * 2);
-#pragma unroll
-    for (int x = 0; x < TM; ++x)
-#pragma unroll
-      for (int y = 0; y < TM; ++y) {
-        const int i = t.i0 + x, j = t.j0 + y;
-        if (i < n && j < n) o[i * n + j] = make_double2((double)gr[x][y], (double)gi[x][y]);
-      }
-  }
 }
 
 // ---------------------------------------------------------------------------
