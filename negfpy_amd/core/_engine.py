@@ -311,6 +311,42 @@ class TransmissionSolver:
         return out
 
 
+class TransmissionGraph:
+    """One batched transmission solve captured as a hipGraph (``torch.cuda.CUDAGraph``).
+
+    The graph holds the whole launch sequence of ``TransmissionSolver.run`` (both
+    decimations, the self-energies, the sweep, the trace; in mixed mode also the
+    refinement and FP64 redo launches, which exit early per instance).  Its inputs
+    ``set_of``, ``omega``, ``eta_lead``, ``eta_dev`` are device tensors owned by the
+    graph: write new values into them in place (``g.omega.copy_(...)``) and
+    ``replay()`` solves the batch again with no host work and no launch overhead.
+    ``out`` is overwritten by every replay.  The captured solve allocates nothing and
+    records no memset node.
+    """
+
+    def __init__(self, solver: "TransmissionSolver", set_of, omega, eta_lead, eta_dev=None):
+        self.solver = solver
+        self.set_of = to_dev_i32(set_of).clone()
+        self.omega = to_dev_f64(omega).clone()
+        self.eta_lead = to_dev_f64(eta_lead).clone()
+        self.eta_dev = (self.eta_lead if eta_dev is None else to_dev_f64(eta_dev)).clone()
+        n_inst = int(self.omega.numel())
+        self.out = solver.alloc_result(n_inst)
+        # one eager run on a side stream sizes the workspace before capture
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            solver.run(self.set_of, self.omega, self.eta_lead, self.eta_dev, out=self.out)
+        torch.cuda.current_stream().wait_stream(side)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            solver.run(self.set_of, self.omega, self.eta_lead, self.eta_dev, out=self.out)
+
+    def replay(self) -> "TransmissionResult":
+        self.graph.replay()
+        return self.out
+
+
 def lead_self_energies(sets: TransportSets, set_of, omega, eta_lead, *, tol=1e-12, maxiter=200):
     """FP64 decimation of both leads and Sigma = V g V^H embedded in the end layers.
 
