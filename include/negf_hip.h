@@ -217,6 +217,27 @@ int negf_thermal_conductance(void* stream, int n_w, const double* omega_rad_s, c
                              int n_t, const double* temps, double* out);
 
 /*
+ * k-space blocks of a force-constant model for n_k k points at once (FP64):
+ * Phi_ab(k) = sum_t Phi_ab,t ph_ab[k][t] (+ pinning on the diagonal of Phi_00),
+ * D = M_l^-1/2 Phi M_r^-1/2.  Replaces the per-k host callbacks of
+ * src/negfpy/models/material_kspace.py:99-193 (_sum_k_terms,
+ * _build_dynamical_blocks, the lead and device builders).
+ *   phi00/phi01/phi10 [t][n][n] complex128 terms (t10 = 0: Phi_10 = Phi_01^H);
+ *   ph00/ph01/ph10 [n_k][t] complex phases e^{i (ky dy + kz dz)};
+ *   s_lead [n] = 1/sqrt(m) of the lead cell (null: no lead blocks);
+ *   s_dev [L][n] per device layer (null: no device blocks);
+ *   lead_out [n_k][3][n][n] (d00, d01, d10); onsite_out [n_k][L][n][n];
+ *   coupling_out [n_k][L-1][n][n].
+ * Products and sums are rounded separately in the reference's order, so the
+ * blocks equal the reference builder's for the same phases and masses.
+ */
+int negf_kspace_blocks(void* stream, int n, int n_layers, int n_k, int t00, int t01, int t10,
+                       const double* phi00, const double* phi01, const double* phi10,
+                       const double* ph00, const double* ph01, const double* ph10,
+                       double pinning, const double* s_lead, const double* s_dev,
+                       double* lead_out, double* onsite_out, double* coupling_out);
+
+/*
  * d G(T_t) / d T(w_i) for the conductance above: weights_out [n_t][n_w] with
  * G(T_t) = sum_i weights_out[t][i] T(w_i).  Chain rule of
  * src/negfpy/workflows/ifc_bulk.py:408-438 for gradients of kappa.

@@ -15,7 +15,7 @@ import numpy as np
 
 from . import _engine
 from ._status import transmission_value
-from .negf import DeviceLike, LeadLike, _resolve_many, solve_points
+from .negf import DeviceLike, LeadLike, _resolve_many, kspace_sets, solve_points
 from .surface_gf import _prec
 
 Array = np.ndarray
@@ -56,15 +56,21 @@ def transmission_spectrum(omegas, device, lead_left, lead_right, *, kpoints=None
     ks = [None] if kpoints is None else list(kpoints)
     if len(ks) == 0:
         raise ValueError("kpoints must contain at least one k-point.")
-    kps = _resolve_many(device, lead_left, lead_right, ks, eta=eta, eta_device=None,
-                        device_to_lead_left=None, device_to_lead_right=None,
-                        contact_left_indices=None, contact_right_indices=None, omega_scale=None,
-                        precision=prec)
     nk = len(ks)
     om = np.repeat(w, nk)
     kk = np.tile(np.arange(nk), w.size)
     et = np.full(om.size, float(eta))
-    t, st = solve_points(kps, kk, om, et, et, precision=prec)
+    sets = kspace_sets(device, lead_left, lead_right, ks, precision=prec) \
+        if kpoints is not None else None
+    if sets is not None:
+        res = _engine.TransmissionSolver(sets, precision=prec).run(kk.astype(np.int32), om, et, et)
+        t, st = res.t.cpu().numpy(), res.status.cpu().numpy()
+    else:
+        kps = _resolve_many(device, lead_left, lead_right, ks, eta=eta, eta_device=None,
+                            device_to_lead_left=None, device_to_lead_right=None,
+                            contact_left_indices=None, contact_right_indices=None,
+                            omega_scale=None, precision=prec)
+        t, st = solve_points(kps, kk, om, et, et, precision=prec)
     vals = np.array([transmission_value(int(c), float(v)) for c, v in zip(st, t)])
     return vals.reshape(w.size, nk).mean(axis=1)
 

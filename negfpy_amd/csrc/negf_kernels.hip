@@ -1238,6 +1238,108 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_trace(TraceArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// k-space block builder (FP64)
+// ---------------------------------------------------------------------------
+//
+// Phi_ab(k) = sum_t Phi_ab,t e^{i theta_t(k)} (plus the onsite pinning on the
+// diagonal of Phi_00), scaled as D = M_l^-1/2 Phi M_r^-1/2, for every k point and
+// every block of the lead (d00, d01, d10) and of the device (L onsite, L-1
+// coupling blocks).  Reference: src/negfpy/models/material_kspace.py:99-128
+// (_sum_k_terms, _build_dynamical_blocks) and :166-193 (per-layer masses).  The
+// phases and 1/sqrt(m) come from the host; each output element is one chain of
+// separately rounded products and sums in the reference's order (no fused
+// multiply-add), so the blocks equal the reference's to the last bit.
+
+struct KspaceArgs {
+  int n, n_layers, n_k;
+  int t00, t01, t10;        // term counts; t10 = 0: Phi_10 = Phi_01^H
+  const double* phi00;      // [t00][n][n] complex
+  const double* phi01;      // [t01][n][n]
+  const double* phi10;      // [t10][n][n]
+  const double* ph00;       // [n_k][t00] complex phases
+  const double* ph01;       // [n_k][t01]
+  const double* ph10;       // [n_k][t10]
+  double pinning;
+  const double* s_lead;     // [n] 1/sqrt(m) of the lead cell (null: no lead output)
+  const double* s_dev;      // [L][n] per device layer (null: no device output)
+  double* lead_out;         // [n_k][3][n][n]
+  double* onsite_out;       // [n_k][L][n][n]
+  double* coupling_out;     // [n_k][L-1][n][n]
+};
+
+__device__ __forceinline__ double2 kspace_sum(const double* __restrict__ phi,
+                                              const double* __restrict__ ph, int nt, size_t nn,
+                                              size_t e) {
+#pragma clang fp contract(off)
+  double ar = 0.0, ai = 0.0;
+  for (int q = 0; q < nt; ++q) {
+    const double br = phi[(q * nn + e) * 2], bi = phi[(q * nn + e) * 2 + 1];
+    const double c = ph[2 * q], sn = ph[2 * q + 1];
+    const double pr = br * c - bi * sn;
+    const double pi = br * sn + bi * c;
+    ar = ar + pr;
+    ai = ai + pi;
+  }
+  return make_double2(ar, ai);
+}
+
+__global__ void __launch_bounds__(256) k_kspace_blocks(KspaceArgs a) {
+#pragma clang fp contract(off)
+  const int k = blockIdx.x;
+  const int b = blockIdx.y;  // 0..2 lead d00, d01, d10; then L onsite; then L-1 couplings
+  const int n = a.n, nl = a.n_layers;
+  const size_t nn = (size_t)n * n;
+  int kind;      // 0: Phi00, 1: Phi01, 2: Phi10
+  const double* sl;
+  const double* sr;
+  double* out;
+  if (b < 3) {
+    if (!a.s_lead) return;
+    kind = b;
+    sl = a.s_lead;
+    sr = a.s_lead;
+    out = a.lead_out + ((size_t)k * 3 + b) * nn * 2;
+  } else if (b < 3 + nl) {
+    if (!a.s_dev) return;
+    const int i = b - 3;
+    kind = 0;
+    sl = a.s_dev + (size_t)i * n;
+    sr = sl;
+    out = a.onsite_out + ((size_t)k * nl + i) * nn * 2;
+  } else {
+    if (!a.s_dev) return;
+    const int i = b - 3 - nl;
+    kind = 1;
+    sl = a.s_dev + (size_t)i * n;
+    sr = a.s_dev + (size_t)(i + 1) * n;
+    out = a.coupling_out + ((size_t)k * (nl - 1) + i) * nn * 2;
+  }
+  for (int e = threadIdx.x; e < (int)nn; e += 256) {
+    const int i = e / n, j = e - (e / n) * n;
+    double2 v;
+    if (kind == 0) {
+      v = kspace_sum(a.phi00, a.ph00 + (size_t)k * a.t00 * 2, a.t00, nn, e);
+      if (a.pinning != 0.0 && i == j) v.x = v.x + a.pinning;
+    } else if (kind == 1 || a.t10 > 0) {
+      const bool is10 = (kind == 2);
+      v = is10 ? kspace_sum(a.phi10, a.ph10 + (size_t)k * a.t10 * 2, a.t10, nn, e)
+               : kspace_sum(a.phi01, a.ph01 + (size_t)k * a.t01 * 2, a.t01, nn, e);
+    } else {
+      // Phi_10(k) = Phi_01(k)^H
+      const double2 w = kspace_sum(a.phi01, a.ph01 + (size_t)k * a.t01 * 2, a.t01, nn,
+                                   (size_t)j * n + i);
+      v = make_double2(w.x, -w.y);
+    }
+    // (M_l^-1/2 Phi) M_r^-1/2 for d00, d01; d10 = M_r^-1/2 Phi_10 M_l^-1/2 of the lead
+    const double li = sl[i], rj = sr[j];
+    double2 o;
+    o.x = (li * v.x) * rj;
+    o.y = (li * v.y) * rj;
+    reinterpret_cast<double2*>(out)[e] = o;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // omega integrals (FP64)
 // ---------------------------------------------------------------------------
 
@@ -1671,6 +1773,31 @@ int negf_adjoint_batched(void* stream, int n, int n_layers, int n_inst, const do
             status_in, weight, reinterpret_cast<double*>(workspace), t_out, grad_onsite,
             grad_coupling, status};
   return run_adj(reinterpret_cast<hipStream_t>(stream), nmax, a);
+}
+
+int negf_kspace_blocks(void* stream, int n, int n_layers, int n_k, int t00, int t01, int t10,
+                       const double* phi00, const double* phi01, const double* phi10,
+                       const double* ph00, const double* ph01, const double* ph10,
+                       double pinning, const double* s_lead, const double* s_dev,
+                       double* lead_out, double* onsite_out, double* coupling_out) {
+  if (n <= 0 || n_k < 0 || t00 <= 0 || t01 <= 0 || t10 < 0 || (s_dev && n_layers <= 0))
+    return fail(NEGF_ERR_ARG, "invalid sizes (n=%d layers=%d k=%d terms=%d/%d/%d)", n, n_layers,
+                n_k, t00, t01, t10);
+  if (!phi00 || !phi01 || (t10 > 0 && (!phi10 || !ph10)) || !ph00 || !ph01)
+    return fail(NEGF_ERR_ARG, "null pointer argument");
+  if ((s_lead && !lead_out) || (s_dev && (!onsite_out || (n_layers > 1 && !coupling_out))))
+    return fail(NEGF_ERR_ARG, "null output for a requested block set");
+  if (!s_lead && !s_dev) return fail(NEGF_ERR_ARG, "neither lead nor device blocks requested");
+  if (n_k == 0) return NEGF_OK;
+  if (n_k > 2147483647 / 2) return fail(NEGF_ERR_ARG, "too many k points");
+  const int nl = s_dev ? n_layers : 0;
+  const int nblk = 3 + nl + (nl > 1 ? nl - 1 : 0);
+  if (nblk > 65535) return fail(NEGF_ERR_ARG, "too many layers (%d)", n_layers);
+  KspaceArgs a{n, nl, n_k, t00, t01, t10, phi00, phi01, phi10, ph00, ph01, ph10, pinning,
+               s_lead, s_dev, lead_out, onsite_out, coupling_out};
+  hipLaunchKernelGGL(k_kspace_blocks, dim3(n_k, nblk), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), a);
+  return hip_check(hipGetLastError(), "k_kspace_blocks launch");
 }
 
 int negf_conductance_weights(void* stream, int n_w, const double* omega_rad_s, int n_t,

@@ -102,6 +102,8 @@ def material_kspace_lead(params: MaterialKspaceParams) -> LeadKSpace:
         ky, kz = _kyz(kpar)
         return _dyn_blocks(params, ky, kz, params.masses, params.masses)
 
+    # lets batched solvers build every k point's blocks on the GPU in one launch
+    build.material_kspace = params
     return LeadKSpace(blocks_builder=build)
 
 
@@ -125,4 +127,5 @@ def material_kspace_device(n_layers: int, params: MaterialKspaceParams,
                     for i in range(n_layers - 1)]
         return Device1D(onsite_blocks=onsite, coupling_blocks=coupling)
 
+    build.material_kspace = (params, int(n_layers), masses)
     return DeviceKSpace(device_builder=build)

@@ -16,7 +16,8 @@ of independent solvers fix:
   ``normalize_per_mode``;
 * ``device_green_function`` (``negf.py:253-303``);
 * ``transmission_kavg_adaptive`` / ``..._global_eta`` decisions and statistics;
-* the k-space builders used by the reference's transport tests;
+* the k-space builders used by the reference's transport tests, and the general
+  force-constant builder (``material_kspace``) with its k-averaged transmission;
 * ``transmission`` at the 24-DOF, 32-layer ribbon shape (a few omega points).
 
 Outputs (data only, no code): ``reference_vectors.npz`` and
@@ -186,6 +187,45 @@ def main(ref_src: str) -> None:
     adaptive["sq_global_eta"] = {"omega": 1.1, "t": t, "info": info}
     t, info = rc.transmission_kavg_adaptive(1.0, cdev, clead, clead, kpoints=[(), ()])
     adaptive["chain_plain"] = {"omega": 1.0, "t": t, "info": info}
+
+    # --- material k-space builder and its k-averaged transmission ----------
+    rngm = np.random.default_rng(21)
+    nd = 6
+
+    def _sym(a):
+        return 0.5 * (a + a.T)
+
+    a0 = _sym(rngm.uniform(-1, 1, (nd, nd))) + 4.0 * np.eye(nd)
+    a1 = rngm.uniform(-0.3, 0.3, (nd, nd))
+    a2 = rngm.uniform(-0.3, 0.3, (nd, nd))
+    b0 = -rngm.uniform(0.2, 0.8, (nd, nd))
+    b1 = rngm.uniform(-0.2, 0.2, (nd, nd))
+    b2 = rngm.uniform(-0.2, 0.2, (nd, nd))
+    fc00 = {(0, 0): a0, (1, 0): a1, (-1, 0): a1.T, (0, 1): a2, (0, -1): a2.T}
+    fc01 = {(0, 0): b0, (1, 0): b1, (0, 1): b2}
+    fc10 = {(0, 0): b0.T, (-1, 0): b1.T, (0, -1): b2.T}
+    mk_masses = np.array([12.0, 28.0])
+    mk_layer_masses = rngm.uniform(10.0, 30.0, (4, 2))
+    mk_k = [(0.3,), (1.1, -0.7), (), (-2.0, 0.5)]
+    out["mk_a"] = np.stack([a0, a1, a2])
+    out["mk_b"] = np.stack([b0, b1, b2])
+    out["mk_masses"] = mk_masses
+    out["mk_layer_masses"] = mk_layer_masses
+    for tag, f10 in (("h", None), ("x", fc10)):
+        mp = rm.MaterialKspaceParams(masses=mk_masses, fc00_terms=fc00, fc01_terms=fc01,
+                                     fc10_terms=f10, dof_per_atom=3, onsite_pinning=0.03)
+        ml = rm.material_kspace_lead(mp)
+        md = rm.material_kspace_device(4, mp, layer_masses=mk_layer_masses)
+        out[f"mk_{tag}_lead"] = np.stack([np.stack(ml.blocks_builder(k)) for k in mk_k])
+        out[f"mk_{tag}_onsite"] = np.stack([np.stack(md.device_builder(k).onsite_blocks)
+                                            for k in mk_k])
+        out[f"mk_{tag}_coupling"] = np.stack([np.stack(md.device_builder(k).coupling_blocks)
+                                              for k in mk_k])
+        if tag == "h":
+            wmk = np.array([0.15, 0.35, 0.55])
+            out["mk_omegas"] = wmk
+            out["mk_t_kavg"] = np.array([rc.transmission_kavg(w, md, ml, ml, kpoints=mk_k, eta=1e-6)
+                                         for w in wmk])
 
     # --- ribbon, 24 DOF x 32 layers -----------------------------------------
     from negfpy_amd.workloads import ribbon_problem

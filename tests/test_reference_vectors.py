@@ -286,3 +286,43 @@ def test_adaptive_chain_defaults_match_reference(oracle_backed):
         kpoints=[(), ()])
     assert abs(t - ref["t"]) <= TOL * max(abs(ref["t"]), 1e-3)
     assert_info_matches(info, ref["info"])
+
+
+@pytest.mark.parametrize("tag", ["h", "x"])
+def test_host_material_kspace_builder_matches_reference(tag):
+    """The general force-constant k-space builder (material_kspace.py:99-193), implicit
+    (Phi_10 = Phi_01^H) and explicit 10 terms, pinning, per-layer masses, kz."""
+    from negfpy_amd.models import (MaterialKspaceParams, material_kspace_device,
+                                   material_kspace_lead)
+
+    mp = material_params(MaterialKspaceParams, explicit_10=(tag == "x"))
+    ml = material_kspace_lead(mp)
+    md = material_kspace_device(4, mp, layer_masses=REF["mk_layer_masses"])
+    for i, k in enumerate(MK_K):
+        np.testing.assert_array_equal(np.stack(ml.blocks_builder(k)), REF[f"mk_{tag}_lead"][i])
+        d = md.device_builder(k)
+        np.testing.assert_array_equal(np.stack(d.onsite_blocks), REF[f"mk_{tag}_onsite"][i])
+        np.testing.assert_array_equal(np.stack(d.coupling_blocks), REF[f"mk_{tag}_coupling"][i])
+
+
+MK_K = [(0.3,), (1.1, -0.7), (), (-2.0, 0.5)]
+
+
+def material_params(cls, explicit_10=False):
+    a0, a1, a2 = REF["mk_a"]
+    b0, b1, b2 = REF["mk_b"]
+    fc00 = {(0, 0): a0, (1, 0): a1, (-1, 0): a1.T, (0, 1): a2, (0, -1): a2.T}
+    fc01 = {(0, 0): b0, (1, 0): b1, (0, 1): b2}
+    fc10 = {(0, 0): b0.T, (-1, 0): b1.T, (0, -1): b2.T} if explicit_10 else None
+    return cls(masses=REF["mk_masses"], fc00_terms=fc00, fc01_terms=fc01, fc10_terms=fc10,
+               dof_per_atom=3, onsite_pinning=0.03)
+
+
+def test_oracle_material_kavg_matches_reference():
+    ons = REF["mk_h_onsite"]
+    cps = REF["mk_h_coupling"]
+    lead = REF["mk_h_lead"]
+    for w, t_ref in zip(REF["mk_omegas"], REF["mk_t_kavg"]):
+        ts = [o.transmission_sparse(w, list(ons[i]), list(cps[i]), tuple(lead[i]), tuple(lead[i]),
+                                    eta=1e-6) for i in range(len(MK_K))]
+        assert abs(np.mean(ts) - t_ref) <= TOL * max(abs(t_ref), 1e-3)
