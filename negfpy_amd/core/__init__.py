@@ -15,6 +15,7 @@ from .negf import (
     transmission_kavg,
     transmission_kavg_adaptive,
     transmission_kavg_adaptive_global_eta,
+    transmission_kavg_adaptive_spectrum,
 )
 from .surface_gf import (
     get_default_precision,
@@ -59,6 +60,7 @@ __all__ = [
     "transmission_kavg",
     "transmission_kavg_adaptive",
     "transmission_kavg_adaptive_global_eta",
+    "transmission_kavg_adaptive_spectrum",
     "transmission_spectrum",
     "heat_current_from_spectrum",
     "heat_current_1d",

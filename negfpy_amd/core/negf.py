@@ -458,22 +458,179 @@ def _resolve_each(device, lead_left, lead_right, kpoints, *, eta_device, precisi
     return out
 
 
-def _attempt(omega, kps, ks, eta, eta_device, prec, nonneg_tol, t_max):
-    """Outcomes of one eta rung for the k points ``ks``: {k: (reason, value)}."""
-    eta_dev = float(eta if eta_device is None else eta_device)
-    codes = {}
-    live = [k for k in ks if kps[k] is not None and eta_dev >= 0.0]
-    for k in ks:
-        if k not in live:
-            codes[k] = ("exception", None)
-    if live:
-        valid = [kps[k] for k in live]
-        m = len(live)
-        t, st = solve_points(valid, np.arange(m), np.full(m, float(omega)), np.full(m, float(eta)),
-                             np.full(m, eta_dev), precision=prec)
-        for j, k in enumerate(live):
-            codes[k] = _classify(st[j], t[j], nonneg_tol, t_max[k])
-    return codes
+class _KSolver:
+    """Solves (k point, omega, eta, eta_device) instances of one k-point list in batches:
+    on blocks built on the GPU for force-constant models, else on k points resolved on
+    the host (a k point whose resolution raised is None: each of its attempts counts as
+    an "exception", as in negf.py:463/612)."""
+
+    def __init__(self, device, lead_left, lead_right, kpoints, *, eta_device, precision, **kw):
+        self.precision = precision
+        self.kpoints = list(kpoints)
+        self.sets = kspace_sets(device, lead_left, lead_right, self.kpoints, precision=precision,
+                                **kw)
+        self.kps = None
+        if self.sets is None:
+            self.kps = _resolve_each(device, lead_left, lead_right, self.kpoints,
+                                     eta_device=eta_device, precision=precision, **kw)
+
+    def live(self, k) -> bool:
+        return self.sets is not None or self.kps[k] is not None
+
+    def solve(self, k_idx, omega, eta, eta_dev):
+        k_idx = np.asarray(k_idx, dtype=np.int64)
+        if k_idx.size == 0:
+            return np.zeros(0), np.zeros(0, dtype=np.int32)
+        if self.sets is not None:
+            res = _engine.TransmissionSolver(self.sets, precision=self.precision).run(
+                k_idx.astype(np.int32), omega, eta, eta_dev)
+            return res.t.cpu().numpy(), res.status.cpu().numpy()
+        uniq, local = np.unique(k_idx, return_inverse=True)
+        return solve_points([self.kps[k] for k in uniq], local, omega, eta, eta_dev,
+                            precision=self.precision)
+
+
+def _t_max(lead_left, kpoints, max_channel_factor):
+    if max_channel_factor is None:
+        return [None] * len(kpoints)
+    return [float(max_channel_factor) * float(_resolve_lead_blocks(lead_left, k).d00.shape[0])
+            for k in kpoints]
+
+
+def _adaptive_batch(omegas, solver, t_max, ladders, *, eta_device, global_eta,
+                    min_success_fraction, nonnegative_tolerance, collect_rejected):
+    """The per-k (negf.py:400-543) or global (:546-688) eta ladder for every omega of
+    ``omegas`` at once: rung r solves all pending (omega, k) instances of all omegas in
+    one batch.  ``ladders[i]`` is the eta tuple of omega i.  Returns, per omega, either
+    ("ok", mean T, info) or ("error", message) with the reference's messages."""
+    nw = len(omegas)
+    nk = len(solver.kpoints)
+    kk = np.arange(nk)
+    outcome = [[None] * nk for _ in range(nw)]     # per-k: accepted value
+    used_eta = [[None] * nk for _ in range(nw)]
+    rejected = [[[] for _ in range(nk)] for _ in range(nw)]   # per-k: by k, then eta
+    pending = [list(kk) for _ in range(nw)]
+    done = [False] * nw                             # global: eta accepted
+    g_rejected = [[] for _ in range(nw)]
+    g_result = [None] * nw
+    max_rungs = max((len(lad) for lad in ladders), default=0)
+    for r in range(max_rungs):
+        inst_w, inst_k, inst_eta = [], [], []
+        for i in range(nw):
+            if r >= len(ladders[i]) or (global_eta and done[i]):
+                continue
+            ks = pending[i] if not global_eta else list(kk)
+            for k in ks:
+                inst_w.append(i)
+                inst_k.append(k)
+                inst_eta.append(float(ladders[i][r]))
+        if not inst_w:
+            continue
+        inst_w = np.asarray(inst_w)
+        inst_k = np.asarray(inst_k)
+        inst_eta = np.asarray(inst_eta)
+        inst_ed = inst_eta if eta_device is None else np.full(inst_eta.size, float(eta_device))
+        live = np.array([solver.live(k) for k in inst_k]) & (inst_ed >= 0.0)
+        t = np.full(inst_w.size, np.nan)
+        st = np.zeros(inst_w.size, dtype=np.int32)
+        sel = np.nonzero(live)[0]
+        if sel.size:
+            t[sel], st[sel] = solver.solve(inst_k[sel], np.asarray(omegas, float)[inst_w[sel]],
+                                           inst_eta[sel], inst_ed[sel])
+        outcomes = [_classify(st[j], t[j], nonnegative_tolerance, t_max[inst_k[j]])
+                    if live[j] else ("exception", None) for j in range(inst_w.size)]
+        if not global_eta:
+            for j, (reason, val) in enumerate(outcomes):
+                i, k = int(inst_w[j]), int(inst_k[j])
+                if reason == "ok":
+                    outcome[i][k] = val
+                    used_eta[i][k] = float(inst_eta[j])
+                else:
+                    if collect_rejected:
+                        rejected[i][k].append({"omega": float(omegas[i]),
+                                               "kpar": _kpar_tuple(solver.kpoints[k]),
+                                               "eta": float(inst_eta[j]),
+                                               "t": None if val is None else float(val),
+                                               "reason": reason})
+            for i in range(nw):
+                pending[i] = [k for k in pending[i] if outcome[i][k] is None]
+        else:
+            by_w: dict = {}
+            for j, oc in enumerate(outcomes):
+                by_w.setdefault(int(inst_w[j]), []).append((int(inst_k[j]), oc))
+            for i, lst in by_w.items():
+                eta = float(ladders[i][r])
+                vals = [v for _, (reason, v) in lst if reason == "ok"]
+                if collect_rejected:
+                    g_rejected[i].extend({"omega": float(omegas[i]),
+                                          "kpar": _kpar_tuple(solver.kpoints[k]), "eta": eta,
+                                          "t": None if v is None else float(v), "reason": reason}
+                                         for k, (reason, v) in lst if reason != "ok")
+                n_success = len(vals)
+                frac = float(n_success / nk)
+                if n_success > 0 and frac >= min_success_fraction:
+                    info = {"n_total": int(nk), "n_success": int(n_success),
+                            "n_failed": int(nk - n_success), "success_fraction": frac,
+                            "eta_selected": eta,
+                            "eta_histogram": {float(e): (int(n_success) if float(e) == eta else 0)
+                                              for e in ladders[i]}}
+                    if collect_rejected:
+                        info["n_rejected"] = int(len(g_rejected[i]))
+                        info["rejected"] = list(g_rejected[i])
+                    g_result[i] = ("ok", float(np.mean(vals)), info)
+                    done[i] = True
+    results = []
+    for i in range(nw):
+        if global_eta:
+            results.append(g_result[i] if g_result[i] is not None else (
+                "error", "No eta value satisfied global adaptive k-averaging quality threshold "
+                f"(required success_fraction >= {min_success_fraction:.3f})."))
+            continue
+        vals = [v for v in outcome[i] if v is not None]
+        n_success = len(vals)
+        frac = float(n_success / nk)
+        if n_success == 0:
+            results.append(("error", "No k-point converged for adaptive k-averaged transmission."))
+            continue
+        if frac < min_success_fraction:
+            results.append(("error", "Adaptive k-averaged transmission failed success-fraction "
+                            f"threshold (got {frac:.3f}, required >= {min_success_fraction:.3f})."))
+            continue
+        hist = {float(e): int(sum(1 for u in used_eta[i] if u == float(e))) for e in ladders[i]}
+        info = {"n_total": int(nk), "n_success": int(n_success), "n_failed": int(nk - n_success),
+                "success_fraction": frac, "eta_histogram": hist}
+        if collect_rejected:
+            flat = [rec for rs in rejected[i] for rec in rs]
+            info["n_rejected"] = int(len(flat))
+            info["rejected"] = flat
+        results.append(("ok", float(np.mean(vals)), info))
+    return results
+
+
+def _adaptive_single(omega, device, lead_left, lead_right, kpoints, eta_values, eta_device,
+                     min_success_fraction, device_to_lead_left, device_to_lead_right,
+                     contact_left_indices, contact_right_indices, surface_gf_method, omega_scale,
+                     nonnegative_tolerance, max_channel_factor, collect_rejected, precision,
+                     global_eta):
+    _validate_adaptive(kpoints, eta_values, min_success_fraction, nonnegative_tolerance,
+                       max_channel_factor)
+    canonical_method(surface_gf_method)
+    kpoints = list(kpoints)
+    prec = _prec(precision)
+    t_max = _t_max(lead_left, kpoints, max_channel_factor)
+    solver = _KSolver(device, lead_left, lead_right, kpoints, eta_device=eta_device,
+                      precision=prec, device_to_lead_left=device_to_lead_left,
+                      device_to_lead_right=device_to_lead_right,
+                      contact_left_indices=contact_left_indices,
+                      contact_right_indices=contact_right_indices, omega_scale=omega_scale)
+    res = _adaptive_batch([float(omega)], solver, t_max, [tuple(eta_values)],
+                          eta_device=eta_device, global_eta=global_eta,
+                          min_success_fraction=min_success_fraction,
+                          nonnegative_tolerance=nonnegative_tolerance,
+                          collect_rejected=collect_rejected)[0]
+    if res[0] == "error":
+        raise RuntimeError(res[1])
+    return res[1], res[2]
 
 
 def transmission_kavg_adaptive(omega: float, device: DeviceLike, lead_left: LeadLike,
@@ -492,62 +649,11 @@ def transmission_kavg_adaptive(omega: float, device: DeviceLike, lead_left: Lead
                                ) -> tuple[float, dict[str, object]]:
     """Per-k eta ladder (negf.py:400-543).  Each rung is one batch over the k points
     that have not yet succeeded; outcomes, filters and statistics follow the reference."""
-    _validate_adaptive(kpoints, eta_values, min_success_fraction, nonnegative_tolerance,
-                       max_channel_factor)
-    canonical_method(surface_gf_method)
-    prec = _prec(precision)
-    kpoints = list(kpoints)
-    nk = len(kpoints)
-    t_max = [None] * nk
-    if max_channel_factor is not None:
-        t_max = [float(max_channel_factor) * float(_resolve_lead_blocks(lead_left, k).d00.shape[0])
-                 for k in kpoints]
-    kps = _resolve_each(device, lead_left, lead_right, kpoints, eta_device=eta_device,
-                        device_to_lead_left=device_to_lead_left,
-                        device_to_lead_right=device_to_lead_right,
-                        contact_left_indices=contact_left_indices,
-                        contact_right_indices=contact_right_indices, omega_scale=omega_scale,
-                        precision=prec)
-    outcome: list = [None] * nk      # accepted value per k
-    used_eta: list = [None] * nk
-    rejected_by_k: list[list] = [[] for _ in range(nk)]
-    pending = list(range(nk))
-    for eta in eta_values:
-        if not pending:
-            break
-        codes = _attempt(omega, kps, pending, eta, eta_device, prec, nonnegative_tolerance, t_max)
-        still = []
-        for k in pending:
-            reason, val = codes[k]
-            if reason == "ok":
-                outcome[k] = val
-                used_eta[k] = float(eta)
-            else:
-                if collect_rejected:
-                    rejected_by_k[k].append({"omega": float(omega), "kpar": _kpar_tuple(kpoints[k]),
-                                             "eta": float(eta),
-                                             "t": None if val is None else float(val),
-                                             "reason": reason})
-                still.append(k)
-        pending = still
-    vals = [v for v in outcome if v is not None]
-    n_success = len(vals)
-    success_fraction = float(n_success / nk)
-    if n_success == 0:
-        raise RuntimeError("No k-point converged for adaptive k-averaged transmission.")
-    if success_fraction < min_success_fraction:
-        raise RuntimeError(
-            "Adaptive k-averaged transmission failed success-fraction threshold "
-            f"(got {success_fraction:.3f}, required >= {min_success_fraction:.3f}).")
-    hist = {float(e): int(sum(1 for u in used_eta if u == float(e))) for e in eta_values}
-    info: dict[str, object] = {"n_total": int(nk), "n_success": int(n_success),
-                               "n_failed": int(nk - n_success),
-                               "success_fraction": success_fraction, "eta_histogram": hist}
-    if collect_rejected:
-        rejected = [r for rs in rejected_by_k for r in rs]
-        info["n_rejected"] = int(len(rejected))
-        info["rejected"] = rejected
-    return float(np.mean(vals)), info
+    return _adaptive_single(omega, device, lead_left, lead_right, kpoints, eta_values,
+                            eta_device, min_success_fraction, device_to_lead_left,
+                            device_to_lead_right, contact_left_indices, contact_right_indices,
+                            surface_gf_method, omega_scale, nonnegative_tolerance,
+                            max_channel_factor, collect_rejected, precision, False)
 
 
 def transmission_kavg_adaptive_global_eta(
@@ -562,45 +668,74 @@ def transmission_kavg_adaptive_global_eta(
         collect_rejected: bool = False, *, precision: str | None = None
 ) -> tuple[float, dict[str, object]]:
     """One eta per omega for all k points (negf.py:546-688); each rung is one batch."""
-    _validate_adaptive(kpoints, eta_values, min_success_fraction, nonnegative_tolerance,
-                       max_channel_factor)
+    return _adaptive_single(omega, device, lead_left, lead_right, kpoints, eta_values,
+                            eta_device, min_success_fraction, device_to_lead_left,
+                            device_to_lead_right, contact_left_indices, contact_right_indices,
+                            surface_gf_method, omega_scale, nonnegative_tolerance,
+                            max_channel_factor, collect_rejected, precision, True)
+
+
+def transmission_kavg_adaptive_spectrum(
+        omegas, device: DeviceLike, lead_left: LeadLike, lead_right: LeadLike,
+        kpoints, eta_values=(1e-8, 1e-7, 1e-6, 1e-5), *, eta_scheme: str = "adaptive",
+        eta_fixed: float = 1e-4, eta_device: float | None = None,
+        min_success_fraction: float = 0.0, device_to_lead_left: DeviceToLeadLike | None = None,
+        device_to_lead_right: DeviceToLeadLike | None = None,
+        contact_left_indices: ContactIndices = None, contact_right_indices: ContactIndices = None,
+        surface_gf_method: str = "sancho_rubio", omega_scale: float | None = None,
+        nonnegative_tolerance: float = 0.0, max_channel_factor: float | None = None,
+        collect_rejected: bool = False, precision: str | None = None):
+    """k-averaged adaptive-eta transmission of a whole spectrum, all omegas at once.
+
+    The batched form of the reference's per-omega spectrum driver
+    (workflows/ifc_bulk.py:951-1029, :1059-1288): ``eta_scheme`` "adaptive" applies
+    ``transmission_kavg_adaptive`` to every omega, "adaptive_global" and "fixed" (the
+    single eta ``eta_fixed``) ``transmission_kavg_adaptive_global_eta``.  ``kpoints`` is
+    one k-point list for all omegas; ``eta_values`` one ladder or one ladder per omega.
+    Every eta rung is one batched solve over all pending (omega, k) instances.
+
+    Returns (values, records): values[i] is the averaged T of omega i (0 for omega <= 0,
+    NaN where the omega failed) and records[i] is {"info": the per-omega statistics
+    dict} or {"error": the reference's exception message} (None for omega <= 0).
+    """
+    w = np.asarray(list(omegas), dtype=float)
+    scheme = str(eta_scheme).lower().replace("-", "_")
+    if scheme not in ("adaptive", "adaptive_global", "fixed"):
+        raise ValueError("eta_scheme must be 'adaptive', 'adaptive_global' or 'fixed'.")
+    if scheme == "fixed":
+        ladders = [(float(eta_fixed),)] * w.size
+    elif len(eta_values) > 0 and np.ndim(eta_values[0]) > 0:
+        if len(eta_values) != w.size:
+            raise ValueError("per-omega eta_values need one ladder per omega.")
+        ladders = [tuple(float(e) for e in lad) for lad in eta_values]
+    else:
+        ladders = [tuple(float(e) for e in eta_values)] * w.size
+    kpoints = list(kpoints)
+    for lad in set(ladders):
+        _validate_adaptive(kpoints, lad, min_success_fraction, nonnegative_tolerance,
+                           max_channel_factor)
     canonical_method(surface_gf_method)
     prec = _prec(precision)
-    kpoints = list(kpoints)
-    nk = len(kpoints)
-    t_max = [None] * nk
-    if max_channel_factor is not None:
-        t_max = [float(max_channel_factor) * float(_resolve_lead_blocks(lead_left, k).d00.shape[0])
-                 for k in kpoints]
-    kps = _resolve_each(device, lead_left, lead_right, kpoints, eta_device=eta_device,
-                        device_to_lead_left=device_to_lead_left,
-                        device_to_lead_right=device_to_lead_right,
-                        contact_left_indices=contact_left_indices,
-                        contact_right_indices=contact_right_indices, omega_scale=omega_scale,
-                        precision=prec)
-    rejected_all: list = []
-    for eta in eta_values:
-        codes = _attempt(omega, kps, list(range(nk)), eta, eta_device, prec,
-                         nonnegative_tolerance, t_max)
-        outcomes = [codes[k] for k in range(nk)]
-        vals = [v for r, v in outcomes if r == "ok"]
-        if collect_rejected:
-            rejected_all.extend({"omega": float(omega), "kpar": _kpar_tuple(kpoints[k]),
-                                 "eta": float(eta), "t": None if v is None else float(v),
-                                 "reason": r} for k, (r, v) in enumerate(outcomes) if r != "ok")
-        n_success = len(vals)
-        success_fraction = float(n_success / nk)
-        if n_success > 0 and success_fraction >= min_success_fraction:
-            info: dict[str, object] = {
-                "n_total": int(nk), "n_success": int(n_success),
-                "n_failed": int(nk - n_success), "success_fraction": success_fraction,
-                "eta_selected": float(eta),
-                "eta_histogram": {float(e): (int(n_success) if float(e) == float(eta) else 0)
-                                  for e in eta_values}}
-            if collect_rejected:
-                info["n_rejected"] = int(len(rejected_all))
-                info["rejected"] = rejected_all
-            return float(np.mean(vals)), info
-    raise RuntimeError(
-        "No eta value satisfied global adaptive k-averaging quality threshold "
-        f"(required success_fraction >= {min_success_fraction:.3f}).")
+    t_max = _t_max(lead_left, kpoints, max_channel_factor)
+    solver = _KSolver(device, lead_left, lead_right, kpoints, eta_device=eta_device,
+                      precision=prec, device_to_lead_left=device_to_lead_left,
+                      device_to_lead_right=device_to_lead_right,
+                      contact_left_indices=contact_left_indices,
+                      contact_right_indices=contact_right_indices, omega_scale=omega_scale)
+    pos = np.nonzero(w > 0.0)[0]
+    res = _adaptive_batch(w[pos], solver, t_max, [ladders[i] for i in pos],
+                          eta_device=eta_device, global_eta=(scheme != "adaptive"),
+                          min_success_fraction=min_success_fraction,
+                          nonnegative_tolerance=nonnegative_tolerance,
+                          collect_rejected=collect_rejected)
+    values = np.zeros(w.size)
+    records: list = [None] * w.size
+    for j, i in enumerate(pos):
+        r = res[j]
+        if r[0] == "ok":
+            values[i] = r[1]
+            records[i] = {"info": r[2]}
+        else:
+            values[i] = np.nan
+            records[i] = {"error": r[1]}
+    return values, records

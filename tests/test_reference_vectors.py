@@ -326,3 +326,31 @@ def test_oracle_material_kavg_matches_reference():
         ts = [o.transmission_sparse(w, list(ons[i]), list(cps[i]), tuple(lead[i]), tuple(lead[i]),
                                     eta=1e-6) for i in range(len(MK_K))]
         assert abs(np.mean(ts) - t_ref) <= TOL * max(abs(t_ref), 1e-3)
+
+
+@pytest.mark.parametrize("scheme", ["adaptive", "adaptive_global"])
+def test_adaptive_spectrum_equals_per_omega_calls(oracle_backed, scheme):
+    """The batched spectrum driver gives, omega by omega, what the per-omega functions
+    give (values, statistics, rejection lists, and the exceptions as error records)."""
+    dev, lead, kpts = _sq_problem()
+    ws = [0.9, 1.1, 0.0, 1.6, -0.3, 0.4]
+    kw = dict(eta_values=(1e-8, 1e-7), nonnegative_tolerance=1e-6, max_channel_factor=0.5,
+              min_success_fraction=0.2, collect_rejected=True)
+    vals, recs = oracle_backed.transmission_kavg_adaptive_spectrum(ws, dev, lead, lead, kpts,
+                                                                   eta_scheme=scheme, **kw)
+    single = (oracle_backed.transmission_kavg_adaptive if scheme == "adaptive"
+              else oracle_backed.transmission_kavg_adaptive_global_eta)
+    n_err = 0
+    for w, v, rec in zip(ws, vals, recs):
+        if w <= 0.0:
+            assert v == 0.0 and rec is None
+            continue
+        try:
+            t, info = single(w, dev, lead, lead, kpoints=kpts, **kw)
+        except RuntimeError as exc:
+            n_err += 1
+            assert np.isnan(v) and rec == {"error": str(exc)}
+            continue
+        assert v == t
+        assert rec["info"] == info
+    assert n_err >= 1  # the channel limit makes at least one omega fail
