@@ -217,6 +217,22 @@ int negf_thermal_conductance(void* stream, int n_w, const double* omega_rad_s, c
                              int n_t, const double* temps, double* out);
 
 /*
+ * Heat-current cumulants of coherent phonon transport for n_pairs temperature
+ * pairs (FP64): out[p] = (C1 [J/s], C2 [J^2/s]) with
+ *   C1 = (1/2pi) int <T> (A_L - A_R) dw,
+ *   C2 = (1/2pi) int <T>(B_L + B_R) + (<T> + <T^2>)(A_L - A_R)^2 dw,
+ * A = k_B T x/expm1(x), B = (k_B T)^2 x^2 e^x/expm1(x)^2, x = hbar w/(k_B T),
+ * trapezoid rule on the given grid (w in rad/s).  t2_mean = null gives the
+ * single-spectrum form T(B_L + B_R) + T(1 + T)(A_L - A_R)^2.  A pair with a
+ * temperature <= 0 gets NaN (the Python layer raises first).  Replaces
+ * heat_current_cumulants_from_spectrum / _from_k_moments,
+ * src/negfpy/core/fcs.py:89-156.
+ */
+int negf_fcs_cumulants(void* stream, int n_w, const double* omega_rad_s, const double* t_mean,
+                       const double* t2_mean, int n_pairs, const double* temps_left,
+                       const double* temps_right, double* out);
+
+/*
  * k-space blocks of a force-constant model for n_k k points at once (FP64):
  * Phi_ab(k) = sum_t Phi_ab,t ph_ab[k][t] (+ pinning on the diagonal of Phi_00),
  * D = M_l^-1/2 Phi M_r^-1/2.  Replaces the per-k host callbacks of

@@ -83,6 +83,7 @@ _SIGS = {
     "negf_thermal_conductance": ([_vp, _c_int, _vp, _vp, _c_int, _vp, _vp], _c_int),
     "negf_heat_current": ([_vp, _c_int, _vp, _vp, _c_dbl, _c_dbl, _vp], _c_int),
     "negf_conductance_weights": ([_vp, _c_int, _vp, _c_int, _vp, _vp], _c_int),
+    "negf_fcs_cumulants": ([_vp, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp], _c_int),
     "negf_kspace_blocks": (
         [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_dbl,
          _vp, _vp, _vp, _vp, _vp], _c_int),

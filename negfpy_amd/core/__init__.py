@@ -2,6 +2,14 @@
 (``src/negfpy/core/__init__.py:1-78``, transport part)."""
 
 from .adjoint import thermal_conductance_torch, transmission_torch
+from .fcs import (
+    HeatCurrentCumulants,
+    HeatCurrentUncertainty,
+    heat_current_cumulants_from_k_moments,
+    heat_current_cumulants_from_spectrum,
+    heat_current_cumulants_scan,
+    heat_current_uncertainty,
+)
 from .dos import (
     lead_surface_dos,
     lead_surface_dos_kavg,
@@ -16,6 +24,7 @@ from .negf import (
     transmission_kavg_adaptive,
     transmission_kavg_adaptive_global_eta,
     transmission_kavg_adaptive_spectrum,
+    transmission_k_moments_spectrum,
 )
 from .surface_gf import (
     get_default_precision,
@@ -39,6 +48,13 @@ from .thermal_current import (
 from .types import Device1D, DeviceKSpace, LeadBlocks, LeadKSpace
 
 __all__ = [
+    "HeatCurrentCumulants",
+    "HeatCurrentUncertainty",
+    "heat_current_cumulants_from_k_moments",
+    "heat_current_cumulants_from_spectrum",
+    "heat_current_cumulants_scan",
+    "heat_current_uncertainty",
+    "transmission_k_moments_spectrum",
     "transmission_torch",
     "thermal_conductance_torch",
     "Device1D",

@@ -577,7 +577,7 @@ def _adaptive_batch(omegas, solver, t_max, ladders, *, eta_device, global_eta,
                     if collect_rejected:
                         info["n_rejected"] = int(len(g_rejected[i]))
                         info["rejected"] = list(g_rejected[i])
-                    g_result[i] = ("ok", float(np.mean(vals)), info)
+                    g_result[i] = ("ok", float(np.mean(vals)), info, np.asarray(vals))
                     done[i] = True
     results = []
     for i in range(nw):
@@ -739,3 +739,61 @@ def transmission_kavg_adaptive_spectrum(
             values[i] = np.nan
             records[i] = {"error": r[1]}
     return values, records
+
+
+def transmission_k_moments_spectrum(
+        omegas, device: DeviceLike, lead_left: LeadLike, lead_right: LeadLike, kpoints,
+        eta_values=(1e-8, 1e-7, 1e-6, 1e-5), *, eta_scheme: str = "adaptive_global",
+        eta_fixed: float = 1e-4, eta_device: float | None = None,
+        min_success_fraction: float = 0.0, device_to_lead_left: DeviceToLeadLike | None = None,
+        device_to_lead_right: DeviceToLeadLike | None = None,
+        surface_gf_method: str = "sancho_rubio", omega_scale: float | None = None,
+        nonnegative_tolerance: float = 0.0, precision: str | None = None) -> dict:
+    """k-moments <T>(omega) and <T^2>(omega) over the accepted k points of one eta per
+    omega, for a whole spectrum in batched solves (one per eta rung).
+
+    The batched form of the reference's FCS moment pass (workflows/ifc_bulk.py:1690-1785):
+    for each omega > 0 the first eta (of ``eta_values``, or ``eta_fixed`` for
+    ``eta_scheme="fixed"``) whose accepted k points (finite, T >= -tolerance, clamped
+    at 0) reach ``min_success_fraction`` gives t_mean = mean(T), t2_mean = mean(T^2).
+    Returns the reference's keys: t_mean, t2_mean (0 for omega <= 0, NaN where no eta
+    passed), eta_selected, n_k_success, omega_failures.
+    """
+    w = np.asarray(list(omegas), dtype=float)
+    scheme = str(eta_scheme).lower().replace("-", "_")
+    if scheme not in ("adaptive_global", "fixed"):
+        scheme = "adaptive_global"
+    ladder = (float(eta_fixed),) if scheme == "fixed" else tuple(float(e) for e in eta_values)
+    kpoints = list(kpoints)
+    _validate_adaptive(kpoints, ladder, min_success_fraction, nonnegative_tolerance, None)
+    canonical_method(surface_gf_method)
+    prec = _prec(precision)
+    solver = _KSolver(device, lead_left, lead_right, kpoints, eta_device=eta_device,
+                      precision=prec, device_to_lead_left=device_to_lead_left,
+                      device_to_lead_right=device_to_lead_right, contact_left_indices=None,
+                      contact_right_indices=None, omega_scale=omega_scale)
+    n = w.size
+    t_mean = np.full(n, np.nan)
+    t2_mean = np.full(n, np.nan)
+    eta_sel = np.full(n, np.nan)
+    n_ok = np.zeros(n, dtype=int)
+    failures = []
+    pos = np.nonzero(w > 0.0)[0]
+    t_mean[w <= 0.0] = 0.0
+    t2_mean[w <= 0.0] = 0.0
+    res = _adaptive_batch(w[pos], solver, [None] * len(kpoints), [ladder] * pos.size,
+                          eta_device=eta_device, global_eta=True,
+                          min_success_fraction=min_success_fraction,
+                          nonnegative_tolerance=nonnegative_tolerance, collect_rejected=False)
+    for j, i in enumerate(pos):
+        r = res[j]
+        if r[0] != "ok":
+            failures.append({"omega": float(w[i]), "reason": "no_eta_passed_threshold"})
+            continue
+        acc = r[3]
+        t_mean[i] = float(np.mean(acc))
+        t2_mean[i] = float(np.mean(acc * acc))
+        eta_sel[i] = float(r[2]["eta_selected"])
+        n_ok[i] = int(acc.size)
+    return {"t_mean": t_mean, "t2_mean": t2_mean, "eta_selected": eta_sel,
+            "n_k_success": n_ok, "omega_failures": failures}

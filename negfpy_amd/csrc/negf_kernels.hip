@@ -1380,6 +1380,72 @@ __global__ void __launch_bounds__(256) k_conductance(int n_w, const double* __re
   if (tid == 0) out[it] = temp > 0.0 ? (1.0 / (2.0 * M_PI)) * s : 0.0;
 }
 
+// Full counting statistics (core/fcs.py:47-156): for each temperature pair,
+// C1 = (1/2pi) int <T> (A_L - A_R) dw and
+// C2 = (1/2pi) int <T> (B_L + B_R) + (<T> + <T^2>) (A_L - A_R)^2 dw
+// (single-spectrum form T (B_L + B_R) + T (1 + T)(A_L - A_R)^2 when t2 is null),
+// A = k_B T phi1(x), B = (k_B T)^2 phi2(x), x = hbar w / (k_B T); phi1, phi2 with
+// the reference's series below |x| = 1e-6, its expm1 forms up to 700 (including
+// their overflow to NaN) and 0 beyond; each term rounded separately, as numpy
+// does, so equal temperatures give exactly C1 = 0.
+__device__ __forceinline__ double fcs_phi1(double x) {
+#pragma clang fp contract(off)
+  const double ax = fabs(x);
+  if (ax < 1e-6) return 1.0 - 0.5 * x + (x * x) / 12.0 - pow(x, 4.0) / 720.0;
+  if (x < 700.0) return x / expm1(x);
+  return 0.0;
+}
+__device__ __forceinline__ double fcs_phi2(double x) {
+#pragma clang fp contract(off)
+  const double ax = fabs(x);
+  if (ax < 1e-6) return 1.0 - (x * x) / 12.0 + pow(x, 4.0) / 240.0;
+  if (x < 700.0) {
+    const double em1 = expm1(x);
+    return (x * x) * (em1 + 1.0) / (em1 * em1);
+  }
+  return 0.0;
+}
+__device__ __forceinline__ void fcs_integrand(double w, double t1, const double* t2, int i,
+                                              double tl, double tr, double& f1, double& f2) {
+#pragma clang fp contract(off)
+  const double sl = KB_J_K * tl, sr = KB_J_K * tr;
+  const double xl = (HBAR_J_S * w) / sl, xr = (HBAR_J_S * w) / sr;
+  const double ad = sl * fcs_phi1(xl) - sr * fcs_phi1(xr);
+  const double bsum = (sl * sl) * fcs_phi2(xl) + (sr * sr) * fcs_phi2(xr);
+  f1 = t1 * ad;
+  f2 = t2 ? t1 * bsum + (t1 + t2[i]) * (ad * ad) : t1 * bsum + t1 * (1.0 + t1) * (ad * ad);
+}
+
+__global__ void __launch_bounds__(256) k_fcs_cumulants(int n_w, const double* __restrict__ w,
+                                                        const double* __restrict__ t1,
+                                                        const double* __restrict__ t2,
+                                                        const double* __restrict__ temps_l,
+                                                        const double* __restrict__ temps_r,
+                                                        double* __restrict__ out) {
+  __shared__ double sred[4];
+  const int tid = threadIdx.x;
+  const int p = blockIdx.x;
+  const double tl = temps_l[p], tr = temps_r[p];
+  double a1 = 0.0, a2 = 0.0;
+  if (tl > 0.0 && tr > 0.0) {
+    for (int i = tid; i + 1 < n_w; i += 256) {
+      double f1a, f2a, f1b, f2b;
+      fcs_integrand(w[i], t1[i], t2, i, tl, tr, f1a, f2a);
+      fcs_integrand(w[i + 1], t1[i + 1], t2, i + 1, tl, tr, f1b, f2b);
+      const double h = w[i + 1] - w[i];
+      a1 += h * (f1b + f1a) / 2.0;
+      a2 += h * (f2b + f2a) / 2.0;
+    }
+  }
+  const double s1 = block_sum<256>(a1, sred, tid);
+  const double s2 = block_sum<256>(a2, sred, tid);
+  if (tid == 0) {
+    const bool ok = tl > 0.0 && tr > 0.0;
+    out[2 * p] = ok ? (1.0 / (2.0 * M_PI)) * s1 : NAN;
+    out[2 * p + 1] = ok ? (1.0 / (2.0 * M_PI)) * s2 : NAN;
+  }
+}
+
 // d kappa(T_t) / d T(w_i): the trapezoid weights of k_conductance times its integrand
 // per unit transmission, so that kappa = weights @ T.
 __global__ void __launch_bounds__(256) k_conductance_weights(int n_w, const double* __restrict__ w,
@@ -1773,6 +1839,20 @@ int negf_adjoint_batched(void* stream, int n, int n_layers, int n_inst, const do
             status_in, weight, reinterpret_cast<double*>(workspace), t_out, grad_onsite,
             grad_coupling, status};
   return run_adj(reinterpret_cast<hipStream_t>(stream), nmax, a);
+}
+
+int negf_fcs_cumulants(void* stream, int n_w, const double* omega_rad_s, const double* t_mean,
+                       const double* t2_mean, int n_pairs, const double* temps_left,
+                       const double* temps_right, double* out) {
+  if (n_w < 2 || n_pairs < 0)
+    return fail(NEGF_ERR_ARG, "invalid sizes (n_w=%d pairs=%d)", n_w, n_pairs);
+  if (n_pairs == 0) return NEGF_OK;
+  if (!omega_rad_s || !t_mean || !temps_left || !temps_right || !out)
+    return fail(NEGF_ERR_ARG, "null pointer argument");
+  hipLaunchKernelGGL(k_fcs_cumulants, dim3(n_pairs), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), n_w, omega_rad_s, t_mean, t2_mean,
+                     temps_left, temps_right, out);
+  return hip_check(hipGetLastError(), "k_fcs_cumulants launch");
 }
 
 int negf_kspace_blocks(void* stream, int n, int n_layers, int n_k, int t00, int t01, int t10,

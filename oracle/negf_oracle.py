@@ -509,3 +509,84 @@ def thermal_conductance(omega_qe, tvals, temps):
         ker[mid] = x[mid] ** 2 * y / (1.0 - y) ** 2
         out[i] = trapezoid(KB * tvals * ker, w) / (2.0 * np.pi)
     return out
+
+
+# --------------------------------------------------------------------------
+# full counting statistics (core/fcs.py)
+# --------------------------------------------------------------------------
+
+def _fcs_grid(values, name):
+    """fcs.py:36-44."""
+    arr = np.asarray(list(values), dtype=float)
+    if arr.ndim != 1:
+        raise ValueError(f"{name} must be 1D.")
+    if arr.size < 2:
+        raise ValueError(f"{name} must contain at least two points.")
+    if np.any(np.diff(arr) <= 0.0):
+        raise ValueError(f"{name} must be strictly increasing.")
+    return arr
+
+
+def fcs_phi1(x):
+    """x / expm1(x) with its series below |x| = 1e-6 and 0 from 700, fcs.py:47-56."""
+    x = np.asarray(x, dtype=float)
+    out = np.zeros_like(x)
+    ax = np.abs(x)
+    small = ax < 1e-6
+    mid = (ax >= 1e-6) & (x < 700.0)
+    xs = x[small]
+    out[small] = 1.0 - 0.5 * xs + (xs * xs) / 12.0 - (xs ** 4) / 720.0
+    out[mid] = x[mid] / np.expm1(x[mid])
+    return out
+
+
+def fcs_phi2(x):
+    """x^2 e^x / expm1(x)^2 with its series below |x| = 1e-6 and 0 from 700, fcs.py:59-69."""
+    x = np.asarray(x, dtype=float)
+    out = np.zeros_like(x)
+    ax = np.abs(x)
+    small = ax < 1e-6
+    mid = (ax >= 1e-6) & (x < 700.0)
+    xs = x[small]
+    out[small] = 1.0 - (xs * xs) / 12.0 + (xs ** 4) / 240.0
+    em1 = np.expm1(x[mid])
+    out[mid] = (x[mid] * x[mid]) * (em1 + 1.0) / (em1 * em1)
+    return out
+
+
+def _fcs_ab(omega_rad_s, temp):
+    """A = k_B T phi1(x), B = (k_B T)^2 phi2(x), x = hbar w / k_B T, fcs.py:72-86."""
+    if temp <= 0.0:
+        raise ValueError("Temperature must be positive in Kelvin.")
+    x = (HBAR * omega_rad_s) / (KB * temp)
+    s = KB * temp
+    return s * fcs_phi1(x), (s * s) * fcs_phi2(x)
+
+
+def fcs_cumulants(omega_rad_s, t_mean, t2_mean, temp_left, temp_right):
+    """(C1, C2): C1 = (1/2pi) int <T> (A_L - A_R), C2 = (1/2pi) int <T>(B_L + B_R)
+    + (<T> + <T^2>)(A_L - A_R)^2 (fcs.py:125-156); with t2_mean=None the single-spectrum
+    form T(B_L + B_R) + T(1 + T)(A_L - A_R)^2 (fcs.py:89-122)."""
+    omega = _fcs_grid(omega_rad_s, "omega_rad_s")
+    t1 = np.asarray(list(t_mean), dtype=float)
+    if t2_mean is None:
+        if t1.shape != omega.shape:
+            raise ValueError("transmission_vals must have the same length as omega_rad_s.")
+        if np.any(t1 < 0.0):
+            raise ValueError("transmission_vals must be non-negative.")
+    else:
+        t2 = np.asarray(list(t2_mean), dtype=float)
+        if t1.shape != omega.shape or t2.shape != omega.shape:
+            raise ValueError("t_mean_vs_omega and t2_mean_vs_omega must match omega_rad_s shape.")
+        if np.any(t1 < 0.0) or np.any(t2 < 0.0):
+            raise ValueError("Transmission moments must be non-negative.")
+    a_l, b_l = _fcs_ab(omega, temp_left)
+    a_r, b_r = _fcs_ab(omega, temp_right)
+    ad = a_l - a_r
+    pref = 1.0 / (2.0 * np.pi)
+    c1 = pref * trapezoid(t1 * ad, omega)
+    if t2_mean is None:
+        c2 = pref * trapezoid(t1 * (b_l + b_r) + t1 * (1.0 + t1) * (ad * ad), omega)
+    else:
+        c2 = pref * trapezoid(t1 * (b_l + b_r) + (t1 + t2) * (ad * ad), omega)
+    return float(c1), float(c2)

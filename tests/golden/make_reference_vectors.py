@@ -15,6 +15,7 @@ of independent solvers fix:
 * ``lead_surface_dos`` on a multi-DOF lead, with and without
   ``normalize_per_mode``;
 * ``device_green_function`` (``negf.py:253-303``);
+* the full-counting-statistics cumulants and uncertainty (``core/fcs.py``);
 * ``transmission_kavg_adaptive`` / ``..._global_eta`` decisions and statistics;
 * the k-space builders used by the reference's transport tests, and the general
   force-constant builder (``material_kspace``) with its k-averaged transmission;
@@ -143,6 +144,30 @@ def main(ref_src: str) -> None:
     out["hc_t"] = ht
     out["hc_j"] = np.array([rc.heat_current_from_spectrum(hw, ht, 310.0, 290.0),
                             rc.heat_current_from_spectrum(hw, ht, 5.0, 1.0)])
+
+    # --- full counting statistics (core/fcs.py) -------------------------------
+    rngf = np.random.default_rng(31)
+    fw = np.sort(rngf.uniform(1e11, 9e13, 200))
+    ft1 = rngf.uniform(0.0, 3.0, 200)
+    ft2 = ft1 * ft1 * rngf.uniform(0.8, 1.0, 200)
+    pairs = np.array([[300.0, 290.0], [10.0, 5.0], [1000.0, 1.0], [2.0, 1.0], [77.0, 77.0]])
+    out["fcs_omega"] = fw
+    out["fcs_t1"] = ft1
+    out["fcs_t2"] = ft2
+    out["fcs_pairs"] = pairs
+    spec, mom = [], []
+    for tl, tr in pairs:
+        c = rc.heat_current_cumulants_from_spectrum(fw, ft1, tl, tr)
+        spec.append([c.c1_j_per_s, c.c2_j2_per_s])
+        c = rc.heat_current_cumulants_from_k_moments(fw, ft1, ft2, tl, tr)
+        mom.append([c.c1_j_per_s, c.c2_j2_per_s])
+    out["fcs_spectrum"] = np.array(spec)
+    out["fcs_moments"] = np.array(mom)
+    u = rc.heat_current_uncertainty(rc.HeatCurrentCumulants(c1_j_per_s=mom[0][0],
+                                                            c2_j2_per_s=mom[0][1]),
+                                    measurement_time_s=1e-9)
+    out["fcs_uncertainty"] = np.array([u.measurement_time_s, u.std_current_j_per_s,
+                                       u.variance_energy_j2, u.std_energy_j])
 
     # --- k-space builders ------------------------------------------------------
     sq = rm.SquareLatticeParams(mass=1.3, spring_x=0.9, spring_y=0.6)

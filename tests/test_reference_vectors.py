@@ -354,3 +354,54 @@ def test_adaptive_spectrum_equals_per_omega_calls(oracle_backed, scheme):
         assert v == t
         assert rec["info"] == info
     assert n_err >= 1  # the channel limit makes at least one omega fail
+
+
+def _moments_by_reference_loop(ws, dev, lead, kpts, ladder, min_frac, tol):
+    """The reference's moment loop (workflows/ifc_bulk.py:1723-1776) on oracle T."""
+    from negfpy_amd.core.negf import resolve_k
+
+    out = []
+    for w in ws:
+        if w <= 0.0:
+            out.append((0.0, 0.0, np.nan, 0))
+            continue
+        got = None
+        for eta in ladder:
+            vals = []
+            for k in kpts:
+                kp = resolve_k(dev, lead, lead, k, eta=eta, eta_device=None,
+                               device_to_lead_left=None, device_to_lead_right=None,
+                               contact_left_indices=None, contact_right_indices=None,
+                               omega_scale=None, max_block=48)
+                nb, nbig = kp.onsite.shape[0], kp.onsite.shape[1]
+                t = o.transmission_sparse(w, list(kp.onsite), list(kp.coupling), tuple(kp.lead_l),
+                                          tuple(kp.lead_r), eta=eta, v_left=kp.v_l,
+                                          v_right=kp.v_r, idx_left=kp.pos_l,
+                                          idx_right=(nb - 1) * nbig + kp.pos_r)
+                if not np.isfinite(t) or t < -tol:
+                    continue
+                vals.append(max(t, 0.0))
+            if vals and len(vals) / len(kpts) >= min_frac:
+                got = (float(np.mean(vals)), float(np.mean(np.square(vals))), eta, len(vals))
+                break
+        out.append(got if got is not None else (np.nan, np.nan, np.nan, 0))
+    return out
+
+
+def test_k_moments_spectrum_matches_reference_loop(oracle_backed):
+    dev, lead, kpts = _sq_problem()
+    ws = [0.9, 1.1, 0.0, 1.6, 0.4]
+    res = oracle_backed.transmission_k_moments_spectrum(ws, dev, lead, lead, kpts,
+                                                        eta_values=(1e-8, 1e-7),
+                                                        min_success_fraction=0.7,
+                                                        nonnegative_tolerance=1e-8)
+    ref = _moments_by_reference_loop(ws, dev, lead, kpts, (1e-8, 1e-7), 0.7, 1e-8)
+    for i, (t1, t2, eta, nk) in enumerate(ref):
+        if np.isnan(t1):
+            assert np.isnan(res["t_mean"][i]) and res["n_k_success"][i] == 0
+            continue
+        assert abs(res["t_mean"][i] - t1) <= 1e-13 * max(t1, 1e-3)
+        assert abs(res["t2_mean"][i] - t2) <= 1e-13 * max(t2, 1e-3)
+        assert res["n_k_success"][i] == nk
+        if ws[i] > 0:
+            assert res["eta_selected"][i] == eta
