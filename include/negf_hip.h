@@ -49,6 +49,17 @@ const char* negf_last_error(void);
 int negf_max_block(int precision);
 
 /*
+ * Block-size class (8, 16, 24, 32, 48, 64: workgroup size, rows per lane, tile
+ * size and LDS footprint of every kernel) used for blocks of n DOF in the given
+ * precision.  Any class >= n is valid; nmax = 0 restores the default (the
+ * smallest class >= n).  Set by the autotuner (negfpy_amd.core.autotune); the
+ * table is process-wide and thread-safe.  No counterpart in the reference, whose
+ * dense/sparse solves have no launch configuration.
+ */
+int negf_set_block_class(int precision, int n, int nmax);
+int negf_block_class(int precision, int n);
+
+/*
  * Surface Green's function of semi-infinite leads by Lopez-Sancho decimation.
  * Replaces surface_gf_sancho_rubio, src/negfpy/core/surface_gf.py:18-81
  * (and the per-k loop of lead_surface_dos*, src/negfpy/core/dos.py:31-251).

@@ -1,0 +1,111 @@
+"""Autotuner for the launch layout of each block size.
+
+Every kernel exists in block-size classes (8, 16, 24, 32, 48, 64 DOF): the class
+fixes the workgroup size, the rows each lane owns in the inversion, the product
+tile per thread and the LDS per instance, and so the occupancy.  A block of n DOF
+runs in any class >= n; which one is fastest depends on how the inversion's
+per-step latency (rows per lane) trades against resident instances per CU.
+``tune_block_class`` times the candidate classes on a synthetic batch of n-DOF
+instances (HIP events on the solve's stream, best of ``repeats``), records the
+fastest in the library's process-wide class table (``negf_set_block_class``) and
+returns the timings.  Results are cached per (precision, n) for the process.
+"""
+
+from __future__ import annotations
+
+import threading
+
+import numpy as np
+import torch
+
+from negfpy_amd import _lib
+
+from . import _engine
+from .surface_gf import pack_lead
+from .types import LeadBlocks
+
+__all__ = ["CLASSES", "candidate_classes", "tune_block_class", "block_class", "reset_block_class"]
+
+CLASSES = (8, 16, 24, 32, 48, 64)
+_lock = threading.Lock()
+_results: dict = {}
+
+
+def candidate_classes(n: int, precision: str = "fp64") -> list[int]:
+    limit = 48 if precision == "fp64" else 64
+    return [c for c in CLASSES if n <= c <= limit]
+
+
+def block_class(n: int, precision: str = "fp64") -> int:
+    """Class currently used for n-DOF blocks."""
+    return int(_lib.load().negf_block_class(_engine.precision_code(precision), int(n)))
+
+
+def reset_block_class(n: int, precision: str = "fp64") -> None:
+    _lib.check(_lib.load().negf_set_block_class(_engine.precision_code(precision), int(n), 0))
+    with _lock:
+        _results.pop((precision, int(n)), None)
+
+
+def _synthetic_sets(n: int, n_layers: int):
+    from negfpy_amd.workloads import omega_max, random_lead
+
+    d00, d01 = random_lead(1000 + n, n)
+    rng = np.random.default_rng(n)
+    onsite = np.stack([d00 + np.diag(rng.uniform(-0.05, 0.05, n)) for _ in range(n_layers)])
+    coupling = np.stack([d01] * max(n_layers - 1, 1))
+    lead = pack_lead(d00, d01)[None]
+    sets = _engine.TransportSets.from_arrays(
+        lead, lead, lead[:, 2], lead[:, 1], np.arange(n), np.arange(n), onsite[None],
+        coupling[None] if n_layers > 1 else None, True)
+    return sets, omega_max(LeadBlocks(d00=d00, d01=d01))
+
+
+def tune_block_class(n: int, precision: str = "fp64", *, n_layers: int = 8, n_inst: int = 2048,
+                     repeats: int = 3, candidates=None, apply: bool = True,
+                     refresh: bool = False) -> dict:
+    """Time each candidate class for n-DOF blocks and (if ``apply``) select the fastest.
+
+    Returns {"best": class, "timings_ms": {class: ms per batch}, "n_inst": ..., "n_layers": ...}.
+    """
+    key = (precision, int(n))
+    with _lock:
+        if not refresh and key in _results:
+            return _results[key]
+    lib = _lib.load()
+    code = _engine.precision_code(precision)
+    cands = list(candidates) if candidates is not None else candidate_classes(n, precision)
+    if not cands:
+        raise ValueError(f"no block-size class holds {n}-DOF blocks in {precision}")
+    sets, wmax = _synthetic_sets(int(n), int(n_layers))
+    dev = sets.onsite.device
+    omega = torch.as_tensor(np.linspace(0.05, 1.05, n_inst) * wmax, device=dev)
+    eta = torch.full((n_inst,), 1e-6, dtype=torch.float64, device=dev)
+    set_of = torch.zeros(n_inst, dtype=torch.int32, device=dev)
+    solver = _engine.TransmissionSolver(sets, precision=precision)
+    out = solver.alloc_result(n_inst)
+    timings = {}
+    previous = int(lib.negf_block_class(code, int(n)))
+    try:
+        for c in cands:
+            _lib.check(lib.negf_set_block_class(code, int(n), int(c)))
+            solver.run(set_of, omega, eta, out=out)
+            best = float("inf")
+            for _ in range(max(int(repeats), 1)):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                solver.run(set_of, omega, eta, out=out)
+                e1.record()
+                e1.synchronize()
+                best = min(best, float(e0.elapsed_time(e1)))
+            timings[int(c)] = best
+    finally:
+        _lib.check(lib.negf_set_block_class(code, int(n), previous))
+    choice = min(timings, key=timings.get)
+    if apply:
+        _lib.check(lib.negf_set_block_class(code, int(n), choice))
+    res = {"best": choice, "timings_ms": timings, "n_inst": int(n_inst), "n_layers": int(n_layers)}
+    with _lock:
+        _results[key] = res
+    return res

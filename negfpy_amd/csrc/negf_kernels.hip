@@ -28,6 +28,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
 #include <mutex>
 
 #include "cmat.hpp"
@@ -1514,7 +1515,12 @@ static int hip_check(hipError_t e, const char* what) {
 }
 
 // smallest configured NMAX >= n for the given element size; 0 if unsupported
-static int pick_nmax(int n, bool fp64) {
+// Block-size class chosen for a block of n DOF: the smallest class >= n unless the
+// autotuner recorded another one for (precision, n).  Entries are read on every
+// launch and written rarely (negf_set_block_class), hence atomics.
+static std::atomic<int> g_class_override[2][65];
+
+static int default_nmax(int n, bool fp64) {
   const int opts[] = {8, 16, 24, 32, 48, 64};
   for (int o : opts) {
     if (n <= o) {
@@ -1523,6 +1529,14 @@ static int pick_nmax(int n, bool fp64) {
     }
   }
   return 0;
+}
+
+static int pick_nmax(int n, bool fp64) {
+  if (n >= 1 && n <= 64) {
+    const int o = g_class_override[fp64 ? 0 : 1][n].load(std::memory_order_relaxed);
+    if (o) return o;
+  }
+  return default_nmax(n, fp64);
 }
 
 #define NEGF_LAUNCH(KERNEL, GRID, NT, LDS, STREAM, ARGS)                                         \
@@ -1692,6 +1706,28 @@ int negf_version(void) { return NEGF_ABI_VERSION; }
 const char* negf_last_error(void) { return g_err; }
 
 int negf_max_block(int precision) { return precision == NEGF_FP64 ? 48 : 64; }
+
+int negf_set_block_class(int precision, int n, int nmax) {
+  if (precision != NEGF_FP64 && precision != NEGF_MIXED)
+    return fail(NEGF_ERR_ARG, "unknown precision %d", precision);
+  const bool fp64 = precision == NEGF_FP64;
+  const int limit = fp64 ? 48 : 64;
+  if (n < 1 || n > limit) return fail(NEGF_ERR_ARG, "block size %d outside 1..%d", n, limit);
+  if (nmax != 0) {
+    const bool known = nmax == 8 || nmax == 16 || nmax == 24 || nmax == 32 || nmax == 48 ||
+                       nmax == 64;
+    if (!known || nmax < n || nmax > limit)
+      return fail(NEGF_ERR_ARG, "class %d cannot hold blocks of %d (classes 8, 16, 24, 32, "
+                  "48%s)", nmax, n, fp64 ? "" : ", 64");
+  }
+  g_class_override[fp64 ? 0 : 1][n].store(nmax, std::memory_order_relaxed);
+  return NEGF_OK;
+}
+
+int negf_block_class(int precision, int n) {
+  if (precision != NEGF_FP64 && precision != NEGF_MIXED) return 0;
+  return pick_nmax(n, precision == NEGF_FP64);
+}
 
 int negf_sancho_rubio_batched(void* stream, int precision, int n, int n_inst, const double* lead,
                               const int32_t* set_of, const double* omega, const double* eta,

@@ -153,3 +153,22 @@ def test_resolve_k_conventions():
         hn.resolve_k(dev, lead, lead, None, eta=1e-8, eta_device=None, device_to_lead_left=None,
                      device_to_lead_right=None, contact_left_indices=[0],
                      contact_right_indices=None, omega_scale=None, max_block=48)
+
+
+def test_block_class_table_on_the_host():
+    """The class override table is host state of the library (no GPU call)."""
+    from negfpy_amd import _lib
+
+    lib = _lib.load_symbols_only()
+    fp64, mixed = 0, 1
+    assert lib.negf_block_class(fp64, 20) == 24 and lib.negf_block_class(mixed, 50) == 64
+    assert lib.negf_set_block_class(fp64, 20, 32) == 0
+    assert lib.negf_block_class(fp64, 20) == 32
+    assert lib.negf_block_class(mixed, 20) == 24          # per precision
+    assert lib.negf_set_block_class(fp64, 20, 16) != 0    # class smaller than the block
+    assert lib.negf_set_block_class(fp64, 20, 64) != 0    # FP64 stops at 48
+    assert lib.negf_set_block_class(fp64, 20, 40) != 0    # not a class
+    assert lib.negf_set_block_class(fp64, 49, 0) != 0     # block too large for FP64
+    assert lib.negf_block_class(fp64, 20) == 32           # failed calls change nothing
+    assert lib.negf_set_block_class(fp64, 20, 0) == 0
+    assert lib.negf_block_class(fp64, 20) == 24
