@@ -330,32 +330,59 @@ def kspace_sets(device, lead_left, lead_right, kpoints, *, device_to_lead_left=N
                 device_to_lead_right=None, contact_left_indices=None, contact_right_indices=None,
                 omega_scale=None, precision: str = "fp64"):
     """TransportSets of every k point built on the GPU in one launch per model, when the
-    leads and the device are force-constant models from ``negfpy_amd.models``
-    (``material_kspace_lead`` / ``material_kspace_device``) with the default couplings
-    and contacts and no ``omega_scale``; otherwise None (the caller resolves per k on
-    the host, as the reference does)."""
-    tl = getattr(getattr(lead_left, "blocks_builder", None), "material_kspace", None)
-    tr = getattr(getattr(lead_right, "blocks_builder", None), "material_kspace", None)
+    leads and the device are force-constant models (``material_kspace_lead`` /
+    ``material_kspace_device`` of ``negfpy_amd.models``, or the interface contacts of
+    ``negfpy_amd.modeling.build_interface_contacts``) with default contacts and no
+    ``omega_scale``; otherwise None (the caller resolves per k on the host, as the
+    reference does)."""
+    bl = getattr(lead_left, "blocks_builder", None)
+    br = getattr(lead_right, "blocks_builder", None)
+    tl = getattr(bl, "material_kspace", None)
+    tr = getattr(br, "material_kspace", None)
     td = getattr(getattr(device, "device_builder", None), "material_kspace", None)
     if tl is None or tr is None or td is None:
         return None
-    if any(x is not None for x in (device_to_lead_left, device_to_lead_right,
-                                   contact_left_indices, contact_right_indices, omega_scale)):
+    if any(x is not None for x in (contact_left_indices, contact_right_indices, omega_scale)):
         return None
+    cl = cr = None
+    for v, side in ((device_to_lead_left, "l"), (device_to_lead_right, "r")):
+        if v is None:
+            continue
+        tag = getattr(v, "material_kspace_contact", None)
+        if tag is None or (tag[1] is not tl and tag[1] is not tr):
+            return None
+        if side == "l":
+            cl = tag
+        else:
+            cr = tag
     params, n_layers, layer_masses = td
     n = int(params.ndof)
     if int(tl.ndof) != n or int(tr.ndof) != n or n > _max_block(precision):
         return None
     from .kspace import kspace_blocks
 
+    def lead_blocks(p, builder):
+        lb = kspace_blocks(p, kpoints, lead=True).lead
+        if getattr(builder, "d10_is_d01_h", False):
+            lb[:, 2] = lb[:, 1].conj().transpose(-1, -2)
+        return lb
+
     kb = kspace_blocks(params, kpoints, lead=False, n_layers=n_layers, layer_masses=layer_masses)
-    kl = kspace_blocks(tl, kpoints, lead=True)
-    share = tl is tr
-    kr = kl if share else kspace_blocks(tr, kpoints, lead=True)
+    share = tl is tr and bool(getattr(bl, "d10_is_d01_h", False)) == bool(
+        getattr(br, "d10_is_d01_h", False))
+    kl = lead_blocks(tl, bl)
+    kr = kl if share else lead_blocks(tr, br)
+
+    def coupling(tag, default):
+        if tag is None:
+            return default
+        src = (kl if tag[1] is tl else kr)[:, 1]
+        return src if tag[0] == "d01" else src.conj().transpose(-1, -2).contiguous()
+
     idx = np.arange(n)
     # default couplings: V_L = d10 of the left lead, V_R = d01 of the right lead
     return _engine.TransportSets.from_arrays(
-        kl.lead, kr.lead, kl.lead[:, 2], kr.lead[:, 1], idx, idx, kb.onsite,
+        kl, kr, coupling(cl, kl[:, 2]), coupling(cr, kr[:, 1]), idx, idx, kb.onsite,
         kb.coupling if n_layers > 1 else None, share)
 
 
@@ -453,6 +480,8 @@ def _resolve_each(device, lead_left, lead_right, kpoints, *, eta_device, precisi
             # eta only enters resolution through the eta_device >= 0 check, made per rung
             out.append(resolve_k(device, lead_left, lead_right, k, eta=1.0, eta_device=eta_device,
                                  max_block=mb, **kw))
+        except NotImplementedError:
+            raise     # a block size beyond the kernels is a limit of this build, not a failure
         except Exception:
             out.append(None)
     return out
@@ -493,6 +522,9 @@ class _KSolver:
 def _t_max(lead_left, kpoints, max_channel_factor):
     if max_channel_factor is None:
         return [None] * len(kpoints)
+    tagged = getattr(getattr(lead_left, "blocks_builder", None), "material_kspace", None)
+    if tagged is not None:
+        return [float(max_channel_factor) * float(tagged.ndof)] * len(kpoints)
     return [float(max_channel_factor) * float(_resolve_lead_blocks(lead_left, k).d00.shape[0])
             for k in kpoints]
 

@@ -1,0 +1,104 @@
+"""The reference's study transmission runs reproduced end to end on the GPU: the QE
+IFC file through the conditioned model, interface contacts, k meshes and omega grid
+to the adaptive-eta k-averaged spectrum and its TSV (``golden/study_transmission``:
+graphene along x with a 24-point 1D k mesh and 80 omegas; silicon along x with
+18x18 / 20x20 2D k meshes and 40 omegas).
+
+Per-omega acceptance (which eta each k point took) must equal the reference's
+histograms exactly.  The reference solved the surface GF by generalized_eigen_svd,
+this path by Sancho-Rubio decimation; the spectra must agree to ``REL`` relative
+(T at 1e-3 and below absolute), the spread of two exact methods at eta = 1e-8 on
+these models.
+"""
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from negfpy_amd.core.negf import kspace_sets, transmission_kavg_adaptive_spectrum
+from negfpy_amd.core.types import DeviceKSpace, LeadKSpace
+from negfpy_amd.workflows import build_transport_model, run_transmission_config
+
+pytestmark = pytest.mark.gpu
+
+STUDY = Path(__file__).resolve().parent / "golden" / "study_transmission"
+CASES = (("graphene", "graphene.fc"), ("silicon_x", "si444.fc"))
+REL = 1e-9
+IV = np.load(Path(__file__).resolve().parent / "golden" / "ifc_vectors.npz")
+
+
+def _model(tag, fc, n_layers=None):
+    cfg = json.loads((STUDY / f"{tag}_input.json").read_text())
+    cfg["ifc"]["path"] = fc
+    if n_layers is not None:
+        cfg["model"]["n_layers"] = n_layers
+    return cfg, build_transport_model(cfg, STUDY)
+
+
+@pytest.mark.parametrize("tag,fc", CASES)
+def test_contact_sets_built_on_gpu_equal_reference_blocks(gpu, tag, fc):
+    """Interface-contact leads, couplings and device blocks of every k point built in
+    one launch per model: bit for bit the reference's per-k host builders."""
+    _, m = _model(tag, fc, n_layers=int(IV[f"{tag}_t_layers"]))
+    ks = [tuple(k) for k in IV[f"{tag}_k"]]
+    sets = kspace_sets(m["device"], m["lead_left"], m["lead_right"], ks,
+                       device_to_lead_left=m["device_to_lead_left"],
+                       device_to_lead_right=m["device_to_lead_right"])
+    assert sets is not None and not sets.share_leads
+    np.testing.assert_array_equal(sets.lead_l.cpu().numpy(), IV[f"{tag}_lead_l"])
+    np.testing.assert_array_equal(sets.lead_r.cpu().numpy(), IV[f"{tag}_lead_r"])
+    np.testing.assert_array_equal(sets.v_l.cpu().numpy(), IV[f"{tag}_v_l"])
+    np.testing.assert_array_equal(sets.v_r.cpu().numpy(), IV[f"{tag}_v_r"])
+    np.testing.assert_array_equal(sets.onsite.cpu().numpy(), IV[f"{tag}_onsite"])
+    np.testing.assert_array_equal(sets.coupling.cpu().numpy(), IV[f"{tag}_coupling"])
+
+
+def test_contact_spectrum_gpu_built_equals_host_resolved(gpu):
+    """The same adaptive spectrum with the blocks built on the GPU and resolved per k
+    on the host (untagged providers): identical acceptance, T to 1e-13."""
+    _, m = _model("silicon_x", "si444.fc", n_layers=8)
+    ks = [(a, b) for a in (-2.5, -0.4, 1.3) for b in (-1.1, 0.7)]
+    w = np.linspace(0.0004, 0.0058, 7)
+    args = (m["device"], m["lead_left"], m["lead_right"])
+    plain = (DeviceKSpace(device_builder=lambda k: m["device"].device_builder(k)),
+             LeadKSpace(blocks_builder=lambda k: m["lead_left"].blocks_builder(k)),
+             LeadKSpace(blocks_builder=lambda k: m["lead_right"].blocks_builder(k)))
+    vl, vr = m["device_to_lead_left"], m["device_to_lead_right"]
+    kw = dict(eta_values=(1e-8, 1e-6, 1e-4), min_success_fraction=0.5, max_channel_factor=1.5)
+    assert kspace_sets(*args, ks, device_to_lead_left=vl, device_to_lead_right=vr) is not None
+    assert kspace_sets(*plain, ks, device_to_lead_left=lambda k: vl(k),
+                       device_to_lead_right=lambda k: vr(k)) is None
+    a, ra = transmission_kavg_adaptive_spectrum(w, *args, ks, device_to_lead_left=vl,
+                                                device_to_lead_right=vr, **kw)
+    b, rb = transmission_kavg_adaptive_spectrum(w, *plain, ks, device_to_lead_left=lambda k: vl(k),
+                                                device_to_lead_right=lambda k: vr(k), **kw)
+    np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-16)
+    assert [r["info"]["eta_histogram"] for r in ra] == [r["info"]["eta_histogram"] for r in rb]
+
+
+@pytest.mark.parametrize("tag,fc", CASES)
+def test_study_spectrum_matches_reference(gpu, tag, fc, tmp_path):
+    cfg = json.loads((STUDY / f"{tag}_input.json").read_text())
+    cfg["ifc"]["path"] = fc
+    out = tmp_path / "transmission.tsv"
+    res = run_transmission_config(cfg, base_dir=STUDY, output_path=out)
+    ref = np.loadtxt(STUDY / f"{tag}_transmission.tsv")
+    got = np.loadtxt(out)
+    assert got.shape == ref.shape
+    np.testing.assert_array_equal(got[:, :3], ref[:, :3])
+    err = np.abs(got[:, 3] - ref[:, 3]) / np.maximum(np.abs(ref[:, 3]), 1e-3)
+    print(f"{tag}: max rel err {err.max():.3e} at omega index {int(err.argmax())}", flush=True)
+    assert err.max() <= REL
+    with open(STUDY / f"{tag}_eta_diag.json") as f:
+        diag = json.load(f)
+    assert res["omega_failures"] == diag["omega_failures"] == []
+    assert len(res["eta_records"]) == len(diag["eta_records"])
+    for mine, theirs in zip(res["eta_records"], diag["eta_records"]):
+        assert mine["omega_qe"] == pytest.approx(theirs["omega_qe"], rel=1e-14)
+        hist = {str(float(k)): int(v) for k, v in mine["eta_histogram"].items()}
+        assert hist == theirs["eta_histogram"], (mine["omega_cm1"], hist, theirs["eta_histogram"])
+        assert mine["n_failed_kpoints"] == theirs["n_failed_kpoints"]
+        assert mine["success_fraction"] == theirs["success_fraction"]
+        assert mine["max_eta_used"] == theirs["max_eta_used"]
