@@ -102,3 +102,58 @@ def test_study_spectrum_matches_reference(gpu, tag, fc, tmp_path):
         assert mine["n_failed_kpoints"] == theirs["n_failed_kpoints"]
         assert mine["success_fraction"] == theirs["success_fraction"]
         assert mine["max_eta_used"] == theirs["max_eta_used"]
+
+
+# --- every current-format QE transmission run of the reference's studies --------
+
+from negfpy_amd.core import transmission_kavg  # noqa: E402
+from negfpy_amd.workflows import build_omega_grid, prepare_kmesh  # noqa: E402
+from negfpy_amd.modeling.units import qe_omega_to_cm1  # noqa: E402
+from study_runs_util import RUNS, WORST, load_run, oracle_kavg, spectrum_bound  # noqa: E402
+
+
+def _check_records(res, diag):
+    assert res["omega_failures"] == diag["omega_failures"]
+    assert len(res["eta_records"]) == len(diag["eta_records"])
+    for mine, theirs in zip(res["eta_records"], diag["eta_records"]):
+        assert mine["omega_qe"] == pytest.approx(theirs["omega_qe"], rel=1e-14)
+        hist = {str(float(k)): int(v) for k, v in mine["eta_histogram"].items()}
+        assert hist == theirs["eta_histogram"], (mine["omega_cm1"], hist, theirs["eta_histogram"])
+        assert mine["n_failed_kpoints"] == theirs["n_failed_kpoints"]
+        assert mine["max_eta_used"] == theirs["max_eta_used"]
+
+
+@pytest.mark.parametrize("name", RUNS)
+def test_study_run_reproduced(gpu, name, tmp_path):
+    cfg, rows, _, diag = load_run(name, tmp_path)
+    m = build_transport_model(cfg)
+    if m["info"]["ndof"] > 48:
+        pytest.skip(f"{m['info']['ndof']}-DOF principal layers exceed the 48-DOF FP64 kernels")
+    out = tmp_path / "transmission.tsv"
+    res = run_transmission_config(cfg, output_path=out)
+    got = np.loadtxt(out)
+    assert got.shape == rows.shape
+    np.testing.assert_array_equal(got[:, :3], rows[:, :3])
+    err = np.abs(got[:, 3] - rows[:, 3]) / np.maximum(np.abs(rows[:, 3]), 1e-3)
+    print(f"{name}: n={m['info']['ndof']} max rel err {err.max():.3e} at omega index "
+          f"{int(err.argmax())}", flush=True)
+    _check_records(res, diag)
+    assert err.max() <= spectrum_bound(name)
+
+
+@pytest.mark.parametrize("name,idx", WORST)
+def test_study_worst_points_equal_decimation_route(gpu, name, idx, tmp_path):
+    """At the points where a study file departs most from this path, the GPU k-average
+    equals the reference's decimation route (the oracle's) to 1e-11."""
+    cfg, rows, _, _ = load_run(name, tmp_path)
+    m = build_transport_model(cfg)
+    km = prepare_kmesh(cfg)
+    w = build_omega_grid(cfg, m["ifc"])[0][idx]
+    c = float(np.asarray(qe_omega_to_cm1(w)))
+    ks = km["low" if c <= km["low_cm1"] else "high"]["kpoints"]
+    t_gpu = transmission_kavg(w, m["device"], m["lead_left"], m["lead_right"], ks, eta=1e-8,
+                              device_to_lead_left=m["device_to_lead_left"],
+                              device_to_lead_right=m["device_to_lead_right"])
+    t_or, _ = oracle_kavg(m, ks, w, 1e-8, "sancho_rubio")
+    print(f"{name}[{idx}]: gpu {t_gpu:.15e} oracle SR {t_or:.15e} file {rows[idx, 3]:.12e}", flush=True)
+    assert abs(t_gpu - t_or) <= 1e-11 * max(abs(t_or), 1e-3)

@@ -151,3 +151,63 @@ def test_different_lead_files_rejected(tmp_path):
     cfg["ifc"]["left_path"] = "si444.fc"
     with pytest.raises(NotImplementedError, match="Different left/center/right"):
         W.build_transport_model(cfg, STUDY)
+
+
+# --- every current-format QE transmission run of the reference's studies --------
+
+from study_runs_util import RUNS, load_run  # noqa: E402
+
+
+@pytest.mark.parametrize("name", RUNS)
+def test_study_run_set_up_matches_reference_file(name, tmp_path):
+    """Model, omega grid and TSV metadata of each study run: the model builds, the grid
+    equals the file's first column, and the writer reproduces the file's header lines
+    and frequency columns byte for byte."""
+    cfg, rows, text, _ = load_run(name, tmp_path)
+    m = W.build_transport_model(cfg)
+    w, _ = W.build_omega_grid(cfg, m["ifc"])
+    np.testing.assert_allclose(w, rows[:, 0], rtol=1e-10, atol=0)
+    meta, area = W.spectrum_metadata(cfg, m["ifc"])
+    extra = None if area is None else {"transmission_per_m2": rows[:, 4]}
+    out = tmp_path / "t.tsv"
+    W.save_spectrum_tsv(out, w, rows[:, 3], metadata=meta, extra_columns=extra)
+    assert out.read_text() == text
+    km = W.prepare_kmesh(cfg)
+    assert len(km["high"]["kpoints"]) > 0 and len(km["low"]["kpoints"]) > 0
+
+
+from study_runs_util import WORST, oracle_kavg  # noqa: E402
+
+
+def _worst_setup(name, idx, tmp_path):
+    cfg, rows, _, diag = load_run(name, tmp_path)
+    m = W.build_transport_model(cfg)
+    km = W.prepare_kmesh(cfg)
+    w, _ = W.build_omega_grid(cfg, m["ifc"])
+    c = float(np.asarray(W.qe_omega_to_cm1(w[idx])))
+    ks = km["low" if c <= km["low_cm1"] else "high"]["kpoints"]
+    hist = [r for r in diag["eta_records"] if r["omega_qe"] == pytest.approx(w[idx], rel=1e-14)][0]
+    assert hist["eta_histogram"]["1e-08"] == len(ks)     # every k point took eta = 1e-8
+    return m, ks, float(w[idx]), rows[idx, 3]
+
+
+@pytest.mark.parametrize("name,idx", WORST)
+def test_study_method_spread_is_the_eigen_route_error(name, idx, tmp_path):
+    """Where the decimation path departs from a study file most, the file is the
+    eigenmode route's answer (the oracle's eigenmode route gives it to 1e-11) and that
+    route is the less accurate one: its surface GF leaves a Riccati residual at least
+    10x the decimation's at the k point where the two differ most."""
+    m, ks, w, t_file = _worst_setup(name, idx, tmp_path)
+    t_ge, leads = oracle_kavg(m, ks, w, 1e-8, "generalized_eigen_svd")
+    assert abs(t_ge - t_file) <= 1e-11 * max(abs(t_file), 1e-3)
+    t_sr, _ = oracle_kavg(m, ks, w, 1e-8, "sancho_rubio")
+    assert abs(t_sr - t_file) > 1e-9 * max(abs(t_file), 1e-3)
+    gaps = []
+    for ll in leads:
+        g_sr = o.surface_gf(w, *ll, eta=1e-8, method="sancho_rubio")
+        g_ge = o.surface_gf(w, *ll, eta=1e-8, method="generalized_eigen_svd")
+        gaps.append((np.abs(g_sr - g_ge).max() / np.abs(g_ge).max(), ll, g_sr, g_ge))
+    _, ll, g_sr, g_ge = max(gaps, key=lambda x: x[0])
+    r_sr = o.riccati_residual(w, ll[0], ll[1], ll[2], 1e-8, g_sr)
+    r_ge = o.riccati_residual(w, ll[0], ll[1], ll[2], 1e-8, g_ge)
+    assert r_ge >= 10.0 * r_sr, (r_sr, r_ge)
