@@ -45,13 +45,21 @@ int negf_version(void);
 /* Description of the last error on this thread. */
 const char* negf_last_error(void);
 
-/* Largest supported principal-layer block size for a precision. */
+/* Largest supported principal-layer block size of the transmission, surface-GF and
+ * self-energy entries for a precision (FP64: 96; mixed: 48). */
 int negf_max_block(int precision);
 
+/* Largest block size of negf_device_green_batched and negf_adjoint_batched (48). */
+int negf_max_block_green(void);
+
 /*
- * Block-size class (8, 16, 24, 32, 48, 64: workgroup size, rows per lane, tile
- * size and LDS footprint of every kernel) used for blocks of n DOF in the given
- * precision.  Any class >= n is valid; nmax = 0 restores the default (the
+ * Block-size class (8, 16, 24, 32, 48 and, in FP64, 96; 64 in mixed precision:
+ * workgroup size, rows per lane, tile size and LDS footprint of every kernel) used
+ * for blocks of n DOF in the given precision.  Class 96 keeps the matrices of an
+ * instance in a device-memory pool the library allocates on first use on a device
+ * (about 1.3 MB per resident workgroup) and keeps until the process ends; calls on
+ * different streams are ordered so they never share it at once, and it cannot be
+ * used under stream capture.  Any class >= n is valid; nmax = 0 restores the default (the
  * smallest class >= n).  Set by the autotuner (negfpy_amd.core.autotune); the
  * table is process-wide and thread-safe.  No counterpart in the reference, whose
  * dense/sparse solves have no launch configuration.

@@ -57,6 +57,7 @@ _SIGS = {
     "negf_version": ([], _c_int),
     "negf_last_error": ([], ctypes.c_char_p),
     "negf_max_block": ([_c_int], _c_int),
+    "negf_max_block_green": ([], _c_int),
     "negf_set_block_class": ([_c_int, _c_int, _c_int], _c_int),
     "negf_block_class": ([_c_int, _c_int], _c_int),
     "negf_sancho_rubio_batched": (

@@ -24,7 +24,7 @@ import torch
 
 from . import _engine
 from ._status import transmission_value
-from .negf import _max_block, resolve_k
+from .negf import _max_block_green, resolve_k
 from .types import Device1D
 
 __all__ = ["transmission_torch", "thermal_conductance_torch"]
@@ -145,7 +145,7 @@ def transmission_torch(omega, onsite, coupling, lead_left, lead_right, *, eta: f
                    device_to_lead_right=device_to_lead_right,
                    contact_left_indices=contact_left_indices,
                    contact_right_indices=contact_right_indices, omega_scale=omega_scale,
-                   max_block=_max_block("fp64"))
+                   max_block=_max_block_green())
     if omega_scale is not None:
         s2 = float(omega_scale) ** 2
         ons = ons / s2

@@ -1,8 +1,9 @@
 """Autotuner for the launch layout of each block size.
 
-Every kernel exists in block-size classes (8, 16, 24, 32, 48, 64 DOF): the class
-fixes the workgroup size, the rows each lane owns in the inversion, the product
-tile per thread and the LDS per instance, and so the occupancy.  A block of n DOF
+Every kernel exists in block-size classes (8, 16, 24, 32, 48 DOF, 64 in mixed
+precision, 96 in FP64 with the matrices in a device-memory pool): the class fixes
+the workgroup size, the rows each lane owns in the inversion, the product tile per
+thread and the LDS per instance, and so the occupancy.  A block of n DOF
 runs in any class >= n; which one is fastest depends on how the inversion's
 per-step latency (rows per lane) trades against resident instances per CU.
 ``tune_block_class`` times the candidate classes on a synthetic batch of n-DOF
@@ -26,14 +27,16 @@ from .types import LeadBlocks
 
 __all__ = ["CLASSES", "candidate_classes", "tune_block_class", "block_class", "reset_block_class"]
 
-CLASSES = (8, 16, 24, 32, 48, 64)
+CLASSES = (8, 16, 24, 32, 48, 64, 96)
 _lock = threading.Lock()
 _results: dict = {}
 
 
 def candidate_classes(n: int, precision: str = "fp64") -> list[int]:
-    limit = 48 if precision == "fp64" else 64
-    return [c for c in CLASSES if n <= c <= limit]
+    if precision == "fp64":
+        # the pooled 96 class only for blocks the LDS classes cannot hold
+        return [c for c in (8, 16, 24, 32, 48) if n <= c] or ([96] if n <= 96 else [])
+    return [c for c in (8, 16, 24, 32, 48, 64) if n <= c]
 
 
 def block_class(n: int, precision: str = "fp64") -> int:

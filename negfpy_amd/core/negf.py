@@ -249,6 +249,13 @@ def _max_block(precision):
     return int(_lib.load().negf_max_block(_engine.precision_code(precision)))
 
 
+def _max_block_green():
+    """Largest block of the full Green's function and adjoint kernels."""
+    from negfpy_amd import _lib
+
+    return int(_lib.load().negf_max_block_green())
+
+
 def _resolve_many(device, lead_left, lead_right, kpoints, *, eta, eta_device, device_to_lead_left,
                   device_to_lead_right, contact_left_indices, contact_right_indices, omega_scale,
                   precision):
@@ -282,7 +289,7 @@ def device_green_function(omega: float, device: DeviceLike, lead_left: LeadLike,
                    device_to_lead_right=device_to_lead_right,
                    contact_left_indices=contact_left_indices,
                    contact_right_indices=contact_right_indices, omega_scale=omega_scale,
-                   max_block=_max_block("fp64"))
+                   max_block=_max_block_green())
     sets = _engine.TransportSets.from_arrays(
         kp.lead_l[None], kp.lead_r[None], kp.v_l[None], kp.v_r[None], kp.pos_l, kp.pos_r,
         kp.onsite[None], kp.coupling[None] if kp.onsite.shape[0] > 1 else None, False)

@@ -1,0 +1,552 @@
+// Block-size class 96 (FP64): principal layers of 49..96 DOF.
+//
+// Three 96 x 97 planar complex matrices already need 447 KB, more than a CU's
+// 160 KB of LDS, so in this class the matrices of an instance live in a slot of a
+// device-memory pool (one slot per resident workgroup, reused instance after
+// instance; the working set of the resident slots stays in the L2 / MALL).  The
+// inversion is the same register-resident Gauss-Jordan as the LDS classes (its
+// pivot buffers stay in LDS) and the products are the same owner-tile loops,
+// reading their operands from the slot; only one accumulator tile is live at a
+// time, so the 768-thread workgroups fit the register file without spilling.
+// The kernels are persistent: the grid is at most the number of slots and each
+// workgroup walks the instances blockIdx.x, blockIdx.x + gridDim.x, ...
+//
+// Each kernel follows its LDS counterpart step for step (same products, same
+// order of the additions), so results agree with the LDS classes to rounding.
+
+#pragma once
+
+#include "cmat.hpp"
+
+namespace negf {
+
+constexpr int BIG = 96;
+constexpr int BIG_NT = 768;
+constexpr int BIG_TM = 4;                         // (96 / 4)^2 = 576 owner tiles
+constexpr int BIG_RG = 8;                         // inversion: 96 columns x 8 lanes
+constexpr int BIG_LD = BIG + 1;
+constexpr size_t BIG_PLANE = (size_t)BIG * BIG_LD;
+constexpr size_t BIG_MAT = 2 * BIG_PLANE;         // doubles per planar matrix
+constexpr int BIG_SLOT_MATS = 9;                  // the decimation needs the most
+constexpr size_t BIG_SLOT = BIG_SLOT_MATS * BIG_MAT;
+
+constexpr int BIG_KT = 16;                        // K chunk of the products staged in LDS
+
+struct BigShared {
+  GjBuf<double, BIG> gj;
+  double sred[BIG_NT / WAVE];
+  int sflag[BIG_NT / WAVE];
+  // op(A)[:, k0:k0+KT] as a[kk][i] and op(B)[k0:k0+KT, :] as b[kk][j]
+  double ar[BIG_KT][BIG], ai[BIG_KT][BIG];
+  double br[BIG_KT][BIG], bi[BIG_KT][BIG];
+};
+
+// Slot matrices are addressed as global memory with 32-bit element offsets, so a
+// load or store is one scalar base plus one vector offset (no 64-bit address per
+// element kept live across the loops).
+using gdouble = __attribute__((address_space(1))) double;
+
+struct BigMat {
+  gdouble* re;
+  gdouble* im;
+};
+
+__device__ __forceinline__ BigMat big_mat(double* slot, int q) {
+  gdouble* p = (gdouble*)(slot) + (size_t)q * BIG_MAT;
+  return BigMat{p, p + BIG_PLANE};
+}
+
+__device__ __forceinline__ unsigned big_off(int i, int j) { return (unsigned)(i * BIG_LD + j); }
+
+// The slot base as an opaque (uniform) value: matrix addresses derived from it are
+// formed where they are used instead of being hoisted out of the instance and
+// iteration loops and kept live in registers.
+__device__ __forceinline__ double* big_fresh(double* p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+// A-operand element (i, kk) of op(A) and B-operand element (kk, j) of op(B)
+template <int OP>
+__device__ __forceinline__ void big_lda(const BigMat& A, int i, int kk, double& r, double& m) {
+  if constexpr (OP == OPN) {
+    r = A.re[big_off(i, kk)];
+    m = A.im[big_off(i, kk)];
+  } else if constexpr (OP == OPH) {
+    r = A.re[big_off(kk, i)];
+    m = -A.im[big_off(kk, i)];
+  } else if constexpr (OP == OPC) {
+    r = A.re[big_off(i, kk)];
+    m = -A.im[big_off(i, kk)];
+  } else {
+    r = A.re[big_off(kk, i)];
+    m = A.im[big_off(kk, i)];
+  }
+}
+template <int OP>
+__device__ __forceinline__ void big_ldb(const BigMat& B, int kk, int j, double& r, double& m) {
+  if constexpr (OP == OPN) {
+    r = B.re[big_off(kk, j)];
+    m = B.im[big_off(kk, j)];
+  } else if constexpr (OP == OPH) {
+    r = B.re[big_off(j, kk)];
+    m = -B.im[big_off(j, kk)];
+  } else if constexpr (OP == OPC) {
+    r = B.re[big_off(kk, j)];
+    m = -B.im[big_off(kk, j)];
+  } else {
+    r = B.re[big_off(j, kk)];
+    m = B.im[big_off(j, kk)];
+  }
+}
+
+// m x p block (complex128, row-major) into a slot matrix, and a slot matrix out
+__device__ __forceinline__ void big_gload(const double* __restrict__ src, int m, int p,
+                                          const BigMat& D, int tid) {
+  const double2* s = reinterpret_cast<const double2*>(src);
+  const int mp = m * p;
+  for (int e = tid; e < mp; e += BIG_NT) {
+    const int i = e / p, j = e - (e / p) * p;
+    const double2 v = s[e];
+    D.re[big_off(i, j)] = v.x;
+    D.im[big_off(i, j)] = v.y;
+  }
+}
+__device__ __forceinline__ void big_gstore(double* __restrict__ dst, int n, const BigMat& S, int tid) {
+  double2* d = reinterpret_cast<double2*>(dst);
+  const int nn = n * n;
+  for (int e = tid; e < nn; e += BIG_NT) {
+    const int i = e / n, j = e - (e / n) * n;
+    d[e] = make_double2(S.re[big_off(i, j)], S.im[big_off(i, j)]);
+  }
+}
+
+using BigTile = Tile<BIG_TM>;
+
+// C_tile = op(A)[0:m, 0:k] op(B)[0:k, 0:p] for the owner tiles of an m x p output.
+// K is walked in chunks of BIG_KT staged in LDS by the whole workgroup (every
+// thread must call); the accumulation order over k is the plain k = 0, 1, ... of
+// the LDS classes.  Ends with a barrier, so the operands may be overwritten next.
+template <int OPA, int OPB>
+__device__ __forceinline__ void big_mul(const BigMat& A, const BigMat& B, int m, int k, int p,
+                                        const BigTile& t, double (&cr)[BIG_TM][BIG_TM],
+                                        double (&ci)[BIG_TM][BIG_TM], BigShared& sh, int tid) {
+  tzero<double, BIG_TM>(cr, ci);
+  for (int k0 = 0; k0 < k; k0 += BIG_KT) {
+    const int kt = (k - k0) < BIG_KT ? (k - k0) : BIG_KT;
+#pragma unroll 1
+    for (int e = tid; e < BIG_KT * BIG; e += BIG_NT) {
+      const int kk = e / BIG, i = e - (e / BIG) * BIG;
+      double xr = 0.0, xi = 0.0, yr = 0.0, yi = 0.0;
+      if (kk < kt) {
+        if (i < m) big_lda<OPA>(A, i, k0 + kk, xr, xi);
+        if (i < p) big_ldb<OPB>(B, k0 + kk, i, yr, yi);
+      }
+      sh.ar[kk][i] = xr;
+      sh.ai[kk][i] = xi;
+      sh.br[kk][i] = yr;
+      sh.bi[kk][i] = yi;
+    }
+    __syncthreads();
+    if (t.own) {
+#pragma unroll 1
+      for (int kk = 0; kk < kt; ++kk) {
+        double xr[BIG_TM], xi[BIG_TM], yr[BIG_TM], yi[BIG_TM];
+#pragma unroll
+        for (int q = 0; q < BIG_TM; ++q) {
+          xr[q] = sh.ar[kk][t.i0 + q];
+          xi[q] = sh.ai[kk][t.i0 + q];
+          yr[q] = sh.br[kk][t.j0 + q];
+          yi[q] = sh.bi[kk][t.j0 + q];
+        }
+#pragma unroll
+        for (int x = 0; x < BIG_TM; ++x)
+#pragma unroll
+          for (int y = 0; y < BIG_TM; ++y) cmac(cr[x][y], ci[x][y], xr[x], xi[x], yr[y], yi[y]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ void big_tstore(const BigMat& C, int m, int p, const BigTile& t,
+                                           const double (&cr)[BIG_TM][BIG_TM],
+                                           const double (&ci)[BIG_TM][BIG_TM]) {
+  if (!t.own) return;
+#pragma unroll
+  for (int x = 0; x < BIG_TM; ++x)
+#pragma unroll
+    for (int y = 0; y < BIG_TM; ++y) {
+      const int i = t.i0 + x, j = t.j0 + y;
+      if (i < m && j < p) {
+        C.re[big_off(i, j)] = cr[x][y];
+        C.im[big_off(i, j)] = ci[x][y];
+      }
+    }
+}
+
+__device__ __forceinline__ void big_store(const BigMat& C, int n, const BigTile& t,
+                                          const double (&cr)[BIG_TM][BIG_TM],
+                                          const double (&ci)[BIG_TM][BIG_TM]) {
+  big_tstore(C, n, n, t, cr, ci);
+}
+
+// ---------------------------------------------------------------------------
+// decimation (surface_gf.py:18-81), slots: a, b, a', b', e, e_s, g, T1 = a g, T2 = b g
+// ---------------------------------------------------------------------------
+
+__device__ void big_sr_one(const SrArgs& a, int inst, double* slot, BigShared& sh, int tid) {
+  if (a.redo && a.status[inst] == ST_OK) return;
+  const int n = a.n;
+  const size_t nn = (size_t)n * n;
+  const double* blk = a.lead + (size_t)a.set_of[inst] * 3 * nn * 2;
+  int qa = 0, qb = 1;  // slots of alpha and beta; the next ones go to 2 - qa, 4 - qb... swapped
+  {
+    double* sl = big_fresh(slot);
+    big_gload(blk + nn * 2, n, n, big_mat(sl, 0), tid);
+    big_gload(blk + 2 * nn * 2, n, n, big_mat(sl, 1), tid);
+    big_gload(blk, n, n, big_mat(sl, 4), tid);
+    big_gload(blk, n, n, big_mat(sl, 5), tid);
+  }
+  double zr, zi;
+  zsq(a.omega[inst], a.eta[inst], zr, zi);
+  const BigTile t(n, n, tid);
+  __syncthreads();
+  int status = ST_OK;
+  int it = 0;
+  bool conv = false;
+  while (it < a.maxiter) {
+    ++it;
+    double* sl = big_fresh(slot);
+    const BigMat A = big_mat(sl, qa), B = big_mat(sl, qb), An = big_mat(sl, qa ^ 2),
+                 Bn = big_mat(sl, qb ^ 2);
+    const BigMat E = big_mat(sl, 4), ES = big_mat(sl, 5), G = big_mat(sl, 6);
+    const BigMat T1 = big_mat(sl, 7), T2 = big_mat(sl, 8);
+    for (int e = tid; e < (int)nn; e += BIG_NT) {
+      const int i = e / n, j = e - (e / n) * n;
+      G.re[big_off(i, j)] = (i == j ? zr : 0.0) - E.re[big_off(i, j)];
+      G.im[big_off(i, j)] = (i == j ? zi : 0.0) - E.im[big_off(i, j)];
+    }
+    __syncthreads();
+    const int st = gj_inverse<double, BIG_NT, BIG, BIG_RG>(G.re, G.im, BIG_LD, n, sh.gj, tid);
+    if (st != ST_OK) {
+      status = st;
+      break;
+    }
+    double cr[BIG_TM][BIG_TM], ci[BIG_TM][BIG_TM];
+    big_mul<OPN, OPN>(A, G, n, n, n, t, cr, ci, sh, tid);
+    big_store(T1, n, t, cr, ci);
+    big_mul<OPN, OPN>(B, G, n, n, n, t, cr, ci, sh, tid);
+    big_store(T2, n, t, cr, ci);
+    __syncthreads();
+    bool bad = false;
+    double emax = 0.0;
+    // P1 = T1 b into e_s and e; P3 = T1 a is the next alpha
+    big_mul<OPN, OPN>(T1, B, n, n, n, t, cr, ci, sh, tid);
+    if (t.own) {
+#pragma unroll
+      for (int x = 0; x < BIG_TM; ++x)
+#pragma unroll
+        for (int y = 0; y < BIG_TM; ++y) {
+          const int i = t.i0 + x, j = t.j0 + y;
+          if (i < n && j < n) {
+            const unsigned o = big_off(i, j);
+            ES.re[o] = ES.re[o] + cr[x][y];
+            ES.im[o] = ES.im[o] + ci[x][y];
+            E.re[o] = E.re[o] + cr[x][y];
+            E.im[o] = E.im[o] + ci[x][y];
+            bad |= !(isfinite(ES.re[o]) && isfinite(ES.im[o]));
+          }
+        }
+    }
+    big_mul<OPN, OPN>(T1, A, n, n, n, t, cr, ci, sh, tid);
+    big_store(An, n, t, cr, ci);
+    if (t.own) {
+#pragma unroll
+      for (int x = 0; x < BIG_TM; ++x)
+#pragma unroll
+        for (int y = 0; y < BIG_TM; ++y)
+          if (t.i0 + x < n && t.j0 + y < n) {
+            bad |= !(isfinite(cr[x][y]) && isfinite(ci[x][y]));
+            emax = fmax(emax, cr[x][y] * cr[x][y] + ci[x][y] * ci[x][y]);
+          }
+    }
+    // P2 = T2 a into e; P4 = T2 b is the next beta
+    big_mul<OPN, OPN>(T2, A, n, n, n, t, cr, ci, sh, tid);
+    if (t.own) {
+#pragma unroll
+      for (int x = 0; x < BIG_TM; ++x)
+#pragma unroll
+        for (int y = 0; y < BIG_TM; ++y) {
+          const int i = t.i0 + x, j = t.j0 + y;
+          if (i < n && j < n) {
+            const unsigned o = big_off(i, j);
+            E.re[o] = E.re[o] + cr[x][y];
+            E.im[o] = E.im[o] + ci[x][y];
+            bad |= !(isfinite(E.re[o]) && isfinite(E.im[o]));
+          }
+        }
+    }
+    big_mul<OPN, OPN>(T2, B, n, n, n, t, cr, ci, sh, tid);
+    big_store(Bn, n, t, cr, ci);
+    if (t.own) {
+#pragma unroll
+      for (int x = 0; x < BIG_TM; ++x)
+#pragma unroll
+        for (int y = 0; y < BIG_TM; ++y)
+          if (t.i0 + x < n && t.j0 + y < n) {
+            bad |= !(isfinite(cr[x][y]) && isfinite(ci[x][y]));
+            emax = fmax(emax, cr[x][y] * cr[x][y] + ci[x][y] * ci[x][y]);
+          }
+    }
+    double gmax;
+    bool gbad;
+    block_max_any<BIG_NT>(emax, bad, sh.sred, sh.sflag, tid, gmax, gbad);
+    qa ^= 2;
+    qb ^= 2;
+    if (gbad) {
+      status = ST_NONFINITE;
+      break;
+    }
+    if (sqrt(gmax) < a.tol) {
+      conv = true;
+      break;
+    }
+  }
+  if (status == ST_OK && !conv) status = ST_NOCONV;
+  if (tid == 0) {
+    a.status[inst] = status;
+    a.iters[inst] = it;
+  }
+  if (status != ST_OK) return;
+  double* sl = big_fresh(slot);
+  const BigMat ES = big_mat(sl, 5), G = big_mat(sl, 6);
+  for (int e = tid; e < (int)nn; e += BIG_NT) {
+    const int i = e / n, j = e - (e / n) * n;
+    G.re[big_off(i, j)] = (i == j ? zr : 0.0) - ES.re[big_off(i, j)];
+    G.im[big_off(i, j)] = (i == j ? zi : 0.0) - ES.im[big_off(i, j)];
+  }
+  __syncthreads();
+  const int st = gj_inverse<double, BIG_NT, BIG, BIG_RG>(G.re, G.im, BIG_LD, n, sh.gj, tid);
+  if (st != ST_OK) {
+    if (tid == 0) a.status[inst] = st;
+    return;
+  }
+  big_gstore(a.g_out + (size_t)inst * nn * 2, n, G, tid);
+  if (tid == 0 && a.dos_out) {
+    double tr = 0.0;
+    for (int i = 0; i < n; ++i) tr += -2.0 * G.im[big_off(i, i)];
+    a.dos_out[inst] = (a.omega[inst] / M_PI) * tr;
+  }
+}
+
+__global__ void __launch_bounds__(BIG_NT) k_sr_big(SrArgs a, double* pool) {
+  __shared__ BigShared sh;
+  double* slot = pool + (size_t)blockIdx.x * BIG_SLOT;
+  for (int inst = blockIdx.x; inst < a.n_inst; inst += gridDim.x) {
+    big_sr_one(a, inst, slot, sh, threadIdx.x);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// self-energy Sigma = V g V^H embedded in the end layer (negf.py:72-97)
+// ---------------------------------------------------------------------------
+
+__device__ void big_sigma_one(const SigmaArgs& a, int inst, double* slot, BigShared& sh, int tid) {
+  const int nl = a.n_lead, nc = a.n_c, nd = a.n_dev;
+  double2* out = reinterpret_cast<double2*>(a.sigma_out + (size_t)inst * nd * nd * 2);
+  if (a.status_in && a.status_in[inst] != ST_OK) return;
+  double* sl = big_fresh(slot);
+  const BigMat V = big_mat(sl, 0), Gm = big_mat(sl, 1), X = big_mat(sl, 2), S = big_mat(sl, 3);
+  big_gload(a.v + (size_t)a.set_of[inst] * nc * nl * 2, nc, nl, V, tid);
+  big_gload(a.g + (size_t)inst * nl * nl * 2, nl, nl, Gm, tid);
+  int* pos = sh.gj.pv;
+  for (int i = tid; i < nd; i += BIG_NT) pos[i] = -1;
+  __syncthreads();
+  for (int c = tid; c < nc; c += BIG_NT) pos[a.idx[c]] = c;
+  __syncthreads();
+  double xr[BIG_TM][BIG_TM], xi[BIG_TM][BIG_TM];
+  {
+    const BigTile t(nc, nl, tid);
+    big_mul<OPN, OPN>(V, Gm, nc, nl, nl, t, xr, xi, sh, tid);
+    big_tstore(X, nc, nl, t, xr, xi);
+  }
+  __syncthreads();
+  {
+    const BigTile t(nc, nc, tid);
+    big_mul<OPN, OPH>(X, V, nc, nl, nc, t, xr, xi, sh, tid);
+    big_tstore(S, nc, nc, t, xr, xi);
+  }
+  __syncthreads();
+  for (int e = tid; e < nd * nd; e += BIG_NT) {
+    const int i = e / nd, j = e - (e / nd) * nd;
+    const int pi = pos[i], pj = pos[j];
+    out[e] = (pi >= 0 && pj >= 0) ? make_double2(S.re[big_off(pi, pj)], S.im[big_off(pi, pj)])
+                                  : make_double2(0.0, 0.0);
+  }
+}
+
+__global__ void __launch_bounds__(BIG_NT) k_sigma_big(SigmaArgs a, double* pool) {
+  __shared__ BigShared sh;
+  double* slot = pool + (size_t)blockIdx.x * BIG_SLOT;
+  for (int inst = blockIdx.x; inst < a.n_inst; inst += gridDim.x) {
+    big_sigma_one(a, inst, slot, sh, threadIdx.x);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// recursive Green's function sweep for G_1N; slots: g (then Y), H, Q, S = H^H Y
+// ---------------------------------------------------------------------------
+
+__device__ void big_rgf_one(const RgfArgs& a, int inst, double* slot, BigShared& sh, int tid) {
+  if (a.redo && a.status[inst] != 10 + ST_NONFINITE && a.status[inst] != 10 + ST_SINGULAR) return;
+  if (a.status_in && a.status_in[inst] != ST_OK) {
+    if (tid == 0) a.status[inst] = a.status_in[inst];
+    return;
+  }
+  const int n = a.n, nl = a.n_layers;
+  const size_t nn = (size_t)n * n;
+  const size_t set = (size_t)a.set_of[inst];
+  const double2* ons = reinterpret_cast<const double2*>(a.onsite + set * nl * nn * 2);
+  const double* cpl = a.coupling + set * (nl > 1 ? nl - 1 : 0) * nn * 2;
+  const double2* sgl = reinterpret_cast<const double2*>(a.sigma_l + (size_t)inst * nn * 2);
+  const double2* sgr = reinterpret_cast<const double2*>(a.sigma_r + (size_t)inst * nn * 2);
+  const BigTile t(n, n, tid);
+  double zr, zi;
+  zsq(a.omega[inst], a.eta_dev[inst], zr, zi);
+  int status = ST_OK;
+  for (int layer = 0; layer < nl; ++layer) {
+    double* sl = big_fresh(slot);
+    const BigMat Gm = big_mat(sl, 0), H = big_mat(sl, 1), Q = big_mat(sl, 2), S = big_mat(sl, 3);
+    const double2* d = ons + (size_t)layer * nn;
+    for (int e = tid; e < (int)nn; e += BIG_NT) {
+      const int i = e / n, j = e - (e / n) * n;
+      const unsigned o = big_off(i, j);
+      const double2 dv = d[e];
+      double vr = (i == j ? zr : 0.0) - dv.x;
+      double vi = (i == j ? zi : 0.0) - dv.y;
+      if (layer > 0) {
+        vr -= S.re[o];
+        vi -= S.im[o];
+      } else {
+        vr -= 0.0;
+        vi -= 0.0;
+      }
+      if (layer == 0) {
+        vr -= sgl[e].x;
+        vi -= sgl[e].y;
+      }
+      if (layer == nl - 1) {
+        vr -= sgr[e].x;
+        vi -= sgr[e].y;
+      }
+      Gm.re[o] = vr;
+      Gm.im[o] = vi;
+    }
+    __syncthreads();
+    const int st = gj_inverse<double, BIG_NT, BIG, BIG_RG>(Gm.re, Gm.im, BIG_LD, n, sh.gj, tid);
+    if (st != ST_OK) {
+      status = 10 + st;
+      break;
+    }
+    double cr[BIG_TM][BIG_TM], ci[BIG_TM][BIG_TM];
+    if (layer == nl - 1) {
+      // G_1N = Q_{N-1} g_N (g_1 itself for one layer)
+      if (nl == 1) {
+        big_gstore(a.g1n_out + (size_t)inst * nn * 2, n, Gm, tid);
+      } else {
+        big_mul<OPN, OPN>(Q, Gm, n, n, n, t, cr, ci, sh, tid);
+        big_store(H, n, t, cr, ci);
+        __syncthreads();
+        big_gstore(a.g1n_out + (size_t)inst * nn * 2, n, H, tid);
+      }
+      break;
+    }
+    big_gload(cpl + (size_t)layer * nn * 2, n, n, H, tid);
+    __syncthreads();
+    big_mul<OPN, OPN>(Gm, H, n, n, n, t, cr, ci, sh, tid);
+    __syncthreads();  // every read of g is done: Y replaces it (and starts Q at layer 0)
+    big_store(Gm, n, t, cr, ci);
+    if (layer == 0) big_store(Q, n, t, cr, ci);
+    __syncthreads();
+    big_mul<OPH, OPN>(H, Gm, n, n, n, t, cr, ci, sh, tid);
+    big_store(S, n, t, cr, ci);
+    if (layer > 0) {
+      big_mul<OPN, OPN>(Q, Gm, n, n, n, t, cr, ci, sh, tid);
+      __syncthreads();
+      big_store(Q, n, t, cr, ci);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) a.status[inst] = status;
+}
+
+__global__ void __launch_bounds__(BIG_NT) k_rgf_big(RgfArgs a, double* pool) {
+  __shared__ BigShared sh;
+  double* slot = pool + (size_t)blockIdx.x * BIG_SLOT;
+  for (int inst = blockIdx.x; inst < a.n_inst; inst += gridDim.x) {
+    big_rgf_one(a, inst, slot, sh, threadIdx.x);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// T = Re Tr[Gamma_L G Gamma_R G^H] (negf.py:115-116, :356-357); slots: Gamma_L,
+// Gamma_R, G, P = Gamma_L G
+// ---------------------------------------------------------------------------
+
+__device__ void big_trace_one(const TraceArgs& a, int inst, double* slot, BigShared& sh, int tid) {
+  if (a.status_in && a.status_in[inst] != ST_OK) {
+    if (tid == 0) a.t_out[inst] = NAN;
+    return;
+  }
+  const int n = a.n;
+  const int nn = n * n;
+  const double2* sl = reinterpret_cast<const double2*>(a.sigma_l + (size_t)inst * nn * 2);
+  const double2* sr = reinterpret_cast<const double2*>(a.sigma_r + (size_t)inst * nn * 2);
+  double* sb = big_fresh(slot);
+  const BigMat GL = big_mat(sb, 0), GR = big_mat(sb, 1), Gm = big_mat(sb, 2), P = big_mat(sb, 3);
+  for (int e = tid; e < nn; e += BIG_NT) {
+    const int i = e / n, j = e - (e / n) * n;
+    const double2 a1 = sl[i * n + j], a2 = sl[j * n + i];
+    GL.re[big_off(i, j)] = -(a1.y + a2.y);
+    GL.im[big_off(i, j)] = a1.x - a2.x;
+    const double2 b1 = sr[i * n + j], b2 = sr[j * n + i];
+    GR.re[big_off(i, j)] = -(b1.y + b2.y);
+    GR.im[big_off(i, j)] = b1.x - b2.x;
+  }
+  big_gload(a.g1n + (size_t)inst * nn * 2, n, n, Gm, tid);
+  __syncthreads();
+  const BigTile t(n, n, tid);
+  double cr[BIG_TM][BIG_TM], ci[BIG_TM][BIG_TM];
+  // P = Gamma_L G (kept by the owner, read back by the same thread)
+  big_mul<OPN, OPN>(GL, Gm, n, n, n, t, cr, ci, sh, tid);
+  big_store(P, n, t, cr, ci);
+  // Q_ij = (Gamma_R G^H)_ji = sum_k conj(G_ik) Gamma_R_jk
+  big_mul<OPC, OPT>(Gm, GR, n, n, n, t, cr, ci, sh, tid);
+  double acc = 0.0;
+  if (t.own) {
+#pragma unroll
+    for (int x = 0; x < BIG_TM; ++x)
+#pragma unroll
+      for (int y = 0; y < BIG_TM; ++y) {
+        const int i = t.i0 + x, j = t.j0 + y;
+        if (i < n && j < n) acc += P.re[big_off(i, j)] * cr[x][y] - P.im[big_off(i, j)] * ci[x][y];
+      }
+  }
+  const double tot = block_sum<BIG_NT>(acc, sh.sred, tid);
+  if (tid == 0) a.t_out[inst] = tot;
+}
+
+__global__ void __launch_bounds__(BIG_NT) k_trace_big(TraceArgs a, double* pool) {
+  __shared__ BigShared sh;
+  double* slot = pool + (size_t)blockIdx.x * BIG_SLOT;
+  for (int inst = blockIdx.x; inst < a.n_inst; inst += gridDim.x) {
+    big_trace_one(a, inst, slot, sh, threadIdx.x);
+    __syncthreads();
+  }
+}
+
+}  // namespace negf

@@ -315,8 +315,8 @@ __device__ __forceinline__ void crecip1(T ar, T ai, T& rr, T& ri) {
 // x' = 0 in column k, and s the scaled pivot row (s_k = 1 / pivot).  All threads
 // must call it; it ends with a barrier.  Returns ST_OK, ST_SINGULAR or
 // ST_NONFINITE.
-template <typename T, int NT, int NMAX, int RG>
-__device__ __forceinline__ int gj_inverse(T* __restrict__ re, T* __restrict__ im, int ld, int n,
+template <typename T, int NT, int NMAX, int RG, typename PT>
+__device__ __forceinline__ int gj_inverse(PT __restrict__ re, PT __restrict__ im, int ld, int n,
                                           GjBuf<T, NMAX>& b, int tid) {
   static_assert(NMAX % RG == 0, "row groups must tile NMAX");
   static_assert(WAVE % RG == 0, "a column group must sit inside one wave");

@@ -1499,6 +1499,12 @@ __global__ void __launch_bounds__(256) k_heat_current(int n_w, const double* __r
 // host-side dispatch
 // ---------------------------------------------------------------------------
 
+}  // namespace negf
+
+#include "big.hpp"
+
+namespace negf {
+
 static thread_local char g_err[512] = "";
 
 static int fail(int code, const char* fmt, ...) {
@@ -1514,30 +1520,101 @@ static int hip_check(hipError_t e, const char* what) {
   return NEGF_OK;
 }
 
-// smallest configured NMAX >= n for the given element size; 0 if unsupported
 // Block-size class chosen for a block of n DOF: the smallest class >= n unless the
 // autotuner recorded another one for (precision, n).  Entries are read on every
-// launch and written rarely (negf_set_block_class), hence atomics.
-static std::atomic<int> g_class_override[2][65];
+// launch and written rarely (negf_set_block_class), hence atomics.  FP64 classes:
+// 8, 16, 24, 32, 48 (LDS-resident) and 96 (matrices in a device-memory pool);
+// mixed precision: 8 .. 64 (its FP64 refinement bounds it at 48 DOF).
+constexpr int FP64_MAX = 96;
+constexpr int MIXED_MAX = 48;
+constexpr int GREEN_MAX = 48;  // full Green's function and adjoint kernels
+static std::atomic<int> g_class_override[2][FP64_MAX + 1];
 
 static int default_nmax(int n, bool fp64) {
-  const int opts[] = {8, 16, 24, 32, 48, 64};
-  for (int o : opts) {
-    if (n <= o) {
-      if (fp64 && o > 48) return 0;
-      return o;
-    }
-  }
-  return 0;
+  if (n < 1) return 0;
+  const int opts[] = {8, 16, 24, 32, 48};
+  for (int o : opts)
+    if (n <= o) return o;
+  if (fp64) return n <= BIG ? BIG : 0;
+  return n <= 64 ? 64 : 0;
 }
 
 static int pick_nmax(int n, bool fp64) {
-  if (n >= 1 && n <= 64) {
+  if (n >= 1 && n <= FP64_MAX) {
     const int o = g_class_override[fp64 ? 0 : 1][n].load(std::memory_order_relaxed);
     if (o) return o;
   }
   return default_nmax(n, fp64);
 }
+
+// Device-memory pool of the 96 class: one slot per resident workgroup, allocated on
+// first use on a device and kept for the life of the process.  Calls on different
+// streams are ordered through an event, so two calls never use the slots at once.
+struct BigPool {
+  double* ptr = nullptr;
+  int slots = 0;
+  hipEvent_t done = nullptr;
+};
+static std::mutex g_big_mu;
+static BigPool g_big[64];
+
+class BigUse {
+ public:
+  explicit BigUse(hipStream_t st) : st_(st), lk_(g_big_mu) {
+    int dev = 0;
+    rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return;
+    if (dev < 0 || dev >= 64) {
+      rc = fail(NEGF_ERR_ARG, "device index %d out of range", dev);
+      return;
+    }
+    p_ = &g_big[dev];
+    if (!p_->ptr) {
+      int cus = 0, per = 0;
+      rc = hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                     "CU count");
+      if (rc) return;
+      rc = hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                         &per, reinterpret_cast<const void*>(&k_sr_big), BIG_NT, 0),
+                     "occupancy");
+      if (rc) return;
+      const int slots = cus * (per > 0 ? per : 1);
+      const size_t bytes = (size_t)slots * BIG_SLOT * sizeof(double);
+      double* ptr = nullptr;
+      rc = hip_check(hipMalloc(reinterpret_cast<void**>(&ptr), bytes), "96-class pool");
+      if (rc) return;
+      rc = hip_check(hipMemsetAsync(ptr, 0, bytes, st_), "96-class pool clear");
+      if (rc) return;
+      rc = hip_check(hipEventCreateWithFlags(&p_->done, hipEventDisableTiming), "pool event");
+      if (rc) return;
+      p_->ptr = ptr;
+      p_->slots = slots;
+    } else {
+      rc = hip_check(hipStreamWaitEvent(st_, p_->done, 0), "pool order");
+    }
+  }
+  // marks the end of this call's use of the pool on its stream
+  int finish() { return hip_check(hipEventRecord(p_->done, st_), "pool event record"); }
+  double* pool() const { return p_->ptr; }
+  int grid(int n_inst) const { return n_inst < p_->slots ? n_inst : p_->slots; }
+  int rc = NEGF_OK;
+
+ private:
+  hipStream_t st_;
+  std::lock_guard<std::mutex> lk_;
+  BigPool* p_ = nullptr;
+};
+
+#define NEGF_BIG_LAUNCH(KERNEL, STREAM, ARGS)                                                \
+  do {                                                                                      \
+    BigUse u_(STREAM);                                                                      \
+    if (u_.rc) return u_.rc;                                                                \
+    hipLaunchKernelGGL(KERNEL, dim3(u_.grid((ARGS).n_inst)), dim3(BIG_NT), 0, (STREAM),     \
+                       (ARGS), u_.pool());                                                  \
+    const hipError_t e_ = hipGetLastError();                                                \
+    if (e_ != hipSuccess) return hip_check(e_, #KERNEL " launch");                          \
+    return u_.finish();                                                                     \
+  } while (0)
 
 #define NEGF_LAUNCH(KERNEL, GRID, NT, LDS, STREAM, ARGS)                                         \
   do {                                                                                          \
@@ -1569,6 +1646,9 @@ static int run_sr(hipStream_t st, int nmax, const SrArgs& a) {
         NEGF_LAUNCH((k_sancho_rubio<T, 64>), a.n_inst, NT, (Lds<T, 64, NT, 3>::bytes()), st, a);
         return NEGF_OK;
       }
+      break;
+    case BIG:
+      if constexpr (sizeof(T) == 8) NEGF_BIG_LAUNCH(k_sr_big, st, a);
       break;
   }
 #undef SR_CASE
@@ -1606,6 +1686,8 @@ static int run_sigma(hipStream_t st, int nmax, const SigmaArgs& a) {
     SG_CASE(24)
     SG_CASE(32)
     SG_CASE(48)
+    case BIG:
+      NEGF_BIG_LAUNCH(k_sigma_big, st, a);
   }
 #undef SG_CASE
   return fail(NEGF_ERR_ARG, "self-energy block size class %d unsupported", nmax);
@@ -1632,6 +1714,9 @@ static int run_rgf(hipStream_t st, int nmax, const RgfArgs& a) {
         return NEGF_OK;
       }
       break;
+    case BIG:
+      if constexpr (sizeof(T) == 8) NEGF_BIG_LAUNCH(k_rgf_big, st, a);
+      break;
   }
 #undef RG_CASE
   return fail(NEGF_ERR_ARG, "unsupported block size class %d", nmax);
@@ -1650,6 +1735,8 @@ static int run_trace(hipStream_t st, int nmax, const TraceArgs& a) {
     TR_CASE(24)
     TR_CASE(32)
     TR_CASE(48)
+    case BIG:
+      NEGF_BIG_LAUNCH(k_trace_big, st, a);
   }
 #undef TR_CASE
   return fail(NEGF_ERR_ARG, "trace block size class %d unsupported", nmax);
@@ -1705,20 +1792,22 @@ int negf_version(void) { return NEGF_ABI_VERSION; }
 
 const char* negf_last_error(void) { return g_err; }
 
-int negf_max_block(int precision) { return precision == NEGF_FP64 ? 48 : 64; }
+int negf_max_block(int precision) { return precision == NEGF_FP64 ? FP64_MAX : MIXED_MAX; }
+
+int negf_max_block_green(void) { return GREEN_MAX; }
 
 int negf_set_block_class(int precision, int n, int nmax) {
   if (precision != NEGF_FP64 && precision != NEGF_MIXED)
     return fail(NEGF_ERR_ARG, "unknown precision %d", precision);
   const bool fp64 = precision == NEGF_FP64;
-  const int limit = fp64 ? 48 : 64;
+  const int limit = fp64 ? FP64_MAX : 64;
   if (n < 1 || n > limit) return fail(NEGF_ERR_ARG, "block size %d outside 1..%d", n, limit);
   if (nmax != 0) {
     const bool known = nmax == 8 || nmax == 16 || nmax == 24 || nmax == 32 || nmax == 48 ||
-                       nmax == 64;
+                       (fp64 ? nmax == BIG : nmax == 64);
     if (!known || nmax < n || nmax > limit)
       return fail(NEGF_ERR_ARG, "class %d cannot hold blocks of %d (classes 8, 16, 24, 32, "
-                  "48%s)", nmax, n, fp64 ? "" : ", 64");
+                  "48, %d)", nmax, n, fp64 ? BIG : 64);
   }
   g_class_override[fp64 ? 0 : 1][n].store(nmax, std::memory_order_relaxed);
   return NEGF_OK;
@@ -1747,8 +1836,9 @@ int negf_sancho_rubio_batched(void* stream, int precision, int n, int n_inst, co
   if (fp64) {
     rc = run_sr<double>(st, nmax, a);
   } else if (precision == NEGF_MIXED) {
-    const int nm64 = pick_nmax(n, true);
-    if (!nm64) return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 refinement limit", n);
+    if (n > MIXED_MAX)
+      return fail(NEGF_ERR_ARG, "block size %d exceeds the mixed-precision limit of %d", n,
+                  MIXED_MAX);
     rc = run_sr<float>(st, nmax, a);
     if (rc == NEGF_OK && refine_steps >= 0) {
       rc = negf_surface_refine_batched(stream, n, n_inst, lead, set_of, omega, eta, refine_steps,
@@ -1770,7 +1860,8 @@ int negf_surface_refine_batched(void* stream, int n, int n_inst, const double* l
     return fail(NEGF_ERR_ARG, "null pointer argument");
   if (n_inst == 0) return NEGF_OK;
   const int nm64 = pick_nmax(n, true);
-  if (!nm64) return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 refinement limit", n);
+  if (!nm64 || nm64 > MIXED_MAX)
+    return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 refinement limit of %d", n, MIXED_MAX);
   RefineArgs r{n, n_inst, lead, set_of, omega, eta, steps, NEGF_REDO_TOL, g_io, dos_out, status};
   return run_refine(reinterpret_cast<hipStream_t>(stream), nm64, r);
 }
@@ -1797,7 +1888,7 @@ int negf_self_energy_batched(void* stream, int n_lead, int n_c, int n_dev, int n
   if (n_inst == 0) return NEGF_OK;
   int m = n_lead > n_dev ? n_lead : n_dev;
   const int nmax = pick_nmax(m, true);
-  if (!nmax) return fail(NEGF_ERR_ARG, "self-energy block %d exceeds 48", m);
+  if (!nmax) return fail(NEGF_ERR_ARG, "self-energy block %d exceeds %d", m, FP64_MAX);
   SigmaArgs a{n_lead, n_c, n_dev, n_inst, v, set_of, g, idx, status_in, sigma_out};
   return run_sigma(reinterpret_cast<hipStream_t>(stream), nmax, a);
 }
@@ -1822,7 +1913,9 @@ int negf_rgf_g1n_batched(void* stream, int precision, int n, int n_layers, int n
   if (fp64) return run_rgf<double>(st, nmax, a);
   if (precision == NEGF_MIXED) {
     const int nm64 = pick_nmax(n, true);
-    if (!nm64) return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 limit", n);
+    if (n > MIXED_MAX)
+      return fail(NEGF_ERR_ARG, "block size %d exceeds the mixed-precision limit of %d", n,
+                  MIXED_MAX);
     int rc = run_rgf<float>(st, nmax, a);
     if (rc != NEGF_OK) return rc;
     a.redo = 1;  // FP32 sweep failures (non-finite / singular) are recomputed in FP64
@@ -1842,7 +1935,9 @@ int negf_device_green_batched(void* stream, int n, int n_layers, int n_inst, con
     return fail(NEGF_ERR_ARG, "null pointer argument");
   if (n_inst == 0) return NEGF_OK;
   const int nmax = pick_nmax(n, true);
-  if (!nmax) return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 limit of 48", n);
+  if (!nmax || nmax > GREEN_MAX)
+    return fail(NEGF_ERR_ARG, "block size %d exceeds the full Green's function limit of %d", n,
+                GREEN_MAX);
   DgfArgs a{n, n_layers, n_inst, onsite, coupling, set_of, omega, eta_dev, sigma_l, sigma_r, g_out,
             status};
   return run_dgf(reinterpret_cast<hipStream_t>(stream), nmax, a);
@@ -1870,7 +1965,8 @@ int negf_adjoint_batched(void* stream, int n, int n_layers, int n_inst, const do
     return fail(NEGF_ERR_ARG, "workspace of %zu bytes is smaller than the %zu needed",
                 workspace_bytes, need);
   const int nmax = pick_nmax(n, true);
-  if (!nmax) return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 limit of 48", n);
+  if (!nmax || nmax > GREEN_MAX)
+    return fail(NEGF_ERR_ARG, "block size %d exceeds the adjoint limit of %d", n, GREEN_MAX);
   AdjArgs a{n, n_layers, n_inst, onsite, coupling, set_of, omega, eta_dev, sigma_l, sigma_r,
             status_in, weight, reinterpret_cast<double*>(workspace), t_out, grad_onsite,
             grad_coupling, status};
@@ -1935,7 +2031,7 @@ int negf_transmission_trace_batched(void* stream, int n, int n_inst, const doubl
   if (!sigma_l || !sigma_r || !g1n || !t_out) return fail(NEGF_ERR_ARG, "null pointer argument");
   if (n_inst == 0) return NEGF_OK;
   const int nmax = pick_nmax(n, true);
-  if (!nmax) return fail(NEGF_ERR_ARG, "trace block %d exceeds 48", n);
+  if (!nmax) return fail(NEGF_ERR_ARG, "trace block %d exceeds %d", n, FP64_MAX);
   TraceArgs a{n, n_inst, sigma_l, sigma_r, g1n, status_in, t_out};
   return run_trace(reinterpret_cast<hipStream_t>(stream), nmax, a);
 }

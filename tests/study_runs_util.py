@@ -29,14 +29,26 @@ def load_run(name: str, tmp: Path):
 
 
 # The reference solved these runs with the eigenmode route (generalized_eigen_svd),
-# this path with the decimation route; on the stanene and phosphorene models the two
+# this path with the decimation route; on the stanene, phosphorene and borophene (y)
+# models the two
 # reference routes differ by up to the bound below (the eigenmode route's surface
 # GF solves the surface Riccati equation 100x less accurately at the worst k point,
 # pinned by test_ifc_pipeline.py).  Other runs agree to 1e-9.
-METHOD_SPREAD = {"stanene": 2e-8, "phosphorene": 1e-7}
+METHOD_SPREAD = {"stanene": 2e-8, "phosphorene": 1e-7, "borophene_b_2x1_pbe_transmission_y": 2e-8}
 # (run, omega index) of the largest route spread of each material
 WORST = (("20260226_030629Z_stanene_1l_orthogonal_transmission", 1),
-         ("20260226_032511Z_phosphorene_p_zerostrain_transmission_y_area", 103))
+         ("20260226_032511Z_phosphorene_p_zerostrain_transmission_y_area", 103),
+         ("20260225_083830Z_borophene_b_2x1_pbe_transmission_y", 98))
+
+
+# Runs where the reference's decimation route itself fails at the smallest eta for
+# some k points (the decimation diverges: its alpha/beta blow up from rounding along
+# strongly growing modes) while the eigenmode route the study used does not.  This
+# path follows the decimation route, so at those omegas the failing k points take a
+# larger eta, as the reference's sancho_rubio route would.  The omegas are the ones
+# whose smallest-eta acceptance count falls below the file's; the listed indices are
+# pinned on the CPU (test_ifc_pipeline.py::test_study_points_where_decimation_fails).
+SR_UNSTABLE = {"20260226_035209Z_borophene_b_beta12_transmission_area_y": (1, 88)}
 
 
 def spectrum_bound(name: str) -> float:

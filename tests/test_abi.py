@@ -24,8 +24,9 @@ def test_library_exports_every_declared_symbol():
     for name in declared_functions():
         assert hasattr(lib, name), name
     assert lib.negf_version() == _lib.ABI_VERSION
-    assert lib.negf_max_block(_lib.FP64) == 48
-    assert lib.negf_max_block(_lib.MIXED) == 64
+    assert lib.negf_max_block(_lib.FP64) == 96
+    assert lib.negf_max_block(_lib.MIXED) == 48
+    assert lib.negf_max_block_green() == 48
 
 
 def test_problem_struct_layout():

@@ -166,9 +166,12 @@ def test_block_class_table_on_the_host():
     assert lib.negf_block_class(fp64, 20) == 32
     assert lib.negf_block_class(mixed, 20) == 24          # per precision
     assert lib.negf_set_block_class(fp64, 20, 16) != 0    # class smaller than the block
-    assert lib.negf_set_block_class(fp64, 20, 64) != 0    # FP64 stops at 48
+    assert lib.negf_set_block_class(fp64, 20, 64) != 0    # 64 is a mixed-precision class only
     assert lib.negf_set_block_class(fp64, 20, 40) != 0    # not a class
-    assert lib.negf_set_block_class(fp64, 49, 0) != 0     # block too large for FP64
+    assert lib.negf_set_block_class(fp64, 97, 0) != 0     # block too large for FP64
+    assert lib.negf_block_class(fp64, 60) == 96 and lib.negf_block_class(fp64, 96) == 96
+    assert lib.negf_block_class(fp64, 97) == 0
+    assert lib.negf_set_block_class(mixed, 20, 96) != 0   # 96 is an FP64 class only
     assert lib.negf_block_class(fp64, 20) == 32           # failed calls change nothing
     assert lib.negf_set_block_class(fp64, 20, 0) == 0
     assert lib.negf_block_class(fp64, 20) == 24

@@ -176,7 +176,7 @@ def test_study_run_set_up_matches_reference_file(name, tmp_path):
     assert len(km["high"]["kpoints"]) > 0 and len(km["low"]["kpoints"]) > 0
 
 
-from study_runs_util import WORST, oracle_kavg  # noqa: E402
+from study_runs_util import SR_UNSTABLE, WORST, oracle_kavg  # noqa: E402
 
 
 def _worst_setup(name, idx, tmp_path):
@@ -196,10 +196,11 @@ def test_study_method_spread_is_the_eigen_route_error(name, idx, tmp_path):
     """Where the decimation path departs from a study file most, the file is the
     eigenmode route's answer (the oracle's eigenmode route gives it to 1e-11) and that
     route is the less accurate one: its surface GF leaves a Riccati residual at least
-    10x the decimation's at the k point where the two differ most."""
+    10x the decimation's at the k point where the two differ most.  (The eigenmode
+    route itself varies by ~1e-11 between LAPACK builds.)"""
     m, ks, w, t_file = _worst_setup(name, idx, tmp_path)
     t_ge, leads = oracle_kavg(m, ks, w, 1e-8, "generalized_eigen_svd")
-    assert abs(t_ge - t_file) <= 1e-11 * max(abs(t_file), 1e-3)
+    assert abs(t_ge - t_file) <= 3e-11 * max(abs(t_file), 1e-3)
     t_sr, _ = oracle_kavg(m, ks, w, 1e-8, "sancho_rubio")
     assert abs(t_sr - t_file) > 1e-9 * max(abs(t_file), 1e-3)
     gaps = []
@@ -211,3 +212,28 @@ def test_study_method_spread_is_the_eigen_route_error(name, idx, tmp_path):
     r_sr = o.riccati_residual(w, ll[0], ll[1], ll[2], 1e-8, g_sr)
     r_ge = o.riccati_residual(w, ll[0], ll[1], ll[2], 1e-8, g_ge)
     assert r_ge >= 10.0 * r_sr, (r_sr, r_ge)
+
+
+@pytest.mark.parametrize("name,idx", [(k, i) for k, v in SR_UNSTABLE.items() for i in v])
+def test_study_points_where_decimation_fails(name, idx, tmp_path):
+    """At these points the reference's own decimation route raises (numerically
+    unstable / non-finite) for some k points at the smallest eta, while the eigenmode route the
+    study used accepts every k point and reproduces the file."""
+    cfg, rows, _, diag = load_run(name, tmp_path)
+    m = W.build_transport_model(cfg)
+    km = W.prepare_kmesh(cfg)
+    w = float(W.build_omega_grid(cfg, m["ifc"])[0][idx])
+    c = float(np.asarray(W.qe_omega_to_cm1(w)))
+    ks = km["low" if c <= km["low_cm1"] else "high"]["kpoints"]
+    rec = [r for r in diag["eta_records"] if r["omega_qe"] == pytest.approx(w, rel=1e-14)][0]
+    assert rec["eta_histogram"]["1e-08"] == len(ks)
+    fails = 0
+    for k in ks:
+        try:
+            o.sancho_rubio(w, *m["lead_left"].blocks_builder(k), eta=1e-8)
+        except RuntimeError as exc:
+            assert "Sancho-Rubio decimation" in str(exc)   # unstable or non-finite
+            fails += 1
+    assert fails > 0
+    t_ge, _ = oracle_kavg(m, ks, w, 1e-8, "generalized_eigen_svd")
+    assert abs(t_ge - rows[idx, 3]) <= 3e-11 * max(abs(rows[idx, 3]), 1e-3)
