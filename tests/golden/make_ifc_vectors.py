@@ -12,7 +12,8 @@ conventions the end-to-end spectra alone do not isolate:
   (``_build_interface_contacts``), and the device blocks there;
 * the k meshes (``_prepare_kmesh``) and the omega grid, including the eigenvalue
   estimate of the maximum frequency (``_estimate_omega_max_from_ifc``);
-* per-k transmissions through the explicit contacts at fixed eta.
+* per-k transmissions through the explicit contacts at fixed eta;
+* the phonopy reader's primitive-cell terms, masses and cell for the TaN study.
 
 Output (data only): ``ifc_vectors.npz`` next to this file.
 """
@@ -92,6 +93,13 @@ def main(ref_src: str) -> None:
                                      for w in ws] for k in sel])
         grid, info = wb._build_omega_grid(cfg, ifc)
         out[f"{tag}_omega_grid"] = np.asarray(grid, dtype=float)
+    # the phonopy reader on the TaN study's supercell force constants
+    tan = read_ifc(Path(ref_src).parent / "studies" / "TaN" / "FORCE_CONSTANTS", reader="phonopy")
+    out["tan_masses"] = np.asarray(tan.masses, dtype=float)
+    out["tan_keys"] = np.array([[t.dx, t.dy, t.dz] for t in tan.terms], dtype=np.int64)
+    out["tan_blocks"] = np.stack([np.asarray(t.block) for t in tan.terms])
+    out["tan_lattice"] = np.asarray(tan.lattice_vectors, dtype=float)
+    out["tan_positions"] = np.asarray(tan.atom_positions, dtype=float)
     # the eigenvalue estimate of the maximum frequency on a small k grid
     ifc = read_ifc(STUDY / "si444.fc", reader="qe")
     for centered in (False, True):

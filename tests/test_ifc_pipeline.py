@@ -214,7 +214,7 @@ def test_study_method_spread_is_the_eigen_route_error(name, idx, tmp_path):
     assert r_ge >= 10.0 * r_sr, (r_sr, r_ge)
 
 
-@pytest.mark.parametrize("name,idx", [(k, i) for k, v in SR_UNSTABLE.items() for i in v])
+@pytest.mark.parametrize("name,idx", [(k, v[0]) for k, v in SR_UNSTABLE.items()])
 def test_study_points_where_decimation_fails(name, idx, tmp_path):
     """At these points the reference's own decimation route raises (numerically
     unstable / non-finite) for some k points at the smallest eta, while the eigenmode route the
@@ -237,3 +237,41 @@ def test_study_points_where_decimation_fails(name, idx, tmp_path):
     assert fails > 0
     t_ge, _ = oracle_kavg(m, ks, w, 1e-8, "generalized_eigen_svd")
     assert abs(t_ge - rows[idx, 3]) <= 3e-11 * max(abs(rows[idx, 3]), 1e-3)
+
+
+def test_atomic_mass_table_matches_reference():
+    import json as _json
+
+    from negfpy_amd.modeling.atomic_masses import ATOMIC_MASS_AMU, get_atomic_mass_amu
+
+    ref = _json.loads((HERE / "golden" / "atomic_masses.json").read_text())
+    assert ATOMIC_MASS_AMU == ref
+    assert get_atomic_mass_amu(" ta ") == 180.94788
+    with pytest.raises(ValueError, match="Unknown atomic symbol"):
+        get_atomic_mass_amu("Xx")
+
+
+def test_phonopy_reader_matches_reference(tmp_path):
+    from study_runs_util import load_run as _load
+
+    cfg, _, _, _ = _load("20260225_113114Z_tan_transmission_x", tmp_path)
+    ifc = read_ifc(cfg["ifc"]["path"], reader="phonopy")
+    np.testing.assert_array_equal(ifc.masses, IV["tan_masses"])
+    np.testing.assert_array_equal(np.array([[t.dx, t.dy, t.dz] for t in ifc.terms]), IV["tan_keys"])
+    np.testing.assert_array_equal(np.stack([t.block for t in ifc.terms]), IV["tan_blocks"])
+    np.testing.assert_array_equal(ifc.lattice_vectors, IV["tan_lattice"])
+    np.testing.assert_array_equal(ifc.atom_positions, IV["tan_positions"])
+    assert ifc.atom_symbols == ("N", "Ta") and ifc.metadata["supercell_repeats"] == (4, 4, 4)
+
+
+def test_phonopy_reader_errors(tmp_path):
+    from negfpy_amd.io import get_reader, list_readers
+
+    assert list_readers() == ("phonopy", "qe")
+    with pytest.raises(KeyError, match="Unknown IFC reader"):
+        get_reader("castep")
+    (tmp_path / "FORCE_CONSTANTS").write_text("2 2\n")
+    with pytest.raises(FileNotFoundError, match="POSCAR"):
+        read_ifc(tmp_path / "FORCE_CONSTANTS", reader="phonopy")
+    with pytest.raises(ValueError, match="Unsupported phonopy source dict"):
+        read_ifc({"foo": 1}, reader="phonopy")
