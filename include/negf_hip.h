@@ -37,6 +37,7 @@ extern "C" {
 #define NEGF_REDO_TOL 1e-6
 
 enum { NEGF_FP64 = 0, NEGF_MIXED = 1 };
+enum { NEGF_SURFACE_DECIMATION = 0, NEGF_SURFACE_EIGEN = 1 };
 enum { NEGF_OK = 0, NEGF_ERR_ARG = 1, NEGF_ERR_HIP = 2 };
 
 /* ABI version (NEGF_ABI_VERSION). */
@@ -212,7 +213,22 @@ typedef struct negf_transmission_problem {
   int32_t* iters_l;       /* [n_inst] decimation steps, left lead */
   int32_t* iters_r;       /* [n_inst] decimation steps, right lead (unused when shared) */
   int32_t* status;        /* [n_inst] */
+  int surface_route;      /* NEGF_SURFACE_DECIMATION, or NEGF_SURFACE_EIGEN: lead instances whose
+                             decimation fails take the eigenmode route (blocks <= 48) */
 } negf_transmission_problem;
+
+/*
+ * Eigenmode surface GF for every instance whose status is nonzero (after
+ * negf_sancho_rubio_batched): the transfer matrix of the n finite eigenmodes of
+ * smallest |lambda| of the lead's quadratic eigenproblem, as the reference's
+ * generalized_eigen[_svd] routes select them (src/negfpy/core/surface_gf.py:84-199),
+ * from a spectral projector (matrix sign function, no eigensolver).  On success g,
+ * the DOS and status (0) are written; instances it cannot do keep their status.
+ * Blocks of n <= 48 (the pencil is 2n); larger blocks are left untouched.
+ */
+int negf_surface_gf_eigen_batched(void* stream, int n, int n_inst, const double* lead,
+                                  const int32_t* set_of, const double* omega, const double* eta,
+                                  double* g_io, double* dos_out, int32_t* status);
 
 /* Device workspace needed by negf_transmission_batched. */
 size_t negf_transmission_workspace_bytes(const negf_transmission_problem* p);

@@ -23,6 +23,8 @@ ABI_VERSION = 1
 
 FP64 = 0
 MIXED = 1
+SURFACE_DECIMATION = 0
+SURFACE_EIGEN = 1
 
 OK = 0
 ST_OK = 0
@@ -49,7 +51,7 @@ class TransmissionProblem(ctypes.Structure):
         ("idx_l", _vp), ("idx_r", _vp), ("onsite", _vp), ("coupling", _vp),
         ("set_of", _vp), ("omega", _vp), ("eta_lead", _vp), ("eta_dev", _vp),
         ("t_out", _vp), ("dos_l", _vp), ("dos_r", _vp),
-        ("iters_l", _vp), ("iters_r", _vp), ("status", _vp),
+        ("iters_l", _vp), ("iters_r", _vp), ("status", _vp), ("surface_route", _c_int),
     ]
 
 
@@ -63,6 +65,8 @@ _SIGS = {
     "negf_sancho_rubio_batched": (
         [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_dbl, _c_int, _c_int, _vp, _vp, _vp, _vp],
         _c_int),
+    "negf_surface_gf_eigen_batched": (
+        [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
     "negf_surface_refine_batched": (
         [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_int, _vp, _vp, _vp], _c_int),
     "negf_sancho_rubio_redo": (

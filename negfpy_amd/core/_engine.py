@@ -64,8 +64,10 @@ class SurfaceResult:
 
 
 def surface_gf_batch(leads, set_of, omega, eta, *, tol=1e-12, maxiter=200, precision="fp64",
-                     refine_steps=DEFAULT_REFINE_STEPS) -> SurfaceResult:
-    """Decimate every instance's lead.  ``leads`` is (n_sets, 3, n, n) = (d00, d01, d10)."""
+                     refine_steps=DEFAULT_REFINE_STEPS, route=0) -> SurfaceResult:
+    """Decimate every instance's lead.  ``leads`` is (n_sets, 3, n, n) = (d00, d01, d10).
+    With ``route`` = SURFACE_EIGEN, instances whose decimation fails take the
+    eigenmode route (blocks up to 48)."""
     lib = _lib.load()
     leads = to_dev_c128(leads)
     if leads.ndim != 4 or leads.shape[1] != 3 or leads.shape[2] != leads.shape[3]:
@@ -86,6 +88,10 @@ def surface_gf_batch(leads, set_of, omega, eta, *, tol=1e-12, maxiter=200, preci
         _lib.stream(), precision_code(precision), n, n_inst, leads.data_ptr(), set_of.data_ptr(),
         omega.data_ptr(), eta.data_ptr(), float(tol), int(maxiter), int(refine_steps),
         g.data_ptr(), dos.data_ptr(), iters.data_ptr(), status.data_ptr()))
+    if route == _lib.SURFACE_EIGEN:
+        _lib.check(lib.negf_surface_gf_eigen_batched(
+            _lib.stream(), n, n_inst, leads.data_ptr(), set_of.data_ptr(), omega.data_ptr(),
+            eta.data_ptr(), g.data_ptr(), dos.data_ptr(), status.data_ptr()))
     return SurfaceResult(g=g, dos=dos, iters=iters, status=status)
 
 
@@ -149,9 +155,10 @@ class TransmissionSolver:
     """Reusable batched solver: workspace is kept between calls of equal size."""
 
     def __init__(self, sets: TransportSets, *, precision="fp64", tol=1e-12, maxiter=200,
-                 refine_steps=DEFAULT_REFINE_STEPS):
+                 refine_steps=DEFAULT_REFINE_STEPS, surface_route=0):
         self.sets = sets
         self.precision = precision_code(precision)
+        self.surface_route = int(surface_route)
         self.tol = float(tol)
         self.maxiter = int(maxiter)
         self.refine_steps = int(refine_steps)
@@ -170,6 +177,7 @@ class TransmissionSolver:
         p.n_c_l = int(s.v_l.shape[1])
         p.n_c_r = int(s.v_r.shape[1])
         p.share_leads = 1 if s.share_leads else 0
+        p.surface_route = self.surface_route
         p.maxiter = self.maxiter
         p.refine_steps = self.refine_steps
         p.tol = self.tol
@@ -289,6 +297,11 @@ class TransmissionSolver:
                           st, nn, n_inst, lead.data_ptr(), set_of.data_ptr(), omega.data_ptr(),
                           eta_lead.data_ptr(), self.tol, self.maxiter, g.data_ptr(),
                           dos.data_ptr(), iters.data_ptr(), stt.data_ptr()))
+            if self.surface_route == _lib.SURFACE_EIGEN:
+                stage(f"eigen_{side}", lambda lead=lead, g=g, dos=dos, stt=stt, nn=nn:
+                      lib.negf_surface_gf_eigen_batched(
+                          st, nn, n_inst, lead.data_ptr(), set_of.data_ptr(), omega.data_ptr(),
+                          eta_lead.data_ptr(), g.data_ptr(), dos.data_ptr(), stt.data_ptr()))
         if s.share_leads:
             st_r.copy_(st_l)
         stage("self_energy_l", lambda: lib.negf_self_energy_batched(
@@ -347,7 +360,8 @@ class TransmissionGraph:
         return self.out
 
 
-def lead_self_energies(sets: TransportSets, set_of, omega, eta_lead, *, tol=1e-12, maxiter=200):
+def lead_self_energies(sets: TransportSets, set_of, omega, eta_lead, *, tol=1e-12, maxiter=200,
+                       route=0):
     """FP64 decimation of both leads and Sigma = V g V^H embedded in the end layers.
 
     Returns (Sigma_L, Sigma_R [n_inst, n, n], lead status [n_inst]) on the GPU.
@@ -370,6 +384,10 @@ def lead_self_energies(sets: TransportSets, set_of, omega, eta_lead, *, tol=1e-1
             st, _lib.FP64, nld, n_inst, lead.data_ptr(), set_of.data_ptr(), omega.data_ptr(),
             eta_lead.data_ptr(), float(tol), int(maxiter), 0, g.data_ptr(), dos.data_ptr(),
             it.data_ptr(), stt.data_ptr()))
+        if route == _lib.SURFACE_EIGEN:
+            _lib.check(lib.negf_surface_gf_eigen_batched(
+                st, nld, n_inst, lead.data_ptr(), set_of.data_ptr(), omega.data_ptr(),
+                eta_lead.data_ptr(), g.data_ptr(), dos.data_ptr(), stt.data_ptr()))
         sg = torch.zeros((n_inst, n, n), dtype=c128, device=dev)
         _lib.check(lib.negf_self_energy_batched(
             st, nld, int(v.shape[1]), n, n_inst, v.data_ptr(), set_of.data_ptr(), g.data_ptr(),
@@ -381,7 +399,7 @@ def lead_self_energies(sets: TransportSets, set_of, omega, eta_lead, *, tol=1e-1
 
 
 def device_green(sets: TransportSets, set_of, omega, eta_lead, eta_dev, *, tol=1e-12,
-                 maxiter=200):
+                 maxiter=200, route=0):
     """Full device Green's function of every instance (FP64).
 
     Returns (G [n_inst, Ln, Ln], Sigma_L, Sigma_R [n_inst, n, n] end-layer blocks,
@@ -395,7 +413,7 @@ def device_green(sets: TransportSets, set_of, omega, eta_lead, eta_dev, *, tol=1
     n_inst = int(omega.numel())
     n, nl = sets.n, sets.n_layers
     sig_l, sig_r, lead_status = lead_self_energies(sets, set_of, omega, eta_lead, tol=tol,
-                                                   maxiter=maxiter)
+                                                   maxiter=maxiter, route=route)
     dev = sets.onsite.device
     gfull = torch.zeros((n_inst, n * nl, n * nl), dtype=torch.complex128, device=dev)
     dst = torch.zeros((n_inst,), dtype=torch.int32, device=dev)
@@ -415,7 +433,7 @@ class AdjointResult:
 
 
 def adjoint(sets: TransportSets, set_of, omega, eta_lead, eta_dev, weight=None, *, tol=1e-12,
-            maxiter=200) -> AdjointResult:
+            maxiter=200, route=0) -> AdjointResult:
     """T and weight * dT/d(device blocks) of every instance (FP64 adjoint kernel)."""
     lib = _lib.load()
     set_of = to_dev_i32(set_of)
@@ -425,7 +443,7 @@ def adjoint(sets: TransportSets, set_of, omega, eta_lead, eta_dev, weight=None, 
     n_inst = int(omega.numel())
     n, nl = sets.n, sets.n_layers
     sig_l, sig_r, lead_status = lead_self_energies(sets, set_of, omega, eta_lead, tol=tol,
-                                                   maxiter=maxiter)
+                                                   maxiter=maxiter, route=route)
     dev = sets.onsite.device
     w = None if weight is None else to_dev_f64(weight)
     ws_bytes = int(lib.negf_adjoint_workspace_bytes(n, nl, n_inst))

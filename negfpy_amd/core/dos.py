@@ -14,7 +14,7 @@ import numpy as np
 from . import _engine
 from ._status import raise_for_lead_status
 from .negf import _resolve_lead_blocks
-from .surface_gf import _prec, canonical_method, pack_lead
+from .surface_gf import _prec, canonical_method, pack_lead, surface_route
 from .types import KPar, LeadBlocks, LeadKSpace
 
 Array = np.ndarray
@@ -28,11 +28,11 @@ def _lead_sets(lead: LeadLike, kpoints):
 
 def _dos_points(lead, kpoints, omegas, etas, normalize_per_mode, method, precision):
     """DOS and status for every (omega_i, k_i, eta_i) triple (k given per point)."""
-    canonical_method(method)
+    route = surface_route(method)
     uniq = list(dict.fromkeys(range(len(kpoints))))
     blocks, sets = _lead_sets(lead, [kpoints[i] for i in uniq])
     res = _engine.surface_gf_batch(sets, np.arange(len(kpoints)), omegas, etas,
-                                   precision=_prec(precision))
+                                   precision=_prec(precision), route=route)
     dos = res.dos.cpu().numpy()
     st = res.status.cpu().numpy()
     if normalize_per_mode:
@@ -137,12 +137,12 @@ def lead_surface_dos_spectrum(omegas: Iterable[float], lead: LeadLike, *,
         if len(ks) == 0:
             raise ValueError("kpoints must contain at least one k-point.")
         nk = len(ks)
-        canonical_method(surface_gf_method)
+        route = surface_route(surface_gf_method)
         blocks, sets = _lead_sets(lead, ks)
         om = np.repeat(w, nk)
         kk = np.tile(np.arange(nk), w.size)
         res = _engine.surface_gf_batch(sets, kk, om, np.full(om.size, float(eta)),
-                                       precision=_prec(precision))
+                                       precision=_prec(precision), route=route)
         st = res.status.cpu().numpy()
         dos = res.dos.cpu().numpy()
         for c in st:

@@ -29,7 +29,7 @@ import numpy as np
 
 from . import _engine
 from ._status import transmission_value
-from .surface_gf import _prec, canonical_method, pack_lead
+from .surface_gf import _prec, canonical_method, pack_lead, surface_route
 from .types import Device1D, DeviceKSpace, KPar, LeadBlocks, LeadKSpace
 
 Array = np.ndarray
@@ -207,7 +207,7 @@ def resolve_k(device, lead_left, lead_right, kpar, *, eta, eta_device, device_to
 
 
 def solve_points(kps: list[KProblem], k_of: Array, omega: Array, eta: Array, eta_dev: Array,
-                 *, precision: str, tol: float = 1e-12, maxiter: int = 200):
+                 *, precision: str, tol: float = 1e-12, maxiter: int = 200, route: int = 0):
     """Transmission of every instance (k_of[i] indexes ``kps``).  Returns (T, status) arrays.
 
     Instances are grouped by block layout; each group is one batched launch.
@@ -235,7 +235,8 @@ def solve_points(kps: list[KProblem], k_of: Array, omega: Array, eta: Array, eta
             kg[0].pos_l, kg[0].pos_r, np.stack([kp.onsite for kp in kg]),
             np.stack([kp.coupling for kp in kg]) if kg[0].onsite.shape[0] > 1 else None, share)
         scale = np.array([kps[k].scale for k in k_of[sel]])
-        solver = _engine.TransmissionSolver(sets, precision=precision, tol=tol, maxiter=maxiter)
+        solver = _engine.TransmissionSolver(sets, precision=precision, tol=tol, maxiter=maxiter,
+                                            surface_route=route)
         res = solver.run(np.array([local[k] for k in k_of[sel]], dtype=np.int32),
                          omega[sel] / scale, eta[sel] / scale, eta_dev[sel] / scale)
         t_all[sel] = res.t.cpu().numpy()
@@ -283,7 +284,7 @@ def device_green_function(omega: float, device: DeviceLike, lead_left: LeadLike,
     """
     from ._status import raise_for_lead_status
 
-    canonical_method(surface_gf_method)
+    route = surface_route(surface_gf_method)
     kp = resolve_k(device, lead_left, lead_right, kpar, eta=eta, eta_device=eta_device,
                    device_to_lead_left=device_to_lead_left,
                    device_to_lead_right=device_to_lead_right,
@@ -295,7 +296,8 @@ def device_green_function(omega: float, device: DeviceLike, lead_left: LeadLike,
         kp.onsite[None], kp.coupling[None] if kp.onsite.shape[0] > 1 else None, False)
     eta_dev = float(eta if eta_device is None else eta_device)
     g, sl, sr, lst, dst = _engine.device_green(sets, [0], [float(omega) / kp.scale],
-                                               [float(eta) / kp.scale], [eta_dev / kp.scale])
+                                               [float(eta) / kp.scale], [eta_dev / kp.scale],
+                                               route=route)
     raise_for_lead_status(int(lst[0].item()))
     code = int(dst[0].item())
     if code == 13:
@@ -320,7 +322,7 @@ def transmission(omega: float, device: DeviceLike, lead_left: LeadLike, lead_rig
                  surface_gf_method: str = "sancho_rubio", omega_scale: float | None = None, *,
                  precision: str | None = None) -> float:
     """Coherent phonon transmission T(omega) (negf.py:306-357)."""
-    canonical_method(surface_gf_method)
+    route = surface_route(surface_gf_method)
     prec = _prec(precision)
     kp = _resolve_many(device, lead_left, lead_right, [kpar], eta=eta, eta_device=eta_device,
                        device_to_lead_left=device_to_lead_left,
@@ -329,7 +331,8 @@ def transmission(omega: float, device: DeviceLike, lead_left: LeadLike, lead_rig
                        contact_right_indices=contact_right_indices, omega_scale=omega_scale,
                        precision=prec)
     eta_dev = float(eta if eta_device is None else eta_device)
-    t, st = solve_points(kp, [0], [float(omega)], [float(eta)], [eta_dev], precision=prec)
+    t, st = solve_points(kp, [0], [float(omega)], [float(eta)], [eta_dev], precision=prec,
+                         route=route)
     return transmission_value(int(st[0]), float(t[0]))
 
 
@@ -393,11 +396,11 @@ def kspace_sets(device, lead_left, lead_right, kpoints, *, device_to_lead_left=N
         kb.coupling if n_layers > 1 else None, share)
 
 
-def _solve_sets(sets, omega, eta, eta_dev, precision):
+def _solve_sets(sets, omega, eta, eta_dev, precision, route=0):
     """(T, status) of instance i = k point i of ``sets``."""
     nk = sets.n_sets
     om = np.broadcast_to(np.asarray(omega, dtype=float), (nk,))
-    solver = _engine.TransmissionSolver(sets, precision=precision)
+    solver = _engine.TransmissionSolver(sets, precision=precision, surface_route=route)
     res = solver.run(np.arange(nk, dtype=np.int32), om, np.full(nk, float(eta)),
                      np.full(nk, float(eta_dev)))
     return res.t.cpu().numpy(), res.status.cpu().numpy()
@@ -415,7 +418,7 @@ def transmission_kavg(omega: float, device: DeviceLike, lead_left: LeadLike, lea
     """k_par-averaged transmission (negf.py:360-397): all k points in one batch."""
     if len(kpoints) == 0:
         raise ValueError("kpoints must contain at least one k-point.")
-    canonical_method(surface_gf_method)
+    route = surface_route(surface_gf_method)
     prec = _prec(precision)
     eta_dev = float(eta if eta_device is None else eta_device)
     if eta_dev < 0.0:
@@ -428,7 +431,7 @@ def transmission_kavg(omega: float, device: DeviceLike, lead_left: LeadLike, lea
                        precision=prec)
     if sets is not None:
         nk = sets.n_sets
-        t, st = _solve_sets(sets, float(omega), eta, eta_dev, prec)
+        t, st = _solve_sets(sets, float(omega), eta, eta_dev, prec, route)
     else:
         kps = _resolve_many(device, lead_left, lead_right, list(kpoints), eta=eta,
                             eta_device=eta_device, device_to_lead_left=device_to_lead_left,
@@ -438,7 +441,8 @@ def transmission_kavg(omega: float, device: DeviceLike, lead_left: LeadLike, lea
                             omega_scale=omega_scale, precision=prec)
         nk = len(kps)
         t, st = solve_points(kps, np.arange(nk), np.full(nk, float(omega)),
-                             np.full(nk, float(eta)), np.full(nk, eta_dev), precision=prec)
+                             np.full(nk, float(eta)), np.full(nk, eta_dev), precision=prec,
+                             route=route)
     vals = [transmission_value(int(st[i]), float(t[i])) for i in range(nk)]
     return float(np.mean(vals))
 
@@ -500,8 +504,10 @@ class _KSolver:
     the host (a k point whose resolution raised is None: each of its attempts counts as
     an "exception", as in negf.py:463/612)."""
 
-    def __init__(self, device, lead_left, lead_right, kpoints, *, eta_device, precision, **kw):
+    def __init__(self, device, lead_left, lead_right, kpoints, *, eta_device, precision,
+                 route=0, **kw):
         self.precision = precision
+        self.route = int(route)
         self.kpoints = list(kpoints)
         self.sets = kspace_sets(device, lead_left, lead_right, self.kpoints, precision=precision,
                                 **kw)
@@ -518,12 +524,13 @@ class _KSolver:
         if k_idx.size == 0:
             return np.zeros(0), np.zeros(0, dtype=np.int32)
         if self.sets is not None:
-            res = _engine.TransmissionSolver(self.sets, precision=self.precision).run(
+            res = _engine.TransmissionSolver(self.sets, precision=self.precision,
+                                             surface_route=self.route).run(
                 k_idx.astype(np.int32), omega, eta, eta_dev)
             return res.t.cpu().numpy(), res.status.cpu().numpy()
         uniq, local = np.unique(k_idx, return_inverse=True)
         return solve_points([self.kps[k] for k in uniq], local, omega, eta, eta_dev,
-                            precision=self.precision)
+                            precision=self.precision, route=self.route)
 
 
 def _t_max(lead_left, kpoints, max_channel_factor):
@@ -658,7 +665,7 @@ def _adaptive_single(omega, device, lead_left, lead_right, kpoints, eta_values, 
     prec = _prec(precision)
     t_max = _t_max(lead_left, kpoints, max_channel_factor)
     solver = _KSolver(device, lead_left, lead_right, kpoints, eta_device=eta_device,
-                      precision=prec, device_to_lead_left=device_to_lead_left,
+                      route=surface_route(surface_gf_method), precision=prec, device_to_lead_left=device_to_lead_left,
                       device_to_lead_right=device_to_lead_right,
                       contact_left_indices=contact_left_indices,
                       contact_right_indices=contact_right_indices, omega_scale=omega_scale)
@@ -757,7 +764,7 @@ def transmission_kavg_adaptive_spectrum(
     prec = _prec(precision)
     t_max = _t_max(lead_left, kpoints, max_channel_factor)
     solver = _KSolver(device, lead_left, lead_right, kpoints, eta_device=eta_device,
-                      precision=prec, device_to_lead_left=device_to_lead_left,
+                      route=surface_route(surface_gf_method), precision=prec, device_to_lead_left=device_to_lead_left,
                       device_to_lead_right=device_to_lead_right,
                       contact_left_indices=contact_left_indices,
                       contact_right_indices=contact_right_indices, omega_scale=omega_scale)
@@ -808,7 +815,7 @@ def transmission_k_moments_spectrum(
     canonical_method(surface_gf_method)
     prec = _prec(precision)
     solver = _KSolver(device, lead_left, lead_right, kpoints, eta_device=eta_device,
-                      precision=prec, device_to_lead_left=device_to_lead_left,
+                      route=surface_route(surface_gf_method), precision=prec, device_to_lead_left=device_to_lead_left,
                       device_to_lead_right=device_to_lead_right, contact_left_indices=None,
                       contact_right_indices=None, omega_scale=omega_scale)
     n = w.size

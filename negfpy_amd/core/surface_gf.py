@@ -65,6 +65,17 @@ def canonical_method(method: str) -> str:
     raise ValueError(_UNKNOWN)
 
 
+def surface_route(method: str) -> int:
+    """Library route for a method: decimation for "sancho_rubio"; for the eigenmode
+    methods, decimation with the eigenmode fallback where it fails."""
+    from negfpy_amd import _lib
+
+    return _lib.SURFACE_DECIMATION if canonical_method(method) == "sancho_rubio" else _lib.SURFACE_EIGEN
+
+
+_EIGEN = 1   # the library's eigenmode route code (SURFACE_EIGEN)
+
+
 def resolve_d10(d01: Array, d10: Array | None) -> Array:
     """d10 defaults to d01^H (surface_gf.py:12-15)."""
     if d10 is None:
@@ -78,9 +89,10 @@ def pack_lead(d00, d01, d10=None) -> Array:
                      resolve_d10(np.asarray(d01), d10)])
 
 
-def _single(omega, d00, d01, d10, eta, tol, maxiter, precision):
+def _single(omega, d00, d01, d10, eta, tol, maxiter, precision, route=0):
     res = _engine.surface_gf_batch(pack_lead(d00, d01, d10)[None], [0], [float(omega)],
-                                   [float(eta)], tol=tol, maxiter=maxiter, precision=_prec(precision))
+                                   [float(eta)], tol=tol, maxiter=maxiter, precision=_prec(precision),
+                                   route=route)
     raise_for_lead_status(int(res.status[0].item()))
     return res.g[0].cpu().numpy()
 
@@ -103,7 +115,7 @@ def surface_gf_generalized_eigen(omega: float, d00: Array, d01: Array, d10: Arra
                                  precision: str | None = None) -> Array:
     """Eigenmode route (surface_gf.py:84-133); see the module docstring."""
     _check_square(d00, d01)
-    return _single(omega, d00, d01, d10, eta, 1e-12, 200, precision)
+    return _single(omega, d00, d01, d10, eta, 1e-12, 200, precision, _EIGEN)
 
 
 def surface_gf_generalized_eigen_svd(omega: float, d00: Array, d01: Array,
@@ -114,7 +126,7 @@ def surface_gf_generalized_eigen_svd(omega: float, d00: Array, d01: Array,
     _check_square(d00, d01)
     if svd_rcond <= 0.0:
         raise ValueError("svd_rcond must be positive.")
-    return _single(omega, d00, d01, d10, eta, 1e-12, 200, precision)
+    return _single(omega, d00, d01, d10, eta, 1e-12, 200, precision, _EIGEN)
 
 
 def surface_gf_legacy_eigen_svd(omega: float, d00: Array, d01: Array, d10: Array | None = None,
@@ -122,7 +134,7 @@ def surface_gf_legacy_eigen_svd(omega: float, d00: Array, d01: Array, d10: Array
                                 precision: str | None = None) -> Array:
     """Legacy Fortran-style eigenmode route (surface_gf.py:201-251)."""
     _check_square(d00, d01)
-    return _single(omega, d00, d01, d10, eta, 1e-12, 200, precision)
+    return _single(omega, d00, d01, d10, eta, 1e-12, 200, precision, _EIGEN)
 
 
 def surface_gf(omega: float, d00: Array, d01: Array, d10: Array | None = None, eta: float = 1e-8,

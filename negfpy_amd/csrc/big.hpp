@@ -550,3 +550,255 @@ __global__ void __launch_bounds__(BIG_NT) k_trace_big(TraceArgs a, double* pool)
 }
 
 }  // namespace negf
+
+namespace negf {
+
+// ---------------------------------------------------------------------------
+// Eigenmode surface GF where decimation failed (n <= 48, pencil 2n <= 96)
+// ---------------------------------------------------------------------------
+//
+// The reference's eigenmode routes (surface_gf.py:84-199) build the transfer matrix
+// from the n finite eigenmodes of smallest |lambda| of the pencil
+//   A = [[-(d00 - z), -d10], [I, 0]],  B = [[d01, 0], [0, I]]
+// (those inside the unit circle, or the n smallest when fewer are inside), as
+// T = Phi Lambda Phi^-1 with Sigma = d01 T, g = (z - d00 - Sigma)^-1.  Only the
+// invariant subspace of those n eigenvalues matters, so no eigensolver is needed:
+// for a radius r, the Moebius map lambda -> (lambda - r)/(lambda + r) sends
+// |lambda| < r to the left half-plane, the matrix sign S of
+// C = (A + rB)^-1 (A - rB) (scaled Newton, S <- (s S + S^-1 / s) / 2) gives the
+// spectral projector P = (I - S)/2 onto it, trace(P) counts it, and r is searched
+// (bracket from 1 outwards, then bisection) until the count is n.  With a basis
+// P[:, n:2n] of the subspace, T = P12 P22^-1.  The first block row of A and B is
+// scaled by 1 / max(|d00|, |d01|, |z|) (eigenvalues unchanged) to balance the
+// pencil.  Slots: A, B, Y = (A + rB)^-1 then T, X (work), S, d00, d01, Sigma/G.
+
+struct EigArgs {
+  int n, n_inst;
+  const double* lead;  // [n_sets][3][n][n]
+  const int32_t* set_of;
+  const double* omega;
+  const double* eta;
+  double* g_io;
+  double* dos_out;
+  int32_t* status;     // instances with status != 0 are (re)computed; 0 on success
+};
+
+constexpr int EIG_SIGN_MAXIT = 100;
+
+// Newton sign iteration of the matrix in S (2n x 2n) in place; X is work.  Returns
+// true when converged (step below 1e-13, or below 1e-9 and no longer halving).
+__device__ bool big_sign(const BigMat& S, const BigMat& X, int m, BigShared& sh, int tid) {
+  double dprev = INFINITY;
+  for (int it = 0; it < EIG_SIGN_MAXIT; ++it) {
+    double ss = 0.0;
+    for (int e = tid; e < m * m; e += BIG_NT) {
+      const int i = e / m, j = e - (e / m) * m;
+      const unsigned o = big_off(i, j);
+      const double xr = S.re[o], xi = S.im[o];
+      X.re[o] = xr;
+      X.im[o] = xi;
+      ss += xr * xr + xi * xi;
+    }
+    const double ns = block_sum<BIG_NT>(ss, sh.sred, tid);
+    const int st = gj_inverse<double, BIG_NT, BIG, BIG_RG>(X.re, X.im, BIG_LD, m, sh.gj, tid);
+    if (st != ST_OK) return false;
+    double xs = 0.0;
+    for (int e = tid; e < m * m; e += BIG_NT) {
+      const int i = e / m, j = e - (e / m) * m;
+      const unsigned o = big_off(i, j);
+      xs += X.re[o] * X.re[o] + X.im[o] * X.im[o];
+    }
+    const double nx = block_sum<BIG_NT>(xs, sh.sred, tid);
+    const double sc = sqrt(sqrt(nx / ns));
+    double dd = 0.0, nn = 0.0;
+    for (int e = tid; e < m * m; e += BIG_NT) {
+      const int i = e / m, j = e - (e / m) * m;
+      const unsigned o = big_off(i, j);
+      const double yr = 0.5 * (sc * S.re[o] + X.re[o] / sc);
+      const double yi = 0.5 * (sc * S.im[o] + X.im[o] / sc);
+      const double dr = yr - S.re[o], di = yi - S.im[o];
+      dd += dr * dr + di * di;
+      nn += yr * yr + yi * yi;
+      S.re[o] = yr;
+      S.im[o] = yi;
+    }
+    const double d = sqrt(block_sum<BIG_NT>(dd, sh.sred, tid) / block_sum<BIG_NT>(nn, sh.sred, tid));
+    if (!(d == d)) return false;
+    if (d < 1e-13 || (d < 1e-9 && d > 0.5 * dprev)) return true;
+    dprev = d;
+  }
+  return dprev < 1e-9;
+}
+
+// Count of eigenvalues with |lambda| < r (-1 when the sign iteration failed); leaves
+// the sign matrix in S.
+__device__ int big_count_inside(double r, const BigMat& A, const BigMat& B, const BigMat& Y,
+                                const BigMat& X, const BigMat& S, int m, BigShared& sh, int tid) {
+  for (int e = tid; e < m * m; e += BIG_NT) {
+    const int i = e / m, j = e - (e / m) * m;
+    const unsigned o = big_off(i, j);
+    Y.re[o] = A.re[o] + r * B.re[o];
+    Y.im[o] = A.im[o] + r * B.im[o];
+    X.re[o] = A.re[o] - r * B.re[o];
+    X.im[o] = A.im[o] - r * B.im[o];
+  }
+  __syncthreads();
+  if (gj_inverse<double, BIG_NT, BIG, BIG_RG>(Y.re, Y.im, BIG_LD, m, sh.gj, tid) != ST_OK) return -1;
+  const BigTile t(m, m, tid);
+  double cr[BIG_TM][BIG_TM], ci[BIG_TM][BIG_TM];
+  big_mul<OPN, OPN>(Y, X, m, m, m, t, cr, ci, sh, tid);
+  big_store(S, m, t, cr, ci);
+  __syncthreads();
+  if (!big_sign(S, X, m, sh, tid)) return -1;
+  double tr = 0.0;
+  for (int i = tid; i < m; i += BIG_NT) tr += S.re[big_off(i, i)];
+  const double trp = 0.5 * ((double)m - block_sum<BIG_NT>(tr, sh.sred, tid));
+  const double c = rint(trp);
+  return fabs(trp - c) < 1e-3 ? (int)c : -1;
+}
+
+__device__ void big_eig_one(const EigArgs& a, int inst, double* slot, BigShared& sh, int tid) {
+  if (a.status[inst] == ST_OK) return;
+  const int n = a.n, m = 2 * n;
+  const size_t nn = (size_t)n * n;
+  const double* blk = a.lead + (size_t)a.set_of[inst] * 3 * nn * 2;
+  double* sl = big_fresh(slot);
+  const BigMat A = big_mat(sl, 0), B = big_mat(sl, 1), Y = big_mat(sl, 2), X = big_mat(sl, 3);
+  const BigMat S = big_mat(sl, 4), D0 = big_mat(sl, 5), D1 = big_mat(sl, 6), G = big_mat(sl, 7);
+  const BigMat D10 = big_mat(sl, 8);
+  big_gload(blk, n, n, D0, tid);
+  big_gload(blk + nn * 2, n, n, D1, tid);
+  big_gload(blk + 2 * nn * 2, n, n, D10, tid);
+  double zr, zi;
+  zsq(a.omega[inst], a.eta[inst], zr, zi);
+  __syncthreads();
+  double amax = 0.0;
+  for (int e = tid; e < (int)nn; e += BIG_NT) {
+    const int i = e / n, j = e - (e / n) * n;
+    const unsigned o = big_off(i, j);
+    amax = fmax(amax, fmax(hypot(D0.re[o], D0.im[o]), hypot(D1.re[o], D1.im[o])));
+  }
+  double gmax;
+  bool gbad;
+  block_max_any<BIG_NT>(amax, false, sh.sred, sh.sflag, tid, gmax, gbad);
+  const double s = 1.0 / fmax(gmax, hypot(zr, zi));
+  for (int e = tid; e < m * m; e += BIG_NT) {
+    const int i = e / m, j = e - (e / m) * m;
+    const unsigned o = big_off(i, j);
+    double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+    if (i < n && j < n) {
+      const unsigned q = big_off(i, j);
+      ar = -s * (D0.re[q] - (i == j ? zr : 0.0));
+      ai = -s * (D0.im[q] - (i == j ? zi : 0.0));
+      br = s * D1.re[q];
+      bi = s * D1.im[q];
+    } else if (i < n) {
+      const unsigned q = big_off(i, j - n);
+      ar = -s * D10.re[q];
+      ai = -s * D10.im[q];
+    } else if (j < n) {
+      ar = (i - n == j) ? 1.0 : 0.0;
+    } else {
+      br = (i == j) ? 1.0 : 0.0;
+    }
+    A.re[o] = ar;
+    A.im[o] = ai;
+    B.re[o] = br;
+    B.im[o] = bi;
+  }
+  __syncthreads();
+  // radius with exactly n eigenvalues inside
+  int c = big_count_inside(1.0, A, B, Y, X, S, m, sh, tid);
+  if (c < 0) return;
+  if (c != n) {
+    const double sgn = c < n ? 1.0 : -1.0;
+    double lo = 0.0, hi = -1.0, h = 1e-8;
+    while (h < 0.5) {
+      c = big_count_inside(1.0 + sgn * h, A, B, Y, X, S, m, sh, tid);
+      if (c >= 0 && (sgn > 0 ? c >= n : c <= n)) {
+        hi = h;
+        break;
+      }
+      lo = h;
+      h *= 4.0;
+    }
+    if (hi < 0.0) return;
+    for (int b = 0; b < 40 && c != n; ++b) {
+      h = lo > 0.0 ? sqrt(fmax(lo, 1e-12) * hi) : 0.25 * hi;
+      c = big_count_inside(1.0 + sgn * h, A, B, Y, X, S, m, sh, tid);
+      if (c < 0) return;
+      if (sgn > 0 ? c >= n : c <= n)
+        hi = h;
+      else
+        lo = h;
+    }
+    if (c != n) return;
+  }
+  // P22 = (I - S22)/2 -> X (n x n), inverted; T = P12 P22^-1 = -S12 P22^-1 / 2 -> Y
+  for (int e = tid; e < (int)nn; e += BIG_NT) {
+    const int i = e / n, j = e - (e / n) * n;
+    const unsigned o = big_off(i, j), q = big_off(n + i, n + j);
+    X.re[o] = 0.5 * ((i == j ? 1.0 : 0.0) - S.re[q]);
+    X.im[o] = -0.5 * S.im[q];
+  }
+  __syncthreads();
+  if (gj_inverse<double, BIG_NT, BIG, BIG_RG>(X.re, X.im, BIG_LD, n, sh.gj, tid) != ST_OK) return;
+  const BigMat S12{S.re + n, S.im + n};
+  const BigTile t(n, n, tid);
+  double cr[BIG_TM][BIG_TM], ci[BIG_TM][BIG_TM];
+  big_mul<OPN, OPN>(S12, X, n, n, n, t, cr, ci, sh, tid);
+#pragma unroll
+  for (int x = 0; x < BIG_TM; ++x)
+#pragma unroll
+    for (int y = 0; y < BIG_TM; ++y) {
+      cr[x][y] *= -0.5;
+      ci[x][y] *= -0.5;
+    }
+  big_store(Y, n, t, cr, ci);
+  __syncthreads();
+  // Sigma = d01 T;  G = (z - d00 - Sigma)^-1
+  big_mul<OPN, OPN>(D1, Y, n, n, n, t, cr, ci, sh, tid);
+  if (t.own) {
+#pragma unroll
+    for (int x = 0; x < BIG_TM; ++x)
+#pragma unroll
+      for (int y = 0; y < BIG_TM; ++y) {
+        const int i = t.i0 + x, j = t.j0 + y;
+        if (i < n && j < n) {
+          const unsigned o = big_off(i, j);
+          G.re[o] = ((i == j ? zr : 0.0) - D0.re[o]) - cr[x][y];
+          G.im[o] = ((i == j ? zi : 0.0) - D0.im[o]) - ci[x][y];
+        }
+      }
+  }
+  __syncthreads();
+  if (gj_inverse<double, BIG_NT, BIG, BIG_RG>(G.re, G.im, BIG_LD, n, sh.gj, tid) != ST_OK) return;
+  bool bad = false;
+  for (int e = tid; e < (int)nn; e += BIG_NT) {
+    const int i = e / n, j = e - (e / n) * n;
+    const unsigned o = big_off(i, j);
+    bad |= !(isfinite(G.re[o]) && isfinite(G.im[o]));
+  }
+  block_max_any<BIG_NT>(0.0, bad, sh.sred, sh.sflag, tid, gmax, gbad);
+  if (gbad) return;
+  big_gstore(a.g_io + (size_t)inst * nn * 2, n, G, tid);
+  if (tid == 0) {
+    if (a.dos_out) {
+      double tr = 0.0;
+      for (int i = 0; i < n; ++i) tr += -2.0 * G.im[big_off(i, i)];
+      a.dos_out[inst] = (a.omega[inst] / M_PI) * tr;
+    }
+    a.status[inst] = ST_OK;
+  }
+}
+
+__global__ void __launch_bounds__(BIG_NT) k_sgf_eigen_big(EigArgs a, double* pool) {
+  __shared__ BigShared sh;
+  double* slot = pool + (size_t)blockIdx.x * BIG_SLOT;
+  for (int inst = blockIdx.x; inst < a.n_inst; inst += gridDim.x) {
+    big_eig_one(a, inst, slot, sh, threadIdx.x);
+    __syncthreads();
+  }
+}
+
+}  // namespace negf

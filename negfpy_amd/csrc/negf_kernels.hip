@@ -1852,6 +1852,18 @@ int negf_sancho_rubio_batched(void* stream, int precision, int n, int n_inst, co
   return rc;
 }
 
+int negf_surface_gf_eigen_batched(void* stream, int n, int n_inst, const double* lead,
+                                  const int32_t* set_of, const double* omega, const double* eta,
+                                  double* g_io, double* dos_out, int32_t* status) {
+  if (n <= 0 || n_inst < 0) return fail(NEGF_ERR_ARG, "invalid sizes (n=%d n_inst=%d)", n, n_inst);
+  if (!lead || !set_of || !omega || !eta || !g_io || !status)
+    return fail(NEGF_ERR_ARG, "null pointer argument");
+  if (n_inst == 0 || 2 * n > BIG) return NEGF_OK;
+  EigArgs a{n, n_inst, lead, set_of, omega, eta, g_io, dos_out, status};
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  NEGF_BIG_LAUNCH(k_sgf_eigen_big, st, a);
+}
+
 int negf_surface_refine_batched(void* stream, int n, int n_inst, const double* lead,
                                 const int32_t* set_of, const double* omega, const double* eta,
                                 int steps, double* g_io, double* dos_out, int32_t* status) {
@@ -2077,15 +2089,28 @@ int negf_transmission_batched(void* stream, const negf_transmission_problem* p, 
   w += ni * sizeof(int32_t);
   int32_t* st_r = reinterpret_cast<int32_t*>(w);
 
+  if (p->surface_route != NEGF_SURFACE_DECIMATION && p->surface_route != NEGF_SURFACE_EIGEN)
+    return fail(NEGF_ERR_ARG, "unknown surface route %d", p->surface_route);
+  const bool eig = p->surface_route == NEGF_SURFACE_EIGEN;
   int rc = negf_sancho_rubio_batched(stream, p->precision, p->n_lead_l, p->n_inst, p->lead_l,
                                      p->set_of, p->omega, p->eta_lead, p->tol, p->maxiter,
                                      p->refine_steps, g_l, p->dos_l, p->iters_l, st_l);
   if (rc) return rc;
+  if (eig) {
+    rc = negf_surface_gf_eigen_batched(stream, p->n_lead_l, p->n_inst, p->lead_l, p->set_of,
+                                       p->omega, p->eta_lead, g_l, p->dos_l, st_l);
+    if (rc) return rc;
+  }
   if (!p->share_leads) {
     rc = negf_sancho_rubio_batched(stream, p->precision, p->n_lead_r, p->n_inst, p->lead_r,
                                    p->set_of, p->omega, p->eta_lead, p->tol, p->maxiter,
                                    p->refine_steps, g_r, p->dos_r, p->iters_r, st_r);
     if (rc) return rc;
+    if (eig) {
+      rc = negf_surface_gf_eigen_batched(stream, p->n_lead_r, p->n_inst, p->lead_r, p->set_of,
+                                         p->omega, p->eta_lead, g_r, p->dos_r, st_r);
+      if (rc) return rc;
+    }
   } else {
     hipError_t e = hipMemcpyAsync(st_r, st_l, ni * sizeof(int32_t), hipMemcpyDeviceToDevice,
                                   reinterpret_cast<hipStream_t>(stream));

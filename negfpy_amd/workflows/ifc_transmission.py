@@ -21,7 +21,7 @@ import numpy as np
 
 from negfpy_amd.core.negf import _adaptive_batch, _KSolver, _t_max
 from negfpy_amd.core.thermal_current import transverse_area_from_vectors, transverse_length_from_vector
-from negfpy_amd.core.surface_gf import _prec, canonical_method
+from negfpy_amd.core.surface_gf import _prec, canonical_method, surface_route
 from negfpy_amd.io import read_ifc
 from negfpy_amd.modeling import (
     BuildConfig,
@@ -275,6 +275,7 @@ def transmission_spectrum_from_config(cfg: dict, model: dict, omegas, kmesh: dic
         if mesh not in solvers:
             solvers[mesh] = (_KSolver(model["device"], model["lead_left"], model["lead_right"],
                                       kpts, eta_device=eta_device, precision=prec,
+                                      route=surface_route(method),
                                       device_to_lead_left=model["device_to_lead_left"],
                                       device_to_lead_right=model["device_to_lead_right"],
                                       contact_left_indices=None, contact_right_indices=None,

@@ -32,5 +32,6 @@ def test_library_exports_every_declared_symbol():
 def test_problem_struct_layout():
     import ctypes
 
-    # 11 ints, a double (8-aligned at offset 48), then 18 pointers
-    assert ctypes.sizeof(_lib.TransmissionProblem) == 48 + 8 + 18 * 8
+    # 11 ints, a double (8-aligned at offset 48), 18 pointers, then surface_route (padded)
+    assert ctypes.sizeof(_lib.TransmissionProblem) == 48 + 8 + 18 * 8 + 8
+    assert _lib.TransmissionProblem.surface_route.offset == 48 + 8 + 18 * 8

@@ -1,0 +1,78 @@
+"""The eigenmode surface-GF route on the GPU (negf_surface_gf_eigen_batched): the
+transfer matrix of the n smallest-|lambda| finite eigenmodes of the lead's
+quadratic eigenproblem, from a spectral projector (matrix sign function with a
+searched radius) instead of an eigensolver.  It runs for the eigenmode methods
+where decimation fails.
+
+* On regular leads, forced through the route, it gives decimation's g.
+* On the TaN leads near Gamma at low frequency, where the reference's own
+  decimation fails at eta = 1e-8, it gives the reference eigenmode route's g (the
+  oracle's generalized_eigen_svd); the decimation method still raises there.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from negfpy_amd import _lib
+from negfpy_amd.core import _engine, surface_gf
+from negfpy_amd.core.surface_gf import pack_lead
+from negfpy_amd.core.types import LeadBlocks
+from negfpy_amd.workflows import build_omega_grid, build_transport_model, prepare_kmesh
+from negfpy_amd.workloads import omega_max, random_lead
+from oracle import negf_oracle as o
+from study_runs_util import load_run
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n", [6, 12, 24, 47, 48])
+def test_route_equals_decimation_on_regular_leads(gpu, n):
+    d00, d01 = random_lead(700 + n, n)
+    wm = omega_max(LeadBlocks(d00=d00, d01=d01))
+    leads = pack_lead(d00, d01)[None]
+    w = np.linspace(0.05, 0.95, 16) * wm
+    eta = np.full(w.size, 1e-6)
+    ref = _engine.surface_gf_batch(leads, np.zeros(w.size, dtype=np.int32), w, eta)
+    assert int((ref.status != 0).sum()) == 0
+    lib = _lib.load()
+    dev = ref.g.device
+    g = torch.zeros_like(ref.g)
+    dos = torch.zeros_like(ref.dos)
+    st = torch.ones(w.size, dtype=torch.int32, device=dev)      # every instance takes the route
+    L = torch.from_numpy(leads).to(dev)
+    so = torch.zeros(w.size, dtype=torch.int32, device=dev)
+    wt = torch.from_numpy(w).to(dev)
+    et = torch.from_numpy(eta).to(dev)
+    _lib.check(lib.negf_surface_gf_eigen_batched(
+        _lib.stream(), n, w.size, L.data_ptr(), so.data_ptr(), wt.data_ptr(), et.data_ptr(),
+        g.data_ptr(), dos.data_ptr(), st.data_ptr()))
+    assert int((st != 0).sum()) == 0
+    err = (torch.abs(g - ref.g).amax(dim=(1, 2)) / torch.abs(ref.g).amax(dim=(1, 2))).max().item()
+    print(f"n={n}: max |g_route - g_decimation| / |g| = {err:.2e}", flush=True)
+    assert err <= 1e-8
+    np.testing.assert_allclose(dos.cpu().numpy(), ref.dos.cpu().numpy(), rtol=1e-8, atol=0)
+
+
+def test_route_matches_eigen_reference_where_decimation_fails(gpu, tmp_path):
+    cfg, _, _, _ = load_run("20260225_113114Z_tan_transmission_x", tmp_path)
+    m = build_transport_model(cfg)
+    km = prepare_kmesh(cfg)
+    w = float(build_omega_grid(cfg, m["ifc"])[0][6])
+    checked = 0
+    for k in km["low"]["kpoints"]:
+        d00, d01, d10 = m["lead_left"].blocks_builder(k)
+        try:
+            o.sancho_rubio(w, d00, d01, d10, eta=1e-8)
+            continue
+        except RuntimeError:
+            pass
+        with pytest.raises(RuntimeError):
+            surface_gf(w, d00, d01, d10, eta=1e-8, method="sancho_rubio")
+        g = surface_gf(w, d00, d01, d10, eta=1e-8, method="generalized_eigen_svd")
+        g_ref = o.surface_gf(w, d00, d01, d10, eta=1e-8, method="generalized_eigen_svd")
+        err = np.abs(g - g_ref).max() / np.abs(g_ref).max()
+        print(f"k={k}: |g - g_eigen_ref| / |g| = {err:.2e}", flush=True)
+        assert err <= 1e-6
+        checked += 1
+    assert checked >= 4

@@ -176,7 +176,7 @@ def test_study_run_set_up_matches_reference_file(name, tmp_path):
     assert len(km["high"]["kpoints"]) > 0 and len(km["low"]["kpoints"]) > 0
 
 
-from study_runs_util import SR_UNSTABLE, WORST, oracle_kavg  # noqa: E402
+from study_runs_util import EIGEN_ROUTE, SR_UNSTABLE, WORST, oracle_kavg  # noqa: E402
 
 
 def _worst_setup(name, idx, tmp_path):
@@ -214,7 +214,7 @@ def test_study_method_spread_is_the_eigen_route_error(name, idx, tmp_path):
     assert r_ge >= 10.0 * r_sr, (r_sr, r_ge)
 
 
-@pytest.mark.parametrize("name,idx", [(k, v[0]) for k, v in SR_UNSTABLE.items()])
+@pytest.mark.parametrize("name,idx", [(k, v[0]) for k, v in {**SR_UNSTABLE, **EIGEN_ROUTE}.items()])
 def test_study_points_where_decimation_fails(name, idx, tmp_path):
     """At these points the reference's own decimation route raises (numerically
     unstable / non-finite) for some k points at the smallest eta, while the eigenmode route the

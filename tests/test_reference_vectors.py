@@ -184,7 +184,8 @@ def test_ribbon_workload_is_the_reference_input():
 # adaptive drivers: host logic with the oracle standing in for the batched solve
 # --------------------------------------------------------------------------
 
-def _oracle_solve_points(kps, k_of, omega, eta, eta_dev, *, precision, tol=1e-12, maxiter=200):
+def _oracle_solve_points(kps, k_of, omega, eta, eta_dev, *, precision, tol=1e-12, maxiter=200,
+                         route=0):
     t_all = np.full(len(k_of), np.nan)
     st = np.zeros(len(k_of), dtype=np.int32)
     for j, k in enumerate(k_of):
