@@ -50,7 +50,7 @@ const char* negf_last_error(void);
  * self-energy entries for a precision (FP64: 96; mixed: 48). */
 int negf_max_block(int precision);
 
-/* Largest block size of negf_device_green_batched and negf_adjoint_batched (48). */
+/* Largest block size of negf_device_green_batched and negf_adjoint_batched (96). */
 int negf_max_block_green(void);
 
 /*

@@ -66,6 +66,11 @@ template <>
 struct Cfg<64> {
   static constexpr int NMAX = 64, NT = 1024, TM = 2;
 };
+// matrices in a device-memory pool slot (the full Green's function and adjoint kernels)
+template <>
+struct Cfg<96> {
+  static constexpr int NMAX = 96, NT = 768, TM = 4;
+};
 
 // Waves per SIMD the main kernels must allow: the LDS footprint of one
 // instance admits about this many resident workgroups per CU, so capping
@@ -101,7 +106,12 @@ struct Lds {
   Gj* gj;
   double* sred;
   int* sflag;
-  __device__ __forceinline__ Lds(unsigned char* base) {
+  unsigned char* base_;
+  __device__ __forceinline__ void zero(int tid) {
+    uint32_t* p = reinterpret_cast<uint32_t*>(base_);
+    for (size_t e = tid; e < mat_bytes() / 4; e += NT) p[e] = 0u;
+  }
+  __device__ __forceinline__ Lds(unsigned char* base) : base_(base) {
     T* p = reinterpret_cast<T*>(base);
 #pragma unroll
     for (int q = 0; q < NMAT; ++q) {
@@ -113,6 +123,46 @@ struct Lds {
     gj = reinterpret_cast<Gj*>(base + gj_off());
     sred = reinterpret_cast<double*>(base + red_off());
     sflag = reinterpret_cast<int*>(sred + NT / WAVE);
+  }
+};
+
+// The same interface with the matrices in a device-memory slot (NMAT planar
+// matrices of NMAX x (NMAX + 1)) and the pivot buffers and reduction scratch in LDS:
+// used where the matrices of one instance exceed the LDS (class 96).
+template <typename T, int NMAX, int NT, int NMAT>
+struct GmShared {
+  GjBuf<T, NMAX> gj;
+  double sred[NT / WAVE];
+  int sflag[NT / WAVE];
+};
+
+template <typename T, int NMAX, int NT, int NMAT>
+struct GmL {
+  static constexpr int LD = NMAX + 1;
+  static constexpr int MSZ = NMAX * LD;
+  static constexpr int RG = Lds<T, NMAX, NT, NMAT>::RG;
+  static constexpr size_t slot_elems() { return (size_t)2 * NMAT * MSZ; }
+  T* re[NMAT];
+  T* im[NMAT];
+  GjBuf<T, NMAX>* gj;
+  double* sred;
+  int* sflag;
+  T* base_;
+  __device__ __forceinline__ void zero(int tid) {
+    for (size_t e = tid; e < slot_elems(); e += NT) base_[e] = T(0);
+  }
+  __device__ __forceinline__ GmL(T* slot, GmShared<T, NMAX, NT, NMAT>* sh) : base_(slot) {
+    T* p = slot;
+#pragma unroll
+    for (int q = 0; q < NMAT; ++q) {
+      re[q] = p;
+      p += MSZ;
+      im[q] = p;
+      p += MSZ;
+    }
+    gj = &sh->gj;
+    sred = sh->sred;
+    sflag = sh->sflag;
   }
 };
 
@@ -730,16 +780,11 @@ __device__ __forceinline__ void tstore_glb(double* __restrict__ g, int gld, int 
     }
 }
 
-template <int NMAX>
-__global__ void __launch_bounds__(Cfg<NMAX>::NT) k_device_green(DgfArgs a) {
+template <int NMAX, class L>
+__device__ void dgf_body(const DgfArgs& a, int inst, L& s, int tid) {
   using T = double;
   constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
-  using L = Lds<T, NMAX, NT, 4>;
   constexpr int LD = L::LD;
-  extern __shared__ __align__(16) unsigned char smem[];
-  L s(smem);
-  const int tid = threadIdx.x;
-  const int inst = blockIdx.x;
   const int n = a.n, nl = a.n_layers, dim = n * nl;
   const size_t nn = (size_t)n * n;
   const size_t set = (size_t)a.set_of[inst];
@@ -749,7 +794,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_device_green(DgfArgs a) {
   const double2* sgr = reinterpret_cast<const double2*>(a.sigma_r + (size_t)inst * nn * 2);
   double* G = a.g_out + (size_t)inst * dim * dim * 2;
   auto blk = [&](int bi, int bj) { return G + ((size_t)bi * n * dim + (size_t)bj * n) * 2; };
-  lds_zero<T, NT>(smem, L::mat_bytes(), tid);
+  s.zero(tid);
   __syncthreads();
   const Tile<TM> t(n, n, tid);
   double zr, zi;
@@ -878,6 +923,13 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_device_green(DgfArgs a) {
   }
 }
 
+template <int NMAX>
+__global__ void __launch_bounds__(Cfg<NMAX>::NT) k_device_green(DgfArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  Lds<double, NMAX, Cfg<NMAX>::NT, 4> s(smem);
+  dgf_body<NMAX>(a, blockIdx.x, s, threadIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Adjoint: T and dT/dD for every device block (FP64)
 // ---------------------------------------------------------------------------
@@ -910,16 +962,11 @@ struct AdjArgs {
   int32_t* status;
 };
 
-template <int NMAX>
-__global__ void __launch_bounds__(Cfg<NMAX>::NT) k_adjoint(AdjArgs a) {
+template <int NMAX, class L>
+__device__ void adj_body(const AdjArgs& a, int inst, L& s, int tid) {
   using T = double;
   constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
-  using L = Lds<T, NMAX, NT, 4>;
   constexpr int LD = L::LD;
-  extern __shared__ __align__(16) unsigned char smem[];
-  L s(smem);
-  const int tid = threadIdx.x;
-  const int inst = blockIdx.x;
   if (a.status_in && a.status_in[inst] != ST_OK) {
     if (tid == 0) {
       a.status[inst] = a.status_in[inst];
@@ -940,7 +987,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_adjoint(AdjArgs a) {
   double* gc = a.grad_coupling + (size_t)inst * (nl > 1 ? nl - 1 : 0) * nn * 2;
   auto blk = [&](double* base, int i) { return base + (size_t)i * nn * 2; };
   const double w2 = 2.0 * (a.weight ? a.weight[inst] : 1.0);
-  lds_zero<T, NT>(smem, L::mat_bytes(), tid);
+  s.zero(tid);
   __syncthreads();
   const Tile<TM> t(n, n, tid);
   double zr, zi;
@@ -1170,6 +1217,13 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_adjoint(AdjArgs a) {
     f1r = xr;
     f1i = xi;
   }
+}
+
+template <int NMAX>
+__global__ void __launch_bounds__(Cfg<NMAX>::NT) k_adjoint(AdjArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  Lds<double, NMAX, Cfg<NMAX>::NT, 4> s(smem);
+  adj_body<NMAX>(a, blockIdx.x, s, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1505,6 +1559,29 @@ __global__ void __launch_bounds__(256) k_heat_current(int n_w, const double* __r
 
 namespace negf {
 
+// full Green's function and adjoint for 49..96-DOF blocks: the same bodies with
+// their four matrices in a pool slot, persistent over the instances
+using GreenBig = GmL<double, BIG, BIG_NT, 4>;
+static_assert(GreenBig::slot_elems() <= BIG_SLOT, "pool slot too small");
+
+__global__ void __launch_bounds__(BIG_NT) k_device_green_big(DgfArgs a, double* pool) {
+  __shared__ GmShared<double, BIG, BIG_NT, 4> sh;
+  GreenBig s(pool + (size_t)blockIdx.x * BIG_SLOT, &sh);
+  for (int inst = blockIdx.x; inst < a.n_inst; inst += gridDim.x) {
+    dgf_body<BIG>(a, inst, s, threadIdx.x);
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(BIG_NT) k_adjoint_big(AdjArgs a, double* pool) {
+  __shared__ GmShared<double, BIG, BIG_NT, 4> sh;
+  GreenBig s(pool + (size_t)blockIdx.x * BIG_SLOT, &sh);
+  for (int inst = blockIdx.x; inst < a.n_inst; inst += gridDim.x) {
+    adj_body<BIG>(a, inst, s, threadIdx.x);
+    __syncthreads();
+  }
+}
+
 static thread_local char g_err[512] = "";
 
 static int fail(int code, const char* fmt, ...) {
@@ -1527,7 +1604,7 @@ static int hip_check(hipError_t e, const char* what) {
 // mixed precision: 8 .. 64 (its FP64 refinement bounds it at 48 DOF).
 constexpr int FP64_MAX = 96;
 constexpr int MIXED_MAX = 48;
-constexpr int GREEN_MAX = 48;  // full Green's function and adjoint kernels
+constexpr int GREEN_MAX = 96;  // full Green's function and adjoint kernels
 static std::atomic<int> g_class_override[2][FP64_MAX + 1];
 
 static int default_nmax(int n, bool fp64) {
@@ -1755,6 +1832,8 @@ static int run_dgf(hipStream_t st, int nmax, const DgfArgs& a) {
     DG_CASE(24)
     DG_CASE(32)
     DG_CASE(48)
+    case BIG:
+      NEGF_BIG_LAUNCH(k_device_green_big, st, a);
   }
 #undef DG_CASE
   return fail(NEGF_ERR_ARG, "block size class %d unsupported", nmax);
@@ -1773,6 +1852,8 @@ static int run_adj(hipStream_t st, int nmax, const AdjArgs& a) {
     ADJ_CASE(24)
     ADJ_CASE(32)
     ADJ_CASE(48)
+    case BIG:
+      NEGF_BIG_LAUNCH(k_adjoint_big, st, a);
   }
 #undef ADJ_CASE
   return fail(NEGF_ERR_ARG, "block size class %d unsupported", nmax);

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     assert lib.negf_version() == _lib.ABI_VERSION
     assert lib.negf_max_block(_lib.FP64) == 96
     assert lib.negf_max_block(_lib.MIXED) == 48
-    assert lib.negf_max_block_green() == 48
+    assert lib.negf_max_block_green() == 96
 
 
 def test_problem_struct_layout():

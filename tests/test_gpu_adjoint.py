@@ -119,7 +119,7 @@ def test_adjoint_single_layer(gpu):
     _assert_grad(g_ons, go, "onsite")
 
 
-@pytest.mark.parametrize("n,nl", [(24, 6), (48, 3)])
+@pytest.mark.parametrize("n,nl", [(24, 6), (48, 3), (60, 3), (90, 2), (96, 2)])
 def test_adjoint_large_blocks(gpu, n, nl):
     d00, d01 = random_lead(40 + n, n)
     rng = np.random.default_rng(n)

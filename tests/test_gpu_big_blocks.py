@@ -3,8 +3,10 @@
 Surface GF, surface DOS and transmission at n = 60, 72, 90, 96 against the
 oracle (the same conditioning-aware bar as every FP64 parity test), explicit
 device-to-lead couplings with two different leads, a batch larger than the
-pool's slot count (every workgroup then walks several instances), and a 40-DOF
-problem forced into class 96, which must give the class-48 result.
+pool's slot count (every workgroup then walks several instances), a 40-DOF
+problem forced into class 96, which must give the class-48 result, and every
+block of the device Green's function against the oracle's dense inverse (the
+adjoint at these sizes is in test_gpu_adjoint.py).
 """
 
 import numpy as np
@@ -118,9 +120,17 @@ def test_class_96_equals_class_48(gpu):
     np.testing.assert_allclose(t96, t48, rtol=1e-12, atol=1e-15)
 
 
-def test_full_green_function_limit(gpu):
-    n = 60
-    lead, ons, cps, wm = _problem(600, n, 2)
+@pytest.mark.parametrize("n,n_layers", [(60, 3), (96, 2)])
+def test_device_green_function(gpu, n, n_layers):
+    """Every block of G for 60- and 96-DOF layers against the oracle's dense inverse."""
+    lead, ons, cps, wm = _problem(600 + n, n, n_layers)
     dev = Device1D(onsite_blocks=ons, coupling_blocks=cps)
-    with pytest.raises(NotImplementedError, match="supported 48"):
-        device_green_function(0.5 * wm, dev, lead, lead, eta=1e-6)
+    w = 0.5 * wm
+    g, sl, sr = device_green_function(w, dev, lead, lead, eta=1e-6)
+    tup = (lead.d00, lead.d01, None)
+    g_ref, sl_ref, sr_ref = o.device_green_dense(w, ons, cps, tup, tup, eta=1e-6)
+    err = np.max(np.abs(g - g_ref)) / np.max(np.abs(g_ref))
+    print(f"n={n} L={n_layers}: |G - G_ref| / |G| = {err:.2e}", flush=True)
+    assert err <= 1e-9
+    assert np.max(np.abs(sl - sl_ref)) <= 1e-10 * np.max(np.abs(sl_ref))
+    assert np.max(np.abs(sr - sr_ref)) <= 1e-10 * np.max(np.abs(sr_ref))
