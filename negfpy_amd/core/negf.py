@@ -342,7 +342,7 @@ def kspace_sets(device, lead_left, lead_right, kpoints, *, device_to_lead_left=N
     """TransportSets of every k point built on the GPU in one launch per model, when the
     leads and the device are force-constant models (``material_kspace_lead`` /
     ``material_kspace_device`` of ``negfpy_amd.models``, or the interface contacts of
-    ``negfpy_amd.modeling.build_interface_contacts``) with default contacts and no
+    ``negfpy_amd.modeling.interface_contacts``) with default contacts and no
     ``omega_scale``; otherwise None (the caller resolves per k on the host, as the
     reference does)."""
     bl = getattr(lead_left, "blocks_builder", None)

@@ -178,3 +178,18 @@ def build_material_kspace_params(ifc: IFCData, config: BuildConfig | None = None
                                 fc00_terms=fc00, fc01_terms=fc01, fc10_terms=fc10,
                                 dof_per_atom=ifc.dof_per_atom,
                                 onsite_pinning=float(config.onsite_pinning))
+
+
+infer_principal_layer_size = principal_layer_size
+
+
+def build_transport_components(ifc: IFCData, n_layers: int, config: BuildConfig | None = None,
+                               layer_masses=None) -> dict:
+    """Force-constant params with their lead and device (reference
+    ``modeling/builders/material_kspace_builder.py:32``)."""
+    from negfpy_amd.models import material_kspace_device, material_kspace_lead
+
+    params = build_material_kspace_params(ifc, config)
+    return {"params": params, "lead": material_kspace_lead(params),
+            "device": material_kspace_device(n_layers=n_layers, params=params,
+                                             layer_masses=layer_masses)}

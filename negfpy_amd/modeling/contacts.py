@@ -38,8 +38,9 @@ def _fourier(terms, ky, kz, nd):
     return out
 
 
-def build_interface_contacts(params):
-    """(lead_left, lead_right, device_to_lead_left, device_to_lead_right)."""
+def interface_contacts(params):
+    """(lead_left, lead_right, device_to_lead_left, device_to_lead_right) of a bulk
+    force-constant model (``_build_interface_contacts``, ifc_bulk.py:690-742)."""
     f00 = {k: np.asarray(v, dtype=np.complex128) for k, v in params.fc00_terms.items()}
     if params.fc10_terms is None:
         f10 = {(-dy, -dz): np.asarray(v, dtype=np.complex128).conj().T
@@ -88,3 +89,17 @@ def build_interface_contacts(params):
     v_right.material_kspace_contact = ("d01", right.material_kspace)
     v_left.material_kspace_contact = ("d01_h", right.material_kspace)
     return LeadKSpace(blocks_builder=left), LeadKSpace(blocks_builder=right), v_left, v_right
+
+
+def build_interface_contacts(interface, ndof: int):
+    """Validated (left_contact, right_contact) couplings of an ``InterfaceData``, each
+    (ndof, ndof) complex or None (reference ``modeling/builders/interface_builder.py:13``)."""
+    def check(mat, side):
+        if mat is None:
+            return None
+        arr = np.asarray(mat, dtype=np.complex128)
+        if arr.shape != (ndof, ndof):
+            raise ValueError(f"{side}_contact must have shape (ndof, ndof).")
+        return arr
+
+    return check(interface.left_contact, "left"), check(interface.right_contact, "right")

@@ -36,6 +36,19 @@ class IFCData:
 
 
 @dataclass(frozen=True)
+class InterfaceData:
+    """Coupling matrices and index maps of an interface (reference
+    ``modeling/schema/interface_data.py``)."""
+
+    left_contact: Array | None = None
+    right_contact: Array | None = None
+    index_map_left: tuple | None = None
+    index_map_right: tuple | None = None
+    convention: str = "direct"
+    metadata: dict[str, Any] = field(default_factory=dict)
+
+
+@dataclass(frozen=True)
 class BuildConfig:
     onsite_pinning: float = 0.0
     principal_layer_size: int | None = None
@@ -67,3 +80,14 @@ def validate_transport_connectivity(ifc: IFCData) -> None:
         raise ValueError("IFCData must include at least one dx=0 term for onsite blocks.")
     if not any(t.dx != 0 for t in ifc.terms):
         raise ValueError("IFCData must include at least one nonzero dx term for layer coupling.")
+
+
+def validate_material_kspace_params(params) -> None:
+    """The (0, 0) same-layer term exists and has the model's shape (reference
+    ``modeling/validators/model_validator.py:10``)."""
+    ndof = params.ndof
+    d00 = params.fc00_terms.get((0, 0))
+    if d00 is None:
+        raise ValueError("fc00_terms must contain the (0, 0) term.")
+    if np.asarray(d00).shape != (ndof, ndof):
+        raise ValueError("fc00_terms[(0, 0)] has invalid shape.")

@@ -27,7 +27,7 @@ from negfpy_amd.modeling import (
     BuildConfig,
     apply_mass_mode,
     apply_transverse_stability_controls,
-    build_interface_contacts,
+    interface_contacts,
     build_material_kspace_params,
     enforce_translational_asr_on_self_term,
     reorient_for_transport,
@@ -209,7 +209,7 @@ def build_transport_model(cfg: dict, cfg_dir: Path | str = ".") -> dict[str, Any
     params, nyq_info, pm_info = apply_transverse_stability_controls(
         params, ifc_metadata=dict(ifc.metadata),
         drop_nyquist=bool(mc.get("drop_nyquist_transverse", False)))
-    lead_l, lead_r, v_l, v_r = build_interface_contacts(params)
+    lead_l, lead_r, v_l, v_r = interface_contacts(params)
     n_layers = int(mc.get("n_layers", 30))
     return {"ifc": ifc, "params": params, "lead_left": lead_l, "lead_right": lead_r,
             "device_to_lead_left": v_l, "device_to_lead_right": v_r,
