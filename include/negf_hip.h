@@ -214,7 +214,7 @@ typedef struct negf_transmission_problem {
   int32_t* iters_r;       /* [n_inst] decimation steps, right lead (unused when shared) */
   int32_t* status;        /* [n_inst] */
   int surface_route;      /* NEGF_SURFACE_DECIMATION, or NEGF_SURFACE_EIGEN: lead instances whose
-                             decimation fails take the eigenmode route (blocks <= 48) */
+                             decimation fails take the eigenmode route */
 } negf_transmission_problem;
 
 /*
@@ -224,7 +224,8 @@ typedef struct negf_transmission_problem {
  * generalized_eigen[_svd] routes select them (src/negfpy/core/surface_gf.py:84-199),
  * from a spectral projector (matrix sign function, no eigensolver).  On success g,
  * the DOS and status (0) are written; instances it cannot do keep their status.
- * Blocks of n <= 48 (the pencil is 2n); larger blocks are left untouched.
+ * Blocks up to 96 DOF (pencils of 2n up to 96 directly, larger ones as 2 x 2 blocks
+ * with a second device-memory pool of about 3.5 MB per resident workgroup).
  */
 int negf_surface_gf_eigen_batched(void* stream, int n, int n_inst, const double* lead,
                                   const int32_t* set_of, const double* omega, const double* eta,

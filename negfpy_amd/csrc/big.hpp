@@ -802,3 +802,321 @@ __global__ void __launch_bounds__(BIG_NT) k_sgf_eigen_big(EigArgs a, double* poo
 }
 
 }  // namespace negf
+
+namespace negf {
+
+// ---------------------------------------------------------------------------
+// Eigenmode surface GF for 49..96-DOF blocks (pencil 2n <= 192)
+// ---------------------------------------------------------------------------
+//
+// The same spectral projector as k_sgf_eigen_big with every 2n x 2n matrix held as
+// 2 x 2 blocks of n x n slot matrices.  A 2n inverse is the block inverse through
+// the Schur complement of the diagonal block with the smaller condition estimate
+// ||M_ii||_F ||M_ii^-1||_F: with P = M11 (or M22 symmetrically), U = P^-1 M12,
+// V = M21 P^-1, K = M22 - M21 U:
+//   [[P^-1 + U K^-1 V, -U K^-1], [-K^-1 V, K^-1]].
+// The first block row of the pencil is scaled as before; B = diag(s d01, I) is not
+// stored (A +- rB only changes the diagonal blocks).  Slots (n x n): A 0-3, S 4-7,
+// X 8-11 (inverse / work), Y 12-15 ((A + rB)^-1, then T), s d01 16, and work 17-22.
+
+constexpr int HUGE_SLOT_MATS = 23;
+constexpr size_t HUGE_SLOT = HUGE_SLOT_MATS * BIG_MAT;
+
+struct Blk4 {
+  BigMat b[4];  // 11, 12, 21, 22
+};
+
+__device__ __forceinline__ Blk4 blk4(double* slot, int q0) {
+  return Blk4{{big_mat(slot, q0), big_mat(slot, q0 + 1), big_mat(slot, q0 + 2), big_mat(slot, q0 + 3)}};
+}
+
+__device__ __forceinline__ double big_fro2(const BigMat& M, int n, int tid) {
+  double s = 0.0;
+  for (int e = tid; e < n * n; e += BIG_NT) {
+    const int i = e / n, j = e - (e / n) * n;
+    const unsigned o = big_off(i, j);
+    s += M.re[o] * M.re[o] + M.im[o] * M.im[o];
+  }
+  return s;
+}
+
+__device__ __forceinline__ void big_copy(const BigMat& D, const BigMat& S, int n, int tid) {
+  for (int e = tid; e < n * n; e += BIG_NT) {
+    const int i = e / n, j = e - (e / n) * n;
+    const unsigned o = big_off(i, j);
+    D.re[o] = S.re[o];
+    D.im[o] = S.im[o];
+  }
+}
+
+// C = alpha op(A) op(B) (+ beta_one * C when acc): result stored after the product's barrier
+template <int OPA, int OPB>
+__device__ void big_gemm(const BigMat& C, const BigMat& A, const BigMat& B, int n, double alpha,
+                         bool acc, BigShared& sh, int tid) {
+  const BigTile t(n, n, tid);
+  double cr[BIG_TM][BIG_TM], ci[BIG_TM][BIG_TM];
+  big_mul<OPA, OPB>(A, B, n, n, n, t, cr, ci, sh, tid);
+  if (t.own) {
+#pragma unroll
+    for (int x = 0; x < BIG_TM; ++x)
+#pragma unroll
+      for (int y = 0; y < BIG_TM; ++y) {
+        const int i = t.i0 + x, j = t.j0 + y;
+        if (i < n && j < n) {
+          const unsigned o = big_off(i, j);
+          const double r = alpha * cr[x][y], m = alpha * ci[x][y];
+          C.re[o] = acc ? C.re[o] + r : r;
+          C.im[o] = acc ? C.im[o] + m : m;
+        }
+      }
+  }
+  __syncthreads();
+}
+
+// Inverse of the 2n matrix M (blocks) into X; W[0..4] are work slots.  False when a
+// pivot block or the Schur complement is singular.
+__device__ bool blk_inverse(const Blk4& M, const Blk4& X, const BigMat* W, int n, BigShared& sh,
+                            int tid) {
+  // condition estimates of both diagonal blocks
+  double cnd[2];
+  for (int d = 0; d < 2; ++d) {
+    const BigMat& Mi = M.b[d == 0 ? 0 : 3];
+    big_copy(W[d], Mi, n, tid);
+    const double nm = block_sum<BIG_NT>(big_fro2(Mi, n, tid), sh.sred, tid);
+    if (gj_inverse<double, BIG_NT, BIG, BIG_RG>(W[d].re, W[d].im, BIG_LD, n, sh.gj, tid) != ST_OK) {
+      cnd[d] = INFINITY;
+      continue;
+    }
+    cnd[d] = sqrt(nm * block_sum<BIG_NT>(big_fro2(W[d], n, tid), sh.sred, tid));
+  }
+  if (!(cnd[0] < INFINITY) && !(cnd[1] < INFINITY)) return false;
+  const int p = cnd[0] <= cnd[1] ? 0 : 1;         // pivot block 11 (p = 0) or 22 (p = 1)
+  const BigMat& Pi = W[p];
+  const BigMat& Mpq = M.b[p == 0 ? 1 : 2];         // M12 (or M21)
+  const BigMat& Mqp = M.b[p == 0 ? 2 : 1];         // M21 (or M12)
+  const BigMat& Mqq = M.b[p == 0 ? 3 : 0];
+  const BigMat &U = W[2], &V = W[3], &K = W[4];
+  big_gemm<OPN, OPN>(U, Pi, Mpq, n, 1.0, false, sh, tid);   // U = P^-1 M_pq
+  big_gemm<OPN, OPN>(V, Mqp, Pi, n, 1.0, false, sh, tid);   // V = M_qp P^-1
+  big_copy(K, Mqq, n, tid);
+  __syncthreads();
+  big_gemm<OPN, OPN>(K, Mqp, U, n, -1.0, true, sh, tid);    // K = M_qq - M_qp U
+  if (gj_inverse<double, BIG_NT, BIG, BIG_RG>(K.re, K.im, BIG_LD, n, sh.gj, tid) != ST_OK) return false;
+  // blocks of the result: (pp) = P^-1 + U K^-1 V, (pq) = -U K^-1, (qp) = -K^-1 V, (qq) = K^-1
+  const BigMat& Xpp = X.b[p == 0 ? 0 : 3];
+  const BigMat& Xpq = X.b[p == 0 ? 1 : 2];
+  const BigMat& Xqp = X.b[p == 0 ? 2 : 1];
+  const BigMat& Xqq = X.b[p == 0 ? 3 : 0];
+  big_gemm<OPN, OPN>(Xpq, U, K, n, -1.0, false, sh, tid);
+  big_gemm<OPN, OPN>(Xqp, K, V, n, -1.0, false, sh, tid);
+  big_copy(Xpp, Pi, n, tid);
+  big_copy(Xqq, K, n, tid);
+  __syncthreads();
+  big_gemm<OPN, OPN>(Xpp, Xpq, V, n, -1.0, true, sh, tid);  // P^-1 - (-U K^-1) V
+  return true;
+}
+
+__device__ bool huge_sign(const Blk4& S, const Blk4& X, const BigMat* W, int n, BigShared& sh,
+                          int tid) {
+  double dprev = INFINITY;
+  for (int it = 0; it < EIG_SIGN_MAXIT; ++it) {
+    double ss = 0.0;
+    for (int q = 0; q < 4; ++q) ss += big_fro2(S.b[q], n, tid);
+    const double ns = block_sum<BIG_NT>(ss, sh.sred, tid);
+    if (!blk_inverse(S, X, W, n, sh, tid)) return false;
+    double xs = 0.0;
+    for (int q = 0; q < 4; ++q) xs += big_fro2(X.b[q], n, tid);
+    const double nx = block_sum<BIG_NT>(xs, sh.sred, tid);
+    const double sc = sqrt(sqrt(nx / ns));
+    double dd = 0.0, nn = 0.0;
+    for (int q = 0; q < 4; ++q) {
+      for (int e = tid; e < n * n; e += BIG_NT) {
+        const int i = e / n, j = e - (e / n) * n;
+        const unsigned o = big_off(i, j);
+        const double yr = 0.5 * (sc * S.b[q].re[o] + X.b[q].re[o] / sc);
+        const double yi = 0.5 * (sc * S.b[q].im[o] + X.b[q].im[o] / sc);
+        const double dr = yr - S.b[q].re[o], di = yi - S.b[q].im[o];
+        dd += dr * dr + di * di;
+        nn += yr * yr + yi * yi;
+        S.b[q].re[o] = yr;
+        S.b[q].im[o] = yi;
+      }
+    }
+    const double d = sqrt(block_sum<BIG_NT>(dd, sh.sred, tid) / block_sum<BIG_NT>(nn, sh.sred, tid));
+    if (!(d == d)) return false;
+    if (d < 1e-13 || (d < 1e-9 && d > 0.5 * dprev)) return true;
+    dprev = d;
+  }
+  return dprev < 1e-9;
+}
+
+// eigenvalues of the pencil with |lambda| < r (-1 on failure); leaves the sign in S
+__device__ int huge_count_inside(double r, const Blk4& A, const BigMat& D1s, const Blk4& S,
+                                 const Blk4& X, const Blk4& Y, const BigMat* W, int n,
+                                 BigShared& sh, int tid) {
+  // Y <- A + rB (inverted below), S <- A - rB
+  for (int q = 0; q < 4; ++q) {
+    for (int e = tid; e < n * n; e += BIG_NT) {
+      const int i = e / n, j = e - (e / n) * n;
+      const unsigned o = big_off(i, j);
+      double br = 0.0, bi = 0.0;
+      if (q == 0) {
+        br = D1s.re[o];
+        bi = D1s.im[o];
+      } else if (q == 3) {
+        br = (i == j) ? 1.0 : 0.0;
+      }
+      Y.b[q].re[o] = A.b[q].re[o] + r * br;
+      Y.b[q].im[o] = A.b[q].im[o] + r * bi;
+      S.b[q].re[o] = A.b[q].re[o] - r * br;
+      S.b[q].im[o] = A.b[q].im[o] - r * bi;
+    }
+  }
+  __syncthreads();
+  if (!blk_inverse(Y, X, W, n, sh, tid)) return -1;
+  // C = (A + rB)^-1 (A - rB): block product X * S into Y
+  for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < 2; ++bj) {
+      big_gemm<OPN, OPN>(Y.b[2 * bi + bj], X.b[2 * bi], S.b[bj], n, 1.0, false, sh, tid);
+      big_gemm<OPN, OPN>(Y.b[2 * bi + bj], X.b[2 * bi + 1], S.b[2 + bj], n, 1.0, true, sh, tid);
+    }
+  for (int q = 0; q < 4; ++q) big_copy(S.b[q], Y.b[q], n, tid);
+  __syncthreads();
+  if (!huge_sign(S, X, W, n, sh, tid)) return -1;
+  double tr = 0.0;
+  for (int i = tid; i < n; i += BIG_NT) tr += S.b[0].re[big_off(i, i)] + S.b[3].re[big_off(i, i)];
+  const double trp = 0.5 * (2.0 * n - block_sum<BIG_NT>(tr, sh.sred, tid));
+  const double c = rint(trp);
+  return fabs(trp - c) < 1e-3 ? (int)c : -1;
+}
+
+__device__ void huge_eig_one(const EigArgs& a, int inst, double* slot, BigShared& sh, int tid) {
+  if (a.status[inst] == ST_OK) return;
+  const int n = a.n;
+  const size_t nn = (size_t)n * n;
+  const double* blk = a.lead + (size_t)a.set_of[inst] * 3 * nn * 2;
+  double* sl = big_fresh(slot);
+  const Blk4 A = blk4(sl, 0), S = blk4(sl, 4), X = blk4(sl, 8), Y = blk4(sl, 12);
+  const BigMat D1s = big_mat(sl, 16);
+  const BigMat W[5] = {big_mat(sl, 17), big_mat(sl, 18), big_mat(sl, 19), big_mat(sl, 20),
+                       big_mat(sl, 21)};
+  const BigMat D0 = big_mat(sl, 22);
+  // raw blocks: d00 -> D0, d01 -> D1s (scaled below), d10 -> A12 (scaled below)
+  big_gload(blk, n, n, D0, tid);
+  big_gload(blk + nn * 2, n, n, D1s, tid);
+  big_gload(blk + 2 * nn * 2, n, n, A.b[1], tid);
+  double zr, zi;
+  zsq(a.omega[inst], a.eta[inst], zr, zi);
+  __syncthreads();
+  double amax = 0.0;
+  for (int e = tid; e < (int)nn; e += BIG_NT) {
+    const int i = e / n, j = e - (e / n) * n;
+    const unsigned o = big_off(i, j);
+    amax = fmax(amax, fmax(hypot(D0.re[o], D0.im[o]), hypot(D1s.re[o], D1s.im[o])));
+  }
+  double gmax;
+  bool gbad;
+  block_max_any<BIG_NT>(amax, false, sh.sred, sh.sflag, tid, gmax, gbad);
+  const double s = 1.0 / fmax(gmax, hypot(zr, zi));
+  for (int e = tid; e < (int)nn; e += BIG_NT) {
+    const int i = e / n, j = e - (e / n) * n;
+    const unsigned o = big_off(i, j);
+    A.b[0].re[o] = -s * (D0.re[o] - (i == j ? zr : 0.0));
+    A.b[0].im[o] = -s * (D0.im[o] - (i == j ? zi : 0.0));
+    A.b[1].re[o] = -s * A.b[1].re[o];
+    A.b[1].im[o] = -s * A.b[1].im[o];
+    A.b[2].re[o] = (i == j) ? 1.0 : 0.0;
+    A.b[2].im[o] = 0.0;
+    A.b[3].re[o] = 0.0;
+    A.b[3].im[o] = 0.0;
+    D1s.re[o] = s * D1s.re[o];
+    D1s.im[o] = s * D1s.im[o];
+  }
+  __syncthreads();
+  const int m = 2 * n;
+  int c = huge_count_inside(1.0, A, D1s, S, X, Y, W, n, sh, tid);
+  if (c < 0) return;
+  if (c != n) {
+    const double sgn = c < n ? 1.0 : -1.0;
+    double lo = 0.0, hi = -1.0, h = 1e-8;
+    while (h < 0.5) {
+      c = huge_count_inside(1.0 + sgn * h, A, D1s, S, X, Y, W, n, sh, tid);
+      if (c >= 0 && (sgn > 0 ? c >= n : c <= n)) {
+        hi = h;
+        break;
+      }
+      lo = h;
+      h *= 4.0;
+    }
+    if (hi < 0.0) return;
+    for (int b = 0; b < 40 && c != n; ++b) {
+      h = lo > 0.0 ? sqrt(fmax(lo, 1e-12) * hi) : 0.25 * hi;
+      c = huge_count_inside(1.0 + sgn * h, A, D1s, S, X, Y, W, n, sh, tid);
+      if (c < 0) return;
+      if (sgn > 0 ? c >= n : c <= n)
+        hi = h;
+      else
+        lo = h;
+    }
+    if (c != n) return;
+  }
+  (void)m;
+  // T = P12 P22^-1 = -S12 ((I - S22)/2)^-1 / 2 -> Y11;  Sigma = d01 T;  G = (z - d00 - Sigma)^-1
+  const BigMat& P22 = W[0];
+  for (int e = tid; e < (int)nn; e += BIG_NT) {
+    const int i = e / n, j = e - (e / n) * n;
+    const unsigned o = big_off(i, j);
+    P22.re[o] = 0.5 * ((i == j ? 1.0 : 0.0) - S.b[3].re[o]);
+    P22.im[o] = -0.5 * S.b[3].im[o];
+  }
+  __syncthreads();
+  if (gj_inverse<double, BIG_NT, BIG, BIG_RG>(P22.re, P22.im, BIG_LD, n, sh.gj, tid) != ST_OK) return;
+  const BigMat& T = Y.b[0];
+  big_gemm<OPN, OPN>(T, S.b[1], P22, n, -0.5, false, sh, tid);
+  // d01 = D1s / s (unscaled) into W1, Sigma = d01 T into W2
+  for (int e = tid; e < (int)nn; e += BIG_NT) {
+    const int i = e / n, j = e - (e / n) * n;
+    const unsigned o = big_off(i, j);
+    W[1].re[o] = D1s.re[o] / s;
+    W[1].im[o] = D1s.im[o] / s;
+  }
+  __syncthreads();
+  big_gemm<OPN, OPN>(W[2], W[1], T, n, 1.0, false, sh, tid);
+  const BigMat& G = W[3];
+  for (int e = tid; e < (int)nn; e += BIG_NT) {
+    const int i = e / n, j = e - (e / n) * n;
+    const unsigned o = big_off(i, j);
+    G.re[o] = ((i == j ? zr : 0.0) - D0.re[o]) - W[2].re[o];
+    G.im[o] = ((i == j ? zi : 0.0) - D0.im[o]) - W[2].im[o];
+  }
+  __syncthreads();
+  if (gj_inverse<double, BIG_NT, BIG, BIG_RG>(G.re, G.im, BIG_LD, n, sh.gj, tid) != ST_OK) return;
+  bool bad = false;
+  for (int e = tid; e < (int)nn; e += BIG_NT) {
+    const int i = e / n, j = e - (e / n) * n;
+    const unsigned o = big_off(i, j);
+    bad |= !(isfinite(G.re[o]) && isfinite(G.im[o]));
+  }
+  block_max_any<BIG_NT>(0.0, bad, sh.sred, sh.sflag, tid, gmax, gbad);
+  if (gbad) return;
+  big_gstore(a.g_io + (size_t)inst * nn * 2, n, G, tid);
+  if (tid == 0) {
+    if (a.dos_out) {
+      double tr = 0.0;
+      for (int i = 0; i < n; ++i) tr += -2.0 * G.im[big_off(i, i)];
+      a.dos_out[inst] = (a.omega[inst] / M_PI) * tr;
+    }
+    a.status[inst] = ST_OK;
+  }
+}
+
+__global__ void __launch_bounds__(BIG_NT) k_sgf_eigen_huge(EigArgs a, double* pool) {
+  __shared__ BigShared sh;
+  double* slot = pool + (size_t)blockIdx.x * HUGE_SLOT;
+  for (int inst = blockIdx.x; inst < a.n_inst; inst += gridDim.x) {
+    huge_eig_one(a, inst, slot, sh, threadIdx.x);
+    __syncthreads();
+  }
+}
+
+}  // namespace negf

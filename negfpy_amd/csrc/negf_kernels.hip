@@ -1633,11 +1633,13 @@ struct BigPool {
   hipEvent_t done = nullptr;
 };
 static std::mutex g_big_mu;
-static BigPool g_big[64];
+static BigPool g_big[64];    // slots of BIG_SLOT doubles
+static BigPool g_huge[64];   // slots of HUGE_SLOT doubles (eigenmode route, 49..96 DOF)
 
 class BigUse {
  public:
-  explicit BigUse(hipStream_t st) : st_(st), lk_(g_big_mu) {
+  explicit BigUse(hipStream_t st, BigPool* table = g_big, size_t slot = BIG_SLOT)
+      : st_(st), lk_(g_big_mu) {
     int dev = 0;
     rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
     if (rc) return;
@@ -1645,7 +1647,7 @@ class BigUse {
       rc = fail(NEGF_ERR_ARG, "device index %d out of range", dev);
       return;
     }
-    p_ = &g_big[dev];
+    p_ = &table[dev];
     if (!p_->ptr) {
       int cus = 0, per = 0;
       rc = hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
@@ -1656,7 +1658,7 @@ class BigUse {
                      "occupancy");
       if (rc) return;
       const int slots = cus * (per > 0 ? per : 1);
-      const size_t bytes = (size_t)slots * BIG_SLOT * sizeof(double);
+      const size_t bytes = (size_t)slots * slot * sizeof(double);
       double* ptr = nullptr;
       rc = hip_check(hipMalloc(reinterpret_cast<void**>(&ptr), bytes), "96-class pool");
       if (rc) return;
@@ -1682,9 +1684,9 @@ class BigUse {
   BigPool* p_ = nullptr;
 };
 
-#define NEGF_BIG_LAUNCH(KERNEL, STREAM, ARGS)                                                \
+#define NEGF_POOL_LAUNCH(KERNEL, STREAM, ARGS, TABLE, SLOT)                                 \
   do {                                                                                      \
-    BigUse u_(STREAM);                                                                      \
+    BigUse u_(STREAM, TABLE, SLOT);                                                         \
     if (u_.rc) return u_.rc;                                                                \
     hipLaunchKernelGGL(KERNEL, dim3(u_.grid((ARGS).n_inst)), dim3(BIG_NT), 0, (STREAM),     \
                        (ARGS), u_.pool());                                                  \
@@ -1692,6 +1694,7 @@ class BigUse {
     if (e_ != hipSuccess) return hip_check(e_, #KERNEL " launch");                          \
     return u_.finish();                                                                     \
   } while (0)
+#define NEGF_BIG_LAUNCH(KERNEL, STREAM, ARGS) NEGF_POOL_LAUNCH(KERNEL, STREAM, ARGS, g_big, BIG_SLOT)
 
 #define NEGF_LAUNCH(KERNEL, GRID, NT, LDS, STREAM, ARGS)                                         \
   do {                                                                                          \
@@ -1939,10 +1942,11 @@ int negf_surface_gf_eigen_batched(void* stream, int n, int n_inst, const double*
   if (n <= 0 || n_inst < 0) return fail(NEGF_ERR_ARG, "invalid sizes (n=%d n_inst=%d)", n, n_inst);
   if (!lead || !set_of || !omega || !eta || !g_io || !status)
     return fail(NEGF_ERR_ARG, "null pointer argument");
-  if (n_inst == 0 || 2 * n > BIG) return NEGF_OK;
+  if (n_inst == 0 || n > BIG) return NEGF_OK;
   EigArgs a{n, n_inst, lead, set_of, omega, eta, g_io, dos_out, status};
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  NEGF_BIG_LAUNCH(k_sgf_eigen_big, st, a);
+  if (2 * n <= BIG) NEGF_BIG_LAUNCH(k_sgf_eigen_big, st, a);
+  NEGF_POOL_LAUNCH(k_sgf_eigen_huge, st, a, g_huge, HUGE_SLOT);
 }
 
 int negf_surface_refine_batched(void* stream, int n, int n_inst, const double* lead,

@@ -45,7 +45,10 @@ def load_run(name: str, tmp: Path):
 # GF solves the surface Riccati equation 100x less accurately at the worst k point,
 # pinned by test_ifc_pipeline.py).  Other runs agree to 1e-9.
 METHOD_SPREAD = {"stanene": 2e-8, "phosphorene": 1e-7, "borophene_b_2x1_pbe_transmission_y": 2e-8,
-                 "tan_transmission": 1e-7}
+                 "tan_transmission": 1e-7, "borophene_b_beta12_transmission_area_y": 5e-6}
+# borophene-beta12 along y (90 DOF): near omega index 42 the reference eigenmode
+# route's own surface GFs solve the surface Riccati equation only to ~3e-6 (pinned by
+# test_ifc_pipeline.py::test_beta12_y_eigen_route_conditioning), hence its bound.
 # (run, omega index) of the largest route spread of each material
 WORST = (("20260226_030629Z_stanene_1l_orthogonal_transmission", 1),
          ("20260226_032511Z_phosphorene_p_zerostrain_transmission_y_area", 103),
@@ -59,11 +62,12 @@ WORST = (("20260226_030629Z_stanene_1l_orthogonal_transmission", 1),
 # larger eta, as the reference's sancho_rubio route would.  The omegas are the ones
 # whose smallest-eta acceptance count falls below the file's; the listed indices are
 # pinned on the CPU (test_ifc_pipeline.py::test_study_points_where_decimation_fails).
-SR_UNSTABLE = {"20260226_035209Z_borophene_b_beta12_transmission_area_y": (1, 88)}
+SR_UNSTABLE: dict = {}
 # Runs whose decimation failures the eigenmode route (blocks <= 48) takes over:
 # the pinned omega indices are decimation failures of the reference's own
 # decimation route, and the run matches its file at every omega.
-EIGEN_ROUTE = {"20260225_113114Z_tan_transmission_x": (1,),
+EIGEN_ROUTE = {"20260226_035209Z_borophene_b_beta12_transmission_area_y": (1,),
+               "20260225_113114Z_tan_transmission_x": (1,),
                "20260225_114126Z_tan_transmission_y": (1,),
                "20260225_115137Z_tan_transmission_z": (2,)}
 

@@ -18,6 +18,7 @@ from negfpy_amd import _lib
 from negfpy_amd.core import _engine, surface_gf
 from negfpy_amd.core.surface_gf import pack_lead
 from negfpy_amd.core.types import LeadBlocks
+from negfpy_amd.modeling.units import qe_omega_to_cm1
 from negfpy_amd.workflows import build_omega_grid, build_transport_model, prepare_kmesh
 from negfpy_amd.workloads import omega_max, random_lead
 from oracle import negf_oracle as o
@@ -26,7 +27,7 @@ from study_runs_util import load_run
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("n", [6, 12, 24, 47, 48])
+@pytest.mark.parametrize("n", [6, 12, 24, 47, 48, 60, 72, 90])
 def test_route_equals_decimation_on_regular_leads(gpu, n):
     d00, d01 = random_lead(700 + n, n)
     wm = omega_max(LeadBlocks(d00=d00, d01=d01))
@@ -54,13 +55,19 @@ def test_route_equals_decimation_on_regular_leads(gpu, n):
     np.testing.assert_allclose(dos.cpu().numpy(), ref.dos.cpu().numpy(), rtol=1e-8, atol=0)
 
 
-def test_route_matches_eigen_reference_where_decimation_fails(gpu, tmp_path):
-    cfg, _, _, _ = load_run("20260225_113114Z_tan_transmission_x", tmp_path)
+@pytest.mark.parametrize("name,idx,step", [("20260225_113114Z_tan_transmission_x", 6, 1),
+                                           ("20260226_035209Z_borophene_b_beta12_transmission_area_y",
+                                            88, 3)])
+def test_route_matches_eigen_reference_where_decimation_fails(gpu, tmp_path, name, idx, step):
+    """TaN (24 DOF, pencil 48) and borophene-beta12 along y (90 DOF, pencil 180 as
+    2 x 2 blocks)."""
+    cfg, _, _, _ = load_run(name, tmp_path)
     m = build_transport_model(cfg)
     km = prepare_kmesh(cfg)
-    w = float(build_omega_grid(cfg, m["ifc"])[0][6])
+    w = float(build_omega_grid(cfg, m["ifc"])[0][idx])
+    c = float(np.asarray(qe_omega_to_cm1(w)))
     checked = 0
-    for k in km["low"]["kpoints"]:
+    for k in km["low" if c <= km["low_cm1"] else "high"]["kpoints"][::step]:
         d00, d01, d10 = m["lead_left"].blocks_builder(k)
         try:
             o.sancho_rubio(w, d00, d01, d10, eta=1e-8)

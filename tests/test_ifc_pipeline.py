@@ -275,3 +275,69 @@ def test_phonopy_reader_errors(tmp_path):
         read_ifc(tmp_path / "FORCE_CONSTANTS", reader="phonopy")
     with pytest.raises(ValueError, match="Unsupported phonopy source dict"):
         read_ifc({"foo": 1}, reader="phonopy")
+
+
+def test_beta12_y_eigen_route_conditioning(tmp_path):
+    """At omega index 42 of the 90-DOF borophene-beta12-y run, for the left lead at
+    k_y = 0.3 pi, decimation fails and the reference's eigenmode route leaves a Riccati
+    residual above 1e-6, while the spectral-projector route (restated here in numpy)
+    reaches below 2e-7: the study's spread there is the eigenmode route's error."""
+    cfg, rows, _, _ = load_run("20260226_035209Z_borophene_b_beta12_transmission_area_y", tmp_path)
+    m = W.build_transport_model(cfg)
+    w = float(W.build_omega_grid(cfg, m["ifc"])[0][42])
+    d00, d01, d10 = m["lead_left"].blocks_builder((0.3 * np.pi,))
+    with pytest.raises(RuntimeError):
+        o.sancho_rubio(w, d00, d01, d10, eta=1e-8)
+    g_ge = o.surface_gf(w, d00, d01, d10, eta=1e-8, method="generalized_eigen_svd")
+    assert o.riccati_residual(w, d00, d01, d10, 1e-8, g_ge) > 1e-6
+    g = _projector_gf(w, 1e-8, d00, d01, d10)
+    assert o.riccati_residual(w, d00, d01, d10, 1e-8, g) < 2e-7
+
+
+def _projector_gf(w, eta, d00, d01, d10):
+    """Surface GF from the n smallest-modulus eigenmodes of the lead pencil by the
+    matrix sign function of (A + rB)^-1 (A - rB), r searched until trace P = n."""
+    n = d00.shape[0]
+    eye, zero = np.eye(n), np.zeros((n, n))
+    z = (w + 1j * eta) ** 2
+    s = 1.0 / max(np.abs(d00).max(), np.abs(d01).max(), abs(z))
+    a = np.block([[-s * (d00 - z * eye), -s * d10], [eye, zero]])
+    b = np.block([[s * d01, zero], [zero, eye]])
+
+    def sign(c):
+        x, prev = c, np.inf
+        for _ in range(100):
+            xi = np.linalg.inv(x)
+            sc = np.sqrt(np.linalg.norm(xi) / np.linalg.norm(x))
+            xn = 0.5 * (sc * x + xi / sc)
+            d = np.linalg.norm(xn - x) / np.linalg.norm(xn)
+            x = xn
+            if d < 1e-13 or (d < 1e-9 and d > 0.5 * prev):
+                return x
+            prev = d
+        return x
+
+    def count(r):
+        sg = sign(np.linalg.solve(a + r * b, a - r * b))
+        return sg, int(round((2 * n - np.trace(sg).real) / 2))
+
+    sg, c = count(1.0)
+    if c != n:
+        sgn = 1.0 if c < n else -1.0
+        lo, h = 0.0, 1e-8
+        while True:
+            sg, c = count(1.0 + sgn * h)
+            if (c >= n) if sgn > 0 else (c <= n):
+                hi = h
+                break
+            lo, h = h, 4 * h
+        while c != n:
+            h = np.sqrt(max(lo, 1e-12) * hi) if lo > 0 else hi / 4
+            sg, c = count(1.0 + sgn * h)
+            if (c >= n) if sgn > 0 else (c <= n):
+                hi = h
+            else:
+                lo = h
+    p = 0.5 * (np.eye(2 * n) - sg)
+    t = p[:n, n:] @ np.linalg.inv(p[n:, n:])
+    return np.linalg.inv(z * eye - d00 - d01 @ t)
