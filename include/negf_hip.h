@@ -243,6 +243,32 @@ size_t negf_transmission_workspace_bytes(const negf_transmission_problem* p);
 int negf_transmission_batched(void* stream, const negf_transmission_problem* p, void* workspace,
                               size_t workspace_bytes);
 
+/* Problems per negf_transmission_multi_batched call. */
+#define NEGF_MULTI_MAX_PROBLEMS 8
+
+/*
+ * Several transmission problems of different block sizes (up to 96 DOF) solved
+ * together, FP64: every instance of every problem goes through one persistent
+ * kernel launch (two, split around the eigenmode fallback, when a problem asks for
+ * NEGF_SURFACE_EIGEN) whose workgroups take tickets of same-problem instances from
+ * an atomic work queue, most expensive problems first; per-instance convergence
+ * flags let the instances of a ticket stop decimating at their own step.  Outputs
+ * per problem as from negf_transmission_batched on that problem (the same
+ * per-stage arithmetic; each problem's blocks use the smallest class that holds
+ * max(n, n_lead_l, n_lead_r), whatever negf_set_block_class says).  Replaces the
+ * per-point loops of transmission_kavg* / heat_current_* over k-points whose
+ * layouts differ (src/negfpy/core/negf.py:360-688), which the reference runs one
+ * point at a time.  Problems with n_inst = 0 are skipped; precision must be
+ * NEGF_FP64; refine_steps is unused.  Capturable in a hipGraph when no problem
+ * needs blocks above 48 DOF (the class-96 pool orders calls through an event).
+ *   workspace >= negf_transmission_multi_workspace_bytes(n_problems, problems)
+ */
+size_t negf_transmission_multi_workspace_bytes(int n_problems,
+                                               const negf_transmission_problem* problems);
+int negf_transmission_multi_batched(void* stream, int n_problems,
+                                    const negf_transmission_problem* problems, void* workspace,
+                                    size_t workspace_bytes);
+
 /*
  * Thermal conductance G(T_i) = (1/2pi) int k_B T(w) x^2 e^-x / (1 - e^-x)^2 dw,
  * x = hbar w / (k_B T_i), trapezoid rule over the given grid (w in rad/s);

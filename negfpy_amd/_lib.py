@@ -25,6 +25,7 @@ FP64 = 0
 MIXED = 1
 SURFACE_DECIMATION = 0
 SURFACE_EIGEN = 1
+MULTI_MAX_PROBLEMS = 8   # NEGF_MULTI_MAX_PROBLEMS
 
 OK = 0
 ST_OK = 0
@@ -85,6 +86,10 @@ _SIGS = {
     "negf_transmission_trace_batched": ([_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp], _c_int),
     "negf_merge_status": ([_vp, _c_int, _vp, _vp], _c_int),
     "negf_transmission_workspace_bytes": ([ctypes.POINTER(TransmissionProblem)], ctypes.c_size_t),
+    "negf_transmission_multi_workspace_bytes": ([_c_int, ctypes.POINTER(TransmissionProblem)],
+                                                ctypes.c_size_t),
+    "negf_transmission_multi_batched": (
+        [_vp, _c_int, ctypes.POINTER(TransmissionProblem), _vp, ctypes.c_size_t], _c_int),
     "negf_transmission_batched": (
         [_vp, ctypes.POINTER(TransmissionProblem), _vp, ctypes.c_size_t], _c_int),
     "negf_thermal_conductance": ([_vp, _c_int, _vp, _vp, _c_int, _vp, _vp], _c_int),

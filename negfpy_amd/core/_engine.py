@@ -324,6 +324,102 @@ class TransmissionSolver:
         return out
 
 
+_multi_ws: dict = {}
+
+
+def _multi_prepare(jobs, outs):
+    prepared = []
+    for j, (solver, set_of, omega, eta_lead, eta_dev) in enumerate(jobs):
+        if solver.precision != _lib.FP64:
+            raise ValueError("the persistent multi-problem launch runs FP64 only")
+        set_of = to_dev_i32(set_of)
+        omega = to_dev_f64(omega)
+        eta_lead = to_dev_f64(eta_lead)
+        eta_dev = eta_lead if eta_dev is None else to_dev_f64(eta_dev)
+        n_inst = int(omega.numel())
+        if not (set_of.numel() == eta_lead.numel() == eta_dev.numel() == n_inst):
+            raise ValueError("set_of, omega, eta_lead, eta_dev need one entry per instance")
+        out = outs[j] if outs is not None else solver.alloc_result(n_inst)
+        prepared.append((solver, set_of, omega, eta_lead, eta_dev, out))
+    chunks = []
+    for c0 in range(0, len(prepared), _lib.MULTI_MAX_PROBLEMS):
+        chunk = prepared[c0:c0 + _lib.MULTI_MAX_PROBLEMS]
+        arr = (_lib.TransmissionProblem * len(chunk))()
+        for i, (solver, so, om, el, ed, out) in enumerate(chunk):
+            arr[i] = solver._problem(int(om.numel()), so, om, el, ed, out)
+        need = int(_lib.load().negf_transmission_multi_workspace_bytes(len(chunk), arr))
+        chunks.append((arr, need))
+    return prepared, chunks
+
+
+def _multi_launch(prepared, chunks, ws):
+    lib = _lib.load()
+    for arr, need in chunks:
+        _lib.check(lib.negf_transmission_multi_batched(_lib.stream(), len(arr), arr,
+                                                       ws.data_ptr(), need))
+    for solver, _, _, _, _, out in prepared:
+        if solver.sets.share_leads:
+            out.iters_r.copy_(out.iters_l)
+            out.dos_r.copy_(out.dos_l)
+    return [p[-1] for p in prepared]
+
+
+def transmission_multi(jobs, outs=None) -> list:
+    """Several FP64 transmission problems of any block sizes solved together.
+
+    ``jobs``: sequence of ``(solver, set_of, omega, eta_lead, eta_dev)`` with a
+    :class:`TransmissionSolver` per problem (its own sets, block sizes and leads)
+    and that problem's instances.  Every instance of every problem goes through one
+    persistent kernel launch per ``MULTI_MAX_PROBLEMS`` problems
+    (``negf_transmission_multi_batched``); results as ``solver.run`` on each.
+    ``outs`` (optional) are preallocated results, one per job.
+    """
+    prepared, chunks = _multi_prepare(jobs, outs)
+    need = max((c[1] for c in chunks), default=0)
+    dev = device()
+    ws = _multi_ws.get(dev)
+    if ws is None or ws.numel() < need:
+        ws = torch.empty((max(need, 1),), dtype=torch.uint8, device=dev)
+        _multi_ws[dev] = ws
+    return _multi_launch(prepared, chunks, ws)
+
+
+class MultiTransmissionGraph:
+    """:func:`transmission_multi` captured as a hipGraph: the setup kernel, the
+    persistent launch and the result copies of every problem.  Inputs are owned by
+    the graph (``omega[j]``, ``eta_lead[j]``, ... per job; update in place), and so
+    is its workspace; ``replay()`` solves all problems again.  Problems must fit the
+    LDS classes (blocks up to 48 DOF): the 96-DOF pool orders its users through an
+    event, which stream capture does not allow."""
+
+    def __init__(self, jobs):
+        self.solvers = [j[0] for j in jobs]
+        self.set_of = [to_dev_i32(j[1]).clone() for j in jobs]
+        self.omega = [to_dev_f64(j[2]).clone() for j in jobs]
+        self.eta_lead = [to_dev_f64(j[3]).clone() for j in jobs]
+        self.eta_dev = [(el if j[4] is None else to_dev_f64(j[4])).clone()
+                        for j, el in zip(jobs, self.eta_lead)]
+        self.out = [s.alloc_result(int(w.numel())) for s, w in zip(self.solvers, self.omega)]
+        prepared, chunks = _multi_prepare(self._jobs(), self.out)
+        need = max((c[1] for c in chunks), default=0)
+        self.workspace = torch.empty((max(need, 1),), dtype=torch.uint8, device=device())
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            _multi_launch(prepared, chunks, self.workspace)
+        torch.cuda.current_stream().wait_stream(side)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            _multi_launch(prepared, chunks, self.workspace)
+
+    def _jobs(self):
+        return list(zip(self.solvers, self.set_of, self.omega, self.eta_lead, self.eta_dev))
+
+    def replay(self) -> list:
+        self.graph.replay()
+        return self.out
+
+
 class TransmissionGraph:
     """One batched transmission solve captured as a hipGraph (``torch.cuda.CUDAGraph``).
 

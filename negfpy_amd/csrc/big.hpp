@@ -195,7 +195,7 @@ __device__ __forceinline__ void big_store(const BigMat& C, int n, const BigTile&
 // decimation (surface_gf.py:18-81), slots: a, b, a', b', e, e_s, g, T1 = a g, T2 = b g
 // ---------------------------------------------------------------------------
 
-__device__ void big_sr_one(const SrArgs& a, int inst, double* slot, BigShared& sh, int tid) {
+__device__ __forceinline__ void big_sr_one(const SrArgs& a, int inst, double* slot, BigShared& sh, int tid) {
   if (a.redo && a.status[inst] == ST_OK) return;
   const int n = a.n;
   const size_t nn = (size_t)n * n;
@@ -353,7 +353,7 @@ __global__ void __launch_bounds__(BIG_NT) k_sr_big(SrArgs a, double* pool) {
 // self-energy Sigma = V g V^H embedded in the end layer (negf.py:72-97)
 // ---------------------------------------------------------------------------
 
-__device__ void big_sigma_one(const SigmaArgs& a, int inst, double* slot, BigShared& sh, int tid) {
+__device__ __forceinline__ void big_sigma_one(const SigmaArgs& a, int inst, double* slot, BigShared& sh, int tid) {
   const int nl = a.n_lead, nc = a.n_c, nd = a.n_dev;
   double2* out = reinterpret_cast<double2*>(a.sigma_out + (size_t)inst * nd * nd * 2);
   if (a.status_in && a.status_in[inst] != ST_OK) return;
@@ -400,7 +400,7 @@ __global__ void __launch_bounds__(BIG_NT) k_sigma_big(SigmaArgs a, double* pool)
 // recursive Green's function sweep for G_1N; slots: g (then Y), H, Q, S = H^H Y
 // ---------------------------------------------------------------------------
 
-__device__ void big_rgf_one(const RgfArgs& a, int inst, double* slot, BigShared& sh, int tid) {
+__device__ __forceinline__ void big_rgf_one(const RgfArgs& a, int inst, double* slot, BigShared& sh, int tid) {
   if (a.redo && a.status[inst] != 10 + ST_NONFINITE && a.status[inst] != 10 + ST_SINGULAR) return;
   if (a.status_in && a.status_in[inst] != ST_OK) {
     if (tid == 0) a.status[inst] = a.status_in[inst];
@@ -497,7 +497,7 @@ __global__ void __launch_bounds__(BIG_NT) k_rgf_big(RgfArgs a, double* pool) {
 // Gamma_R, G, P = Gamma_L G
 // ---------------------------------------------------------------------------
 
-__device__ void big_trace_one(const TraceArgs& a, int inst, double* slot, BigShared& sh, int tid) {
+__device__ __forceinline__ void big_trace_one(const TraceArgs& a, int inst, double* slot, BigShared& sh, int tid) {
   if (a.status_in && a.status_in[inst] != ST_OK) {
     if (tid == 0) a.t_out[inst] = NAN;
     return;

@@ -315,9 +315,14 @@ __device__ __forceinline__ void crecip1(T ar, T ai, T& rr, T& ri) {
 // x' = 0 in column k, and s the scaled pivot row (s_k = 1 / pivot).  All threads
 // must call it; it ends with a barrier.  Returns ST_OK, ST_SINGULAR or
 // ST_NONFINITE.
-template <typename T, int NT, int NMAX, int RG, typename PT>
-__device__ __forceinline__ int gj_inverse(PT __restrict__ re, PT __restrict__ im, int ld, int n,
-                                          GjBuf<T, NMAX>& b, int tid) {
+//
+// LS (lockstep): several sub-blocks of one workgroup invert their own matrices
+// of the same n side by side, so every sub-block must pass the same n + 2
+// barriers: a failed pivot then keeps stepping through the remaining barriers
+// instead of returning early (gj_inverse_ls).
+template <typename T, int NT, int NMAX, int RG, bool LS, typename PT>
+__device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, int ld, int n,
+                                       GjBuf<T, NMAX>& b, int tid) {
   static_assert(NMAX % RG == 0, "row groups must tile NMAX");
   static_assert(WAVE % RG == 0, "a column group must sit inside one wave");
   constexpr int MAXR = NMAX / RG;
@@ -392,7 +397,11 @@ __device__ __forceinline__ int gj_inverse(PT __restrict__ re, PT __restrict__ im
     const Pivot<T> pk = b.piv[par];
     const int flag = __builtin_amdgcn_readfirstlane(pk.flag);
     if (flag != ST_OK) {
-      __syncthreads();
+      if constexpr (LS) {
+        for (int r = k; r <= n; ++r) __syncthreads();
+      } else {
+        __syncthreads();
+      }
       return flag;
     }
     const int p = __builtin_amdgcn_readfirstlane(pk.p);
@@ -447,6 +456,24 @@ __device__ __forceinline__ int gj_inverse(PT __restrict__ re, PT __restrict__ im
   }
   __syncthreads();
   return ST_OK;
+}
+
+template <typename T, int NT, int NMAX, int RG, typename PT>
+__device__ __forceinline__ int gj_inverse(PT __restrict__ re, PT __restrict__ im, int ld, int n,
+                                          GjBuf<T, NMAX>& b, int tid) {
+  return gj_core<T, NT, NMAX, RG, false>(re, im, ld, n, b, tid);
+}
+
+// Lockstep form: a sub-block with live == false (finished, failed or without an
+// instance) passes the n + 2 barriers of an inversion and touches nothing.
+template <typename T, int NT, int NMAX, int RG, typename PT>
+__device__ __forceinline__ int gj_inverse_ls(PT __restrict__ re, PT __restrict__ im, int ld, int n,
+                                             GjBuf<T, NMAX>& b, int tid, bool live) {
+  if (!live) {
+    for (int r = 0; r < n + 2; ++r) __syncthreads();
+    return ST_SINGULAR;
+  }
+  return gj_core<T, NT, NMAX, RG, true>(re, im, ld, n, b, tid);
 }
 
 // Workgroup max-reduction of a non-negative value plus an "any" flag.

@@ -1556,6 +1556,7 @@ __global__ void __launch_bounds__(256) k_heat_current(int n_w, const double* __r
 }  // namespace negf
 
 #include "big.hpp"
+#include "persist.hpp"
 
 namespace negf {
 
@@ -2216,6 +2217,226 @@ int negf_transmission_batched(void* stream, const negf_transmission_problem* p, 
   if (rc) return rc;
   return negf_transmission_trace_batched(stream, p->n, p->n_inst, sig_l, sig_r, g1n, p->status,
                                          p->t_out);
+}
+
+// ---------------------------------------------------------------------------
+// several problems of different block sizes in one persistent launch
+// ---------------------------------------------------------------------------
+
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+size_t negf_transmission_multi_workspace_bytes(int n_problems,
+                                               const negf_transmission_problem* problems) {
+  if (n_problems <= 0 || !problems) return 0;
+  size_t b = align256(sizeof(PlArgs)) + 256;  // problem table, ticket counter
+  for (int i = 0; i < n_problems; ++i)
+    b += align256(negf_transmission_workspace_bytes(&problems[i]));
+  return b;
+}
+
+static int pl_check(const negf_transmission_problem& p, int i) {
+  if (p.precision != NEGF_FP64)
+    return fail(NEGF_ERR_ARG, "problem %d: the persistent multi-problem launch is FP64 only", i);
+  if (p.n <= 0 || p.n_layers <= 0 || p.n_inst < 0 || p.n_lead_l <= 0 || p.n_lead_r <= 0 ||
+      p.n_c_l <= 0 || p.n_c_r <= 0 || p.n_c_l > p.n || p.n_c_r > p.n || p.maxiter < 0)
+    return fail(NEGF_ERR_ARG,
+                "problem %d: invalid sizes (n=%d layers=%d n_inst=%d leads %d/%d contacts %d/%d "
+                "maxiter=%d)",
+                i, p.n, p.n_layers, p.n_inst, p.n_lead_l, p.n_lead_r, p.n_c_l, p.n_c_r, p.maxiter);
+  if (p.share_leads && p.n_lead_r != p.n_lead_l)
+    return fail(NEGF_ERR_ARG, "problem %d: shared leads of different sizes", i);
+  if (p.surface_route != NEGF_SURFACE_DECIMATION && p.surface_route != NEGF_SURFACE_EIGEN)
+    return fail(NEGF_ERR_ARG, "problem %d: unknown surface route %d", i, p.surface_route);
+  int m = p.n > p.n_lead_l ? p.n : p.n_lead_l;
+  m = m > p.n_lead_r ? m : p.n_lead_r;
+  if (m > BIG) return fail(NEGF_ERR_ARG, "problem %d: block size %d exceeds %d", i, m, BIG);
+  if (p.n_inst == 0) return NEGF_OK;
+  if (!p.lead_l || (!p.share_leads && !p.lead_r) || !p.v_l || !p.v_r || !p.idx_l || !p.idx_r ||
+      !p.onsite || (p.n_layers > 1 && !p.coupling) || !p.set_of || !p.omega || !p.eta_lead ||
+      !p.eta_dev || !p.t_out || !p.iters_l || (!p.share_leads && !p.iters_r) || !p.status)
+    return fail(NEGF_ERR_ARG, "problem %d: null pointer argument", i);
+  return NEGF_OK;
+}
+
+static int pl_launch(hipStream_t st, const PlArgs* table, int* counter, int phase, bool reset,
+                     double* pool, int grid, size_t lds) {
+  if (reset) {
+    hipLaunchKernelGGL(k_pl_reset, dim3(1), dim3(1), 0, st, counter);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_check(e, "k_pl_reset launch");
+  }
+  hipLaunchKernelGGL(k_persistent, dim3(grid), dim3(PL_NT), lds, st, table, phase, pool);
+  return hip_check(hipGetLastError(), "k_persistent launch");
+}
+
+int negf_transmission_multi_batched(void* stream, int n_problems,
+                                    const negf_transmission_problem* problems, void* workspace,
+                                    size_t workspace_bytes) {
+  if (n_problems < 0 || n_problems > NEGF_MULTI_MAX_PROBLEMS)
+    return fail(NEGF_ERR_ARG, "%d problems (at most %d per call)", n_problems,
+                NEGF_MULTI_MAX_PROBLEMS);
+  if (n_problems == 0) return NEGF_OK;
+  if (!problems) return fail(NEGF_ERR_ARG, "null problem list");
+  for (int i = 0; i < n_problems; ++i) {
+    const int rc = pl_check(problems[i], i);
+    if (rc) return rc;
+  }
+  const size_t need = negf_transmission_multi_workspace_bytes(n_problems, problems);
+  if (!workspace || workspace_bytes < need)
+    return fail(NEGF_ERR_ARG, "workspace too small (%zu < %zu bytes)", workspace_bytes, need);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+
+  // problems in order of decreasing ticket cost (class^3 x decimations and layers)
+  int order[NEGF_MULTI_MAX_PROBLEMS], cls[NEGF_MULTI_MAX_PROBLEMS];
+  double cost[NEGF_MULTI_MAX_PROBLEMS];
+  int np = 0;
+  for (int i = 0; i < n_problems; ++i) {
+    const negf_transmission_problem& p = problems[i];
+    if (p.n_inst == 0) continue;
+    int m = p.n > p.n_lead_l ? p.n : p.n_lead_l;
+    m = m > p.n_lead_r ? m : p.n_lead_r;
+    cls[i] = default_nmax(m, true);
+    const double c = (double)cls[i];
+    cost[i] = c * c * c * ((p.share_leads ? 1.0 : 2.0) * 30.0 * 7.0 + 4.0 * p.n_layers);
+    int k = np++;
+    while (k > 0 && cost[order[k - 1]] < cost[i]) {
+      order[k] = order[k - 1];
+      --k;
+    }
+    order[k] = i;
+  }
+  if (np == 0) return NEGF_OK;
+
+  PlArgs a{};
+  unsigned char* w = reinterpret_cast<unsigned char*>(workspace);
+  PlArgs* table = reinterpret_cast<PlArgs*>(w);
+  w += align256(sizeof(PlArgs));
+  a.counter = reinterpret_cast<int*>(w);
+  w += 256;
+  a.n_prob = np;
+  int tickets = 0;
+  size_t lds = 0;
+  bool big = false, eig = false;
+  for (int i = 0; i < n_problems; ++i) {
+    // workspace slices in the caller's order, laid out as in negf_transmission_batched
+    const negf_transmission_problem& p = problems[i];
+    unsigned char* base = w;
+    w += align256(negf_transmission_workspace_bytes(&p));
+    if (p.n_inst == 0) continue;
+    int slot = 0;
+    while (order[slot] != i) ++slot;
+    PlProb& q = a.prob[slot];
+    const size_t ni = (size_t)p.n_inst, nd = (size_t)p.n * p.n;
+    q.g_l = reinterpret_cast<double*>(base);
+    base += ni * p.n_lead_l * p.n_lead_l * 16;
+    q.g_r = q.g_l;
+    if (!p.share_leads) {
+      q.g_r = reinterpret_cast<double*>(base);
+      base += ni * p.n_lead_r * p.n_lead_r * 16;
+    }
+    q.sig_l = reinterpret_cast<double*>(base);
+    base += ni * nd * 16;
+    q.sig_r = reinterpret_cast<double*>(base);
+    base += ni * nd * 16;
+    q.g1n = reinterpret_cast<double*>(base);
+    base += ni * nd * 16;
+    q.st_l = reinterpret_cast<int32_t*>(base);
+    base += ni * sizeof(int32_t);
+    q.st_r = reinterpret_cast<int32_t*>(base);
+    q.n = p.n;
+    q.n_layers = p.n_layers;
+    q.n_inst = p.n_inst;
+    q.n_lead_l = p.n_lead_l;
+    q.n_lead_r = p.n_lead_r;
+    q.n_c_l = p.n_c_l;
+    q.n_c_r = p.n_c_r;
+    q.share = p.share_leads ? 1 : 0;
+    q.maxiter = p.maxiter;
+    q.cls = cls[i];
+    q.route = p.surface_route;
+    q.tol = p.tol;
+    q.lead_l = p.lead_l;
+    q.lead_r = p.share_leads ? p.lead_l : p.lead_r;
+    q.v_l = p.v_l;
+    q.v_r = p.v_r;
+    q.idx_l = p.idx_l;
+    q.idx_r = p.idx_r;
+    q.onsite = p.onsite;
+    q.coupling = p.coupling ? p.coupling : p.onsite;  // never read for one layer
+    q.set_of = p.set_of;
+    q.omega = p.omega;
+    q.eta_lead = p.eta_lead;
+    q.eta_dev = p.eta_dev;
+    q.t_out = p.t_out;
+    q.dos_l = p.dos_l;
+    q.dos_r = p.dos_r;
+    q.iters_l = p.iters_l;
+    q.iters_r = p.iters_r;
+    q.status = p.status;
+    big = big || cls[i] == BIG;
+    eig = eig || p.surface_route == NEGF_SURFACE_EIGEN;
+    const size_t l = pl_lds_bytes(cls[i]);
+    lds = l > lds ? l : lds;
+  }
+  for (int k = 0; k < np; ++k) {
+    PlProb& q = a.prob[k];
+    q.group = q.cls == BIG ? 1
+              : q.cls == 8 ? PlCfg<8>::G
+              : q.cls == 16 ? PlCfg<16>::G
+              : q.cls == 24 ? PlCfg<24>::G
+              : q.cls == 32 ? PlCfg<32>::G
+                            : PlCfg<48>::G;
+    q.ticket0 = tickets;
+    tickets += (q.n_inst + q.group - 1) / q.group;
+  }
+  a.n_tickets = tickets;
+
+  int dev = 0, cus = 0, per = 0;
+  int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  rc = hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+  if (rc) return rc;
+  rc = hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_persistent),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                 "k_persistent attribute");
+  if (rc) return rc;
+  rc = hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                     &per, reinterpret_cast<const void*>(&k_persistent), PL_NT, lds),
+                 "k_persistent occupancy");
+  if (rc) return rc;
+  if (per < 1) return fail(NEGF_ERR_HIP, "k_persistent cannot be resident with %zu B of LDS", lds);
+  int grid = cus * per;
+  grid = tickets < grid ? tickets : grid;
+
+  hipLaunchKernelGGL(k_pl_setup, dim3(1), dim3(64), 0, st, a, table);
+  rc = hip_check(hipGetLastError(), "k_pl_setup launch");
+  if (rc) return rc;
+  // one launch per phase; the eigenmode fallback (instances whose decimation
+  // failed, eigenmode-route problems only) runs between the lead and device phases
+  auto run_phase = [&](int phase, bool reset) -> int {
+    if (!big) return pl_launch(st, table, a.counter, phase, reset, nullptr, grid, lds);
+    BigUse u(st);
+    if (u.rc) return u.rc;
+    const int r = pl_launch(st, table, a.counter, phase, reset, u.pool(), u.grid(grid), lds);
+    if (r) return r;
+    return u.finish();
+  };
+  if (!eig) return run_phase(PL_ALL, false);
+  rc = run_phase(PL_LEADS, false);
+  if (rc) return rc;
+  for (int k = 0; k < np; ++k) {
+    const PlProb& q = a.prob[k];
+    if (q.route != NEGF_SURFACE_EIGEN) continue;
+    rc = negf_surface_gf_eigen_batched(stream, q.n_lead_l, q.n_inst, q.lead_l, q.set_of, q.omega,
+                                       q.eta_lead, q.g_l, q.dos_l, q.st_l);
+    if (rc) return rc;
+    if (!q.share) {
+      rc = negf_surface_gf_eigen_batched(stream, q.n_lead_r, q.n_inst, q.lead_r, q.set_of,
+                                         q.omega, q.eta_lead, q.g_r, q.dos_r, q.st_r);
+      if (rc) return rc;
+    }
+  }
+  return run_phase(PL_DEVICE, true);
 }
 
 }  // extern "C"

@@ -1,0 +1,654 @@
+// One persistent launch over several transmission problems of different block
+// sizes (FP64): negf_transmission_multi_batched.
+//
+// Every (problem, instance) pair is solved by the whole transmission pipeline of
+// the staged path (negf_transmission_batched: both surface GFs by decimation, both
+// self-energies, the recursive sweep for G_1N and the trace), fused per instance
+// and run by one 768-thread workgroup per CU that takes tickets from an atomic
+// work queue until none are left.  A ticket is G consecutive instances of one
+// problem, solved side by side by G sub-blocks of that problem's block-size class
+// (G x the class's own workgroup size <= 768 threads; 12 x 8, 11 x 16, 4 x 24,
+// 3 x 32, 1 x 48 DOF in LDS, 1 x 96 DOF in a device-memory pool slot).  Tickets of
+// the most expensive problems come first, so the queue ends on short tickets.
+//
+// Sub-blocks share the workgroup barrier, so each pipeline stage is written in
+// lockstep form: every sub-block passes the same barriers, and a sub-block that
+// has no instance, has converged, or has failed keeps passing them with its work
+// switched off.  The decimation loop runs until no sub-block of the ticket is
+// still iterating (a per-instance convergence flag in LDS); each instance stops
+// at its own step, so its result and step count are those of the staged kernel.
+// The arithmetic of every stage is the staged kernels' own, in the same order.
+
+#pragma once
+
+#include "big.hpp"
+
+namespace negf {
+
+constexpr int PL_NT = 768;
+constexpr int PL_MAX_PROBLEMS = NEGF_MULTI_MAX_PROBLEMS;
+enum { PL_ALL = 0, PL_LEADS = 1, PL_DEVICE = 2 };
+
+// sub-blocks per workgroup of each LDS class (limited by the 160 KB of LDS for 16 DOF)
+template <int NMAX>
+struct PlCfg;
+template <>
+struct PlCfg<8> {
+  static constexpr int G = 12;
+};
+template <>
+struct PlCfg<16> {
+  static constexpr int G = 11;
+};
+template <>
+struct PlCfg<24> {
+  static constexpr int G = 4;
+};
+template <>
+struct PlCfg<32> {
+  static constexpr int G = 3;
+};
+template <>
+struct PlCfg<48> {
+  static constexpr int G = 1;
+};
+
+template <int NMAX>
+struct PlSub {
+  static constexpr int NT = Cfg<NMAX>::NT;
+  static constexpr int G = PlCfg<NMAX>::G;
+  using L = Lds<double, NMAX, NT, 3>;
+  static constexpr size_t STRIDE = (L::bytes() + 15) & ~size_t(15);
+  static constexpr size_t BYTES = STRIDE * G;
+  static_assert(G * NT <= PL_NT, "sub-blocks exceed the workgroup");
+};
+
+// One problem of the launch; the workspace slices are those of the staged path.
+struct PlProb {
+  int n, n_layers, n_inst, n_lead_l, n_lead_r, n_c_l, n_c_r, share, maxiter;
+  int cls;      // block-size class: 8, 16, 24, 32, 48 or BIG
+  int group;    // instances per ticket
+  int ticket0;  // first ticket of this problem
+  int route;    // NEGF_SURFACE_DECIMATION or NEGF_SURFACE_EIGEN
+  double tol;
+  const double *lead_l, *lead_r, *v_l, *v_r;
+  const int32_t *idx_l, *idx_r;
+  const double *onsite, *coupling;
+  const int32_t* set_of;
+  const double *omega, *eta_lead, *eta_dev;
+  double *t_out, *dos_l, *dos_r;
+  int32_t *iters_l, *iters_r, *status;
+  double *g_l, *g_r, *sig_l, *sig_r, *g1n;
+  int32_t *st_l, *st_r;
+};
+
+struct PlArgs {
+  int n_prob, n_tickets;
+  int* counter;  // ticket counter, 0 at launch
+  PlProb prob[PL_MAX_PROBLEMS];
+};
+
+struct PlShared {
+  int run[16];          // sub-block still iterating (decimation loop)
+  int ticket;
+  double idle_red[PL_NT / WAVE];  // reduction scratch of threads outside every sub-block
+  int idle_flag[PL_NT / WAVE];
+};
+
+// ---------------------------------------------------------------------------
+// lockstep stages (the staged kernels k_sancho_rubio, k_self_energy, k_rgf and
+// k_trace in FP64, with their work switched off where a sub-block has none)
+// ---------------------------------------------------------------------------
+
+// Decimation of one lead; returns the instance status.
+template <int NMAX, class L>
+__device__ int pl_sr(const SrArgs& a, int inst, bool active, L& s, int tid, PlShared& ps, int sub,
+                     int nsub) {
+  using T = double;
+  constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
+  constexpr int LD = L::LD;
+  const int n = a.n;
+  const size_t nn = (size_t)n * n;
+  const double* blk = a.lead + (active ? (size_t)a.set_of[inst] * 3 * nn * 2 : 0);
+  if (active) s.zero(tid);
+  __syncthreads();
+  if (active) {
+    gload<T, NT>(blk + nn * 2, n, n, s.re[0], s.im[0], LD, tid);
+    gload<T, NT>(blk + 2 * nn * 2, n, n, s.re[1], s.im[1], LD, tid);
+  }
+  const Tile<TM> t(n, n, tid);
+  T er[TM][TM], ei[TM][TM], sr[TM][TM], si[TM][TM];
+  tzero<T, TM>(er, ei);
+  if (active && t.own) {
+    const double2* d00 = reinterpret_cast<const double2*>(blk);
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+      for (int y = 0; y < TM; ++y) {
+        const int i = t.i0 + x, j = t.j0 + y;
+        if (i < n && j < n) {
+          const double2 v = d00[i * n + j];
+          er[x][y] = v.x;
+          ei[x][y] = v.y;
+        }
+      }
+  }
+#pragma unroll
+  for (int x = 0; x < TM; ++x)
+#pragma unroll
+    for (int y = 0; y < TM; ++y) {
+      sr[x][y] = er[x][y];
+      si[x][y] = ei[x][y];
+    }
+  double zr = 0.0, zi = 0.0;
+  if (active) zsq(a.omega[inst], a.eta[inst], zr, zi);
+  const T tol = a.tol;
+  __syncthreads();
+
+  int status = ST_OK;
+  int it = 0;
+  bool conv = false;
+  bool run = active;
+  for (;;) {
+    if (run && it >= a.maxiter) run = false;
+    if (tid == 0 && sub < nsub) ps.run[sub] = run ? 1 : 0;
+    __syncthreads();
+    int any = 0;
+    for (int q = 0; q < nsub; ++q) any |= ps.run[q];
+    any = __builtin_amdgcn_readfirstlane(any);  // the same in every lane
+    // the next write to ps.run follows this step's inversion barriers
+    if (!any) break;
+    if (run) ++it;
+    // G = z - e
+    if (run && t.own) {
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          const int i = t.i0 + x, j = t.j0 + y;
+          if (i < n && j < n) {
+            s.re[2][i * LD + j] = (i == j ? zr : T(0)) - er[x][y];
+            s.im[2][i * LD + j] = (i == j ? zi : T(0)) - ei[x][y];
+          }
+        }
+    }
+    __syncthreads();
+    const int st = gj_inverse_ls<T, NT, NMAX, L::RG>(s.re[2], s.im[2], LD, n, *s.gj, tid, run);
+    if (run && st != ST_OK) {
+      status = st;
+      run = false;
+    }
+    // T1 = a g, T2 = b g
+    T t1r[TM][TM], t1i[TM][TM], t2r[TM][TM], t2i[TM][TM];
+    tzero<T, TM>(t1r, t1i);
+    tzero<T, TM>(t2r, t2i);
+    if (run && t.own)
+      mac2b<T, TM>(s.re[0], s.im[0], s.re[1], s.im[1], s.re[2], s.im[2], LD, n, t.i0, t.j0, t1r, t1i,
+                   t2r, t2i);
+    __syncthreads();
+    if (run) tstore<T, TM>(s.re[2], s.im[2], LD, n, n, t, t1r, t1i);
+    __syncthreads();
+    // P1 = T1 b (into e_s and e at once), P3 = T1 a (the next alpha)
+    T p1r[TM][TM], p1i[TM][TM], p3r[TM][TM], p3i[TM][TM];
+    tzero<T, TM>(p1r, p1i);
+    tzero<T, TM>(p3r, p3i);
+    if (run && t.own)
+      mac2a<T, TM>(s.re[2], s.im[2], s.re[1], s.im[1], s.re[0], s.im[0], LD, n, t.i0, t.j0, p1r, p1i,
+                   p3r, p3i);
+    if (run) {
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          sr[x][y] = sr[x][y] + p1r[x][y];
+          si[x][y] = si[x][y] + p1i[x][y];
+          er[x][y] = er[x][y] + p1r[x][y];
+          ei[x][y] = ei[x][y] + p1i[x][y];
+        }
+    }
+    __syncthreads();
+    if (run) tstore<T, TM>(s.re[2], s.im[2], LD, n, n, t, t2r, t2i);
+    __syncthreads();
+    // P2 = T2 a (into e), P4 = T2 b (the next beta)
+    T p2r[TM][TM], p2i[TM][TM], p4r[TM][TM], p4i[TM][TM];
+    tzero<T, TM>(p2r, p2i);
+    tzero<T, TM>(p4r, p4i);
+    if (run && t.own)
+      mac2a<T, TM>(s.re[2], s.im[2], s.re[0], s.im[0], s.re[1], s.im[1], LD, n, t.i0, t.j0, p2r, p2i,
+                   p4r, p4i);
+    __syncthreads();
+    bool bad = false;
+    double emax = 0.0;
+    if (run && t.own) {
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          const int i = t.i0 + x, j = t.j0 + y;
+          if (i < n && j < n) {
+            er[x][y] = er[x][y] + p2r[x][y];
+            ei[x][y] = ei[x][y] + p2i[x][y];
+            s.re[0][i * LD + j] = p3r[x][y];
+            s.im[0][i * LD + j] = p3i[x][y];
+            s.re[1][i * LD + j] = p4r[x][y];
+            s.im[1][i * LD + j] = p4i[x][y];
+            bad |= !(finite_d(sr[x][y]) && finite_d(si[x][y]) && finite_d(er[x][y]) &&
+                     finite_d(ei[x][y]) && finite_d(p3r[x][y]) && finite_d(p3i[x][y]) &&
+                     finite_d(p4r[x][y]) && finite_d(p4i[x][y]));
+            const T m3 = p3r[x][y] * p3r[x][y] + p3i[x][y] * p3i[x][y];
+            const T m4 = p4r[x][y] * p4r[x][y] + p4i[x][y] * p4i[x][y];
+            emax = fmax(emax, (double)fmax(m3, m4));
+          }
+        }
+    }
+    double gmax;
+    bool gbad;
+    block_max_any<NT>(emax, bad, s.sred, s.sflag, tid, gmax, gbad);
+    if (run) {
+      if (gbad) {
+        status = ST_NONFINITE;
+        run = false;
+      } else if ((T)sqrt(gmax) < tol) {
+        conv = true;
+        run = false;
+      }
+    }
+  }
+  if (active && status == ST_OK && !conv) status = ST_NOCONV;
+  if (active && tid == 0) {
+    a.status[inst] = status;
+    a.iters[inst] = it;
+  }
+  // g_s = (z - e_s)^-1
+  const bool fin = active && status == ST_OK;
+  if (fin && t.own) {
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+      for (int y = 0; y < TM; ++y) {
+        const int i = t.i0 + x, j = t.j0 + y;
+        if (i < n && j < n) {
+          s.re[2][i * LD + j] = (i == j ? zr : T(0)) - sr[x][y];
+          s.im[2][i * LD + j] = (i == j ? zi : T(0)) - si[x][y];
+        }
+      }
+  }
+  __syncthreads();
+  const int st = gj_inverse_ls<T, NT, NMAX, L::RG>(s.re[2], s.im[2], LD, n, *s.gj, tid, fin);
+  if (fin && st != ST_OK) {
+    status = st;
+    if (tid == 0) a.status[inst] = st;
+  }
+  if (fin && st == ST_OK) {
+    gstore<T, NT>(a.g_out + (size_t)inst * nn * 2, n, s.re[2], s.im[2], LD, tid);
+    if (tid == 0 && a.dos_out) {
+      double tr = 0.0;
+      for (int i = 0; i < n; ++i) tr += -2.0 * s.im[2][i * LD + i];
+      a.dos_out[inst] = (a.omega[inst] / M_PI) * tr;
+    }
+  }
+  return status;
+}
+
+// Sigma = V g V^H embedded in the end layer (skipped unless live).
+template <int NMAX, class L>
+__device__ void pl_sigma(const SigmaArgs& a, int inst, bool live, L& s, int tid) {
+  using T = double;
+  constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
+  constexpr int LD = L::LD;
+  const int nl = a.n_lead, nc = a.n_c, nd = a.n_dev;
+  if (live) s.zero(tid);
+  __syncthreads();
+  int* pos = s.gj->pv;  // inverse contact map: device-layer index -> contact row
+  if (live) {
+    gload<T, NT>(a.v + (size_t)a.set_of[inst] * nc * nl * 2, nc, nl, s.re[0], s.im[0], LD, tid);
+    gload<T, NT>(a.g + (size_t)inst * nl * nl * 2, nl, nl, s.re[1], s.im[1], LD, tid);
+    for (int i = tid; i < nd; i += NT) pos[i] = -1;
+  }
+  __syncthreads();
+  if (live)
+    for (int c = tid; c < nc; c += NT) pos[a.idx[c]] = c;
+  __syncthreads();
+  {
+    const Tile<TM> t(nc, nl, tid);
+    T xr[TM][TM], xi[TM][TM];
+    tzero<T, TM>(xr, xi);
+    if (live && t.own)
+      mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[1], s.im[1], LD, nl, t.i0, t.j0, xr, xi);
+    if (live) tstore<T, TM>(s.re[2], s.im[2], LD, nc, nl, t, xr, xi);
+  }
+  __syncthreads();
+  {
+    const Tile<TM> t(nc, nc, tid);
+    T xr[TM][TM], xi[TM][TM];
+    tzero<T, TM>(xr, xi);
+    if (live && t.own)
+      mac<T, TM, OPN, OPH>(s.re[2], s.im[2], s.re[0], s.im[0], LD, nl, t.i0, t.j0, xr, xi);
+    __syncthreads();
+    if (live) tstore<T, TM>(s.re[1], s.im[1], LD, nc, nc, t, xr, xi);
+  }
+  __syncthreads();
+  if (live) {
+    double2* out = reinterpret_cast<double2*>(a.sigma_out + (size_t)inst * nd * nd * 2);
+    for (int e = tid; e < nd * nd; e += NT) {
+      const int i = e / nd, j = e - (e / nd) * nd;
+      const int pi = pos[i], pj = pos[j];
+      out[e] = (pi >= 0 && pj >= 0) ? make_double2(s.re[1][pi * LD + pj], s.im[1][pi * LD + pj])
+                                    : make_double2(0.0, 0.0);
+    }
+  }
+}
+
+// Recursive sweep for G_1N; returns the device status (st_in when nonzero).
+template <int NMAX, class L>
+__device__ int pl_rgf(const RgfArgs& a, int inst, bool active, int st_in, L& s, int tid) {
+  using T = double;
+  constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
+  constexpr int LD = L::LD;
+  bool live = active && st_in == ST_OK;
+  const int n = a.n, nl = a.n_layers;
+  const size_t nn = (size_t)n * n;
+  const size_t set = active ? (size_t)a.set_of[inst] : 0;
+  const double2* ons = reinterpret_cast<const double2*>(a.onsite + set * nl * nn * 2);
+  const double* cpl = a.coupling + set * (nl > 1 ? nl - 1 : 0) * nn * 2;
+  const size_t io = active ? (size_t)inst * nn * 2 : 0;
+  const double2* sgl = reinterpret_cast<const double2*>(a.sigma_l + io);
+  const double2* sgr = reinterpret_cast<const double2*>(a.sigma_r + io);
+  // LDS: [0] g (inverted in place), then Y = g H; [1] H; [2] Q = G_{1,i} H_i
+  if (live) s.zero(tid);
+  __syncthreads();
+  const Tile<TM> t(n, n, tid);
+  double zr = 0.0, zi = 0.0;
+  if (active) zsq(a.omega[inst], a.eta_dev[inst], zr, zi);
+  T sr[TM][TM], si[TM][TM];
+  tzero<T, TM>(sr, si);
+  int status = ST_OK;
+  for (int layer = 0; layer < nl; ++layer) {
+    if (live && t.own) {
+      const double2* d = ons + (size_t)layer * nn;
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          const int i = t.i0 + x, j = t.j0 + y;
+          if (i < n && j < n) {
+            const double2 dv = d[i * n + j];
+            s.re[0][i * LD + j] = (i == j ? zr : T(0)) - dv.x - sr[x][y];
+            s.im[0][i * LD + j] = (i == j ? zi : T(0)) - dv.y - si[x][y];
+          }
+        }
+    }
+    if (layer == 0 || layer == nl - 1) {
+      __syncthreads();
+      if (live) {
+        for (int e = tid; e < (int)nn; e += NT) {
+          const int i = e / n, j = e - (e / n) * n;
+          T vr = s.re[0][i * LD + j], vi = s.im[0][i * LD + j];
+          if (layer == 0) {
+            vr -= sgl[e].x;
+            vi -= sgl[e].y;
+          }
+          if (layer == nl - 1) {
+            vr -= sgr[e].x;
+            vi -= sgr[e].y;
+          }
+          s.re[0][i * LD + j] = vr;
+          s.im[0][i * LD + j] = vi;
+        }
+      }
+    }
+    __syncthreads();
+    const int st = gj_inverse_ls<T, NT, NMAX, L::RG>(s.re[0], s.im[0], LD, n, *s.gj, tid, live);
+    if (live && st != ST_OK) {
+      status = 10 + st;
+      live = false;
+    }
+    if (layer == nl - 1) {
+      // G_1N = Q_{N-1} g_N (g_1 itself for one layer)
+      T gr[TM][TM], gi[TM][TM];
+      tzero<T, TM>(gr, gi);
+      if (nl == 1) {
+        if (live && t.own) {
+#pragma unroll
+          for (int x = 0; x < TM; ++x)
+#pragma unroll
+            for (int y = 0; y < TM; ++y) {
+              const int i = t.i0 + x, j = t.j0 + y;
+              if (i < n && j < n) {
+                gr[x][y] = s.re[0][i * LD + j];
+                gi[x][y] = s.im[0][i * LD + j];
+              }
+            }
+        }
+      } else if (live && t.own) {
+        mac<T, TM, OPN, OPN>(s.re[2], s.im[2], s.re[0], s.im[0], LD, n, t.i0, t.j0, gr, gi);
+      }
+      if (live) tstore<T, TM>(s.re[1], s.im[1], LD, n, n, t, gr, gi);
+      __syncthreads();
+      if (live) gstore<T, NT>(a.g1n_out + (size_t)inst * nn * 2, n, s.re[1], s.im[1], LD, tid);
+      break;
+    }
+    if (live) gload<T, NT>(cpl + (size_t)layer * nn * 2, n, n, s.re[1], s.im[1], LD, tid);
+    __syncthreads();
+    T yr[TM][TM], yi[TM][TM];
+    tzero<T, TM>(yr, yi);
+    if (live && t.own)
+      mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[1], s.im[1], LD, n, t.i0, t.j0, yr, yi);
+    __syncthreads();  // every read of g is done: Y replaces it (and starts Q at layer 0)
+    if (live) {
+      tstore<T, TM>(s.re[0], s.im[0], LD, n, n, t, yr, yi);
+      if (layer == 0) tstore<T, TM>(s.re[2], s.im[2], LD, n, n, t, yr, yi);
+    }
+    __syncthreads();
+    tzero<T, TM>(sr, si);
+    if (live && t.own)
+      mac<T, TM, OPH, OPN>(s.re[1], s.im[1], s.re[0], s.im[0], LD, n, t.i0, t.j0, sr, si);
+    if (layer > 0) {
+      T qr[TM][TM], qi[TM][TM];
+      tzero<T, TM>(qr, qi);
+      if (live && t.own)
+        mac<T, TM, OPN, OPN>(s.re[2], s.im[2], s.re[0], s.im[0], LD, n, t.i0, t.j0, qr, qi);
+      __syncthreads();
+      if (live) tstore<T, TM>(s.re[2], s.im[2], LD, n, n, t, qr, qi);
+    } else {
+      __syncthreads();
+    }
+  }
+  return st_in != ST_OK ? st_in : status;
+}
+
+// T = Re Tr[Gamma_L G_1N Gamma_R G_1N^H] (NaN for a failed instance).
+template <int NMAX, class L>
+__device__ void pl_trace(const TraceArgs& a, int inst, bool active, int status, L& s, int tid) {
+  using T = double;
+  constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
+  constexpr int LD = L::LD;
+  const bool live = active && status == ST_OK;
+  if (active && !live && tid == 0) a.t_out[inst] = NAN;
+  const int n = a.n;
+  const int nn = n * n;
+  if (live) s.zero(tid);
+  __syncthreads();
+  if (live) {
+    const double2* sl = reinterpret_cast<const double2*>(a.sigma_l + (size_t)inst * nn * 2);
+    const double2* sr = reinterpret_cast<const double2*>(a.sigma_r + (size_t)inst * nn * 2);
+    for (int e = tid; e < nn; e += NT) {
+      const int i = e / n, j = e - (e / n) * n;
+      const double2 a1 = sl[i * n + j], a2 = sl[j * n + i];
+      s.re[0][i * LD + j] = -(a1.y + a2.y);
+      s.im[0][i * LD + j] = a1.x - a2.x;
+      const double2 b1 = sr[i * n + j], b2 = sr[j * n + i];
+      s.re[1][i * LD + j] = -(b1.y + b2.y);
+      s.im[1][i * LD + j] = b1.x - b2.x;
+    }
+    gload<T, NT>(a.g1n + (size_t)inst * nn * 2, n, n, s.re[2], s.im[2], LD, tid);
+  }
+  __syncthreads();
+  const Tile<TM> t(n, n, tid);
+  T pr[TM][TM], pi[TM][TM], qr[TM][TM], qi[TM][TM];
+  tzero<T, TM>(pr, pi);
+  tzero<T, TM>(qr, qi);
+  double acc = 0.0;
+  if (live && t.own) {
+    mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[2], s.im[2], LD, n, t.i0, t.j0, pr, pi);
+    mac<T, TM, OPC, OPT>(s.re[2], s.im[2], s.re[1], s.im[1], LD, n, t.i0, t.j0, qr, qi);
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+      for (int y = 0; y < TM; ++y) {
+        const int i = t.i0 + x, j = t.j0 + y;
+        if (i < n && j < n) acc += pr[x][y] * qr[x][y] - pi[x][y] * qi[x][y];
+      }
+  }
+  const double tot = block_sum<NT>(acc, s.sred, tid);
+  if (live && tid == 0) a.t_out[inst] = tot;
+}
+
+// ---------------------------------------------------------------------------
+// one ticket
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ SrArgs pl_sr_args(const PlProb& P, bool right) {
+  return right ? SrArgs{P.n_lead_r, P.n_inst, P.lead_r, P.set_of, P.omega, P.eta_lead, P.tol,
+                        P.maxiter, P.g_r, P.dos_r, P.iters_r, P.st_r, 0}
+               : SrArgs{P.n_lead_l, P.n_inst, P.lead_l, P.set_of, P.omega, P.eta_lead, P.tol,
+                        P.maxiter, P.g_l, P.dos_l, P.iters_l, P.st_l, 0};
+}
+
+__device__ __forceinline__ SigmaArgs pl_sigma_args(const PlProb& P, bool right,
+                                                   const int32_t* status_in) {
+  return right ? SigmaArgs{P.n_lead_r, P.n_c_r, P.n, P.n_inst, P.v_r, P.set_of, P.g_r, P.idx_r,
+                           status_in, P.sig_r}
+               : SigmaArgs{P.n_lead_l, P.n_c_l, P.n, P.n_inst, P.v_l, P.set_of, P.g_l, P.idx_l,
+                           status_in, P.sig_l};
+}
+
+template <int NMAX>
+__device__ void pl_ticket(const PlProb& P, int first, int phase, unsigned char* smem, PlShared& ps,
+                          int tid) {
+  using S = PlSub<NMAX>;
+  using L = typename S::L;
+  const int sub = tid / S::NT;
+  const int lt = tid - sub * S::NT;
+  const bool real = sub < S::G;
+  L s(smem + (size_t)(real ? sub : 0) * S::STRIDE);
+  if (!real) {  // threads past the last sub-block: barriers and reductions only
+    s.sred = ps.idle_red;
+    s.sflag = ps.idle_flag;
+  }
+  const int inst = first + sub;
+  const bool active = real && inst < P.n_inst;
+  int st_l = ST_OK, st_r = ST_OK;
+  if (phase != PL_DEVICE) {
+    st_l = pl_sr<NMAX>(pl_sr_args(P, false), inst, active, s, lt, ps, sub, S::G);
+    st_r = st_l;
+    if (!P.share) st_r = pl_sr<NMAX>(pl_sr_args(P, true), inst, active, s, lt, ps, sub, S::G);
+    if (phase == PL_LEADS) return;
+  } else if (active) {
+    st_l = P.st_l[inst];
+    st_r = P.share ? st_l : P.st_r[inst];
+  }
+  pl_sigma<NMAX>(pl_sigma_args(P, false, nullptr), inst, active && st_l == ST_OK, s, lt);
+  pl_sigma<NMAX>(pl_sigma_args(P, true, nullptr), inst, active && st_r == ST_OK, s, lt);
+  const int merged = st_l != ST_OK ? st_l : st_r;
+  const RgfArgs ra{P.n, P.n_layers, P.n_inst, P.onsite, P.coupling, P.set_of, P.omega, P.eta_dev,
+                   P.sig_l, P.sig_r, nullptr, P.g1n, P.status, 0};
+  const int st = pl_rgf<NMAX>(ra, inst, active, merged, s, lt);
+  if (active && lt == 0) P.status[inst] = st;
+  const TraceArgs ta{P.n, P.n_inst, P.sig_l, P.sig_r, P.g1n, nullptr, P.t_out};
+  pl_trace<NMAX>(ta, inst, active, st, s, lt);
+}
+
+// Class 96: one instance per ticket, the class's own per-instance stages on a pool
+// slot; statuses travel through the workspace as in the staged path.
+__device__ __forceinline__ void pl_ticket_big(const PlProb& P, int inst, int phase, double* slot, BigShared& sh,
+                              int tid) {
+  if (phase != PL_DEVICE) {
+    big_sr_one(pl_sr_args(P, false), inst, slot, sh, tid);
+    __syncthreads();
+    if (!P.share) {
+      big_sr_one(pl_sr_args(P, true), inst, slot, sh, tid);
+      __syncthreads();
+    }
+    if (phase == PL_LEADS) return;
+  }
+  const int32_t* st_r = P.share ? P.st_l : P.st_r;
+  big_sigma_one(pl_sigma_args(P, false, P.st_l), inst, slot, sh, tid);
+  __syncthreads();
+  big_sigma_one(pl_sigma_args(P, true, st_r), inst, slot, sh, tid);
+  __syncthreads();
+  // left failure wins, as the reference evaluates the left lead first
+  if (tid == 0 && P.st_l[inst] == ST_OK) P.st_l[inst] = st_r[inst];
+  __syncthreads();
+  const RgfArgs ra{P.n, P.n_layers, P.n_inst, P.onsite, P.coupling, P.set_of, P.omega, P.eta_dev,
+                   P.sig_l, P.sig_r, P.st_l, P.g1n, P.status, 0};
+  big_rgf_one(ra, inst, slot, sh, tid);
+  __syncthreads();
+  const TraceArgs ta{P.n, P.n_inst, P.sig_l, P.sig_r, P.g1n, P.status, P.t_out};
+  big_trace_one(ta, inst, slot, sh, tid);
+}
+
+// Copies the problem table into the workspace and clears the ticket counter.
+__global__ void k_pl_setup(PlArgs a, PlArgs* table) {
+  if (threadIdx.x == 0) {
+    *table = a;
+    *a.counter = 0;
+  }
+}
+
+__global__ void k_pl_reset(int* counter) { *counter = 0; }
+
+__global__ void __launch_bounds__(PL_NT) k_persistent(const PlArgs* __restrict__ t, int phase,
+                                                      double* pool) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ PlShared ps;
+  const int tid = threadIdx.x;
+  double* slot = pool ? pool + (size_t)blockIdx.x * BIG_SLOT : nullptr;
+  const int n_prob = t->n_prob, n_tickets = t->n_tickets;
+  int* counter = t->counter;
+  for (;;) {
+    if (tid == 0) ps.ticket = atomicAdd(counter, 1);
+    __syncthreads();
+    const int tk = __builtin_amdgcn_readfirstlane(ps.ticket);
+    __syncthreads();
+    if (tk >= n_tickets) break;
+    // the ticket's problem; its fields are read from the table where they are used
+    int q = 0;
+    while (q + 1 < n_prob && tk >= t->prob[q + 1].ticket0) ++q;
+    const PlProb& P = t->prob[q];
+    const int first = (tk - P.ticket0) * P.group;
+    switch (P.cls) {
+      case 8:
+        pl_ticket<8>(P, first, phase, smem, ps, tid);
+        break;
+      case 16:
+        pl_ticket<16>(P, first, phase, smem, ps, tid);
+        break;
+      case 24:
+        pl_ticket<24>(P, first, phase, smem, ps, tid);
+        break;
+      case 32:
+        pl_ticket<32>(P, first, phase, smem, ps, tid);
+        break;
+      case 48:
+        pl_ticket<48>(P, first, phase, smem, ps, tid);
+        break;
+      default:
+        pl_ticket_big(P, first, phase, slot, *reinterpret_cast<BigShared*>(smem), tid);
+        break;
+    }
+    __syncthreads();
+  }
+}
+
+// dynamic LDS of k_persistent: the largest class in use
+__host__ __device__ constexpr size_t pl_lds_bytes(int cls) {
+  return cls == 8    ? PlSub<8>::BYTES
+         : cls == 16 ? PlSub<16>::BYTES
+         : cls == 24 ? PlSub<24>::BYTES
+         : cls == 32 ? PlSub<32>::BYTES
+         : cls == 48 ? PlSub<48>::BYTES
+                     : sizeof(BigShared);
+}
+
+}  // namespace negf

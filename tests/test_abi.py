@@ -35,3 +35,36 @@ def test_problem_struct_layout():
     # 11 ints, a double (8-aligned at offset 48), 18 pointers, then surface_route (padded)
     assert ctypes.sizeof(_lib.TransmissionProblem) == 48 + 8 + 18 * 8 + 8
     assert _lib.TransmissionProblem.surface_route.offset == 48 + 8 + 18 * 8
+
+
+def test_multi_problem_entry_rejects_bad_arguments():
+    """Argument checks of negf_transmission_multi_batched, which return before any
+    HIP call (so they run without a GPU)."""
+    import ctypes
+
+    lib = _lib.load_symbols_only()
+    arr = (_lib.TransmissionProblem * 9)()
+    for p in arr:
+        p.precision, p.n, p.n_layers, p.n_inst = _lib.FP64, 4, 2, 1
+        p.n_lead_l = p.n_lead_r = p.n_c_l = p.n_c_r = 4
+        p.maxiter = 10
+    assert lib.negf_transmission_multi_batched(None, 0, arr, None, 0) == _lib.OK
+    assert lib.negf_transmission_multi_batched(None, 9, arr, None, 0) == 1
+    assert b"at most 8" in lib.negf_last_error()
+    arr[0].precision = _lib.MIXED
+    assert lib.negf_transmission_multi_batched(None, 2, arr, None, 0) == 1
+    assert b"problem 0" in lib.negf_last_error() and b"FP64 only" in lib.negf_last_error()
+    arr[0].precision = _lib.FP64
+    arr[0].n = 97
+    assert lib.negf_transmission_multi_batched(None, 1, arr, None, 0) == 1
+    assert b"exceeds 96" in lib.negf_last_error()
+    arr[0].n = 4
+    arr[0].n_c_l = 5
+    assert lib.negf_transmission_multi_batched(None, 1, arr, None, 0) == 1
+    assert b"invalid sizes" in lib.negf_last_error()
+    arr[0].n_c_l = 4
+    assert lib.negf_transmission_multi_batched(None, 1, arr, None, 0) == 1  # null pointers
+    assert b"null pointer" in lib.negf_last_error()
+    assert lib.negf_transmission_multi_workspace_bytes(0, arr) == 0
+    assert lib.negf_transmission_multi_workspace_bytes(1, arr) > \
+        lib.negf_transmission_workspace_bytes(ctypes.byref(arr[0]))

@@ -1,0 +1,180 @@
+"""One persistent launch over problems of different block sizes
+(negf_transmission_multi_batched, _engine.transmission_multi).
+
+Every output of every problem (T, status, decimation step counts, surface DOS)
+must equal what the staged launch sequence (negf_transmission_batched) gives for
+that problem alone, bit for bit: the persistent kernel runs the same per-stage
+arithmetic, and each instance of a ticket stops decimating at its own step.  The
+problems cover every block-size class (8, 16, 24, 32, 48 in LDS and 96 in the
+pool), tickets cut short at the end of a problem, frequencies clustered at band
+edges (so the instances of one ticket need different step counts), failures
+(iteration limit, eigenmode fallback), two different leads with explicit
+couplings on contact subsets, more problems than one launch takes, and hipGraph
+replays.  A few points per problem are also checked against the oracle.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from negfpy_amd import _lib
+from negfpy_amd.core import _engine
+from negfpy_amd.workloads import omega_max, random_lead
+from oracle import negf_oracle as o
+from parity_util import assert_parity, oracle_t
+
+pytestmark = pytest.mark.gpu
+
+
+def _device(seed, n, n_layers):
+    d00, d01 = random_lead(seed, n)
+    rng = np.random.default_rng(seed + 7)
+    ons = [d00 + np.diag(rng.uniform(-0.05, 0.05, n)) for _ in range(n_layers)]
+    cps = [d01 * (1.0 + 0.1 * rng.standard_normal()) for _ in range(n_layers - 1)]
+    return d00, d01, ons, cps
+
+
+def _problem(seed, n, n_layers, n_w, *, maxiter=200, route=0, edges=True):
+    """A shared-lead problem with n_w frequencies, half of them packed at the band top."""
+    d00, d01, ons, cps = _device(seed, n, n_layers)
+    lb = np.stack([d00, d01, d01.conj().T])[None]
+    sets = _engine.TransportSets.from_arrays(
+        lb, lb, d01.conj().T[None], d01[None], np.arange(n), np.arange(n), np.stack(ons)[None],
+        np.stack(cps)[None] if n_layers > 1 else None, True)
+    wm = omega_max(_engine_lead(d00, d01))
+    w = np.linspace(0.03, 1.05, n_w) * wm
+    if edges:
+        w[1::2] = wm * (1.0 + np.linspace(-0.01, 0.01, w[1::2].size))
+    solver = _engine.TransmissionSolver(sets, maxiter=maxiter, surface_route=route)
+    return solver, w, (d00, d01, ons, cps)
+
+
+def _engine_lead(d00, d01):
+    from negfpy_amd.core import LeadBlocks
+
+    return LeadBlocks(d00=d00, d01=d01)
+
+
+def _job(solver, w, eta=1e-6):
+    m = w.size
+    return (solver, torch.zeros(m, dtype=torch.int32, device="cuda"),
+            torch.as_tensor(w, device="cuda"), torch.full((m,), eta, dtype=torch.float64,
+                                                          device="cuda"), None)
+
+
+def _host(res):
+    return {k: getattr(res, k).cpu().numpy().copy()
+            for k in ("t", "status", "iters_l", "iters_r", "dos_l", "dos_r")}
+
+
+def _assert_same(multi, staged, tag):
+    for k in staged:
+        np.testing.assert_array_equal(multi[k], staged[k], err_msg=f"{tag}: {k}")
+
+
+# n -> class: 6 -> 8, 12 -> 16, 20 -> 24, 30 -> 32, 40 -> 48, 60 and 96 -> pool
+SIZES = [(6, 4, 30), (12, 3, 23), (20, 5, 9), (30, 2, 7), (40, 3, 5), (60, 2, 3), (96, 2, 2)]
+
+
+def test_mixed_classes_equal_staged(gpu):
+    probs = [_problem(1000 + n, n, nl, nw) for n, nl, nw in SIZES]
+    jobs = [_job(s, w) for s, w, _ in probs]
+    staged = [_host(s.run(*j[1:])) for (s, _, _), j in zip(probs, jobs)]
+    multi = [_host(r) for r in _engine.transmission_multi(jobs)]
+    for (n, nl, nw), a, b in zip(SIZES, multi, staged):
+        print(f"n={n} L={nl}: steps {b['iters_l'].min()}..{b['iters_l'].max()}, "
+              f"T[0]={b['t'][0]:.6e}", flush=True)
+        _assert_same(a, b, f"n={n}")
+    # uneven step counts inside one ticket (the 24 and 16 classes hold 4 and 11)
+    assert np.ptp(staged[2]["iters_l"][:4]) > 0 and np.ptp(staged[1]["iters_l"][:11]) > 0
+    # and the oracle at a few points of each problem
+    for (s, w, (d00, d01, ons, cps)), a in zip(probs, multi):
+        tup = (d00, d01, None)
+        for k in (0, w.size // 2):
+            ref = oracle_t(w[k], ons, cps, tup, tup, eta=1e-6)
+            assert_parity([a["t"][k]], [ref])
+
+
+def test_failures_and_eigen_fallback(gpu):
+    """Iteration limit 4: most decimations fail (status 1, T = NaN) in both paths; with
+    the eigenmode route those instances are recovered the same way in both."""
+    for route in (_lib.SURFACE_DECIMATION, _lib.SURFACE_EIGEN):
+        probs = [_problem(2000 + n, n, 3, 13, maxiter=4, route=route) for n in (8, 24, 40)]
+        jobs = [_job(s, w) for s, w, _ in probs]
+        staged = [_host(s.run(*j[1:])) for (s, _, _), j in zip(probs, jobs)]
+        multi = [_host(r) for r in _engine.transmission_multi(jobs)]
+        for a, b in zip(multi, staged):
+            _assert_same(a, b, f"route={route}")
+        nfail = sum(int(np.sum(b["status"] != 0)) for b in staged)
+        print(f"route={route}: {nfail} failed instances", flush=True)
+        if route == _lib.SURFACE_DECIMATION:
+            assert nfail > 0 and all(np.isnan(b["t"][b["status"] != 0]).all() for b in staged)
+
+
+def test_two_leads_contact_subsets(gpu):
+    """Different left/right leads (two decimations per instance), explicit couplings
+    into contact subsets of the end layers."""
+    jobs, probs = [], []
+    for n, nl, seed in ((16, 3, 31), (24, 4, 32), (72, 2, 33)):
+        d00, d01, ons, cps = _device(seed, n, nl)
+        e00, e01 = random_lead(seed + 50, n)
+        rng = np.random.default_rng(seed)
+        nc = n - 2
+        vl = rng.standard_normal((nc, n)) * 0.2 + 1j * rng.standard_normal((nc, n)) * 0.01
+        vr = rng.standard_normal((nc, n)) * 0.2
+        sets = _engine.TransportSets.from_arrays(
+            np.stack([d00, d01, d01.conj().T])[None], np.stack([e00, e01, e01.conj().T])[None],
+            vl[None], vr[None], np.arange(1, n - 1), np.arange(0, nc), np.stack(ons)[None],
+            np.stack(cps)[None], False)
+        s = _engine.TransmissionSolver(sets)
+        w = np.linspace(0.05, 0.95, 17) * min(omega_max(_engine_lead(d00, d01)),
+                                              omega_max(_engine_lead(e00, e01)))
+        probs.append(s)
+        jobs.append(_job(s, w))
+    staged = [_host(s.run(*j[1:])) for s, j in zip(probs, jobs)]
+    multi = [_host(r) for r in _engine.transmission_multi(jobs)]
+    for a, b in zip(multi, staged):
+        _assert_same(a, b, "two leads")
+        assert np.all(b["status"] == 0)
+
+
+def test_more_problems_than_one_launch(gpu):
+    """11 problems: two persistent launches; also problems with 0 and 1 instances."""
+    probs = [_problem(3000 + i, 4 + 3 * i, 2, 1 + 3 * i, edges=False) for i in range(11)]
+    jobs = [_job(s, w) for s, w, _ in probs]
+    jobs[4] = _job(probs[4][0], np.zeros(0))
+    staged = [_host(s.run(*j[1:])) for (s, _, _), j in zip(probs, jobs)]
+    multi = [_host(r) for r in _engine.transmission_multi(jobs)]
+    for a, b in zip(multi, staged):
+        _assert_same(a, b, "11 problems")
+    assert multi[4]["t"].size == 0 and multi[0]["t"].size == 1
+
+
+def test_graph_replays(gpu):
+    probs = [_problem(4000 + n, n, 3, 21) for n in (6, 24, 48)]
+    jobs = [_job(s, w) for s, w, _ in probs]
+    eager = [_host(r) for r in _engine.transmission_multi(jobs)]
+    g = _engine.MultiTransmissionGraph(jobs)
+    for r in range(4):
+        other = _engine.transmission_multi([_job(s, 0.5 * w) for s, w, _ in probs])
+        torch.cuda.synchronize()
+        _ = [(x.t * 2.0).cpu() for x in other]
+        outs = [_host(x) for x in g.replay()]
+        torch.cuda.synchronize()
+        for a, b in zip(outs, eager):
+            _assert_same(a, b, f"replay {r}")
+    w2 = [0.7 * w for _, w, _ in probs]
+    for gw, w in zip(g.omega, w2):
+        gw.copy_(torch.as_tensor(w, device="cuda"))
+    outs = [_host(x) for x in g.replay()]
+    ref = [_host(r) for r in _engine.transmission_multi([_job(s, w) for (s, _, _), w in
+                                                          zip(probs, w2)])]
+    for a, b in zip(outs, ref):
+        _assert_same(a, b, "new inputs")
+
+
+def test_mixed_precision_rejected(gpu):
+    s, w, _ = _problem(5000, 8, 2, 4)
+    sm = _engine.TransmissionSolver(s.sets, precision="mixed")
+    with pytest.raises(ValueError, match="FP64 only"):
+        _engine.transmission_multi([_job(sm, w)])
