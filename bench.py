@@ -90,6 +90,60 @@ def flops_per_instance(n, n_layers, iters, refine_steps, mixed):
             "trace": 16.0 * n3}
 
 
+def uneven_batch(steps: int, warmup: int) -> dict:
+    """The uneven batch (24-, 48- and 96-DOF ribbons, band-edge clustering) solved
+    by the staged launches problem after problem and by one persistent launch."""
+    import torch
+
+    from negfpy_amd.core import _engine
+    from negfpy_amd.workloads import uneven_problems
+
+    jobs = []
+    for p in uneven_problems():
+        lead, dev = p.lead, p.device
+        lb = np.stack([lead.d00, lead.d01, np.conj(lead.d01).T])[None]
+        n = lead.d00.shape[0]
+        sets = _engine.TransportSets.from_arrays(
+            lb, lb, np.conj(lead.d01).T[None], lead.d01[None], np.arange(n), np.arange(n),
+            np.stack(dev.onsite_blocks)[None], np.stack(dev.coupling_blocks)[None], True)
+        m = p.omegas.size
+        jobs.append((_engine.TransmissionSolver(sets),
+                     torch.zeros(m, dtype=torch.int32, device="cuda"),
+                     torch.tensor(p.omegas, dtype=torch.float64, device="cuda"),
+                     torch.full((m,), p.eta, dtype=torch.float64, device="cuda"), None))
+    n_inst = sum(int(j[2].numel()) for j in jobs)
+    outs_s = [j[0].alloc_result(int(j[2].numel())) for j in jobs]
+    outs_p = [j[0].alloc_result(int(j[2].numel())) for j in jobs]
+
+    def staged():
+        for j, o in zip(jobs, outs_s):
+            j[0].run(*j[1:], out=o)
+
+    def persistent():
+        _engine.transmission_multi(jobs, outs_p)
+
+    times = {}
+    for name, fn in (("staged", staged), ("persistent", persistent)):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        times[name] = (time.perf_counter() - t0) / steps
+    same = all(torch.equal(a.t.nan_to_num(-1.0), b.t.nan_to_num(-1.0)) and
+               torch.equal(a.iters_l, b.iters_l) for a, b in zip(outs_s, outs_p))
+    iters = np.concatenate([o.iters_l.cpu().numpy() for o in outs_p])
+    return {"workload": "uneven batch: ribbons of 24/48/96 DOF per layer, 8 layers, "
+                        "4096/1024/256 omega, a quarter within 1% of the band top, eta=1e-6",
+            "instances": n_inst, "decimation_steps": [int(iters.min()), int(iters.max())],
+            "staged_solves_per_s": round(n_inst / times["staged"], 1),
+            "persistent_solves_per_s": round(n_inst / times["persistent"], 1),
+            "persistent_over_staged": round(times["staged"] / times["persistent"], 3),
+            "identical_outputs": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,6 +302,8 @@ def main():
                             "frac": round(achieved / peak, 5)}}),
         "stage_ms": {"mixed": mix["stage_ms"], "fp64": results["fp64"]["stage_ms"]},
     }
+    line["uneven_batch"] = uneven_batch(max(1, min(args.steps, 5)), 1)
+    print(f"uneven batch: {line['uneven_batch']}", file=sys.stderr, flush=True)
     if cpu is not None:
         sel = cpu["sel"]
         err64 = np.abs(t64[sel] - cpu["vals"]) / np.maximum(np.abs(cpu["vals"]), 1e-3)

@@ -41,6 +41,7 @@ from .thermal_current import (
     heat_current_density_3d,
     heat_current_from_spectrum,
     heat_current_per_length_2d,
+    transmission_spectra,
     transmission_spectrum,
     transverse_area_from_vectors,
     transverse_length_from_vector,
@@ -77,6 +78,7 @@ __all__ = [
     "transmission_kavg_adaptive",
     "transmission_kavg_adaptive_global_eta",
     "transmission_kavg_adaptive_spectrum",
+    "transmission_spectra",
     "transmission_spectrum",
     "heat_current_from_spectrum",
     "heat_current_1d",

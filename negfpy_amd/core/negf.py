@@ -206,21 +206,19 @@ def resolve_k(device, lead_left, lead_right, kpar, *, eta, eta_device, device_to
                     onsite=ons, coupling=cpl, idx_l=idx_l, idx_r=idx_r, scale=scale, dim=dim)
 
 
-def solve_points(kps: list[KProblem], k_of: Array, omega: Array, eta: Array, eta_dev: Array,
-                 *, precision: str, tol: float = 1e-12, maxiter: int = 200, route: int = 0):
-    """Transmission of every instance (k_of[i] indexes ``kps``).  Returns (T, status) arrays.
-
-    Instances are grouped by block layout; each group is one batched launch.
-    """
+def point_jobs(kps: list[KProblem], k_of: Array, omega: Array, eta: Array, eta_dev: Array,
+               *, precision: str, tol: float = 1e-12, maxiter: int = 200, route: int = 0):
+    """Group instances by block layout: one ``(solver, set_of, omega, eta, eta_dev)`` job
+    per layout (for ``TransmissionSolver.run`` or ``_engine.transmission_multi``) and
+    the instance indices each job covers."""
     k_of = np.asarray(k_of, dtype=np.int64)
     omega = np.asarray(omega, dtype=float)
     eta = np.asarray(eta, dtype=float)
     eta_dev = np.asarray(eta_dev, dtype=float)
-    t_all = np.full(omega.shape, np.nan)
-    st_all = np.zeros(omega.shape, dtype=np.int32)
     groups: dict = {}
     for ki, kp in enumerate(kps):
         groups.setdefault(kp.signature(), []).append(ki)
+    jobs, sels = [], []
     for members in groups.values():
         sel = np.nonzero(np.isin(k_of, members))[0]
         if sel.size == 0:
@@ -237,8 +235,34 @@ def solve_points(kps: list[KProblem], k_of: Array, omega: Array, eta: Array, eta
         scale = np.array([kps[k].scale for k in k_of[sel]])
         solver = _engine.TransmissionSolver(sets, precision=precision, tol=tol, maxiter=maxiter,
                                             surface_route=route)
-        res = solver.run(np.array([local[k] for k in k_of[sel]], dtype=np.int32),
-                         omega[sel] / scale, eta[sel] / scale, eta_dev[sel] / scale)
+        jobs.append((solver, np.array([local[k] for k in k_of[sel]], dtype=np.int32),
+                     omega[sel] / scale, eta[sel] / scale, eta_dev[sel] / scale))
+        sels.append(sel)
+    return jobs, sels
+
+
+def run_jobs(jobs, precision: str):
+    """Run transmission jobs: several FP64 jobs (different block layouts) in one
+    persistent launch, otherwise one staged solve per job."""
+    if len(jobs) > 1 and precision == "fp64":
+        return _engine.transmission_multi(jobs)
+    return [job[0].run(*job[1:]) for job in jobs]
+
+
+def solve_points(kps: list[KProblem], k_of: Array, omega: Array, eta: Array, eta_dev: Array,
+                 *, precision: str, tol: float = 1e-12, maxiter: int = 200, route: int = 0):
+    """Transmission of every instance (k_of[i] indexes ``kps``).  Returns (T, status) arrays.
+
+    Instances are grouped by block layout.  One group is one staged batched solve;
+    several groups (k points whose block sizes or contact layouts differ) in FP64 go
+    through one persistent launch together (``_engine.transmission_multi``).
+    """
+    omega = np.asarray(omega, dtype=float)
+    t_all = np.full(omega.shape, np.nan)
+    st_all = np.zeros(omega.shape, dtype=np.int32)
+    jobs, sels = point_jobs(kps, k_of, omega, eta, eta_dev, precision=precision, tol=tol,
+                            maxiter=maxiter, route=route)
+    for sel, res in zip(sels, run_jobs(jobs, precision)):
         t_all[sel] = res.t.cpu().numpy()
         st_all[sel] = res.status.cpu().numpy()
     return t_all, st_all

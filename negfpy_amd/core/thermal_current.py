@@ -15,7 +15,8 @@ import numpy as np
 
 from . import _engine
 from ._status import transmission_value
-from .negf import DeviceLike, LeadLike, _resolve_many, kspace_sets, solve_points
+from .negf import (DeviceLike, LeadLike, _resolve_many, kspace_sets, point_jobs, run_jobs,
+                   solve_points)
 from .surface_gf import _prec
 
 Array = np.ndarray
@@ -73,6 +74,61 @@ def transmission_spectrum(omegas, device, lead_left, lead_right, *, kpoints=None
         t, st = solve_points(kps, kk, om, et, et, precision=prec)
     vals = np.array([transmission_value(int(c), float(v)) for c, v in zip(st, t)])
     return vals.reshape(w.size, nk).mean(axis=1)
+
+
+def transmission_spectra(problems, *, eta=1e-8, precision=None) -> list[Array]:
+    """k-averaged T(omega) of several systems in one call, one array per system.
+
+    ``problems``: sequence of ``(omegas, device, lead_left, lead_right)`` or
+    ``(omegas, device, lead_left, lead_right, kpoints)``, each as in
+    :func:`transmission_spectrum`, with any block sizes.  In FP64 every (omega, k)
+    point of every system goes through one persistent kernel launch (per eight
+    block layouts), whose workgroups take the points of all systems from one work
+    queue; mixed precision solves each layout with its own staged launches.
+    Raises the reference's exception of the first failing point of the first
+    system that has one.  No counterpart in the reference, which solves one point
+    per call (negf.py:306-357).
+    """
+    prec = _prec(precision)
+    jobs, where, shapes = [], [], []
+    for pi, prob in enumerate(problems):
+        if len(prob) not in (4, 5):
+            raise ValueError("each problem is (omegas, device, lead_left, lead_right[, kpoints]).")
+        omegas, device, lead_left, lead_right = prob[:4]
+        kpoints = prob[4] if len(prob) == 5 else None
+        w = np.asarray(list(omegas), dtype=float)
+        ks = [None] if kpoints is None else list(kpoints)
+        if len(ks) == 0:
+            raise ValueError("kpoints must contain at least one k-point.")
+        nk = len(ks)
+        om = np.repeat(w, nk)
+        kk = np.tile(np.arange(nk), w.size)
+        et = np.full(om.size, float(eta))
+        shapes.append((w.size, nk))
+        sets = kspace_sets(device, lead_left, lead_right, ks, precision=prec) \
+            if kpoints is not None else None
+        if sets is not None:
+            jobs.append((_engine.TransmissionSolver(sets, precision=prec), kk.astype(np.int32),
+                         om, et, et))
+            where.append((pi, np.arange(om.size)))
+            continue
+        kps = _resolve_many(device, lead_left, lead_right, ks, eta=eta, eta_device=None,
+                            device_to_lead_left=None, device_to_lead_right=None,
+                            contact_left_indices=None, contact_right_indices=None,
+                            omega_scale=None, precision=prec)
+        pj, sels = point_jobs(kps, kk, om, et, et, precision=prec)
+        jobs.extend(pj)
+        where.extend((pi, sel) for sel in sels)
+    t_all = [np.full(a * b, np.nan) for a, b in shapes]
+    st_all = [np.zeros(a * b, dtype=np.int32) for a, b in shapes]
+    for (pi, sel), res in zip(where, run_jobs(jobs, prec)):
+        t_all[pi][sel] = res.t.cpu().numpy()
+        st_all[pi][sel] = res.status.cpu().numpy()
+    out = []
+    for (nw, nk), t, st in zip(shapes, t_all, st_all):
+        vals = np.array([transmission_value(int(c), float(v)) for c, v in zip(st, t)])
+        out.append(vals.reshape(nw, nk).mean(axis=1) if vals.size else np.zeros(nw))
+    return out
 
 
 def heat_current_1d(omegas: Iterable[float], device: DeviceLike, lead_left: LeadLike,

@@ -111,3 +111,25 @@ def random_lead(seed: int, n: int, scale: float = 1.0):
     d00 = d00 - d01 - d01.conj().T
     d00 = 0.5 * (d00 + d00.conj().T)
     return d00, d01
+
+
+# (atoms per layer -> 24, 48, 96 DOF, omega points) of the uneven batch
+UNEVEN_PARTS = ((8, 4096), (16, 1024), (32, 256))
+
+
+def uneven_problems(seed: int = 0, n_layers: int = 8, edge_fraction: float = 0.25,
+                    eta: float = 1e-6) -> list[RibbonProblem]:
+    """An uneven batch: ribbons of 24-, 48- and 96-DOF layers (4096, 1024 and 256
+    omega points, n_layers device layers each).  A quarter of each spectrum is
+    packed within 1 % of the band top, where the decimation needs the most steps, so
+    step counts vary strongly between neighbouring instances."""
+    out = []
+    for i, (n_atoms, n_omega) in enumerate(UNEVEN_PARTS):
+        p = ribbon_problem(seed + 10 * i, n_atoms=n_atoms, n_layers=n_layers, n_omega=n_omega,
+                           eta=eta)
+        m = int(round(edge_fraction * n_omega))
+        w = p.omegas.copy()
+        w[::max(n_omega // m, 1)][:m] = p.wmax * (1.0 + np.linspace(-0.01, 0.01, m))
+        p.omegas = np.sort(w)
+        out.append(p)
+    return out

@@ -178,3 +178,31 @@ def test_mixed_precision_rejected(gpu):
     sm = _engine.TransmissionSolver(s.sets, precision="mixed")
     with pytest.raises(ValueError, match="FP64 only"):
         _engine.transmission_multi([_job(sm, w)])
+
+
+def test_transmission_spectra_matches_single_spectra(gpu):
+    """Public entry: three systems of different block sizes (a 1-DOF chain, a 24-DOF
+    ribbon, a k-resolved square lattice) in one call equal their own
+    transmission_spectrum calls bit for bit, and the chain its closed form."""
+    from negfpy_amd.core import transmission_spectra, transmission_spectrum
+    from negfpy_amd.models import (ChainParams, SquareLatticeParams, device_perfect_chain,
+                                   lead_blocks, square_lattice_device, square_lattice_lead)
+    from negfpy_amd.workloads import ribbon_problem
+
+    cp = ChainParams(mass=1.0, spring=1.0)
+    chain = (np.linspace(0.05, 2.3, 40), device_perfect_chain(6, cp), lead_blocks(cp),
+             lead_blocks(cp))
+    rp = ribbon_problem(0, n_atoms=8, n_layers=6, n_omega=64)
+    ribbon = (rp.omegas, rp.device, rp.lead, rp.lead)
+    sp = SquareLatticeParams(mass=1.0, spring_x=1.0, spring_y=0.5)
+    ks = [(ky,) for ky in np.linspace(-np.pi, np.pi, 5, endpoint=False)]
+    square = (np.linspace(0.1, 2.5, 25), square_lattice_device(4, sp), square_lattice_lead(sp),
+              square_lattice_lead(sp), ks)
+    got = transmission_spectra([chain, ribbon, square], eta=1e-6)
+    for g, prob in zip(got, (chain, ribbon, square)):
+        kw = {"kpoints": prob[4]} if len(prob) == 5 else {}
+        ref = transmission_spectrum(prob[0], *prob[1:4], eta=1e-6, **kw)
+        np.testing.assert_array_equal(g, ref)
+    inside = chain[0] < 1.99
+    np.testing.assert_allclose(got[0][inside], 1.0, atol=1e-4)
+    assert np.all(np.abs(got[0][chain[0] > 2.01]) < 1e-4)
