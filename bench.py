@@ -144,6 +144,50 @@ def uneven_batch(steps: int, warmup: int) -> dict:
             "identical_outputs": bool(same)}
 
 
+def adjoint_batch(steps: int, warmup: int) -> dict:
+    """The adjoint workload (48-DOF mass-mismatch interface between two different
+    leads, 64 layers): forward transmission solves and adjoint solves (T and
+    dT/d(every device block)) per second, one k point."""
+    import torch
+
+    from negfpy_amd.core import _engine
+    from negfpy_amd.workloads import interface_problem
+
+    p = interface_problem()
+    ll, lr, dev = p.lead_left, p.lead_right, p.device
+    n = ll.d00.shape[0]
+    sets = _engine.TransportSets.from_arrays(
+        np.stack([ll.d00, ll.d01, np.conj(ll.d01).T])[None],
+        np.stack([lr.d00, lr.d01, np.conj(lr.d01).T])[None], np.conj(ll.d01).T[None],
+        lr.d01[None], np.arange(n), np.arange(n), np.stack(dev.onsite_blocks)[None],
+        np.stack(dev.coupling_blocks)[None], False)
+    m = p.omegas.size
+    set_of = torch.zeros(m, dtype=torch.int32, device="cuda")
+    omega = torch.tensor(p.omegas, dtype=torch.float64, device="cuda")
+    eta = torch.full((m,), p.eta, dtype=torch.float64, device="cuda")
+    solver = _engine.TransmissionSolver(sets)
+    times, res = {}, {}
+    for name, fn in (("forward", lambda: solver.run(set_of, omega, eta)),
+                     ("adjoint", lambda: _engine.adjoint(sets, set_of, omega, eta, eta))):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            res[name] = fn()
+        torch.cuda.synchronize()
+        times[name] = (time.perf_counter() - t0) / steps
+    tf, ta = res["forward"].t.cpu().numpy(), res["adjoint"].t.cpu().numpy()
+    ok = (res["forward"].status.cpu().numpy() == 0) & (res["adjoint"].status.cpu().numpy() == 0)
+    return {"workload": f"mass-mismatch interface: n={n} DOF/layer, {dev.n_layers} layers, "
+                        f"28 | 73 amu leads, last 4 layers perturbed, {m} omega, eta=1e-6",
+            "instances": m, "forward_solves_per_s": round(m / times["forward"], 1),
+            "adjoint_solves_per_s": round(m / times["adjoint"], 1),
+            "adjoint_over_forward_time": round(times["adjoint"] / times["forward"], 3),
+            "t_adjoint_vs_forward_max_rel": float(np.max(np.abs(ta[ok] - tf[ok])
+                                                         / np.maximum(np.abs(tf[ok]), 1e-3)))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -304,6 +348,8 @@ def main():
     }
     line["uneven_batch"] = uneven_batch(max(1, min(args.steps, 5)), 1)
     print(f"uneven batch: {line['uneven_batch']}", file=sys.stderr, flush=True)
+    line["adjoint_batch"] = adjoint_batch(max(1, min(args.steps, 3)), 1)
+    print(f"adjoint batch: {line['adjoint_batch']}", file=sys.stderr, flush=True)
     if cpu is not None:
         sel = cpu["sel"]
         err64 = np.abs(t64[sel] - cpu["vals"]) / np.maximum(np.abs(cpu["vals"]), 1e-3)

@@ -133,3 +133,43 @@ def uneven_problems(seed: int = 0, n_layers: int = 8, edge_fraction: float = 0.2
         p.omegas = np.sort(w)
         out.append(p)
     return out
+
+
+@dataclass
+class InterfaceProblem:
+    """Mass-mismatch interface between two different leads."""
+
+    lead_left: LeadBlocks
+    lead_right: LeadBlocks
+    device: Device1D
+    omegas: np.ndarray
+    eta: float
+
+
+def interface_problem(seed: int = 0, n_atoms: int = 16, n_layers: int = 64, n_omega: int = 512,
+                      m_left: float = 28.0, m_right: float = 73.0, perturb: float = 0.2,
+                      n_perturbed: int = 4, eta: float = 1e-6) -> InterfaceProblem:
+    """The adjoint workload: a ribbon cell's force constants (3 n_atoms DOF per layer)
+    with every atom of mass m_left in the left lead and the left half of the device
+    and m_right in the right half and the right lead; the onsite force constants of
+    the last n_perturbed device layers get random symmetric relative perturbations
+    of up to +-perturb.  omega_i = 1.1 w_max (i + 1/2) / n_omega with w_max the band
+    top of the lighter (left) lead."""
+    base = ribbon_params(seed, n_atoms)
+    left = MultiDofCellParams(masses=np.full(n_atoms, m_left), fc00=base.fc00, fc01=base.fc01)
+    right = MultiDofCellParams(masses=np.full(n_atoms, m_right), fc00=base.fc00,
+                               fc01=base.fc01)
+    masses = np.where(np.arange(n_layers)[:, None] < n_layers // 2, m_left, m_right)
+    dev = multidof_device(n_layers, left, layer_masses=masses * np.ones((1, n_atoms)))
+    rng = np.random.default_rng(seed + 3)
+    onsite = [np.array(b) for b in dev.onsite_blocks]
+    nd = 3 * n_atoms
+    for i in range(n_layers - n_perturbed, n_layers):
+        r = rng.uniform(-1.0, 1.0, (nd, nd))
+        onsite[i] = onsite[i] * (1.0 + perturb * 0.5 * (r + r.T))
+    device = Device1D(onsite_blocks=onsite, coupling_blocks=dev.coupling_blocks)
+    lead_l = multidof_lead_blocks(left)
+    wmax = omega_max(lead_l)
+    omegas = 1.1 * wmax * (np.arange(n_omega) + 0.5) / n_omega
+    return InterfaceProblem(lead_left=lead_l, lead_right=multidof_lead_blocks(right),
+                            device=device, omegas=omegas, eta=eta)
