@@ -175,3 +175,16 @@ def test_block_class_table_on_the_host():
     assert lib.negf_block_class(fp64, 20) == 32           # failed calls change nothing
     assert lib.negf_set_block_class(fp64, 20, 0) == 0
     assert lib.negf_block_class(fp64, 20) == 24
+
+
+def test_transmission_spectra_rejects_malformed_problems():
+    """Each problem is (omegas, device, lead_left, lead_right[, kpoints]); the check
+    runs before anything touches the GPU."""
+    import pytest
+
+    from negfpy_amd.core import transmission_spectra
+
+    with pytest.raises(ValueError, match="each problem is"):
+        transmission_spectra([([1.0], None, None)])
+    with pytest.raises(ValueError, match="at least one k-point"):
+        transmission_spectra([([1.0], None, None, None, [])])
