@@ -181,6 +181,27 @@ int negf_transmission_trace_batched(void* stream, int n, int n_inst, const doubl
 /* Copy st_r into st_l wherever st_l is 0 (left-lead failure takes precedence). */
 int negf_merge_status(void* stream, int n_inst, int32_t* st_l, const int32_t* st_r);
 
+/*
+ * An eta ladder per instance (negf_transmission_multi_batched only, decimation
+ * route): each instance tries its rungs in order, eta_lead = the rung's eta, inside
+ * the persistent launch, until one is accepted (status 0, finite T,
+ * T >= -nonneg_tol, T <= t_max) or none is left.  Replaces the per-k eta loop of
+ * transmission_kavg_adaptive, src/negfpy/core/negf.py:446-514, which retries one
+ * point at a time.  The caller keeps every index inside etas.
+ */
+typedef struct negf_eta_ladder {
+  const double* etas;      /* rungs of all instances, concatenated */
+  const int32_t* first;    /* [n_inst] index in etas of the instance's first rung */
+  const int32_t* count;    /* [n_inst] rungs of the instance */
+  const double* t_max;     /* [n_inst] channel limit, or NULL for none */
+  int32_t* rung_out;       /* [n_inst] accepted rung, or count when none was accepted */
+  double* t_rungs;         /* [n_inst][max_rungs] T of every attempted rung */
+  int32_t* status_rungs;   /* [n_inst][max_rungs] status of every attempted rung */
+  double nonneg_tol;       /* acceptance: T >= -nonneg_tol */
+  int max_rungs;           /* row length of t_rungs / status_rungs (>= every count) */
+  int eta_dev_fixed;       /* 1: the problem's eta_dev[] at every rung; 0: eta_dev = rung eta */
+} negf_eta_ladder;
+
 /* One batched transmission problem (see negf_transmission_batched). */
 typedef struct negf_transmission_problem {
   int precision;    /* NEGF_FP64 or NEGF_MIXED */
@@ -215,6 +236,8 @@ typedef struct negf_transmission_problem {
   int32_t* status;        /* [n_inst] */
   int surface_route;      /* NEGF_SURFACE_DECIMATION, or NEGF_SURFACE_EIGEN: lead instances whose
                              decimation fails take the eigenmode route */
+  const negf_eta_ladder* ladder; /* NULL, or an eta ladder per instance (replaces eta_lead;
+                                    negf_transmission_multi_batched only) */
 } negf_transmission_problem;
 
 /*
@@ -261,6 +284,9 @@ int negf_transmission_batched(void* stream, const negf_transmission_problem* p, 
  * point at a time.  Problems with n_inst = 0 are skipped; precision must be
  * NEGF_FP64; refine_steps is unused.  Capturable in a hipGraph when no problem
  * needs blocks above 48 DOF (the class-96 pool orders calls through an event).
+ * A problem with an eta ladder runs its rungs inside the launch (outputs: those of
+ * the last attempted rung, and the ladder's arrays); ladders and the eigenmode route
+ * cannot be mixed in one call.
  *   workspace >= negf_transmission_multi_workspace_bytes(n_problems, problems)
  */
 size_t negf_transmission_multi_workspace_bytes(int n_problems,

@@ -40,6 +40,16 @@ _c_dbl = ctypes.c_double
 _vp = ctypes.c_void_p
 
 
+class EtaLadder(ctypes.Structure):
+    """Mirror of ``negf_eta_ladder``."""
+
+    _fields_ = [
+        ("etas", _vp), ("first", _vp), ("count", _vp), ("t_max", _vp), ("rung_out", _vp),
+        ("t_rungs", _vp), ("status_rungs", _vp), ("nonneg_tol", _c_dbl), ("max_rungs", _c_int),
+        ("eta_dev_fixed", _c_int),
+    ]
+
+
 class TransmissionProblem(ctypes.Structure):
     """Mirror of ``negf_transmission_problem``."""
 
@@ -53,6 +63,7 @@ class TransmissionProblem(ctypes.Structure):
         ("set_of", _vp), ("omega", _vp), ("eta_lead", _vp), ("eta_dev", _vp),
         ("t_out", _vp), ("dos_l", _vp), ("dos_r", _vp),
         ("iters_l", _vp), ("iters_r", _vp), ("status", _vp), ("surface_route", _c_int),
+        ("ladder", ctypes.POINTER(EtaLadder)),
     ]
 
 

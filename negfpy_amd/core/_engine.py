@@ -142,6 +142,13 @@ class TransportSets:
 
 
 @dataclass
+class LadderResult:
+    rung: torch.Tensor     # (n_inst,) int32: accepted rung, or the rung count if none was
+    t: torch.Tensor        # (n_inst, max_rungs) float64: T of every attempted rung
+    status: torch.Tensor   # (n_inst, max_rungs) int32: status of every attempted rung
+
+
+@dataclass
 class TransmissionResult:
     t: torch.Tensor        # (n_inst,) float64
     status: torch.Tensor   # (n_inst,) int32
@@ -149,6 +156,32 @@ class TransmissionResult:
     iters_r: torch.Tensor
     dos_l: torch.Tensor
     dos_r: torch.Tensor
+    ladder: LadderResult | None = None
+
+
+class EtaLadder:
+    """Eta ladders of the instances of one :func:`transmission_multi` job: instance i
+    tries ``ladders[i]`` in order (eta_lead = the rung's eta; eta_device the same, or
+    the job's ``eta_dev`` when ``eta_dev_fixed``) until a rung is accepted: status 0,
+    finite T, T >= -nonneg_tol, T <= t_max[i] (None: no limit)."""
+
+    def __init__(self, ladders, *, t_max=None, nonneg_tol=0.0, eta_dev_fixed=False):
+        count = np.array([len(lad) for lad in ladders], dtype=np.int32)
+        first = np.zeros(count.size, dtype=np.int32)
+        if count.size > 1:
+            first[1:] = np.cumsum(count)[:-1]
+        etas = (np.concatenate([np.asarray(lad, dtype=float) for lad in ladders])
+                if count.sum() else np.zeros(1))
+        self.n_inst = int(count.size)
+        self.max_rungs = max(int(count.max()) if count.size else 1, 1)
+        self.etas = to_dev_f64(etas)
+        self.first = to_dev_i32(first)
+        self.count = to_dev_i32(count)
+        self.t_max = None
+        if t_max is not None:
+            self.t_max = to_dev_f64([np.inf if v is None else float(v) for v in t_max])
+        self.nonneg_tol = float(nonneg_tol)
+        self.eta_dev_fixed = bool(eta_dev_fixed)
 
 
 class TransmissionSolver:
@@ -327,55 +360,87 @@ class TransmissionSolver:
 _multi_ws: dict = {}
 
 
+def _ladder_struct(lad: EtaLadder, out: TransmissionResult):
+    dev = out.t.device
+    out.ladder = LadderResult(
+        rung=torch.zeros((lad.n_inst,), dtype=torch.int32, device=dev),
+        t=torch.full((lad.n_inst, lad.max_rungs), float("nan"), dtype=torch.float64, device=dev),
+        status=torch.full((lad.n_inst, lad.max_rungs), -1, dtype=torch.int32, device=dev))
+    c = _lib.EtaLadder()
+    c.etas = lad.etas.data_ptr()
+    c.first = lad.first.data_ptr()
+    c.count = lad.count.data_ptr()
+    c.t_max = None if lad.t_max is None else lad.t_max.data_ptr()
+    c.rung_out = out.ladder.rung.data_ptr()
+    c.t_rungs = out.ladder.t.data_ptr()
+    c.status_rungs = out.ladder.status.data_ptr()
+    c.nonneg_tol = lad.nonneg_tol
+    c.max_rungs = lad.max_rungs
+    c.eta_dev_fixed = 1 if lad.eta_dev_fixed else 0
+    return c
+
+
 def _multi_prepare(jobs, outs):
-    prepared = []
-    for j, (solver, set_of, omega, eta_lead, eta_dev) in enumerate(jobs):
+    prepared, keep = [], []
+    for j, job in enumerate(jobs):
+        solver, set_of, omega, eta_lead, eta_dev = job[:5]
+        lad = job[5] if len(job) > 5 else None
         if solver.precision != _lib.FP64:
             raise ValueError("the persistent multi-problem launch runs FP64 only")
         set_of = to_dev_i32(set_of)
         omega = to_dev_f64(omega)
+        n_inst = int(omega.numel())
+        if lad is not None and eta_lead is None:
+            eta_lead = torch.zeros((n_inst,), dtype=torch.float64, device=omega.device)
         eta_lead = to_dev_f64(eta_lead)
         eta_dev = eta_lead if eta_dev is None else to_dev_f64(eta_dev)
-        n_inst = int(omega.numel())
         if not (set_of.numel() == eta_lead.numel() == eta_dev.numel() == n_inst):
             raise ValueError("set_of, omega, eta_lead, eta_dev need one entry per instance")
+        if lad is not None and lad.n_inst != n_inst:
+            raise ValueError("the eta ladder needs one ladder per instance")
         out = outs[j] if outs is not None else solver.alloc_result(n_inst)
-        prepared.append((solver, set_of, omega, eta_lead, eta_dev, out))
+        prepared.append((solver, set_of, omega, eta_lead, eta_dev, out, lad))
     chunks = []
     for c0 in range(0, len(prepared), _lib.MULTI_MAX_PROBLEMS):
         chunk = prepared[c0:c0 + _lib.MULTI_MAX_PROBLEMS]
         arr = (_lib.TransmissionProblem * len(chunk))()
-        for i, (solver, so, om, el, ed, out) in enumerate(chunk):
+        for i, (solver, so, om, el, ed, out, lad) in enumerate(chunk):
             arr[i] = solver._problem(int(om.numel()), so, om, el, ed, out)
+            if lad is not None:
+                c = _ladder_struct(lad, out)
+                keep.append(c)
+                arr[i].ladder = ctypes.pointer(c)
         need = int(_lib.load().negf_transmission_multi_workspace_bytes(len(chunk), arr))
         chunks.append((arr, need))
-    return prepared, chunks
+    return prepared, (chunks, keep)
 
 
 def _multi_launch(prepared, chunks, ws):
     lib = _lib.load()
-    for arr, need in chunks:
+    for arr, need in chunks[0]:
         _lib.check(lib.negf_transmission_multi_batched(_lib.stream(), len(arr), arr,
                                                        ws.data_ptr(), need))
-    for solver, _, _, _, _, out in prepared:
+    for solver, _, _, _, _, out, _ in prepared:
         if solver.sets.share_leads:
             out.iters_r.copy_(out.iters_l)
             out.dos_r.copy_(out.dos_l)
-    return [p[-1] for p in prepared]
+    return [p[5] for p in prepared]
 
 
 def transmission_multi(jobs, outs=None) -> list:
     """Several FP64 transmission problems of any block sizes solved together.
 
-    ``jobs``: sequence of ``(solver, set_of, omega, eta_lead, eta_dev)`` with a
-    :class:`TransmissionSolver` per problem (its own sets, block sizes and leads)
+    ``jobs``: sequence of ``(solver, set_of, omega, eta_lead, eta_dev[, ladder])`` with
+    a :class:`TransmissionSolver` per problem (its own sets, block sizes and leads)
     and that problem's instances.  Every instance of every problem goes through one
     persistent kernel launch per ``MULTI_MAX_PROBLEMS`` problems
-    (``negf_transmission_multi_batched``); results as ``solver.run`` on each.
-    ``outs`` (optional) are preallocated results, one per job.
+    (``negf_transmission_multi_batched``); results as ``solver.run`` on each.  With an
+    :class:`EtaLadder` (``eta_lead`` may then be None) each instance retries at its
+    next eta inside the launch; ``result.ladder`` holds the accepted rung and every
+    attempted rung's T and status.  ``outs`` (optional) are preallocated results.
     """
     prepared, chunks = _multi_prepare(jobs, outs)
-    need = max((c[1] for c in chunks), default=0)
+    need = max((c[1] for c in chunks[0]), default=0)
     dev = device()
     ws = _multi_ws.get(dev)
     if ws is None or ws.numel() < need:
@@ -401,7 +466,7 @@ class MultiTransmissionGraph:
                         for j, el in zip(jobs, self.eta_lead)]
         self.out = [s.alloc_result(int(w.numel())) for s, w in zip(self.solvers, self.omega)]
         prepared, chunks = _multi_prepare(self._jobs(), self.out)
-        need = max((c[1] for c in chunks), default=0)
+        need = max((c[1] for c in chunks[0]), default=0)
         self.workspace = torch.empty((max(need, 1),), dtype=torch.uint8, device=device())
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())

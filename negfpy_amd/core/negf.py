@@ -556,6 +556,57 @@ class _KSolver:
         return solve_points([self.kps[k] for k in uniq], local, omega, eta, eta_dev,
                             precision=self.precision, route=self.route)
 
+    def ladder_ok(self, ladders, eta_device) -> bool:
+        """Whether the eta ladders can run inside the persistent launch: FP64, the
+        decimation route, and no rung with a negative device eta (such a rung is an
+        "exception" of its own, decided on the host)."""
+        if self.precision != "fp64" or self.route != 0:
+            return False
+        if eta_device is None:
+            return all(float(e) >= 0.0 for lad in ladders for e in lad)
+        return float(eta_device) >= 0.0
+
+    def solve_ladder(self, k_idx, omega, ladders, eta_device, nonneg_tol, t_max):
+        """Instance i (k point k_idx[i], omega[i]) tries the etas ladders[i] in order
+        inside one persistent launch until one is accepted (status 0, finite T,
+        T >= -nonneg_tol, T <= t_max[i]).  Returns (accepted rung or the rung count,
+        T of every attempted rung, status of every attempted rung (-1: not tried))."""
+        k_idx = np.asarray(k_idx, dtype=np.int64)
+        omega = np.asarray(omega, dtype=float)
+        n = k_idx.size
+        max_rungs = max((len(lad) for lad in ladders), default=1)
+        rung = np.zeros(n, dtype=np.int64)
+        t_r = np.full((n, max_rungs), np.nan)
+        st_r = np.full((n, max_rungs), -1, dtype=np.int32)
+        if n == 0:
+            return rung, t_r, st_r
+        fixed = eta_device is not None
+        ed = np.full(n, float(eta_device) if fixed else 0.0)
+        if self.sets is not None:
+            lad = _engine.EtaLadder(ladders, t_max=t_max, nonneg_tol=nonneg_tol,
+                                    eta_dev_fixed=fixed)
+            jobs = [(_engine.TransmissionSolver(self.sets, precision=self.precision),
+                     k_idx.astype(np.int32), omega, None, ed, lad)]
+            sels = [np.arange(n)]
+        else:
+            uniq, local = np.unique(k_idx, return_inverse=True)
+            kps = [self.kps[k] for k in uniq]
+            jobs, sels = point_jobs(kps, local, omega, np.zeros(n), ed, precision=self.precision,
+                                    route=self.route)
+            # omega_scale divides every eta of the point as it divides omega
+            scale = np.array([kps[q].scale for q in local])
+            jobs = [job + (_engine.EtaLadder([np.asarray(ladders[j], dtype=float) / scale[j]
+                                              for j in sel],
+                                             t_max=[t_max[j] for j in sel],
+                                             nonneg_tol=nonneg_tol, eta_dev_fixed=fixed),)
+                    for job, sel in zip(jobs, sels)]
+        for sel, res in zip(sels, _engine.transmission_multi(jobs)):
+            rung[sel] = res.ladder.rung.cpu().numpy()
+            lt = res.ladder.t.cpu().numpy()
+            t_r[sel, :lt.shape[1]] = lt
+            st_r[sel, :lt.shape[1]] = res.ladder.status.cpu().numpy()
+        return rung, t_r, st_r
+
 
 def _t_max(lead_left, kpoints, max_channel_factor):
     if max_channel_factor is None:
@@ -584,6 +635,36 @@ def _adaptive_batch(omegas, solver, t_max, ladders, *, eta_device, global_eta,
     g_rejected = [[] for _ in range(nw)]
     g_result = [None] * nw
     max_rungs = max((len(lad) for lad in ladders), default=0)
+    if not global_eta and solver.ladder_ok(ladders, eta_device):
+        # per-k ladders: every (omega, k) walks its ladder inside one persistent launch;
+        # the attempts it reports are classified here exactly as the rung loop below would
+        inst_w = np.repeat(np.arange(nw), nk)
+        inst_k = np.tile(kk, nw)
+        live = np.array([solver.live(int(k)) for k in inst_k], dtype=bool)
+        sel = np.nonzero(live)[0]
+        rung, t_r, st_r = solver.solve_ladder(
+            inst_k[sel], np.asarray(omegas, float)[inst_w[sel]],
+            [ladders[int(inst_w[j])] for j in sel], eta_device, nonnegative_tolerance,
+            [t_max[int(inst_k[j])] for j in sel])
+        pos = {int(j): q for q, j in enumerate(sel)}
+        for j in range(inst_w.size):
+            i, k = int(inst_w[j]), int(inst_k[j])
+            lad = ladders[i]
+            q = pos.get(j)
+            tried = len(lad) if q is None else min(int(rung[q]) + 1, len(lad))
+            for r in range(tried):
+                reason, val = (("exception", None) if q is None else
+                               _classify(st_r[q, r], t_r[q, r], nonnegative_tolerance, t_max[k]))
+                if reason == "ok":
+                    outcome[i][k] = val
+                    used_eta[i][k] = float(lad[r])
+                elif collect_rejected:
+                    rejected[i][k].append({"omega": float(omegas[i]),
+                                           "kpar": _kpar_tuple(solver.kpoints[k]),
+                                           "eta": float(lad[r]),
+                                           "t": None if val is None else float(val),
+                                           "reason": reason})
+        max_rungs = 0   # nothing left for the rung loop
     for r in range(max_rungs):
         inst_w, inst_k, inst_eta = [], [], []
         for i in range(nw):

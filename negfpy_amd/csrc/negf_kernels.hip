@@ -2177,6 +2177,8 @@ int negf_transmission_batched(void* stream, const negf_transmission_problem* p, 
 
   if (p->surface_route != NEGF_SURFACE_DECIMATION && p->surface_route != NEGF_SURFACE_EIGEN)
     return fail(NEGF_ERR_ARG, "unknown surface route %d", p->surface_route);
+  if (p->ladder)
+    return fail(NEGF_ERR_ARG, "eta ladders are solved by negf_transmission_multi_batched");
   const bool eig = p->surface_route == NEGF_SURFACE_EIGEN;
   int rc = negf_sancho_rubio_batched(stream, p->precision, p->n_lead_l, p->n_inst, p->lead_l,
                                      p->set_of, p->omega, p->eta_lead, p->tol, p->maxiter,
@@ -2225,12 +2227,17 @@ int negf_transmission_batched(void* stream, const negf_transmission_problem* p, 
 
 static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
+// workspace of one problem: the staged path's, plus the current rung's etas of a ladder
+static size_t pl_problem_bytes(const negf_transmission_problem& p) {
+  const size_t ladder = p.ladder ? align256(2 * (size_t)p.n_inst * sizeof(double)) : 0;
+  return align256(negf_transmission_workspace_bytes(&p)) + ladder;
+}
+
 size_t negf_transmission_multi_workspace_bytes(int n_problems,
                                                const negf_transmission_problem* problems) {
   if (n_problems <= 0 || !problems) return 0;
   size_t b = align256(sizeof(PlArgs)) + 256;  // problem table, ticket counter
-  for (int i = 0; i < n_problems; ++i)
-    b += align256(negf_transmission_workspace_bytes(&problems[i]));
+  for (int i = 0; i < n_problems; ++i) b += pl_problem_bytes(problems[i]);
   return b;
 }
 
@@ -2250,10 +2257,21 @@ static int pl_check(const negf_transmission_problem& p, int i) {
   int m = p.n > p.n_lead_l ? p.n : p.n_lead_l;
   m = m > p.n_lead_r ? m : p.n_lead_r;
   if (m > BIG) return fail(NEGF_ERR_ARG, "problem %d: block size %d exceeds %d", i, m, BIG);
+  if (p.ladder) {
+    const negf_eta_ladder& l = *p.ladder;
+    if (p.surface_route != NEGF_SURFACE_DECIMATION)
+      return fail(NEGF_ERR_ARG, "problem %d: eta ladders run with the decimation route only", i);
+    if (l.max_rungs < 1 || !(l.nonneg_tol >= 0.0))
+      return fail(NEGF_ERR_ARG, "problem %d: invalid eta ladder (max_rungs=%d)", i, l.max_rungs);
+    if (p.n_inst > 0 && (!l.etas || !l.first || !l.count || !l.rung_out || !l.t_rungs ||
+                         !l.status_rungs))
+      return fail(NEGF_ERR_ARG, "problem %d: null pointer in the eta ladder", i);
+  }
   if (p.n_inst == 0) return NEGF_OK;
   if (!p.lead_l || (!p.share_leads && !p.lead_r) || !p.v_l || !p.v_r || !p.idx_l || !p.idx_r ||
-      !p.onsite || (p.n_layers > 1 && !p.coupling) || !p.set_of || !p.omega || !p.eta_lead ||
-      !p.eta_dev || !p.t_out || !p.iters_l || (!p.share_leads && !p.iters_r) || !p.status)
+      !p.onsite || (p.n_layers > 1 && !p.coupling) || !p.set_of || !p.omega ||
+      (!p.ladder && !p.eta_lead) || ((!p.ladder || p.ladder->eta_dev_fixed) && !p.eta_dev) ||
+      !p.t_out || !p.iters_l || (!p.share_leads && !p.iters_r) || !p.status)
     return fail(NEGF_ERR_ARG, "problem %d: null pointer argument", i);
   return NEGF_OK;
 }
@@ -2316,12 +2334,13 @@ int negf_transmission_multi_batched(void* stream, int n_problems,
   a.n_prob = np;
   int tickets = 0;
   size_t lds = 0;
-  bool big = false, eig = false;
+  bool big = false, eig = false, ladder = false;
   for (int i = 0; i < n_problems; ++i) {
     // workspace slices in the caller's order, laid out as in negf_transmission_batched
     const negf_transmission_problem& p = problems[i];
     unsigned char* base = w;
-    w += align256(negf_transmission_workspace_bytes(&p));
+    unsigned char* lad_ws = w + align256(negf_transmission_workspace_bytes(&p));
+    w += pl_problem_bytes(p);
     if (p.n_inst == 0) continue;
     int slot = 0;
     while (order[slot] != i) ++slot;
@@ -2373,6 +2392,22 @@ int negf_transmission_multi_batched(void* stream, int n_problems,
     q.iters_l = p.iters_l;
     q.iters_r = p.iters_r;
     q.status = p.status;
+    if (p.ladder) {
+      const negf_eta_ladder& l = *p.ladder;
+      q.lad_eta = l.etas;
+      q.lad_first = l.first;
+      q.lad_count = l.count;
+      q.lad_tmax = l.t_max;
+      q.lad_rung = l.rung_out;
+      q.lad_t = l.t_rungs;
+      q.lad_st = l.status_rungs;
+      q.eta_cur = reinterpret_cast<double*>(lad_ws);
+      q.eta_dev_cur = q.eta_cur + ni;
+      q.lad_tol = l.nonneg_tol;
+      q.lad_max = l.max_rungs;
+      q.lad_dev_fixed = l.eta_dev_fixed ? 1 : 0;
+      ladder = true;
+    }
     big = big || cls[i] == BIG;
     eig = eig || p.surface_route == NEGF_SURFACE_EIGEN;
     const size_t l = pl_lds_bytes(cls[i]);
@@ -2390,6 +2425,8 @@ int negf_transmission_multi_batched(void* stream, int n_problems,
     tickets += (q.n_inst + q.group - 1) / q.group;
   }
   a.n_tickets = tickets;
+  if (ladder && eig)
+    return fail(NEGF_ERR_ARG, "eta ladders and the eigenmode route cannot share a call");
 
   int dev = 0, cus = 0, per = 0;
   int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");

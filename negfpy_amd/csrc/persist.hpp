@@ -80,6 +80,16 @@ struct PlProb {
   int32_t *iters_l, *iters_r, *status;
   double *g_l, *g_r, *sig_l, *sig_r, *g1n;
   int32_t *st_l, *st_r;
+  // eta ladder (lad_eta null: one eta per instance from eta_lead / eta_dev)
+  const double* lad_eta;
+  const int32_t *lad_first, *lad_count;
+  const double* lad_tmax;
+  int32_t* lad_rung;
+  double* lad_t;
+  int32_t* lad_st;
+  double *eta_cur, *eta_dev_cur;  // the current rung's etas (workspace)
+  double lad_tol;
+  int lad_max, lad_dev_fixed;
 };
 
 struct PlArgs {
@@ -90,6 +100,7 @@ struct PlArgs {
 
 struct PlShared {
   int run[16];          // sub-block still iterating (decimation loop)
+  int more[16];         // sub-block retrying at its next eta rung
   int ticket;
   double idle_red[PL_NT / WAVE];  // reduction scratch of threads outside every sub-block
   int idle_flag[PL_NT / WAVE];
@@ -457,9 +468,10 @@ __device__ int pl_rgf(const RgfArgs& a, int inst, bool active, int st_in, L& s, 
   return st_in != ST_OK ? st_in : status;
 }
 
-// T = Re Tr[Gamma_L G_1N Gamma_R G_1N^H] (NaN for a failed instance).
+// T = Re Tr[Gamma_L G_1N Gamma_R G_1N^H] (NaN for a failed instance); returns it to
+// every thread of the sub-block.
 template <int NMAX, class L>
-__device__ void pl_trace(const TraceArgs& a, int inst, bool active, int status, L& s, int tid) {
+__device__ double pl_trace(const TraceArgs& a, int inst, bool active, int status, L& s, int tid) {
   using T = double;
   constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
   constexpr int LD = L::LD;
@@ -502,16 +514,17 @@ __device__ void pl_trace(const TraceArgs& a, int inst, bool active, int status, 
   }
   const double tot = block_sum<NT>(acc, s.sred, tid);
   if (live && tid == 0) a.t_out[inst] = tot;
+  return live ? tot : NAN;
 }
 
 // ---------------------------------------------------------------------------
 // one ticket
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ SrArgs pl_sr_args(const PlProb& P, bool right) {
-  return right ? SrArgs{P.n_lead_r, P.n_inst, P.lead_r, P.set_of, P.omega, P.eta_lead, P.tol,
+__device__ __forceinline__ SrArgs pl_sr_args(const PlProb& P, bool right, const double* eta) {
+  return right ? SrArgs{P.n_lead_r, P.n_inst, P.lead_r, P.set_of, P.omega, eta, P.tol,
                         P.maxiter, P.g_r, P.dos_r, P.iters_r, P.st_r, 0}
-               : SrArgs{P.n_lead_l, P.n_inst, P.lead_l, P.set_of, P.omega, P.eta_lead, P.tol,
+               : SrArgs{P.n_lead_l, P.n_inst, P.lead_l, P.set_of, P.omega, eta, P.tol,
                         P.maxiter, P.g_l, P.dos_l, P.iters_l, P.st_l, 0};
 }
 
@@ -521,6 +534,55 @@ __device__ __forceinline__ SigmaArgs pl_sigma_args(const PlProb& P, bool right,
                            status_in, P.sig_r}
                : SigmaArgs{P.n_lead_l, P.n_c_l, P.n, P.n_inst, P.v_l, P.set_of, P.g_l, P.idx_l,
                            status_in, P.sig_l};
+}
+
+// Acceptance of a rung as the reference's adaptive loop sees it (negf.py:446-514):
+// status 0, finite T, T >= -nonneg_tol, T <= the channel limit.
+__device__ __forceinline__ bool pl_accept(const PlProb& P, int inst, int st, double t) {
+  const double tmax = P.lad_tmax ? P.lad_tmax[inst] : INFINITY;
+  return st == ST_OK && isfinite(t) && t >= -P.lad_tol && t <= tmax;
+}
+
+// The eta of an instance's rung into the per-instance arrays the stages read.
+__device__ __forceinline__ void pl_set_rung(const PlProb& P, int inst, int rung) {
+  const double e = P.lad_eta[P.lad_first[inst] + rung];
+  P.eta_cur[inst] = e;
+  P.eta_dev_cur[inst] = P.lad_dev_fixed ? P.eta_dev[inst] : e;
+}
+
+__device__ __forceinline__ void pl_record_rung(const PlProb& P, int inst, int rung, int st,
+                                               double t) {
+  P.lad_t[(size_t)inst * P.lad_max + rung] = t;
+  P.lad_st[(size_t)inst * P.lad_max + rung] = st;
+}
+
+// The pipeline of one ticket's instances at the given etas; returns T and the status.
+template <int NMAX, class L>
+__device__ void pl_pipeline(const PlProb& P, const double* eta, const double* eta_dev, int inst,
+                            bool active, int phase, L& s, int lt, PlShared& ps, int sub,
+                            double& t, int& st) {
+  constexpr int G = PlSub<NMAX>::G;
+  int st_l = ST_OK, st_r = ST_OK;
+  t = NAN;
+  st = ST_OK;
+  if (phase != PL_DEVICE) {
+    st_l = pl_sr<NMAX>(pl_sr_args(P, false, eta), inst, active, s, lt, ps, sub, G);
+    st_r = st_l;
+    if (!P.share) st_r = pl_sr<NMAX>(pl_sr_args(P, true, eta), inst, active, s, lt, ps, sub, G);
+    if (phase == PL_LEADS) return;
+  } else if (active) {
+    st_l = P.st_l[inst];
+    st_r = P.share ? st_l : P.st_r[inst];
+  }
+  pl_sigma<NMAX>(pl_sigma_args(P, false, nullptr), inst, active && st_l == ST_OK, s, lt);
+  pl_sigma<NMAX>(pl_sigma_args(P, true, nullptr), inst, active && st_r == ST_OK, s, lt);
+  const int merged = st_l != ST_OK ? st_l : st_r;
+  const RgfArgs ra{P.n, P.n_layers, P.n_inst, P.onsite, P.coupling, P.set_of, P.omega, eta_dev,
+                   P.sig_l, P.sig_r, nullptr, P.g1n, P.status, 0};
+  st = pl_rgf<NMAX>(ra, inst, active, merged, s, lt);
+  if (active && lt == 0) P.status[inst] = st;
+  const TraceArgs ta{P.n, P.n_inst, P.sig_l, P.sig_r, P.g1n, nullptr, P.t_out};
+  t = pl_trace<NMAX>(ta, inst, active, st, s, lt);
 }
 
 template <int NMAX>
@@ -538,36 +600,53 @@ __device__ void pl_ticket(const PlProb& P, int first, int phase, unsigned char* 
   }
   const int inst = first + sub;
   const bool active = real && inst < P.n_inst;
-  int st_l = ST_OK, st_r = ST_OK;
-  if (phase != PL_DEVICE) {
-    st_l = pl_sr<NMAX>(pl_sr_args(P, false), inst, active, s, lt, ps, sub, S::G);
-    st_r = st_l;
-    if (!P.share) st_r = pl_sr<NMAX>(pl_sr_args(P, true), inst, active, s, lt, ps, sub, S::G);
-    if (phase == PL_LEADS) return;
-  } else if (active) {
-    st_l = P.st_l[inst];
-    st_r = P.share ? st_l : P.st_r[inst];
+  // One pass of the pipeline per rung.  Without a ladder there is one rung at the
+  // problem's etas; with one, every instance retries at its next rung until one is
+  // accepted, and the ticket runs until none of its sub-blocks retries.
+  const bool lad = P.lad_eta != nullptr;
+  const double* eta = lad ? P.eta_cur : P.eta_lead;
+  const double* eta_dev = lad ? P.eta_dev_cur : P.eta_dev;
+  const int count = !active ? 0 : lad ? min(P.lad_count[inst], P.lad_max) : 1;
+  bool go = count > 0;
+  if (active && !go && lt == 0) P.lad_rung[inst] = 0;
+  int rung = 0;
+  for (;;) {
+    if (lad && go && lt == 0) pl_set_rung(P, inst, rung);
+    if (lt == 0 && real) ps.more[sub] = go ? 1 : 0;
+    __syncthreads();
+    int any = 0;
+    for (int q = 0; q < S::G; ++q) any |= ps.more[q];
+    any = __builtin_amdgcn_readfirstlane(any);  // the same in every lane
+    if (!any) break;
+    double t;
+    int st;
+    pl_pipeline<NMAX>(P, eta, eta_dev, inst, go, phase, s, lt, ps, sub, t, st);
+    if (go) {
+      bool ok = true;
+      if (lad) {
+        if (lt == 0) pl_record_rung(P, inst, rung, st, t);
+        ok = pl_accept(P, inst, st, t);
+        if ((ok || rung + 1 >= count) && lt == 0) P.lad_rung[inst] = ok ? rung : count;
+      }
+      if (ok || rung + 1 >= count) {
+        go = false;
+      } else {
+        ++rung;
+      }
+    }
   }
-  pl_sigma<NMAX>(pl_sigma_args(P, false, nullptr), inst, active && st_l == ST_OK, s, lt);
-  pl_sigma<NMAX>(pl_sigma_args(P, true, nullptr), inst, active && st_r == ST_OK, s, lt);
-  const int merged = st_l != ST_OK ? st_l : st_r;
-  const RgfArgs ra{P.n, P.n_layers, P.n_inst, P.onsite, P.coupling, P.set_of, P.omega, P.eta_dev,
-                   P.sig_l, P.sig_r, nullptr, P.g1n, P.status, 0};
-  const int st = pl_rgf<NMAX>(ra, inst, active, merged, s, lt);
-  if (active && lt == 0) P.status[inst] = st;
-  const TraceArgs ta{P.n, P.n_inst, P.sig_l, P.sig_r, P.g1n, nullptr, P.t_out};
-  pl_trace<NMAX>(ta, inst, active, st, s, lt);
 }
 
 // Class 96: one instance per ticket, the class's own per-instance stages on a pool
 // slot; statuses travel through the workspace as in the staged path.
-__device__ __forceinline__ void pl_ticket_big(const PlProb& P, int inst, int phase, double* slot, BigShared& sh,
-                              int tid) {
+__device__ __forceinline__ void pl_pipeline_big(const PlProb& P, const double* eta,
+                                                const double* eta_dev, int inst, int phase,
+                                                double* slot, BigShared& sh, int tid) {
   if (phase != PL_DEVICE) {
-    big_sr_one(pl_sr_args(P, false), inst, slot, sh, tid);
+    big_sr_one(pl_sr_args(P, false, eta), inst, slot, sh, tid);
     __syncthreads();
     if (!P.share) {
-      big_sr_one(pl_sr_args(P, true), inst, slot, sh, tid);
+      big_sr_one(pl_sr_args(P, true, eta), inst, slot, sh, tid);
       __syncthreads();
     }
     if (phase == PL_LEADS) return;
@@ -580,12 +659,39 @@ __device__ __forceinline__ void pl_ticket_big(const PlProb& P, int inst, int pha
   // left failure wins, as the reference evaluates the left lead first
   if (tid == 0 && P.st_l[inst] == ST_OK) P.st_l[inst] = st_r[inst];
   __syncthreads();
-  const RgfArgs ra{P.n, P.n_layers, P.n_inst, P.onsite, P.coupling, P.set_of, P.omega, P.eta_dev,
+  const RgfArgs ra{P.n, P.n_layers, P.n_inst, P.onsite, P.coupling, P.set_of, P.omega, eta_dev,
                    P.sig_l, P.sig_r, P.st_l, P.g1n, P.status, 0};
   big_rgf_one(ra, inst, slot, sh, tid);
   __syncthreads();
   const TraceArgs ta{P.n, P.n_inst, P.sig_l, P.sig_r, P.g1n, P.status, P.t_out};
   big_trace_one(ta, inst, slot, sh, tid);
+}
+
+__device__ __forceinline__ void pl_ticket_big(const PlProb& P, int inst, int phase, double* slot,
+                                              BigShared& sh, int tid) {
+  // one pass per rung, as in pl_ticket (one instance per ticket here)
+  const bool lad = P.lad_eta != nullptr;
+  const double* eta = lad ? P.eta_cur : P.eta_lead;
+  const double* eta_dev = lad ? P.eta_dev_cur : P.eta_dev;
+  const int count =
+      lad ? __builtin_amdgcn_readfirstlane(min(P.lad_count[inst], P.lad_max)) : 1;
+  if (count <= 0 && tid == 0) P.lad_rung[inst] = 0;
+  for (int rung = 0; rung < count; ++rung) {
+    if (lad && tid == 0) pl_set_rung(P, inst, rung);
+    __syncthreads();
+    pl_pipeline_big(P, eta, eta_dev, inst, phase, slot, sh, tid);
+    if (!lad) break;
+    __syncthreads();
+    const double t = P.t_out[inst];
+    const int st = P.status[inst];
+    __syncthreads();  // every thread has read them before the next rung writes them
+    if (tid == 0) pl_record_rung(P, inst, rung, st, t);
+    const bool ok = __builtin_amdgcn_readfirstlane(pl_accept(P, inst, st, t) ? 1 : 0) != 0;
+    if (ok || rung + 1 >= count) {
+      if (tid == 0) P.lad_rung[inst] = ok ? rung : count;
+      break;
+    }
+  }
 }
 
 // Copies the problem table into the workspace and clears the ticket counter.

@@ -32,9 +32,14 @@ def test_library_exports_every_declared_symbol():
 def test_problem_struct_layout():
     import ctypes
 
-    # 11 ints, a double (8-aligned at offset 48), 18 pointers, then surface_route (padded)
-    assert ctypes.sizeof(_lib.TransmissionProblem) == 48 + 8 + 18 * 8 + 8
+    # 11 ints, a double (8-aligned at offset 48), 18 pointers, surface_route (padded),
+    # the ladder pointer
+    assert ctypes.sizeof(_lib.TransmissionProblem) == 48 + 8 + 18 * 8 + 8 + 8
     assert _lib.TransmissionProblem.surface_route.offset == 48 + 8 + 18 * 8
+    assert _lib.TransmissionProblem.ladder.offset == 48 + 8 + 18 * 8 + 8
+    # 7 pointers, a double, two ints
+    assert ctypes.sizeof(_lib.EtaLadder) == 7 * 8 + 8 + 8
+    assert _lib.EtaLadder.max_rungs.offset == 64
 
 
 def test_multi_problem_entry_rejects_bad_arguments():

@@ -206,3 +206,56 @@ def test_transmission_spectra_matches_single_spectra(gpu):
     inside = chain[0] < 1.99
     np.testing.assert_allclose(got[0][inside], 1.0, atol=1e-4)
     assert np.all(np.abs(got[0][chain[0] > 2.01]) < 1e-4)
+
+
+def _spectrum_both_ways(monkeypatch, *args, **kw):
+    """The adaptive spectrum with the eta ladders inside the persistent launch, and
+    with the host's rung-by-rung loop (ladder_ok switched off)."""
+    from negfpy_amd.core import negf as nm
+
+    in_kernel = nm.transmission_kavg_adaptive_spectrum(*args, **kw)
+    with monkeypatch.context() as m:
+        m.setattr(nm._KSolver, "ladder_ok", lambda self, ladders, eta_device: False)
+        host = nm.transmission_kavg_adaptive_spectrum(*args, **kw)
+    return in_kernel, host
+
+
+def test_eta_ladder_in_kernel_equals_rung_loop(gpu, monkeypatch):
+    """Per-k adaptive eta: the ladder walked inside the persistent launch gives the
+    rung-by-rung driver's values, statistics and rejection records exactly.  The
+    ladder starts at eta = 0, where decimation inside the band does not converge
+    (exception, retried), a channel limit below one channel rejects the in-band
+    points of the 1-DOF lattice (over_channel_limit at every rung), and k points are
+    resolved on the host (callbacks), with and without omega_scale and a fixed
+    device eta."""
+    from negfpy_amd.models import SquareLatticeParams, square_lattice_device, square_lattice_lead
+    from negfpy_amd.workloads import ribbon_problem
+
+    sp = SquareLatticeParams(mass=1.0, spring_x=1.0, spring_y=0.5)
+    ks = [(ky,) for ky in np.linspace(-np.pi, np.pi, 6, endpoint=False)]
+    lead, dev = square_lattice_lead(sp), square_lattice_device(5, sp)
+    w = np.linspace(0.1, 2.6, 23)
+    cases = [
+        dict(eta_values=(0.0, 1e-9, 1e-6), collect_rejected=True),
+        dict(eta_values=(0.0, 1e-7), collect_rejected=True, max_channel_factor=0.5,
+             min_success_fraction=0.3),
+        dict(eta_values=(1e-8, 1e-6), eta_device=1e-9, omega_scale=1.7, collect_rejected=True),
+    ]
+    for kw in cases:
+        a, b = _spectrum_both_ways(monkeypatch, w, dev, lead, lead, ks, **kw)
+        _same_spectrum(a, b, kw)
+    # a multi-DOF spectrum, one k point (kpar = None)
+    rp = ribbon_problem(0, n_atoms=4, n_layers=6, n_omega=40)
+    a, b = _spectrum_both_ways(monkeypatch, rp.omegas, rp.device, rp.lead, rp.lead, [None],
+                               eta_values=(0.0, 1e-8, 1e-6), collect_rejected=True)
+    _same_spectrum(a, b, "ribbon")
+
+
+def _same_spectrum(a, b, tag):
+    import json
+
+    np.testing.assert_array_equal(a[0], b[0], err_msg=str(tag))
+    dump = lambda r: json.dumps(r, sort_keys=True, default=str)  # NaN-safe comparison
+    assert dump(a[1]) == dump(b[1]), tag
+    n_rej = sum(len(r.get("info", {}).get("rejected", [])) for r in a[1] if isinstance(r, dict))
+    print(f"{tag}: {n_rej} rejected attempts", flush=True)
