@@ -19,6 +19,7 @@ from .dos import (
 )
 from .negf import (
     device_green_function,
+    set_eta_ladders_in_kernel,
     transmission,
     transmission_kavg,
     transmission_kavg_adaptive,
@@ -89,5 +90,6 @@ __all__ = [
     "qe_omega_to_rad_s",
     "thermal_conductance_from_spectrum",
     "set_default_precision",
+    "set_eta_ladders_in_kernel",
     "get_default_precision",
 ]

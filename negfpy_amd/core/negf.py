@@ -556,11 +556,18 @@ class _KSolver:
         return solve_points([self.kps[k] for k in uniq], local, omega, eta, eta_dev,
                             precision=self.precision, route=self.route)
 
+    # Per-k eta ladders walked inside one persistent launch (set_eta_ladders_in_kernel)
+    # instead of one batched solve per rung.  Off by default: on one MI355X the
+    # rung-by-rung batches were faster (0.66x-0.91x for the in-launch form on
+    # spectra, 0.83x-0.88x on single-omega calls), since a ticket's instances wait in
+    # lockstep for the slowest one's rungs.
+    in_kernel_ladders = False
+
     def ladder_ok(self, ladders, eta_device) -> bool:
-        """Whether the eta ladders can run inside the persistent launch: FP64, the
-        decimation route, and no rung with a negative device eta (such a rung is an
-        "exception" of its own, decided on the host)."""
-        if self.precision != "fp64" or self.route != 0:
+        """Whether the eta ladders run inside the persistent launch: switched on, FP64,
+        the decimation route, and no rung with a negative device eta (such a rung is
+        an "exception" of its own, decided on the host)."""
+        if not self.in_kernel_ladders or self.precision != "fp64" or self.route != 0:
             return False
         if eta_device is None:
             return all(float(e) >= 0.0 for lad in ladders for e in lad)
@@ -606,6 +613,14 @@ class _KSolver:
             t_r[sel, :lt.shape[1]] = lt
             st_r[sel, :lt.shape[1]] = res.ladder.status.cpu().numpy()
         return rung, t_r, st_r
+
+
+def set_eta_ladders_in_kernel(flag: bool) -> None:
+    """Per-k adaptive eta (``transmission_kavg_adaptive*``, ``eta_scheme="adaptive"``):
+    True walks every point's eta ladder inside one persistent launch (FP64, decimation
+    route; one launch, e.g. for capture in a hipGraph), False (default) solves one
+    batch per rung.  Both give identical values, statistics and rejection records."""
+    _KSolver.in_kernel_ladders = bool(flag)
 
 
 def _t_max(lead_left, kpoints, max_channel_factor):

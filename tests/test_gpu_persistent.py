@@ -210,13 +210,18 @@ def test_transmission_spectra_matches_single_spectra(gpu):
 
 def _spectrum_both_ways(monkeypatch, *args, **kw):
     """The adaptive spectrum with the eta ladders inside the persistent launch, and
-    with the host's rung-by-rung loop (ladder_ok switched off)."""
+    with the host's rung-by-rung loop (the default)."""
     from negfpy_amd.core import negf as nm
 
-    in_kernel = nm.transmission_kavg_adaptive_spectrum(*args, **kw)
+    host = nm.transmission_kavg_adaptive_spectrum(*args, **kw)
     with monkeypatch.context() as m:
-        m.setattr(nm._KSolver, "ladder_ok", lambda self, ladders, eta_device: False)
-        host = nm.transmission_kavg_adaptive_spectrum(*args, **kw)
+        m.setattr(nm._KSolver, "in_kernel_ladders", True)
+        calls = []
+        orig = nm._KSolver.solve_ladder
+        m.setattr(nm._KSolver, "solve_ladder",
+                  lambda self, *a, **k: calls.append(1) or orig(self, *a, **k))
+        in_kernel = nm.transmission_kavg_adaptive_spectrum(*args, **kw)
+        assert calls, "the in-launch ladder was not used"
     return in_kernel, host
 
 

@@ -210,7 +210,7 @@ def oracle_backed(monkeypatch):
     monkeypatch.setattr(negf, "solve_points", _oracle_solve_points)
     monkeypatch.setattr(negf, "_max_block", lambda precision: 48)
     # the host's rung-by-rung eta loop (the in-launch ladder needs the GPU)
-    monkeypatch.setattr(negf._KSolver, "ladder_ok", lambda self, ladders, eta_device: False)
+    monkeypatch.setattr(negf._KSolver, "in_kernel_ladders", False)
     return negf
 
 
