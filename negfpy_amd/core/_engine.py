@@ -185,7 +185,9 @@ class EtaLadder:
 
 
 class TransmissionSolver:
-    """Reusable batched solver: workspace is kept between calls of equal size."""
+    """Reusable batched solver: workspace is kept between calls of equal size, so the
+    calls of one solver must stay on one stream (use one solver per stream or thread;
+    the public functions make a solver per call)."""
 
     def __init__(self, sets: TransportSets, *, precision="fp64", tol=1e-12, maxiter=200,
                  refine_steps=DEFAULT_REFINE_STEPS, surface_route=0):
@@ -357,9 +359,6 @@ class TransmissionSolver:
         return out
 
 
-_multi_ws: dict = {}
-
-
 def _ladder_struct(lad: EtaLadder, out: TransmissionResult):
     dev = out.t.device
     out.ladder = LadderResult(
@@ -441,11 +440,9 @@ def transmission_multi(jobs, outs=None) -> list:
     """
     prepared, chunks = _multi_prepare(jobs, outs)
     need = max((c[1] for c in chunks[0]), default=0)
-    dev = device()
-    ws = _multi_ws.get(dev)
-    if ws is None or ws.numel() < need:
-        ws = torch.empty((max(need, 1),), dtype=torch.uint8, device=dev)
-        _multi_ws[dev] = ws
+    # a workspace per call (the caching allocator makes it cheap): calls from several
+    # threads or streams never share one
+    ws = torch.empty((max(need, 1),), dtype=torch.uint8, device=device())
     return _multi_launch(prepared, chunks, ws)
 
 
