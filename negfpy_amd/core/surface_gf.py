@@ -10,10 +10,13 @@ of the lead's quadratic eigenvalue problem and return the retarded surface
 Green's function they span.  For eta > 0 the unit circle separates those modes
 from the growing ones and that Green's function is the unique retarded
 solution of g = (z - d00 - d01 g d10)^-1, which is the fixed point decimation
-converges to; those entry points therefore keep their argument checks and
-return the decimation result.  The reference's fallbacks for eta = 0 (picking
-the n smallest |lambda| when fewer than n lie inside the circle) and its
-rank/SVD-cutoff handling of degenerate mode sets are not reproduced.
+converges to; those entry points therefore keep their argument checks, take the
+decimation result, and where decimation fails take the eigenmode kernel's
+(``negf_surface_gf_eigen_batched``: the subspace of the n smallest-|lambda| modes
+from a spectral projector).  Not reproduced: eta = 0 inside the band, where
+modes of modulus 1 sit on both sides of the n-th and the reference keeps some of
+them in the order LAPACK's rounding gives them, and the rank/SVD-cutoff
+handling of degenerate mode sets.
 """
 
 from __future__ import annotations

@@ -80,9 +80,11 @@ def test_surface_gf_matches_reference(gpu, method):
         assert np.max(np.abs(g - g_ref)) <= 1e-9 * np.max(np.abs(g_ref)), w
 
 
-@pytest.mark.skip(reason="eta = 0 inside the band: the eigen routes' fallback to the n "
-                         "smallest |lambda| (surface_gf.py:123-124) is not served by the "
-                         "decimation kernel, which reports non-convergence there")
+@pytest.mark.skip(reason="eta = 0 inside the band: the pencil has six eigenvalues of modulus 1 "
+                         "(to rounding) between three decaying and three growing ones, and the "
+                         "reference's n-smallest-|lambda| fallback (surface_gf.py:123-124) keeps "
+                         "three of the six as LAPACK's rounding orders them; no spectral "
+                         "projector (or other eigensolver) reproduces that choice")
 def test_eigen_fallback_at_zero_eta(gpu):
     pass
 
