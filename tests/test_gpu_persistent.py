@@ -264,3 +264,49 @@ def _same_spectrum(a, b, tag):
     assert dump(a[1]) == dump(b[1]), tag
     n_rej = sum(len(r.get("info", {}).get("rejected", [])) for r in a[1] if isinstance(r, dict))
     print(f"{tag}: {n_rej} rejected attempts", flush=True)
+
+
+def test_eta_ladder_every_class_equals_staged_rungs(gpu):
+    """Engine-level ladders on 20-, 40- and 60-DOF problems (classes 24, 48 and the
+    pool) with rungs that fail (eta = 0 inside the band: no convergence within the
+    iteration limit) and a channel limit: each instance's accepted rung and the T and
+    status of every attempted rung equal the staged solves of those rungs, bit for
+    bit."""
+    ladders_of = lambda m: [((0.0, 1e-8, 1e-6) if i % 3 else (1e-7, 0.0, 1e-6)) for i in range(m)]
+    jobs, probs = [], []
+    for n, nl, nw, seed in ((20, 3, 9, 61), (40, 2, 5, 62), (60, 2, 3, 63)):
+        s, w, _ = _problem(seed, n, nl, nw, maxiter=60)
+        lads = ladders_of(nw)
+        t_max = [(2.5 if i % 2 else None) for i in range(nw)]
+        lad = _engine.EtaLadder(lads, t_max=t_max, nonneg_tol=0.0)
+        jobs.append((s, torch.zeros(nw, dtype=torch.int32, device="cuda"),
+                     torch.as_tensor(w, device="cuda"), None, None, lad))
+        probs.append((s, w, lads, t_max))
+    res = _engine.transmission_multi(jobs)
+    for (s, w, lads, t_max), r in zip(probs, res):
+        rung = r.ladder.rung.cpu().numpy()
+        t_r = r.ladder.t.cpu().numpy()
+        st_r = r.ladder.status.cpu().numpy()
+        # the staged solve of every rung for every instance
+        stage = []
+        for q in range(3):
+            eta = np.array([lad[q] for lad in lads])
+            o = s.run(torch.zeros(w.size, dtype=torch.int32, device="cuda"),
+                      torch.as_tensor(w, device="cuda"), torch.as_tensor(eta, device="cuda"))
+            stage.append((o.t.cpu().numpy(), o.status.cpu().numpy()))
+        n_retried = 0
+        for i in range(w.size):
+            want = 3
+            for q in range(3):
+                t, st = stage[q][0][i], stage[q][1][i]
+                assert st_r[i, q] == st, (i, q)
+                np.testing.assert_array_equal(t_r[i, q], t)
+                lim = np.inf if t_max[i] is None else t_max[i]
+                if st == 0 and np.isfinite(t) and t >= 0.0 and t <= lim:
+                    want = q
+                    break
+            assert rung[i] == want, (i, rung[i], want)
+            n_retried += int(want > 0)
+            assert np.all(st_r[i, min(want, 2) + 1:] == -1)   # rungs after the accepted one
+        print(f"n={s.sets.n}: {n_retried} of {w.size} instances retried", flush=True)
+        assert n_retried > 0
