@@ -18,6 +18,9 @@
 // still iterating (a per-instance convergence flag in LDS); each instance stops
 // at its own step, so its result and step count are those of the staged kernel.
 // The arithmetic of every stage is the staged kernels' own, in the same order.
+// Consecutive stages reuse the same LDS matrices, so each stage opens with a
+// barrier: the previous stage's last reads (its output stores from LDS) must be
+// done before the matrices are cleared.
 
 #pragma once
 
@@ -121,6 +124,7 @@ __device__ int pl_sr(const SrArgs& a, int inst, bool active, L& s, int tid, PlSh
   const int n = a.n;
   const size_t nn = (size_t)n * n;
   const double* blk = a.lead + (active ? (size_t)a.set_of[inst] * 3 * nn * 2 : 0);
+  __syncthreads();  // the previous stage's last reads of these matrices are done
   if (active) s.zero(tid);
   __syncthreads();
   if (active) {
@@ -308,6 +312,7 @@ __device__ void pl_sigma(const SigmaArgs& a, int inst, bool live, L& s, int tid)
   constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
   constexpr int LD = L::LD;
   const int nl = a.n_lead, nc = a.n_c, nd = a.n_dev;
+  __syncthreads();  // the previous stage's last reads of these matrices are done
   if (live) s.zero(tid);
   __syncthreads();
   int* pos = s.gj->pv;  // inverse contact map: device-layer index -> contact row
@@ -366,6 +371,7 @@ __device__ int pl_rgf(const RgfArgs& a, int inst, bool active, int st_in, L& s, 
   const double2* sgl = reinterpret_cast<const double2*>(a.sigma_l + io);
   const double2* sgr = reinterpret_cast<const double2*>(a.sigma_r + io);
   // LDS: [0] g (inverted in place), then Y = g H; [1] H; [2] Q = G_{1,i} H_i
+  __syncthreads();  // the previous stage's last reads of these matrices are done
   if (live) s.zero(tid);
   __syncthreads();
   const Tile<TM> t(n, n, tid);
@@ -479,6 +485,7 @@ __device__ double pl_trace(const TraceArgs& a, int inst, bool active, int status
   if (active && !live && tid == 0) a.t_out[inst] = NAN;
   const int n = a.n;
   const int nn = n * n;
+  __syncthreads();  // the previous stage's last reads of these matrices are done
   if (live) s.zero(tid);
   __syncthreads();
   if (live) {

@@ -310,3 +310,27 @@ def test_eta_ladder_every_class_equals_staged_rungs(gpu):
             assert np.all(st_r[i, min(want, 2) + 1:] == -1)   # rungs after the accepted one
         print(f"n={s.sets.n}: {n_retried} of {w.size} instances retried", flush=True)
         assert n_retried > 0
+
+
+def test_uneven_batch_equals_staged_repeatedly(gpu):
+    """The uneven batch (4096 x 24, 1024 x 48, 256 x 96 DOF; long 96-DOF tickets run
+    beside short ones) equals the staged launches bit for bit on every one of several
+    calls: consecutive pipeline stages of a ticket reuse the same LDS matrices, and a
+    stage may clear them only after the previous stage's last reads (a race that showed
+    as a few wrong T values per call under this load)."""
+    from negfpy_amd.workloads import uneven_problems
+
+    jobs = []
+    for p in uneven_problems():
+        lead, dev = p.lead, p.device
+        lb = np.stack([lead.d00, lead.d01, np.conj(lead.d01).T])[None]
+        n = lead.d00.shape[0]
+        sets = _engine.TransportSets.from_arrays(
+            lb, lb, np.conj(lead.d01).T[None], lead.d01[None], np.arange(n), np.arange(n),
+            np.stack(dev.onsite_blocks)[None], np.stack(dev.coupling_blocks)[None], True)
+        jobs.append(_job(_engine.TransmissionSolver(sets), p.omegas, p.eta))
+    staged = [_host(j[0].run(*j[1:])) for j in jobs]
+    for rep in range(4):
+        multi = [_host(r) for r in _engine.transmission_multi(jobs)]
+        for k, (a, b) in enumerate(zip(multi, staged)):
+            _assert_same(a, b, f"call {rep}, problem {k}")
