@@ -558,8 +558,8 @@ class _KSolver:
 
     # Per-k eta ladders walked inside one persistent launch (set_eta_ladders_in_kernel)
     # instead of one batched solve per rung.  Off by default: on one MI355X the
-    # rung-by-rung batches were faster (0.66x-0.91x for the in-launch form on
-    # spectra, 0.83x-0.88x on single-omega calls), since a ticket's instances wait in
+    # rung-by-rung batches were faster (0.64x-0.92x for the in-launch form on
+    # spectra, 0.82x-0.88x on single-omega calls), since a ticket's instances wait in
     # lockstep for the slowest one's rungs.
     in_kernel_ladders = False
 
