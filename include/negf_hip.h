@@ -38,6 +38,13 @@ extern "C" {
 
 enum { NEGF_FP64 = 0, NEGF_MIXED = 1 };
 enum { NEGF_SURFACE_DECIMATION = 0, NEGF_SURFACE_EIGEN = 1 };
+/* negf_transmission_problem.share_leads: independent leads; the right lead is the left
+ * one (one decimation serves both); the right lead is the left one mirrored, lead_r =
+ * (d00_L, d10_L, d01_L) for every set, as the interface contacts of a bulk force-constant
+ * model are -- one decimation yields both surfaces where the block fits (<= 48 DOF),
+ * two decimations otherwise.  The caller guarantees the mirror relation; lead_r must
+ * still be passed. */
+enum { NEGF_LEADS_INDEPENDENT = 0, NEGF_LEADS_SHARED = 1, NEGF_LEADS_MIRRORED = 2 };
 enum { NEGF_OK = 0, NEGF_ERR_ARG = 1, NEGF_ERR_HIP = 2 };
 
 /* ABI version (NEGF_ABI_VERSION). */
@@ -212,7 +219,7 @@ typedef struct negf_transmission_problem {
   int n_lead_r;     /* right lead block size */
   int n_c_l;        /* left contact size (rows of v_l) */
   int n_c_r;        /* right contact size (rows of v_r) */
-  int share_leads;  /* 1: the right lead equals the left one, one decimation serves both */
+  int share_leads;  /* NEGF_LEADS_INDEPENDENT, _SHARED or _MIRRORED */
   int maxiter;      /* decimation iteration limit */
   int refine_steps; /* NEGF_MIXED: FP64 residual steps per surface GF */
   double tol;       /* decimation tolerance */

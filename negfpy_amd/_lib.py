@@ -25,6 +25,9 @@ FP64 = 0
 MIXED = 1
 SURFACE_DECIMATION = 0
 SURFACE_EIGEN = 1
+LEADS_INDEPENDENT = 0   # NEGF_LEADS_*: negf_transmission_problem.share_leads
+LEADS_SHARED = 1
+LEADS_MIRRORED = 2
 MULTI_MAX_PROBLEMS = 8   # NEGF_MULTI_MAX_PROBLEMS
 
 OK = 0

@@ -112,6 +112,9 @@ class TransportSets:
     onsite: torch.Tensor     # (S, L, n, n)
     coupling: torch.Tensor   # (S, max(L-1,1), n, n)
     share_leads: bool
+    # the right lead is the left one with d01 and d10 exchanged in every set (the
+    # interface contacts of a bulk force-constant model): one decimation gives both
+    mirror_leads: bool = False
 
     @property
     def n(self) -> int:
@@ -135,10 +138,17 @@ class TransportSets:
         else:
             coupling = torch.zeros((s, 1, n, n), dtype=torch.complex128, device=onsite.device)
         lead_l = to_dev_c128(lead_l)
-        return cls(lead_l=lead_l, lead_r=lead_l if share_leads else to_dev_c128(lead_r),
-                   v_l=to_dev_c128(v_l), v_r=to_dev_c128(v_r), idx_l=to_dev_i32(idx_l),
-                   idx_r=to_dev_i32(idx_r), onsite=onsite, coupling=coupling,
-                   share_leads=bool(share_leads))
+        lead_r = lead_l if share_leads else to_dev_c128(lead_r)
+        return cls(lead_l=lead_l, lead_r=lead_r, v_l=to_dev_c128(v_l), v_r=to_dev_c128(v_r),
+                   idx_l=to_dev_i32(idx_l), idx_r=to_dev_i32(idx_r), onsite=onsite,
+                   coupling=coupling, share_leads=bool(share_leads),
+                   mirror_leads=not share_leads and mirrored(lead_l, lead_r))
+
+
+def mirrored(lead_l: torch.Tensor, lead_r: torch.Tensor) -> bool:
+    """lead_r = (d00, d10, d01) of lead_l in every set, exactly."""
+    return (lead_l.shape == lead_r.shape and torch.equal(lead_l[:, 0], lead_r[:, 0])
+            and torch.equal(lead_l[:, 1], lead_r[:, 2]) and torch.equal(lead_l[:, 2], lead_r[:, 1]))
 
 
 @dataclass
@@ -211,7 +221,8 @@ class TransmissionSolver:
         p.n_lead_r = int(s.lead_r.shape[2])
         p.n_c_l = int(s.v_l.shape[1])
         p.n_c_r = int(s.v_r.shape[1])
-        p.share_leads = 1 if s.share_leads else 0
+        p.share_leads = (_lib.LEADS_SHARED if s.share_leads else
+                         _lib.LEADS_MIRRORED if s.mirror_leads else _lib.LEADS_INDEPENDENT)
         p.surface_route = self.surface_route
         p.maxiter = self.maxiter
         p.refine_steps = self.refine_steps

@@ -220,9 +220,66 @@ struct SrArgs {
   int32_t* iters;
   int32_t* status;
   int redo;  // 1: only instances whose status is nonzero are (re)computed
+  // mirrored lead (d00, d10, d01), from the same decimation (k_sancho_rubio<T, NMAX, true>)
+  double* g_out2;
+  double* dos_out2;
+  int32_t* iters2;
+  int32_t* status2;
 };
 
-template <typename T, int NMAX>
+// g = (z - e_s)^-1 from the VGPR tiles of e_s into out (FP64), or e_s itself (FP32
+// mode, for the FP64 refinement); the surface DOS (omega / pi) tr(-2 Im g).
+template <typename T, int NMAX, int TM, typename L>
+__device__ __forceinline__ void sr_write(const T (&er)[TM][TM], const T (&ei)[TM][TM],
+                                         const Tile<TM>& t, L& s, int n, T zr, T zi, double omega,
+                                         double* out, double* dos, int32_t* status, int inst,
+                                         int tid) {
+  constexpr int NT = Cfg<NMAX>::NT;
+  constexpr int LD = L::LD;
+  if constexpr (sizeof(T) == 4) {
+    if (t.own) {
+      double2* o = reinterpret_cast<double2*>(out);
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          const int i = t.i0 + x, j = t.j0 + y;
+          if (i < n && j < n) o[i * n + j] = make_double2((double)er[x][y], (double)ei[x][y]);
+        }
+    }
+  } else {
+    __syncthreads();  // earlier reads of plane 2 are done
+    if (t.own) {
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          const int i = t.i0 + x, j = t.j0 + y;
+          if (i < n && j < n) {
+            s.re[2][i * LD + j] = (i == j ? zr : T(0)) - er[x][y];
+            s.im[2][i * LD + j] = (i == j ? zi : T(0)) - ei[x][y];
+          }
+        }
+    }
+    __syncthreads();
+    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[2], s.im[2], LD, n, *s.gj, tid);
+    if (st != ST_OK) {
+      if (tid == 0) status[inst] = st;
+      return;
+    }
+    gstore<T, NT>(out, n, s.re[2], s.im[2], LD, tid);
+    if (tid == 0 && dos) {
+      double tr = 0.0;
+      for (int i = 0; i < n; ++i) tr += -2.0 * (double)s.im[2][i * LD + i];
+      dos[inst] = (omega / M_PI) * tr;
+    }
+  }
+}
+
+// MIRROR: the decimation also sums e~_s += b g a, the surface term of the lead with
+// d01 and d10 exchanged (the same e, alpha and beta iterates; the reference decimates
+// that lead on its own, surface_gf.py:39-79), and writes its g to g_out2.
+template <typename T, int NMAX, bool MIRROR>
 __global__ void __launch_bounds__(Cfg<NMAX>::NT)
     __attribute__((amdgpu_waves_per_eu(Occ<T, NMAX>::value))) k_sancho_rubio(SrArgs a) {
   constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
@@ -232,7 +289,11 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
   L s(smem);
   const int tid = threadIdx.x;
   const int inst = blockIdx.x;
-  if (a.redo && a.status[inst] == ST_OK) return;  // uniform per block
+  // redo pass: only the surfaces whose status is nonzero (uniform per block)
+  const bool do1 = !a.redo || a.status[inst] != ST_OK;
+  bool do2 = false;
+  if constexpr (MIRROR) do2 = !a.redo || a.status2[inst] != ST_OK;
+  if (!do1 && !do2) return;
   const int n = a.n;
   const size_t nn = (size_t)n * n;
   const double* blk = a.lead + (size_t)a.set_of[inst] * 3 * nn * 2;
@@ -260,12 +321,17 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
         }
       }
   }
+  T mr[MIRROR ? TM : 1][MIRROR ? TM : 1], mi[MIRROR ? TM : 1][MIRROR ? TM : 1];
 #pragma unroll
   for (int x = 0; x < TM; ++x)
 #pragma unroll
     for (int y = 0; y < TM; ++y) {
       sr[x][y] = er[x][y];
       si[x][y] = ei[x][y];
+      if constexpr (MIRROR) {
+        mr[x][y] = er[x][y];
+        mi[x][y] = ei[x][y];
+      }
     }
   double zrd, zid;
   zsq(a.omega[inst], a.eta[inst], zrd, zid);
@@ -345,6 +411,11 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
           if (i < n && j < n) {
             er[x][y] = er[x][y] + p2r[x][y];
             ei[x][y] = ei[x][y] + p2i[x][y];
+            if constexpr (MIRROR) {
+              mr[x][y] = mr[x][y] + p2r[x][y];
+              mi[x][y] = mi[x][y] + p2i[x][y];
+              bad |= !(finite_d(mr[x][y]) && finite_d(mi[x][y]));
+            }
             s.re[0][i * LD + j] = p3r[x][y];
             s.im[0][i * LD + j] = p3i[x][y];
             s.re[1][i * LD + j] = p4r[x][y];
@@ -372,50 +443,26 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
   }
   if (status == ST_OK && !conv) status = ST_NOCONV;
   if (tid == 0) {
-    a.status[inst] = status;
-    a.iters[inst] = it;
+    if (do1) {
+      a.status[inst] = status;
+      a.iters[inst] = it;
+    }
+    if constexpr (MIRROR) {
+      if (do2) {
+        a.status2[inst] = status;
+        a.iters2[inst] = it;
+      }
+    }
   }
   if (status != ST_OK) return;
-
-  double* out = a.g_out + (size_t)inst * nn * 2;
-  if constexpr (sizeof(T) == 4) {
-    // FP32 mode: hand e_s to the FP64 refinement.
-    if (t.own) {
-      double2* o = reinterpret_cast<double2*>(out);
-#pragma unroll
-      for (int x = 0; x < TM; ++x)
-#pragma unroll
-        for (int y = 0; y < TM; ++y) {
-          const int i = t.i0 + x, j = t.j0 + y;
-          if (i < n && j < n) o[i * n + j] = make_double2((double)sr[x][y], (double)si[x][y]);
-        }
-    }
-    return;
-  } else {
-    if (t.own) {
-#pragma unroll
-      for (int x = 0; x < TM; ++x)
-#pragma unroll
-        for (int y = 0; y < TM; ++y) {
-          const int i = t.i0 + x, j = t.j0 + y;
-          if (i < n && j < n) {
-            s.re[2][i * LD + j] = (i == j ? zr : T(0)) - sr[x][y];
-            s.im[2][i * LD + j] = (i == j ? zi : T(0)) - si[x][y];
-          }
-        }
-    }
-    __syncthreads();
-    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[2], s.im[2], LD, n, *s.gj, tid);
-    if (st != ST_OK) {
-      if (tid == 0) a.status[inst] = st;
-      return;
-    }
-    gstore<T, NT>(out, n, s.re[2], s.im[2], LD, tid);
-    if (tid == 0 && a.dos_out) {
-      double tr = 0.0;
-      for (int i = 0; i < n; ++i) tr += -2.0 * (double)s.im[2][i * LD + i];
-      a.dos_out[inst] = (a.omega[inst] / M_PI) * tr;
-    }
+  const double w = a.omega[inst];
+  if (do1)
+    sr_write<T, NMAX, TM>(sr, si, t, s, n, zr, zi, w, a.g_out + (size_t)inst * nn * 2, a.dos_out,
+                          a.status, inst, tid);
+  if constexpr (MIRROR) {
+    if (do2)
+      sr_write<T, NMAX, TM>(mr, mi, t, s, n, zr, zi, w, a.g_out2 + (size_t)inst * nn * 2,
+                            a.dos_out2, a.status2, inst, tid);
   }
 }
 
@@ -1707,14 +1754,20 @@ class BigUse {
     if (e_ != hipSuccess) return hip_check(e_, #KERNEL " launch");                              \
   } while (0)
 
-template <typename T>
+// block classes of the mirrored-lead decimation
+static bool sr_mirror_class(int nmax) { return nmax >= 8 && nmax <= 48; }
+
+template <typename T, bool MIRROR = false>
 static int run_sr(hipStream_t st, int nmax, const SrArgs& a) {
-#define SR_CASE(NM)                                                                   \
-  case NM: {                                                                          \
-    constexpr int NT = Cfg<NM>::NT;                                                   \
-    NEGF_LAUNCH((k_sancho_rubio<T, NM>), a.n_inst, NT, (Lds<T, NM, NT, 3>::bytes()), st, a); \
-    return NEGF_OK;                                                                   \
+#define SR_CASE(NM)                                                                           \
+  case NM: {                                                                                  \
+    constexpr int NT = Cfg<NM>::NT;                                                           \
+    NEGF_LAUNCH((k_sancho_rubio<T, NM, MIRROR>), a.n_inst, NT, (Lds<T, NM, NT, 3>::bytes()), st, \
+                a);                                                                           \
+    return NEGF_OK;                                                                           \
   }
+  if (MIRROR && !sr_mirror_class(nmax))
+    return fail(NEGF_ERR_ARG, "no mirrored decimation for block class %d", nmax);
   switch (nmax) {
     SR_CASE(8)
     SR_CASE(16)
@@ -1722,14 +1775,15 @@ static int run_sr(hipStream_t st, int nmax, const SrArgs& a) {
     SR_CASE(32)
     SR_CASE(48)
     case 64:
-      if constexpr (sizeof(T) == 4) {
+      if constexpr (sizeof(T) == 4 && !MIRROR) {
         constexpr int NT = Cfg<64>::NT;
-        NEGF_LAUNCH((k_sancho_rubio<T, 64>), a.n_inst, NT, (Lds<T, 64, NT, 3>::bytes()), st, a);
+        NEGF_LAUNCH((k_sancho_rubio<T, 64, false>), a.n_inst, NT, (Lds<T, 64, NT, 3>::bytes()), st,
+                    a);
         return NEGF_OK;
       }
       break;
     case BIG:
-      if constexpr (sizeof(T) == 8) NEGF_BIG_LAUNCH(k_sr_big, st, a);
+      if constexpr (sizeof(T) == 8 && !MIRROR) NEGF_BIG_LAUNCH(k_sr_big, st, a);
       break;
   }
 #undef SR_CASE
@@ -2134,6 +2188,43 @@ int negf_transmission_trace_batched(void* stream, int n, int n_inst, const doubl
   return run_trace(reinterpret_cast<hipStream_t>(stream), nmax, a);
 }
 
+// a mirrored lead pair (right lead = left lead with d01 and d10 exchanged) whose blocks
+// fit the mirrored decimation: one decimation yields both surfaces
+static bool sr_mirror_fits(const negf_transmission_problem* p) {
+  if (p->share_leads != NEGF_LEADS_MIRRORED || p->n_lead_r != p->n_lead_l) return false;
+  if (p->precision == NEGF_FP64) return sr_mirror_class(pick_nmax(p->n_lead_l, true));
+  if (p->precision == NEGF_MIXED)
+    return p->n_lead_l <= MIXED_MAX && sr_mirror_class(pick_nmax(p->n_lead_l, false)) &&
+           sr_mirror_class(pick_nmax(p->n_lead_l, true));
+  return false;
+}
+
+// g_L = (z - e_s)^-1 and g_R = (z - e~_s)^-1 from one decimation of the left lead;
+// mixed precision refines each surface against its own lead and redoes the failures
+// of either in FP64, as negf_sancho_rubio_batched does for one lead.
+static int sr_mirrored(void* stream, const negf_transmission_problem* p, double* g_l,
+                       double* g_r, int32_t* st_l, int32_t* st_r) {
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const int n = p->n_lead_l;
+  if (p->maxiter < 0) return fail(NEGF_ERR_ARG, "invalid maxiter %d", p->maxiter);
+  if (!p->lead_l || !p->set_of || !p->omega || !p->eta_lead || !p->iters_l)
+    return fail(NEGF_ERR_ARG, "null pointer argument");
+  SrArgs a{n,        p->n_inst,   p->lead_l, p->set_of, p->omega, p->eta_lead, p->tol,
+           p->maxiter, g_l,       p->dos_l,  p->iters_l, st_l,     0,           g_r,
+           p->dos_r, p->iters_r,  st_r};
+  if (p->precision == NEGF_FP64) return run_sr<double, true>(st, pick_nmax(n, true), a);
+  int rc = run_sr<float, true>(st, pick_nmax(n, false), a);
+  if (rc || p->refine_steps < 0) return rc;
+  rc = negf_surface_refine_batched(stream, n, p->n_inst, p->lead_l, p->set_of, p->omega,
+                                   p->eta_lead, p->refine_steps, g_l, p->dos_l, st_l);
+  if (rc) return rc;
+  rc = negf_surface_refine_batched(stream, n, p->n_inst, p->lead_r, p->set_of, p->omega,
+                                   p->eta_lead, p->refine_steps, g_r, p->dos_r, st_r);
+  if (rc) return rc;
+  a.redo = 1;
+  return run_sr<double, true>(st, pick_nmax(n, true), a);
+}
+
 size_t negf_transmission_workspace_bytes(const negf_transmission_problem* p) {
   if (!p) return 0;
   const size_t ni = (size_t)p->n_inst;
@@ -2141,7 +2232,7 @@ size_t negf_transmission_workspace_bytes(const negf_transmission_problem* p) {
   const size_t nd = (size_t)p->n * p->n;
   size_t b = 0;
   b += ni * nll * 16;                              // g_L
-  b += p->share_leads ? 0 : ni * nlr * 16;         // g_R
+  b += p->share_leads == NEGF_LEADS_SHARED ? 0 : ni * nlr * 16;  // g_R
   b += 2 * ni * nd * 16;                           // Sigma_L, Sigma_R
   b += ni * nd * 16;                               // G_1N
   b += 2 * ni * sizeof(int32_t);                   // lead status L/R
@@ -2160,7 +2251,7 @@ int negf_transmission_batched(void* stream, const negf_transmission_problem* p, 
   double* g_l = reinterpret_cast<double*>(w);
   w += ni * p->n_lead_l * p->n_lead_l * 16;
   double* g_r = g_l;
-  if (!p->share_leads) {
+  if (p->share_leads != NEGF_LEADS_SHARED) {
     g_r = reinterpret_cast<double*>(w);
     w += ni * p->n_lead_r * p->n_lead_r * 16;
   }
@@ -2179,21 +2270,33 @@ int negf_transmission_batched(void* stream, const negf_transmission_problem* p, 
     return fail(NEGF_ERR_ARG, "unknown surface route %d", p->surface_route);
   if (p->ladder)
     return fail(NEGF_ERR_ARG, "eta ladders are solved by negf_transmission_multi_batched");
+  if (p->share_leads < 0 || p->share_leads > NEGF_LEADS_MIRRORED)
+    return fail(NEGF_ERR_ARG, "share_leads %d is not 0, 1 or 2", p->share_leads);
+  if (p->share_leads != NEGF_LEADS_SHARED && (!p->lead_r || !p->iters_r))
+    return fail(NEGF_ERR_ARG, "null pointer argument (right lead)");
   const bool eig = p->surface_route == NEGF_SURFACE_EIGEN;
-  int rc = negf_sancho_rubio_batched(stream, p->precision, p->n_lead_l, p->n_inst, p->lead_l,
+  int rc;
+  if (sr_mirror_fits(p)) {
+    rc = sr_mirrored(stream, p, g_l, g_r, st_l, st_r);
+    if (rc) return rc;
+  } else {
+    rc = negf_sancho_rubio_batched(stream, p->precision, p->n_lead_l, p->n_inst, p->lead_l,
+                                   p->set_of, p->omega, p->eta_lead, p->tol, p->maxiter,
+                                   p->refine_steps, g_l, p->dos_l, p->iters_l, st_l);
+    if (rc) return rc;
+    if (p->share_leads != NEGF_LEADS_SHARED) {
+      rc = negf_sancho_rubio_batched(stream, p->precision, p->n_lead_r, p->n_inst, p->lead_r,
                                      p->set_of, p->omega, p->eta_lead, p->tol, p->maxiter,
-                                     p->refine_steps, g_l, p->dos_l, p->iters_l, st_l);
-  if (rc) return rc;
+                                     p->refine_steps, g_r, p->dos_r, p->iters_r, st_r);
+      if (rc) return rc;
+    }
+  }
   if (eig) {
     rc = negf_surface_gf_eigen_batched(stream, p->n_lead_l, p->n_inst, p->lead_l, p->set_of,
                                        p->omega, p->eta_lead, g_l, p->dos_l, st_l);
     if (rc) return rc;
   }
-  if (!p->share_leads) {
-    rc = negf_sancho_rubio_batched(stream, p->precision, p->n_lead_r, p->n_inst, p->lead_r,
-                                   p->set_of, p->omega, p->eta_lead, p->tol, p->maxiter,
-                                   p->refine_steps, g_r, p->dos_r, p->iters_r, st_r);
-    if (rc) return rc;
+  if (p->share_leads != NEGF_LEADS_SHARED) {
     if (eig) {
       rc = negf_surface_gf_eigen_batched(stream, p->n_lead_r, p->n_inst, p->lead_r, p->set_of,
                                          p->omega, p->eta_lead, g_r, p->dos_r, st_r);
@@ -2250,6 +2353,8 @@ static int pl_check(const negf_transmission_problem& p, int i) {
                 "problem %d: invalid sizes (n=%d layers=%d n_inst=%d leads %d/%d contacts %d/%d "
                 "maxiter=%d)",
                 i, p.n, p.n_layers, p.n_inst, p.n_lead_l, p.n_lead_r, p.n_c_l, p.n_c_r, p.maxiter);
+  if (p.share_leads < 0 || p.share_leads > NEGF_LEADS_MIRRORED)
+    return fail(NEGF_ERR_ARG, "problem %d: share_leads %d is not 0, 1 or 2", i, p.share_leads);
   if (p.share_leads && p.n_lead_r != p.n_lead_l)
     return fail(NEGF_ERR_ARG, "problem %d: shared leads of different sizes", i);
   if (p.surface_route != NEGF_SURFACE_DECIMATION && p.surface_route != NEGF_SURFACE_EIGEN)
@@ -2268,10 +2373,10 @@ static int pl_check(const negf_transmission_problem& p, int i) {
       return fail(NEGF_ERR_ARG, "problem %d: null pointer in the eta ladder", i);
   }
   if (p.n_inst == 0) return NEGF_OK;
-  if (!p.lead_l || (!p.share_leads && !p.lead_r) || !p.v_l || !p.v_r || !p.idx_l || !p.idx_r ||
+  if (!p.lead_l || (p.share_leads != NEGF_LEADS_SHARED && !p.lead_r) || !p.v_l || !p.v_r || !p.idx_l || !p.idx_r ||
       !p.onsite || (p.n_layers > 1 && !p.coupling) || !p.set_of || !p.omega ||
       (!p.ladder && !p.eta_lead) || ((!p.ladder || p.ladder->eta_dev_fixed) && !p.eta_dev) ||
-      !p.t_out || !p.iters_l || (!p.share_leads && !p.iters_r) || !p.status)
+      !p.t_out || !p.iters_l || (p.share_leads != NEGF_LEADS_SHARED && !p.iters_r) || !p.status)
     return fail(NEGF_ERR_ARG, "problem %d: null pointer argument", i);
   return NEGF_OK;
 }
@@ -2315,7 +2420,7 @@ int negf_transmission_multi_batched(void* stream, int n_problems,
     m = m > p.n_lead_r ? m : p.n_lead_r;
     cls[i] = default_nmax(m, true);
     const double c = (double)cls[i];
-    cost[i] = c * c * c * ((p.share_leads ? 1.0 : 2.0) * 30.0 * 7.0 + 4.0 * p.n_layers);
+    cost[i] = c * c * c * ((p.share_leads == NEGF_LEADS_SHARED ? 1.0 : 2.0) * 30.0 * 7.0 + 4.0 * p.n_layers);
     int k = np++;
     while (k > 0 && cost[order[k - 1]] < cost[i]) {
       order[k] = order[k - 1];
@@ -2349,7 +2454,7 @@ int negf_transmission_multi_batched(void* stream, int n_problems,
     q.g_l = reinterpret_cast<double*>(base);
     base += ni * p.n_lead_l * p.n_lead_l * 16;
     q.g_r = q.g_l;
-    if (!p.share_leads) {
+    if (p.share_leads != NEGF_LEADS_SHARED) {
       q.g_r = reinterpret_cast<double*>(base);
       base += ni * p.n_lead_r * p.n_lead_r * 16;
     }
@@ -2369,13 +2474,13 @@ int negf_transmission_multi_batched(void* stream, int n_problems,
     q.n_lead_r = p.n_lead_r;
     q.n_c_l = p.n_c_l;
     q.n_c_r = p.n_c_r;
-    q.share = p.share_leads ? 1 : 0;
+    q.share = p.share_leads == NEGF_LEADS_SHARED ? 1 : 0;
     q.maxiter = p.maxiter;
     q.cls = cls[i];
     q.route = p.surface_route;
     q.tol = p.tol;
     q.lead_l = p.lead_l;
-    q.lead_r = p.share_leads ? p.lead_l : p.lead_r;
+    q.lead_r = p.share_leads == NEGF_LEADS_SHARED ? p.lead_l : p.lead_r;
     q.v_l = p.v_l;
     q.v_r = p.v_r;
     q.idx_l = p.idx_l;

@@ -43,8 +43,9 @@ def test_problem_struct_layout():
 
 
 def test_multi_problem_entry_rejects_bad_arguments():
-    """Argument checks of negf_transmission_multi_batched, which return before any
-    HIP call (so they run without a GPU)."""
+    """Argument checks of negf_transmission_multi_batched (and the lead-sharing mode of
+    negf_transmission_batched), which return before any HIP call (so they run
+    without a GPU)."""
     import ctypes
 
     lib = _lib.load_symbols_only()
@@ -70,6 +71,15 @@ def test_multi_problem_entry_rejects_bad_arguments():
     arr[0].n_c_l = 4
     assert lib.negf_transmission_multi_batched(None, 1, arr, None, 0) == 1  # null pointers
     assert b"null pointer" in lib.negf_last_error()
+    arr[0].share_leads = 3  # not NEGF_LEADS_INDEPENDENT, _SHARED or _MIRRORED
+    assert lib.negf_transmission_multi_batched(None, 1, arr, None, 0) == 1
+    assert b"share_leads 3" in lib.negf_last_error()
+    need = lib.negf_transmission_workspace_bytes(ctypes.byref(arr[0]))
+    # the staged entry checks it before any HIP call too (the workspace is not touched)
+    assert lib.negf_transmission_batched(None, ctypes.byref(arr[0]), ctypes.c_void_p(256),
+                                         need) == 1
+    assert b"share_leads 3" in lib.negf_last_error()
+    arr[0].share_leads = _lib.LEADS_INDEPENDENT
     assert lib.negf_transmission_multi_workspace_bytes(0, arr) == 0
     assert lib.negf_transmission_multi_workspace_bytes(1, arr) > \
         lib.negf_transmission_workspace_bytes(ctypes.byref(arr[0]))
