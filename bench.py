@@ -19,6 +19,7 @@ instances, over a fork pool of the host's cores with one BLAS thread each.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -188,6 +189,80 @@ def adjoint_batch(steps: int, warmup: int) -> dict:
                                                          / np.maximum(np.abs(tf[ok]), 1e-3)))}
 
 
+def sheet_kavg(steps: int, warmup: int) -> dict:
+    """The 2-D sheet workload (12-DOF graphene-like cells, 16 layers, interface-contact
+    leads; 128 ky x 2048 omega = 262,144 instances per step).  A step builds the blocks
+    of every k point on the GPU, solves every (omega, k) in one batch, averages T over
+    k and integrates kappa(300 K) on the GPU; FP64 and mixed precision."""
+    import torch
+
+    from negfpy_amd.core import _engine
+    from negfpy_amd.core.negf import kspace_sets
+    from negfpy_amd.workloads import OMEGA_UNIT_RAD_S, sheet_problem
+
+    p = sheet_problem()
+    nk, nw = len(p.kpoints), p.omegas.size
+    n_inst = nk * nw
+    set_of = torch.arange(nk, dtype=torch.int32, device="cuda").repeat(nw)
+    omega = torch.tensor(np.repeat(p.omegas, nk), dtype=torch.float64, device="cuda")
+    eta = torch.full((n_inst,), p.eta, dtype=torch.float64, device="cuda")
+    w_rad = torch.tensor(p.omegas * OMEGA_UNIT_RAD_S, dtype=torch.float64, device="cuda")
+    temps = torch.tensor([300.0], dtype=torch.float64, device="cuda")
+    res = {}
+    # the leads are mirrored (one decimation gives both surfaces); "two" decimates each
+    for name, prec, mirror in (("fp64", "fp64", True), ("mixed", "mixed", True),
+                               ("two", "fp64", False)):
+        out = None
+
+        def step():
+            nonlocal out
+            sets = kspace_sets(p.device, p.lead_left, p.lead_right, p.kpoints,
+                               device_to_lead_left=p.v_left, device_to_lead_right=p.v_right,
+                               precision=prec)
+            assert sets.mirror_leads
+            if not mirror:
+                sets = dataclasses.replace(sets, mirror_leads=False)
+            solver = _engine.TransmissionSolver(sets, precision=prec)
+            if out is None:
+                out = solver.alloc_result(n_inst)
+            solver.run(set_of, omega, eta, out=out)
+            t = out.t.view(nw, nk).mean(dim=1)
+            return t, _engine.thermal_conductance_tensor(w_rad, t, temps)
+
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t, kap = step()
+        torch.cuda.synchronize()
+        res[name] = {"dt": (time.perf_counter() - t0) / steps, "t": t.cpu().numpy(),
+                     "kappa": float(kap.cpu()[0]),
+                     "failed": int((out.status != 0).sum().item())}
+    t64, t32 = res["fp64"]["t"], res["mixed"]["t"]
+    ok = np.isfinite(t64) & np.isfinite(t32)
+    rel_t = np.abs(t32[ok] - t64[ok]) / np.maximum(np.abs(t64[ok]), 1e-3)
+    rel_k = abs(res["mixed"]["kappa"] - res["fp64"]["kappa"]) / abs(res["fp64"]["kappa"])
+    return {"workload": f"graphene-like sheet: n=12 DOF/layer, 16 layers, interface-contact "
+                        f"leads, {nk} ky x {nw} omega, eta={p.eta:g}; k-point blocks built "
+                        f"on the GPU, T(omega) k-averaged and kappa(300 K) in each step",
+            "instances": n_inst,
+            "fp64_solves_per_s": round(n_inst / res["fp64"]["dt"], 1),
+            "fp64_ms_per_step": round(res["fp64"]["dt"] * 1e3, 3),
+            "fp64_two_decimations_ms_per_step": round(res["two"]["dt"] * 1e3, 3),
+            "one_over_two_decimations": round(res["two"]["dt"] / res["fp64"]["dt"], 3),
+            "t_one_vs_two_decimations_max_rel": float(np.max(
+                np.abs(res["two"]["t"] - t64) / np.maximum(np.abs(res["two"]["t"]), 1e-3))),
+            "mixed_solves_per_s": round(n_inst / res["mixed"]["dt"], 1),
+            "mixed_over_fp64": round(res["fp64"]["dt"] / res["mixed"]["dt"], 3),
+            "kappa_300K_W_per_K": res["fp64"]["kappa"],
+            "mixed_kappa_rel_err": rel_k,
+            "mixed_t_kavg_rel_err_max": float(rel_t.max()) if rel_t.size else None,
+            "mixed_meets_1e-6": bool(rel_t.size and rel_t.max() <= 1e-6 and rel_k <= 1e-6),
+            "failed_instances": {"fp64": res["fp64"]["failed"],
+                                 "mixed": res["mixed"]["failed"]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -350,6 +425,8 @@ def main():
     print(f"uneven batch: {line['uneven_batch']}", file=sys.stderr, flush=True)
     line["adjoint_batch"] = adjoint_batch(max(1, min(args.steps, 3)), 1)
     print(f"adjoint batch: {line['adjoint_batch']}", file=sys.stderr, flush=True)
+    line["sheet_kavg"] = sheet_kavg(max(1, min(args.steps, 3)), 1)
+    print(f"sheet k-average: {line['sheet_kavg']}", file=sys.stderr, flush=True)
     if cpu is not None:
         sel = cpu["sel"]
         err64 = np.abs(t64[sel] - cpu["vals"]) / np.maximum(np.abs(cpu["vals"]), 1e-3)

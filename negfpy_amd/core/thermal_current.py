@@ -49,9 +49,12 @@ def heat_current_from_spectrum(omegas: Iterable[float], transmission_vals: Itera
 
 
 def transmission_spectrum(omegas, device, lead_left, lead_right, *, kpoints=None, eta=1e-8,
-                          precision=None) -> Array:
+                          precision=None, device_to_lead_left=None,
+                          device_to_lead_right=None) -> Array:
     """k-averaged T(omega) of a whole spectrum in one batch (kpoints=None: one point, kpar=None).
-    Raises the reference's exception of the first failing (omega, k) point."""
+    ``device_to_lead_left`` / ``_right`` are the couplings of ``transmission_kavg``
+    (negf.py:360-397).  Raises the reference's exception of the first failing
+    (omega, k) point."""
     prec = _prec(precision)
     w = np.asarray(list(omegas), dtype=float)
     ks = [None] if kpoints is None else list(kpoints)
@@ -61,14 +64,16 @@ def transmission_spectrum(omegas, device, lead_left, lead_right, *, kpoints=None
     om = np.repeat(w, nk)
     kk = np.tile(np.arange(nk), w.size)
     et = np.full(om.size, float(eta))
-    sets = kspace_sets(device, lead_left, lead_right, ks, precision=prec) \
+    sets = kspace_sets(device, lead_left, lead_right, ks, device_to_lead_left=device_to_lead_left,
+                       device_to_lead_right=device_to_lead_right, precision=prec) \
         if kpoints is not None else None
     if sets is not None:
         res = _engine.TransmissionSolver(sets, precision=prec).run(kk.astype(np.int32), om, et, et)
         t, st = res.t.cpu().numpy(), res.status.cpu().numpy()
     else:
         kps = _resolve_many(device, lead_left, lead_right, ks, eta=eta, eta_device=None,
-                            device_to_lead_left=None, device_to_lead_right=None,
+                            device_to_lead_left=device_to_lead_left,
+                            device_to_lead_right=device_to_lead_right,
                             contact_left_indices=None, contact_right_indices=None,
                             omega_scale=None, precision=prec)
         t, st = solve_points(kps, kk, om, et, et, precision=prec)

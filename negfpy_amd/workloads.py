@@ -173,3 +173,111 @@ def interface_problem(seed: int = 0, n_atoms: int = 16, n_layers: int = 64, n_om
     omegas = 1.1 * wmax * (np.arange(n_omega) + 0.5) / n_omega
     return InterfaceProblem(lead_left=lead_l, lead_right=multidof_lead_blocks(right),
                             device=device, omegas=omegas, eta=eta)
+
+
+# bond length (A) and (radial, in-plane tangential, out-of-plane) springs in eV/A^2 of
+# the first, second and third neighbour shells of the graphene-like sheet
+SHEET_BOND = 1.42
+SHEET_SPRINGS = ((22.8, 15.3, 6.1), (5.5, 2.0, 0.5), (1.9, 1.0, 0.2))
+
+
+def sheet_params(seed: int = 0, jitter: float = 0.05, mass: float = 12.0):
+    """Graphene-like honeycomb sheet in its 4-atom orthorhombic cell (12 DOF):
+    transport along x (period 3 d), one transverse direction y (period sqrt(3) d),
+    d = SHEET_BOND.  Every bond of the first three neighbour shells (d, sqrt(3) d,
+    2 d) is a spring K = kr e e^T + kt t t^T + kz z z^T (e the bond direction, t its
+    in-plane normal) with the shell's springs times a seeded factor in
+    [1 - jitter, 1 + jitter] per bond; Phi_ab = -K off the diagonal, acoustic sum
+    rule on it.  Returns ``MaterialKspaceParams`` with (dy, 0) terms."""
+    from negfpy_amd.models.material_kspace import MaterialKspaceParams
+
+    d = SHEET_BOND
+    pos = np.array([[0.0, 0.0], [d, 0.0], [1.5 * d, 0.5 * np.sqrt(3) * d],
+                    [2.5 * d, 0.5 * np.sqrt(3) * d]])
+    a1, a2 = np.array([3.0 * d, 0.0]), np.array([0.0, np.sqrt(3) * d])
+    shells = (d, np.sqrt(3) * d, 2.0 * d)
+    rng = np.random.default_rng(seed)
+    factor: dict = {}
+    phi: dict = {}
+    onsite = np.zeros((12, 12))
+    for a in range(4):
+        for b in range(4):
+            for sx in (-1, 0, 1):
+                for sy in (-1, 0, 1):
+                    r = pos[b] + sx * a1 + sy * a2 - pos[a]
+                    dist = float(np.hypot(*r))
+                    shell = [i for i, s in enumerate(shells) if abs(dist - s) < 1e-6]
+                    if not shell:
+                        continue
+                    # one factor per bond: (a, b, s) and (b, a, -s) are the same bond
+                    key = (a, b, sx, sy)
+                    twin = (b, a, -sx, -sy)
+                    if twin in factor:
+                        factor[key] = factor[twin]
+                    elif key not in factor:
+                        factor[key] = 1.0 + jitter * rng.uniform(-1.0, 1.0)
+                    kr, kt, kz = (factor[key] * k for k in SHEET_SPRINGS[shell[0]])
+                    e = np.array([r[0], r[1], 0.0]) / dist
+                    t = np.array([-e[1], e[0], 0.0])
+                    z = np.array([0.0, 0.0, 1.0])
+                    k = kr * np.outer(e, e) + kt * np.outer(t, t) + kz * np.outer(z, z)
+                    blk = phi.setdefault((sx, sy), np.zeros((12, 12)))
+                    blk[3 * a:3 * a + 3, 3 * b:3 * b + 3] -= k
+                    onsite[3 * a:3 * a + 3, 3 * a:3 * a + 3] += k
+    phi.setdefault((0, 0), np.zeros((12, 12)))
+    phi[(0, 0)] += onsite
+    fc00 = {(sy, 0): phi[(0, sy)] for sy in (-1, 0, 1) if (0, sy) in phi}
+    fc01 = {(sy, 0): phi[(1, sy)] for sy in (-1, 0, 1) if (1, sy) in phi}
+    return MaterialKspaceParams(masses=np.full(4, float(mass)), fc00_terms=fc00,
+                                fc01_terms=fc01)
+
+
+def sheet_omega_max(params, nk: int = 24) -> float:
+    """Top of the sheet's bulk band: max over a (kx, ky) grid of sqrt(eig D)."""
+    from negfpy_amd.models.material_kspace import material_kspace_lead
+
+    build = material_kspace_lead(params).blocks_builder
+    top = 0.0
+    for ky in np.linspace(-np.pi, np.pi, nk, endpoint=False):
+        d00, d01, d10 = build((ky,))
+        for kx in np.linspace(0.0, np.pi, nk):
+            dk = d00 + d01 * np.exp(1j * kx) + d10 * np.exp(-1j * kx)
+            top = max(top, float(np.linalg.eigvalsh(0.5 * (dk + dk.conj().T)).max()))
+    return float(np.sqrt(top))
+
+
+@dataclass
+class SheetProblem:
+    """k-averaged transport through a pristine graphene-like sheet."""
+
+    params: object
+    lead_left: object
+    lead_right: object
+    v_left: object
+    v_right: object
+    device: object
+    kpoints: list
+    omegas: np.ndarray
+    eta: float
+    wmax: float
+
+
+def sheet_problem(seed: int = 0, n_layers: int = 16, n_k: int = 128, n_omega: int = 2048,
+                  eta: float = 1e-6, span: float = 1.1) -> SheetProblem:
+    """The 2-D sheet workload: an n_layers device of ``sheet_params`` cells
+    (``material_kspace_device``) between leads and couplings from
+    ``interface_contacts`` (the left lead's outward coupling is Phi_10, so its
+    surface Green's function faces left; the sheet's cell has no x mirror), all
+    built per k point; ky_j = -pi + 2 pi (j + 1/2) / n_k and
+    omega_i = span * w_max (i + 1/2) / n_omega: n_k * n_omega instances."""
+    from negfpy_amd.modeling.contacts import interface_contacts
+    from negfpy_amd.models.material_kspace import material_kspace_device
+
+    params = sheet_params(seed)
+    wmax = sheet_omega_max(params)
+    kpoints = [(-np.pi + 2.0 * np.pi * (j + 0.5) / n_k,) for j in range(n_k)]
+    omegas = span * wmax * (np.arange(n_omega) + 0.5) / n_omega
+    lead_l, lead_r, v_l, v_r = interface_contacts(params)
+    return SheetProblem(params=params, lead_left=lead_l, lead_right=lead_r, v_left=v_l,
+                        v_right=v_r, device=material_kspace_device(n_layers, params),
+                        kpoints=kpoints, omegas=omegas, eta=eta, wmax=wmax)

@@ -29,7 +29,7 @@ from negfpy_amd.models import (
     square_lattice_device,
     square_lattice_lead,
 )
-from negfpy_amd.workloads import omega_max, random_lead, ribbon_problem
+from negfpy_amd.workloads import omega_max, random_lead, ribbon_problem, sheet_problem
 from oracle import negf_oracle as o
 from parity_util import T_TOL, assert_parity, oracle_t, rel
 
@@ -151,6 +151,30 @@ def test_kspace_lattice_matches_oracle(gpu):
             vals.append(o.transmission_sparse(w, dv.onsite_blocks, dv.coupling_blocks,
                                               (d00, d01, None), (d00, d01, None), eta=1e-7))
         assert rel(tk, np.mean(vals)) < T_TOL
+
+
+def test_sheet_kspace_spectrum_matches_oracle(gpu):
+    """Graphene-like sheet between interface-contact leads: the k-averaged spectrum
+    (blocks of every k built on the GPU, all (omega, k) in one batch) against the
+    oracle's mean over k, and the pristine sheet's integer channel counts."""
+    p = sheet_problem(n_layers=16, n_k=8, n_omega=24)
+    w = p.omegas[1::3]
+    t = transmission_spectrum(w, p.device, p.lead_left, p.lead_right, kpoints=p.kpoints,
+                              eta=p.eta, device_to_lead_left=p.v_left,
+                              device_to_lead_right=p.v_right)
+    refs = []
+    for wi in w:
+        per_k = []
+        for k in p.kpoints:
+            dv = p.device.device_builder(k)
+            per_k.append(oracle_t(wi, dv.onsite_blocks, dv.coupling_blocks,
+                                  p.lead_left.blocks_builder(k), p.lead_right.blocks_builder(k),
+                                  eta=p.eta, v_left=p.v_left(k), v_right=p.v_right(k)))
+        refs.append((float(np.mean([a for a, _ in per_k])), max(b for _, b in per_k)))
+    assert_parity(t, refs)
+    # every (omega, k) of a pristine sheet carries a whole number of channels
+    # (within the eta broadening of slow channels near band edges, up to ~1e-2 here)
+    assert np.all(np.abs(t * len(p.kpoints) - np.round(t * len(p.kpoints))) < 5e-2)
 
 
 def test_identity_blocks_channel_count(gpu):

@@ -12,7 +12,10 @@ sign or transpose, wrong branch, wrong integration weight) fails it:
   division of default couplings (negf.py:177-204);
 * thermal conductance against the reference's own study files T(w) -> k(T)
   and against the quantum of thermal conductance;
-* the heat current against its closed form for a unit step T(w).
+* the heat current against its closed form for a unit step T(w);
+* T of the pristine graphene-like sheet against its channel count, counted from
+  the bulk bands (and the reference's default contacts, which reuse the right-facing
+  surface Green's function for the left lead, missing it on that mirror-less cell).
 """
 
 import os
@@ -21,7 +24,7 @@ import numpy as np
 import pytest
 
 from negfpy_amd.core.types import LeadBlocks
-from negfpy_amd.workloads import omega_max, random_lead, ribbon_problem
+from negfpy_amd.workloads import omega_max, random_lead, ribbon_problem, sheet_problem
 from oracle import negf_oracle as o
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "study_kappa")
@@ -185,3 +188,37 @@ def test_ribbon_workload_oracle_consistency():
                                    eta=p.eta)
         tr = o.transmission_rgf(w, p.device.onsite_blocks, p.device.coupling_blocks, lead, eta=p.eta)
         assert abs(ts - tr) <= 1e-9 * max(abs(ts), 1e-3)
+
+
+def _channels(lead_blocks, w, nkx=4001):
+    """Propagating channels at w: crossings of the bulk bands w_b(kx) over one period / 2."""
+    d00, d01, d10 = lead_blocks
+    kx = np.linspace(-np.pi, np.pi, nkx)
+    bands = np.array([np.sqrt(np.abs(np.linalg.eigvalsh(d00 + d01 * np.exp(1j * k)
+                                                         + d10 * np.exp(-1j * k))))
+                      for k in kx])
+    s = np.sign(bands - w)
+    return int(np.sum(s[1:] != s[:-1])) // 2
+
+
+def test_sheet_workload_channel_count():
+    """Pristine sheet: T = number of channels; acoustic sum rule and three zero modes."""
+    p = sheet_problem(n_layers=4, n_k=4, n_omega=8)
+    d00, d01, d10 = p.lead_right.blocks_builder((0.0,))
+    assert np.abs(d00 - d00.conj().T).max() < 1e-12
+    ev = np.linalg.eigvalsh(d00 + d01 + d10)
+    assert np.all(np.abs(ev[:3]) < 1e-12) and ev[3] > 1e-3 and ev.min() > -1e-12
+    for kp in [(0.0,), p.kpoints[1], p.kpoints[2]]:
+        ll = p.lead_left.blocks_builder(kp)
+        lr = p.lead_right.blocks_builder(kp)
+        dv = p.device.device_builder(kp)
+        for w in (0.05, 0.37 * p.wmax, 0.7 * p.wmax):
+            t = o.transmission_sparse(w, dv.onsite_blocks, dv.coupling_blocks, ll, lr, eta=1e-6,
+                                      v_left=p.v_left(kp), v_right=p.v_right(kp))
+            assert abs(t - _channels(lr, w)) < 1e-3, (kp, w, t)
+    # the default contacts (V_L = d10, both leads through the same right-facing
+    # surface GF, negf.py:192-201 and :72-97) do not give the channel count here
+    mk = p.device.device_builder((0.0,))
+    lead = p.lead_right.blocks_builder((0.0,))
+    t_def = o.transmission_sparse(0.05, mk.onsite_blocks, mk.coupling_blocks, lead, lead, eta=1e-6)
+    assert abs(t_def - 3.0) > 0.5
