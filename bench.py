@@ -263,6 +263,54 @@ def sheet_kavg(steps: int, warmup: int) -> dict:
                                  "mixed": res["mixed"]["failed"]}}
 
 
+def kappa_gradient_3d() -> dict:
+    """The 3-D gradient workload at full size: 48-DOF simple-cubic cells, 64 layers,
+    28 | 73 amu leads, the last 4 layers perturbed, a 16 x 16 (ky, kz) mesh x 1024
+    omega = 262,144 instances.  kappa(300 K) and d kappa / d(every device
+    force-constant term, 64 x 9 same-layer and 63 x 9 next-layer 48 x 48 blocks)
+    through the k-point blocks (torch), transmission_sets_torch and
+    thermal_conductance_torch; the forward and the backward pass timed once each."""
+    import torch
+
+    from negfpy_amd.core import thermal_conductance_torch, transmission_sets_torch
+    from negfpy_amd.workloads import OMEGA_UNIT_RAD_S, interface3d_blocks, interface3d_problem
+
+    p = interface3d_problem()
+    nk, nw = len(p.kpoints), p.omegas.size
+    n_inst = nk * nw
+    k_of = torch.arange(nk, dtype=torch.int32, device="cuda").repeat(nw)
+    om = torch.tensor(np.repeat(p.omegas, nk), dtype=torch.float64, device="cuda")
+    w_rad = p.omegas * OMEGA_UNIT_RAD_S
+    up = [torch.as_tensor(x, device="cuda") for x in (p.lead_left, p.lead_right, p.v_left,
+                                                      p.v_right)]
+    f00 = torch.tensor(p.fc00, dtype=torch.float64, device="cuda", requires_grad=True)
+    f01 = torch.tensor(p.fc01, dtype=torch.float64, device="cuda", requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ons, cpl = interface3d_blocks(p, f00, f01)
+    t = transmission_sets_torch(om, k_of, ons, cpl, *up, eta=p.eta)
+    kap = thermal_conductance_torch(w_rad, t.view(nw, nk).mean(dim=1), [300.0])[0]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kap.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    g00, g01 = f00.grad, f01.grad
+    n = int(p.fc00.shape[-1])
+    return {"workload": f"3-D mass-mismatch interface: n={n} DOF/layer (4x2x2 simple-cubic "
+                        f"cell), {p.fc00.shape[0]} layers, 28 | 73 amu, last 4 layers "
+                        f"perturbed +-20%, {int(round(nk ** 0.5))}x{int(round(nk ** 0.5))} k "
+                        f"x {nw} omega, eta={p.eta:g}, FP64",
+            "instances": n_inst,
+            "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
+            "forward_solves_per_s": round(n_inst / (t1 - t0), 1),
+            "adjoint_solves_per_s": round(n_inst / (t2 - t1), 1),
+            "kappa_300K_W_per_K": float(kap.detach()),
+            "gradient_terms": int(g00.numel() + g01.numel()),
+            "grad_norm_perturbed_layers": float(g00[-4:].norm()),
+            "grad_norm_all": float((g00.norm() ** 2 + g01.norm() ** 2).sqrt())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,6 +318,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-gradient", action="store_true",
+                    help="skip the full-size d kappa / d(force constants) run (about 30 s)")
     ap.add_argument("--paths", default="fp64,mixed",
                     help="comma list of precision paths to run (profiling one path alone)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -427,6 +477,9 @@ def main():
     print(f"adjoint batch: {line['adjoint_batch']}", file=sys.stderr, flush=True)
     line["sheet_kavg"] = sheet_kavg(max(1, min(args.steps, 3)), 1)
     print(f"sheet k-average: {line['sheet_kavg']}", file=sys.stderr, flush=True)
+    if not args.no_gradient:
+        line["kappa_gradient_3d"] = kappa_gradient_3d()
+        print(f"kappa gradient 3-D: {line['kappa_gradient_3d']}", file=sys.stderr, flush=True)
     if cpu is not None:
         sel = cpu["sel"]
         err64 = np.abs(t64[sel] - cpu["vals"]) / np.maximum(np.abs(cpu["vals"]), 1e-3)

@@ -1,7 +1,7 @@
 """GPU phonon-NEGF transport core with the API of the reference ``negfpy.core``
 (``src/negfpy/core/__init__.py:1-78``, transport part)."""
 
-from .adjoint import thermal_conductance_torch, transmission_torch
+from .adjoint import thermal_conductance_torch, transmission_sets_torch, transmission_torch
 from .fcs import (
     HeatCurrentCumulants,
     HeatCurrentUncertainty,
@@ -58,6 +58,7 @@ __all__ = [
     "heat_current_uncertainty",
     "transmission_k_moments_spectrum",
     "transmission_torch",
+    "transmission_sets_torch",
     "thermal_conductance_torch",
     "Device1D",
     "DeviceKSpace",

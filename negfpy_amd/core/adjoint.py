@@ -27,7 +27,7 @@ from ._status import transmission_value
 from .negf import _max_block_green, resolve_k
 from .types import Device1D
 
-__all__ = ["transmission_torch", "thermal_conductance_torch"]
+__all__ = ["transmission_torch", "transmission_sets_torch", "thermal_conductance_torch"]
 
 # Device memory one backward launch may use for adjoint workspace and per-instance
 # gradients; larger omega batches are processed in chunks of this size.
@@ -155,6 +155,120 @@ def transmission_torch(omega, onsite, coupling, lead_left, lead_right, *, eta: f
         ons, cpl = _merge_layers_torch(ons, cpl, m)
     eta_dev = float(eta if eta_device is None else eta_device)
     return _TransmissionFn.apply(ons, cpl, _Static(kp, w, eta, eta_dev))
+
+
+def _chunk_instances(per_inst: int, n_inst: int, dev) -> int:
+    """Instances per backward launch: the per-instance adjoint memory within
+    ADJOINT_CHUNK_BYTES, or within a quarter of the free device memory if that is more."""
+    budget = ADJOINT_CHUNK_BYTES
+    if dev.type == "cuda":
+        budget = max(budget, torch.cuda.mem_get_info(dev)[0] // 4)
+    return max(1, min(n_inst, budget // per_inst))
+
+
+class _SetsTransmissionFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, ons, cpl, static):
+        sets = static.sets(ons, cpl)
+        res = _engine.TransmissionSolver(sets, precision="fp64").run(
+            static.set_of, static.omega, static.eta, static.eta_dev)
+        if bool((res.status != 0).any()):
+            st = res.status.cpu().numpy()
+            tc = res.t.cpu().numpy()
+            for c, v in zip(st, tc):
+                transmission_value(int(c), float(v))  # raises the reference's exception
+        ctx.static = static
+        ctx.save_for_backward(ons, cpl)
+        return res.t
+
+    @staticmethod
+    def backward(ctx, grad_t):
+        ons, cpl = ctx.saved_tensors
+        static = ctx.static
+        sets = static.sets(ons, cpl)
+        weight = grad_t.detach().to(torch.float64).contiguous()
+        nl, n = int(ons.shape[1]), int(ons.shape[2])
+        n_inst = int(static.omega.numel())
+        per_inst = 2 * (2 * nl - 1) * n * n * 16
+        chunk = _chunk_instances(per_inst, n_inst, ons.device)
+        g_ons = torch.zeros_like(ons)
+        g_cpl = torch.zeros((ons.shape[0], max(nl - 1, 1), n, n), dtype=ons.dtype,
+                            device=ons.device)
+        for a in range(0, n_inst, chunk):
+            b = min(n_inst, a + chunk)
+            k = static.set_of[a:b].long()
+            res = _engine.adjoint(sets, static.set_of[a:b], static.omega[a:b], static.eta[a:b],
+                                  static.eta_dev[a:b], weight=weight[a:b])
+            # per-set sums of the per-instance gradients (real view: float64 atomics)
+            torch.view_as_real(g_ons).index_add_(0, k, torch.view_as_real(res.grad_onsite))
+            torch.view_as_real(g_cpl).index_add_(0, k, torch.view_as_real(res.grad_coupling))
+            del res
+        return g_ons, (g_cpl[:, :nl - 1] if nl > 1 else None), None
+
+
+class _SetsStatic:
+    """Leads, couplings and instances of a k-resolved transmission problem."""
+
+    def __init__(self, lead_l, lead_r, v_l, v_r, set_of, omega, eta, eta_dev):
+        dev = _engine.device()
+        c128 = torch.complex128
+        self.lead_l = torch.as_tensor(lead_l).to(device=dev, dtype=c128)
+        self.lead_r = torch.as_tensor(lead_r).to(device=dev, dtype=c128)
+        self.v_l = torch.as_tensor(v_l).to(device=dev, dtype=c128)
+        self.v_r = torch.as_tensor(v_r).to(device=dev, dtype=c128)
+        self.share = bool(torch.equal(self.lead_l, self.lead_r))
+        self.set_of = _engine.to_dev_i32(set_of)
+        self.omega = _engine.to_dev_f64(omega)
+        m = int(self.omega.numel())
+        self.eta = _engine.to_dev_f64(np.broadcast_to(np.asarray(eta, dtype=float), (m,)).copy())
+        self.eta_dev = _engine.to_dev_f64(
+            np.broadcast_to(np.asarray(eta_dev, dtype=float), (m,)).copy())
+
+    def sets(self, ons: torch.Tensor, cpl: torch.Tensor) -> _engine.TransportSets:
+        n = int(ons.shape[2])
+        idx = np.arange(n)
+        return _engine.TransportSets.from_arrays(
+            self.lead_l, self.lead_r, self.v_l, self.v_r, idx, idx, ons.detach(),
+            cpl.detach() if ons.shape[1] > 1 else None, self.share)
+
+
+def transmission_sets_torch(omega, set_of, onsite, coupling, lead_left, lead_right, v_left,
+                            v_right, *, eta: float = 1e-8,
+                            eta_device: float | None = None) -> torch.Tensor:
+    """T of every instance i = (omega[i], set set_of[i]) as a differentiable GPU tensor,
+    for device blocks given per set (e.g. per k point): ``onsite`` (S, L, n, n) and
+    ``coupling`` (S, L-1, n, n) torch tensors, gradients flow to both.
+
+    The leads ``lead_left`` / ``lead_right`` are (S, 3, n, n) blocks (d00, d01, d10)
+    per set, each passed to the decimation as given (``surface_gf.py:18-81``), and
+    ``v_left`` / ``v_right`` (S, n, n) the device-to-lead couplings of the first and
+    last layer (all of its DOFs are contacts); Sigma = V g V^H as in
+    ``negf.py:72-97``.  They are held fixed.  The batched form of ``transmission``
+    (``negf.py:306-357``) over the points of a k mesh, for gradients with respect to
+    force constants that enter every k point's blocks.
+    """
+    dev = _engine.device()
+    ons = onsite.to(device=dev, dtype=torch.complex128)
+    cpl = coupling.to(device=dev, dtype=torch.complex128)
+    if ons.ndim != 4 or ons.shape[2] != ons.shape[3]:
+        raise ValueError("onsite must have shape (n_sets, n_layers, n, n).")
+    s, nl, n = int(ons.shape[0]), int(ons.shape[1]), int(ons.shape[2])
+    if cpl.ndim != 4 or tuple(cpl.shape) != (s, nl - 1, n, n):
+        raise ValueError("coupling must have shape (n_sets, n_layers - 1, n, n).")
+    eta_dev = float(eta if eta_device is None else eta_device)
+    static = _SetsStatic(lead_left, lead_right, v_left, v_right, set_of, omega, eta, eta_dev)
+    for name, t in (("lead_left", static.lead_l), ("lead_right", static.lead_r)):
+        if tuple(t.shape) != (s, 3, n, n):
+            raise ValueError(f"{name} must have shape (n_sets, 3, n, n).")
+    for name, t in (("v_left", static.v_l), ("v_right", static.v_r)):
+        if tuple(t.shape) != (s, n, n):
+            raise ValueError(f"{name} must have shape (n_sets, n, n).")
+    k = static.set_of
+    if k.numel() != static.omega.numel():
+        raise ValueError("set_of needs one entry per omega.")
+    if k.numel() and (int(k.min()) < 0 or int(k.max()) >= s):
+        raise ValueError("set_of entries must index the sets.")
+    return _SetsTransmissionFn.apply(ons, cpl, static)
 
 
 class _ConductanceFn(torch.autograd.Function):

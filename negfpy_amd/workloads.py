@@ -281,3 +281,141 @@ def sheet_problem(seed: int = 0, n_layers: int = 16, n_k: int = 128, n_omega: in
     return SheetProblem(params=params, lead_left=lead_l, lead_right=lead_r, v_left=v_l,
                         v_right=v_r, device=material_kspace_device(n_layers, params),
                         kpoints=kpoints, omegas=omegas, eta=eta, wmax=wmax)
+
+
+# (radial, tangential) springs in eV/A^2 of the first and second neighbour shells of
+# the simple-cubic cell of the 3-D interface workload
+CUBIC_SPRINGS = ((10.0, 2.0), (3.0, 0.5))
+
+
+def _cubic_phi(cell=(4, 2, 2)):
+    """Force constants Phi(dx, dy, dz) (3 N x 3 N, N = prod(cell)) between a simple-cubic
+    cell of nx x ny x nz atoms (unit spacing; transport along x) and its image shifted
+    by whole cells; first and second neighbour springs K = kr e e^T + kt (I - e e^T),
+    Phi_ab = -K off the diagonal and the acoustic sum rule on it."""
+    nx, ny, nz = cell
+    pos = np.array([(i, j, k) for i in range(nx) for j in range(ny) for k in range(nz)], float)
+    na = len(pos)
+    size = np.array(cell, float)
+    phi: dict = {}
+    onsite = np.zeros((3 * na, 3 * na))
+    for a in range(na):
+        for b in range(na):
+            for s in np.ndindex(3, 3, 3):
+                sh = np.array(s) - 1
+                r = pos[b] + sh * size - pos[a]
+                d = float(np.linalg.norm(r))
+                shell = 0 if abs(d - 1.0) < 1e-9 else 1 if abs(d - np.sqrt(2.0)) < 1e-9 else -1
+                if shell < 0:
+                    continue
+                kr, kt = CUBIC_SPRINGS[shell]
+                e = r / d
+                k = kr * np.outer(e, e) + kt * (np.eye(3) - np.outer(e, e))
+                blk = phi.setdefault(tuple(sh), np.zeros((3 * na, 3 * na)))
+                blk[3 * a:3 * a + 3, 3 * b:3 * b + 3] -= k
+                onsite[3 * a:3 * a + 3, 3 * a:3 * a + 3] += k
+    phi.setdefault((0, 0, 0), np.zeros((3 * na, 3 * na)))
+    phi[(0, 0, 0)] += onsite
+    return phi
+
+
+@dataclass
+class Interface3DProblem:
+    """Mass-mismatch interface in a 3-D crystal, k-resolved, with the device force
+    constants as per-layer terms (gradients of kappa flow to them)."""
+
+    terms: list          # transverse shifts (dy, dz) of the force-constant terms
+    fc00: np.ndarray     # (L, n_terms, n, n) same-layer force constants of every layer
+    fc01: np.ndarray     # (L - 1, n_terms, n, n) layer l -> l + 1
+    masses: np.ndarray   # (L, n) per-DOF masses of every device layer
+    lead_left: np.ndarray    # (S, 3, n, n) left lead, facing left: (d00, d10, d01)
+    lead_right: np.ndarray   # (S, 3, n, n)
+    v_left: np.ndarray   # (S, n, n) device layer 0 -> left lead
+    v_right: np.ndarray  # (S, n, n) last device layer -> right lead
+    phases: np.ndarray   # (S, n_terms) e^{i (ky dy + kz dz)}
+    kpoints: list
+    omegas: np.ndarray
+    eta: float
+
+
+def interface3d_problem(seed: int = 0, cell=(4, 2, 2), n_layers: int = 64, nk: int = 16,
+                        n_omega: int = 1024, m_left: float = 28.0, m_right: float = 73.0,
+                        perturb: float = 0.2, n_perturbed: int = 4,
+                        eta: float = 1e-6) -> Interface3DProblem:
+    """The 3-D gradient workload: simple-cubic cells of prod(cell) atoms (48 DOF for
+    4 x 2 x 2) with first and second neighbour springs, n_layers device layers, mass
+    m_left in the left lead and the left half of the device, m_right in the right half
+    and the right lead; the same-layer force constants of the last n_perturbed layers
+    get random symmetric relative perturbations of up to +-perturb (Phi00(-d) =
+    Phi00(d)^T kept).  An nk x nk (ky, kz) mesh, ky_j = -pi + 2 pi (j + 1/2) / nk, and
+    omega_i = 1.1 w_max (i + 1/2) / n_omega, w_max the band top of the light side.
+    Leads as the interface contacts of a bulk model (the left lead's outward coupling
+    is Phi_10)."""
+    phi = _cubic_phi(cell)
+    n = next(iter(phi.values())).shape[0]
+    terms = sorted({(dy, dz) for (_, dy, dz) in phi})
+    zero = np.zeros((n, n))
+    f00 = np.stack([phi.get((0,) + t, zero) for t in terms])
+    f01 = np.stack([phi.get((1,) + t, zero) for t in terms])
+    rng = np.random.default_rng(seed)
+    fc00 = np.repeat(f00[None], n_layers, axis=0)
+    index = {t: i for i, t in enumerate(terms)}
+    for layer in range(n_layers - n_perturbed, n_layers):
+        done = set()
+        for t in terms:
+            if t in done:
+                continue
+            mt = (-t[0], -t[1])
+            r = rng.uniform(-1.0, 1.0, (n, n))
+            if mt == t:
+                r = 0.5 * (r + r.T)
+            fc00[layer, index[t]] = f00[index[t]] * (1.0 + perturb * r)
+            fc00[layer, index[mt]] = fc00[layer, index[t]].T if mt != t else fc00[layer, index[t]]
+            done.update({t, mt})
+    fc01 = np.repeat(f01[None], n_layers - 1, axis=0)
+    dof_mass = np.where(np.arange(n_layers)[:, None] < n_layers // 2, m_left, m_right)
+    masses = dof_mass * np.ones((1, n))
+    kpoints = [(-np.pi + 2 * np.pi * (a + 0.5) / nk, -np.pi + 2 * np.pi * (b + 0.5) / nk)
+               for a in range(nk) for b in range(nk)]
+    ph = np.array([[np.exp(1j * (ky * dy + kz * dz)) for (dy, dz) in terms]
+                   for (ky, kz) in kpoints])
+    p00 = np.einsum("st,tij->sij", ph, f00)
+    p01 = np.einsum("st,tij->sij", ph, f01)
+    sl, sr = 1.0 / np.sqrt(m_left), 1.0 / np.sqrt(m_right)
+    d01_l, d01_r = sl * sl * p01, sr * sr * p01
+    d10_l = np.conj(np.swapaxes(d01_l, 1, 2))
+    d10_r = np.conj(np.swapaxes(d01_r, 1, 2))
+    lead_left = np.stack([sl * sl * p00, d10_l, d01_l], axis=1)
+    lead_right = np.stack([sr * sr * p00, d01_r, d10_r], axis=1)
+    # V_L = D(layer 0, lead layer -1) = M_0^-1/2 Phi_10 M_L^-1/2; V_R = M_{L-1}^-1/2 Phi_01 M_R^-1/2
+    v_left = np.conj(np.swapaxes(p01, 1, 2)) * (sl / np.sqrt(masses[0, 0]))
+    v_right = p01 * (sr / np.sqrt(masses[-1, 0]))
+    # band top of the light side, over a coarse (kx, k) sample
+    top = 0.0
+    for s in range(0, len(kpoints), max(1, len(kpoints) // 16)):
+        for kx in np.linspace(0.0, np.pi, 9):
+            dk = lead_left[s, 0] + d01_l[s] * np.exp(1j * kx) + d10_l[s] * np.exp(-1j * kx)
+            top = max(top, float(np.linalg.eigvalsh(0.5 * (dk + dk.conj().T)).max()))
+    omegas = 1.1 * np.sqrt(top) * (np.arange(n_omega) + 0.5) / n_omega
+    return Interface3DProblem(terms=terms, fc00=fc00, fc01=fc01, masses=masses,
+                              lead_left=lead_left, lead_right=lead_right, v_left=v_left,
+                              v_right=v_right, phases=ph, kpoints=kpoints, omegas=omegas,
+                              eta=eta)
+
+
+def interface3d_blocks(p: Interface3DProblem, fc00, fc01):
+    """Device blocks of every k point from the layer force constants ``fc00`` (L, T, n, n)
+    and ``fc01`` (L - 1, T, n, n) as a differentiable torch expression:
+    D_l(k) = M_l^-1/2 sum_t Phi_l(t) e^{i k.t} M_l^-1/2 and the same between layers
+    l and l + 1.  Returns (onsite (S, L, n, n), coupling (S, L - 1, n, n))."""
+    import torch
+
+    dev = fc00.device
+    c128 = torch.complex128
+    ph = torch.as_tensor(p.phases, dtype=c128, device=dev)
+    inv = torch.as_tensor(1.0 / np.sqrt(p.masses), dtype=torch.float64, device=dev)
+    d00 = torch.einsum("st,ltij->slij", ph, fc00.to(c128))
+    d01 = torch.einsum("st,ltij->slij", ph, fc01.to(c128))
+    ons = inv[None, :, :, None] * d00 * inv[None, :, None, :]
+    cpl = inv[None, :-1, :, None] * d01 * inv[None, 1:, None, :]
+    return ons, cpl
