@@ -1,7 +1,12 @@
 """GPU phonon-NEGF transport core with the API of the reference ``negfpy.core``
 (``src/negfpy/core/__init__.py:1-78``, transport part)."""
 
-from .adjoint import thermal_conductance_torch, transmission_sets_torch, transmission_torch
+from .adjoint import (
+    thermal_conductance_torch,
+    transmission_sets_torch,
+    transmission_sum_torch,
+    transmission_torch,
+)
 from .fcs import (
     HeatCurrentCumulants,
     HeatCurrentUncertainty,
@@ -59,6 +64,7 @@ __all__ = [
     "transmission_k_moments_spectrum",
     "transmission_torch",
     "transmission_sets_torch",
+    "transmission_sum_torch",
     "thermal_conductance_torch",
     "Device1D",
     "DeviceKSpace",

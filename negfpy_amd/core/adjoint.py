@@ -27,7 +27,8 @@ from ._status import transmission_value
 from .negf import _max_block_green, resolve_k
 from .types import Device1D
 
-__all__ = ["transmission_torch", "transmission_sets_torch", "thermal_conductance_torch"]
+__all__ = ["transmission_torch", "transmission_sets_torch", "transmission_sum_torch",
+           "thermal_conductance_torch"]
 
 # Device memory one backward launch may use for adjoint workspace and per-instance
 # gradients; larger omega batches are processed in chunks of this size.
@@ -269,6 +270,79 @@ def transmission_sets_torch(omega, set_of, onsite, coupling, lead_left, lead_rig
     if k.numel() and (int(k.min()) < 0 or int(k.max()) >= s):
         raise ValueError("set_of entries must index the sets.")
     return _SetsTransmissionFn.apply(ons, cpl, static)
+
+
+class _SetsSumFn(torch.autograd.Function):
+    """sum_i c_i T_i with its gradient from one adjoint pass (the weights c_i are known
+    before T is): the forward runs the adjoint kernel and keeps the per-set gradient."""
+
+    @staticmethod
+    def forward(ctx, ons, cpl, static, coef):
+        sets = static.sets(ons, cpl)
+        nl, n = int(ons.shape[1]), int(ons.shape[2])
+        n_inst = int(static.omega.numel())
+        per_inst = 2 * (2 * nl - 1) * n * n * 16
+        chunk = _chunk_instances(per_inst, n_inst, ons.device)
+        g_ons = torch.zeros_like(ons)
+        g_cpl = torch.zeros((ons.shape[0], max(nl - 1, 1), n, n), dtype=ons.dtype,
+                            device=ons.device)
+        t_all = torch.empty((n_inst,), dtype=torch.float64, device=ons.device)
+        st_all = torch.empty((n_inst,), dtype=torch.int32, device=ons.device)
+        for a in range(0, n_inst, chunk):
+            b = min(n_inst, a + chunk)
+            k = static.set_of[a:b].long()
+            res = _engine.adjoint(sets, static.set_of[a:b], static.omega[a:b], static.eta[a:b],
+                                  static.eta_dev[a:b], weight=coef[a:b])
+            t_all[a:b] = res.t
+            st_all[a:b] = res.status
+            torch.view_as_real(g_ons).index_add_(0, k, torch.view_as_real(res.grad_onsite))
+            torch.view_as_real(g_cpl).index_add_(0, k, torch.view_as_real(res.grad_coupling))
+            del res
+        if bool((st_all != 0).any()):
+            for c, v in zip(st_all.cpu().numpy(), t_all.cpu().numpy()):
+                transmission_value(int(c), float(v))  # raises the reference's exception
+        ctx.save_for_backward(g_ons, g_cpl)
+        ctx.nl = nl
+        return (coef * t_all).sum()
+
+    @staticmethod
+    def backward(ctx, grad):
+        g_ons, g_cpl = ctx.saved_tensors
+        nl = ctx.nl
+        g = grad.to(g_ons.dtype)
+        return g * g_ons, (g * g_cpl[:, :nl - 1] if nl > 1 else None), None, None
+
+
+def transmission_sum_torch(coef, omega, set_of, onsite, coupling, lead_left, lead_right, v_left,
+                           v_right, *, eta: float = 1e-8,
+                           eta_device: float | None = None) -> torch.Tensor:
+    """sum_i coef[i] T_i over the instances of ``transmission_sets_torch`` (same
+    arguments), differentiable in ``onsite`` and ``coupling`` but not in ``coef``.
+
+    Because coef is known before T, one adjoint pass gives the sum and its gradient
+    (the adjoint kernel weighted by coef); there is no separate forward solve.  For
+    kappa(T) of a k-averaged spectrum, coef[i] = (conductance weight of omega_i) / n_k
+    (``_engine.conductance_weights``).
+    """
+    dev = _engine.device()
+    ons = onsite.to(device=dev, dtype=torch.complex128)
+    cpl = coupling.to(device=dev, dtype=torch.complex128)
+    if ons.ndim != 4 or ons.shape[2] != ons.shape[3]:
+        raise ValueError("onsite must have shape (n_sets, n_layers, n, n).")
+    s, nl, n = int(ons.shape[0]), int(ons.shape[1]), int(ons.shape[2])
+    if cpl.ndim != 4 or tuple(cpl.shape) != (s, nl - 1, n, n):
+        raise ValueError("coupling must have shape (n_sets, n_layers - 1, n, n).")
+    eta_dev = float(eta if eta_device is None else eta_device)
+    static = _SetsStatic(lead_left, lead_right, v_left, v_right, set_of, omega, eta, eta_dev)
+    c = torch.as_tensor(coef, dtype=torch.float64, device=dev).detach().contiguous()
+    if c.shape != static.omega.shape:
+        raise ValueError("coef needs one entry per instance.")
+    k = static.set_of
+    if k.numel() != static.omega.numel():
+        raise ValueError("set_of needs one entry per omega.")
+    if k.numel() and (int(k.min()) < 0 or int(k.max()) >= s):
+        raise ValueError("set_of entries must index the sets.")
+    return _SetsSumFn.apply(ons, cpl, static, c)
 
 
 class _ConductanceFn(torch.autograd.Function):

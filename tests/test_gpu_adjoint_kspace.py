@@ -17,7 +17,8 @@ import pytest
 import torch
 
 from adjoint_util import dense_grads
-from negfpy_amd.core import thermal_conductance_torch, transmission_sets_torch
+from negfpy_amd.core import (_engine, thermal_conductance_torch, transmission_sets_torch,
+                             transmission_sum_torch)
 from negfpy_amd.workloads import OMEGA_UNIT_RAD_S, interface3d_blocks, interface3d_problem
 from parity_util import assert_parity, oracle_t
 from test_gpu_adjoint import richardson
@@ -141,6 +142,33 @@ def test_kappa_gradient_matches_finite_differences(gpu):
         print(f"d kappa: finite difference {fd:.10e}, adjoint {an:.10e}, "
               f"rel {abs(fd - an) / abs(an):.2e}", flush=True)
         assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+
+
+def test_one_pass_kappa_equals_autograd_path(gpu):
+    """kappa as sum_i c_i T_i with c_i = conductance weight / n_k: one adjoint pass
+    gives the value and the gradient of the forward + backward path."""
+    p = _small()
+    f00, f01 = _leaves(p)
+    k = _kappa(p, f00, f01)
+    k.backward()
+    g00, g01 = f00.grad.clone(), f01.grad.clone()
+    f00.grad = f01.grad = None
+    nk = len(p.kpoints)
+    wts = _engine.conductance_weights(
+        torch.as_tensor(p.omegas * OMEGA_UNIT_RAD_S, device="cuda"),
+        torch.tensor([300.0], dtype=torch.float64, device="cuda"))[0]
+    coef = wts.repeat_interleave(nk) / nk
+    ons, cpl = interface3d_blocks(p, f00, f01)
+    k_of, om = _instances(p)
+    k1 = transmission_sum_torch(coef, om, k_of, ons, cpl, p.lead_left, p.lead_right, p.v_left,
+                                p.v_right, eta=p.eta)
+    k1.backward()
+    assert abs(float(k1.detach()) - float(k.detach())) <= 1e-10 * abs(float(k.detach()))
+    for a, b in ((f00.grad, g00), (f01.grad, g01)):
+        assert float((a - b).abs().max()) <= 1e-10 * float(b.abs().max())
+    with pytest.raises(ValueError):
+        transmission_sum_torch(coef[:-1], om, k_of, ons, cpl, p.lead_left, p.lead_right,
+                               p.v_left, p.v_right)
 
 
 def test_argument_errors(gpu):
