@@ -169,6 +169,18 @@ def test_one_pass_kappa_equals_autograd_path(gpu):
     with pytest.raises(ValueError):
         transmission_sum_torch(coef[:-1], om, k_of, ons, cpl, p.lead_left, p.lead_right,
                                p.v_left, p.v_right)
+    # instances in another order (not omega-major with the k points repeated): the
+    # per-set sums take the index_add_ route and give the same gradient
+    perm = np.random.default_rng(1).permutation(om.size)
+    f00.grad = f01.grad = None
+    ons, cpl = interface3d_blocks(p, f00, f01)
+    k2 = transmission_sum_torch(coef[torch.as_tensor(perm, device="cuda")], om[perm], k_of[perm],
+                                ons, cpl, p.lead_left, p.lead_right, p.v_left, p.v_right,
+                                eta=p.eta)
+    k2.backward()
+    assert abs(float(k2.detach()) - float(k.detach())) <= 1e-10 * abs(float(k.detach()))
+    for a, b in ((f00.grad, g00), (f01.grad, g01)):
+        assert float((a - b).abs().max()) <= 1e-10 * float(b.abs().max())
 
 
 def test_argument_errors(gpu):
