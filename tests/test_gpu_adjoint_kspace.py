@@ -169,8 +169,8 @@ def test_one_pass_kappa_equals_autograd_path(gpu):
     with pytest.raises(ValueError):
         transmission_sum_torch(coef[:-1], om, k_of, ons, cpl, p.lead_left, p.lead_right,
                                p.v_left, p.v_right)
-    # instances in another order (not omega-major with the k points repeated): the
-    # per-set sums take the index_add_ route and give the same gradient
+    # instances in another order (not omega-major with the k points repeated) give the
+    # same value and gradient
     perm = np.random.default_rng(1).permutation(om.size)
     f00.grad = f01.grad = None
     ons, cpl = interface3d_blocks(p, f00, f01)
