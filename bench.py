@@ -145,6 +145,66 @@ def uneven_batch(steps: int, warmup: int) -> dict:
             "identical_outputs": bool(same)}
 
 
+def uneven_batch_3d(steps: int, warmup: int) -> dict:
+    """The 3-D uneven batch (workloads.uneven3d_problems): 24-DOF cells on a 32 x 32
+    k mesh (1,048,576 instances), 48-DOF on 8 x 8 and 96-DOF on 4 x 4, 1024 omega
+    each with a quarter within 1 % of the band top, 8 layers; k-point blocks built
+    on the GPU.  Solved by the staged launches problem after problem and by one
+    persistent launch."""
+    import torch
+
+    from negfpy_amd.core import _engine
+    from negfpy_amd.core.negf import kspace_sets
+    from negfpy_amd.workloads import uneven3d_problems
+
+    jobs = []
+    for p in uneven3d_problems():
+        sets = kspace_sets(p.device, p.lead, p.lead, p.kpoints)
+        nk, nw = len(p.kpoints), p.omegas.size
+        m = nk * nw
+        jobs.append((_engine.TransmissionSolver(sets),
+                     torch.arange(nk, dtype=torch.int32, device="cuda").repeat(nw),
+                     torch.tensor(np.repeat(p.omegas, nk), dtype=torch.float64, device="cuda"),
+                     torch.full((m,), p.eta, dtype=torch.float64, device="cuda"), None))
+    sizes = [int(j[2].numel()) for j in jobs]
+    n_inst = sum(sizes)
+    outs_s = [j[0].alloc_result(m) for j, m in zip(jobs, sizes)]
+    outs_p = [j[0].alloc_result(m) for j, m in zip(jobs, sizes)]
+
+    def staged():
+        for j, o in zip(jobs, outs_s):
+            j[0].run(*j[1:], out=o)
+
+    def persistent():
+        _engine.transmission_multi(jobs, outs_p)
+
+    times = {}
+    for name, fn in (("staged", staged), ("persistent", persistent)):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        times[name] = (time.perf_counter() - t0) / steps
+    same = all(torch.equal(a.t.nan_to_num(-1.0), b.t.nan_to_num(-1.0)) and
+               torch.equal(a.iters_l, b.iters_l) and torch.equal(a.status, b.status)
+               for a, b in zip(outs_s, outs_p))
+    iters = torch.cat([o.iters_l for o in outs_p])
+    failed = int(sum(int((o.status != 0).sum()) for o in outs_p))
+    return {"workload": "3-D uneven batch: simple-cubic cells of 24/48/96 DOF on 32x32/8x8/4x4 "
+                        "k meshes, 8 layers, 1024 omega each (a quarter within 1% of the band "
+                        "top), eta=1e-6, FP64",
+            "instances": n_inst, "instances_per_part": sizes,
+            "decimation_steps": [int(iters.min()), int(iters.max())],
+            "failed_instances": failed,
+            "staged_solves_per_s": round(n_inst / times["staged"], 1),
+            "persistent_solves_per_s": round(n_inst / times["persistent"], 1),
+            "persistent_over_staged": round(times["staged"] / times["persistent"], 3),
+            "identical_outputs": bool(same)}
+
+
 def adjoint_batch(steps: int, warmup: int) -> dict:
     """The adjoint workload (48-DOF mass-mismatch interface between two different
     leads, 64 layers): forward transmission solves and adjoint solves (T and
@@ -473,6 +533,8 @@ def main():
     }
     line["uneven_batch"] = uneven_batch(max(1, min(args.steps, 5)), 1)
     print(f"uneven batch: {line['uneven_batch']}", file=sys.stderr, flush=True)
+    line["uneven_batch_3d"] = uneven_batch_3d(max(1, min(args.steps, 2)), 1)
+    print(f"uneven batch 3-D: {line['uneven_batch_3d']}", file=sys.stderr, flush=True)
     line["adjoint_batch"] = adjoint_batch(max(1, min(args.steps, 3)), 1)
     print(f"adjoint batch: {line['adjoint_batch']}", file=sys.stderr, flush=True)
     line["sheet_kavg"] = sheet_kavg(max(1, min(args.steps, 3)), 1)

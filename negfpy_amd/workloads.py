@@ -288,15 +288,19 @@ def sheet_problem(seed: int = 0, n_layers: int = 16, n_k: int = 128, n_omega: in
 CUBIC_SPRINGS = ((10.0, 2.0), (3.0, 0.5))
 
 
-def _cubic_phi(cell=(4, 2, 2)):
+def _cubic_phi(cell=(4, 2, 2), seed: int | None = None, jitter: float = 0.0):
     """Force constants Phi(dx, dy, dz) (3 N x 3 N, N = prod(cell)) between a simple-cubic
     cell of nx x ny x nz atoms (unit spacing; transport along x) and its image shifted
     by whole cells; first and second neighbour springs K = kr e e^T + kt (I - e e^T),
-    Phi_ab = -K off the diagonal and the acoustic sum rule on it."""
+    each bond's springs times a seeded factor in [1 - jitter, 1 + jitter] (the same for
+    the bond seen from either atom), Phi_ab = -K off the diagonal and the acoustic sum
+    rule on it."""
     nx, ny, nz = cell
     pos = np.array([(i, j, k) for i in range(nx) for j in range(ny) for k in range(nz)], float)
     na = len(pos)
     size = np.array(cell, float)
+    rng = np.random.default_rng(seed)
+    factor: dict = {}
     phi: dict = {}
     onsite = np.zeros((3 * na, 3 * na))
     for a in range(na):
@@ -308,7 +312,10 @@ def _cubic_phi(cell=(4, 2, 2)):
                 shell = 0 if abs(d - 1.0) < 1e-9 else 1 if abs(d - np.sqrt(2.0)) < 1e-9 else -1
                 if shell < 0:
                     continue
-                kr, kt = CUBIC_SPRINGS[shell]
+                key, twin = (a, b) + tuple(sh), (b, a) + tuple(-sh)
+                if key not in factor:
+                    factor[key] = factor.get(twin, 1.0 + jitter * rng.uniform(-1.0, 1.0))
+                kr, kt = (factor[key] * x for x in CUBIC_SPRINGS[shell])
                 e = r / d
                 k = kr * np.outer(e, e) + kt * (np.eye(3) - np.outer(e, e))
                 blk = phi.setdefault(tuple(sh), np.zeros((3 * na, 3 * na)))
@@ -419,3 +426,65 @@ def interface3d_blocks(p: Interface3DProblem, fc00, fc01):
     ons = inv[None, :, :, None] * d00 * inv[None, :, None, :]
     cpl = inv[None, :-1, :, None] * d01 * inv[None, 1:, None, :]
     return ons, cpl
+
+
+# (cell, transverse k mesh) of the 3-D uneven batch: 24-, 48- and 96-DOF layers
+UNEVEN3D_PARTS = (((2, 2, 2), 32), ((4, 2, 2), 8), ((4, 4, 2), 4))
+
+
+@dataclass
+class KspaceProblem:
+    """A bulk force-constant model solved over a transverse k mesh."""
+
+    params: object
+    lead: object
+    device: object
+    kpoints: list
+    omegas: np.ndarray
+    eta: float
+    wmax: float
+
+
+def cubic_params(cell, seed: int = 0, jitter: float = 0.05, mass: float = 28.0):
+    """``MaterialKspaceParams`` of a simple-cubic cell (``_cubic_phi``, seeded bond
+    jitter) with (dy, dz) terms for the same layer and the next one."""
+    from negfpy_amd.models.material_kspace import MaterialKspaceParams
+
+    phi = _cubic_phi(cell, seed, jitter)
+    na = int(np.prod(cell))
+    f00 = {(dy, dz): v for (dx, dy, dz), v in phi.items() if dx == 0}
+    f01 = {(dy, dz): v for (dx, dy, dz), v in phi.items() if dx == 1}
+    return MaterialKspaceParams(masses=np.full(na, float(mass)), fc00_terms=f00,
+                                fc01_terms=f01)
+
+
+def uneven3d_problems(seed: int = 0, n_layers: int = 8, n_omega: int = 1024,
+                      edge_fraction: float = 0.25, eta: float = 1e-6) -> list[KspaceProblem]:
+    """The 3-D uneven batch: simple-cubic cells of 8, 16 and 32 atoms (24-, 48- and
+    96-DOF layers, mass 28 amu, first and second neighbour springs with 5 % seeded bond
+    jitter) on 32 x 32, 8 x 8 and 4 x 4 (ky, kz) meshes, n_layers device layers of
+    the same cells and n_omega frequencies each; a quarter of each spectrum packed
+    within 1 % of the band top.  The 24-DOF part alone is 1,048,576 instances."""
+    from negfpy_amd.models.material_kspace import material_kspace_device, material_kspace_lead
+
+    out = []
+    for i, (cell, nk) in enumerate(UNEVEN3D_PARTS):
+        params = cubic_params(cell, seed + i)
+        build = material_kspace_lead(params).blocks_builder
+        top = 0.0
+        for ky in np.linspace(0.0, np.pi, 5):
+            for kz in np.linspace(0.0, np.pi, 5):
+                d00, d01, d10 = build((ky, kz))
+                for kx in np.linspace(0.0, np.pi, 9):
+                    dk = d00 + d01 * np.exp(1j * kx) + d10 * np.exp(-1j * kx)
+                    top = max(top, float(np.linalg.eigvalsh(0.5 * (dk + dk.conj().T)).max()))
+        wmax = float(np.sqrt(top))
+        w = 1.1 * wmax * (np.arange(n_omega) + 0.5) / n_omega
+        m = int(round(edge_fraction * n_omega))
+        w[::max(n_omega // m, 1)][:m] = wmax * (1.0 + np.linspace(-0.01, 0.01, m))
+        ks = [(-np.pi + 2 * np.pi * (a + 0.5) / nk, -np.pi + 2 * np.pi * (b + 0.5) / nk)
+              for a in range(nk) for b in range(nk)]
+        out.append(KspaceProblem(params=params, lead=material_kspace_lead(params),
+                                 device=material_kspace_device(n_layers, params), kpoints=ks,
+                                 omegas=np.sort(w), eta=eta, wmax=wmax))
+    return out

@@ -177,6 +177,28 @@ def test_sheet_kspace_spectrum_matches_oracle(gpu):
     assert np.all(np.abs(t * len(p.kpoints) - np.round(t * len(p.kpoints))) < 5e-2)
 
 
+def test_cubic_kspace_spectrum_matches_oracle(gpu):
+    """3-D simple-cubic cells with (ky, kz) transverse terms (the uneven 3-D batch's
+    24-DOF part, reduced): the k-averaged spectrum, blocks built on the GPU, against
+    the oracle's mean over k."""
+    from negfpy_amd.workloads import uneven3d_problems
+
+    p = uneven3d_problems(n_layers=4, n_omega=16)[0]
+    ks = p.kpoints[::257][:4]
+    w = p.omegas[1::4]
+    t = transmission_spectrum(w, p.device, p.lead, p.lead, kpoints=ks, eta=p.eta)
+    refs = []
+    for wi in w:
+        per_k = []
+        for k in ks:
+            d00, d01, d10 = p.lead.blocks_builder(k)
+            dv = p.device.device_builder(k)
+            per_k.append(oracle_t(wi, dv.onsite_blocks, dv.coupling_blocks, (d00, d01, d10),
+                                  (d00, d01, d10), eta=p.eta))
+        refs.append((float(np.mean([a for a, _ in per_k])), max(b for _, b in per_k)))
+    assert_parity(t, refs)
+
+
 def test_identity_blocks_channel_count(gpu):
     params = MultiDofCellParams(masses=np.ones(4), fc00=2.0 * np.eye(12), fc01=-np.eye(12))
     lead = multidof_lead_blocks(params)
