@@ -453,7 +453,8 @@ def main():
         stage_ms = {k: float(np.mean([a.elapsed_time(b) for a, b in v])) for k, v in ev.items()}
         status = out.status.cpu().numpy()
         results[prec] = {"t": out.t.cpu().numpy(), "iters": out.iters_l.cpu().numpy(),
-                         "status": status, "s_per_step": dt, "stage_ms": stage_ms}
+                         "dos": out.dos_l.cpu().numpy(), "status": status, "s_per_step": dt,
+                         "stage_ms": stage_ms}
         print(f"{prec}: {dt * 1e3:.3f} ms/step, {n_inst / dt:.1f} solves/s, stages {stage_ms}",
               file=sys.stderr, flush=True)
 
@@ -467,6 +468,14 @@ def main():
     ok = (results["fp64"]["status"] == 0) & (results["mixed"]["status"] == 0)
     rel = np.abs(t32 - t64) / np.maximum(np.abs(t64), 1e-3)
     rel = rel[ok]
+    d64, d32 = results["fp64"]["dos"][ok], results["mixed"]["dos"][ok]
+    rel_dos = np.abs(d32 - d64) / np.maximum(np.abs(d64), 1e-3 * np.abs(d64).max())
+    from negfpy_amd.core import _engine as _eng
+    from negfpy_amd.workloads import OMEGA_UNIT_RAD_S
+
+    w_rad = problem.omegas * OMEGA_UNIT_RAD_S
+    k64 = _eng.thermal_conductance(w_rad, np.where(ok, t64, 0.0), [300.0])[0]
+    k32 = _eng.thermal_conductance(w_rad, np.where(ok, t32, 0.0), [300.0])[0]
 
     mix = results["mixed"]
     mean_iters = float(np.mean(mix["iters"]))
@@ -515,6 +524,10 @@ def main():
         "mixed_rel_err_vs_fp64": {"max": float(rel.max()), "median": float(np.median(rel)),
                                   "frac_above_1e-6": float(np.mean(rel > 1e-6)),
                                   "floor": "relative to max(|T|, 1e-3)"},
+        "mixed_surface_dos_rel_err_vs_fp64": {"max": float(rel_dos.max()),
+                                              "median": float(np.median(rel_dos)),
+                                              "floor": "relative to max(|DOS|, 1e-3 max DOS)"},
+        "mixed_kappa_300K_rel_err_vs_fp64": float(abs(k32 - k64) / abs(k64)),
         "mean_decimation_steps": mean_iters,
         "roofline": (
             {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": peak,
