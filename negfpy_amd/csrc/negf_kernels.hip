@@ -74,10 +74,13 @@ struct Cfg<96> {
 
 // Waves per SIMD the main kernels must allow: the LDS footprint of one
 // instance admits about this many resident workgroups per CU, so capping
-// registers below that would only leave LDS unused.
+// registers below that would only leave LDS unused.  FP64 class 16 (one wave per
+// instance, 13 KB of LDS) fits twelve instances per CU: at three waves per SIMD
+// the decimation keeps its state in 168 VGPRs without spilling (the mirrored
+// form spills 25) and runs 8-12 % faster than at two.
 template <typename T, int NMAX>
 struct Occ {
-  static constexpr int value = (NMAX <= 32) ? (sizeof(T) == 8 ? (NMAX <= 16 ? 2 : 3) : 4) : 1;
+  static constexpr int value = (NMAX <= 32) ? (sizeof(T) == 8 ? 3 : 4) : 1;
 };
 // The recursive sweep keeps three matrices per instance and fits 128 VGPRs: at
 // n = 24 in FP64 five workgroups (15 waves) fit a CU.  (The decimation also keeps
