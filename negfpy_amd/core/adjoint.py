@@ -31,8 +31,11 @@ __all__ = ["transmission_torch", "transmission_sets_torch", "transmission_sum_to
            "thermal_conductance_torch"]
 
 # Device memory one backward launch may use for adjoint workspace and per-instance
-# gradients; larger omega batches are processed in chunks of this size.
+# gradients; larger omega batches are processed in chunks of this size.  The
+# k-resolved functions may use this fraction of the free device memory instead
+# when that is more.
 ADJOINT_CHUNK_BYTES = 2 << 30
+ADJOINT_FREE_FRACTION = 0.25
 
 
 def _merge_layers_torch(onsite: torch.Tensor, coupling: torch.Tensor, m: int):
@@ -160,10 +163,11 @@ def transmission_torch(omega, onsite, coupling, lead_left, lead_right, *, eta: f
 
 def _chunk_instances(per_inst: int, n_inst: int, dev) -> int:
     """Instances per backward launch: the per-instance adjoint memory within
-    ADJOINT_CHUNK_BYTES, or within a quarter of the free device memory if that is more."""
+    ADJOINT_CHUNK_BYTES, or within ADJOINT_FREE_FRACTION of the free device memory if
+    that is more."""
     budget = ADJOINT_CHUNK_BYTES
     if dev.type == "cuda":
-        budget = max(budget, torch.cuda.mem_get_info(dev)[0] // 4)
+        budget = max(budget, int(torch.cuda.mem_get_info(dev)[0] * ADJOINT_FREE_FRACTION))
     return max(1, min(n_inst, budget // per_inst))
 
 

@@ -183,6 +183,39 @@ def test_one_pass_kappa_equals_autograd_path(gpu):
         assert float((a - b).abs().max()) <= 1e-10 * float(b.abs().max())
 
 
+def test_chunked_passes_equal_one_launch(gpu, monkeypatch):
+    """Backward and one-pass sums in chunks of one instance give the gradients of a
+    single launch (the per-set sums across chunks)."""
+    from negfpy_amd.core import adjoint as adj
+
+    p = _small()
+    nk = len(p.kpoints)
+    coef = torch.linspace(0.5, 1.5, p.omegas.size * nk, dtype=torch.float64, device="cuda")
+    k_of, om = _instances(p)
+
+    def grads(fn):
+        f00, f01 = _leaves(p)
+        ons, cpl = interface3d_blocks(p, f00, f01)
+        fn(ons, cpl).backward()
+        return f00.grad.clone(), f01.grad.clone()
+
+    def via_sets(ons, cpl):
+        t = transmission_sets_torch(om, k_of, ons, cpl, p.lead_left, p.lead_right, p.v_left,
+                                    p.v_right, eta=p.eta)
+        return (coef * t).sum()
+
+    def via_sum(ons, cpl):
+        return transmission_sum_torch(coef, om, k_of, ons, cpl, p.lead_left, p.lead_right,
+                                      p.v_left, p.v_right, eta=p.eta)
+
+    ref = {f: grads(f) for f in (via_sets, via_sum)}
+    monkeypatch.setattr(adj, "ADJOINT_CHUNK_BYTES", 1)
+    monkeypatch.setattr(adj, "ADJOINT_FREE_FRACTION", 0.0)
+    for f in (via_sets, via_sum):
+        for a, b in zip(grads(f), ref[f]):
+            assert float((a - b).abs().max()) <= 1e-12 * float(b.abs().max())
+
+
 def test_argument_errors(gpu):
     p = _small()
     f00, f01 = _leaves(p)
