@@ -24,7 +24,8 @@ import numpy as np
 import pytest
 
 from negfpy_amd.core.types import LeadBlocks
-from negfpy_amd.workloads import omega_max, random_lead, ribbon_problem, sheet_problem
+from negfpy_amd.workloads import (cubic_params, interface3d_problem, omega_max, random_lead,
+                                  ribbon_problem, sheet_problem)
 from oracle import negf_oracle as o
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "study_kappa")
@@ -222,3 +223,35 @@ def test_sheet_workload_channel_count():
     lead = p.lead_right.blocks_builder((0.0,))
     t_def = o.transmission_sparse(0.05, mk.onsite_blocks, mk.coupling_blocks, lead, lead, eta=1e-6)
     assert abs(t_def - 3.0) > 0.5
+
+
+def test_cubic_workloads_are_physical():
+    """Simple-cubic force constants (3-D uneven batch, 3-D interface): bond jitter
+    shared by both ends, so Phi00(-d) = Phi00(d)^T; Hermitian D(k); three zero modes
+    at k = 0 (acoustic sum rule); D(k) >= 0 on a (kx, ky, kz) sample; and the pristine
+    interface problem (equal masses, no perturbation) transmits its channel count."""
+    from negfpy_amd.models.material_kspace import material_kspace_lead
+
+    p = cubic_params((2, 2, 2), seed=4, jitter=0.05)
+    for (dy, dz), blk in p.fc00_terms.items():
+        np.testing.assert_array_equal(p.fc00_terms[(-dy, -dz)], blk.T)
+    build = material_kspace_lead(p).blocks_builder
+    d00, d01, d10 = build((0.0, 0.0))
+    ev = np.linalg.eigvalsh(d00 + d01 + d10)
+    assert np.all(np.abs(ev[:3]) < 1e-12) and ev[3] > 1e-3
+    for ky, kz in ((0.3, -1.1), (np.pi, 0.5), (-2.0, 2.9)):
+        d00, d01, d10 = build((ky, kz))
+        assert np.abs(d00 - d00.conj().T).max() < 1e-12
+        for kx in (0.0, 0.7, np.pi):
+            dk = d00 + d01 * np.exp(1j * kx) + d10 * np.exp(-1j * kx)
+            assert np.linalg.eigvalsh(dk).min() > -1e-12
+    q = interface3d_problem(cell=(2, 1, 1), n_layers=4, nk=2, n_omega=8, m_right=28.0,
+                            perturb=0.0)
+    s = 1
+    ons = [np.einsum("t,tij->ij", q.phases[s], q.fc00[i]) / q.masses[i][0] for i in range(4)]
+    cps = [np.einsum("t,tij->ij", q.phases[s], q.fc01[i]) / q.masses[i][0] for i in range(3)]
+    lead = (q.lead_right[s, 0], q.lead_right[s, 1], q.lead_right[s, 2])
+    for w in q.omegas[[1, 4, 6]]:
+        t = o.transmission_sparse(w, ons, cps, tuple(q.lead_left[s]), tuple(q.lead_right[s]),
+                                  eta=1e-6, v_left=q.v_left[s], v_right=q.v_right[s])
+        assert abs(t - _channels(lead, w)) < 1e-3, (w, t)
