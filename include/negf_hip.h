@@ -41,9 +41,10 @@ enum { NEGF_SURFACE_DECIMATION = 0, NEGF_SURFACE_EIGEN = 1 };
 /* negf_transmission_problem.share_leads: independent leads; the right lead is the left
  * one (one decimation serves both); the right lead is the left one mirrored, lead_r =
  * (d00_L, d10_L, d01_L) for every set, as the interface contacts of a bulk force-constant
- * model are -- one decimation yields both surfaces where the block fits (<= 48 DOF),
- * two decimations otherwise.  The caller guarantees the mirror relation; lead_r must
- * still be passed. */
+ * model are -- in negf_transmission_batched one decimation yields both surfaces where
+ * the block fits (<= 48 DOF), two decimations otherwise and in
+ * negf_transmission_multi_batched.  The caller guarantees the mirror relation; lead_r
+ * must still be passed. */
 enum { NEGF_LEADS_INDEPENDENT = 0, NEGF_LEADS_SHARED = 1, NEGF_LEADS_MIRRORED = 2 };
 enum { NEGF_OK = 0, NEGF_ERR_ARG = 1, NEGF_ERR_HIP = 2 };
 
