@@ -86,7 +86,9 @@ int negf_block_class(int precision, int n);
  *   tol, maxiter               stop when max(|alpha|, |beta|) < tol
  *   refine_steps               NEGF_MIXED only: FP64 residual steps after the FP32 decimation;
  *                              < 0 leaves the FP32 decimation's eps_s in g_out (complex128)
- *                              for a separate negf_surface_refine_batched call
+ *                              for a separate negf_surface_refine_batched call.  NEGF_MIXED
+ *                              decimates instances with eta < 1e-3 omega in FP64 only: the
+ *                              FP32 pass marks them ST_REDO (7) for negf_sancho_rubio_redo
  *   g_out   [n_inst][n][n]     g_s = (z - eps_s)^-1
  *   dos_out [n_inst] or NULL   (omega/pi) Re Tr[i (g - g^H)]
  *   iters, status [n_inst]

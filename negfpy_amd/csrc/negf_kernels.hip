@@ -228,7 +228,14 @@ struct SrArgs {
   double* dos_out2;
   int32_t* iters2;
   int32_t* status2;
+  // FP32 pass: instances with eta < fp32_eta_ratio * omega skip straight to the FP64 redo
+  double fp32_eta_ratio;
 };
+
+// Mixed precision: below this eta / omega the FP32 decimation of a propagating mode
+// does not survive the FP64 refinement (its fixed-point map contracts like
+// 1 - O(eta / omega)), so such instances are decimated in FP64 directly.
+constexpr double FP32_MIN_ETA_RATIO = 1e-3;
 
 // g = (z - e_s)^-1 from the VGPR tiles of e_s into out (FP64), or e_s itself (FP32
 // mode, for the FP64 refinement); the surface DOS (omega / pi) tr(-2 Im g).
@@ -297,6 +304,19 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
   bool do2 = false;
   if constexpr (MIRROR) do2 = !a.redo || a.status2[inst] != ST_OK;
   if (!do1 && !do2) return;
+  if constexpr (sizeof(T) == 4) {
+    if (a.eta[inst] < a.fp32_eta_ratio * a.omega[inst]) {  // uniform per block
+      if (tid == 0) {
+        a.status[inst] = ST_REDO;
+        a.iters[inst] = 0;
+        if constexpr (MIRROR) {
+          a.status2[inst] = ST_REDO;
+          a.iters2[inst] = 0;
+        }
+      }
+      return;
+    }
+  }
   const int n = a.n;
   const size_t nn = (size_t)n * n;
   const double* blk = a.lead + (size_t)a.set_of[inst] * 3 * nn * 2;
@@ -1981,6 +2001,7 @@ int negf_sancho_rubio_batched(void* stream, int precision, int n, int n_inst, co
     if (n > MIXED_MAX)
       return fail(NEGF_ERR_ARG, "block size %d exceeds the mixed-precision limit of %d", n,
                   MIXED_MAX);
+    a.fp32_eta_ratio = FP32_MIN_ETA_RATIO;
     rc = run_sr<float>(st, nmax, a);
     if (rc == NEGF_OK && refine_steps >= 0) {
       rc = negf_surface_refine_batched(stream, n, n_inst, lead, set_of, omega, eta, refine_steps,
@@ -2216,6 +2237,7 @@ static int sr_mirrored(void* stream, const negf_transmission_problem* p, double*
            p->maxiter, g_l,       p->dos_l,  p->iters_l, st_l,     0,           g_r,
            p->dos_r, p->iters_r,  st_r};
   if (p->precision == NEGF_FP64) return run_sr<double, true>(st, pick_nmax(n, true), a);
+  a.fp32_eta_ratio = FP32_MIN_ETA_RATIO;
   int rc = run_sr<float, true>(st, pick_nmax(n, false), a);
   if (rc || p->refine_steps < 0) return rc;
   rc = negf_surface_refine_batched(stream, n, p->n_inst, p->lead_l, p->set_of, p->omega,
@@ -2225,6 +2247,7 @@ static int sr_mirrored(void* stream, const negf_transmission_problem* p, double*
                                    p->eta_lead, p->refine_steps, g_r, p->dos_r, st_r);
   if (rc) return rc;
   a.redo = 1;
+  a.fp32_eta_ratio = 0.0;
   return run_sr<double, true>(st, pick_nmax(n, true), a);
 }
 
