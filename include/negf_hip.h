@@ -32,9 +32,13 @@ extern "C" {
 
 #define NEGF_ABI_VERSION 1
 
-/* Mixed precision: a refined FP32 surface GF whose last FP64 residual step still
- * changes it by more than this (relative Frobenius norm) is recomputed in FP64. */
-#define NEGF_REDO_TOL 1e-6
+/* Mixed precision: certificate of a Newton-refined FP32 decimation (relative Frobenius
+ * norms).  Accepted when the FP64 residual |I - (z - d00 - d01 g d10) g| / (|M| |g|) is
+ * below RTOL, the last Newton correction below XTOL and the drift from the FP32
+ * decimation's g below DTOL; otherwise the instance is decimated again in FP64. */
+#define NEGF_NEWTON_RTOL 1e-13
+#define NEGF_NEWTON_XTOL 1e-8
+#define NEGF_NEWTON_DTOL 1e-4
 
 enum { NEGF_FP64 = 0, NEGF_MIXED = 1 };
 enum { NEGF_SURFACE_DECIMATION = 0, NEGF_SURFACE_EIGEN = 1 };
@@ -77,6 +81,20 @@ int negf_set_block_class(int precision, int n, int nmax);
 int negf_block_class(int precision, int n);
 
 /*
+ * Kernel family for blocks of up to 32 DOF.  The one-wavefront family solves each
+ * instance on one 64-lane wavefront with its products on the matrix cores
+ * (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32) and a register-resident wave
+ * Gauss-Jordan; the other family gives an instance a workgroup (the block-size
+ * classes above).  mode 1 (default): the wave family where it measured faster on
+ * MI355X (FP64 blocks up to 16 DOF, the FP32 sweep up to 24); mode 2: the wave family
+ * for every block up to 32 DOF; mode 0: never.  Both give the same results to
+ * rounding.  Returns the previous mode (-1 and NEGF_ERR_ARG text for a bad mode);
+ * process-wide.  No counterpart in the reference (a launch-configuration switch, like
+ * negf_set_block_class).
+ */
+int negf_set_wave_kernels(int mode);
+
+/*
  * Surface Green's function of semi-infinite leads by Lopez-Sancho decimation.
  * Replaces surface_gf_sancho_rubio, src/negfpy/core/surface_gf.py:18-81
  * (and the per-k loop of lead_surface_dos*, src/negfpy/core/dos.py:31-251).
@@ -84,11 +102,11 @@ int negf_block_class(int precision, int n);
  *   set_of  [n_inst]           set index of each instance
  *   omega, eta [n_inst]        z = (omega + i eta)^2
  *   tol, maxiter               stop when max(|alpha|, |beta|) < tol
- *   refine_steps               NEGF_MIXED only: FP64 residual steps after the FP32 decimation;
- *                              < 0 leaves the FP32 decimation's eps_s in g_out (complex128)
- *                              for a separate negf_surface_refine_batched call.  NEGF_MIXED
- *                              decimates instances with eta < 1e-3 omega in FP64 only: the
- *                              FP32 pass marks them ST_REDO (7) for negf_sancho_rubio_redo
+ *   refine_steps               NEGF_MIXED only: Newton steps of the FP64 completion after
+ *                              the FP32 decimation (negf_surface_refine_batched, then
+ *                              negf_sancho_rubio_redo for what it does not certify); < 0
+ *                              leaves the FP32 decimation's eps_s in g_out (complex128) for
+ *                              separate calls
  *   g_out   [n_inst][n][n]     g_s = (z - eps_s)^-1
  *   dos_out [n_inst] or NULL   (omega/pi) Re Tr[i (g - g^H)]
  *   iters, status [n_inst]
@@ -99,10 +117,15 @@ int negf_sancho_rubio_batched(void* stream, int precision, int n, int n_inst, co
                               double* dos_out, int32_t* iters, int32_t* status);
 
 /*
- * FP64 completion of a mixed-precision decimation: g = (z - eps_s)^-1 from the
- * eps_s in g_io, then `steps` residual steps g <- (z - d00 - d01 g d10)^-1 (the
- * fixed-point form of surface_gf.py:81 at convergence), the surface DOS, and g
- * written back to g_io.  Instances with status != 0 are skipped.
+ * FP64 completion of a mixed-precision decimation (blocks up to 32 DOF; 33..48 are all
+ * marked for the FP64 redo): g = (z - eps_s)^-1 from the eps_s in g_io, then `steps`
+ * Newton steps on F(g) = I - (z - d00 - d01 g d10) g = 0 (the fixed point of the
+ * decimation, surface_gf.py:41-81): an FP64 residual, the correction X - P X Q = g R
+ * with P = g d01, Q = d10 g solved in FP32 by doubling, g += X.  A final FP64 residual
+ * and the NEGF_NEWTON_* tolerances certify the instance: certified ones get g and the
+ * surface DOS written back, the others status ST_REDO (7) for negf_sancho_rubio_redo.
+ * Instances with status != 0 on entry are skipped.  No counterpart in the reference
+ * (which decimates in FP64 only).
  */
 int negf_surface_refine_batched(void* stream, int n, int n_inst, const double* lead,
                                 const int32_t* set_of, const double* omega, const double* eta,
@@ -110,7 +133,7 @@ int negf_surface_refine_batched(void* stream, int n, int n_inst, const double* l
 
 /*
  * FP64 decimation of exactly the instances whose status is nonzero (FP32
- * failures and refinements that moved by more than NEGF_REDO_TOL); others are
+ * failures and refinements the Newton certificate rejected); others are
  * left untouched.  negf_sancho_rubio_batched(NEGF_MIXED) ends with this call.
  */
 int negf_sancho_rubio_redo(void* stream, int n, int n_inst, const double* lead,
@@ -224,7 +247,7 @@ typedef struct negf_transmission_problem {
   int n_c_r;        /* right contact size (rows of v_r) */
   int share_leads;  /* NEGF_LEADS_INDEPENDENT, _SHARED or _MIRRORED */
   int maxiter;      /* decimation iteration limit */
-  int refine_steps; /* NEGF_MIXED: FP64 residual steps per surface GF */
+  int refine_steps; /* NEGF_MIXED: Newton steps per surface GF (>= 0) */
   double tol;       /* decimation tolerance */
   const double* lead_l;   /* [n_sets][3][n_lead_l][n_lead_l] */
   const double* lead_r;   /* [n_sets][3][n_lead_r][n_lead_r] */
@@ -272,6 +295,12 @@ size_t negf_transmission_workspace_bytes(const negf_transmission_problem* p);
  * self-energies, the device sweep and the trace.  Replaces transmission(),
  * src/negfpy/core/negf.py:306-357, and the k loops of transmission_kavg*,
  * :360-688, which call it once per point.
+ * NEGF_MIXED: FP32 decimation, FP64 Newton completion with FP32 Stein corrections and
+ * a certificate, FP64 decimation of the uncertified instances; the device sweep,
+ * self-energies, Gamma and the trace in FP64 (an FP32 sweep leaves about a quarter
+ * of a disordered 32-layer ribbon's frequencies above 1e-6 relative error in T at
+ * eta = 1e-6, and correcting it by FP64-residual refinement costs more than the FP64
+ * sweep).  refine_steps must be >= 0 in NEGF_MIXED.
  */
 int negf_transmission_batched(void* stream, const negf_transmission_problem* p, void* workspace,
                               size_t workspace_bytes);

@@ -77,6 +77,7 @@ _SIGS = {
     "negf_max_block_green": ([], _c_int),
     "negf_set_block_class": ([_c_int, _c_int, _c_int], _c_int),
     "negf_block_class": ([_c_int, _c_int], _c_int),
+    "negf_set_wave_kernels": ([_c_int], _c_int),
     "negf_sancho_rubio_batched": (
         [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_dbl, _c_int, _c_int, _vp, _vp, _vp, _vp],
         _c_int),

@@ -282,13 +282,15 @@ class TransmissionSolver:
         return out
 
 
-    def run_staged(self, set_of, omega, eta_lead, eta_dev=None, events=None
+    def run_staged(self, set_of, omega, eta_lead, eta_dev=None, events=None, stats=None
                    ) -> TransmissionResult:
         """Same computation as :meth:`run`, one C-ABI call per stage.
 
         ``events`` (a dict) receives a pair of torch.cuda.Event per stage, recorded
         on the launch stream around the stage's single kernel, so callers can time
-        every kernel separately.
+        every kernel separately.  ``stats`` (a dict), in mixed precision, receives
+        per lead side the number of instances whose FP32 decimation the Newton
+        completion certified (``fp32_certified_l``/``_r``) out of ``instances``.
         """
         lib = self._lib
         s = self.sets
@@ -338,6 +340,9 @@ class TransmissionSolver:
                           st, nn, n_inst, lead.data_ptr(), set_of.data_ptr(), omega.data_ptr(),
                           eta_lead.data_ptr(), self.refine_steps, g.data_ptr(), dos.data_ptr(),
                           stt.data_ptr()))
+                if stats is not None:
+                    stats[f"fp32_certified_{side}"] = int((stt == 0).sum().item())
+                    stats["instances"] = n_inst
                 stage(f"redo_{side}", lambda lead=lead, g=g, dos=dos, iters=iters, stt=stt, nn=nn:
                       lib.negf_sancho_rubio_redo(
                           st, nn, n_inst, lead.data_ptr(), set_of.data_ptr(), omega.data_ptr(),
@@ -357,8 +362,9 @@ class TransmissionSolver:
             st, nr, int(s.v_r.shape[1]), n, n_inst, s.v_r.data_ptr(), set_of.data_ptr(),
             g_r.data_ptr(), s.idx_r.data_ptr(), st_r.data_ptr(), sig_r.data_ptr()))
         _lib.check(lib.negf_merge_status(st, n_inst, st_l.data_ptr(), st_r.data_ptr()))
+        # the device sweep runs in FP64 in both precisions, as in negf_transmission_batched
         stage("rgf", lambda: lib.negf_rgf_g1n_batched(
-            st, self.precision, n, s.n_layers, n_inst, s.onsite.data_ptr(), s.coupling.data_ptr(),
+            st, _lib.FP64, n, s.n_layers, n_inst, s.onsite.data_ptr(), s.coupling.data_ptr(),
             set_of.data_ptr(), omega.data_ptr(), eta_dev.data_ptr(), sig_l.data_ptr(),
             sig_r.data_ptr(), st_l.data_ptr(), g1n.data_ptr(), out.status.data_ptr()))
         stage("trace", lambda: lib.negf_transmission_trace_batched(

@@ -228,14 +228,8 @@ struct SrArgs {
   double* dos_out2;
   int32_t* iters2;
   int32_t* status2;
-  // FP32 pass: instances with eta < fp32_eta_ratio * omega skip straight to the FP64 redo
-  double fp32_eta_ratio;
 };
 
-// Mixed precision: below this eta / omega the FP32 decimation of a propagating mode
-// does not survive the FP64 refinement (its fixed-point map contracts like
-// 1 - O(eta / omega)), so such instances are decimated in FP64 directly.
-constexpr double FP32_MIN_ETA_RATIO = 1e-3;
 
 // g = (z - e_s)^-1 from the VGPR tiles of e_s into out (FP64), or e_s itself (FP32
 // mode, for the FP64 refinement); the surface DOS (omega / pi) tr(-2 Im g).
@@ -304,19 +298,6 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
   bool do2 = false;
   if constexpr (MIRROR) do2 = !a.redo || a.status2[inst] != ST_OK;
   if (!do1 && !do2) return;
-  if constexpr (sizeof(T) == 4) {
-    if (a.eta[inst] < a.fp32_eta_ratio * a.omega[inst]) {  // uniform per block
-      if (tid == 0) {
-        a.status[inst] = ST_REDO;
-        a.iters[inst] = 0;
-        if constexpr (MIRROR) {
-          a.status2[inst] = ST_REDO;
-          a.iters2[inst] = 0;
-        }
-      }
-      return;
-    }
-  }
   const int n = a.n;
   const size_t nn = (size_t)n * n;
   const double* blk = a.lead + (size_t)a.set_of[inst] * 3 * nn * 2;
@@ -490,9 +471,25 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
 }
 
 // ---------------------------------------------------------------------------
-// FP64 refinement of an FP32 decimation
+// FP64 completion of an FP32 decimation: Newton on the surface equation with FP32
+// Stein corrections, and a certificate
 // ---------------------------------------------------------------------------
-
+//
+// The retarded surface GF solves F(g) = I - (z - d00 - d01 g d10) g = 0, the fixed
+// point of the decimation (src/negfpy/core/surface_gf.py:41-81).  From the FP32
+// decimation's e_s, g = (z - e_s)^-1 in FP64, then each Newton step
+//   R = F(g)                            (FP64: three products)
+//   X - P X Q = g R,  P = g d01, Q = d10 g   (the linearised equation, a Stein equation)
+//   g <- g + X
+// where the Stein equation is solved in FP32 by doubling (Smith's method:
+// X += P X Q, P <- P^2, Q <- Q^2) until |P| |Q| < 1e-10 or 64 doublings.  A final
+// FP64 residual certifies the result: accepted when the relative residual
+// |R| / (|M| |g|) < rtol, the last correction |X| / |g| < xtol (quadratic
+// convergence reached) and g moved less than dtol from the FP32 decimation's g (the
+// Newton root is the retarded one the decimation approached, not another solution of
+// the quadratic equation).  Anything else -- a doubling that diverges because FP32
+// cannot separate |lambda| = 1 - eta/v from the unit circle, a non-finite value, a
+// singular pivot -- marks the instance ST_REDO for the FP64 decimation.
 struct RefineArgs {
   int n;
   int n_inst;
@@ -500,21 +497,32 @@ struct RefineArgs {
   const int32_t* set_of;
   const double* omega;
   const double* eta;
-  int steps;
-  double redo_tol;  // last-step relative change above this marks the instance for FP64 redo
-  double* g_io;     // in: e_s, out: g_s
+  int steps;       // Newton steps
+  double rtol, xtol, dtol;
+  double* g_io;    // in: e_s, out: g_s (certified instances)
   double* dos_out;
   int32_t* status;
 };
+
+constexpr int STEIN_MAX_DOUBLINGS = 64;
 
 template <int NMAX>
 __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_refine(RefineArgs a) {
   using T = double;
   constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
-  using L = Lds<T, NMAX, NT, 4>;
-  constexpr int LD = L::LD;
+  using L = Lds<T, NMAX, NT, 4>;  // [0] d01 [1] d10 [2] g [3] scratch (FP64)
+  constexpr int LD = L::LD, MSZ = L::MSZ;
   extern __shared__ __align__(16) unsigned char smem[];
   L s(smem);
+  // FP32 Stein matrices after the FP64 part: X, P, Q, W
+  float* fb = reinterpret_cast<float*>(smem + ((L::bytes() + 15) & ~size_t(15)));
+  float* fr[4];
+  float* fi[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    fr[q] = fb + (2 * q) * MSZ;
+    fi[q] = fb + (2 * q + 1) * MSZ;
+  }
   const int tid = threadIdx.x;
   const int inst = blockIdx.x;
   if (a.status[inst] != ST_OK) return;  // uniform per block
@@ -523,13 +531,13 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_refine(RefineArgs a) {
   const double* blk = a.lead + (size_t)a.set_of[inst] * 3 * nn * 2;
   double* io = a.g_io + (size_t)inst * nn * 2;
   lds_zero<T, NT>(smem, L::mat_bytes(), tid);
+  lds_zero<float, NT>(reinterpret_cast<unsigned char*>(fb), sizeof(float) * 8 * MSZ, tid);
   __syncthreads();
   gload<T, NT>(blk + nn * 2, n, n, s.re[0], s.im[0], LD, tid);      // d01
   gload<T, NT>(blk + 2 * nn * 2, n, n, s.re[1], s.im[1], LD, tid);  // d10
   const Tile<TM> t(n, n, tid);
   double zr, zi;
   zsq(a.omega[inst], a.eta[inst], zr, zi);
-  // G = z - e_s
   {
     const double2* es = reinterpret_cast<const double2*>(io);
     for (int e = tid; e < (int)nn; e += NT) {
@@ -541,17 +549,165 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_refine(RefineArgs a) {
   }
   __syncthreads();
   int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[2], s.im[2], LD, n, *s.gj, tid);
+  if (st != ST_OK) {
+    if (tid == 0) a.status[inst] = ST_REDO;
+    return;
+  }
+  // the FP32 decimation's g, for the drift test at the end
+  gstore<T, NT>(io, n, s.re[2], s.im[2], LD, tid);
   const double2* d00 = reinterpret_cast<const double2*>(blk);
-  double change = 0.0;
-  for (int step = 0; step < a.steps && st == ST_OK; ++step) {
-    T xr[TM][TM], xi[TM][TM], gr[TM][TM], gi[TM][TM];
+  double rres = INFINITY, xrel = INFINITY;
+  bool bad = false;
+  for (int step = 0; step <= a.steps; ++step) {
+    // T = d01 g -> [3]
+    T xr[TM][TM], xi[TM][TM];
     tzero<T, TM>(xr, xi);
-    tzero<T, TM>(gr, gi);
     if (t.own) mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[2], s.im[2], LD, n, t.i0, t.j0, xr, xi);
     tstore<T, TM>(s.re[3], s.im[3], LD, n, n, t, xr, xi);
     __syncthreads();
+    // M = (z - d00) - T d10 -> [3]
     tzero<T, TM>(xr, xi);
     if (t.own) mac<T, TM, OPN, OPN>(s.re[3], s.im[3], s.re[1], s.im[1], LD, n, t.i0, t.j0, xr, xi);
+    double mn = 0.0;
+    if (t.own) {
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          const int i = t.i0 + x, j = t.j0 + y;
+          if (i < n && j < n) {
+            const double2 d = d00[i * n + j];
+            xr[x][y] = ((i == j ? zr : 0.0) - d.x) - xr[x][y];
+            xi[x][y] = ((i == j ? zi : 0.0) - d.y) - xi[x][y];
+            mn += xr[x][y] * xr[x][y] + xi[x][y] * xi[x][y];
+          } else {
+            xr[x][y] = 0.0;
+            xi[x][y] = 0.0;
+          }
+        }
+    }
+    __syncthreads();
+    tstore<T, TM>(s.re[3], s.im[3], LD, n, n, t, xr, xi);
+    __syncthreads();
+    // R = I - M g (tiles), norms of R and g
+    tzero<T, TM>(xr, xi);
+    if (t.own) mac<T, TM, OPN, OPN>(s.re[3], s.im[3], s.re[2], s.im[2], LD, n, t.i0, t.j0, xr, xi);
+    double rn = 0.0, gn = 0.0;
+    if (t.own) {
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          const int i = t.i0 + x, j = t.j0 + y;
+          if (i < n && j < n) {
+            xr[x][y] = (i == j ? 1.0 : 0.0) - xr[x][y];
+            xi[x][y] = -xi[x][y];
+            rn += xr[x][y] * xr[x][y] + xi[x][y] * xi[x][y];
+            const double gr = s.re[2][i * LD + j], gi = s.im[2][i * LD + j];
+            gn += gr * gr + gi * gi;
+          }
+        }
+    }
+    rn = block_sum<NT>(rn, s.sred, tid);
+    mn = block_sum<NT>(mn, s.sred, tid);
+    gn = block_sum<NT>(gn, s.sred, tid);
+    rres = sqrt(rn / (mn * gn));
+    if (!(rres < INFINITY)) {
+      bad = true;
+      break;
+    }
+    if (step == a.steps) break;
+    // C = g R -> X (FP32); P = g d01, Q = d10 g (FP32)
+    tstore<T, TM>(s.re[3], s.im[3], LD, n, n, t, xr, xi);
+    __syncthreads();
+    {
+      T cr[TM][TM], ci[TM][TM];
+      tzero<T, TM>(cr, ci);
+      if (t.own) mac<T, TM, OPN, OPN>(s.re[2], s.im[2], s.re[3], s.im[3], LD, n, t.i0, t.j0, cr, ci);
+      float fcr[TM][TM], fci[TM][TM];
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          fcr[x][y] = (float)cr[x][y];
+          fci[x][y] = (float)ci[x][y];
+        }
+      tstore<float, TM>(fr[0], fi[0], LD, n, n, t, fcr, fci);
+      tzero<T, TM>(cr, ci);
+      if (t.own) mac<T, TM, OPN, OPN>(s.re[2], s.im[2], s.re[0], s.im[0], LD, n, t.i0, t.j0, cr, ci);
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          fcr[x][y] = (float)cr[x][y];
+          fci[x][y] = (float)ci[x][y];
+        }
+      tstore<float, TM>(fr[1], fi[1], LD, n, n, t, fcr, fci);
+      tzero<T, TM>(cr, ci);
+      if (t.own) mac<T, TM, OPN, OPN>(s.re[1], s.im[1], s.re[2], s.im[2], LD, n, t.i0, t.j0, cr, ci);
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          fcr[x][y] = (float)cr[x][y];
+          fci[x][y] = (float)ci[x][y];
+        }
+      tstore<float, TM>(fr[2], fi[2], LD, n, n, t, fcr, fci);
+    }
+    __syncthreads();
+    // Stein doubling in FP32: X += P X Q, P <- P^2, Q <- Q^2
+    for (int k = 0; k < STEIN_MAX_DOUBLINGS; ++k) {
+      float wr[TM][TM], wi[TM][TM];
+      tzero<float, TM>(wr, wi);
+      if (t.own) mac<float, TM, OPN, OPN>(fr[1], fi[1], fr[0], fi[0], LD, n, t.i0, t.j0, wr, wi);
+      tstore<float, TM>(fr[3], fi[3], LD, n, n, t, wr, wi);  // W = P X
+      __syncthreads();
+      float vr[TM][TM], vi[TM][TM], pr[TM][TM], pi[TM][TM], qr[TM][TM], qi[TM][TM];
+      tzero<float, TM>(vr, vi);
+      tzero<float, TM>(pr, pi);
+      tzero<float, TM>(qr, qi);
+      if (t.own) {
+#pragma unroll
+        for (int x = 0; x < TM; ++x)
+#pragma unroll
+          for (int y = 0; y < TM; ++y) {
+            const int i = t.i0 + x, j = t.j0 + y;
+            if (i < n && j < n) {
+              vr[x][y] = fr[0][i * LD + j];
+              vi[x][y] = fi[0][i * LD + j];
+            }
+          }
+        mac<float, TM, OPN, OPN>(fr[3], fi[3], fr[2], fi[2], LD, n, t.i0, t.j0, vr, vi);
+        mac<float, TM, OPN, OPN>(fr[1], fi[1], fr[1], fi[1], LD, n, t.i0, t.j0, pr, pi);
+        mac<float, TM, OPN, OPN>(fr[2], fi[2], fr[2], fi[2], LD, n, t.i0, t.j0, qr, qi);
+      }
+      double pn = 0.0, qn = 0.0;
+      bool nf = false;
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TM; ++y) {
+          pn += (double)pr[x][y] * pr[x][y] + (double)pi[x][y] * pi[x][y];
+          qn += (double)qr[x][y] * qr[x][y] + (double)qi[x][y] * qi[x][y];
+          nf |= !(isfinite(vr[x][y]) && isfinite(vi[x][y]));
+        }
+      __syncthreads();
+      tstore<float, TM>(fr[0], fi[0], LD, n, n, t, vr, vi);
+      tstore<float, TM>(fr[1], fi[1], LD, n, n, t, pr, pi);
+      tstore<float, TM>(fr[2], fi[2], LD, n, n, t, qr, qi);
+      double pq, dummy;
+      bool anynf;
+      block_max_any<NT>(0.0, nf, s.sred, s.sflag, tid, dummy, anynf);
+      pq = sqrt(block_sum<NT>(pn, s.sred, tid) * block_sum<NT>(qn, s.sred, tid));
+      if (anynf || !(pq < 1e30)) {
+        bad = true;
+        break;
+      }
+      if (pq < 1e-10) break;
+    }
+    if (bad) break;
+    // g <- g + X
+    double xn = 0.0;
     __syncthreads();
     if (t.own) {
 #pragma unroll
@@ -560,39 +716,38 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_refine(RefineArgs a) {
         for (int y = 0; y < TM; ++y) {
           const int i = t.i0 + x, j = t.j0 + y;
           if (i < n && j < n) {
-            gr[x][y] = s.re[2][i * LD + j];
-            gi[x][y] = s.im[2][i * LD + j];
-            const double2 d = d00[i * n + j];
-            s.re[2][i * LD + j] = ((i == j ? zr : 0.0) - d.x) - xr[x][y];
-            s.im[2][i * LD + j] = ((i == j ? zi : 0.0) - d.y) - xi[x][y];
+            const double dr = fr[0][i * LD + j], di = fi[0][i * LD + j];
+            s.re[2][i * LD + j] += dr;
+            s.im[2][i * LD + j] += di;
+            xn += dr * dr + di * di;
           }
         }
     }
-    __syncthreads();
-    st = gj_inverse<T, NT, NMAX, L::RG>(s.re[2], s.im[2], LD, n, *s.gj, tid);
-    if (st == ST_OK && step == a.steps - 1) {
-      double dn = 0.0, gn = 0.0;
-      if (t.own) {
-#pragma unroll
-        for (int x = 0; x < TM; ++x)
-#pragma unroll
-          for (int y = 0; y < TM; ++y) {
-            const int i = t.i0 + x, j = t.j0 + y;
-            if (i < n && j < n) {
-              const double vr = s.re[2][i * LD + j], vi = s.im[2][i * LD + j];
-              dn += (vr - gr[x][y]) * (vr - gr[x][y]) + (vi - gi[x][y]) * (vi - gi[x][y]);
-              gn += vr * vr + vi * vi;
-            }
-          }
-      }
-      dn = block_sum<NT>(dn, s.sred, tid);
-      gn = block_sum<NT>(gn, s.sred, tid);
-      change = sqrt(dn / gn);
-    }
+    xn = block_sum<NT>(xn, s.sred, tid);
+    xrel = sqrt(xn / gn);
   }
-  if (st == ST_OK && !(change <= a.redo_tol)) st = ST_REDO;
-  if (st != ST_OK) {
-    if (tid == 0) a.status[inst] = st;
+  // drift from the FP32 decimation's g
+  double dn = 0.0, gn = 0.0;
+  if (!bad && t.own) {
+    const double2* g0 = reinterpret_cast<const double2*>(io);
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+      for (int y = 0; y < TM; ++y) {
+        const int i = t.i0 + x, j = t.j0 + y;
+        if (i < n && j < n) {
+          const double gr = s.re[2][i * LD + j], gi = s.im[2][i * LD + j];
+          const double2 v = g0[i * n + j];
+          dn += (gr - v.x) * (gr - v.x) + (gi - v.y) * (gi - v.y);
+          gn += gr * gr + gi * gi;
+        }
+      }
+  }
+  dn = block_sum<NT>(dn, s.sred, tid);
+  gn = block_sum<NT>(gn, s.sred, tid);
+  const bool ok = !bad && rres < a.rtol && xrel < a.xtol && sqrt(dn / gn) < a.dtol;
+  if (!ok) {
+    if (tid == 0) a.status[inst] = ST_REDO;
     return;
   }
   gstore<T, NT>(io, n, s.re[2], s.im[2], LD, tid);
@@ -1627,6 +1782,7 @@ __global__ void __launch_bounds__(256) k_heat_current(int n_w, const double* __r
 
 #include "big.hpp"
 #include "persist.hpp"
+#include "wpi.hpp"
 
 namespace negf {
 
@@ -1686,6 +1842,26 @@ static int default_nmax(int n, bool fp64) {
   if (fp64) return n <= BIG ? BIG : 0;
   return n <= 64 ? 64 : 0;
 }
+
+// One-instance-per-wavefront MFMA kernels (wpi.hpp) for blocks of up to 32 DOF.
+// Mode 1 (default) uses them where they measured faster than the workgroup classes on
+// MI355X (8192 instances, 16 layers, eta = 1e-6): FP64 up to 16 DOF (n = 12: decimation
+// 1.18x, sweep 1.45x) and the FP32 sweep up to 24 DOF (1.06x at 24); above that the
+// wave's register-resident Gauss-Jordan step (one wavefront per instance, no other
+// wave to hide its latency) loses to the workgroup form.  Mode 2 uses them for every
+// block up to 32 DOF, mode 0 never (negf_set_wave_kernels).
+constexpr int WAVE_MAX = 32;
+static std::atomic<int> g_wave_kernels{1};
+static bool use_wave(int n, bool fp64, bool sweep) {
+  if (n < 1 || n > WAVE_MAX) return false;
+  const int mode = g_wave_kernels.load();
+  if (mode == 2) return true;
+  if (mode != 1) return false;
+  if (fp64) return n <= 16;
+  return n <= (sweep ? 24 : 16);
+}
+// rows per lane of the wave inversion: n <= 8 R
+static int wave_rows(int n) { return n <= 16 ? 2 : (n <= 24 ? 3 : 4); }
 
 static int pick_nmax(int n, bool fp64) {
   if (n >= 1 && n <= FP64_MAX) {
@@ -1789,6 +1965,16 @@ static int run_sr(hipStream_t st, int nmax, const SrArgs& a) {
                 a);                                                                           \
     return NEGF_OK;                                                                           \
   }
+  if constexpr (!MIRROR) {
+    if (use_wave(a.n, sizeof(T) == 8, false)) {
+      const size_t lds = wpi::lds_bytes<T>(a.n, 4);
+      switch (wave_rows(a.n)) {
+        case 2: NEGF_LAUNCH((wpi::k_sr_wave<T, 2>), a.n_inst, WAVE, lds, st, a); return NEGF_OK;
+        case 3: NEGF_LAUNCH((wpi::k_sr_wave<T, 3>), a.n_inst, WAVE, lds, st, a); return NEGF_OK;
+        default: NEGF_LAUNCH((wpi::k_sr_wave<T, 4>), a.n_inst, WAVE, lds, st, a); return NEGF_OK;
+      }
+    }
+  }
   if (MIRROR && !sr_mirror_class(nmax))
     return fail(NEGF_ERR_ARG, "no mirrored decimation for block class %d", nmax);
   switch (nmax) {
@@ -1813,19 +1999,36 @@ static int run_sr(hipStream_t st, int nmax, const SrArgs& a) {
   return fail(NEGF_ERR_ARG, "unsupported block size class %d", nmax);
 }
 
+__global__ void k_mark_redo(int n_inst, int32_t* status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_inst && status[i] == ST_OK) status[i] = ST_REDO;
+}
+
+// LDS of k_refine: four FP64 and four FP32 planar matrices plus the inversion buffers
+template <int NM>
+static constexpr size_t refine_lds() {
+  using L = Lds<double, NM, Cfg<NM>::NT, 4>;
+  return ((L::bytes() + 15) & ~size_t(15)) + sizeof(float) * 8 * L::MSZ;
+}
+
 static int run_refine(hipStream_t st, int nmax, const RefineArgs& a) {
-#define RF_CASE(NM)                                                                          \
-  case NM: {                                                                                 \
-    constexpr int NT = Cfg<NM>::NT;                                                          \
-    NEGF_LAUNCH((k_refine<NM>), a.n_inst, NT, (Lds<double, NM, NT, 4>::bytes()), st, a);     \
-    return NEGF_OK;                                                                          \
+#define RF_CASE(NM)                                                                 \
+  case NM: {                                                                        \
+    constexpr int NT = Cfg<NM>::NT;                                                 \
+    NEGF_LAUNCH((k_refine<NM>), a.n_inst, NT, refine_lds<NM>(), st, a);             \
+    return NEGF_OK;                                                                 \
   }
   switch (nmax) {
     RF_CASE(8)
     RF_CASE(16)
     RF_CASE(24)
     RF_CASE(32)
-    RF_CASE(48)
+    case 48:
+      // the Newton step's eight matrices of 48 DOF exceed a CU's LDS: those instances
+      // are decimated in FP64 by the redo pass
+      hipLaunchKernelGGL(k_mark_redo, dim3((a.n_inst + 255) / 256), dim3(256), 0, st, a.n_inst,
+                         a.status);
+      return hip_check(hipGetLastError(), "k_mark_redo launch");
   }
 #undef RF_CASE
   return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 limit of 48", a.n);
@@ -1858,6 +2061,14 @@ static int run_rgf(hipStream_t st, int nmax, const RgfArgs& a) {
     constexpr int NT = Cfg<NM>::NT;                                                  \
     NEGF_LAUNCH((k_rgf<T, NM>), a.n_inst, NT, (Lds<T, NM, NT, 3>::bytes()), st, a);  \
     return NEGF_OK;                                                                  \
+  }
+  if (use_wave(a.n, sizeof(T) == 8, true)) {
+    const size_t lds = wpi::lds_bytes<T>(a.n, 3);
+    switch (wave_rows(a.n)) {
+      case 2: NEGF_LAUNCH((wpi::k_rgf_wave<T, 2>), a.n_inst, WAVE, lds, st, a); return NEGF_OK;
+      case 3: NEGF_LAUNCH((wpi::k_rgf_wave<T, 3>), a.n_inst, WAVE, lds, st, a); return NEGF_OK;
+      default: NEGF_LAUNCH((wpi::k_rgf_wave<T, 4>), a.n_inst, WAVE, lds, st, a); return NEGF_OK;
+    }
   }
   switch (nmax) {
     RG_CASE(8)
@@ -1975,6 +2186,11 @@ int negf_set_block_class(int precision, int n, int nmax) {
   return NEGF_OK;
 }
 
+int negf_set_wave_kernels(int mode) {
+  if (mode < 0 || mode > 2) return fail(NEGF_ERR_ARG, "wave-kernel mode %d is not 0, 1 or 2", mode), -1;
+  return g_wave_kernels.exchange(mode);
+}
+
 int negf_block_class(int precision, int n) {
   if (precision != NEGF_FP64 && precision != NEGF_MIXED) return 0;
   return pick_nmax(n, precision == NEGF_FP64);
@@ -2001,7 +2217,6 @@ int negf_sancho_rubio_batched(void* stream, int precision, int n, int n_inst, co
     if (n > MIXED_MAX)
       return fail(NEGF_ERR_ARG, "block size %d exceeds the mixed-precision limit of %d", n,
                   MIXED_MAX);
-    a.fp32_eta_ratio = FP32_MIN_ETA_RATIO;
     rc = run_sr<float>(st, nmax, a);
     if (rc == NEGF_OK && refine_steps >= 0) {
       rc = negf_surface_refine_batched(stream, n, n_inst, lead, set_of, omega, eta, refine_steps,
@@ -2038,7 +2253,9 @@ int negf_surface_refine_batched(void* stream, int n, int n_inst, const double* l
   const int nm64 = pick_nmax(n, true);
   if (!nm64 || nm64 > MIXED_MAX)
     return fail(NEGF_ERR_ARG, "block size %d exceeds the FP64 refinement limit of %d", n, MIXED_MAX);
-  RefineArgs r{n, n_inst, lead, set_of, omega, eta, steps, NEGF_REDO_TOL, g_io, dos_out, status};
+  RefineArgs r{n,          n_inst,           lead,           set_of,  omega, eta,
+               steps,      NEGF_NEWTON_RTOL, NEGF_NEWTON_XTOL, NEGF_NEWTON_DTOL, g_io, dos_out,
+               status};
   return run_refine(reinterpret_cast<hipStream_t>(stream), nm64, r);
 }
 
@@ -2237,7 +2454,6 @@ static int sr_mirrored(void* stream, const negf_transmission_problem* p, double*
            p->maxiter, g_l,       p->dos_l,  p->iters_l, st_l,     0,           g_r,
            p->dos_r, p->iters_r,  st_r};
   if (p->precision == NEGF_FP64) return run_sr<double, true>(st, pick_nmax(n, true), a);
-  a.fp32_eta_ratio = FP32_MIN_ETA_RATIO;
   int rc = run_sr<float, true>(st, pick_nmax(n, false), a);
   if (rc || p->refine_steps < 0) return rc;
   rc = negf_surface_refine_batched(stream, n, p->n_inst, p->lead_l, p->set_of, p->omega,
@@ -2247,7 +2463,6 @@ static int sr_mirrored(void* stream, const negf_transmission_problem* p, double*
                                    p->eta_lead, p->refine_steps, g_r, p->dos_r, st_r);
   if (rc) return rc;
   a.redo = 1;
-  a.fp32_eta_ratio = 0.0;
   return run_sr<double, true>(st, pick_nmax(n, true), a);
 }
 
@@ -2298,6 +2513,8 @@ int negf_transmission_batched(void* stream, const negf_transmission_problem* p, 
     return fail(NEGF_ERR_ARG, "eta ladders are solved by negf_transmission_multi_batched");
   if (p->share_leads < 0 || p->share_leads > NEGF_LEADS_MIRRORED)
     return fail(NEGF_ERR_ARG, "share_leads %d is not 0, 1 or 2", p->share_leads);
+  if (p->precision == NEGF_MIXED && p->refine_steps < 0)
+    return fail(NEGF_ERR_ARG, "mixed precision needs refine_steps >= 0 (got %d)", p->refine_steps);
   if (p->share_leads != NEGF_LEADS_SHARED && (!p->lead_r || !p->iters_r))
     return fail(NEGF_ERR_ARG, "null pointer argument (right lead)");
   const bool eig = p->surface_route == NEGF_SURFACE_EIGEN;
@@ -2342,7 +2559,8 @@ int negf_transmission_batched(void* stream, const negf_transmission_problem* p, 
   // merge lead status into st_l (left failure wins, as the reference evaluates the left lead first)
   rc = negf_merge_status(stream, p->n_inst, st_l, st_r);
   if (rc) return rc;
-  rc = negf_rgf_g1n_batched(stream, p->precision, p->n, p->n_layers, p->n_inst, p->onsite,
+  // the device sweep runs in FP64 in both precisions (see negf_transmission_batched)
+  rc = negf_rgf_g1n_batched(stream, NEGF_FP64, p->n, p->n_layers, p->n_inst, p->onsite,
                             p->coupling, p->set_of, p->omega, p->eta_dev, sig_l, sig_r, st_l, g1n,
                             p->status);
   if (rc) return rc;

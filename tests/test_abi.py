@@ -83,3 +83,15 @@ def test_multi_problem_entry_rejects_bad_arguments():
     assert lib.negf_transmission_multi_workspace_bytes(0, arr) == 0
     assert lib.negf_transmission_multi_workspace_bytes(1, arr) > \
         lib.negf_transmission_workspace_bytes(ctypes.byref(arr[0]))
+
+
+def test_wave_kernel_mode_switch():
+    """negf_set_wave_kernels takes modes 0, 1, 2 and returns the previous one; other
+    values are rejected with -1 (host-only, no GPU work)."""
+    lib = _lib.load_symbols_only()
+    prev = lib.negf_set_wave_kernels(2)
+    assert prev in (0, 1, 2)
+    assert lib.negf_set_wave_kernels(0) == 2
+    assert lib.negf_set_wave_kernels(7) == -1
+    assert b"mode" in lib.negf_last_error()
+    assert lib.negf_set_wave_kernels(prev) == 0

@@ -42,9 +42,10 @@ def _mirror_problem(seed, n, n_layers):
 
 def _run(sets, w, *, precision="fp64", maxiter=200, route=0, eta=1e-6):
     m = w.size
+    eta = np.full(m, 1e-6) if eta is None else np.array(np.broadcast_to(np.asarray(eta, float), (m,)))
     res = _engine.TransmissionSolver(sets, precision=precision, maxiter=maxiter,
                                      surface_route=route).run(
-        np.zeros(m, dtype=np.int32), w, np.full(m, eta), np.full(m, eta))
+        np.zeros(m, dtype=np.int32), w, eta, eta)
     return {k: getattr(res, k).cpu().numpy().copy()
             for k in ("t", "status", "iters_l", "iters_r", "dos_l", "dos_r")}
 
@@ -73,19 +74,25 @@ def test_mirrored_equals_independent_and_oracle(gpu, n):
 
 
 @pytest.mark.parametrize("n", [12, 24, 48])
-def test_mirrored_mixed_precision(gpu, n):
-    """FP32 decimation of both surfaces at once, each refined in FP64 against its own
-    lead; failures of either recomputed in FP64."""
+@pytest.mark.parametrize("eta_rel", [None, 1e-2])
+def test_mirrored_mixed_precision(gpu, n, eta_rel):
+    """FP32 decimation of both surfaces at once, each completed by the FP64 Newton step
+    against its own lead and certified; what either certificate rejects is decimated
+    again in FP64.  At eta = 1e-6 most instances go to the FP64 redo; at eta = 1e-2
+    omega the FP32 pass is certified (n <= 32; 48-DOF blocks always take the FP64
+    redo) -- both must meet the 1e-6 bar against FP64."""
     sets, wm, _ = _mirror_problem(70 + n, n, 4)
     ind = dataclasses.replace(sets, mirror_leads=False)
     w = np.linspace(0.04, 1.08, 31) * wm
-    ref = _run(sets, w)
-    mir, two = _run(sets, w, precision="mixed"), _run(ind, w, precision="mixed")
+    eta = None if eta_rel is None else eta_rel * w
+    ref = _run(sets, w, eta=eta)
+    mir, two = (_run(sets, w, precision="mixed", eta=eta),
+                _run(ind, w, precision="mixed", eta=eta))
     np.testing.assert_array_equal(mir["status"], ref["status"])
     np.testing.assert_array_equal(two["status"], ref["status"])
     for got in (mir, two):
         err = np.abs(got["t"] - ref["t"]) / np.maximum(np.abs(ref["t"]), 1e-3)
-        assert np.all(np.isfinite(got["t"])) and np.median(err) < 1e-6, np.median(err)
+        assert np.all(np.isfinite(got["t"])) and np.max(err) <= 1e-6, np.max(err)
 
 
 def test_mirrored_failures_and_eigen_fallback(gpu):

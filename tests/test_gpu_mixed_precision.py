@@ -1,0 +1,103 @@
+"""Mixed precision (FP32 decimation, FP64 Newton completion with FP32 Stein
+corrections and a certificate, FP64 decimation of what it does not certify; FP64
+sweep, self-energies, Gamma and trace) against the FP64 oracle at the north star's
+bar: <= 1e-6 relative error in T (relative to max(|T|, 1e-3)), DOS and kappa(300 K)."""
+
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from negfpy_amd.core import _engine, transmission_spectrum
+from negfpy_amd.core.negf import resolve_k
+from negfpy_amd.workloads import OMEGA_UNIT_RAD_S, ribbon_problem
+from oracle import negf_oracle as o
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "ribbon24_oracle.npz")
+BAR = 1e-6
+
+
+def _rel(a, b, floor=1e-3):
+    return np.abs(np.asarray(a) - np.asarray(b)) / np.maximum(np.abs(b), floor)
+
+
+def _ribbon_solver(p, precision):
+    kp = resolve_k(p.device, p.lead, p.lead, None, eta=p.eta, eta_device=None,
+                   device_to_lead_left=None, device_to_lead_right=None, contact_left_indices=None,
+                   contact_right_indices=None, omega_scale=None, max_block=64)
+    sets = _engine.TransportSets.from_arrays(kp.lead_l[None], kp.lead_r[None], kp.v_l[None],
+                                             kp.v_r[None], kp.pos_l, kp.pos_r, kp.onsite[None],
+                                             kp.coupling[None], True)
+    return _engine.TransmissionSolver(sets, precision=precision)
+
+
+def test_ribbon_mixed_meets_bar_at_every_frequency(gpu):
+    """All 4096 frequencies of the 24-DOF, 32-layer ribbon at eta = 1e-6, against the
+    oracle's T at every point (committed fixture), and DOS and kappa(300 K) against the
+    FP64 path; the FP32 decimation must serve part of the batch (certified)."""
+    gold = np.load(GOLD)
+    p = ribbon_problem(n_omega=4096)
+    assert np.array_equal(p.omegas, gold["omega"])
+    m = p.omegas.size
+    args = (torch.zeros(m, dtype=torch.int32, device="cuda"), torch.tensor(p.omegas, device="cuda"),
+            torch.full((m,), p.eta, device="cuda"))
+    stats = {}
+    mix = _ribbon_solver(p, "mixed").run_staged(*args, stats=stats)
+    f64 = _ribbon_solver(p, "fp64").run(*args)
+    assert int((mix.status != 0).sum()) == 0
+    t32, t64 = mix.t.cpu().numpy(), f64.t.cpu().numpy()
+    err = _rel(t32, gold["t"])
+    print(f"mixed vs oracle: max {err.max():.3e}, FP32-certified decimations "
+          f"{stats['fp32_certified_l']}/{stats['instances']}", flush=True)
+    assert err.max() <= BAR
+    assert _rel(t32, t64).max() <= BAR
+    d32, d64 = mix.dos_l.cpu().numpy(), f64.dos_l.cpu().numpy()
+    assert np.max(np.abs(d32 - d64) / np.maximum(np.abs(d64), 1e-3 * np.abs(d64).max())) <= BAR
+    w = p.omegas * OMEGA_UNIT_RAD_S
+    k32 = _engine.thermal_conductance(w, t32, [300.0])[0]
+    k64 = _engine.thermal_conductance(w, gold["t"], [300.0])[0]
+    assert abs(k32 - k64) <= BAR * abs(k64)
+    assert stats["fp32_certified_l"] >= 0.3 * m
+
+
+def test_mixed_serves_well_conditioned_points_in_fp32(gpu):
+    """At eta = 1e-2 omega the FP32 decimation plus Newton completion is certified for
+    nearly every instance (the FP32 path itself is what is checked, not the FP64 redo)."""
+    p = ribbon_problem(n_omega=4096)
+    sel = np.arange(8, 4096, 32)
+    w = p.omegas[sel]
+    eta = 1e-2 * w
+    m = w.size
+    args = (torch.zeros(m, dtype=torch.int32, device="cuda"), torch.tensor(w, device="cuda"),
+            torch.tensor(eta, device="cuda"))
+    stats = {}
+    mix = _ribbon_solver(p, "mixed").run_staged(*args, stats=stats)
+    f64 = _ribbon_solver(p, "fp64").run(*args)
+    assert stats["fp32_certified_l"] >= 0.9 * m, stats
+    assert _rel(mix.t.cpu().numpy(), f64.t.cpu().numpy()).max() <= BAR
+    lead = (p.lead.d00, p.lead.d01, None)
+    for k in range(0, m, 32):
+        ref = o.transmission_sparse(w[k], p.device.onsite_blocks, p.device.coupling_blocks, lead,
+                                    lead, eta=eta[k])
+        assert _rel(mix.t.cpu().numpy()[k], ref) <= BAR
+
+
+def test_ribbon_fp64_parity_at_every_frequency(gpu):
+    """FP64 path at all 4096 ribbon frequencies against the oracle fixture: within
+    1e-10 relative, or within 10x the oracle's own ~1-ulp input spread where the point
+    is conditioned worse than that; the exceedances of 1e-10 are counted and printed."""
+    gold = np.load(GOLD)
+    p = ribbon_problem(n_omega=4096)
+    t = transmission_spectrum(p.omegas, p.device, p.lead, p.lead, eta=p.eta)
+    tr, sp = gold["t"], gold["spread"]
+    scale = np.maximum(np.abs(tr), 1e-3)
+    dev = np.abs(t - tr)
+    above = dev > 1e-10 * scale
+    ratio = dev[above] / np.maximum(sp[above], 1e-300)
+    print(f"FP64 vs oracle: max rel {np.max(dev / scale):.3e}; {int(above.sum())} of {t.size} "
+          f"points above 1e-10, max |dT| / spread there {ratio.max() if ratio.size else 0:.2f}",
+          flush=True)
+    assert np.all(dev <= np.maximum(1e-10 * scale, 10.0 * sp))
+    assert np.all(t > -1e-9)

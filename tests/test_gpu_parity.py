@@ -259,7 +259,9 @@ def test_ribbon_workload_parity(gpu):
 
 
 def test_mixed_precision_error_bound(gpu):
-    """Mixed mode (FP32 decimation + sweep, FP64 refinement and trace) against FP64."""
+    """Mixed mode (certified FP32 decimation, FP64 sweep and trace) against FP64 on an
+    eighth of the ribbon: <= 1e-6 relative at every point (every frequency:
+    tests/test_gpu_mixed_precision.py)."""
     p = ribbon_problem(n_omega=4096)
     sel = np.arange(0, 4096, 8)
     t64 = transmission_spectrum(p.omegas[sel], p.device, p.lead, p.lead, eta=p.eta, precision="fp64")
@@ -267,10 +269,8 @@ def test_mixed_precision_error_bound(gpu):
                                 precision="mixed")
     err = np.abs(t32 - t64) / np.maximum(np.abs(t64), 1e-3)
     print(f"mixed vs fp64: max {err.max():.3e} median {np.median(err):.3e}", flush=True)
-    # FP32 sweeps through disordered layers lose digits (documented); failures of the
-    # FP32 kernels are recomputed in FP64, so every instance returns a finite T
     assert np.all(np.isfinite(t32))
-    assert np.median(err) < 1e-4
+    assert err.max() <= 1e-6
 
 
 @pytest.mark.parametrize("case", ["chain", "multidof", "contacts", "merged"])

@@ -26,6 +26,17 @@ from parity_util import assert_parity, oracle_t
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True)
+def classic_kernels(gpu):
+    """The persistent launch runs the workgroup-per-instance stage bodies, so the staged
+    launches it is compared with bit for bit are that family too (the one-wave MFMA
+    kernels agree with them to rounding only; tests/test_gpu_wave_kernels.py)."""
+    lib = _lib.load()
+    prev = lib.negf_set_wave_kernels(0)
+    yield
+    lib.negf_set_wave_kernels(prev)
+
+
 def _device(seed, n, n_layers):
     d00, d01 = random_lead(seed, n)
     rng = np.random.default_rng(seed + 7)
