@@ -42,7 +42,7 @@ def test_ribbon_mixed_meets_bar_at_every_frequency(gpu):
     assert np.array_equal(p.omegas, gold["omega"])
     m = p.omegas.size
     args = (torch.zeros(m, dtype=torch.int32, device="cuda"), torch.tensor(p.omegas, device="cuda"),
-            torch.full((m,), p.eta, device="cuda"))
+            torch.full((m,), p.eta, dtype=torch.float64, device="cuda"))
     stats = {}
     mix = _ribbon_solver(p, "mixed").run_staged(*args, stats=stats)
     f64 = _ribbon_solver(p, "fp64").run(*args)

@@ -100,7 +100,7 @@ def test_wave_transmission_matches_oracle_and_classic(wave_mode, n, n_layers):
     omegas = np.linspace(0.04, 1.05, 12) * wmax
     m = omegas.size
     args = (torch.zeros(m, dtype=torch.int32, device="cuda"),
-            torch.tensor(omegas, device="cuda"), torch.full((m,), 1e-6, device="cuda"))
+            torch.tensor(omegas, device="cuda"), torch.full((m,), 1e-6, dtype=torch.float64, device="cuda"))
     t_wave = _engine.TransmissionSolver(sets).run(*args).t.cpu().numpy()
     wave_mode(0)
     t_classic = _engine.TransmissionSolver(sets).run(*args).t.cpu().numpy()
