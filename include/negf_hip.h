@@ -305,6 +305,25 @@ size_t negf_transmission_workspace_bytes(const negf_transmission_problem* p);
 int negf_transmission_batched(void* stream, const negf_transmission_problem* p, void* workspace,
                               size_t workspace_bytes);
 
+/* Scratch of negf_transmission_batched_noalloc per GPU: 64 MiB. */
+#define NEGF_NOALLOC_SCRATCH_BYTES (64 << 20)
+
+/*
+ * negf_transmission_batched without a workspace argument and without allocating:
+ * no hipMalloc, hipFree or allocating library call, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * it).  No counterpart in the reference.  The workspace is a file-scope device array
+ * of NEGF_NOALLOC_SCRATCH_BYTES on each GPU, held from the library's load on that
+ * device until the process ends; its address is looked up (hipGetSymbolAddress) on
+ * the first call on a device.  Limits: negf_transmission_workspace_bytes(p) must not
+ * exceed the scratch (24-DOF blocks: up to 1,800 instances per call; split larger
+ * batches); blocks up to 48 DOF; decimation route only.  Errors are NEGF_ERR_ARG with
+ * a message.  Results equal negf_transmission_batched's.  Every call on a GPU uses
+ * that GPU's one array: calls that overlap on one GPU (other streams, other threads)
+ * race and must be ordered by the caller.
+ */
+int negf_transmission_batched_noalloc(void* stream, const negf_transmission_problem* p);
+
 /* Problems per negf_transmission_multi_batched call. */
 #define NEGF_MULTI_MAX_PROBLEMS 8
 

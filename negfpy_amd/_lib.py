@@ -29,6 +29,7 @@ LEADS_INDEPENDENT = 0   # NEGF_LEADS_*: negf_transmission_problem.share_leads
 LEADS_SHARED = 1
 LEADS_MIRRORED = 2
 MULTI_MAX_PROBLEMS = 8   # NEGF_MULTI_MAX_PROBLEMS
+NOALLOC_SCRATCH_BYTES = 64 << 20   # NEGF_NOALLOC_SCRATCH_BYTES
 
 OK = 0
 ST_OK = 0
@@ -107,6 +108,7 @@ _SIGS = {
         [_vp, _c_int, ctypes.POINTER(TransmissionProblem), _vp, ctypes.c_size_t], _c_int),
     "negf_transmission_batched": (
         [_vp, ctypes.POINTER(TransmissionProblem), _vp, ctypes.c_size_t], _c_int),
+    "negf_transmission_batched_noalloc": ([_vp, ctypes.POINTER(TransmissionProblem)], _c_int),
     "negf_thermal_conductance": ([_vp, _c_int, _vp, _vp, _c_int, _vp, _vp], _c_int),
     "negf_heat_current": ([_vp, _c_int, _vp, _vp, _c_dbl, _c_dbl, _vp], _c_int),
     "negf_conductance_weights": ([_vp, _c_int, _vp, _c_int, _vp, _vp], _c_int),

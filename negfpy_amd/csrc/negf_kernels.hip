@@ -2569,6 +2569,69 @@ int negf_transmission_batched(void* stream, const negf_transmission_problem* p, 
 }
 
 // ---------------------------------------------------------------------------
+// allocation-free transmission: the workspace is a file-scope device array
+// ---------------------------------------------------------------------------
+//
+// negf_transmission_batched_noalloc runs exactly negf_transmission_batched with the
+// workspace carved from `scratch` (NEGF_NOALLOC_SCRATCH_BYTES per GPU, held from the
+// library's load on that device until the process ends) instead of a caller buffer.
+// `scratch` names a host-side stand-in in host code; its device address on each GPU
+// comes from hipGetSymbolAddress, looked up once per device on the first call with
+// that device current and kept in g_scratch_ptr.  Blocks that need the 96-class
+// device-memory pool (more than 48 DOF) or the eigenmode route (also pooled) are
+// rejected, because that pool is allocated on first use.  Nothing in the workspace is
+// read before the same call writes it, so no clearing is needed.  Calls that overlap
+// on one GPU (different streams or threads) race on the array.
+static __device__ __align__(256) unsigned char scratch[NEGF_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64];
+
+static int scratch_on_current_device(void** out) {
+  int dev = 0;
+  int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  if (dev < 0 || dev >= 64) return fail(NEGF_ERR_ARG, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    rc = hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (rc) return rc;
+    if (!ptr) return fail(NEGF_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return NEGF_OK;
+}
+
+int negf_transmission_batched_noalloc(void* stream, const negf_transmission_problem* p) {
+  if (!p) return fail(NEGF_ERR_ARG, "null problem");
+  if (p->n_inst == 0) return NEGF_OK;
+  const size_t need = negf_transmission_workspace_bytes(p);
+  if (need > (size_t)NEGF_NOALLOC_SCRATCH_BYTES)
+    return fail(NEGF_ERR_ARG,
+                "problem needs %zu workspace bytes, above the %d-byte scratch of the "
+                "allocation-free entry point: split the instances into smaller calls",
+                need, NEGF_NOALLOC_SCRATCH_BYTES);
+  if (p->surface_route != NEGF_SURFACE_DECIMATION)
+    return fail(NEGF_ERR_ARG, "the allocation-free entry point takes the decimation route only");
+  // every FP64 class the staged path picks (leads, self-energies, sweep, trace) must be
+  // an LDS class: the 96 class (also reachable through negf_set_block_class) is pooled
+  const int sizes[5] = {p->n, p->n_lead_l, p->n_lead_r, p->n_lead_l > p->n ? p->n_lead_l : p->n,
+                        p->n_lead_r > p->n ? p->n_lead_r : p->n};
+  for (int q = 0; q < 5; ++q) {
+    const int c = pick_nmax(sizes[q], true);
+    if (sizes[q] < 1 || c == 0 || c > MIXED_MAX)
+      return fail(NEGF_ERR_ARG,
+                  "blocks of %d DOF run in block class %d, which uses the device-memory pool the "
+                  "allocation-free entry point does not (limit: class %d)", sizes[q], c, MIXED_MAX);
+  }
+  void* ws = nullptr;
+  const int rc = scratch_on_current_device(&ws);
+  if (rc) return rc;
+  return negf_transmission_batched(stream, p, ws, (size_t)NEGF_NOALLOC_SCRATCH_BYTES);
+}
+
+// ---------------------------------------------------------------------------
 // several problems of different block sizes in one persistent launch
 // ---------------------------------------------------------------------------
 

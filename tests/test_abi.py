@@ -95,3 +95,21 @@ def test_wave_kernel_mode_switch():
     assert lib.negf_set_wave_kernels(7) == -1
     assert b"mode" in lib.negf_last_error()
     assert lib.negf_set_wave_kernels(prev) == 0
+
+
+def test_noalloc_entry_rejects_without_touching_a_gpu():
+    """The allocation-free entry point validates its problem before any HIP call."""
+    import ctypes
+
+    lib = _lib.load_symbols_only()
+    assert lib.negf_transmission_batched_noalloc(None, None) == 1
+    p = _lib.TransmissionProblem()
+    p.n = p.n_lead_l = p.n_lead_r = p.n_c_l = p.n_c_r = 24
+    p.n_layers = 32
+    p.n_inst = 1 << 20          # far beyond the 64 MiB scratch
+    assert lib.negf_transmission_batched_noalloc(None, ctypes.byref(p)) == 1
+    assert b"scratch" in lib.negf_last_error()
+    p.n_inst = 4
+    p.n = p.n_lead_l = p.n_lead_r = p.n_c_l = p.n_c_r = 60   # pooled 96 class
+    assert lib.negf_transmission_batched_noalloc(None, ctypes.byref(p)) == 1
+    assert b"pool" in lib.negf_last_error()
