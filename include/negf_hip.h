@@ -13,6 +13,8 @@
  * set_of[].  Per-instance outcomes are reported in int32 status arrays:
  *   0 ok, 1 decimation did not converge within maxiter, 2 non-finite values,
  *   3 exactly singular pivot in a lead inverse,
+ *   4 eigenmode route at eta = 0 where the decimation fails (propagating modes on the
+ *     unit circle): the reference's choice among them is not reproduced,
  *   7 (transient, mixed precision only) marked for FP64 recomputation,
  *   12 / 13 non-finite / singular in the device sweep.
  *

@@ -36,6 +36,7 @@ ST_OK = 0
 ST_NOCONV = 1
 ST_NONFINITE = 2
 ST_SINGULAR = 3
+ST_EIGEN_ZERO_ETA = 4
 ST_DEV_NONFINITE = 12
 ST_DEV_SINGULAR = 13
 

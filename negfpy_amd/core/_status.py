@@ -16,6 +16,10 @@ from negfpy_amd import _lib
 MSG_NOCONV = "Sancho-Rubio decimation did not converge within maxiter."
 MSG_NONFINITE = ("Sancho-Rubio decimation produced non-finite values. "
                  "Try a larger eta or a different surface_gf method.")
+MSG_EIGEN_ZERO_ETA = (
+    "eigenmode surface GF at eta = 0 inside a band is not supported on the GPU: the "
+    "propagating modes lie on the unit circle and the reference keeps n of them in the "
+    "order its dense eigensolver returns them (surface_gf.py:118-124); use eta > 0")
 
 
 def raise_for_lead_status(code: int) -> None:
@@ -27,12 +31,14 @@ def raise_for_lead_status(code: int) -> None:
         raise RuntimeError(MSG_NONFINITE)
     if code == _lib.ST_SINGULAR:
         raise np.linalg.LinAlgError("Singular matrix")
+    if code == _lib.ST_EIGEN_ZERO_ETA:
+        raise NotImplementedError(MSG_EIGEN_ZERO_ETA)
     raise RuntimeError(f"unexpected lead status {code}")
 
 
 def transmission_value(code: int, t: float) -> float:
     """Return T for one instance or raise the reference's exception for its failure."""
-    if code in (_lib.ST_NOCONV, _lib.ST_NONFINITE, _lib.ST_SINGULAR):
+    if code in (_lib.ST_NOCONV, _lib.ST_NONFINITE, _lib.ST_SINGULAR, _lib.ST_EIGEN_ZERO_ETA):
         raise_for_lead_status(code)
     if code == _lib.ST_DEV_SINGULAR:
         raise RuntimeError("Factor is exactly singular")

@@ -104,8 +104,7 @@ class _TransmissionFn(torch.autograd.Function):
         weight = grad_t.detach().to(torch.float64).contiguous()
         nl, n = int(ons.shape[0]), int(ons.shape[1])
         n_inst = int(static.omega.numel())
-        # per instance: (2L - 1) workspace blocks and as many gradient blocks
-        per_inst = 2 * (2 * nl - 1) * n * n * 16
+        per_inst = _adjoint_bytes_per_instance(sets, nl, n)
         chunk = max(1, min(n_inst, ADJOINT_CHUNK_BYTES // per_inst))
         g_ons = torch.zeros_like(ons)
         g_cpl = torch.zeros((max(nl - 1, 1), n, n), dtype=ons.dtype, device=ons.device)
@@ -161,6 +160,15 @@ def transmission_torch(omega, onsite, coupling, lead_left, lead_right, *, eta: f
     return _TransmissionFn.apply(ons, cpl, _Static(kp, w, eta, eta_dev))
 
 
+def _adjoint_bytes_per_instance(sets, nl: int, n: int) -> int:
+    """Device bytes one instance of an adjoint chunk holds: the kernel's (2L - 1) stored
+    sweep blocks, as many per-instance gradient blocks, and the lead surface GFs and
+    self-energies of _engine.adjoint (complex128)."""
+    n_l = int(sets.lead_l.shape[2])
+    n_r = int(sets.lead_r.shape[2])
+    return (2 * (2 * nl - 1) * n * n + n_l * n_l + n_r * n_r + 2 * n * n) * 16
+
+
 def _chunk_instances(per_inst: int, n_inst: int, dev) -> int:
     """Instances per backward launch: the per-instance adjoint memory within
     ADJOINT_CHUNK_BYTES, or within ADJOINT_FREE_FRACTION of the free device memory if
@@ -194,7 +202,7 @@ class _SetsTransmissionFn(torch.autograd.Function):
         weight = grad_t.detach().to(torch.float64).contiguous()
         nl, n = int(ons.shape[1]), int(ons.shape[2])
         n_inst = int(static.omega.numel())
-        per_inst = 2 * (2 * nl - 1) * n * n * 16
+        per_inst = _adjoint_bytes_per_instance(sets, nl, n)
         chunk = _chunk_instances(per_inst, n_inst, ons.device)
         g_ons = torch.zeros_like(ons)
         g_cpl = torch.zeros((ons.shape[0], max(nl - 1, 1), n, n), dtype=ons.dtype,
@@ -285,7 +293,7 @@ class _SetsSumFn(torch.autograd.Function):
         sets = static.sets(ons, cpl)
         nl, n = int(ons.shape[1]), int(ons.shape[2])
         n_inst = int(static.omega.numel())
-        per_inst = 2 * (2 * nl - 1) * n * n * 16
+        per_inst = _adjoint_bytes_per_instance(sets, nl, n)
         chunk = _chunk_instances(per_inst, n_inst, ons.device)
         g_ons = torch.zeros_like(ons)
         g_cpl = torch.zeros((ons.shape[0], max(nl - 1, 1), n, n), dtype=ons.dtype,

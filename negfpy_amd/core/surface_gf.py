@@ -13,10 +13,13 @@ solution of g = (z - d00 - d01 g d10)^-1, which is the fixed point decimation
 converges to; those entry points therefore keep their argument checks, take the
 decimation result, and where decimation fails take the eigenmode kernel's
 (``negf_surface_gf_eigen_batched``: the subspace of the n smallest-|lambda| modes
-from a spectral projector).  Not reproduced: eta = 0 inside the band, where
-modes of modulus 1 sit on both sides of the n-th and the reference keeps some of
-them in the order LAPACK's rounding gives them, and the rank/SVD-cutoff
-handling of degenerate mode sets.
+from a spectral projector).  Not reproduced, and reported as such: eta = 0
+inside the band, where modes of modulus 1 sit on both sides of the n-th and the
+reference keeps some of them in the order LAPACK's rounding gives them
+(NotImplementedError, kernel status 4); legacy_eigen_svd where a decaying mode
+lies within unit_circle_tol of the unit circle, which the reference then drops
+(NotImplementedError); and the
+SVD cutoff of rank-deficient mode sets (an exactly singular pivot instead).
 """
 
 from __future__ import annotations
@@ -116,7 +119,15 @@ def _check_square(d00, d01):
 def surface_gf_generalized_eigen(omega: float, d00: Array, d01: Array, d10: Array | None = None,
                                  eta: float = 1e-8, unit_circle_tol: float = 1e-10, *,
                                  precision: str | None = None) -> Array:
-    """Eigenmode route (surface_gf.py:84-133); see the module docstring."""
+    """Eigenmode route (surface_gf.py:84-133); see the module docstring.
+
+    ``unit_circle_tol``: the reference keeps the modes with |lambda| < 1 - tol and, when
+    fewer than n are left, the n of smallest |lambda| (:118-124).  For eta > 0 exactly
+    n of the 2n modes lie inside the unit circle and they are the n smallest, so the
+    selection -- and the result, the decaying (retarded) solution computed here -- is the
+    same for every tol.  At eta = 0 inside a band the modes on the unit circle are chosen
+    in the order the dense eigensolver returns them; that case raises
+    NotImplementedError."""
     _check_square(d00, d01)
     return _single(omega, d00, d01, d10, eta, 1e-12, 200, precision, _EIGEN)
 
@@ -125,7 +136,14 @@ def surface_gf_generalized_eigen_svd(omega: float, d00: Array, d01: Array,
                                      d10: Array | None = None, eta: float = 1e-8,
                                      unit_circle_tol: float = 1e-10, svd_rcond: float = 1e-12, *,
                                      precision: str | None = None) -> Array:
-    """Eigenmode route with SVD pseudo-inverse (surface_gf.py:136-185)."""
+    """Eigenmode route with SVD pseudo-inverse (surface_gf.py:136-185).
+
+    Mode selection as :func:`surface_gf_generalized_eigen`.  ``svd_rcond`` sets the
+    reference's pseudo-inverse cutoff (:182); for a mode matrix whose condition number
+    is below 1 / svd_rcond (every lead of the tests and studies) the pseudo-inverse is
+    the inverse and the result the decaying solution computed here.  A mode matrix
+    beyond that (rank-deficient modes) ends as an exactly singular pivot on the GPU
+    (numpy.linalg.LinAlgError) rather than the reference's truncated pseudo-inverse."""
     _check_square(d00, d01)
     if svd_rcond <= 0.0:
         raise ValueError("svd_rcond must be positive.")
@@ -135,9 +153,25 @@ def surface_gf_generalized_eigen_svd(omega: float, d00: Array, d01: Array,
 def surface_gf_legacy_eigen_svd(omega: float, d00: Array, d01: Array, d10: Array | None = None,
                                 eta: float = 1e-8, unit_circle_tol: float = 0.0, *,
                                 precision: str | None = None) -> Array:
-    """Legacy Fortran-style eigenmode route (surface_gf.py:201-251)."""
+    """Legacy Fortran-style eigenmode route (surface_gf.py:201-251).
+
+    The reference keeps every mode with |lambda| < 1 - unit_circle_tol (:218-223).  When
+    those are the n decaying modes -- for eta > 0 unless a decaying mode lies within
+    unit_circle_tol of the unit circle (1 - |lambda| is of order eta / group velocity) --
+    its result is the decaying solution computed here.  The decaying modes' lambda are
+    the eigenvalues of g d10 (the transfer matrix); if one of them is within the tolerance
+    the reference would drop it and work with fewer than n modes and a pseudo-inverse,
+    which is not reproduced: NotImplementedError."""
     _check_square(d00, d01)
-    return _single(omega, d00, d01, d10, eta, 1e-12, 200, precision, _EIGEN)
+    g = _single(omega, d00, d01, d10, eta, 1e-12, 200, precision, _EIGEN)
+    if unit_circle_tol > 0.0:
+        lam = np.linalg.eigvals(g @ resolve_d10(np.asarray(d01), d10))
+        if np.any(1.0 - np.abs(lam) < unit_circle_tol):
+            raise NotImplementedError(
+                "legacy_eigen_svd: a decaying mode lies within unit_circle_tol of the unit "
+                "circle, so the reference keeps fewer than n modes (pseudo-inverse); the GPU "
+                "route computes the solution with all n decaying modes")
+    return g
 
 
 def surface_gf(omega: float, d00: Array, d01: Array, d10: Array | None = None, eta: float = 1e-8,

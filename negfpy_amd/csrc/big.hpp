@@ -659,6 +659,12 @@ __device__ int big_count_inside(double r, const BigMat& A, const BigMat& B, cons
 
 __device__ void big_eig_one(const EigArgs& a, int inst, double* slot, BigShared& sh, int tid) {
   if (a.status[inst] == ST_OK) return;
+  // eta = 0 with the decimation failing: modes on the unit circle, whose selection the
+  // reference leaves to LAPACK's ordering (surface_gf.py:118-124) -- not reproduced
+  if (a.eta[inst] == 0.0) {
+    if (tid == 0) a.status[inst] = ST_EIGEN_ZERO_ETA;
+    return;
+  }
   const int n = a.n, m = 2 * n;
   const size_t nn = (size_t)n * n;
   const double* blk = a.lead + (size_t)a.set_of[inst] * 3 * nn * 2;
@@ -992,6 +998,12 @@ __device__ int huge_count_inside(double r, const Blk4& A, const BigMat& D1s, con
 
 __device__ void huge_eig_one(const EigArgs& a, int inst, double* slot, BigShared& sh, int tid) {
   if (a.status[inst] == ST_OK) return;
+  // eta = 0 with the decimation failing: modes on the unit circle, whose selection the
+  // reference leaves to LAPACK's ordering (surface_gf.py:118-124) -- not reproduced
+  if (a.eta[inst] == 0.0) {
+    if (tid == 0) a.status[inst] = ST_EIGEN_ZERO_ETA;
+    return;
+  }
   const int n = a.n;
   const size_t nn = (size_t)n * n;
   const double* blk = a.lead + (size_t)a.set_of[inst] * 3 * nn * 2;

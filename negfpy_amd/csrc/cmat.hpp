@@ -207,7 +207,8 @@ __device__ __forceinline__ void crecip(T ar, T ai, T& rr, T& ri) {
 }
 
 // Status codes shared with the host (see include/negf_hip.h).
-enum Status { ST_OK = 0, ST_NOCONV = 1, ST_NONFINITE = 2, ST_SINGULAR = 3, ST_REDO = 7 };
+enum Status { ST_OK = 0, ST_NOCONV = 1, ST_NONFINITE = 2, ST_SINGULAR = 3, ST_EIGEN_ZERO_ETA = 4,
+              ST_REDO = 7 };
 
 // 32-bit DPP lane moves used by the pivot search (row_mask = bank_mask = 0xF).
 template <int CTRL>

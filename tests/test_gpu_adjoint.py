@@ -250,7 +250,7 @@ def test_conductance_weights_give_the_kernel_kappa(gpu):
     k2 = thermal_conductance_torch(w, t.detach() + torch.as_tensor(dt, device="cuda"), temps)
     lhs = float((k2 - k.detach()).sum())
     rhs = float((t.grad.cpu() * torch.as_tensor(dt)).sum())
-    assert abs(lhs - rhs) <= 1e-10 * abs(float(k.sum()))
+    assert abs(lhs - rhs) <= 1e-10 * abs(float(k.detach().sum()))
 
 
 def test_transmission_torch_argument_errors(gpu):

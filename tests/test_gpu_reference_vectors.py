@@ -80,13 +80,36 @@ def test_surface_gf_matches_reference(gpu, method):
         assert np.max(np.abs(g - g_ref)) <= 1e-9 * np.max(np.abs(g_ref)), w
 
 
-@pytest.mark.skip(reason="eta = 0 inside the band: the pencil has six eigenvalues of modulus 1 "
-                         "(to rounding) between three decaying and three growing ones, and the "
-                         "reference's n-smallest-|lambda| fallback (surface_gf.py:123-124) keeps "
-                         "three of the six as LAPACK's rounding orders them; no spectral "
-                         "projector (or other eigensolver) reproduces that choice")
-def test_eigen_fallback_at_zero_eta(gpu):
-    pass
+@pytest.mark.parametrize("method", ["generalized_eigen", "generalized_eigen_svd",
+                                    "legacy_eigen_svd"])
+def test_eigen_fallback_at_zero_eta_is_refused(gpu, method):
+    """eta = 0 inside the band: the pencil has eigenvalues of modulus 1 (to rounding)
+    between the decaying and the growing ones, and the reference's n-smallest-|lambda|
+    fallback (surface_gf.py:123-124) keeps some of them in the order LAPACK's rounding
+    gives them -- a choice no spectral projector reproduces.  The GPU route refuses that
+    case with NotImplementedError instead of returning another solution; the reference's
+    own vector for it is therefore not compared.  Out of the band (all modes evanescent)
+    eta = 0 decimates normally."""
+    d00, d01 = REF["md_d00"], REF["md_d01"]
+    w_in = float(REF["sg_omegas"][1])  # half the band top: the reference's sg0_* point
+    with pytest.raises(NotImplementedError, match="eta = 0"):
+        surface_gf(w_in, d00, d01, eta=0.0, method=method)
+    w_out = float(REF["sg_omegas"][3])  # 1.3 x the band top
+    g = surface_gf(w_out, d00, d01, eta=0.0, method=method)
+    g_ref = o.sancho_rubio(w_out, d00, d01, None, 0.0)
+    assert np.max(np.abs(g - g_ref)) <= 1e-10 * np.max(np.abs(g_ref))
+
+
+def test_legacy_route_refuses_dropped_modes(gpu):
+    """At eta = 1e-7 the propagating modes have 1 - |lambda| ~ 1e-7: a tolerance of 1e-3
+    makes the reference drop them; the default 1e-10 of the dispatcher keeps all n."""
+    w = float(REF["sg_omegas"][1])
+    with pytest.raises(NotImplementedError, match="unit_circle_tol"):
+        surface_gf(w, REF["md_d00"], REF["md_d01"], eta=1e-7, method="legacy_eigen_svd",
+                   unit_circle_tol=1e-3)
+    g = surface_gf(w, REF["md_d00"], REF["md_d01"], eta=1e-7, method="legacy_eigen_svd")
+    g_ref = REF["sg_legacy_eigen_svd"][1]
+    assert np.max(np.abs(g - g_ref)) <= 1e-9 * np.max(np.abs(g_ref))
 
 
 def test_dos_matches_reference(gpu):
