@@ -32,6 +32,7 @@
 #include <mutex>
 
 #include "cmat.hpp"
+#include "wave.hpp"
 #include "../../include/negf_hip.h"
 
 namespace negf {
