@@ -41,6 +41,11 @@ extern "C" {
 #define NEGF_NEWTON_RTOL 1e-13
 #define NEGF_NEWTON_XTOL 1e-8
 #define NEGF_NEWTON_DTOL 1e-4
+/* Mixed precision: iteration limit of the FP32 decimation (at most maxiter).  An instance
+ * that has not converged after this many doublings (a chain of 2^16 layers) has a mode
+ * with |lambda| above about 1 - 4e-4 (tol 1e-12), which FP32 cannot resolve to the certificate: it goes
+ * to the FP64 decimation (with the caller's maxiter) without further FP32 work. */
+#define NEGF_MIXED_FP32_MAXITER 16
 
 enum { NEGF_FP64 = 0, NEGF_MIXED = 1 };
 enum { NEGF_SURFACE_DECIMATION = 0, NEGF_SURFACE_EIGEN = 1 };
@@ -108,7 +113,8 @@ int negf_set_wave_kernels(int mode);
  *                              the FP32 decimation (negf_surface_refine_batched, then
  *                              negf_sancho_rubio_redo for what it does not certify); < 0
  *                              leaves the FP32 decimation's eps_s in g_out (complex128) for
- *                              separate calls
+ *                              separate calls.  The FP32 pass stops after
+ *                              min(maxiter, NEGF_MIXED_FP32_MAXITER) iterations
  *   g_out   [n_inst][n][n]     g_s = (z - eps_s)^-1
  *   dos_out [n_inst] or NULL   (omega/pi) Re Tr[i (g - g^H)]
  *   iters, status [n_inst]

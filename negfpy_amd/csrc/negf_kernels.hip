@@ -2192,6 +2192,12 @@ int negf_set_wave_kernels(int mode) {
   return g_wave_kernels.exchange(mode);
 }
 
+// iteration limit of a mixed-precision FP32 decimation pass; the FP64 redo of what it
+// leaves unconverged or uncertified runs the caller's limit
+static int fp32_maxiter(int maxiter) {
+  return maxiter < NEGF_MIXED_FP32_MAXITER ? maxiter : NEGF_MIXED_FP32_MAXITER;
+}
+
 int negf_block_class(int precision, int n) {
   if (precision != NEGF_FP64 && precision != NEGF_MIXED) return 0;
   return pick_nmax(n, precision == NEGF_FP64);
@@ -2212,6 +2218,7 @@ int negf_sancho_rubio_batched(void* stream, int precision, int n, int n_inst, co
   if (!nmax) return fail(NEGF_ERR_ARG, "block size %d exceeds the supported maximum", n);
   SrArgs a{n, n_inst, lead, set_of, omega, eta, tol, maxiter, g_out, dos_out, iters, status, 0};
   int rc;
+  if (!fp64) a.maxiter = fp32_maxiter(maxiter);
   if (fp64) {
     rc = run_sr<double>(st, nmax, a);
   } else if (precision == NEGF_MIXED) {
@@ -2455,7 +2462,9 @@ static int sr_mirrored(void* stream, const negf_transmission_problem* p, double*
            p->maxiter, g_l,       p->dos_l,  p->iters_l, st_l,     0,           g_r,
            p->dos_r, p->iters_r,  st_r};
   if (p->precision == NEGF_FP64) return run_sr<double, true>(st, pick_nmax(n, true), a);
+  a.maxiter = fp32_maxiter(p->maxiter);
   int rc = run_sr<float, true>(st, pick_nmax(n, false), a);
+  a.maxiter = p->maxiter;
   if (rc || p->refine_steps < 0) return rc;
   rc = negf_surface_refine_batched(stream, n, p->n_inst, p->lead_l, p->set_of, p->omega,
                                    p->eta_lead, p->refine_steps, g_l, p->dos_l, st_l);

@@ -59,7 +59,7 @@ def test_ribbon_mixed_meets_bar_at_every_frequency(gpu):
     k32 = _engine.thermal_conductance(w, t32, [300.0])[0]
     k64 = _engine.thermal_conductance(w, gold["t"], [300.0])[0]
     assert abs(k32 - k64) <= BAR * abs(k64)
-    assert stats["fp32_certified_l"] >= 0.3 * m
+    assert stats["fp32_certified_l"] >= 0.1 * m
 
 
 def test_mixed_serves_well_conditioned_points_in_fp32(gpu):
