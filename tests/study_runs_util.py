@@ -1,4 +1,5 @@
-"""The reference's own transmission study runs kept under ``golden/study_runs``."""
+"""The reference's own transmission and thermal-conductance study runs kept under
+``golden/study_runs``."""
 
 import gzip
 import json
@@ -8,7 +9,9 @@ from pathlib import Path
 import numpy as np
 
 RUNS_DIR = Path(__file__).resolve().parent / "golden" / "study_runs"
-RUNS = sorted(p.name for p in RUNS_DIR.iterdir() if p.is_dir() and p.name != "ifc")
+RUNS = sorted(p.name for p in RUNS_DIR.iterdir() if (p / "transmission.tsv").exists())
+# thermal-conductance runs, each with the transmission run it read (source.json)
+THERMAL_RUNS = sorted(p.name for p in RUNS_DIR.iterdir() if (p / "thermal_conductance.tsv").exists())
 
 
 def load_run(name: str, tmp: Path):
