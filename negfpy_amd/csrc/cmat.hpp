@@ -289,6 +289,10 @@ __device__ __forceinline__ float rcp_nr(float d) {
   const float e = fmaf(-d, r, 1.0f);
   return fmaf(r, e, r);
 }
+// An empty side effect inside a wave-uniform branch: the compiler then keeps the
+// branch (a scalar jump) instead of turning its body into selects on every lane.
+__device__ __forceinline__ void uniform_branch() { asm volatile(""); }
+
 template <typename T>
 __device__ __forceinline__ void crecip1(T ar, T ai, T& rr, T& ri) {
   const T d = rcp_nr(ar * ar + ai * ai);
@@ -311,10 +315,13 @@ __device__ __forceinline__ void crecip1(T ar, T ai, T& rr, T& ri) {
 // lane shuffle.  The pivot search is an RG-lane DPP arg-max among the owners of
 // the next column, so one barrier per step suffices.  Requires NMAX % RG == 0:
 // every lane then works on rows and columns < NMAX of the padded matrix, and the
-// padding (rows/columns >= n, marked used) never feeds a live entry, so the
-// update is branch-free: new = x' - c s with c = -1 and x' = 0 on the pivot row,
-// x' = 0 in column k, and s the scaled pivot row (s_k = 1 / pivot).  All threads
-// must call it; it ends with a barrier.  Returns ST_OK, ST_SINGULAR or
+// padding (rows/columns >= n, marked used) never feeds a live entry.  The
+// update is new = x - c s on every row, with s the scaled pivot row (s_k =
+// 1 / pivot) and x = 0 in column k (its owners clear their registers when they
+// publish it); the pivot row is then set to s.  Row p sits in one register slot
+// (wave-uniform) of one lane, so that costs scalar branches, not selects on
+// every row.  All
+// threads must call it; it ends with a barrier.  Returns ST_OK, ST_SINGULAR or
 // ST_NONFINITE.
 //
 // LS (lockstep): several sub-blocks of one workgroup invert their own matrices
@@ -383,6 +390,12 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
           b.piv[par].flag = ST_OK;
         }
       }
+      // the published column becomes -c s in its step: it starts from zero
+#pragma unroll
+      for (int m = 0; m < MAXR; ++m) {
+        xr[m] = T(0);
+        xi[m] = T(0);
+      }
     }
   };
   publish(0, 0);
@@ -406,36 +419,43 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
       return flag;
     }
     const int p = __builtin_amdgcn_readfirstlane(pk.p);
-    // pivot row entry of this lane's column, from the group lane holding row p
-    const int mp = p / RG;
+    // row p sits in register slot mp of the group lane gp: both are wave-uniform, so
+    // the slot is chosen by scalar branches (uniform_branch keeps them branches
+    // rather than per-row selects)
+    const int mp = p / RG, gp = p - (p / RG) * RG;
     T vr = xr[0], vi = xi[0];
 #pragma unroll
-    for (int m = 1; m < MAXR; ++m) {
-      vr = (m == mp) ? xr[m] : vr;
-      vi = (m == mp) ? xi[m] : vi;
-    }
-    const int src = gbase + (p % RG);
-    const T pr = lane_shfl(vr, src);
-    const T pi = lane_shfl(vi, src);
+    for (int m = 1; m < MAXR; ++m)
+      if (m == mp) {
+        uniform_branch();
+        vr = xr[m];
+        vi = xi[m];
+      }
+    // pivot row entry of this lane's column, from the group lane holding row p
+    const T pr = lane_shfl(vr, gbase + gp);
+    const T pi = lane_shfl(vi, gbase + gp);
     const bool jk = (j == k);
     const T sr = jk ? pk.ir : pr * pk.ir - pi * pk.ii;
     const T si = jk ? pk.ii : pr * pk.ii + pi * pk.ir;
 #pragma unroll
     for (int m = 0; m < MAXR; ++m) {
-      const int i = g + RG * m;
-      const bool isp = (i == p);
-      const T c_r = isp ? T(-1) : cr[m];
-      const T c_i = isp ? T(0) : ci[m];
-      T yr = (isp || jk) ? T(0) : xr[m];
-      T yi = (isp || jk) ? T(0) : xi[m];
-      yr = fma(-c_r, sr, yr);
-      yr = fma(c_i, si, yr);
-      yi = fma(-c_r, si, yi);
-      yi = fma(-c_i, sr, yi);
+      T yr = fma(-cr[m], sr, xr[m]);
+      yr = fma(ci[m], si, yr);
+      T yi = fma(-cr[m], si, xi[m]);
+      yi = fma(-ci[m], sr, yi);
       xr[m] = yr;
       xi[m] = yi;
-      used |= isp ? (1u << m) : 0u;
     }
+    // the pivot row becomes s (the update above left rounding residue there)
+#pragma unroll
+    for (int m = 0; m < MAXR; ++m)
+      if (m == mp) {
+        uniform_branch();
+        const bool own = (g == gp);
+        xr[m] = own ? sr : xr[m];
+        xi[m] = own ? si : xi[m];
+        used |= own ? (1u << m) : 0u;
+      }
     if (tid == 0) {
       b.pv[k] = p;
       b.qv[p] = k;
