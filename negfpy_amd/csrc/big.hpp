@@ -600,7 +600,7 @@ __device__ bool big_sign(const BigMat& S, const BigMat& X, int m, BigShared& sh,
       ss += xr * xr + xi * xi;
     }
     const double ns = block_sum<BIG_NT>(ss, sh.sred, tid);
-    const int st = gj_inverse<double, BIG_NT, BIG, BIG_RG>(X.re, X.im, BIG_LD, m, sh.gj, tid);
+    const int st = gj_inverse_sel<double, BIG_NT, BIG, BIG_RG>(X.re, X.im, BIG_LD, m, sh.gj, tid);
     if (st != ST_OK) return false;
     double xs = 0.0;
     for (int e = tid; e < m * m; e += BIG_NT) {
@@ -643,7 +643,7 @@ __device__ int big_count_inside(double r, const BigMat& A, const BigMat& B, cons
     X.im[o] = A.im[o] - r * B.im[o];
   }
   __syncthreads();
-  if (gj_inverse<double, BIG_NT, BIG, BIG_RG>(Y.re, Y.im, BIG_LD, m, sh.gj, tid) != ST_OK) return -1;
+  if (gj_inverse_sel<double, BIG_NT, BIG, BIG_RG>(Y.re, Y.im, BIG_LD, m, sh.gj, tid) != ST_OK) return -1;
   const BigTile t(m, m, tid);
   double cr[BIG_TM][BIG_TM], ci[BIG_TM][BIG_TM];
   big_mul<OPN, OPN>(Y, X, m, m, m, t, cr, ci, sh, tid);
@@ -748,7 +748,7 @@ __device__ void big_eig_one(const EigArgs& a, int inst, double* slot, BigShared&
     X.im[o] = -0.5 * S.im[q];
   }
   __syncthreads();
-  if (gj_inverse<double, BIG_NT, BIG, BIG_RG>(X.re, X.im, BIG_LD, n, sh.gj, tid) != ST_OK) return;
+  if (gj_inverse_sel<double, BIG_NT, BIG, BIG_RG>(X.re, X.im, BIG_LD, n, sh.gj, tid) != ST_OK) return;
   const BigMat S12{S.re + n, S.im + n};
   const BigTile t(n, n, tid);
   double cr[BIG_TM][BIG_TM], ci[BIG_TM][BIG_TM];
@@ -778,7 +778,7 @@ __device__ void big_eig_one(const EigArgs& a, int inst, double* slot, BigShared&
       }
   }
   __syncthreads();
-  if (gj_inverse<double, BIG_NT, BIG, BIG_RG>(G.re, G.im, BIG_LD, n, sh.gj, tid) != ST_OK) return;
+  if (gj_inverse_sel<double, BIG_NT, BIG, BIG_RG>(G.re, G.im, BIG_LD, n, sh.gj, tid) != ST_OK) return;
   bool bad = false;
   for (int e = tid; e < (int)nn; e += BIG_NT) {
     const int i = e / n, j = e - (e / n) * n;
@@ -889,7 +889,7 @@ __device__ bool blk_inverse(const Blk4& M, const Blk4& X, const BigMat* W, int n
     const BigMat& Mi = M.b[d == 0 ? 0 : 3];
     big_copy(W[d], Mi, n, tid);
     const double nm = block_sum<BIG_NT>(big_fro2(Mi, n, tid), sh.sred, tid);
-    if (gj_inverse<double, BIG_NT, BIG, BIG_RG>(W[d].re, W[d].im, BIG_LD, n, sh.gj, tid) != ST_OK) {
+    if (gj_inverse_sel<double, BIG_NT, BIG, BIG_RG>(W[d].re, W[d].im, BIG_LD, n, sh.gj, tid) != ST_OK) {
       cnd[d] = INFINITY;
       continue;
     }
@@ -907,7 +907,7 @@ __device__ bool blk_inverse(const Blk4& M, const Blk4& X, const BigMat* W, int n
   big_copy(K, Mqq, n, tid);
   __syncthreads();
   big_gemm<OPN, OPN>(K, Mqp, U, n, -1.0, true, sh, tid);    // K = M_qq - M_qp U
-  if (gj_inverse<double, BIG_NT, BIG, BIG_RG>(K.re, K.im, BIG_LD, n, sh.gj, tid) != ST_OK) return false;
+  if (gj_inverse_sel<double, BIG_NT, BIG, BIG_RG>(K.re, K.im, BIG_LD, n, sh.gj, tid) != ST_OK) return false;
   // blocks of the result: (pp) = P^-1 + U K^-1 V, (pq) = -U K^-1, (qp) = -K^-1 V, (qq) = K^-1
   const BigMat& Xpp = X.b[p == 0 ? 0 : 3];
   const BigMat& Xpq = X.b[p == 0 ? 1 : 2];
@@ -1082,7 +1082,7 @@ __device__ void huge_eig_one(const EigArgs& a, int inst, double* slot, BigShared
     P22.im[o] = -0.5 * S.b[3].im[o];
   }
   __syncthreads();
-  if (gj_inverse<double, BIG_NT, BIG, BIG_RG>(P22.re, P22.im, BIG_LD, n, sh.gj, tid) != ST_OK) return;
+  if (gj_inverse_sel<double, BIG_NT, BIG, BIG_RG>(P22.re, P22.im, BIG_LD, n, sh.gj, tid) != ST_OK) return;
   const BigMat& T = Y.b[0];
   big_gemm<OPN, OPN>(T, S.b[1], P22, n, -0.5, false, sh, tid);
   // d01 = D1s / s (unscaled) into W1, Sigma = d01 T into W2
@@ -1102,7 +1102,7 @@ __device__ void huge_eig_one(const EigArgs& a, int inst, double* slot, BigShared
     G.im[o] = ((i == j ? zi : 0.0) - D0.im[o]) - W[2].im[o];
   }
   __syncthreads();
-  if (gj_inverse<double, BIG_NT, BIG, BIG_RG>(G.re, G.im, BIG_LD, n, sh.gj, tid) != ST_OK) return;
+  if (gj_inverse_sel<double, BIG_NT, BIG, BIG_RG>(G.re, G.im, BIG_LD, n, sh.gj, tid) != ST_OK) return;
   bool bad = false;
   for (int e = tid; e < (int)nn; e += BIG_NT) {
     const int i = e / n, j = e - (e / n) * n;

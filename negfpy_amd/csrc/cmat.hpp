@@ -328,7 +328,7 @@ __device__ __forceinline__ void crecip1(T ar, T ai, T& rr, T& ri) {
 // of the same n side by side, so every sub-block must pass the same n + 2
 // barriers: a failed pivot then keeps stepping through the remaining barriers
 // instead of returning early (gj_inverse_ls).
-template <typename T, int NT, int NMAX, int RG, bool LS, typename PT>
+template <typename T, int NT, int NMAX, int RG, bool LS, typename PT, bool SF = true>
 __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, int ld, int n,
                                        GjBuf<T, NMAX>& b, int tid) {
   static_assert(NMAX % RG == 0, "row groups must tile NMAX");
@@ -391,10 +391,12 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
         }
       }
       // the published column becomes -c s in its step: it starts from zero
+      if constexpr (SF) {
 #pragma unroll
-      for (int m = 0; m < MAXR; ++m) {
-        xr[m] = T(0);
-        xi[m] = T(0);
+        for (int m = 0; m < MAXR; ++m) {
+          xr[m] = T(0);
+          xi[m] = T(0);
+        }
       }
     }
   };
@@ -419,43 +421,74 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
       return flag;
     }
     const int p = __builtin_amdgcn_readfirstlane(pk.p);
-    // row p sits in register slot mp of the group lane gp: both are wave-uniform, so
-    // the slot is chosen by scalar branches (uniform_branch keeps them branches
-    // rather than per-row selects)
-    const int mp = p / RG, gp = p - (p / RG) * RG;
-    T vr = xr[0], vi = xi[0];
+    if constexpr (SF) {
+      // row p sits in register slot mp of the group lane gp: both are wave-uniform, so
+      // the slot is chosen by scalar branches (uniform_branch keeps them branches
+      // rather than per-row selects)
+      const int mp = p / RG, gp = p - (p / RG) * RG;
+      T vr = xr[0], vi = xi[0];
 #pragma unroll
-    for (int m = 1; m < MAXR; ++m)
-      if (m == mp) {
-        uniform_branch();
-        vr = xr[m];
-        vi = xi[m];
+      for (int m = 1; m < MAXR; ++m)
+        if (m == mp) {
+          uniform_branch();
+          vr = xr[m];
+          vi = xi[m];
+        }
+      // pivot row entry of this lane's column, from the group lane holding row p
+      const T pr = lane_shfl(vr, gbase + gp);
+      const T pi = lane_shfl(vi, gbase + gp);
+      const bool jk = (j == k);
+      const T sr = jk ? pk.ir : pr * pk.ir - pi * pk.ii;
+      const T si = jk ? pk.ii : pr * pk.ii + pi * pk.ir;
+#pragma unroll
+      for (int m = 0; m < MAXR; ++m) {
+        T yr = fma(-cr[m], sr, xr[m]);
+        yr = fma(ci[m], si, yr);
+        T yi = fma(-cr[m], si, xi[m]);
+        yi = fma(-ci[m], sr, yi);
+        xr[m] = yr;
+        xi[m] = yi;
       }
-    // pivot row entry of this lane's column, from the group lane holding row p
-    const T pr = lane_shfl(vr, gbase + gp);
-    const T pi = lane_shfl(vi, gbase + gp);
-    const bool jk = (j == k);
-    const T sr = jk ? pk.ir : pr * pk.ir - pi * pk.ii;
-    const T si = jk ? pk.ii : pr * pk.ii + pi * pk.ir;
+      // the pivot row becomes s (the update above left rounding residue there)
 #pragma unroll
-    for (int m = 0; m < MAXR; ++m) {
-      T yr = fma(-cr[m], sr, xr[m]);
-      yr = fma(ci[m], si, yr);
-      T yi = fma(-cr[m], si, xi[m]);
-      yi = fma(-ci[m], sr, yi);
-      xr[m] = yr;
-      xi[m] = yi;
+      for (int m = 0; m < MAXR; ++m)
+        if (m == mp) {
+          uniform_branch();
+          const bool own = (g == gp);
+          xr[m] = own ? sr : xr[m];
+          xi[m] = own ? si : xi[m];
+          used |= own ? (1u << m) : 0u;
+        }
+    } else {
+      // the per-row select form: c = -1 and x' = 0 on the pivot row, x' = 0 in column k
+      const int mp = p / RG;
+      T vr = xr[0], vi = xi[0];
+#pragma unroll
+      for (int m = 1; m < MAXR; ++m) {
+        vr = (m == mp) ? xr[m] : vr;
+        vi = (m == mp) ? xi[m] : vi;
+      }
+      const T pr = lane_shfl(vr, gbase + (p % RG));
+      const T pi = lane_shfl(vi, gbase + (p % RG));
+      const bool jk = (j == k);
+      const T sr = jk ? pk.ir : pr * pk.ir - pi * pk.ii;
+      const T si = jk ? pk.ii : pr * pk.ii + pi * pk.ir;
+#pragma unroll
+      for (int m = 0; m < MAXR; ++m) {
+        const bool isp = (g + RG * m == p);
+        const T c_r = isp ? T(-1) : cr[m];
+        const T c_i = isp ? T(0) : ci[m];
+        T yr = (isp || jk) ? T(0) : xr[m];
+        T yi = (isp || jk) ? T(0) : xi[m];
+        yr = fma(-c_r, sr, yr);
+        yr = fma(c_i, si, yr);
+        yi = fma(-c_r, si, yi);
+        yi = fma(-c_i, sr, yi);
+        xr[m] = yr;
+        xi[m] = yi;
+        used |= isp ? (1u << m) : 0u;
+      }
     }
-    // the pivot row becomes s (the update above left rounding residue there)
-#pragma unroll
-    for (int m = 0; m < MAXR; ++m)
-      if (m == mp) {
-        uniform_branch();
-        const bool own = (g == gp);
-        xr[m] = own ? sr : xr[m];
-        xi[m] = own ? si : xi[m];
-        used |= own ? (1u << m) : 0u;
-      }
     if (tid == 0) {
       b.pv[k] = p;
       b.qv[p] = k;
@@ -483,6 +516,16 @@ template <typename T, int NT, int NMAX, int RG, typename PT>
 __device__ __forceinline__ int gj_inverse(PT __restrict__ re, PT __restrict__ im, int ld, int n,
                                           GjBuf<T, NMAX>& b, int tid) {
   return gj_core<T, NT, NMAX, RG, false>(re, im, ld, n, b, tid);
+}
+
+// The per-row select form of the same step (same results on the matrices where both
+// were compared), kept for the eigenmode route's spectral-projector kernels: a
+// 90-DOF case of that route stopped converging when they took the select-free step,
+// for a reason not found.
+template <typename T, int NT, int NMAX, int RG, typename PT>
+__device__ __forceinline__ int gj_inverse_sel(PT __restrict__ re, PT __restrict__ im, int ld, int n,
+                                              GjBuf<T, NMAX>& b, int tid) {
+  return gj_core<T, NT, NMAX, RG, false, PT, false>(re, im, ld, n, b, tid);
 }
 
 // Lockstep form: a sub-block with live == false (finished, failed or without an

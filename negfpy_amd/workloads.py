@@ -288,17 +288,16 @@ def sheet_problem(seed: int = 0, n_layers: int = 16, n_k: int = 128, n_omega: in
 CUBIC_SPRINGS = ((10.0, 2.0), (3.0, 0.5))
 
 
-def _cubic_phi(cell=(4, 2, 2), seed: int | None = None, jitter: float = 0.0):
-    """Force constants Phi(dx, dy, dz) (3 N x 3 N, N = prod(cell)) between a simple-cubic
-    cell of nx x ny x nz atoms (unit spacing; transport along x) and its image shifted
-    by whole cells; first and second neighbour springs K = kr e e^T + kt (I - e e^T),
-    each bond's springs times a seeded factor in [1 - jitter, 1 + jitter] (the same for
+def _pair_phi(pos, size, shells, springs, seed: int | None, jitter: float):
+    """Force constants Phi(dx, dy, dz) (3 N x 3 N) between a cell of N atoms at ``pos``
+    (box ``size``, transport along x) and its images shifted by whole cells: for every
+    pair at one of the ``shells`` distances, K = kr e e^T + kt (I - e e^T) with that
+    shell's (kr, kt) times a seeded factor in [1 - jitter, 1 + jitter] (the same for
     the bond seen from either atom), Phi_ab = -K off the diagonal and the acoustic sum
-    rule on it."""
-    nx, ny, nz = cell
-    pos = np.array([(i, j, k) for i in range(nx) for j in range(ny) for k in range(nz)], float)
+    rule on it.  Bonds must be shorter than the cell so only adjacent cells couple."""
+    pos = np.asarray(pos, float)
+    size = np.asarray(size, float)
     na = len(pos)
-    size = np.array(cell, float)
     rng = np.random.default_rng(seed)
     factor: dict = {}
     phi: dict = {}
@@ -309,13 +308,13 @@ def _cubic_phi(cell=(4, 2, 2), seed: int | None = None, jitter: float = 0.0):
                 sh = np.array(s) - 1
                 r = pos[b] + sh * size - pos[a]
                 d = float(np.linalg.norm(r))
-                shell = 0 if abs(d - 1.0) < 1e-9 else 1 if abs(d - np.sqrt(2.0)) < 1e-9 else -1
+                shell = next((q for q, ds in enumerate(shells) if abs(d - ds) < 1e-9), -1)
                 if shell < 0:
                     continue
                 key, twin = (a, b) + tuple(sh), (b, a) + tuple(-sh)
                 if key not in factor:
                     factor[key] = factor.get(twin, 1.0 + jitter * rng.uniform(-1.0, 1.0))
-                kr, kt = (factor[key] * x for x in CUBIC_SPRINGS[shell])
+                kr, kt = (factor[key] * x for x in springs[shell])
                 e = r / d
                 k = kr * np.outer(e, e) + kt * (np.eye(3) - np.outer(e, e))
                 blk = phi.setdefault(tuple(sh), np.zeros((3 * na, 3 * na)))
@@ -324,6 +323,32 @@ def _cubic_phi(cell=(4, 2, 2), seed: int | None = None, jitter: float = 0.0):
     phi.setdefault((0, 0, 0), np.zeros((3 * na, 3 * na)))
     phi[(0, 0, 0)] += onsite
     return phi
+
+
+def _cubic_phi(cell=(4, 2, 2), seed: int | None = None, jitter: float = 0.0):
+    """``_pair_phi`` of a simple-cubic cell of nx x ny x nz atoms (unit spacing) with
+    first and second neighbour springs (``CUBIC_SPRINGS``)."""
+    nx, ny, nz = cell
+    pos = [(i, j, k) for i in range(nx) for j in range(ny) for k in range(nz)]
+    return _pair_phi(pos, cell, (1.0, np.sqrt(2.0)), CUBIC_SPRINGS, seed, jitter)
+
+
+# (radial, tangential) springs in eV/A^2 of the first (a sqrt(3) / 4, along <111>) and
+# second (a / sqrt(2), along <110>) neighbour shells of the silicon-like diamond cells
+DIAMOND_SPRINGS = ((9.0, 1.5), (1.5, 0.25))
+_FCC = ((0.0, 0.0, 0.0), (0.0, 0.5, 0.5), (0.5, 0.0, 0.5), (0.5, 0.5, 0.0))
+
+
+def _diamond_phi(cells=(1, 1, 1), seed: int | None = None, jitter: float = 0.0):
+    """``_pair_phi`` of cx x cy x cz conventional diamond cubes (cube edge 1, 8 atoms
+    each: fcc sites and the (1/4, 1/4, 1/4) basis) with first and second neighbour
+    springs (``DIAMOND_SPRINGS``)."""
+    cx, cy, cz = cells
+    pos = [(i + f[0] + o, j + f[1] + o, k + f[2] + o)
+           for i in range(cx) for j in range(cy) for k in range(cz)
+           for o in (0.0, 0.25) for f in _FCC]
+    return _pair_phi(pos, cells, (np.sqrt(3.0) / 4.0, np.sqrt(2.0) / 2.0), DIAMOND_SPRINGS,
+                     seed, jitter)
 
 
 @dataclass
@@ -429,7 +454,8 @@ def interface3d_blocks(p: Interface3DProblem, fc00, fc01):
 
 
 # (cell, transverse k mesh) of the 3-D uneven batch: 24-, 48- and 96-DOF layers
-UNEVEN3D_PARTS = (((2, 2, 2), 32), ((4, 2, 2), 8), ((4, 4, 2), 4))
+# (conventional diamond cubes along x, y, z; k mesh per transverse direction)
+UNEVEN3D_PARTS = (((1, 1, 1), 32), ((2, 1, 1), 8), ((2, 2, 1), 4))
 
 
 @dataclass
@@ -445,31 +471,41 @@ class KspaceProblem:
     wmax: float
 
 
-def cubic_params(cell, seed: int = 0, jitter: float = 0.05, mass: float = 28.0):
-    """``MaterialKspaceParams`` of a simple-cubic cell (``_cubic_phi``, seeded bond
-    jitter) with (dy, dz) terms for the same layer and the next one."""
+def _kspace_params(phi, mass: float):
     from negfpy_amd.models.material_kspace import MaterialKspaceParams
 
-    phi = _cubic_phi(cell, seed, jitter)
-    na = int(np.prod(cell))
+    na = phi[(0, 0, 0)].shape[0] // 3
     f00 = {(dy, dz): v for (dx, dy, dz), v in phi.items() if dx == 0}
     f01 = {(dy, dz): v for (dx, dy, dz), v in phi.items() if dx == 1}
     return MaterialKspaceParams(masses=np.full(na, float(mass)), fc00_terms=f00,
                                 fc01_terms=f01)
 
 
+def cubic_params(cell, seed: int = 0, jitter: float = 0.05, mass: float = 28.0):
+    """``MaterialKspaceParams`` of a simple-cubic cell (``_cubic_phi``, seeded bond
+    jitter) with (dy, dz) terms for the same layer and the next one."""
+    return _kspace_params(_cubic_phi(cell, seed, jitter), mass)
+
+
+def diamond_params(cells, seed: int = 0, jitter: float = 0.05, mass: float = 28.0):
+    """``MaterialKspaceParams`` of cx x cy x cz conventional silicon-like diamond cubes
+    (``_diamond_phi``, seeded bond jitter), mass 28 amu."""
+    return _kspace_params(_diamond_phi(cells, seed, jitter), mass)
+
+
 def uneven3d_problems(seed: int = 0, n_layers: int = 8, n_omega: int = 1024,
                       edge_fraction: float = 0.25, eta: float = 1e-6) -> list[KspaceProblem]:
-    """The 3-D uneven batch: simple-cubic cells of 8, 16 and 32 atoms (24-, 48- and
-    96-DOF layers, mass 28 amu, first and second neighbour springs with 5 % seeded bond
-    jitter) on 32 x 32, 8 x 8 and 4 x 4 (ky, kz) meshes, n_layers device layers of
-    the same cells and n_omega frequencies each; a quarter of each spectrum packed
-    within 1 % of the band top.  The 24-DOF part alone is 1,048,576 instances."""
+    """The 3-D uneven batch: silicon-like diamond cells of 8, 16 and 32 atoms (one, two
+    and four conventional cubes: 24-, 48- and 96-DOF layers, mass 28 amu, first and
+    second neighbour springs with 5 % seeded bond jitter) on 32 x 32, 8 x 8 and 4 x 4
+    (ky, kz) meshes, n_layers device layers of the same cells and n_omega frequencies
+    each; a quarter of each spectrum packed within 1 % of the band top.  The 24-DOF
+    part alone is 1,048,576 instances."""
     from negfpy_amd.models.material_kspace import material_kspace_device, material_kspace_lead
 
     out = []
     for i, (cell, nk) in enumerate(UNEVEN3D_PARTS):
-        params = cubic_params(cell, seed + i)
+        params = diamond_params(cell, seed + i)
         build = material_kspace_lead(params).blocks_builder
         top = 0.0
         for ky in np.linspace(0.0, np.pi, 5):
