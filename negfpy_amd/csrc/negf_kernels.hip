@@ -1846,11 +1846,15 @@ static int default_nmax(int n, bool fp64) {
 
 // One-instance-per-wavefront MFMA kernels (wpi.hpp) for blocks of up to 32 DOF.
 // Mode 1 (default) uses them where they measured faster than the workgroup classes on
-// MI355X (8192 instances, 16 layers, eta = 1e-6): FP64 up to 16 DOF (n = 12: decimation
-// 1.18x, sweep 1.45x) and the FP32 sweep up to 24 DOF (1.06x at 24); above that the
-// wave's register-resident Gauss-Jordan step (one wavefront per instance, no other
-// wave to hide its latency) loses to the workgroup form.  Mode 2 uses them for every
-// block up to 32 DOF, mode 0 never (negf_set_wave_kernels).
+// MI355X (8192 instances, 16 layers, eta = 1e-6, ms per launch):
+//   FP64, n = 12: decimation 1.57 vs 1.71, sweep 0.79 vs 1.05; at n = 6 (class 8) the
+//   wave form is 2x slower, at n = 16 and above 6-40 % slower;
+//   FP32 decimation: 0.64 vs 0.72 at n = 12, 2.73 vs 2.80 at 20, 3.20 vs 3.48 at 24,
+//   slower at 6 and from 30;
+// so FP64 blocks of 9-12 DOF and FP32 decimations of 9-12 and 17-24 DOF (FP32 sweeps of
+// 9-24).  Elsewhere the wave's register-resident Gauss-Jordan step (one wavefront per
+// instance, no other wave to hide its latency) loses to the workgroup form.  Mode 2
+// uses them for every block up to 32 DOF, mode 0 never (negf_set_wave_kernels).
 constexpr int WAVE_MAX = 32;
 static std::atomic<int> g_wave_kernels{1};
 static bool use_wave(int n, bool fp64, bool sweep) {
@@ -1858,8 +1862,10 @@ static bool use_wave(int n, bool fp64, bool sweep) {
   const int mode = g_wave_kernels.load();
   if (mode == 2) return true;
   if (mode != 1) return false;
-  if (fp64) return n <= 16;
-  return n <= (sweep ? 24 : 16);
+  if (n <= 8) return false;
+  if (fp64) return n <= 12;
+  if (sweep) return n <= 24;
+  return n <= 12 || (n >= 17 && n <= 24);
 }
 // rows per lane of the wave inversion: n <= 8 R
 static int wave_rows(int n) { return n <= 16 ? 2 : (n <= 24 ? 3 : 4); }
