@@ -37,8 +37,8 @@ sys.path.insert(0, ROOT)
 METRIC = "(ω,k) transmission solves/s, mixed vs FP64 GPU path at ≤1e-6 rel err in T"
 PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}  # MI355X dense matrix peaks (FP64, FP32)
 # HBM bytes per launch of the FP64 kernels on this workload, from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE scaled by a copy-kernel calibration (profiles/ribbon24_fp64_hbm_counters.md)
-TRAFFIC: dict = {"rgf": 120.9e6, "sancho_rubio": 39.9e6}
+# WRITE_SIZE scaled by a copy-kernel calibration (profiles/ribbon24_hbm_counters.md)
+TRAFFIC: dict = {"rgf": 121.3e6, "sancho_rubio": 39.9e6}
 
 
 def _cpu_worker(args):
