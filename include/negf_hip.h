@@ -103,6 +103,18 @@ int negf_block_class(int precision, int n);
 int negf_set_wave_kernels(int mode);
 
 /*
+ * Per-size kernel family (blocks of 1..32 DOF), for the autotuner: family
+ * NEGF_FAMILY_WORKGROUP or NEGF_FAMILY_WAVE for the decimations and sweeps of that
+ * precision's arithmetic (NEGF_FP64: the FP64 decimations and sweeps of both
+ * precisions; NEGF_MIXED: the FP32 decimations), 0 for the default rule of mode 1.
+ * Applies in mode 1 only.  negf_kernel_family returns the family a decimation of n-DOF
+ * blocks uses now.  Process-wide, thread-safe.  No counterpart in the reference.
+ */
+enum { NEGF_FAMILY_WORKGROUP = 1, NEGF_FAMILY_WAVE = 2 };
+int negf_set_kernel_family(int precision, int n, int family);
+int negf_kernel_family(int precision, int n);
+
+/*
  * Surface Green's function of semi-infinite leads by Lopez-Sancho decimation.
  * Replaces surface_gf_sancho_rubio, src/negfpy/core/surface_gf.py:18-81
  * (and the per-k loop of lead_surface_dos*, src/negfpy/core/dos.py:31-251).

@@ -25,6 +25,7 @@ FP64 = 0
 MIXED = 1
 SURFACE_DECIMATION = 0
 SURFACE_EIGEN = 1
+FAMILY_WORKGROUP, FAMILY_WAVE = 1, 2   # NEGF_FAMILY_*
 LEADS_INDEPENDENT = 0   # NEGF_LEADS_*: negf_transmission_problem.share_leads
 LEADS_SHARED = 1
 LEADS_MIRRORED = 2
@@ -80,6 +81,8 @@ _SIGS = {
     "negf_set_block_class": ([_c_int, _c_int, _c_int], _c_int),
     "negf_block_class": ([_c_int, _c_int], _c_int),
     "negf_set_wave_kernels": ([_c_int], _c_int),
+    "negf_set_kernel_family": ([_c_int, _c_int, _c_int], _c_int),
+    "negf_kernel_family": ([_c_int, _c_int], _c_int),
     "negf_sancho_rubio_batched": (
         [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_dbl, _c_int, _c_int, _vp, _vp, _vp, _vp],
         _c_int),

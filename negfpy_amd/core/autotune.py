@@ -5,11 +5,15 @@ precision, 96 in FP64 with the matrices in a device-memory pool): the class fixe
 the workgroup size, the rows each lane owns in the inversion, the product tile per
 thread and the LDS per instance, and so the occupancy.  A block of n DOF
 runs in any class >= n; which one is fastest depends on how the inversion's
-per-step latency (rows per lane) trades against resident instances per CU.
-``tune_block_class`` times the candidate classes on a synthetic batch of n-DOF
-instances (HIP events on the solve's stream, best of ``repeats``), records the
-fastest in the library's process-wide class table (``negf_set_block_class``) and
-returns the timings.  Results are cached per (precision, n) for the process.
+per-step latency (rows per lane) trades against resident instances per CU.  Blocks of
+up to 32 DOF can also run one instance per wavefront ("wave": 64-lane workgroups,
+products on MFMA tiles, a register-resident wave inversion), which trades the
+workgroup's barrier-bound step for more resident instances.
+``tune_block_class`` times every candidate -- the classes >= n and, up to 32 DOF, the
+wave family -- on a synthetic batch of n-DOF instances (HIP events on the solve's
+stream, best of ``repeats``), records the fastest in the library's process-wide
+tables (``negf_set_block_class``, ``negf_set_kernel_family``) and returns the timings.
+Results are cached per (precision, n) for the process.
 """
 
 from __future__ import annotations
@@ -25,18 +29,32 @@ from . import _engine
 from .surface_gf import pack_lead
 from .types import LeadBlocks
 
-__all__ = ["CLASSES", "candidate_classes", "tune_block_class", "block_class", "reset_block_class"]
+__all__ = ["CLASSES", "WAVE", "candidate_classes", "tune_block_class", "block_class",
+           "kernel_family", "reset_block_class"]
 
 CLASSES = (8, 16, 24, 32, 48, 64, 96)
+WAVE = "wave"          # the one-wavefront family, a candidate for blocks up to 32 DOF
+WAVE_MAX = 32
 _lock = threading.Lock()
 _results: dict = {}
 
 
-def candidate_classes(n: int, precision: str = "fp64") -> list[int]:
+def candidate_classes(n: int, precision: str = "fp64", *, wave: bool = True) -> list:
+    """Workgroup classes that hold n-DOF blocks, then ``WAVE`` for n <= 32."""
     if precision == "fp64":
         # the pooled 96 class only for blocks the LDS classes cannot hold
-        return [c for c in (8, 16, 24, 32, 48) if n <= c] or ([96] if n <= 96 else [])
-    return [c for c in (8, 16, 24, 32, 48, 64) if n <= c]
+        out = [c for c in (8, 16, 24, 32, 48) if n <= c] or ([96] if n <= 96 else [])
+    else:
+        out = [c for c in (8, 16, 24, 32, 48, 64) if n <= c]
+    if wave and 1 <= n <= WAVE_MAX and out:
+        out.append(WAVE)
+    return out
+
+
+def kernel_family(n: int, precision: str = "fp64") -> str:
+    """"wave" or "workgroup": the family n-DOF decimations use now."""
+    f = int(_lib.load().negf_kernel_family(_engine.precision_code(precision), int(n)))
+    return WAVE if f == _lib.FAMILY_WAVE else "workgroup"
 
 
 def block_class(n: int, precision: str = "fp64") -> int:
@@ -45,7 +63,11 @@ def block_class(n: int, precision: str = "fp64") -> int:
 
 
 def reset_block_class(n: int, precision: str = "fp64") -> None:
-    _lib.check(_lib.load().negf_set_block_class(_engine.precision_code(precision), int(n), 0))
+    lib = _lib.load()
+    code = _engine.precision_code(precision)
+    _lib.check(lib.negf_set_block_class(code, int(n), 0))
+    if 1 <= n <= WAVE_MAX:
+        _lib.check(lib.negf_set_kernel_family(code, int(n), 0))
     with _lock:
         _results.pop((precision, int(n)), None)
 
@@ -67,9 +89,13 @@ def _synthetic_sets(n: int, n_layers: int):
 def tune_block_class(n: int, precision: str = "fp64", *, n_layers: int = 8, n_inst: int = 2048,
                      repeats: int = 3, candidates=None, apply: bool = True,
                      refresh: bool = False) -> dict:
-    """Time each candidate class for n-DOF blocks and (if ``apply``) select the fastest.
+    """Time each candidate (class, or ``WAVE``) for n-DOF blocks and (if ``apply``)
+    select the fastest.  A class candidate runs with the workgroup family, ``WAVE`` with
+    the one-wavefront family for the decimations and sweeps (the other stages keep the
+    default class).
 
-    Returns {"best": class, "timings_ms": {class: ms per batch}, "n_inst": ..., "n_layers": ...}.
+    Returns {"best": class or "wave", "timings_ms": {candidate: ms per batch},
+    "n_inst": ..., "n_layers": ...}.
     """
     key = (precision, int(n))
     with _lock:
@@ -89,9 +115,25 @@ def tune_block_class(n: int, precision: str = "fp64", *, n_layers: int = 8, n_in
     out = solver.alloc_result(n_inst)
     timings = {}
     previous = int(lib.negf_block_class(code, int(n)))
+    fam_ok = 1 <= n <= WAVE_MAX
+    # the families of both arithmetic tables: FP64 (decimation and sweep of FP64, the
+    # FP64 stages of mixed) and FP32 (the mixed decimation)
+    tables = (_lib.FP64,) if precision == "fp64" else (_lib.FP64, _lib.MIXED)
+    prev_fam = {t: int(lib.negf_kernel_family(t, int(n))) for t in tables} if fam_ok else {}
+
+    def family(f):
+        if fam_ok:
+            for t in tables:
+                _lib.check(lib.negf_set_kernel_family(t, int(n), f))
+
     try:
         for c in cands:
-            _lib.check(lib.negf_set_block_class(code, int(n), int(c)))
+            if c == WAVE:
+                _lib.check(lib.negf_set_block_class(code, int(n), 0))
+                family(_lib.FAMILY_WAVE)
+            else:
+                _lib.check(lib.negf_set_block_class(code, int(n), int(c)))
+                family(_lib.FAMILY_WORKGROUP)
             solver.run(set_of, omega, eta, out=out)
             best = float("inf")
             for _ in range(max(int(repeats), 1)):
@@ -102,12 +144,19 @@ def tune_block_class(n: int, precision: str = "fp64", *, n_layers: int = 8, n_in
                 e1.record()
                 e1.synchronize()
                 best = min(best, float(e0.elapsed_time(e1)))
-            timings[int(c)] = best
+            timings[c if c == WAVE else int(c)] = best
     finally:
         _lib.check(lib.negf_set_block_class(code, int(n), previous))
+        for t, f in prev_fam.items():
+            _lib.check(lib.negf_set_kernel_family(t, int(n), f))
     choice = min(timings, key=timings.get)
     if apply:
-        _lib.check(lib.negf_set_block_class(code, int(n), choice))
+        if choice == WAVE:
+            _lib.check(lib.negf_set_block_class(code, int(n), 0))
+            family(_lib.FAMILY_WAVE)
+        else:
+            _lib.check(lib.negf_set_block_class(code, int(n), choice))
+            family(_lib.FAMILY_WORKGROUP)
     res = {"best": choice, "timings_ms": timings, "n_inst": int(n_inst), "n_layers": int(n_layers)}
     with _lock:
         _results[key] = res

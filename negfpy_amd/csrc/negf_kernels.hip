@@ -1855,13 +1855,18 @@ static int default_nmax(int n, bool fp64) {
 // 9-24).  Elsewhere the wave's register-resident Gauss-Jordan step (one wavefront per
 // instance, no other wave to hide its latency) loses to the workgroup form.  Mode 2
 // uses them for every block up to 32 DOF, mode 0 never (negf_set_wave_kernels).
+// In mode 1 a per-size choice (negf_set_kernel_family, the autotuner's) overrides the
+// default: [0] FP64 decimations and sweeps, [1] FP32 decimations and sweeps.
 constexpr int WAVE_MAX = 32;
 static std::atomic<int> g_wave_kernels{1};
+static std::atomic<int> g_family_override[2][WAVE_MAX + 1];
 static bool use_wave(int n, bool fp64, bool sweep) {
   if (n < 1 || n > WAVE_MAX) return false;
   const int mode = g_wave_kernels.load();
   if (mode == 2) return true;
   if (mode != 1) return false;
+  const int f = g_family_override[fp64 ? 0 : 1][n].load(std::memory_order_relaxed);
+  if (f) return f == NEGF_FAMILY_WAVE;
   if (n <= 8) return false;
   if (fp64) return n <= 12;
   if (sweep) return n <= 24;
@@ -2191,6 +2196,24 @@ int negf_set_block_class(int precision, int n, int nmax) {
   }
   g_class_override[fp64 ? 0 : 1][n].store(nmax, std::memory_order_relaxed);
   return NEGF_OK;
+}
+
+int negf_set_kernel_family(int precision, int n, int family) {
+  if (precision != NEGF_FP64 && precision != NEGF_MIXED)
+    return fail(NEGF_ERR_ARG, "unknown precision %d", precision);
+  if (n < 1 || n > WAVE_MAX)
+    return fail(NEGF_ERR_ARG, "kernel families are chosen for blocks of 1..%d DOF (got %d)",
+                WAVE_MAX, n);
+  if (family != 0 && family != NEGF_FAMILY_WORKGROUP && family != NEGF_FAMILY_WAVE)
+    return fail(NEGF_ERR_ARG, "kernel family %d is not 0, %d or %d", family,
+                NEGF_FAMILY_WORKGROUP, NEGF_FAMILY_WAVE);
+  g_family_override[precision == NEGF_FP64 ? 0 : 1][n].store(family, std::memory_order_relaxed);
+  return NEGF_OK;
+}
+
+int negf_kernel_family(int precision, int n) {
+  if (precision != NEGF_FP64 && precision != NEGF_MIXED) return 0;
+  return use_wave(n, precision == NEGF_FP64, false) ? NEGF_FAMILY_WAVE : NEGF_FAMILY_WORKGROUP;
 }
 
 int negf_set_wave_kernels(int mode) {

@@ -97,6 +97,32 @@ def test_wave_kernel_mode_switch():
     assert lib.negf_set_wave_kernels(prev) == 0
 
 
+def test_kernel_family_table():
+    """negf_set_kernel_family stores a per-size family (0 default, 1 workgroup, 2 wave)
+    for blocks of 1..32 DOF and negf_kernel_family reports the one in force in mode 1;
+    bad arguments are rejected (host-only, no GPU work)."""
+    lib = _lib.load_symbols_only()
+    prev = lib.negf_set_wave_kernels(1)
+    try:
+        assert lib.negf_kernel_family(_lib.FP64, 12) == _lib.FAMILY_WAVE      # default rule
+        assert lib.negf_kernel_family(_lib.FP64, 24) == _lib.FAMILY_WORKGROUP
+        assert lib.negf_set_kernel_family(_lib.FP64, 24, _lib.FAMILY_WAVE) == 0
+        assert lib.negf_kernel_family(_lib.FP64, 24) == _lib.FAMILY_WAVE
+        assert lib.negf_set_kernel_family(_lib.FP64, 12, _lib.FAMILY_WORKGROUP) == 0
+        assert lib.negf_kernel_family(_lib.FP64, 12) == _lib.FAMILY_WORKGROUP
+        assert lib.negf_kernel_family(_lib.MIXED, 20) == _lib.FAMILY_WAVE     # FP32 decimation
+        lib.negf_set_wave_kernels(0)  # mode 0: never, whatever the table says
+        assert lib.negf_kernel_family(_lib.FP64, 24) == _lib.FAMILY_WORKGROUP
+        for args, word in (((_lib.FP64, 33, 1), b"1..32"), ((_lib.FP64, 0, 1), b"1..32"),
+                           ((_lib.FP64, 12, 3), b"family"), ((5, 12, 1), b"precision")):
+            assert lib.negf_set_kernel_family(*args) == 1
+            assert word in lib.negf_last_error()
+    finally:
+        for n in (12, 24):
+            lib.negf_set_kernel_family(_lib.FP64, n, 0)
+        lib.negf_set_wave_kernels(prev)
+
+
 def test_noalloc_entry_rejects_without_touching_a_gpu():
     """The allocation-free entry point validates its problem before any HIP call."""
     import ctypes
