@@ -350,7 +350,7 @@ int negf_transmission_batched_noalloc(void* stream, const negf_transmission_prob
 
 /*
  * Several transmission problems of different block sizes (up to 96 DOF) solved
- * together, FP64: every instance of every problem goes through one persistent
+ * together: every instance of every problem goes through one persistent
  * kernel launch (two, split around the eigenmode fallback, when a problem asks for
  * NEGF_SURFACE_EIGEN) whose workgroups take tickets of same-problem instances from
  * an atomic work queue, most expensive problems first; per-instance convergence
@@ -360,9 +360,15 @@ int negf_transmission_batched_noalloc(void* stream, const negf_transmission_prob
  * max(n, n_lead_l, n_lead_r), whatever negf_set_block_class says).  Replaces the
  * per-point loops of transmission_kavg* / heat_current_* over k-points whose
  * layouts differ (src/negfpy/core/negf.py:360-688), which the reference runs one
- * point at a time.  Problems with n_inst = 0 are skipped; precision must be
- * NEGF_FP64; refine_steps is unused.  Capturable in a hipGraph when no problem
- * needs blocks above 48 DOF (the class-96 pool orders calls through an event).
+ * point at a time.  Problems with n_inst = 0 are skipped.  When any problem of the
+ * call is NEGF_MIXED (leads up to 48 DOF, refine_steps >= 0, no ladder), every
+ * problem's surface GFs are computed first by the staged decimation of its own
+ * precision (for mixed problems: FP32 decimation, FP64 Newton certificate, FP64
+ * redo; one decimation per mirrored pair), and one persistent launch then runs the
+ * self-energies, sweeps and traces of all problems in FP64.  Otherwise (FP64 only)
+ * refine_steps is unused and the decimations run inside the persistent launch.
+ * Capturable in a hipGraph when no problem needs blocks above 48 DOF (the class-96
+ * pool orders calls through an event).
  * A problem with an eta ladder runs its rungs inside the launch (outputs: those of
  * the last attempted rung, and the ladder's arrays); ladders and the eigenmode route
  * cannot be mixed in one call.

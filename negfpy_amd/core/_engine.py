@@ -401,8 +401,8 @@ def _multi_prepare(jobs, outs):
     for j, job in enumerate(jobs):
         solver, set_of, omega, eta_lead, eta_dev = job[:5]
         lad = job[5] if len(job) > 5 else None
-        if solver.precision != _lib.FP64:
-            raise ValueError("the persistent multi-problem launch runs FP64 only")
+        if solver.precision != _lib.FP64 and len(job) > 5 and job[5] is not None:
+            raise ValueError("eta ladders in the persistent launch run FP64 only")
         set_of = to_dev_i32(set_of)
         omega = to_dev_f64(omega)
         n_inst = int(omega.numel())
@@ -444,13 +444,15 @@ def _multi_launch(prepared, chunks, ws):
 
 
 def transmission_multi(jobs, outs=None) -> list:
-    """Several FP64 transmission problems of any block sizes solved together.
+    """Several transmission problems of any block sizes solved together.
 
     ``jobs``: sequence of ``(solver, set_of, omega, eta_lead, eta_dev[, ladder])`` with
     a :class:`TransmissionSolver` per problem (its own sets, block sizes and leads)
     and that problem's instances.  Every instance of every problem goes through one
     persistent kernel launch per ``MULTI_MAX_PROBLEMS`` problems
-    (``negf_transmission_multi_batched``); results as ``solver.run`` on each.  With an
+    (``negf_transmission_multi_batched``); results as ``solver.run`` on each.  A
+    mixed-precision solver's surface GFs come from the staged mixed decimation before
+    that launch, which then runs the FP64 stages of every problem.  With an
     :class:`EtaLadder` (``eta_lead`` may then be None) each instance retries at its
     next eta inside the launch; ``result.ladder`` holds the accepted rung and every
     attempted rung's T and status.  ``outs`` (optional) are preallocated results.

@@ -2691,8 +2691,19 @@ size_t negf_transmission_multi_workspace_bytes(int n_problems,
 }
 
 static int pl_check(const negf_transmission_problem& p, int i) {
-  if (p.precision != NEGF_FP64)
-    return fail(NEGF_ERR_ARG, "problem %d: the persistent multi-problem launch is FP64 only", i);
+  if (p.precision != NEGF_FP64 && p.precision != NEGF_MIXED)
+    return fail(NEGF_ERR_ARG, "problem %d: unknown precision %d", i, p.precision);
+  if (p.precision == NEGF_MIXED) {
+    if (p.ladder)
+      return fail(NEGF_ERR_ARG, "problem %d: eta ladders run in FP64 (the ladder is walked "
+                  "inside the persistent launch, the mixed decimation is not)", i);
+    if (p.refine_steps < 0)
+      return fail(NEGF_ERR_ARG, "problem %d: mixed precision needs refine_steps >= 0", i);
+    const int ml = p.n_lead_l > p.n_lead_r ? p.n_lead_l : p.n_lead_r;
+    if (ml > MIXED_MAX)
+      return fail(NEGF_ERR_ARG, "problem %d: lead blocks of %d DOF exceed the mixed-precision "
+                  "limit of %d", i, ml, MIXED_MAX);
+  }
   if (p.n <= 0 || p.n_layers <= 0 || p.n_inst < 0 || p.n_lead_l <= 0 || p.n_lead_r <= 0 ||
       p.n_c_l <= 0 || p.n_c_r <= 0 || p.n_c_l > p.n || p.n_c_r > p.n || p.maxiter < 0)
     return fail(NEGF_ERR_ARG,
@@ -2899,6 +2910,8 @@ int negf_transmission_multi_batched(void* stream, int n_problems,
   hipLaunchKernelGGL(k_pl_setup, dim3(1), dim3(64), 0, st, a, table);
   rc = hip_check(hipGetLastError(), "k_pl_setup launch");
   if (rc) return rc;
+  bool mixed = false;
+  for (int i = 0; i < n_problems; ++i) mixed = mixed || problems[i].precision == NEGF_MIXED;
   // one launch per phase; the eigenmode fallback (instances whose decimation
   // failed, eigenmode-route problems only) runs between the lead and device phases
   auto run_phase = [&](int phase, bool reset) -> int {
@@ -2909,9 +2922,38 @@ int negf_transmission_multi_batched(void* stream, int n_problems,
     if (r) return r;
     return u.finish();
   };
-  if (!eig) return run_phase(PL_ALL, false);
-  rc = run_phase(PL_LEADS, false);
-  if (rc) return rc;
+  if (!eig && !mixed) return run_phase(PL_ALL, false);
+  if (mixed) {
+    // mixed precision: every problem's surface GFs by the staged decimation of its own
+    // precision (FP32 decimation, FP64 Newton certificate and FP64 redo for mixed
+    // problems; one decimation per mirrored pair), into the workspace slices the
+    // persistent device phase reads
+    for (int i = 0; i < n_problems; ++i) {
+      const negf_transmission_problem& p = problems[i];
+      if (p.n_inst == 0) continue;
+      int slot = 0;
+      while (order[slot] != i) ++slot;
+      const PlProb& q = a.prob[slot];
+      if (sr_mirror_fits(&p)) {
+        rc = sr_mirrored(stream, &p, q.g_l, q.g_r, q.st_l, q.st_r);
+        if (rc) return rc;
+        continue;
+      }
+      rc = negf_sancho_rubio_batched(stream, p.precision, p.n_lead_l, p.n_inst, p.lead_l, p.set_of,
+                                     p.omega, p.eta_lead, p.tol, p.maxiter, p.refine_steps, q.g_l,
+                                     p.dos_l, p.iters_l, q.st_l);
+      if (rc) return rc;
+      if (p.share_leads != NEGF_LEADS_SHARED) {
+        rc = negf_sancho_rubio_batched(stream, p.precision, p.n_lead_r, p.n_inst, p.lead_r,
+                                       p.set_of, p.omega, p.eta_lead, p.tol, p.maxiter,
+                                       p.refine_steps, q.g_r, p.dos_r, p.iters_r, q.st_r);
+        if (rc) return rc;
+      }
+    }
+  } else {
+    rc = run_phase(PL_LEADS, false);
+    if (rc) return rc;
+  }
   for (int k = 0; k < np; ++k) {
     const PlProb& q = a.prob[k];
     if (q.route != NEGF_SURFACE_EIGEN) continue;

@@ -9,8 +9,9 @@ problems cover every block-size class (8, 16, 24, 32, 48 in LDS and 96 in the
 pool), tickets cut short at the end of a problem, frequencies clustered at band
 edges (so the instances of one ticket need different step counts), failures
 (iteration limit, eigenmode fallback), two different leads with explicit
-couplings on contact subsets, more problems than one launch takes, and hipGraph
-replays.  A few points per problem are also checked against the oracle.
+couplings on contact subsets, more problems than one launch takes, hipGraph
+replays and mixed precision (surface GFs by the staged mixed decimation, the rest in
+the persistent launch).  A few points per problem are also checked against the oracle.
 """
 
 import numpy as np
@@ -184,11 +185,62 @@ def test_graph_replays(gpu):
         _assert_same(a, b, "new inputs")
 
 
-def test_mixed_precision_rejected(gpu):
+def _mirrored(seed, n, n_layers, n_w):
+    """Interface-style contacts: the right lead is the left one mirrored."""
+    d00, d01, ons, cps = _device(seed, n, n_layers)
+    ll = np.stack([d00, d01, d01.conj().T])[None]
+    lr = np.stack([d00, d01.conj().T, d01])[None]
+    sets = _engine.TransportSets.from_arrays(
+        ll, lr, d01.conj().T[None], d01[None], np.arange(n), np.arange(n), np.stack(ons)[None],
+        np.stack(cps)[None], False)
+    assert sets.mirror_leads
+    w = np.linspace(0.03, 1.05, n_w) * omega_max(_engine_lead(d00, d01))
+    return sets, w
+
+
+def test_mixed_precision_equals_staged(gpu):
+    """Mixed-precision problems (shared and mirrored leads, classes 8-48) next to an FP64
+    one: every output equals the problem's own staged solve bit for bit (the staged
+    mixed decimation feeds the persistent FP64 stages), eta = 1e-6 and 1e-2 omega (where
+    the FP32 decimation is certified); T within 1e-6 of the oracle."""
+    for rel_eta in (None, 1e-2):
+        jobs, solvers, refs = [], [], []
+        for k, (n, nl, nw) in enumerate(((6, 3, 17), (12, 3, 13), (20, 3, 9), (30, 2, 7),
+                                         (40, 2, 5))):
+            s, w, blocks = _problem(6000 + n, n, nl, nw, edges=False)
+            sm = _engine.TransmissionSolver(s.sets, precision="fp64" if k == 2 else "mixed")
+            eta = 1e-6 if rel_eta is None else rel_eta * w
+            job = list(_job(sm, w))
+            job[3] = torch.as_tensor(np.broadcast_to(eta, w.shape).copy(), device="cuda")
+            jobs.append(tuple(job))
+            solvers.append(sm)
+            refs.append((w, eta, blocks))
+        sets, w = _mirrored(6100, 16, 3, 11)
+        sm = _engine.TransmissionSolver(sets, precision="mixed")
+        eta = 1e-6 if rel_eta is None else rel_eta * w
+        job = list(_job(sm, w))
+        job[3] = torch.as_tensor(np.broadcast_to(eta, w.shape).copy(), device="cuda")
+        jobs.append(tuple(job))
+        staged = [_host(j[0].run(*j[1:])) for j in jobs]
+        multi = [_host(r) for r in _engine.transmission_multi(jobs)]
+        for a, b in zip(multi, staged):
+            _assert_same(a, b, f"mixed, eta/omega={rel_eta}")
+            assert np.all(b["status"] == 0)
+        for (w, eta, (d00, d01, ons, cps)), a in zip(refs, multi):
+            tup = (d00, d01, None)
+            for k in (0, w.size // 2, w.size - 1):
+                e = eta if np.ndim(eta) == 0 else eta[k]
+                ref = o.transmission_sparse(w[k], ons, cps, tup, tup, eta=e)
+                assert abs(a["t"][k] - ref) <= 1e-6 * max(abs(ref), 1e-3)
+
+
+def test_mixed_precision_with_ladder_rejected(gpu):
     s, w, _ = _problem(5000, 8, 2, 4)
     sm = _engine.TransmissionSolver(s.sets, precision="mixed")
+    lad = _engine.EtaLadder([(1e-6, 1e-4)] * w.size)
+    job = _job(sm, w)
     with pytest.raises(ValueError, match="FP64 only"):
-        _engine.transmission_multi([_job(sm, w)])
+        _engine.transmission_multi([job[:5] + (lad,)])
 
 
 def test_transmission_spectra_matches_single_spectra(gpu):
