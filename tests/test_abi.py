@@ -57,9 +57,23 @@ def test_multi_problem_entry_rejects_bad_arguments():
     assert lib.negf_transmission_multi_batched(None, 0, arr, None, 0) == _lib.OK
     assert lib.negf_transmission_multi_batched(None, 9, arr, None, 0) == 1
     assert b"at most 8" in lib.negf_last_error()
-    arr[0].precision = _lib.MIXED
+    arr[0].precision = 5
     assert lib.negf_transmission_multi_batched(None, 2, arr, None, 0) == 1
-    assert b"problem 0" in lib.negf_last_error() and b"FP64 only" in lib.negf_last_error()
+    assert b"problem 0" in lib.negf_last_error() and b"precision" in lib.negf_last_error()
+    arr[0].precision = _lib.MIXED
+    arr[0].refine_steps = -1
+    assert lib.negf_transmission_multi_batched(None, 2, arr, None, 0) == 1
+    assert b"refine_steps" in lib.negf_last_error()
+    arr[0].refine_steps = 2
+    arr[0].n = arr[0].n_lead_l = arr[0].n_lead_r = 60
+    assert lib.negf_transmission_multi_batched(None, 1, arr, None, 0) == 1
+    assert b"mixed-precision limit" in lib.negf_last_error()
+    arr[0].n = arr[0].n_lead_l = arr[0].n_lead_r = 4
+    lad = _lib.EtaLadder()
+    arr[0].ladder = ctypes.pointer(lad)
+    assert lib.negf_transmission_multi_batched(None, 1, arr, None, 0) == 1
+    assert b"eta ladders run in FP64" in lib.negf_last_error()
+    arr[0].ladder = None
     arr[0].precision = _lib.FP64
     arr[0].n = 97
     assert lib.negf_transmission_multi_batched(None, 1, arr, None, 0) == 1
