@@ -167,7 +167,8 @@ def uneven_batch(steps: int, warmup: int) -> dict:
 
 
 def uneven_batch_3d(steps: int, warmup: int) -> dict:
-    """The 3-D uneven batch (workloads.uneven3d_problems): 24-DOF cells on a 32 x 32
+    """The 3-D uneven batch (workloads.uneven3d_problems, silicon-like diamond cells):
+    24-DOF cells on a 32 x 32
     k mesh (1,048,576 instances), 48-DOF on 8 x 8 and 96-DOF on 4 x 4, 1024 omega
     each with a quarter within 1 % of the band top, 8 layers; k-point blocks built
     on the GPU.  Solved by the staged launches problem after problem and by one
@@ -214,9 +215,9 @@ def uneven_batch_3d(steps: int, warmup: int) -> dict:
                for a, b in zip(outs_s, outs_p))
     iters = torch.cat([o.iters_l for o in outs_p])
     failed = int(sum(int((o.status != 0).sum()) for o in outs_p))
-    return {"workload": "3-D uneven batch: simple-cubic cells of 24/48/96 DOF on 32x32/8x8/4x4 "
-                        "k meshes, 8 layers, 1024 omega each (a quarter within 1% of the band "
-                        "top), eta=1e-6, FP64",
+    return {"workload": "3-D uneven batch: silicon-like diamond cells (1, 2 and 4 conventional "
+                        "cubes) of 24/48/96 DOF on 32x32/8x8/4x4 k meshes, 8 layers, 1024 "
+                        "omega each (a quarter within 1% of the band top), eta=1e-6, FP64",
             "instances": n_inst, "instances_per_part": sizes,
             "decimation_steps": [int(iters.min()), int(iters.max())],
             "failed_instances": failed,
