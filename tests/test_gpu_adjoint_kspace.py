@@ -98,7 +98,7 @@ def test_block_gradients_match_dense_autograd(gpu):
                                  cpl0[s].detach().cpu().numpy(), tuple(p.lead_left[s]),
                                  tuple(p.lead_right[s]), eta=p.eta, v_left=p.v_left[s],
                                  v_right=p.v_right[s])
-        assert abs(float(t[i]) - tr) <= 1e-9 * max(abs(tr), 1e-3)
+        assert abs(float(t[i].detach()) - tr) <= 1e-9 * max(abs(tr), 1e-3)
         for g, ref in ((ons.grad, go), (cpl.grad, gc)):
             g = g.cpu().numpy()
             scale = max(np.abs(ref).max(), 1e-12)
