@@ -115,6 +115,16 @@ int negf_set_kernel_family(int precision, int n, int family);
 int negf_kernel_family(int precision, int n);
 
 /*
+ * Block products of the FP64 workgroup decimation and sweep (classes 8..48): mode 1
+ * (default) runs them on the matrix cores (v_mfma_f64_4x4x4f64, complex products as
+ * four real chains), mode 0 as register-tiled FMA loops on the vector ALUs.  Both give
+ * the same results to rounding.  Returns the previous mode (-1 and NEGF_ERR_ARG text
+ * for a bad mode); process-wide.  No counterpart in the reference (a
+ * launch-configuration switch, like negf_set_wave_kernels).
+ */
+int negf_set_matrix_core_products(int mode);
+
+/*
  * Surface Green's function of semi-infinite leads by Lopez-Sancho decimation.
  * Replaces surface_gf_sancho_rubio, src/negfpy/core/surface_gf.py:18-81
  * (and the per-k loop of lead_surface_dos*, src/negfpy/core/dos.py:31-251).

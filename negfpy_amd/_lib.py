@@ -81,6 +81,7 @@ _SIGS = {
     "negf_set_block_class": ([_c_int, _c_int, _c_int], _c_int),
     "negf_block_class": ([_c_int, _c_int], _c_int),
     "negf_set_wave_kernels": ([_c_int], _c_int),
+    "negf_set_matrix_core_products": ([_c_int], _c_int),
     "negf_set_kernel_family": ([_c_int, _c_int, _c_int], _c_int),
     "negf_kernel_family": ([_c_int, _c_int], _c_int),
     "negf_sancho_rubio_batched": (

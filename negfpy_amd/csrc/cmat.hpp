@@ -247,6 +247,75 @@ __device__ __forceinline__ void group_argmax(T& best, int& idx) {
   if constexpr (RG >= 16) argmax_step<T, 0x140>(best, idx); // row_mirror: the other half-row
 }
 
+// Arg-max over aligned groups of RG lanes on 64-bit keys: the bits of a non-negative
+// magnitude with the low 6 bits replaced by 63 - row, so that the largest key is the
+// largest magnitude (compared to 2^-46 relative) and, among those, the first row.
+// A NaN magnitude takes the key of +inf.  Every lane of a group ends with the key.
+__device__ __forceinline__ unsigned long long pivot_key(double mg, int row) {
+  const unsigned long long b =
+      (mg != mg) ? 0x7FF0000000000000ull : (unsigned long long)__double_as_longlong(mg);
+  return (b & ~63ull) | (unsigned long long)(63 - row);
+}
+constexpr unsigned long long PIVOT_KEY_INF = 0x7FF0000000000000ull;
+
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
+  const int lo = dpp_i<CTRL>((int)(v & 0xffffffffull));
+  const int hi = dpp_i<CTRL>((int)(v >> 32));
+  return ((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo;
+}
+template <int CTRL>
+__device__ __forceinline__ void keymax_step(unsigned long long& k) {
+  const unsigned long long o = dpp_u64<CTRL>(k);
+  k = o > k ? o : k;
+}
+template <int RG>
+__device__ __forceinline__ void group_keymax(unsigned long long& k) {
+  if constexpr (RG >= 2) keymax_step<0xB1>(k);
+  if constexpr (RG >= 4) keymax_step<0x4E>(k);
+  if constexpr (RG >= 8) keymax_step<0x141>(k);
+  if constexpr (RG >= 16) keymax_step<0x140>(k);
+}
+
+// Broadcast lane gp (wave-uniform, < RG) of every aligned group of RG lanes to its
+// group, for the four dwords of a complex double, by DPP: a quad broadcast (one
+// scalar branch on gp & 3 picks its lane), then the half-row and row mirrors bring in
+// the other quad and the other half.  Must run with every lane of the group active.
+template <int Q>
+__device__ __forceinline__ void quad_bcast4(int (&v)[4]) {
+#pragma unroll
+  for (int d = 0; d < 4; ++d) v[d] = dpp_i<Q * 0x55>(v[d]);
+}
+template <int RG>
+__device__ __forceinline__ void group_bcast(double& vr, double& vi, int gp, int lane) {
+  const long long ur = __double_as_longlong(vr), ui = __double_as_longlong(vi);
+  int v[4] = {(int)(ur & 0xffffffffLL), (int)(ur >> 32), (int)(ui & 0xffffffffLL), (int)(ui >> 32)};
+  switch (gp & 3) {
+    case 0: quad_bcast4<0>(v); break;
+    case 1: quad_bcast4<1>(v); break;
+    case 2: quad_bcast4<2>(v); break;
+    default: quad_bcast4<3>(v); break;
+  }
+  if constexpr (RG >= 8) {
+    const bool keep = (lane & 4) == (gp & 4);  // this quad holds lane gp's half-row copy
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      const int u = dpp_i<0x141>(v[d]);
+      v[d] = keep ? v[d] : u;
+    }
+  }
+  if constexpr (RG >= 16) {
+    const bool keep = (lane & 8) == (gp & 8);
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      const int u = dpp_i<0x140>(v[d]);
+      v[d] = keep ? v[d] : u;
+    }
+  }
+  vr = __longlong_as_double(((long long)v[1] << 32) | (unsigned int)v[0]);
+  vi = __longlong_as_double(((long long)v[3] << 32) | (unsigned int)v[2]);
+}
+
 // Pivot of one elimination step, double-buffered by step parity in LDS.
 template <typename T>
 struct Pivot {
@@ -359,18 +428,31 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
       }
       T best = T(-1);
       int bidx = n;
-      bool bad = false;
+      if constexpr (SF && RG <= 16 && NMAX <= 64) {
+        unsigned long long key = 0;
 #pragma unroll
-      for (int m = 0; m < MAXR; ++m) {
-        const T mg = fabs(xr[m]) + fabs(xi[m]);
-        const bool live = !((used >> m) & 1u);
-        bad = bad || (live && mg != mg);
-        const bool better = live && mg > best;
-        best = better ? mg : best;
-        bidx = better ? g + RG * m : bidx;
+        for (int m = 0; m < MAXR; ++m) {
+          const bool live = !((used >> m) & 1u);
+          const unsigned long long kb = pivot_key((double)(fabs(xr[m]) + fabs(xi[m])), g + RG * m);
+          key = (live && kb > key) ? kb : key;
+        }
+        group_keymax<RG>(key);
+        bidx = 63 - (int)(key & 63ull);
+        best = key >= PIVOT_KEY_INF ? T(INFINITY) : (key < 64ull ? T(0) : T(1));
+      } else {
+        bool bad = false;
+#pragma unroll
+        for (int m = 0; m < MAXR; ++m) {
+          const T mg = fabs(xr[m]) + fabs(xi[m]);
+          const bool live = !((used >> m) & 1u);
+          bad = bad || (live && mg != mg);
+          const bool better = live && mg > best;
+          best = better ? mg : best;
+          bidx = better ? g + RG * m : bidx;
+        }
+        if (bad) best = T(INFINITY);  // a NaN anywhere in the column wins and marks it
+        group_argmax<T, RG>(best, bidx);
       }
-      if (bad) best = T(INFINITY);  // a NaN anywhere in the column wins and marks it
-      group_argmax<T, RG>(best, bidx);
       if (g == 0 || (bidx < n && g == bidx % RG)) {
         const int flag = isinf(best) ? ST_NONFINITE : (best > T(0) ? ST_OK : ST_SINGULAR);
         if (g == 0 && flag != ST_OK) {
@@ -410,7 +492,13 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
       cr[m] = b.colr[par][g + RG * m];
       ci[m] = b.coli[par][g + RG * m];
     }
-    const Pivot<T> pk = b.piv[par];
+    Pivot<T> pk = b.piv[par];
+    if constexpr (SF) {
+      // one LDS round trip: the column and the pivot are loaded before the branch
+#pragma unroll
+      for (int m = 0; m < MAXR; ++m) asm volatile("" : "+v"(cr[m]), "+v"(ci[m]));
+      asm volatile("" : "+v"(pk.ir), "+v"(pk.ii), "+v"(pk.p), "+v"(pk.flag));
+    }
     const int flag = __builtin_amdgcn_readfirstlane(pk.flag);
     if (flag != ST_OK) {
       if constexpr (LS) {
@@ -435,8 +523,15 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
           vi = xi[m];
         }
       // pivot row entry of this lane's column, from the group lane holding row p
-      const T pr = lane_shfl(vr, gbase + gp);
-      const T pi = lane_shfl(vi, gbase + gp);
+      T pr, pi;
+      if constexpr (sizeof(T) == 8 && (RG == 4 || RG == 8 || RG == 16)) {
+        pr = vr;
+        pi = vi;
+        group_bcast<RG>(pr, pi, gp, tid & (WAVE - 1));
+      } else {
+        pr = lane_shfl(vr, gbase + gp);
+        pi = lane_shfl(vi, gbase + gp);
+      }
       const bool jk = (j == k);
       const T sr = jk ? pk.ir : pr * pk.ir - pi * pk.ii;
       const T si = jk ? pk.ii : pr * pk.ii + pi * pk.ir;

@@ -32,6 +32,7 @@
 #include <mutex>
 
 #include "cmat.hpp"
+#include "mm.hpp"
 #include "wave.hpp"
 #include "../../include/negf_hip.h"
 
@@ -468,6 +469,207 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
     if (do2)
       sr_write<T, NMAX, TM>(mr, mi, t, s, n, zr, zi, w, a.g_out2 + (size_t)inst * nn * 2,
                             a.dos_out2, a.status2, inst, tid);
+  }
+}
+
+// g = (z - e_s)^-1 from e_s held in the matrix-core ownership (FP64), and the
+// surface DOS (omega / pi) tr(-2 Im g).
+template <int NMAX, typename L>
+__device__ __forceinline__ void sr_write_mm(const double (&er)[MOwn<NMAX, Cfg<NMAX>::NT>::TPW],
+                                            const double (&ei)[MOwn<NMAX, Cfg<NMAX>::NT>::TPW],
+                                            const MOwn<NMAX, Cfg<NMAX>::NT>& o, L& s, int n,
+                                            double zr, double zi, double omega, double* out,
+                                            double* dos, int32_t* status, int inst, int tid) {
+  constexpr int NT = Cfg<NMAX>::NT;
+  constexpr int LD = L::LD;
+  __syncthreads();  // earlier reads of plane 2 are done
+#pragma unroll
+  for (int q = 0; q < MOwn<NMAX, NT>::TPW; ++q) {
+    const int i = o.row(q), j = o.col(q);
+    if (i < n && j < n) {
+      s.re[2][i * LD + j] = (i == j ? zr : 0.0) - er[q];
+      s.im[2][i * LD + j] = (i == j ? zi : 0.0) - ei[q];
+    }
+  }
+  __syncthreads();
+  const int st = gj_inverse<double, NT, NMAX, L::RG>(s.re[2], s.im[2], LD, n, *s.gj, tid);
+  if (st != ST_OK) {
+    if (tid == 0) status[inst] = st;
+    return;
+  }
+  gstore<double, NT>(out, n, s.re[2], s.im[2], LD, tid);
+  if (tid == 0 && dos) {
+    double tr = 0.0;
+    for (int i = 0; i < n; ++i) tr += -2.0 * s.im[2][i * LD + i];
+    dos[inst] = (omega / M_PI) * tr;
+  }
+}
+
+// The FP64 decimation with every block product on the matrix cores (mm.hpp): the
+// same recurrence, step for step, as k_sancho_rubio; e, e_s (and the mirror's
+// e~_s) are held in the matrix-core ownership.
+template <int NMAX, bool MIRROR>
+__global__ void __launch_bounds__(Cfg<NMAX>::NT)
+    __attribute__((amdgpu_waves_per_eu(Occ<double, NMAX>::value))) k_sr_mm(SrArgs a) {
+  using T = double;
+  constexpr int NT = Cfg<NMAX>::NT;
+  using L = Lds<T, NMAX, NT, 3>;
+  using M = MOwn<NMAX, NT>;
+  constexpr int LD = L::LD, TPW = M::TPW;
+  extern __shared__ __align__(16) unsigned char smem[];
+  L s(smem);
+  const int tid = threadIdx.x;
+  const int inst = blockIdx.x;
+  const bool do1 = !a.redo || a.status[inst] != ST_OK;
+  bool do2 = false;
+  if constexpr (MIRROR) do2 = !a.redo || a.status2[inst] != ST_OK;
+  if (!do1 && !do2) return;
+  const int n = a.n;
+  const int ns = (n + 3) / 4;
+  const size_t nn = (size_t)n * n;
+  const double* blk = a.lead + (size_t)a.set_of[inst] * 3 * nn * 2;
+
+  lds_zero<T, NT>(smem, L::mat_bytes(), tid);
+  __syncthreads();
+  // alpha = d01 -> [0], beta = d10 -> [1]; g, then T1 = alpha g, then T2 = beta g -> [2]
+  gload<T, NT>(blk + nn * 2, n, n, s.re[0], s.im[0], LD, tid);
+  gload<T, NT>(blk + 2 * nn * 2, n, n, s.re[1], s.im[1], LD, tid);
+
+  const M o(tid);
+  T er[TPW], ei[TPW], sr[TPW], si[TPW];
+  T mr[MIRROR ? TPW : 1], mi[MIRROR ? TPW : 1];
+  {
+    const double2* d00 = reinterpret_cast<const double2*>(blk);
+#pragma unroll
+    for (int q = 0; q < TPW; ++q) {
+      const int i = o.row(q), j = o.col(q);
+      double2 v = make_double2(0.0, 0.0);
+      if (i < n && j < n) v = d00[i * n + j];
+      er[q] = v.x;
+      ei[q] = v.y;
+      sr[q] = v.x;
+      si[q] = v.y;
+      if constexpr (MIRROR) {
+        mr[q] = v.x;
+        mi[q] = v.y;
+      }
+    }
+  }
+  double zr, zi;
+  zsq(a.omega[inst], a.eta[inst], zr, zi);
+  const T tol = a.tol;
+  __syncthreads();
+
+  int status = ST_OK;
+  int it = 0;
+  bool conv = false;
+  while (it < a.maxiter) {
+    ++it;
+    // G = z - e
+#pragma unroll
+    for (int q = 0; q < TPW; ++q) {
+      const int i = o.row(q), j = o.col(q);
+      if (i < n && j < n) {
+        s.re[2][i * LD + j] = (i == j ? zr : T(0)) - er[q];
+        s.im[2][i * LD + j] = (i == j ? zi : T(0)) - ei[q];
+      }
+    }
+    __syncthreads();
+    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[2], s.im[2], LD, n, *s.gj, tid);
+    if (st != ST_OK) {
+      status = st;
+      break;
+    }
+    // T1 = a g, T2 = b g
+    T t1r[TPW], t1i[TPW], t2r[TPW], t2i[TPW];
+    mzero<TPW>(t1r, t1i);
+    mzero<TPW>(t2r, t2i);
+    mm2b<NMAX, NT, OPN, OPN>(s.re[0], s.im[0], s.re[1], s.im[1], s.re[2], s.im[2], LD, ns, o, t1r,
+                             t1i, t2r, t2i);
+    __syncthreads();
+    mstore<NMAX, NT>(s.re[2], s.im[2], LD, n, o, t1r, t1i);
+    __syncthreads();
+    // P1 = T1 b (into e_s and e at once), P3 = T1 a (the next alpha)
+    T p1r[TPW], p1i[TPW], p3r[TPW], p3i[TPW];
+    mzero<TPW>(p1r, p1i);
+    mzero<TPW>(p3r, p3i);
+    mm2a<NMAX, NT>(s.re[2], s.im[2], s.re[1], s.im[1], s.re[0], s.im[0], LD, ns, o, p1r, p1i, p3r,
+                   p3i);
+#pragma unroll
+    for (int q = 0; q < TPW; ++q) {
+      sr[q] = sr[q] + p1r[q];
+      si[q] = si[q] + p1i[q];
+      er[q] = er[q] + p1r[q];
+      ei[q] = ei[q] + p1i[q];
+    }
+    __syncthreads();
+    mstore<NMAX, NT>(s.re[2], s.im[2], LD, n, o, t2r, t2i);
+    __syncthreads();
+    // P2 = T2 a (into e), P4 = T2 b (the next beta)
+    T p2r[TPW], p2i[TPW], p4r[TPW], p4i[TPW];
+    mzero<TPW>(p2r, p2i);
+    mzero<TPW>(p4r, p4i);
+    mm2a<NMAX, NT>(s.re[2], s.im[2], s.re[0], s.im[0], s.re[1], s.im[1], LD, ns, o, p2r, p2i, p4r,
+                   p4i);
+    __syncthreads();
+    bool bad = false;
+    double emax = 0.0;
+#pragma unroll
+    for (int q = 0; q < TPW; ++q) {
+      const int i = o.row(q), j = o.col(q);
+      if (i < n && j < n) {
+        er[q] = er[q] + p2r[q];
+        ei[q] = ei[q] + p2i[q];
+        if constexpr (MIRROR) {
+          mr[q] = mr[q] + p2r[q];
+          mi[q] = mi[q] + p2i[q];
+          bad |= !(finite_d(mr[q]) && finite_d(mi[q]));
+        }
+        s.re[0][i * LD + j] = p3r[q];
+        s.im[0][i * LD + j] = p3i[q];
+        s.re[1][i * LD + j] = p4r[q];
+        s.im[1][i * LD + j] = p4i[q];
+        bad |= !(finite_d(sr[q]) && finite_d(si[q]) && finite_d(er[q]) && finite_d(ei[q]) &&
+                 finite_d(p3r[q]) && finite_d(p3i[q]) && finite_d(p4r[q]) && finite_d(p4i[q]));
+        const T m3 = p3r[q] * p3r[q] + p3i[q] * p3i[q];
+        const T m4 = p4r[q] * p4r[q] + p4i[q] * p4i[q];
+        emax = fmax(emax, fmax(m3, m4));
+      }
+    }
+    double gmax;
+    bool gbad;
+    block_max_any<NT>(emax, bad, s.sred, s.sflag, tid, gmax, gbad);
+    if (gbad) {
+      status = ST_NONFINITE;
+      break;
+    }
+    if (sqrt(gmax) < tol) {
+      conv = true;
+      break;
+    }
+  }
+  if (status == ST_OK && !conv) status = ST_NOCONV;
+  if (tid == 0) {
+    if (do1) {
+      a.status[inst] = status;
+      a.iters[inst] = it;
+    }
+    if constexpr (MIRROR) {
+      if (do2) {
+        a.status2[inst] = status;
+        a.iters2[inst] = it;
+      }
+    }
+  }
+  if (status != ST_OK) return;
+  const double w = a.omega[inst];
+  if (do1)
+    sr_write_mm<NMAX>(sr, si, o, s, n, zr, zi, w, a.g_out + (size_t)inst * nn * 2, a.dos_out,
+                      a.status, inst, tid);
+  if constexpr (MIRROR) {
+    if (do2)
+      sr_write_mm<NMAX>(mr, mi, o, s, n, zr, zi, w, a.g_out2 + (size_t)inst * nn * 2, a.dos_out2,
+                        a.status2, inst, tid);
   }
 }
 
@@ -956,6 +1158,119 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
       __syncthreads();
       tstore<T, TM>(s.re[2], s.im[2], LD, n, n, t, qr, qi);
     } else {
+      __syncthreads();
+    }
+  }
+  if (tid == 0) a.status[inst] = status;
+}
+
+// The same sweep in FP64 with every block product on the matrix cores (mm.hpp):
+// Y = g H, then S = H^H Y and Q = Q Y from one pass over Y.  The Schur term S
+// stays in registers in the matrix-core ownership.
+template <int NMAX>
+__global__ void __launch_bounds__(Cfg<NMAX>::NT)
+    __attribute__((amdgpu_waves_per_eu(Occ3<double, NMAX>::value))) k_rgf_mm(RgfArgs a) {
+  using T = double;
+  constexpr int NT = Cfg<NMAX>::NT;
+  using L = Lds<T, NMAX, NT, 3>;
+  using M = MOwn<NMAX, NT>;
+  constexpr int LD = L::LD, TPW = M::TPW;
+  extern __shared__ __align__(16) unsigned char smem[];
+  L s(smem);
+  const int tid = threadIdx.x;
+  const int inst = blockIdx.x;
+  if (a.redo && a.status[inst] != 10 + ST_NONFINITE && a.status[inst] != 10 + ST_SINGULAR) return;
+  if (a.status_in && a.status_in[inst] != ST_OK) {
+    if (tid == 0) a.status[inst] = a.status_in[inst];
+    return;
+  }
+  const int n = a.n, nl = a.n_layers;
+  const int ns = (n + 3) / 4;
+  const size_t nn = (size_t)n * n;
+  const size_t set = (size_t)a.set_of[inst];
+  const double2* ons = reinterpret_cast<const double2*>(a.onsite + set * nl * nn * 2);
+  const double* cpl = a.coupling + set * (nl > 1 ? nl - 1 : 0) * nn * 2;
+  const double2* sgl = reinterpret_cast<const double2*>(a.sigma_l + (size_t)inst * nn * 2);
+  const double2* sgr = reinterpret_cast<const double2*>(a.sigma_r + (size_t)inst * nn * 2);
+  // LDS: [0] g (inverted in place), then Y = g H; [1] H; [2] Q = G_{1,i} H_i
+  lds_zero<T, NT>(smem, L::mat_bytes(), tid);
+  __syncthreads();
+  const M o(tid);
+  double zrd, zid;
+  zsq(a.omega[inst], a.eta_dev[inst], zrd, zid);
+  const T zr = zrd, zi = zid;
+  T sr[TPW], si[TPW];
+  mzero<TPW>(sr, si);
+  int status = ST_OK;
+  for (int layer = 0; layer < nl; ++layer) {
+    {
+      const double2* d = ons + (size_t)layer * nn;
+#pragma unroll
+      for (int q = 0; q < TPW; ++q) {
+        const int i = o.row(q), j = o.col(q);
+        if (i < n && j < n) {
+          const double2 dv = d[i * n + j];
+          s.re[0][i * LD + j] = (i == j ? zr : T(0)) - dv.x - sr[q];
+          s.im[0][i * LD + j] = (i == j ? zi : T(0)) - dv.y - si[q];
+        }
+      }
+    }
+    if (layer == 0 || layer == nl - 1) {
+      __syncthreads();
+      for (int e = tid; e < (int)nn; e += NT) {
+        const int i = e / n, j = e - (e / n) * n;
+        T vr = s.re[0][i * LD + j], vi = s.im[0][i * LD + j];
+        if (layer == 0) {
+          vr -= sgl[e].x;
+          vi -= sgl[e].y;
+        }
+        if (layer == nl - 1) {
+          vr -= sgr[e].x;
+          vi -= sgr[e].y;
+        }
+        s.re[0][i * LD + j] = vr;
+        s.im[0][i * LD + j] = vi;
+      }
+    }
+    __syncthreads();
+    const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[0], s.im[0], LD, n, *s.gj, tid);
+    if (st != ST_OK) {
+      status = 10 + st;
+      break;
+    }
+    if (layer == nl - 1) {
+      // G_1N = Q_{N-1} g_N (g_1 itself for one layer), through LDS to a coalesced store
+      if (nl > 1) {
+        T gr[TPW], gi[TPW];
+        mzero<TPW>(gr, gi);
+        mm<NMAX, NT, OPN, OPN>(s.re[2], s.im[2], s.re[0], s.im[0], LD, ns, o, gr, gi);
+        mstore<NMAX, NT>(s.re[1], s.im[1], LD, n, o, gr, gi);
+        __syncthreads();
+        gstore<T, NT>(a.g1n_out + (size_t)inst * nn * 2, n, s.re[1], s.im[1], LD, tid);
+      } else {
+        gstore<T, NT>(a.g1n_out + (size_t)inst * nn * 2, n, s.re[0], s.im[0], LD, tid);
+      }
+      break;
+    }
+    gload<T, NT>(cpl + (size_t)layer * nn * 2, n, n, s.re[1], s.im[1], LD, tid);
+    __syncthreads();
+    T yr[TPW], yi[TPW];
+    mzero<TPW>(yr, yi);
+    mm<NMAX, NT, OPN, OPN>(s.re[0], s.im[0], s.re[1], s.im[1], LD, ns, o, yr, yi);
+    __syncthreads();  // every read of g is done: Y replaces it (and starts Q at layer 0)
+    mstore<NMAX, NT>(s.re[0], s.im[0], LD, n, o, yr, yi);
+    if (layer == 0) mstore<NMAX, NT>(s.re[2], s.im[2], LD, n, o, yr, yi);
+    __syncthreads();
+    mzero<TPW>(sr, si);
+    if (layer > 0) {
+      T qr[TPW], qi[TPW];
+      mzero<TPW>(qr, qi);
+      mm2b<NMAX, NT, OPH, OPN>(s.re[1], s.im[1], s.re[2], s.im[2], s.re[0], s.im[0], LD, ns, o, sr,
+                               si, qr, qi);
+      __syncthreads();
+      mstore<NMAX, NT>(s.re[2], s.im[2], LD, n, o, qr, qi);
+    } else {
+      mm<NMAX, NT, OPH, OPN>(s.re[1], s.im[1], s.re[0], s.im[0], LD, ns, o, sr, si);
       __syncthreads();
     }
   }
@@ -1872,6 +2187,10 @@ static bool use_wave(int n, bool fp64, bool sweep) {
   if (sweep) return n <= 24;
   return n <= 12 || (n >= 17 && n <= 24);
 }
+// Matrix-core products (mm.hpp) in the FP64 workgroup decimation and sweep of
+// classes 8..48: mode 1 (default) on, 0 off (negf_set_matrix_core_products).
+static std::atomic<int> g_mm_products{1};
+static bool use_mm(int nmax) { return g_mm_products.load() == 1 && nmax >= 8 && nmax <= 48; }
 // rows per lane of the wave inversion: n <= 8 R
 static int wave_rows(int n) { return n <= 16 ? 2 : (n <= 24 ? 3 : 4); }
 
@@ -1989,6 +2308,24 @@ static int run_sr(hipStream_t st, int nmax, const SrArgs& a) {
   }
   if (MIRROR && !sr_mirror_class(nmax))
     return fail(NEGF_ERR_ARG, "no mirrored decimation for block class %d", nmax);
+  if constexpr (sizeof(T) == 8) {
+    if (use_mm(nmax)) {
+#define SRM_CASE(NM)                                                                        \
+  case NM: {                                                                                \
+    constexpr int NT = Cfg<NM>::NT;                                                         \
+    NEGF_LAUNCH((k_sr_mm<NM, MIRROR>), a.n_inst, NT, (Lds<double, NM, NT, 3>::bytes()), st, a); \
+    return NEGF_OK;                                                                         \
+  }
+      switch (nmax) {
+        SRM_CASE(8)
+        SRM_CASE(16)
+        SRM_CASE(24)
+        SRM_CASE(32)
+        SRM_CASE(48)
+      }
+#undef SRM_CASE
+    }
+  }
   switch (nmax) {
     SR_CASE(8)
     SR_CASE(16)
@@ -2080,6 +2417,24 @@ static int run_rgf(hipStream_t st, int nmax, const RgfArgs& a) {
       case 2: NEGF_LAUNCH((wpi::k_rgf_wave<T, 2>), a.n_inst, WAVE, lds, st, a); return NEGF_OK;
       case 3: NEGF_LAUNCH((wpi::k_rgf_wave<T, 3>), a.n_inst, WAVE, lds, st, a); return NEGF_OK;
       default: NEGF_LAUNCH((wpi::k_rgf_wave<T, 4>), a.n_inst, WAVE, lds, st, a); return NEGF_OK;
+    }
+  }
+  if constexpr (sizeof(T) == 8) {
+    if (use_mm(nmax)) {
+#define RGM_CASE(NM)                                                                       \
+  case NM: {                                                                               \
+    constexpr int NT = Cfg<NM>::NT;                                                        \
+    NEGF_LAUNCH((k_rgf_mm<NM>), a.n_inst, NT, (Lds<double, NM, NT, 3>::bytes()), st, a);   \
+    return NEGF_OK;                                                                        \
+  }
+      switch (nmax) {
+        RGM_CASE(8)
+        RGM_CASE(16)
+        RGM_CASE(24)
+        RGM_CASE(32)
+        RGM_CASE(48)
+      }
+#undef RGM_CASE
     }
   }
   switch (nmax) {
@@ -2214,6 +2569,12 @@ int negf_set_kernel_family(int precision, int n, int family) {
 int negf_kernel_family(int precision, int n) {
   if (precision != NEGF_FP64 && precision != NEGF_MIXED) return 0;
   return use_wave(n, precision == NEGF_FP64, false) ? NEGF_FAMILY_WAVE : NEGF_FAMILY_WORKGROUP;
+}
+
+int negf_set_matrix_core_products(int mode) {
+  if (mode < 0 || mode > 1)
+    return fail(NEGF_ERR_ARG, "matrix-core product mode %d is not 0 or 1", mode), -1;
+  return g_mm_products.exchange(mode);
 }
 
 int negf_set_wave_kernels(int mode) {

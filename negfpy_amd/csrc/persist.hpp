@@ -17,7 +17,9 @@
 // switched off.  The decimation loop runs until no sub-block of the ticket is
 // still iterating (a per-instance convergence flag in LDS); each instance stops
 // at its own step, so its result and step count are those of the staged kernel.
-// The arithmetic of every stage is the staged kernels' own, in the same order.
+// The arithmetic of every stage is the staged kernels' own, in the same order: the
+// decimation and the sweep with their block products on the matrix cores (mm.hpp), as
+// the staged kernels run them by default.
 // Consecutive stages reuse the same LDS matrices, so each stage opens with a
 // barrier: the previous stage's last reads (its output stores from LDS) must be
 // done before the matrices are cleared.
@@ -119,9 +121,12 @@ template <int NMAX, class L>
 __device__ int pl_sr(const SrArgs& a, int inst, bool active, L& s, int tid, PlShared& ps, int sub,
                      int nsub) {
   using T = double;
-  constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
+  constexpr int NT = Cfg<NMAX>::NT;
   constexpr int LD = L::LD;
+  using M = MOwn<NMAX, NT>;
+  constexpr int TPW = M::TPW;
   const int n = a.n;
+  const int ns = (n + 3) / 4;
   const size_t nn = (size_t)n * n;
   const double* blk = a.lead + (active ? (size_t)a.set_of[inst] * 3 * nn * 2 : 0);
   __syncthreads();  // the previous stage's last reads of these matrices are done
@@ -131,30 +136,21 @@ __device__ int pl_sr(const SrArgs& a, int inst, bool active, L& s, int tid, PlSh
     gload<T, NT>(blk + nn * 2, n, n, s.re[0], s.im[0], LD, tid);
     gload<T, NT>(blk + 2 * nn * 2, n, n, s.re[1], s.im[1], LD, tid);
   }
-  const Tile<TM> t(n, n, tid);
-  T er[TM][TM], ei[TM][TM], sr[TM][TM], si[TM][TM];
-  tzero<T, TM>(er, ei);
-  if (active && t.own) {
+  const M o(tid);
+  T er[TPW], ei[TPW], sr[TPW], si[TPW];
+  {
     const double2* d00 = reinterpret_cast<const double2*>(blk);
 #pragma unroll
-    for (int x = 0; x < TM; ++x)
-#pragma unroll
-      for (int y = 0; y < TM; ++y) {
-        const int i = t.i0 + x, j = t.j0 + y;
-        if (i < n && j < n) {
-          const double2 v = d00[i * n + j];
-          er[x][y] = v.x;
-          ei[x][y] = v.y;
-        }
-      }
-  }
-#pragma unroll
-  for (int x = 0; x < TM; ++x)
-#pragma unroll
-    for (int y = 0; y < TM; ++y) {
-      sr[x][y] = er[x][y];
-      si[x][y] = ei[x][y];
+    for (int q = 0; q < TPW; ++q) {
+      const int i = o.row(q), j = o.col(q);
+      double2 v = make_double2(0.0, 0.0);
+      if (active && i < n && j < n) v = d00[i * n + j];
+      er[q] = v.x;
+      ei[q] = v.y;
+      sr[q] = v.x;
+      si[q] = v.y;
     }
+  }
   double zr = 0.0, zi = 0.0;
   if (active) zsq(a.omega[inst], a.eta[inst], zr, zi);
   const T tol = a.tol;
@@ -175,17 +171,15 @@ __device__ int pl_sr(const SrArgs& a, int inst, bool active, L& s, int tid, PlSh
     if (!any) break;
     if (run) ++it;
     // G = z - e
-    if (run && t.own) {
+    if (run) {
 #pragma unroll
-      for (int x = 0; x < TM; ++x)
-#pragma unroll
-        for (int y = 0; y < TM; ++y) {
-          const int i = t.i0 + x, j = t.j0 + y;
-          if (i < n && j < n) {
-            s.re[2][i * LD + j] = (i == j ? zr : T(0)) - er[x][y];
-            s.im[2][i * LD + j] = (i == j ? zi : T(0)) - ei[x][y];
-          }
+      for (int q = 0; q < TPW; ++q) {
+        const int i = o.row(q), j = o.col(q);
+        if (i < n && j < n) {
+          s.re[2][i * LD + j] = (i == j ? zr : T(0)) - er[q];
+          s.im[2][i * LD + j] = (i == j ? zi : T(0)) - ei[q];
         }
+      }
     }
     __syncthreads();
     const int st = gj_inverse_ls<T, NT, NMAX, L::RG>(s.re[2], s.im[2], LD, n, *s.gj, tid, run);
@@ -194,67 +188,61 @@ __device__ int pl_sr(const SrArgs& a, int inst, bool active, L& s, int tid, PlSh
       run = false;
     }
     // T1 = a g, T2 = b g
-    T t1r[TM][TM], t1i[TM][TM], t2r[TM][TM], t2i[TM][TM];
-    tzero<T, TM>(t1r, t1i);
-    tzero<T, TM>(t2r, t2i);
-    if (run && t.own)
-      mac2b<T, TM>(s.re[0], s.im[0], s.re[1], s.im[1], s.re[2], s.im[2], LD, n, t.i0, t.j0, t1r, t1i,
-                   t2r, t2i);
+    T t1r[TPW], t1i[TPW], t2r[TPW], t2i[TPW];
+    mzero<TPW>(t1r, t1i);
+    mzero<TPW>(t2r, t2i);
+    if (run)
+      mm2b<NMAX, NT, OPN, OPN>(s.re[0], s.im[0], s.re[1], s.im[1], s.re[2], s.im[2], LD, ns, o,
+                               t1r, t1i, t2r, t2i);
     __syncthreads();
-    if (run) tstore<T, TM>(s.re[2], s.im[2], LD, n, n, t, t1r, t1i);
+    if (run) mstore<NMAX, NT>(s.re[2], s.im[2], LD, n, o, t1r, t1i);
     __syncthreads();
     // P1 = T1 b (into e_s and e at once), P3 = T1 a (the next alpha)
-    T p1r[TM][TM], p1i[TM][TM], p3r[TM][TM], p3i[TM][TM];
-    tzero<T, TM>(p1r, p1i);
-    tzero<T, TM>(p3r, p3i);
-    if (run && t.own)
-      mac2a<T, TM>(s.re[2], s.im[2], s.re[1], s.im[1], s.re[0], s.im[0], LD, n, t.i0, t.j0, p1r, p1i,
-                   p3r, p3i);
+    T p1r[TPW], p1i[TPW], p3r[TPW], p3i[TPW];
+    mzero<TPW>(p1r, p1i);
+    mzero<TPW>(p3r, p3i);
     if (run) {
+      mm2a<NMAX, NT>(s.re[2], s.im[2], s.re[1], s.im[1], s.re[0], s.im[0], LD, ns, o, p1r, p1i,
+                     p3r, p3i);
 #pragma unroll
-      for (int x = 0; x < TM; ++x)
-#pragma unroll
-        for (int y = 0; y < TM; ++y) {
-          sr[x][y] = sr[x][y] + p1r[x][y];
-          si[x][y] = si[x][y] + p1i[x][y];
-          er[x][y] = er[x][y] + p1r[x][y];
-          ei[x][y] = ei[x][y] + p1i[x][y];
-        }
+      for (int q = 0; q < TPW; ++q) {
+        sr[q] = sr[q] + p1r[q];
+        si[q] = si[q] + p1i[q];
+        er[q] = er[q] + p1r[q];
+        ei[q] = ei[q] + p1i[q];
+      }
     }
     __syncthreads();
-    if (run) tstore<T, TM>(s.re[2], s.im[2], LD, n, n, t, t2r, t2i);
+    if (run) mstore<NMAX, NT>(s.re[2], s.im[2], LD, n, o, t2r, t2i);
     __syncthreads();
     // P2 = T2 a (into e), P4 = T2 b (the next beta)
-    T p2r[TM][TM], p2i[TM][TM], p4r[TM][TM], p4i[TM][TM];
-    tzero<T, TM>(p2r, p2i);
-    tzero<T, TM>(p4r, p4i);
-    if (run && t.own)
-      mac2a<T, TM>(s.re[2], s.im[2], s.re[0], s.im[0], s.re[1], s.im[1], LD, n, t.i0, t.j0, p2r, p2i,
-                   p4r, p4i);
+    T p2r[TPW], p2i[TPW], p4r[TPW], p4i[TPW];
+    mzero<TPW>(p2r, p2i);
+    mzero<TPW>(p4r, p4i);
+    if (run)
+      mm2a<NMAX, NT>(s.re[2], s.im[2], s.re[0], s.im[0], s.re[1], s.im[1], LD, ns, o, p2r, p2i,
+                     p4r, p4i);
     __syncthreads();
     bool bad = false;
     double emax = 0.0;
-    if (run && t.own) {
+    if (run) {
 #pragma unroll
-      for (int x = 0; x < TM; ++x)
-#pragma unroll
-        for (int y = 0; y < TM; ++y) {
-          const int i = t.i0 + x, j = t.j0 + y;
-          if (i < n && j < n) {
-            er[x][y] = er[x][y] + p2r[x][y];
-            ei[x][y] = ei[x][y] + p2i[x][y];
-            s.re[0][i * LD + j] = p3r[x][y];
-            s.im[0][i * LD + j] = p3i[x][y];
-            s.re[1][i * LD + j] = p4r[x][y];
-            s.im[1][i * LD + j] = p4i[x][y];
-            bad |= !(finite_d(sr[x][y]) && finite_d(si[x][y]) && finite_d(er[x][y]) &&
-                     finite_d(ei[x][y]) && finite_d(p3r[x][y]) && finite_d(p3i[x][y]) &&
-                     finite_d(p4r[x][y]) && finite_d(p4i[x][y]));
-            const T m3 = p3r[x][y] * p3r[x][y] + p3i[x][y] * p3i[x][y];
-            const T m4 = p4r[x][y] * p4r[x][y] + p4i[x][y] * p4i[x][y];
-            emax = fmax(emax, (double)fmax(m3, m4));
-          }
+      for (int q = 0; q < TPW; ++q) {
+        const int i = o.row(q), j = o.col(q);
+        if (i < n && j < n) {
+          er[q] = er[q] + p2r[q];
+          ei[q] = ei[q] + p2i[q];
+          s.re[0][i * LD + j] = p3r[q];
+          s.im[0][i * LD + j] = p3i[q];
+          s.re[1][i * LD + j] = p4r[q];
+          s.im[1][i * LD + j] = p4i[q];
+          bad |= !(finite_d(sr[q]) && finite_d(si[q]) && finite_d(er[q]) && finite_d(ei[q]) &&
+                   finite_d(p3r[q]) && finite_d(p3i[q]) && finite_d(p4r[q]) && finite_d(p4i[q]));
+          const T m3 = p3r[q] * p3r[q] + p3i[q] * p3i[q];
+          const T m4 = p4r[q] * p4r[q] + p4i[q] * p4i[q];
+          emax = fmax(emax, fmax(m3, m4));
         }
+      }
     }
     double gmax;
     bool gbad;
@@ -263,7 +251,7 @@ __device__ int pl_sr(const SrArgs& a, int inst, bool active, L& s, int tid, PlSh
       if (gbad) {
         status = ST_NONFINITE;
         run = false;
-      } else if ((T)sqrt(gmax) < tol) {
+      } else if (sqrt(gmax) < tol) {
         conv = true;
         run = false;
       }
@@ -276,17 +264,16 @@ __device__ int pl_sr(const SrArgs& a, int inst, bool active, L& s, int tid, PlSh
   }
   // g_s = (z - e_s)^-1
   const bool fin = active && status == ST_OK;
-  if (fin && t.own) {
+  __syncthreads();  // earlier reads of plane 2 are done
+  if (fin) {
 #pragma unroll
-    for (int x = 0; x < TM; ++x)
-#pragma unroll
-      for (int y = 0; y < TM; ++y) {
-        const int i = t.i0 + x, j = t.j0 + y;
-        if (i < n && j < n) {
-          s.re[2][i * LD + j] = (i == j ? zr : T(0)) - sr[x][y];
-          s.im[2][i * LD + j] = (i == j ? zi : T(0)) - si[x][y];
-        }
+    for (int q = 0; q < TPW; ++q) {
+      const int i = o.row(q), j = o.col(q);
+      if (i < n && j < n) {
+        s.re[2][i * LD + j] = (i == j ? zr : T(0)) - sr[q];
+        s.im[2][i * LD + j] = (i == j ? zi : T(0)) - si[q];
       }
+    }
   }
   __syncthreads();
   const int st = gj_inverse_ls<T, NT, NMAX, L::RG>(s.re[2], s.im[2], LD, n, *s.gj, tid, fin);
@@ -359,10 +346,13 @@ __device__ void pl_sigma(const SigmaArgs& a, int inst, bool live, L& s, int tid)
 template <int NMAX, class L>
 __device__ int pl_rgf(const RgfArgs& a, int inst, bool active, int st_in, L& s, int tid) {
   using T = double;
-  constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
+  constexpr int NT = Cfg<NMAX>::NT;
   constexpr int LD = L::LD;
+  using M = MOwn<NMAX, NT>;
+  constexpr int TPW = M::TPW;
   bool live = active && st_in == ST_OK;
   const int n = a.n, nl = a.n_layers;
+  const int ns = (n + 3) / 4;
   const size_t nn = (size_t)n * n;
   const size_t set = active ? (size_t)a.set_of[inst] : 0;
   const double2* ons = reinterpret_cast<const double2*>(a.onsite + set * nl * nn * 2);
@@ -374,26 +364,24 @@ __device__ int pl_rgf(const RgfArgs& a, int inst, bool active, int st_in, L& s, 
   __syncthreads();  // the previous stage's last reads of these matrices are done
   if (live) s.zero(tid);
   __syncthreads();
-  const Tile<TM> t(n, n, tid);
+  const M o(tid);
   double zr = 0.0, zi = 0.0;
   if (active) zsq(a.omega[inst], a.eta_dev[inst], zr, zi);
-  T sr[TM][TM], si[TM][TM];
-  tzero<T, TM>(sr, si);
+  T sr[TPW], si[TPW];
+  mzero<TPW>(sr, si);
   int status = ST_OK;
   for (int layer = 0; layer < nl; ++layer) {
-    if (live && t.own) {
+    if (live) {
       const double2* d = ons + (size_t)layer * nn;
 #pragma unroll
-      for (int x = 0; x < TM; ++x)
-#pragma unroll
-        for (int y = 0; y < TM; ++y) {
-          const int i = t.i0 + x, j = t.j0 + y;
-          if (i < n && j < n) {
-            const double2 dv = d[i * n + j];
-            s.re[0][i * LD + j] = (i == j ? zr : T(0)) - dv.x - sr[x][y];
-            s.im[0][i * LD + j] = (i == j ? zi : T(0)) - dv.y - si[x][y];
-          }
+      for (int q = 0; q < TPW; ++q) {
+        const int i = o.row(q), j = o.col(q);
+        if (i < n && j < n) {
+          const double2 dv = d[i * n + j];
+          s.re[0][i * LD + j] = (i == j ? zr : T(0)) - dv.x - sr[q];
+          s.im[0][i * LD + j] = (i == j ? zi : T(0)) - dv.y - si[q];
         }
+      }
     }
     if (layer == 0 || layer == nl - 1) {
       __syncthreads();
@@ -421,53 +409,43 @@ __device__ int pl_rgf(const RgfArgs& a, int inst, bool active, int st_in, L& s, 
       live = false;
     }
     if (layer == nl - 1) {
-      // G_1N = Q_{N-1} g_N (g_1 itself for one layer)
-      T gr[TM][TM], gi[TM][TM];
-      tzero<T, TM>(gr, gi);
-      if (nl == 1) {
-        if (live && t.own) {
-#pragma unroll
-          for (int x = 0; x < TM; ++x)
-#pragma unroll
-            for (int y = 0; y < TM; ++y) {
-              const int i = t.i0 + x, j = t.j0 + y;
-              if (i < n && j < n) {
-                gr[x][y] = s.re[0][i * LD + j];
-                gi[x][y] = s.im[0][i * LD + j];
-              }
-            }
+      // G_1N = Q_{N-1} g_N (g_1 itself for one layer), through LDS to a coalesced store
+      if (nl > 1) {
+        T gr[TPW], gi[TPW];
+        mzero<TPW>(gr, gi);
+        if (live) {
+          mm<NMAX, NT, OPN, OPN>(s.re[2], s.im[2], s.re[0], s.im[0], LD, ns, o, gr, gi);
+          mstore<NMAX, NT>(s.re[1], s.im[1], LD, n, o, gr, gi);
         }
-      } else if (live && t.own) {
-        mac<T, TM, OPN, OPN>(s.re[2], s.im[2], s.re[0], s.im[0], LD, n, t.i0, t.j0, gr, gi);
+        __syncthreads();
+        if (live) gstore<T, NT>(a.g1n_out + (size_t)inst * nn * 2, n, s.re[1], s.im[1], LD, tid);
+      } else if (live) {
+        gstore<T, NT>(a.g1n_out + (size_t)inst * nn * 2, n, s.re[0], s.im[0], LD, tid);
       }
-      if (live) tstore<T, TM>(s.re[1], s.im[1], LD, n, n, t, gr, gi);
-      __syncthreads();
-      if (live) gstore<T, NT>(a.g1n_out + (size_t)inst * nn * 2, n, s.re[1], s.im[1], LD, tid);
       break;
     }
     if (live) gload<T, NT>(cpl + (size_t)layer * nn * 2, n, n, s.re[1], s.im[1], LD, tid);
     __syncthreads();
-    T yr[TM][TM], yi[TM][TM];
-    tzero<T, TM>(yr, yi);
-    if (live && t.own)
-      mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[1], s.im[1], LD, n, t.i0, t.j0, yr, yi);
+    T yr[TPW], yi[TPW];
+    mzero<TPW>(yr, yi);
+    if (live) mm<NMAX, NT, OPN, OPN>(s.re[0], s.im[0], s.re[1], s.im[1], LD, ns, o, yr, yi);
     __syncthreads();  // every read of g is done: Y replaces it (and starts Q at layer 0)
     if (live) {
-      tstore<T, TM>(s.re[0], s.im[0], LD, n, n, t, yr, yi);
-      if (layer == 0) tstore<T, TM>(s.re[2], s.im[2], LD, n, n, t, yr, yi);
+      mstore<NMAX, NT>(s.re[0], s.im[0], LD, n, o, yr, yi);
+      if (layer == 0) mstore<NMAX, NT>(s.re[2], s.im[2], LD, n, o, yr, yi);
     }
     __syncthreads();
-    tzero<T, TM>(sr, si);
-    if (live && t.own)
-      mac<T, TM, OPH, OPN>(s.re[1], s.im[1], s.re[0], s.im[0], LD, n, t.i0, t.j0, sr, si);
+    mzero<TPW>(sr, si);
     if (layer > 0) {
-      T qr[TM][TM], qi[TM][TM];
-      tzero<T, TM>(qr, qi);
-      if (live && t.own)
-        mac<T, TM, OPN, OPN>(s.re[2], s.im[2], s.re[0], s.im[0], LD, n, t.i0, t.j0, qr, qi);
+      T qr[TPW], qi[TPW];
+      mzero<TPW>(qr, qi);
+      if (live)
+        mm2b<NMAX, NT, OPH, OPN>(s.re[1], s.im[1], s.re[2], s.im[2], s.re[0], s.im[0], LD, ns, o,
+                                 sr, si, qr, qi);
       __syncthreads();
-      if (live) tstore<T, TM>(s.re[2], s.im[2], LD, n, n, t, qr, qi);
+      if (live) mstore<NMAX, NT>(s.re[2], s.im[2], LD, n, o, qr, qi);
     } else {
+      if (live) mm<NMAX, NT, OPH, OPN>(s.re[1], s.im[1], s.re[0], s.im[0], LD, ns, o, sr, si);
       __syncthreads();
     }
   }

@@ -111,6 +111,18 @@ def test_wave_kernel_mode_switch():
     assert lib.negf_set_wave_kernels(prev) == 0
 
 
+def test_matrix_core_product_switch():
+    """negf_set_matrix_core_products takes modes 0 and 1 and returns the previous one;
+    other values are rejected with -1 (host-only, no GPU work)."""
+    lib = _lib.load_symbols_only()
+    prev = lib.negf_set_matrix_core_products(0)
+    assert prev in (0, 1)
+    assert lib.negf_set_matrix_core_products(1) == 0
+    assert lib.negf_set_matrix_core_products(2) == -1
+    assert b"matrix-core" in lib.negf_last_error()
+    assert lib.negf_set_matrix_core_products(prev) == 1
+
+
 def test_kernel_family_table():
     """negf_set_kernel_family stores a per-size family (0 default, 1 workgroup, 2 wave)
     for blocks of 1..32 DOF and negf_kernel_family reports the one in force in mode 1;

@@ -4,7 +4,7 @@ Surface GF, surface DOS and transmission at n = 60, 72, 90, 96 against the
 oracle (the same conditioning-aware bar as every FP64 parity test), explicit
 device-to-lead couplings with two different leads, a batch larger than the
 pool's slot count (every workgroup then walks several instances), a 40-DOF
-problem forced into class 96, which must give the class-48 result, and every
+problem forced into class 96, which must give the class-48 result to rounding, and every
 block of the device Green's function against the oracle's dense inverse (the
 adjoint at these sizes is in test_gpu_adjoint.py).
 """
@@ -103,7 +103,10 @@ def test_batch_larger_than_pool(gpu):
 
 
 def test_class_96_equals_class_48(gpu):
-    """A 40-DOF problem solved in the pooled class and in the LDS class 48."""
+    """A 40-DOF problem solved in the pooled class and in the LDS class 48: the same T
+    to rounding (class 48 runs its products on the matrix cores, class 96 on the vector
+    ALUs, so the two agree to the problem's conditioning, not bit for bit), and both
+    within the oracle bar."""
     from negfpy_amd.core.autotune import reset_block_class
 
     n = 40
@@ -117,7 +120,13 @@ def test_class_96_equals_class_48(gpu):
         t96 = transmission_spectrum(w, dev, lead, lead, eta=1e-6)
     finally:
         reset_block_class(n)
-    np.testing.assert_allclose(t96, t48, rtol=1e-12, atol=1e-15)
+    rel = np.abs(t96 - t48) / np.maximum(np.abs(t48), 1e-3)
+    assert np.median(rel) < 1e-12 and rel.max() < 1e-7, (np.median(rel), rel.max())
+    ld = (lead.d00, lead.d01, None)
+    sel = np.argsort(rel)[-4:]  # the points where the two classes differ most
+    refs = [oracle_t(w[i], ons, cps, ld, ld, eta=1e-6) for i in sel]
+    assert_parity(t48[sel], refs)
+    assert_parity(t96[sel], refs)
 
 
 @pytest.mark.parametrize("n,n_layers", [(60, 3), (96, 2)])
