@@ -92,6 +92,16 @@ struct Occ3 {
   static constexpr int value = (sizeof(T) == 8 && NMAX == 24) ? 4 : Occ<T, NMAX>::value;
 };
 
+// The decimation with matrix-core products keeps its state in fewer registers than
+// the register-tiled form: at 24 DOF it runs at four waves per SIMD, so the five
+// instances per CU that its LDS admits are all resident (a few VGPRs spill around
+// the inversion; measured 3.36 -> 3.11 ms on the 4096-instance ribbon).  The
+// mirrored form carries one more state matrix and stays at three.
+template <int NMAX, bool MIRROR>
+struct OccMM {
+  static constexpr int value = (NMAX == 24 && !MIRROR) ? 4 : Occ<double, NMAX>::value;
+};
+
 template <typename T, int NMAX, int NT, int NMAT>
 struct Lds {
   static constexpr int LD = NMAX + 1;
@@ -510,7 +520,7 @@ __device__ __forceinline__ void sr_write_mm(const double (&er)[MOwn<NMAX, Cfg<NM
 // e~_s) are held in the matrix-core ownership.
 template <int NMAX, bool MIRROR>
 __global__ void __launch_bounds__(Cfg<NMAX>::NT)
-    __attribute__((amdgpu_waves_per_eu(Occ<double, NMAX>::value))) k_sr_mm(SrArgs a) {
+    __attribute__((amdgpu_waves_per_eu(OccMM<NMAX, MIRROR>::value))) k_sr_mm(SrArgs a) {
   using T = double;
   constexpr int NT = Cfg<NMAX>::NT;
   using L = Lds<T, NMAX, NT, 3>;
