@@ -5,9 +5,10 @@
 // device-memory pool (one slot per resident workgroup, reused instance after
 // instance; the working set of the resident slots stays in the L2 / MALL).  The
 // inversion is the same register-resident Gauss-Jordan as the LDS classes (its
-// pivot buffers stay in LDS) and the products are the same owner-tile loops,
-// reading their operands from the slot; only one accumulator tile is live at a
-// time, so the 768-thread workgroups fit the register file without spilling.
+// pivot buffers stay in LDS).  The decimation and the sweep run their products on
+// the matrix cores with the operands streamed from the slot (big_mm); the
+// self-energy, the trace and the eigenmode route keep the owner-tile loops, their
+// operands staged through LDS in K chunks (big_mul).
 // The kernels are persistent: the grid is at most the number of slots and each
 // workgroup walks the instances blockIdx.x, blockIdx.x + gridDim.x, ...
 //
@@ -17,6 +18,7 @@
 #pragma once
 
 #include "cmat.hpp"
+#include "mm.hpp"
 
 namespace negf {
 
@@ -169,6 +171,112 @@ __device__ __forceinline__ void big_mul(const BigMat& A, const BigMat& B, int m,
   }
 }
 
+// ---------------------------------------------------------------------------
+// products on the matrix cores (v_mfma_f64_16x16x4f64; 96 = 6 x 16 output tiles)
+// ---------------------------------------------------------------------------
+//
+// The 36 output tiles of 16 x 16 go three to a wave: wave w (of 12) owns the tiles
+// (R, C) = (w / 2, 3 (w % 2) + q), q < 3, so its A fragment is shared by its three
+// tiles.  Lane l of the wave loads A[16 R + l % 16][4 s + l / 16] and, per tile,
+// B[4 s + l / 16][16 C + l % 16] straight from the slot (L2) at K step s -- no LDS
+// staging and no barrier inside a product -- the fragments of step s + 1 in flight
+// while step s runs on the matrix cores.  Lane l holds element
+// (16 R + 4 v + l / 16, 16 C + l % 16) of tile q in accumulator register v.
+// Operand entries outside [0, m) x [0, k) and [0, k) x [0, p) are read as zero.
+// A complex product is four real chains per tile, the minus of Ai Bi (and the
+// conjugation of an OPH / OPC operand) in the negation bits of the instruction.
+
+typedef double big_v4d __attribute__((ext_vector_type(4)));
+constexpr int BIG_TPW = 3;  // output tiles per wave
+
+struct BigOwn {
+  int R, C0, l;
+  __device__ __forceinline__ explicit BigOwn(int tid) {
+    const int w = tid / WAVE;
+    l = tid & (WAVE - 1);
+    R = w >> 1;
+    C0 = 3 * (w & 1);
+  }
+  __device__ __forceinline__ int row(int v) const { return 16 * R + 4 * v + (l >> 4); }
+  __device__ __forceinline__ int col(int q) const { return 16 * (C0 + q) + (l & 15); }
+};
+
+template <int NEG>
+__device__ __forceinline__ big_v4d bmf(double a, double b, big_v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, NEG);
+}
+
+template <int OP>
+__device__ __forceinline__ void big_frag(const BigMat& X, int r, int c, bool ok, double& xr,
+                                         double& xi) {
+  xr = 0.0;
+  xi = 0.0;
+  if (ok) {
+    const unsigned o = (OP == OPN || OP == OPC) ? big_off(r, c) : big_off(c, r);
+    xr = X.re[o];
+    xi = X.im[o];
+  }
+}
+
+template <int OPA, int OPB>
+__device__ __forceinline__ void big_mm(const BigMat& A, const BigMat& B, int m, int k, int p,
+                                       const BigOwn& o, big_v4d (&cr)[BIG_TPW],
+                                       big_v4d (&ci)[BIG_TPW]) {
+  constexpr int sa = imsign<OPA>(), sb = imsign<OPB>();
+  const big_v4d z = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < BIG_TPW; ++q) {
+    cr[q] = z;
+    ci[q] = z;
+  }
+  const int ar = 16 * o.R + (o.l & 15);
+  const int kl = o.l >> 4;
+  const int ns = (k + 3) / 4;
+  double xr, xi, yr[BIG_TPW], yi[BIG_TPW];
+  big_frag<OPA>(A, ar, kl, ar < m && kl < k, xr, xi);
+#pragma unroll
+  for (int q = 0; q < BIG_TPW; ++q) big_frag<OPB>(B, kl, o.col(q), kl < k && o.col(q) < p, yr[q], yi[q]);
+  for (int s = 0; s < ns; ++s) {
+    // the next step's fragments, in flight while this step's chains run
+    const int kn = 4 * (s + 1) + kl;
+    double nxr, nxi, nyr[BIG_TPW], nyi[BIG_TPW];
+    big_frag<OPA>(A, ar, kn, s + 1 < ns && ar < m && kn < k, nxr, nxi);
+#pragma unroll
+    for (int q = 0; q < BIG_TPW; ++q)
+      big_frag<OPB>(B, kn, o.col(q), s + 1 < ns && kn < k && o.col(q) < p, nyr[q], nyi[q]);
+#pragma unroll
+    for (int q = 0; q < BIG_TPW; ++q) {
+      cr[q] = bmf<0>(xr, yr[q], cr[q]);
+      ci[q] = bmf<(sb < 0) ? 1 : 0>(xr, yi[q], ci[q]);
+      cr[q] = bmf<(sa * sb > 0) ? 1 : 0>(xi, yi[q], cr[q]);
+      ci[q] = bmf<(sa < 0) ? 1 : 0>(xi, yr[q], ci[q]);
+    }
+    xr = nxr;
+    xi = nxi;
+#pragma unroll
+    for (int q = 0; q < BIG_TPW; ++q) {
+      yr[q] = nyr[q];
+      yi[q] = nyi[q];
+    }
+  }
+}
+
+// owned elements inside [0, m) x [0, p) into a slot matrix
+__device__ __forceinline__ void big_mstore(const BigMat& C, int m, int p, const BigOwn& o,
+                                           const big_v4d (&cr)[BIG_TPW],
+                                           const big_v4d (&ci)[BIG_TPW]) {
+#pragma unroll
+  for (int q = 0; q < BIG_TPW; ++q)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = o.row(v), j = o.col(q);
+      if (i < m && j < p) {
+        C.re[big_off(i, j)] = cr[q][v];
+        C.im[big_off(i, j)] = ci[q][v];
+      }
+    }
+}
+
 __device__ __forceinline__ void big_tstore(const BigMat& C, int m, int p, const BigTile& t,
                                            const double (&cr)[BIG_TM][BIG_TM],
                                            const double (&ci)[BIG_TM][BIG_TM]) {
@@ -210,7 +318,7 @@ __device__ __forceinline__ void big_sr_one(const SrArgs& a, int inst, double* sl
   }
   double zr, zi;
   zsq(a.omega[inst], a.eta[inst], zr, zi);
-  const BigTile t(n, n, tid);
+  const BigOwn o(tid);
   __syncthreads();
   int status = ST_OK;
   int it = 0;
@@ -233,72 +341,64 @@ __device__ __forceinline__ void big_sr_one(const SrArgs& a, int inst, double* sl
       status = st;
       break;
     }
-    double cr[BIG_TM][BIG_TM], ci[BIG_TM][BIG_TM];
-    big_mul<OPN, OPN>(A, G, n, n, n, t, cr, ci, sh, tid);
-    big_store(T1, n, t, cr, ci);
-    big_mul<OPN, OPN>(B, G, n, n, n, t, cr, ci, sh, tid);
-    big_store(T2, n, t, cr, ci);
+    big_v4d cr[BIG_TPW], ci[BIG_TPW];
+    big_mm<OPN, OPN>(A, G, n, n, n, o, cr, ci);
+    big_mstore(T1, n, n, o, cr, ci);
+    big_mm<OPN, OPN>(B, G, n, n, n, o, cr, ci);
+    big_mstore(T2, n, n, o, cr, ci);
     __syncthreads();
     bool bad = false;
     double emax = 0.0;
     // P1 = T1 b into e_s and e; P3 = T1 a is the next alpha
-    big_mul<OPN, OPN>(T1, B, n, n, n, t, cr, ci, sh, tid);
-    if (t.own) {
+    big_mm<OPN, OPN>(T1, B, n, n, n, o, cr, ci);
 #pragma unroll
-      for (int x = 0; x < BIG_TM; ++x)
+    for (int q = 0; q < BIG_TPW; ++q)
 #pragma unroll
-        for (int y = 0; y < BIG_TM; ++y) {
-          const int i = t.i0 + x, j = t.j0 + y;
-          if (i < n && j < n) {
-            const unsigned o = big_off(i, j);
-            ES.re[o] = ES.re[o] + cr[x][y];
-            ES.im[o] = ES.im[o] + ci[x][y];
-            E.re[o] = E.re[o] + cr[x][y];
-            E.im[o] = E.im[o] + ci[x][y];
-            bad |= !(isfinite(ES.re[o]) && isfinite(ES.im[o]));
-          }
+      for (int v = 0; v < 4; ++v) {
+        const int i = o.row(v), j = o.col(q);
+        if (i < n && j < n) {
+          const unsigned off = big_off(i, j);
+          ES.re[off] = ES.re[off] + cr[q][v];
+          ES.im[off] = ES.im[off] + ci[q][v];
+          E.re[off] = E.re[off] + cr[q][v];
+          E.im[off] = E.im[off] + ci[q][v];
+          bad |= !(isfinite(ES.re[off]) && isfinite(ES.im[off]));
         }
-    }
-    big_mul<OPN, OPN>(T1, A, n, n, n, t, cr, ci, sh, tid);
-    big_store(An, n, t, cr, ci);
-    if (t.own) {
+      }
+    big_mm<OPN, OPN>(T1, A, n, n, n, o, cr, ci);
+    big_mstore(An, n, n, o, cr, ci);
 #pragma unroll
-      for (int x = 0; x < BIG_TM; ++x)
+    for (int q = 0; q < BIG_TPW; ++q)
 #pragma unroll
-        for (int y = 0; y < BIG_TM; ++y)
-          if (t.i0 + x < n && t.j0 + y < n) {
-            bad |= !(isfinite(cr[x][y]) && isfinite(ci[x][y]));
-            emax = fmax(emax, cr[x][y] * cr[x][y] + ci[x][y] * ci[x][y]);
-          }
-    }
+      for (int v = 0; v < 4; ++v)
+        if (o.row(v) < n && o.col(q) < n) {
+          bad |= !(isfinite(cr[q][v]) && isfinite(ci[q][v]));
+          emax = fmax(emax, cr[q][v] * cr[q][v] + ci[q][v] * ci[q][v]);
+        }
     // P2 = T2 a into e; P4 = T2 b is the next beta
-    big_mul<OPN, OPN>(T2, A, n, n, n, t, cr, ci, sh, tid);
-    if (t.own) {
+    big_mm<OPN, OPN>(T2, A, n, n, n, o, cr, ci);
 #pragma unroll
-      for (int x = 0; x < BIG_TM; ++x)
+    for (int q = 0; q < BIG_TPW; ++q)
 #pragma unroll
-        for (int y = 0; y < BIG_TM; ++y) {
-          const int i = t.i0 + x, j = t.j0 + y;
-          if (i < n && j < n) {
-            const unsigned o = big_off(i, j);
-            E.re[o] = E.re[o] + cr[x][y];
-            E.im[o] = E.im[o] + ci[x][y];
-            bad |= !(isfinite(E.re[o]) && isfinite(E.im[o]));
-          }
+      for (int v = 0; v < 4; ++v) {
+        const int i = o.row(v), j = o.col(q);
+        if (i < n && j < n) {
+          const unsigned off = big_off(i, j);
+          E.re[off] = E.re[off] + cr[q][v];
+          E.im[off] = E.im[off] + ci[q][v];
+          bad |= !(isfinite(E.re[off]) && isfinite(E.im[off]));
         }
-    }
-    big_mul<OPN, OPN>(T2, B, n, n, n, t, cr, ci, sh, tid);
-    big_store(Bn, n, t, cr, ci);
-    if (t.own) {
+      }
+    big_mm<OPN, OPN>(T2, B, n, n, n, o, cr, ci);
+    big_mstore(Bn, n, n, o, cr, ci);
 #pragma unroll
-      for (int x = 0; x < BIG_TM; ++x)
+    for (int q = 0; q < BIG_TPW; ++q)
 #pragma unroll
-        for (int y = 0; y < BIG_TM; ++y)
-          if (t.i0 + x < n && t.j0 + y < n) {
-            bad |= !(isfinite(cr[x][y]) && isfinite(ci[x][y]));
-            emax = fmax(emax, cr[x][y] * cr[x][y] + ci[x][y] * ci[x][y]);
-          }
-    }
+      for (int v = 0; v < 4; ++v)
+        if (o.row(v) < n && o.col(q) < n) {
+          bad |= !(isfinite(cr[q][v]) && isfinite(ci[q][v]));
+          emax = fmax(emax, cr[q][v] * cr[q][v] + ci[q][v] * ci[q][v]);
+        }
     double gmax;
     bool gbad;
     block_max_any<BIG_NT>(emax, bad, sh.sred, sh.sflag, tid, gmax, gbad);
@@ -413,7 +513,7 @@ __device__ __forceinline__ void big_rgf_one(const RgfArgs& a, int inst, double* 
   const double* cpl = a.coupling + set * (nl > 1 ? nl - 1 : 0) * nn * 2;
   const double2* sgl = reinterpret_cast<const double2*>(a.sigma_l + (size_t)inst * nn * 2);
   const double2* sgr = reinterpret_cast<const double2*>(a.sigma_r + (size_t)inst * nn * 2);
-  const BigTile t(n, n, tid);
+  const BigOwn o(tid);
   double zr, zi;
   zsq(a.omega[inst], a.eta_dev[inst], zr, zi);
   int status = ST_OK;
@@ -451,14 +551,14 @@ __device__ __forceinline__ void big_rgf_one(const RgfArgs& a, int inst, double* 
       status = 10 + st;
       break;
     }
-    double cr[BIG_TM][BIG_TM], ci[BIG_TM][BIG_TM];
+    big_v4d cr[BIG_TPW], ci[BIG_TPW];
     if (layer == nl - 1) {
       // G_1N = Q_{N-1} g_N (g_1 itself for one layer)
       if (nl == 1) {
         big_gstore(a.g1n_out + (size_t)inst * nn * 2, n, Gm, tid);
       } else {
-        big_mul<OPN, OPN>(Q, Gm, n, n, n, t, cr, ci, sh, tid);
-        big_store(H, n, t, cr, ci);
+        big_mm<OPN, OPN>(Q, Gm, n, n, n, o, cr, ci);
+        big_mstore(H, n, n, o, cr, ci);
         __syncthreads();
         big_gstore(a.g1n_out + (size_t)inst * nn * 2, n, H, tid);
       }
@@ -466,17 +566,17 @@ __device__ __forceinline__ void big_rgf_one(const RgfArgs& a, int inst, double* 
     }
     big_gload(cpl + (size_t)layer * nn * 2, n, n, H, tid);
     __syncthreads();
-    big_mul<OPN, OPN>(Gm, H, n, n, n, t, cr, ci, sh, tid);
+    big_mm<OPN, OPN>(Gm, H, n, n, n, o, cr, ci);
     __syncthreads();  // every read of g is done: Y replaces it (and starts Q at layer 0)
-    big_store(Gm, n, t, cr, ci);
-    if (layer == 0) big_store(Q, n, t, cr, ci);
+    big_mstore(Gm, n, n, o, cr, ci);
+    if (layer == 0) big_mstore(Q, n, n, o, cr, ci);
     __syncthreads();
-    big_mul<OPH, OPN>(H, Gm, n, n, n, t, cr, ci, sh, tid);
-    big_store(S, n, t, cr, ci);
+    big_mm<OPH, OPN>(H, Gm, n, n, n, o, cr, ci);
+    big_mstore(S, n, n, o, cr, ci);
     if (layer > 0) {
-      big_mul<OPN, OPN>(Q, Gm, n, n, n, t, cr, ci, sh, tid);
+      big_mm<OPN, OPN>(Q, Gm, n, n, n, o, cr, ci);
       __syncthreads();
-      big_store(Q, n, t, cr, ci);
+      big_mstore(Q, n, n, o, cr, ci);
     }
     __syncthreads();
   }

@@ -405,6 +405,11 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
   constexpr int MAXR = NMAX / RG;
   static_assert(NMAX * RG <= NT, "not enough threads for the inversion layout");
   static_assert(MAXR <= 32, "row mask is 32 bits");
+  // The short step (the column and pivot read with one LDS wait, the pivot-row entry
+  // broadcast by DPP, the arg-max on 64-bit keys) for the workgroup classes, whose
+  // lanes hold at most four rows; with more rows per lane (the pooled class) its extra
+  // live registers spill, and the step keeps the lane shuffle and the compare arg-max.
+  constexpr bool FAST = SF && MAXR <= 4 && (RG == 4 || RG == 8 || RG == 16) && NMAX <= 64;
   const bool on = tid < NMAX * RG;
   const int j = on ? tid / RG : 0;
   const int g = tid % RG;
@@ -428,7 +433,7 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
       }
       T best = T(-1);
       int bidx = n;
-      if constexpr (SF && RG <= 16 && NMAX <= 64) {
+      if constexpr (FAST) {
         unsigned long long key = 0;
 #pragma unroll
         for (int m = 0; m < MAXR; ++m) {
@@ -493,7 +498,7 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
       ci[m] = b.coli[par][g + RG * m];
     }
     Pivot<T> pk = b.piv[par];
-    if constexpr (SF) {
+    if constexpr (FAST) {
       // one LDS round trip: the column and the pivot are loaded before the branch
 #pragma unroll
       for (int m = 0; m < MAXR; ++m) asm volatile("" : "+v"(cr[m]), "+v"(ci[m]));
@@ -524,7 +529,7 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
         }
       // pivot row entry of this lane's column, from the group lane holding row p
       T pr, pi;
-      if constexpr (sizeof(T) == 8 && (RG == 4 || RG == 8 || RG == 16)) {
+      if constexpr (FAST && sizeof(T) == 8) {
         pr = vr;
         pi = vi;
         group_bcast<RG>(pr, pi, gp, tid & (WAVE - 1));
