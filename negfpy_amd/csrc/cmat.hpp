@@ -247,75 +247,6 @@ __device__ __forceinline__ void group_argmax(T& best, int& idx) {
   if constexpr (RG >= 16) argmax_step<T, 0x140>(best, idx); // row_mirror: the other half-row
 }
 
-// Arg-max over aligned groups of RG lanes on 64-bit keys: the bits of a non-negative
-// magnitude with the low 6 bits replaced by 63 - row, so that the largest key is the
-// largest magnitude (compared to 2^-46 relative) and, among those, the first row.
-// A NaN magnitude takes the key of +inf.  Every lane of a group ends with the key.
-__device__ __forceinline__ unsigned long long pivot_key(double mg, int row) {
-  const unsigned long long b =
-      (mg != mg) ? 0x7FF0000000000000ull : (unsigned long long)__double_as_longlong(mg);
-  return (b & ~63ull) | (unsigned long long)(63 - row);
-}
-constexpr unsigned long long PIVOT_KEY_INF = 0x7FF0000000000000ull;
-
-template <int CTRL>
-__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
-  const int lo = dpp_i<CTRL>((int)(v & 0xffffffffull));
-  const int hi = dpp_i<CTRL>((int)(v >> 32));
-  return ((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo;
-}
-template <int CTRL>
-__device__ __forceinline__ void keymax_step(unsigned long long& k) {
-  const unsigned long long o = dpp_u64<CTRL>(k);
-  k = o > k ? o : k;
-}
-template <int RG>
-__device__ __forceinline__ void group_keymax(unsigned long long& k) {
-  if constexpr (RG >= 2) keymax_step<0xB1>(k);
-  if constexpr (RG >= 4) keymax_step<0x4E>(k);
-  if constexpr (RG >= 8) keymax_step<0x141>(k);
-  if constexpr (RG >= 16) keymax_step<0x140>(k);
-}
-
-// Broadcast lane gp (wave-uniform, < RG) of every aligned group of RG lanes to its
-// group, for the four dwords of a complex double, by DPP: a quad broadcast (one
-// scalar branch on gp & 3 picks its lane), then the half-row and row mirrors bring in
-// the other quad and the other half.  Must run with every lane of the group active.
-template <int Q>
-__device__ __forceinline__ void quad_bcast4(int (&v)[4]) {
-#pragma unroll
-  for (int d = 0; d < 4; ++d) v[d] = dpp_i<Q * 0x55>(v[d]);
-}
-template <int RG>
-__device__ __forceinline__ void group_bcast(double& vr, double& vi, int gp, int lane) {
-  const long long ur = __double_as_longlong(vr), ui = __double_as_longlong(vi);
-  int v[4] = {(int)(ur & 0xffffffffLL), (int)(ur >> 32), (int)(ui & 0xffffffffLL), (int)(ui >> 32)};
-  switch (gp & 3) {
-    case 0: quad_bcast4<0>(v); break;
-    case 1: quad_bcast4<1>(v); break;
-    case 2: quad_bcast4<2>(v); break;
-    default: quad_bcast4<3>(v); break;
-  }
-  if constexpr (RG >= 8) {
-    const bool keep = (lane & 4) == (gp & 4);  // this quad holds lane gp's half-row copy
-#pragma unroll
-    for (int d = 0; d < 4; ++d) {
-      const int u = dpp_i<0x141>(v[d]);
-      v[d] = keep ? v[d] : u;
-    }
-  }
-  if constexpr (RG >= 16) {
-    const bool keep = (lane & 8) == (gp & 8);
-#pragma unroll
-    for (int d = 0; d < 4; ++d) {
-      const int u = dpp_i<0x140>(v[d]);
-      v[d] = keep ? v[d] : u;
-    }
-  }
-  vr = __longlong_as_double(((long long)v[1] << 32) | (unsigned int)v[0]);
-  vi = __longlong_as_double(((long long)v[3] << 32) | (unsigned int)v[2]);
-}
-
 // Pivot of one elimination step, double-buffered by step parity in LDS.
 template <typename T>
 struct Pivot {
@@ -397,6 +328,176 @@ __device__ __forceinline__ void crecip1(T ar, T ai, T& rr, T& ri) {
 // of the same n side by side, so every sub-block must pass the same n + 2
 // barriers: a failed pivot then keeps stepping through the remaining barriers
 // instead of returning early (gj_inverse_ls).
+// The same elimination with a shorter pivot step, for the workgroup classes (at most
+// four rows per lane, column groups of 4, 8 or 16 lanes).  A step's cost is the
+// instruction stream of the wave that publishes the next column, so that stream is
+// cut to the essentials:
+//  - the pivot is the largest |re| + |im| compared to 2^-14 relative (the magnitude's
+//    high word with its low 6 bits replaced by 63 - row, maximised with v_max_u32 over
+//    the group by DPP; the first row among magnitudes equal to that resolution), and
+//    its failure flag travels in the pivot-row word (-flag);
+//  - the column and the pivot are read with one LDS wait;
+//  - the permutation stays in registers: the owner lanes of column k keep its pivot
+//    row, and the lane holding row p keeps the step at which it was the pivot, so the
+//    final scatter needs no LDS tables.
+// Partial pivoting within that tolerance is as stable as the exact maximum; results
+// differ from gj_core's only where two candidates agree to 2^-14.
+template <typename T, int NMAX>
+struct GjLeanBuf {
+  T colr[2][NMAX], coli[2][NMAX];  // column k of the current state, by step parity
+  T pir[2], pii[2];                // reciprocal of the pivot
+  int pp[2];                       // pivot row, or -flag when the pivot failed
+};
+
+template <typename T, int NT, int NMAX, int RG, bool LS, typename PT>
+__device__ __forceinline__ int gj_lean(PT __restrict__ re, PT __restrict__ im, int ld, int n,
+                                       GjLeanBuf<T, NMAX>& b, int tid) {
+  constexpr int MAXR = NMAX / RG;
+  static_assert(NMAX * RG == NT || NMAX * RG <= NT, "not enough threads for the inversion layout");
+  static_assert(NMAX <= 64, "row index in 6 bits");
+  const bool on = tid < NMAX * RG;
+  const int jc = on ? tid / RG : 0;
+  const int j = on ? jc : 0x7fff;
+  const int g = tid % RG;
+  const int lane = tid & (WAVE - 1);
+  const int gbase = lane & ~(RG - 1);
+  const int wave = tid / WAVE;
+  T xr[MAXR], xi[MAXR];
+  int qrow[MAXR];     // step at which row g + RG m was the pivot
+  int pcol = 0;       // pivot row of the step of this lane's column
+  unsigned used = 0;  // bit m: row g + RG m is a padding row or has been a pivot
+#pragma unroll
+  for (int m = 0; m < MAXR; ++m) {
+    const int i = g + RG * m;
+    if (i >= n) used |= 1u << m;
+    xr[m] = re[i * ld + jc];
+    xi[m] = im[i * ld + jc];
+    qrow[m] = 0;
+  }
+  // the owners of column `col` publish it and choose its pivot
+  auto publish = [&](int col, int par) {
+    if (wave != (col * RG) / WAVE) return;  // wave-uniform
+    if (j == col) {
+#pragma unroll
+      for (int m = 0; m < MAXR; ++m) {
+        b.colr[par][g + RG * m] = xr[m];
+        b.coli[par][g + RG * m] = xi[m];
+      }
+      unsigned key = 0;
+#pragma unroll
+      for (int m = 0; m < MAXR; ++m) {
+        const double mg = (double)(fabs(xr[m]) + fabs(xi[m]));
+        const unsigned kb = ((unsigned)((unsigned long long)__double_as_longlong(mg) >> 32) & ~63u) |
+                            (unsigned)(63 - (g + RG * m));
+        key = ((used >> m) & 1u) ? key : max(key, kb);
+      }
+      key = max(key, (unsigned)dpp_i<0xB1>((int)key));
+      if constexpr (RG >= 4) key = max(key, (unsigned)dpp_i<0x4E>((int)key));
+      if constexpr (RG >= 8) key = max(key, (unsigned)dpp_i<0x141>((int)key));
+      if constexpr (RG >= 16) key = max(key, (unsigned)dpp_i<0x140>((int)key));
+      const int bidx = 63 - (int)(key & 63u);
+      // NaN and infinity sort above every finite magnitude; zero (or subnormal) is singular
+      const int flag = key >= 0x7FF00000u ? ST_NONFINITE : (key < 64u ? ST_SINGULAR : ST_OK);
+      if (g == (bidx & (RG - 1))) {  // the lane holding the pivot row
+        const int mw = bidx / RG;
+        T pr = xr[0], pi = xi[0];
+#pragma unroll
+        for (int m = 1; m < MAXR; ++m) {
+          pr = (m == mw) ? xr[m] : pr;
+          pi = (m == mw) ? xi[m] : pi;
+        }
+        T ir, ii;
+        crecip1(pr, pi, ir, ii);
+        b.pir[par] = ir;
+        b.pii[par] = ii;
+        b.pp[par] = flag == ST_OK ? bidx : -flag;
+      }
+      // the published column becomes -c s in its step: it starts from zero
+#pragma unroll
+      for (int m = 0; m < MAXR; ++m) {
+        xr[m] = T(0);
+        xi[m] = T(0);
+      }
+    }
+  };
+  publish(0, 0);
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    const int par = k & 1;
+    T cr[MAXR], ci[MAXR];
+#pragma unroll
+    for (int m = 0; m < MAXR; ++m) {
+      cr[m] = b.colr[par][g + RG * m];
+      ci[m] = b.coli[par][g + RG * m];
+    }
+    T pir = b.pir[par], pii = b.pii[par];
+    int pp = b.pp[par];
+    // one LDS round trip: every load is issued before the branch on the pivot
+#pragma unroll
+    for (int m = 0; m < MAXR; ++m) asm volatile("" : "+v"(cr[m]), "+v"(ci[m]));
+    asm volatile("" : "+v"(pir), "+v"(pii), "+v"(pp));
+    const int p = __builtin_amdgcn_readfirstlane(pp);
+    if (p < 0) {
+      if constexpr (LS) {
+        for (int r = k; r <= n; ++r) __syncthreads();
+      } else {
+        __syncthreads();
+      }
+      return -p;
+    }
+    const int mp = p / RG, gp = p - (p / RG) * RG;
+    T vr = xr[0], vi = xi[0];
+#pragma unroll
+    for (int m = 1; m < MAXR; ++m)
+      if (m == mp) {
+        uniform_branch();
+        vr = xr[m];
+        vi = xi[m];
+      }
+    // pivot row entry of this lane's column, from the group lane holding row p
+    const T pr = lane_shfl(vr, gbase + gp);
+    const T pi = lane_shfl(vi, gbase + gp);
+    const bool jk = (j == k);
+    const T sr = jk ? pir : pr * pir - pi * pii;
+    const T si = jk ? pii : pr * pii + pi * pir;
+#pragma unroll
+    for (int m = 0; m < MAXR; ++m) {
+      T yr = fma(-cr[m], sr, xr[m]);
+      yr = fma(ci[m], si, yr);
+      T yi = fma(-cr[m], si, xi[m]);
+      yi = fma(-ci[m], sr, yi);
+      xr[m] = yr;
+      xi[m] = yi;
+    }
+    // the pivot row becomes s (the update above left rounding residue there)
+#pragma unroll
+    for (int m = 0; m < MAXR; ++m)
+      if (m == mp) {
+        uniform_branch();
+        const bool own = (g == gp);
+        xr[m] = own ? sr : xr[m];
+        xi[m] = own ? si : xi[m];
+        used |= own ? (1u << m) : 0u;
+        qrow[m] = own ? k : qrow[m];
+      }
+    pcol = jk ? p : pcol;
+    if (k + 1 < n) publish(k + 1, par ^ 1);
+    __syncthreads();
+  }
+  if (on && jc < n) {
+#pragma unroll
+    for (int m = 0; m < MAXR; ++m) {
+      const int i = g + RG * m;
+      if (i < n) {
+        re[qrow[m] * ld + pcol] = xr[m];
+        im[qrow[m] * ld + pcol] = xi[m];
+      }
+    }
+  }
+  __syncthreads();
+  return ST_OK;
+}
+
 template <typename T, int NT, int NMAX, int RG, bool LS, typename PT, bool SF = true>
 __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, int ld, int n,
                                        GjBuf<T, NMAX>& b, int tid) {
@@ -405,11 +506,11 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
   constexpr int MAXR = NMAX / RG;
   static_assert(NMAX * RG <= NT, "not enough threads for the inversion layout");
   static_assert(MAXR <= 32, "row mask is 32 bits");
-  // The short step (the column and pivot read with one LDS wait, the pivot-row entry
-  // broadcast by DPP, the arg-max on 64-bit keys) for the workgroup classes, whose
-  // lanes hold at most four rows; with more rows per lane (the pooled class) its extra
-  // live registers spill, and the step keeps the lane shuffle and the compare arg-max.
-  constexpr bool FAST = SF && MAXR <= 4 && (RG == 4 || RG == 8 || RG == 16) && NMAX <= 64;
+  if constexpr (SF && MAXR <= 4 && (RG == 4 || RG == 8 || RG == 16) && NMAX <= 64 &&
+                sizeof(GjLeanBuf<T, NMAX>) <= sizeof(GjBuf<T, NMAX>)) {
+    return gj_lean<T, NT, NMAX, RG, LS>(re, im, ld, n,
+                                       *reinterpret_cast<GjLeanBuf<T, NMAX>*>(&b), tid);
+  }
   const bool on = tid < NMAX * RG;
   const int j = on ? tid / RG : 0;
   const int g = tid % RG;
@@ -433,31 +534,18 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
       }
       T best = T(-1);
       int bidx = n;
-      if constexpr (FAST) {
-        unsigned long long key = 0;
+      bool bad = false;
 #pragma unroll
-        for (int m = 0; m < MAXR; ++m) {
-          const bool live = !((used >> m) & 1u);
-          const unsigned long long kb = pivot_key((double)(fabs(xr[m]) + fabs(xi[m])), g + RG * m);
-          key = (live && kb > key) ? kb : key;
-        }
-        group_keymax<RG>(key);
-        bidx = 63 - (int)(key & 63ull);
-        best = key >= PIVOT_KEY_INF ? T(INFINITY) : (key < 64ull ? T(0) : T(1));
-      } else {
-        bool bad = false;
-#pragma unroll
-        for (int m = 0; m < MAXR; ++m) {
-          const T mg = fabs(xr[m]) + fabs(xi[m]);
-          const bool live = !((used >> m) & 1u);
-          bad = bad || (live && mg != mg);
-          const bool better = live && mg > best;
-          best = better ? mg : best;
-          bidx = better ? g + RG * m : bidx;
-        }
-        if (bad) best = T(INFINITY);  // a NaN anywhere in the column wins and marks it
-        group_argmax<T, RG>(best, bidx);
+      for (int m = 0; m < MAXR; ++m) {
+        const T mg = fabs(xr[m]) + fabs(xi[m]);
+        const bool live = !((used >> m) & 1u);
+        bad = bad || (live && mg != mg);
+        const bool better = live && mg > best;
+        best = better ? mg : best;
+        bidx = better ? g + RG * m : bidx;
       }
+      if (bad) best = T(INFINITY);  // a NaN anywhere in the column wins and marks it
+      group_argmax<T, RG>(best, bidx);
       if (g == 0 || (bidx < n && g == bidx % RG)) {
         const int flag = isinf(best) ? ST_NONFINITE : (best > T(0) ? ST_OK : ST_SINGULAR);
         if (g == 0 && flag != ST_OK) {
@@ -497,13 +585,7 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
       cr[m] = b.colr[par][g + RG * m];
       ci[m] = b.coli[par][g + RG * m];
     }
-    Pivot<T> pk = b.piv[par];
-    if constexpr (FAST) {
-      // one LDS round trip: the column and the pivot are loaded before the branch
-#pragma unroll
-      for (int m = 0; m < MAXR; ++m) asm volatile("" : "+v"(cr[m]), "+v"(ci[m]));
-      asm volatile("" : "+v"(pk.ir), "+v"(pk.ii), "+v"(pk.p), "+v"(pk.flag));
-    }
+    const Pivot<T> pk = b.piv[par];
     const int flag = __builtin_amdgcn_readfirstlane(pk.flag);
     if (flag != ST_OK) {
       if constexpr (LS) {
@@ -528,15 +610,8 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
           vi = xi[m];
         }
       // pivot row entry of this lane's column, from the group lane holding row p
-      T pr, pi;
-      if constexpr (FAST && sizeof(T) == 8) {
-        pr = vr;
-        pi = vi;
-        group_bcast<RG>(pr, pi, gp, tid & (WAVE - 1));
-      } else {
-        pr = lane_shfl(vr, gbase + gp);
-        pi = lane_shfl(vi, gbase + gp);
-      }
+      const T pr = lane_shfl(vr, gbase + gp);
+      const T pi = lane_shfl(vi, gbase + gp);
       const bool jk = (j == k);
       const T sr = jk ? pk.ir : pr * pk.ir - pi * pk.ii;
       const T si = jk ? pk.ii : pr * pk.ii + pi * pk.ir;
