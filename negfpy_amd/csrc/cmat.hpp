@@ -391,10 +391,14 @@ __device__ __forceinline__ int gj_lean(PT __restrict__ re, PT __restrict__ im, i
                             (unsigned)(63 - (g + RG * m));
         key = ((used >> m) & 1u) ? key : max(key, kb);
       }
-      key = max(key, (unsigned)dpp_i<0xB1>((int)key));
-      if constexpr (RG >= 4) key = max(key, (unsigned)dpp_i<0x4E>((int)key));
-      if constexpr (RG >= 8) key = max(key, (unsigned)dpp_i<0x141>((int)key));
-      if constexpr (RG >= 16) key = max(key, (unsigned)dpp_i<0x140>((int)key));
+      // (mov_dpp with bound_ctrl lets the compiler fold each move into v_max_u32_dpp)
+      key = max(key, (unsigned)__builtin_amdgcn_mov_dpp((int)key, 0xB1, 0xF, 0xF, true));
+      if constexpr (RG >= 4)
+        key = max(key, (unsigned)__builtin_amdgcn_mov_dpp((int)key, 0x4E, 0xF, 0xF, true));
+      if constexpr (RG >= 8)
+        key = max(key, (unsigned)__builtin_amdgcn_mov_dpp((int)key, 0x141, 0xF, 0xF, true));
+      if constexpr (RG >= 16)
+        key = max(key, (unsigned)__builtin_amdgcn_mov_dpp((int)key, 0x140, 0xF, 0xF, true));
       const int bidx = 63 - (int)(key & 63u);
       // NaN and infinity sort above every finite magnitude; zero (or subnormal) is singular
       const int flag = key >= 0x7FF00000u ? ST_NONFINITE : (key < 64u ? ST_SINGULAR : ST_OK);

@@ -1242,6 +1242,17 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
         s.im[0][i * LD + j] = vi;
       }
     }
+    // this layer's coupling block: its loads stay in flight through the inversion
+    constexpr int HPT = (NMAX * NMAX + NT - 1) / NT;
+    double2 hv[HPT];
+    if (layer < nl - 1) {
+      const double2* h = reinterpret_cast<const double2*>(cpl) + (size_t)layer * nn;
+#pragma unroll
+      for (int r = 0; r < HPT; ++r) {
+        const int e = tid + r * NT;
+        hv[r] = e < (int)nn ? h[e] : make_double2(0.0, 0.0);
+      }
+    }
     __syncthreads();
     const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[0], s.im[0], LD, n, *s.gj, tid);
     if (st != ST_OK) {
@@ -1262,7 +1273,15 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
       }
       break;
     }
-    gload<T, NT>(cpl + (size_t)layer * nn * 2, n, n, s.re[1], s.im[1], LD, tid);
+#pragma unroll
+    for (int r = 0; r < HPT; ++r) {
+      const int e = tid + r * NT;
+      if (e < (int)nn) {
+        const int i = e / n, j = e - (e / n) * n;
+        s.re[1][i * LD + j] = hv[r].x;
+        s.im[1][i * LD + j] = hv[r].y;
+      }
+    }
     __syncthreads();
     T yr[TPW], yi[TPW];
     mzero<TPW>(yr, yi);
