@@ -92,13 +92,12 @@ int negf_block_class(int precision, int n);
  * instance on one 64-lane wavefront with its products on the matrix cores
  * (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32) and a register-resident wave
  * Gauss-Jordan; the other family gives an instance a workgroup (the block-size
- * classes above).  mode 1 (default): the wave family where it measured faster on
- * MI355X (FP64 blocks of 9-12 DOF, FP32 decimations of 9-12 and 17-24, FP32 sweeps of
- * 9-24); mode 2: the wave family
- * for every block up to 32 DOF; mode 0: never.  Both give the same results to
- * rounding.  Returns the previous mode (-1 and NEGF_ERR_ARG text for a bad mode);
- * process-wide.  No counterpart in the reference (a launch-configuration switch, like
- * negf_set_block_class).
+ * classes above).  mode 1 (default): the wave family only for the sizes a per-size
+ * choice (negf_set_kernel_family) asks for -- the workgroup classes measured faster
+ * at every size on MI355X; mode 2: the wave family for every block up to 32 DOF;
+ * mode 0: never.  Both give the same results to rounding.  Returns the previous mode
+ * (-1 and NEGF_ERR_ARG text for a bad mode); process-wide.  No counterpart in the
+ * reference (a launch-configuration switch, like negf_set_block_class).
  */
 int negf_set_wave_kernels(int mode);
 

@@ -332,8 +332,9 @@ __device__ __forceinline__ void crecip1(T ar, T ai, T& rr, T& ri) {
 // four rows per lane, column groups of 4, 8 or 16 lanes).  A step's cost is the
 // instruction stream of the wave that publishes the next column, so that stream is
 // cut to the essentials:
-//  - the pivot is the largest |re| + |im| compared to 2^-14 relative (the magnitude's
-//    high word with its low 6 bits replaced by 63 - row, maximised with v_max_u32 over
+//  - the pivot is the largest |re| + |im| compared to 2^-14 relative (2^-13 above 64
+//    rows: the magnitude's high word with its low 6 (7) bits replaced by 63 - row
+//    (127 - row), maximised with v_max_u32 over
 //    the group by DPP; the first row among magnitudes equal to that resolution), and
 //    its failure flag travels in the pivot-row word (-flag);
 //  - the column and the pivot are read with one LDS wait;
@@ -354,7 +355,10 @@ __device__ __forceinline__ int gj_lean(PT __restrict__ re, PT __restrict__ im, i
                                        GjLeanBuf<T, NMAX>& b, int tid) {
   constexpr int MAXR = NMAX / RG;
   static_assert(NMAX * RG == NT || NMAX * RG <= NT, "not enough threads for the inversion layout");
-  static_assert(NMAX <= 64, "row index in 6 bits");
+  static_assert(NMAX <= 128, "row index in 7 bits");
+  // the key's low RB bits carry (2^RB - 1 - row); the magnitude keeps 20 - RB mantissa bits
+  constexpr int RB = NMAX <= 64 ? 6 : 7;
+  constexpr unsigned RMASK = (1u << RB) - 1u;
   const bool on = tid < NMAX * RG;
   const int jc = on ? tid / RG : 0;
   const int j = on ? jc : 0x7fff;
@@ -387,8 +391,8 @@ __device__ __forceinline__ int gj_lean(PT __restrict__ re, PT __restrict__ im, i
 #pragma unroll
       for (int m = 0; m < MAXR; ++m) {
         const double mg = (double)(fabs(xr[m]) + fabs(xi[m]));
-        const unsigned kb = ((unsigned)((unsigned long long)__double_as_longlong(mg) >> 32) & ~63u) |
-                            (unsigned)(63 - (g + RG * m));
+        const unsigned kb = ((unsigned)((unsigned long long)__double_as_longlong(mg) >> 32) & ~RMASK) |
+                            (unsigned)((int)RMASK - (g + RG * m));
         key = ((used >> m) & 1u) ? key : max(key, kb);
       }
       // (mov_dpp with bound_ctrl lets the compiler fold each move into v_max_u32_dpp)
@@ -399,9 +403,9 @@ __device__ __forceinline__ int gj_lean(PT __restrict__ re, PT __restrict__ im, i
         key = max(key, (unsigned)__builtin_amdgcn_mov_dpp((int)key, 0x141, 0xF, 0xF, true));
       if constexpr (RG >= 16)
         key = max(key, (unsigned)__builtin_amdgcn_mov_dpp((int)key, 0x140, 0xF, 0xF, true));
-      const int bidx = 63 - (int)(key & 63u);
+      const int bidx = (int)RMASK - (int)(key & RMASK);
       // NaN and infinity sort above every finite magnitude; zero (or subnormal) is singular
-      const int flag = key >= 0x7FF00000u ? ST_NONFINITE : (key < 64u ? ST_SINGULAR : ST_OK);
+      const int flag = key >= 0x7FF00000u ? ST_NONFINITE : (key <= RMASK ? ST_SINGULAR : ST_OK);
       if (g == (bidx & (RG - 1))) {  // the lane holding the pivot row
         const int mw = bidx / RG;
         T pr = xr[0], pi = xi[0];
@@ -510,7 +514,9 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
   constexpr int MAXR = NMAX / RG;
   static_assert(NMAX * RG <= NT, "not enough threads for the inversion layout");
   static_assert(MAXR <= 32, "row mask is 32 bits");
-  if constexpr (SF && MAXR <= 4 && (RG == 4 || RG == 8 || RG == 16) && NMAX <= 64 &&
+  // (the pooled class, twelve rows per lane, keeps this step: in the lean one its rows
+  // and the published column do not fit the registers together and spill)
+  if constexpr (SF && MAXR <= 4 && (RG == 4 || RG == 8 || RG == 16) && NMAX <= 128 &&
                 sizeof(GjLeanBuf<T, NMAX>) <= sizeof(GjBuf<T, NMAX>)) {
     return gj_lean<T, NT, NMAX, RG, LS>(re, im, ld, n,
                                        *reinterpret_cast<GjLeanBuf<T, NMAX>*>(&b), tid);

@@ -2188,33 +2188,26 @@ static int default_nmax(int n, bool fp64) {
   return n <= 64 ? 64 : 0;
 }
 
-// One-instance-per-wavefront MFMA kernels (wpi.hpp) for blocks of up to 32 DOF.
-// Mode 1 (default) uses them where they measured faster than the workgroup classes on
-// MI355X (8192 instances, 16 layers, eta = 1e-6, ms per launch):
-//   FP64, n = 12: decimation 1.57 vs 1.71, sweep 0.79 vs 1.05; at n = 6 (class 8) the
-//   wave form is 2x slower, at n = 16 and above 6-40 % slower;
-//   FP32 decimation: 0.64 vs 0.72 at n = 12, 2.73 vs 2.80 at 20, 3.20 vs 3.48 at 24,
-//   slower at 6 and from 30;
-// so FP64 blocks of 9-12 DOF and FP32 decimations of 9-12 and 17-24 DOF (FP32 sweeps of
-// 9-24).  Elsewhere the wave's register-resident Gauss-Jordan step (one wavefront per
-// instance, no other wave to hide its latency) loses to the workgroup form.  Mode 2
-// uses them for every block up to 32 DOF, mode 0 never (negf_set_wave_kernels).
-// In mode 1 a per-size choice (negf_set_kernel_family, the autotuner's) overrides the
-// default: [0] FP64 decimations and sweeps, [1] FP32 decimations and sweeps.
+// One-instance-per-wavefront MFMA kernels (wpi.hpp) for blocks of up to 32 DOF.  With
+// the matrix-core products and the lean pivot step of the workgroup classes, the
+// classes are faster at every size on MI355X (8192 instances, 16 layers, eta = 1e-6,
+// ms per solve, class / wave): FP64 n = 6: 0.50 / 1.40, 12: 2.26 / 2.48, 16: 3.09 / 4.36,
+// 24: 8.33 / 15.5; the FP32 decimation of the mixed path at n = 24: 1.46 / 1.61.  So mode 1
+// (default) uses the wave family only where a per-size choice (negf_set_kernel_family,
+// the autotuner's) asks for it; mode 2 uses it for every block up to 32 DOF, mode 0 never
+// (negf_set_wave_kernels).  [0] of the per-size table: FP64 decimations and sweeps,
+// [1]: FP32 decimations and sweeps.
 constexpr int WAVE_MAX = 32;
 static std::atomic<int> g_wave_kernels{1};
 static std::atomic<int> g_family_override[2][WAVE_MAX + 1];
 static bool use_wave(int n, bool fp64, bool sweep) {
+  (void)sweep;
   if (n < 1 || n > WAVE_MAX) return false;
   const int mode = g_wave_kernels.load();
   if (mode == 2) return true;
   if (mode != 1) return false;
   const int f = g_family_override[fp64 ? 0 : 1][n].load(std::memory_order_relaxed);
-  if (f) return f == NEGF_FAMILY_WAVE;
-  if (n <= 8) return false;
-  if (fp64) return n <= 12;
-  if (sweep) return n <= 24;
-  return n <= 12 || (n >= 17 && n <= 24);
+  return f == NEGF_FAMILY_WAVE;
 }
 // Matrix-core products (mm.hpp) in the FP64 workgroup decimation and sweep of
 // classes 8..48: mode 1 (default) on, 0 off (negf_set_matrix_core_products).

@@ -130,13 +130,19 @@ def test_kernel_family_table():
     lib = _lib.load_symbols_only()
     prev = lib.negf_set_wave_kernels(1)
     try:
-        assert lib.negf_kernel_family(_lib.FP64, 12) == _lib.FAMILY_WAVE      # default rule
-        assert lib.negf_kernel_family(_lib.FP64, 24) == _lib.FAMILY_WORKGROUP
+        for prec in (_lib.FP64, _lib.MIXED):                                  # default rule
+            for n in (1, 8, 12, 20, 24, 32):
+                assert lib.negf_kernel_family(prec, n) == _lib.FAMILY_WORKGROUP
         assert lib.negf_set_kernel_family(_lib.FP64, 24, _lib.FAMILY_WAVE) == 0
         assert lib.negf_kernel_family(_lib.FP64, 24) == _lib.FAMILY_WAVE
         assert lib.negf_set_kernel_family(_lib.FP64, 12, _lib.FAMILY_WORKGROUP) == 0
         assert lib.negf_kernel_family(_lib.FP64, 12) == _lib.FAMILY_WORKGROUP
-        assert lib.negf_kernel_family(_lib.MIXED, 20) == _lib.FAMILY_WAVE     # FP32 decimation
+        assert lib.negf_set_kernel_family(_lib.MIXED, 20, _lib.FAMILY_WAVE) == 0  # FP32 side
+        assert lib.negf_kernel_family(_lib.MIXED, 20) == _lib.FAMILY_WAVE
+        assert lib.negf_kernel_family(_lib.FP64, 20) == _lib.FAMILY_WORKGROUP
+        assert lib.negf_set_wave_kernels(2) == 1  # mode 2: always, up to 32 DOF
+        assert lib.negf_kernel_family(_lib.FP64, 32) == _lib.FAMILY_WAVE
+        assert lib.negf_kernel_family(_lib.FP64, 33) == _lib.FAMILY_WORKGROUP
         lib.negf_set_wave_kernels(0)  # mode 0: never, whatever the table says
         assert lib.negf_kernel_family(_lib.FP64, 24) == _lib.FAMILY_WORKGROUP
         for args, word in (((_lib.FP64, 33, 1), b"1..32"), ((_lib.FP64, 0, 1), b"1..32"),
@@ -146,6 +152,7 @@ def test_kernel_family_table():
     finally:
         for n in (12, 24):
             lib.negf_set_kernel_family(_lib.FP64, n, 0)
+        lib.negf_set_kernel_family(_lib.MIXED, 20, 0)
         lib.negf_set_wave_kernels(prev)
 
 

@@ -48,7 +48,7 @@ def test_tuner_times_every_candidate_and_applies_the_fastest(gpu, restore_classe
     print(f"tuned: fp64 n=12 {res['timings_ms']} -> {res['best']}; mixed n=20 "
           f"{res_m['timings_ms']} -> {res_m['best']}", flush=True)
     reset_block_class(12, "fp64")
-    assert block_class(12, "fp64") == 16 and kernel_family(12, "fp64") == WAVE  # default rule
+    assert block_class(12, "fp64") == 16 and kernel_family(12, "fp64") == "workgroup"  # default
     assert 96 not in candidate_classes(40, "fp64") and WAVE not in candidate_classes(40, "fp64")
 
 
