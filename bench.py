@@ -38,7 +38,7 @@ METRIC = "(ω,k) transmission solves/s, mixed vs FP64 GPU path at ≤1e-6 rel er
 PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}  # MI355X dense matrix peaks (FP64, FP32)
 # HBM bytes per launch of the FP64 kernels on this workload, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE scaled by a copy-kernel calibration (profiles/ribbon24_hbm_counters.md)
-TRAFFIC: dict = {"rgf": 122.1e6, "sancho_rubio": 136.9e6}
+TRAFFIC: dict = {"sweep": None, "sancho_rubio": 136.9e6}
 
 
 def _cpu_worker(args):
@@ -525,17 +525,18 @@ def main():
     # algorithmic real flops per instance of each stage (complex multiply-add = 8
     # flops).  "survey": the hot-path table's formulas -- decimation (56 s + 8) n^3
     # per lead (one decimation serves both identical leads), sweep 8 n^3 + 40 (L - 1)
-    # n^3, self-energy 16 n^3 per lead, trace 16 n^3.  "kernel": what the kernels
-    # execute -- the sweep carries G_1N with one product per layer, 8 n^3 + 32 (L - 1) n^3.
+    # n^3, self-energy 16 n^3 per lead, trace 16 n^3; the "sweep" stage is one kernel
+    # that forms both self-energies, runs the sweep and takes the trace.  "kernel": what
+    # the kernels execute -- the sweep carries G_1N with one product per layer,
+    # 8 n^3 + 32 (L - 1) n^3.
     fl_survey = {"sancho_rubio_l": (56.0 * s64 + 8.0) * n3,
-                 "rgf": 8.0 * n3 + 40.0 * (n_layers - 1) * n3,
-                 "self_energy_l": 16.0 * n3, "self_energy_r": 16.0 * n3, "trace": 16.0 * n3}
-    fl_kernel = dict(fl_survey, rgf=8.0 * n3 + 32.0 * (n_layers - 1) * n3)
+                 "sweep": 8.0 * n3 + 40.0 * (n_layers - 1) * n3 + 3 * 16.0 * n3}
+    fl_kernel = dict(fl_survey, sweep=8.0 * n3 + 32.0 * (n_layers - 1) * n3 + 3 * 16.0 * n3)
     fp64_fl_total = sum(fl_survey.values())
 
     def roofline(path):
         st = results[path]["stage_ms"]
-        cand = [k for k in ("sancho_rubio_l", "rgf") if k in st]
+        cand = [k for k in ("sancho_rubio_l", "sweep") if k in st]
         dom = max(cand, key=lambda k: st[k])
         fs, fk = fl_survey[dom], fl_kernel[dom]
         peak = PEAK_TFLOPS["fp64"]
@@ -545,7 +546,7 @@ def main():
             peak = PEAK_TFLOPS["mixed"]
         ach = fs * n_inst / (st[dom] * 1e-3) / 1e12
         ach_k = fk * n_inst / (st[dom] * 1e-3) / 1e12
-        name = {"sancho_rubio_l": "sancho_rubio", "rgf": "rgf"}[dom]
+        name = {"sancho_rubio_l": "sancho_rubio", "sweep": "sweep"}[dom]
         return {"bound": "mfma", "kernel": name, "achieved": round(ach, 3), "peak": peak,
                 "unit": "TFLOP/s", "frac": round(ach / peak, 5), "traffic": TRAFFIC.get(name),
                 "flops_per_instance": fs,

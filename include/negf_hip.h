@@ -245,6 +245,29 @@ int negf_transmission_trace_batched(void* stream, int n, int n_inst, const doubl
 int negf_merge_status(void* stream, int n_inst, int32_t* st_l, const int32_t* st_r);
 
 /*
+ * The stages of negf_transmission_batched after the surface GFs, in one launch where
+ * the block class allows it: Sigma_L and Sigma_R (as negf_self_energy_batched, into
+ * sigma_l / sigma_r), the FP64 sweep for G_1N (as negf_rgf_g1n_batched, into g1n_out)
+ * and T (as negf_transmission_trace_batched), with every block product on the matrix
+ * cores.  Replaces the self-energies, the solve and the trace of
+ * src/negfpy/core/negf.py:306-357 (with :115-116) in one kernel.  Fused for FP64
+ * workgroup classes 8..48 with matrix-core products (negf_set_matrix_core_products 1)
+ * and leads no larger than the device block's class; otherwise the four kernels run
+ * in turn (and status_l then holds the merged lead status).
+ *   g_l, g_r [n_inst][n_lead][n_lead]  surface GFs; v_*, idx_*: as negf_self_energy_batched
+ *   status_l, status_r [n_inst]        lead statuses, merged left first into status
+ *   sigma_l, sigma_r, g1n_out [n_inst][n][n]   written
+ *   status, t_out [n_inst]             0 / lead status / 12, 13; NaN T where status != 0
+ */
+int negf_transmission_sweep_batched(
+    void* stream, int n, int n_layers, int n_inst, const double* onsite, const double* coupling,
+    const int32_t* set_of, const double* omega, const double* eta_dev, int n_lead_l, int n_c_l,
+    const double* v_l, const double* g_l, const int32_t* idx_l, int32_t* status_l, int n_lead_r,
+    int n_c_r, const double* v_r, const double* g_r, const int32_t* idx_r,
+    const int32_t* status_r, double* sigma_l, double* sigma_r, double* g1n_out, int32_t* status,
+    double* t_out);
+
+/*
  * An eta ladder per instance (negf_transmission_multi_batched only, decimation
  * route): each instance tries its rungs in order, eta_lead = the rung's eta, inside
  * the persistent launch, until one is accepted (status 0, finite T,

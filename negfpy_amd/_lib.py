@@ -106,6 +106,9 @@ _SIGS = {
          ctypes.c_size_t, _vp, _vp, _vp, _vp], _c_int),
     "negf_transmission_trace_batched": ([_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp], _c_int),
     "negf_merge_status": ([_vp, _c_int, _vp, _vp], _c_int),
+    "negf_transmission_sweep_batched": (
+        [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp, _vp,
+         _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
     "negf_transmission_workspace_bytes": ([ctypes.POINTER(TransmissionProblem)], ctypes.c_size_t),
     "negf_transmission_multi_workspace_bytes": ([_c_int, ctypes.POINTER(TransmissionProblem)],
                                                 ctypes.c_size_t),

@@ -355,20 +355,15 @@ class TransmissionSolver:
                           eta_lead.data_ptr(), g.data_ptr(), dos.data_ptr(), stt.data_ptr()))
         if s.share_leads:
             st_r.copy_(st_l)
-        stage("self_energy_l", lambda: lib.negf_self_energy_batched(
-            st, nl, int(s.v_l.shape[1]), n, n_inst, s.v_l.data_ptr(), set_of.data_ptr(),
-            g_l.data_ptr(), s.idx_l.data_ptr(), st_l.data_ptr(), sig_l.data_ptr()))
-        stage("self_energy_r", lambda: lib.negf_self_energy_batched(
-            st, nr, int(s.v_r.shape[1]), n, n_inst, s.v_r.data_ptr(), set_of.data_ptr(),
-            g_r.data_ptr(), s.idx_r.data_ptr(), st_r.data_ptr(), sig_r.data_ptr()))
-        _lib.check(lib.negf_merge_status(st, n_inst, st_l.data_ptr(), st_r.data_ptr()))
-        # the device sweep runs in FP64 in both precisions, as in negf_transmission_batched
-        stage("rgf", lambda: lib.negf_rgf_g1n_batched(
-            st, _lib.FP64, n, s.n_layers, n_inst, s.onsite.data_ptr(), s.coupling.data_ptr(),
-            set_of.data_ptr(), omega.data_ptr(), eta_dev.data_ptr(), sig_l.data_ptr(),
-            sig_r.data_ptr(), st_l.data_ptr(), g1n.data_ptr(), out.status.data_ptr()))
-        stage("trace", lambda: lib.negf_transmission_trace_batched(
-            st, n, n_inst, sig_l.data_ptr(), sig_r.data_ptr(), g1n.data_ptr(),
+        # the self-energies, the device sweep (FP64 in both precisions) and the trace:
+        # one kernel where the block class has the fused form, as in
+        # negf_transmission_batched
+        stage("sweep", lambda: lib.negf_transmission_sweep_batched(
+            st, n, s.n_layers, n_inst, s.onsite.data_ptr(), s.coupling.data_ptr(),
+            set_of.data_ptr(), omega.data_ptr(), eta_dev.data_ptr(), nl, int(s.v_l.shape[1]),
+            s.v_l.data_ptr(), g_l.data_ptr(), s.idx_l.data_ptr(), st_l.data_ptr(), nr,
+            int(s.v_r.shape[1]), s.v_r.data_ptr(), g_r.data_ptr(), s.idx_r.data_ptr(),
+            st_r.data_ptr(), sig_l.data_ptr(), sig_r.data_ptr(), g1n.data_ptr(),
             out.status.data_ptr(), out.t.data_ptr()))
         if s.share_leads:
             out.iters_r.copy_(out.iters_l)

@@ -1174,12 +1174,133 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
   if (tid == 0) a.status[inst] = status;
 }
 
+// Inputs of the contact self-energies and the trace when they run inside the sweep
+// (k_rgf_mm<NMAX, true>): the self-energy arguments of each lead (their sigma_out is
+// the end-layer block the sweep reads back, status_in the lead status) and T.
+struct RgfFuse {
+  SigmaArgs l, r;
+  double* t_out;  // [n_inst]
+};
+
+// Sigma = V g V^H of one lead on the matrix cores, embedded into the n x n end-layer
+// block at the contact positions idx and written to sigma_out (coalesced).  Uses all
+// three LDS matrices and the pivot-row array as scratch and leaves them dirty; the
+// lead block must fit the class (n_lead, n_c <= NMAX).  Every thread reaches every
+// barrier; only `live` sub-blocks work (the persistent launch shares barriers between
+// sub-blocks).  The persistent launch and the fused sweep share this arithmetic, so
+// their self-energies are identical.
+template <int NMAX, typename L>
+__device__ __forceinline__ void sigma_mm(const SigmaArgs& sa, int inst, int n, L& s, int tid,
+                                         bool live) {
+  constexpr int NT = Cfg<NMAX>::NT;
+  constexpr int LD = L::LD;
+  using M = MOwn<NMAX, NT>;
+  constexpr int TPW = M::TPW;
+  const M o(tid);
+  const int nlead = sa.n_lead, nc = sa.n_c;
+  const int nsl = (nlead + 3) / 4;
+  int* pos = s.gj->pv;  // device-layer index -> contact row
+  __syncthreads();  // the previous stage's last reads of these matrices are done
+  if (live) s.zero(tid);
+  __syncthreads();
+  if (live) {
+    gload<double, NT>(sa.v + (size_t)sa.set_of[inst] * nc * nlead * 2, nc, nlead, s.re[1],
+                      s.im[1], LD, tid);
+    gload<double, NT>(sa.g + (size_t)inst * nlead * nlead * 2, nlead, nlead, s.re[2], s.im[2],
+                      LD, tid);
+    for (int i = tid; i < n; i += NT) pos[i] = -1;
+  }
+  __syncthreads();
+  double xr[TPW], xi[TPW];
+  mzero<TPW>(xr, xi);
+  if (live) {
+    for (int c = tid; c < nc; c += NT) pos[sa.idx[c]] = c;
+    mm<NMAX, NT, OPN, OPN>(s.re[1], s.im[1], s.re[2], s.im[2], LD, nsl, o, xr, xi);  // V g
+    // rows >= nc and columns >= nlead of V g are exact zeros (zero operand padding)
+    mstore<NMAX, NT>(s.re[0], s.im[0], LD, nc > nlead ? nc : nlead, o, xr, xi);
+  }
+  __syncthreads();
+  if (live) {
+    mzero<TPW>(xr, xi);
+    mm<NMAX, NT, OPN, OPH>(s.re[0], s.im[0], s.re[1], s.im[1], LD, nsl, o, xr, xi);  // (V g) V^H
+    mstore<NMAX, NT>(s.re[2], s.im[2], LD, nc, o, xr, xi);
+  }
+  __syncthreads();
+  if (live) {
+    double2* out = reinterpret_cast<double2*>(sa.sigma_out + (size_t)inst * n * n * 2);
+    for (int e = tid; e < n * n; e += NT) {
+      const int i = e / n, j = e - (e / n) * n;
+      const int pi = pos[i], pj = pos[j];
+      out[e] = (pi >= 0 && pj >= 0) ? make_double2(s.re[2][pi * LD + pj], s.im[2][pi * LD + pj])
+                                    : make_double2(0.0, 0.0);
+    }
+  }
+  // the trace reads elements other threads of the workgroup wrote here: a workgroup
+  // barrier orders them (one CU, one L1)
+  __syncthreads();
+}
+
+// T = Re Tr[Gamma_L G Gamma_R G^H] = Re sum_ij U_ij conj(G_ij), W = G Gamma_R, U = Gamma_L W,
+// Gamma = i (Sigma - Sigma^H), on the matrix cores.  G (n x n) is in LDS matrix [1] and in
+// (gr, gi) in the matrix-core ownership; Gamma_L is built in [0] and Gamma_R in [2] from
+// the self-energies in global memory, and [2] then takes W.  Returns T to every thread
+// (0 where not live).  Shared by the fused sweep and the persistent launch.
+template <int NMAX, typename L>
+__device__ __forceinline__ double trace_mm(const double* sigma_l, const double* sigma_r, int inst,
+                                           int n, L& s, const double (&gr)[MOwn<NMAX, Cfg<NMAX>::NT>::TPW],
+                                           const double (&gi)[MOwn<NMAX, Cfg<NMAX>::NT>::TPW],
+                                           int tid, bool live) {
+  constexpr int NT = Cfg<NMAX>::NT;
+  constexpr int LD = L::LD;
+  using M = MOwn<NMAX, NT>;
+  constexpr int TPW = M::TPW;
+  const M o(tid);
+  const int ns = (n + 3) / 4;
+  const int nn = n * n;
+  if (live) {
+    const double2* sl = reinterpret_cast<const double2*>(sigma_l + (size_t)inst * nn * 2);
+    const double2* sr = reinterpret_cast<const double2*>(sigma_r + (size_t)inst * nn * 2);
+    // Gamma_ij = i (S_ij - conj(S_ji)) = (-(Im S_ij + Im S_ji), Re S_ij - Re S_ji)
+    for (int e = tid; e < nn; e += NT) {
+      const int i = e / n, j = e - (e / n) * n;
+      const double2 a1 = sl[e], a2 = sl[j * n + i];
+      s.re[0][i * LD + j] = -(a1.y + a2.y);
+      s.im[0][i * LD + j] = a1.x - a2.x;
+      const double2 b1 = sr[e], b2 = sr[j * n + i];
+      s.re[2][i * LD + j] = -(b1.y + b2.y);
+      s.im[2][i * LD + j] = b1.x - b2.x;
+    }
+  }
+  __syncthreads();
+  double wr[TPW], wi[TPW];
+  mzero<TPW>(wr, wi);
+  if (live) mm<NMAX, NT, OPN, OPN>(s.re[1], s.im[1], s.re[2], s.im[2], LD, ns, o, wr, wi);
+  __syncthreads();
+  if (live) mstore<NMAX, NT>(s.re[2], s.im[2], LD, n, o, wr, wi);
+  __syncthreads();
+  double acc = 0.0;
+  if (live) {
+    mzero<TPW>(wr, wi);
+    mm<NMAX, NT, OPN, OPN>(s.re[0], s.im[0], s.re[2], s.im[2], LD, ns, o, wr, wi);
+#pragma unroll
+    for (int q = 0; q < TPW; ++q)
+      if (o.row(q) < n && o.col(q) < n) acc = __fma_rn(wi[q], gi[q], __fma_rn(wr[q], gr[q], acc));
+  }
+  return block_sum<NT>(acc, s.sred, tid);
+}
+
 // The same sweep in FP64 with every block product on the matrix cores (mm.hpp):
 // Y = g H, then S = H^H Y and Q = Q Y from one pass over Y.  The Schur term S
-// stays in registers in the matrix-core ownership.
-template <int NMAX>
+// stays in registers in the matrix-core ownership.  FUSE: the launch also forms
+// both contact self-energies first (sigma_mm, into a.sigma_l / a.sigma_r) and, after
+// G_1N, T = Re Tr[Gamma_L G_1N Gamma_R G_1N^H] as Re sum_ij U_ij conj(G_ij) with
+// W = G_1N Gamma_R and U = Gamma_L W (two more matrix-core products), so the lead
+// surface GFs go in and T comes out of one kernel; G_1N is still written when
+// a.g1n_out is set.  The lead status is merged as negf_merge_status does (left first).
+template <int NMAX, bool FUSE>
 __global__ void __launch_bounds__(Cfg<NMAX>::NT)
-    __attribute__((amdgpu_waves_per_eu(Occ3<double, NMAX>::value))) k_rgf_mm(RgfArgs a) {
+    __attribute__((amdgpu_waves_per_eu(Occ3<double, NMAX>::value)))
+    k_rgf_mm(RgfArgs a, RgfFuse f) {
   using T = double;
   constexpr int NT = Cfg<NMAX>::NT;
   using L = Lds<T, NMAX, NT, 3>;
@@ -1190,7 +1311,17 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
   const int tid = threadIdx.x;
   const int inst = blockIdx.x;
   if (a.redo && a.status[inst] != 10 + ST_NONFINITE && a.status[inst] != 10 + ST_SINGULAR) return;
-  if (a.status_in && a.status_in[inst] != ST_OK) {
+  if constexpr (FUSE) {
+    const int stl = f.l.status_in[inst], str = f.r.status_in[inst];
+    const int lead_st = stl != ST_OK ? stl : str;
+    if (lead_st != ST_OK) {
+      if (tid == 0) {
+        a.status[inst] = lead_st;
+        f.t_out[inst] = NAN;
+      }
+      return;
+    }
+  } else if (a.status_in && a.status_in[inst] != ST_OK) {
     if (tid == 0) a.status[inst] = a.status_in[inst];
     return;
   }
@@ -1205,6 +1336,12 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
   // LDS: [0] g (inverted in place), then Y = g H; [1] H; [2] Q = G_{1,i} H_i
   lds_zero<T, NT>(smem, L::mat_bytes(), tid);
   __syncthreads();
+  if constexpr (FUSE) {
+    sigma_mm<NMAX>(f.l, inst, n, s, tid, true);
+    sigma_mm<NMAX>(f.r, inst, n, s, tid, true);
+    lds_zero<T, NT>(smem, L::mat_bytes(), tid);
+    __syncthreads();
+  }
   const M o(tid);
   double zrd, zid;
   zsq(a.omega[inst], a.eta_dev[inst], zrd, zid);
@@ -1261,16 +1398,24 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
     }
     if (layer == nl - 1) {
       // G_1N = Q_{N-1} g_N (g_1 itself for one layer), through LDS to a coalesced store
+      T gr[TPW], gi[TPW];
+      mzero<TPW>(gr, gi);
       if (nl > 1) {
-        T gr[TPW], gi[TPW];
-        mzero<TPW>(gr, gi);
         mm<NMAX, NT, OPN, OPN>(s.re[2], s.im[2], s.re[0], s.im[0], LD, ns, o, gr, gi);
-        mstore<NMAX, NT>(s.re[1], s.im[1], LD, n, o, gr, gi);
-        __syncthreads();
-        gstore<T, NT>(a.g1n_out + (size_t)inst * nn * 2, n, s.re[1], s.im[1], LD, tid);
       } else {
-        gstore<T, NT>(a.g1n_out + (size_t)inst * nn * 2, n, s.re[0], s.im[0], LD, tid);
+#pragma unroll
+        for (int q = 0; q < TPW; ++q) {
+          const int i = o.row(q), j = o.col(q);
+          if (i < n && j < n) {
+            gr[q] = s.re[0][i * LD + j];
+            gi[q] = s.im[0][i * LD + j];
+          }
+        }
+        __syncthreads();
       }
+      mstore<NMAX, NT>(s.re[1], s.im[1], LD, n, o, gr, gi);
+      __syncthreads();
+      if (a.g1n_out) gstore<T, NT>(a.g1n_out + (size_t)inst * nn * 2, n, s.re[1], s.im[1], LD, tid);
       break;
     }
 #pragma unroll
@@ -1302,6 +1447,25 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
       mm<NMAX, NT, OPH, OPN>(s.re[1], s.im[1], s.re[0], s.im[0], LD, ns, o, sr, si);
       __syncthreads();
     }
+  }
+  if constexpr (FUSE) {
+    // the trace, outside the layer loop so its registers do not add to the loop's: G_1N
+    // back from LDS matrix [1] into the matrix-core ownership
+    const bool ok = status == ST_OK;
+    T gr[TPW], gi[TPW];
+    mzero<TPW>(gr, gi);
+    if (ok) {
+#pragma unroll
+      for (int q = 0; q < TPW; ++q) {
+        const int i = o.row(q), j = o.col(q);
+        if (i < n && j < n) {
+          gr[q] = s.re[1][i * LD + j];
+          gi[q] = s.im[1][i * LD + j];
+        }
+      }
+    }
+    const double tot = trace_mm<NMAX>(a.sigma_l, a.sigma_r, inst, n, s, gr, gi, tid, ok);
+    if (tid == 0) f.t_out[inst] = ok ? tot : NAN;
   }
   if (tid == 0) a.status[inst] = status;
 }
@@ -2296,12 +2460,12 @@ class BigUse {
   } while (0)
 #define NEGF_BIG_LAUNCH(KERNEL, STREAM, ARGS) NEGF_POOL_LAUNCH(KERNEL, STREAM, ARGS, g_big, BIG_SLOT)
 
-#define NEGF_LAUNCH(KERNEL, GRID, NT, LDS, STREAM, ARGS)                                         \
+#define NEGF_LAUNCH(KERNEL, GRID, NT, LDS, STREAM, ...)                                          \
   do {                                                                                          \
     hipError_t e_ = hipFuncSetAttribute(reinterpret_cast<const void*>(&KERNEL),                 \
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)(LDS)); \
     if (e_ != hipSuccess) return hip_check(e_, #KERNEL " attribute");                           \
-    hipLaunchKernelGGL(KERNEL, dim3(GRID), dim3(NT), (LDS), (STREAM), ARGS);                   \
+    hipLaunchKernelGGL(KERNEL, dim3(GRID), dim3(NT), (LDS), (STREAM), __VA_ARGS__);            \
     e_ = hipGetLastError();                                                                     \
     if (e_ != hipSuccess) return hip_check(e_, #KERNEL " launch");                              \
   } while (0)
@@ -2446,7 +2610,8 @@ static int run_rgf(hipStream_t st, int nmax, const RgfArgs& a) {
 #define RGM_CASE(NM)                                                                       \
   case NM: {                                                                               \
     constexpr int NT = Cfg<NM>::NT;                                                        \
-    NEGF_LAUNCH((k_rgf_mm<NM>), a.n_inst, NT, (Lds<double, NM, NT, 3>::bytes()), st, a);   \
+    NEGF_LAUNCH((k_rgf_mm<NM, false>), a.n_inst, NT, (Lds<double, NM, NT, 3>::bytes()), st, a, \
+                RgfFuse{});                                                                \
     return NEGF_OK;                                                                        \
   }
       switch (nmax) {
@@ -2849,6 +3014,67 @@ int negf_transmission_trace_batched(void* stream, int n, int n_inst, const doubl
   return run_trace(reinterpret_cast<hipStream_t>(stream), nmax, a);
 }
 
+int negf_transmission_sweep_batched(
+    void* stream, int n, int n_layers, int n_inst, const double* onsite, const double* coupling,
+    const int32_t* set_of, const double* omega, const double* eta_dev, int n_lead_l, int n_c_l,
+    const double* v_l, const double* g_l, const int32_t* idx_l, int32_t* status_l, int n_lead_r,
+    int n_c_r, const double* v_r, const double* g_r, const int32_t* idx_r,
+    const int32_t* status_r, double* sigma_l, double* sigma_r, double* g1n_out, int32_t* status,
+    double* t_out) {
+  if (n <= 0 || n_layers <= 0 || n_inst < 0)
+    return fail(NEGF_ERR_ARG, "invalid sweep sizes (n=%d layers=%d)", n, n_layers);
+  if (n_lead_l <= 0 || n_c_l <= 0 || n_c_l > n || n_lead_r <= 0 || n_c_r <= 0 || n_c_r > n)
+    return fail(NEGF_ERR_ARG, "invalid contact sizes (lead %d/%d, contact %d/%d, layer %d)",
+                n_lead_l, n_lead_r, n_c_l, n_c_r, n);
+  if (!onsite || (n_layers > 1 && !coupling) || !set_of || !omega || !eta_dev || !v_l || !g_l ||
+      !idx_l || !status_l || !v_r || !g_r || !idx_r || !status_r || !sigma_l || !sigma_r ||
+      !g1n_out || !status || !t_out)
+    return fail(NEGF_ERR_ARG, "null pointer argument");
+  if (n_inst == 0) return NEGF_OK;
+  if (!pick_nmax(n, true)) return fail(NEGF_ERR_ARG, "block size %d exceeds the supported maximum", n);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  // one class holds the device block and both lead blocks (the persistent launch picks
+  // its class the same way)
+  int m = n > n_lead_l ? n : n_lead_l;
+  m = m > n_lead_r ? m : n_lead_r;
+  const int nmax = pick_nmax(m, true);
+  const bool fused = nmax && !use_wave(n, true, true) && use_mm(nmax);
+  if (fused) {
+    RgfArgs a{n, n_layers, n_inst, onsite, coupling, set_of, omega, eta_dev,
+              sigma_l, sigma_r, nullptr, g1n_out, status, 0};
+    RgfFuse f{SigmaArgs{n_lead_l, n_c_l, n, n_inst, v_l, set_of, g_l, idx_l, status_l, sigma_l},
+              SigmaArgs{n_lead_r, n_c_r, n, n_inst, v_r, set_of, g_r, idx_r, status_r, sigma_r},
+              t_out};
+#define RGF_CASE(NM)                                                                         \
+  case NM: {                                                                                 \
+    constexpr int NT = Cfg<NM>::NT;                                                          \
+    NEGF_LAUNCH((k_rgf_mm<NM, true>), n_inst, NT, (Lds<double, NM, NT, 3>::bytes()), st, a, f); \
+    return NEGF_OK;                                                                          \
+  }
+    switch (nmax) {
+      RGF_CASE(8)
+      RGF_CASE(16)
+      RGF_CASE(24)
+      RGF_CASE(32)
+      RGF_CASE(48)
+    }
+#undef RGF_CASE
+  }
+  int rc = negf_self_energy_batched(stream, n_lead_l, n_c_l, n, n_inst, v_l, set_of, g_l, idx_l,
+                                    status_l, sigma_l);
+  if (rc) return rc;
+  rc = negf_self_energy_batched(stream, n_lead_r, n_c_r, n, n_inst, v_r, set_of, g_r, idx_r,
+                                status_r, sigma_r);
+  if (rc) return rc;
+  rc = negf_merge_status(stream, n_inst, status_l, status_r);
+  if (rc) return rc;
+  rc = negf_rgf_g1n_batched(stream, NEGF_FP64, n, n_layers, n_inst, onsite, coupling, set_of, omega,
+                            eta_dev, sigma_l, sigma_r, status_l, g1n_out, status);
+  if (rc) return rc;
+  return negf_transmission_trace_batched(stream, n, n_inst, sigma_l, sigma_r, g1n_out, status,
+                                         t_out);
+}
+
 // a mirrored lead pair (right lead = left lead with d01 and d10 exchanged) whose blocks
 // fit the mirrored decimation: one decimation yields both surfaces
 static bool sr_mirror_fits(const negf_transmission_problem* p) {
@@ -2972,22 +3198,12 @@ int negf_transmission_batched(void* stream, const negf_transmission_problem* p, 
                                   reinterpret_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_check(e, "status copy");
   }
-  rc = negf_self_energy_batched(stream, p->n_lead_l, p->n_c_l, p->n, p->n_inst, p->v_l, p->set_of,
-                                g_l, p->idx_l, st_l, sig_l);
-  if (rc) return rc;
-  rc = negf_self_energy_batched(stream, p->n_lead_r, p->n_c_r, p->n, p->n_inst, p->v_r, p->set_of,
-                                g_r, p->idx_r, st_r, sig_r);
-  if (rc) return rc;
-  // merge lead status into st_l (left failure wins, as the reference evaluates the left lead first)
-  rc = negf_merge_status(stream, p->n_inst, st_l, st_r);
-  if (rc) return rc;
-  // the device sweep runs in FP64 in both precisions (see negf_transmission_batched)
-  rc = negf_rgf_g1n_batched(stream, NEGF_FP64, p->n, p->n_layers, p->n_inst, p->onsite,
-                            p->coupling, p->set_of, p->omega, p->eta_dev, sig_l, sig_r, st_l, g1n,
-                            p->status);
-  if (rc) return rc;
-  return negf_transmission_trace_batched(stream, p->n, p->n_inst, sig_l, sig_r, g1n, p->status,
-                                         p->t_out);
+  // self-energies, the device sweep (FP64 in both precisions) and the trace; the lead
+  // status is merged left first, as the reference evaluates the left lead first
+  return negf_transmission_sweep_batched(
+      stream, p->n, p->n_layers, p->n_inst, p->onsite, p->coupling, p->set_of, p->omega,
+      p->eta_dev, p->n_lead_l, p->n_c_l, p->v_l, g_l, p->idx_l, st_l, p->n_lead_r, p->n_c_r, p->v_r,
+      g_r, p->idx_r, st_r, sig_l, sig_r, g1n, p->status, p->t_out);
 }
 
 // ---------------------------------------------------------------------------

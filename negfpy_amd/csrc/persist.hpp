@@ -292,54 +292,11 @@ __device__ int pl_sr(const SrArgs& a, int inst, bool active, L& s, int tid, PlSh
   return status;
 }
 
-// Sigma = V g V^H embedded in the end layer (skipped unless live).
+// Sigma = V g V^H embedded in the end layer (skipped unless live): the fused sweep's
+// arithmetic (sigma_mm), so both launches give identical self-energies.
 template <int NMAX, class L>
 __device__ void pl_sigma(const SigmaArgs& a, int inst, bool live, L& s, int tid) {
-  using T = double;
-  constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
-  constexpr int LD = L::LD;
-  const int nl = a.n_lead, nc = a.n_c, nd = a.n_dev;
-  __syncthreads();  // the previous stage's last reads of these matrices are done
-  if (live) s.zero(tid);
-  __syncthreads();
-  int* pos = s.gj->pv;  // inverse contact map: device-layer index -> contact row
-  if (live) {
-    gload<T, NT>(a.v + (size_t)a.set_of[inst] * nc * nl * 2, nc, nl, s.re[0], s.im[0], LD, tid);
-    gload<T, NT>(a.g + (size_t)inst * nl * nl * 2, nl, nl, s.re[1], s.im[1], LD, tid);
-    for (int i = tid; i < nd; i += NT) pos[i] = -1;
-  }
-  __syncthreads();
-  if (live)
-    for (int c = tid; c < nc; c += NT) pos[a.idx[c]] = c;
-  __syncthreads();
-  {
-    const Tile<TM> t(nc, nl, tid);
-    T xr[TM][TM], xi[TM][TM];
-    tzero<T, TM>(xr, xi);
-    if (live && t.own)
-      mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[1], s.im[1], LD, nl, t.i0, t.j0, xr, xi);
-    if (live) tstore<T, TM>(s.re[2], s.im[2], LD, nc, nl, t, xr, xi);
-  }
-  __syncthreads();
-  {
-    const Tile<TM> t(nc, nc, tid);
-    T xr[TM][TM], xi[TM][TM];
-    tzero<T, TM>(xr, xi);
-    if (live && t.own)
-      mac<T, TM, OPN, OPH>(s.re[2], s.im[2], s.re[0], s.im[0], LD, nl, t.i0, t.j0, xr, xi);
-    __syncthreads();
-    if (live) tstore<T, TM>(s.re[1], s.im[1], LD, nc, nc, t, xr, xi);
-  }
-  __syncthreads();
-  if (live) {
-    double2* out = reinterpret_cast<double2*>(a.sigma_out + (size_t)inst * nd * nd * 2);
-    for (int e = tid; e < nd * nd; e += NT) {
-      const int i = e / nd, j = e - (e / nd) * nd;
-      const int pi = pos[i], pj = pos[j];
-      out[e] = (pi >= 0 && pj >= 0) ? make_double2(s.re[1][pi * LD + pj], s.im[1][pi * LD + pj])
-                                    : make_double2(0.0, 0.0);
-    }
-  }
+  sigma_mm<NMAX>(a, inst, a.n_dev, s, tid, live);
 }
 
 // Recursive sweep for G_1N; returns the device status (st_in when nonzero).
@@ -453,51 +410,38 @@ __device__ int pl_rgf(const RgfArgs& a, int inst, bool active, int st_in, L& s, 
 }
 
 // T = Re Tr[Gamma_L G_1N Gamma_R G_1N^H] (NaN for a failed instance); returns it to
-// every thread of the sub-block.
+// every thread of the sub-block.  G_1N comes back from global memory into LDS matrix [1]
+// and the registers of the matrix-core ownership; the product and sum are the fused
+// sweep's (trace_mm), so both launches give identical T.
 template <int NMAX, class L>
 __device__ double pl_trace(const TraceArgs& a, int inst, bool active, int status, L& s, int tid) {
   using T = double;
-  constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
+  constexpr int NT = Cfg<NMAX>::NT;
   constexpr int LD = L::LD;
+  using M = MOwn<NMAX, NT>;
+  constexpr int TPW = M::TPW;
   const bool live = active && status == ST_OK;
   if (active && !live && tid == 0) a.t_out[inst] = NAN;
   const int n = a.n;
-  const int nn = n * n;
   __syncthreads();  // the previous stage's last reads of these matrices are done
   if (live) s.zero(tid);
   __syncthreads();
-  if (live) {
-    const double2* sl = reinterpret_cast<const double2*>(a.sigma_l + (size_t)inst * nn * 2);
-    const double2* sr = reinterpret_cast<const double2*>(a.sigma_r + (size_t)inst * nn * 2);
-    for (int e = tid; e < nn; e += NT) {
-      const int i = e / n, j = e - (e / n) * n;
-      const double2 a1 = sl[i * n + j], a2 = sl[j * n + i];
-      s.re[0][i * LD + j] = -(a1.y + a2.y);
-      s.im[0][i * LD + j] = a1.x - a2.x;
-      const double2 b1 = sr[i * n + j], b2 = sr[j * n + i];
-      s.re[1][i * LD + j] = -(b1.y + b2.y);
-      s.im[1][i * LD + j] = b1.x - b2.x;
-    }
-    gload<T, NT>(a.g1n + (size_t)inst * nn * 2, n, n, s.re[2], s.im[2], LD, tid);
-  }
+  if (live) gload<T, NT>(a.g1n + (size_t)inst * n * n * 2, n, n, s.re[1], s.im[1], LD, tid);
   __syncthreads();
-  const Tile<TM> t(n, n, tid);
-  T pr[TM][TM], pi[TM][TM], qr[TM][TM], qi[TM][TM];
-  tzero<T, TM>(pr, pi);
-  tzero<T, TM>(qr, qi);
-  double acc = 0.0;
-  if (live && t.own) {
-    mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[2], s.im[2], LD, n, t.i0, t.j0, pr, pi);
-    mac<T, TM, OPC, OPT>(s.re[2], s.im[2], s.re[1], s.im[1], LD, n, t.i0, t.j0, qr, qi);
+  const M o(tid);
+  T gr[TPW], gi[TPW];
+  mzero<TPW>(gr, gi);
+  if (live) {
 #pragma unroll
-    for (int x = 0; x < TM; ++x)
-#pragma unroll
-      for (int y = 0; y < TM; ++y) {
-        const int i = t.i0 + x, j = t.j0 + y;
-        if (i < n && j < n) acc += pr[x][y] * qr[x][y] - pi[x][y] * qi[x][y];
+    for (int q = 0; q < TPW; ++q) {
+      const int i = o.row(q), j = o.col(q);
+      if (i < n && j < n) {
+        gr[q] = s.re[1][i * LD + j];
+        gi[q] = s.im[1][i * LD + j];
       }
+    }
   }
-  const double tot = block_sum<NT>(acc, s.sred, tid);
+  const double tot = trace_mm<NMAX>(a.sigma_l, a.sigma_r, inst, n, s, gr, gi, tid, live);
   if (live && tid == 0) a.t_out[inst] = tot;
   return live ? tot : NAN;
 }
