@@ -124,6 +124,19 @@ int negf_kernel_family(int precision, int n);
 int negf_set_matrix_core_products(int mode);
 
 /*
+ * Per-size product form of the FP64 workgroup decimations and sweeps (blocks of
+ * 1..48 DOF), for the autotuner: NEGF_PRODUCTS_MATRIX_CORE (v_mfma_f64_4x4x4f64 tiles,
+ * four per 8 x 8 tile-group of a wave) or NEGF_PRODUCTS_VECTOR (a TM x TM register tile
+ * of FMA loops per thread), 0 for the default (matrix cores).  Applies while
+ * negf_set_matrix_core_products is 1.  negf_product_form returns the form n-DOF blocks
+ * use now (0 outside 1..48 DOF).
+ * Process-wide, thread-safe.  No counterpart in the reference.
+ */
+enum { NEGF_PRODUCTS_MATRIX_CORE = 1, NEGF_PRODUCTS_VECTOR = 2 };
+int negf_set_product_form(int n, int form);
+int negf_product_form(int n);
+
+/*
  * Surface Green's function of semi-infinite leads by Lopez-Sancho decimation.
  * Replaces surface_gf_sancho_rubio, src/negfpy/core/surface_gf.py:18-81
  * (and the per-k loop of lead_surface_dos*, src/negfpy/core/dos.py:31-251).

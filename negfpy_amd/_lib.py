@@ -26,6 +26,7 @@ MIXED = 1
 SURFACE_DECIMATION = 0
 SURFACE_EIGEN = 1
 FAMILY_WORKGROUP, FAMILY_WAVE = 1, 2   # NEGF_FAMILY_*
+PRODUCTS_MATRIX_CORE, PRODUCTS_VECTOR = 1, 2  # NEGF_PRODUCTS_*
 LEADS_INDEPENDENT = 0   # NEGF_LEADS_*: negf_transmission_problem.share_leads
 LEADS_SHARED = 1
 LEADS_MIRRORED = 2
@@ -84,6 +85,8 @@ _SIGS = {
     "negf_set_matrix_core_products": ([_c_int], _c_int),
     "negf_set_kernel_family": ([_c_int, _c_int, _c_int], _c_int),
     "negf_kernel_family": ([_c_int, _c_int], _c_int),
+    "negf_set_product_form": ([_c_int, _c_int], _c_int),
+    "negf_product_form": ([_c_int], _c_int),
     "negf_sancho_rubio_batched": (
         [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_dbl, _c_int, _c_int, _vp, _vp, _vp, _vp],
         _c_int),

@@ -8,12 +8,16 @@ runs in any class >= n; which one is fastest depends on how the inversion's
 per-step latency (rows per lane) trades against resident instances per CU.  Blocks of
 up to 32 DOF can also run one instance per wavefront ("wave": 64-lane workgroups,
 products on MFMA tiles, a register-resident wave inversion), which trades the
-workgroup's barrier-bound step for more resident instances.
-``tune_block_class`` times every candidate -- the classes >= n and, up to 32 DOF, the
-wave family -- on a synthetic batch of n-DOF instances (HIP events on the solve's
-stream, best of ``repeats``), records the fastest in the library's process-wide
-tables (``negf_set_block_class``, ``negf_set_kernel_family``) and returns the timings.
-Results are cached per (precision, n) for the process.
+workgroup's barrier-bound step for more resident instances.  In FP64 the block
+products of a class (8..48) have two tile shapes: v_mfma_f64_4x4x4f64 tiles on the
+matrix cores (four per 8 x 8 tile-group of a wave) or a TM x TM register tile of FMA
+loops per thread on the vector ALUs, the candidate ``(class, VECTOR)``.
+``tune_block_class`` times every candidate -- the classes >= n in each product form and,
+up to 32 DOF, the wave family -- on a synthetic batch of n-DOF instances (HIP events on
+the solve's stream, best of ``repeats``), records the fastest in the library's
+process-wide tables (``negf_set_block_class``, ``negf_set_kernel_family``,
+``negf_set_product_form``) and returns the timings.  Results are cached per
+(precision, n) for the process.
 """
 
 from __future__ import annotations
@@ -29,21 +33,28 @@ from . import _engine
 from .surface_gf import pack_lead
 from .types import LeadBlocks
 
-__all__ = ["CLASSES", "WAVE", "candidate_classes", "tune_block_class", "block_class",
-           "kernel_family", "reset_block_class"]
+__all__ = ["CLASSES", "WAVE", "VECTOR", "candidate_classes", "tune_block_class", "block_class",
+           "kernel_family", "product_form", "reset_block_class"]
 
 CLASSES = (8, 16, 24, 32, 48, 64, 96)
 WAVE = "wave"          # the one-wavefront family, a candidate for blocks up to 32 DOF
 WAVE_MAX = 32
+VECTOR = "vector"      # (class, VECTOR): the class with its products on the vector ALUs
+FORM_MAX = 48          # FP64 classes with both product forms
 _lock = threading.Lock()
 _results: dict = {}
 
 
-def candidate_classes(n: int, precision: str = "fp64", *, wave: bool = True) -> list:
-    """Workgroup classes that hold n-DOF blocks, then ``WAVE`` for n <= 32."""
+def candidate_classes(n: int, precision: str = "fp64", *, wave: bool = True,
+                      vector: bool = True) -> list:
+    """Workgroup classes that hold n-DOF blocks (matrix-core products in FP64), in FP64
+    then ``(class, VECTOR)`` for the same classes with vector-ALU products, then ``WAVE``
+    for n <= 32."""
     if precision == "fp64":
         # the pooled 96 class only for blocks the LDS classes cannot hold
         out = [c for c in (8, 16, 24, 32, 48) if n <= c] or ([96] if n <= 96 else [])
+        if vector and 1 <= n <= FORM_MAX:
+            out += [(c, VECTOR) for c in out]
     else:
         out = [c for c in (8, 16, 24, 32, 48, 64) if n <= c]
     if wave and 1 <= n <= WAVE_MAX and out:
@@ -57,6 +68,15 @@ def kernel_family(n: int, precision: str = "fp64") -> str:
     return WAVE if f == _lib.FAMILY_WAVE else "workgroup"
 
 
+def product_form(n: int) -> str:
+    """"matrix_core" or "vector": the product form of FP64 n-DOF workgroup kernels now
+    (n <= 48; larger blocks always use the matrix cores)."""
+    if not 1 <= n <= FORM_MAX:
+        return "matrix_core"
+    f = int(_lib.load().negf_product_form(int(n)))
+    return VECTOR if f == _lib.PRODUCTS_VECTOR else "matrix_core"
+
+
 def block_class(n: int, precision: str = "fp64") -> int:
     """Class currently used for n-DOF blocks."""
     return int(_lib.load().negf_block_class(_engine.precision_code(precision), int(n)))
@@ -68,6 +88,8 @@ def reset_block_class(n: int, precision: str = "fp64") -> None:
     _lib.check(lib.negf_set_block_class(code, int(n), 0))
     if 1 <= n <= WAVE_MAX:
         _lib.check(lib.negf_set_kernel_family(code, int(n), 0))
+    if precision == "fp64" and 1 <= n <= FORM_MAX:
+        _lib.check(lib.negf_set_product_form(int(n), 0))
     with _lock:
         _results.pop((precision, int(n)), None)
 
@@ -89,12 +111,12 @@ def _synthetic_sets(n: int, n_layers: int):
 def tune_block_class(n: int, precision: str = "fp64", *, n_layers: int = 8, n_inst: int = 2048,
                      repeats: int = 3, candidates=None, apply: bool = True,
                      refresh: bool = False) -> dict:
-    """Time each candidate (class, or ``WAVE``) for n-DOF blocks and (if ``apply``)
-    select the fastest.  A class candidate runs with the workgroup family, ``WAVE`` with
-    the one-wavefront family for the decimations and sweeps (the other stages keep the
-    default class).
+    """Time each candidate (class, ``(class, VECTOR)`` or ``WAVE``) for n-DOF blocks and
+    (if ``apply``) select the fastest.  A class candidate runs with the workgroup family
+    (and, in FP64, matrix-core or vector-ALU products), ``WAVE`` with the one-wavefront
+    family for the decimations and sweeps (the other stages keep the default class).
 
-    Returns {"best": class or "wave", "timings_ms": {candidate: ms per batch},
+    Returns {"best": candidate, "timings_ms": {candidate: ms per batch},
     "n_inst": ..., "n_layers": ...}.
     """
     key = (precision, int(n))
@@ -121,19 +143,30 @@ def tune_block_class(n: int, precision: str = "fp64", *, n_layers: int = 8, n_in
     tables = (_lib.FP64,) if precision == "fp64" else (_lib.FP64, _lib.MIXED)
     prev_fam = {t: int(lib.negf_kernel_family(t, int(n))) for t in tables} if fam_ok else {}
 
+    form_ok = 1 <= n <= FORM_MAX
+    prev_form = int(lib.negf_product_form(int(n))) if form_ok else 0
+
     def family(f):
         if fam_ok:
             for t in tables:
                 _lib.check(lib.negf_set_kernel_family(t, int(n), f))
 
+    def select(c):
+        if c == WAVE:
+            _lib.check(lib.negf_set_block_class(code, int(n), 0))
+            family(_lib.FAMILY_WAVE)
+            form = 0
+        else:
+            vec = isinstance(c, tuple)
+            _lib.check(lib.negf_set_block_class(code, int(n), int(c[0] if vec else c)))
+            family(_lib.FAMILY_WORKGROUP)
+            form = _lib.PRODUCTS_VECTOR if vec else _lib.PRODUCTS_MATRIX_CORE
+        if form_ok:
+            _lib.check(lib.negf_set_product_form(int(n), form))
+
     try:
         for c in cands:
-            if c == WAVE:
-                _lib.check(lib.negf_set_block_class(code, int(n), 0))
-                family(_lib.FAMILY_WAVE)
-            else:
-                _lib.check(lib.negf_set_block_class(code, int(n), int(c)))
-                family(_lib.FAMILY_WORKGROUP)
+            select(c)
             solver.run(set_of, omega, eta, out=out)
             best = float("inf")
             for _ in range(max(int(repeats), 1)):
@@ -144,19 +177,16 @@ def tune_block_class(n: int, precision: str = "fp64", *, n_layers: int = 8, n_in
                 e1.record()
                 e1.synchronize()
                 best = min(best, float(e0.elapsed_time(e1)))
-            timings[c if c == WAVE else int(c)] = best
+            timings[c if (c == WAVE or isinstance(c, tuple)) else int(c)] = best
     finally:
         _lib.check(lib.negf_set_block_class(code, int(n), previous))
         for t, f in prev_fam.items():
             _lib.check(lib.negf_set_kernel_family(t, int(n), f))
+        if form_ok:
+            _lib.check(lib.negf_set_product_form(int(n), prev_form))
     choice = min(timings, key=timings.get)
     if apply:
-        if choice == WAVE:
-            _lib.check(lib.negf_set_block_class(code, int(n), 0))
-            family(_lib.FAMILY_WAVE)
-        else:
-            _lib.check(lib.negf_set_block_class(code, int(n), choice))
-            family(_lib.FAMILY_WORKGROUP)
+        select(choice)
     res = {"best": choice, "timings_ms": timings, "n_inst": int(n_inst), "n_layers": int(n_layers)}
     with _lock:
         _results[key] = res

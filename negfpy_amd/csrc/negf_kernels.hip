@@ -2374,9 +2374,17 @@ static bool use_wave(int n, bool fp64, bool sweep) {
   return f == NEGF_FAMILY_WAVE;
 }
 // Matrix-core products (mm.hpp) in the FP64 workgroup decimation and sweep of
-// classes 8..48: mode 1 (default) on, 0 off (negf_set_matrix_core_products).
+// classes 8..48: mode 1 (default) on unless a per-size choice (negf_set_product_form,
+// the autotuner's) asks for the vector-ALU tiles, mode 0 off
+// (negf_set_matrix_core_products).
+constexpr int FORM_MAX = 48;
 static std::atomic<int> g_mm_products{1};
-static bool use_mm(int nmax) { return g_mm_products.load() == 1 && nmax >= 8 && nmax <= 48; }
+static std::atomic<int> g_product_form[FORM_MAX + 1];
+static bool use_mm(int nmax, int n) {
+  if (g_mm_products.load() != 1 || nmax < 8 || nmax > 48) return false;
+  const int f = (n >= 1 && n <= FORM_MAX) ? g_product_form[n].load(std::memory_order_relaxed) : 0;
+  return f != NEGF_PRODUCTS_VECTOR;
+}
 // rows per lane of the wave inversion: n <= 8 R
 static int wave_rows(int n) { return n <= 16 ? 2 : (n <= 24 ? 3 : 4); }
 
@@ -2495,7 +2503,7 @@ static int run_sr(hipStream_t st, int nmax, const SrArgs& a) {
   if (MIRROR && !sr_mirror_class(nmax))
     return fail(NEGF_ERR_ARG, "no mirrored decimation for block class %d", nmax);
   if constexpr (sizeof(T) == 8) {
-    if (use_mm(nmax)) {
+    if (use_mm(nmax, a.n)) {
 #define SRM_CASE(NM)                                                                        \
   case NM: {                                                                                \
     constexpr int NT = Cfg<NM>::NT;                                                         \
@@ -2606,7 +2614,7 @@ static int run_rgf(hipStream_t st, int nmax, const RgfArgs& a) {
     }
   }
   if constexpr (sizeof(T) == 8) {
-    if (use_mm(nmax)) {
+    if (use_mm(nmax, a.n)) {
 #define RGM_CASE(NM)                                                                       \
   case NM: {                                                                               \
     constexpr int NT = Cfg<NM>::NT;                                                        \
@@ -2756,6 +2764,23 @@ int negf_set_kernel_family(int precision, int n, int family) {
 int negf_kernel_family(int precision, int n) {
   if (precision != NEGF_FP64 && precision != NEGF_MIXED) return 0;
   return use_wave(n, precision == NEGF_FP64, false) ? NEGF_FAMILY_WAVE : NEGF_FAMILY_WORKGROUP;
+}
+
+int negf_set_product_form(int n, int form) {
+  if (n < 1 || n > FORM_MAX)
+    return fail(NEGF_ERR_ARG, "product forms are chosen for blocks of 1..%d DOF (got %d)",
+                FORM_MAX, n);
+  if (form != 0 && form != NEGF_PRODUCTS_MATRIX_CORE && form != NEGF_PRODUCTS_VECTOR)
+    return fail(NEGF_ERR_ARG, "product form %d is not 0, %d or %d", form,
+                NEGF_PRODUCTS_MATRIX_CORE, NEGF_PRODUCTS_VECTOR);
+  g_product_form[n].store(form, std::memory_order_relaxed);
+  return NEGF_OK;
+}
+
+int negf_product_form(int n) {
+  if (n < 1 || n > FORM_MAX) return 0;
+  const int nmax = pick_nmax(n, true);
+  return use_mm(nmax, n) ? NEGF_PRODUCTS_MATRIX_CORE : NEGF_PRODUCTS_VECTOR;
 }
 
 int negf_set_matrix_core_products(int mode) {
@@ -3038,7 +3063,7 @@ int negf_transmission_sweep_batched(
   int m = n > n_lead_l ? n : n_lead_l;
   m = m > n_lead_r ? m : n_lead_r;
   const int nmax = pick_nmax(m, true);
-  const bool fused = nmax && !use_wave(n, true, true) && use_mm(nmax);
+  const bool fused = nmax && !use_wave(n, true, true) && use_mm(nmax, n);
   if (fused) {
     RgfArgs a{n, n_layers, n_inst, onsite, coupling, set_of, omega, eta_dev,
               sigma_l, sigma_r, nullptr, g1n_out, status, 0};

@@ -123,6 +123,31 @@ def test_matrix_core_product_switch():
     assert lib.negf_set_matrix_core_products(prev) == 1
 
 
+def test_product_form_table():
+    """negf_set_product_form stores a per-size product form (0 default, 1 matrix cores,
+    2 vector ALUs) for FP64 blocks of 1..48 DOF; negf_product_form reports the one in
+    force (0 outside that range); matrix-core mode 0 overrides the table (host-only)."""
+    lib = _lib.load_symbols_only()
+    prev = lib.negf_set_matrix_core_products(1)
+    try:
+        for n in (1, 6, 24, 48):
+            assert lib.negf_product_form(n) == _lib.PRODUCTS_MATRIX_CORE
+        assert lib.negf_product_form(0) == lib.negf_product_form(60) == 0
+        assert lib.negf_set_product_form(24, _lib.PRODUCTS_VECTOR) == 0
+        assert lib.negf_product_form(24) == _lib.PRODUCTS_VECTOR
+        assert lib.negf_product_form(23) == _lib.PRODUCTS_MATRIX_CORE
+        assert lib.negf_set_product_form(24, _lib.PRODUCTS_MATRIX_CORE) == 0
+        assert lib.negf_product_form(24) == _lib.PRODUCTS_MATRIX_CORE
+        lib.negf_set_matrix_core_products(0)
+        assert lib.negf_product_form(24) == _lib.PRODUCTS_VECTOR
+        for args, word in (((49, 1), b"1..48"), ((0, 1), b"1..48"), ((24, 3), b"form")):
+            assert lib.negf_set_product_form(*args) == 1
+            assert word in lib.negf_last_error()
+    finally:
+        lib.negf_set_product_form(24, 0)
+        lib.negf_set_matrix_core_products(prev)
+
+
 def test_kernel_family_table():
     """negf_set_kernel_family stores a per-size family (0 default, 1 workgroup, 2 wave)
     for blocks of 1..32 DOF and negf_kernel_family reports the one in force in mode 1;

@@ -6,10 +6,12 @@ import pytest
 
 from negfpy_amd.core import LeadBlocks, transmission_spectrum
 from negfpy_amd.core.autotune import (
+    VECTOR,
     WAVE,
     block_class,
     candidate_classes,
     kernel_family,
+    product_form,
     reset_block_class,
     tune_block_class,
 )
@@ -32,12 +34,17 @@ def restore_classes():
 def _applied(n, prec, best):
     if best == WAVE:
         return kernel_family(n, prec) == WAVE
-    return kernel_family(n, prec) == "workgroup" and block_class(n, prec) == best
+    if isinstance(best, tuple):
+        return (kernel_family(n, prec) == "workgroup" and block_class(n, prec) == best[0]
+                and product_form(n) == VECTOR)
+    return (kernel_family(n, prec) == "workgroup" and block_class(n, prec) == best
+            and (prec != "fp64" or product_form(n) == "matrix_core"))
 
 
 def test_tuner_times_every_candidate_and_applies_the_fastest(gpu, restore_classes):
     res = tune_block_class(12, "fp64", n_layers=4, n_inst=512, repeats=2)
-    assert set(res["timings_ms"]) == set(candidate_classes(12, "fp64")) == {16, 24, 32, 48, WAVE}
+    assert set(res["timings_ms"]) == set(candidate_classes(12, "fp64")) == {
+        16, 24, 32, 48, (16, VECTOR), (24, VECTOR), (32, VECTOR), (48, VECTOR), WAVE}
     assert all(t > 0 for t in res["timings_ms"].values())
     assert res["best"] == min(res["timings_ms"], key=res["timings_ms"].get)
     assert _applied(12, "fp64", res["best"])
@@ -49,6 +56,7 @@ def test_tuner_times_every_candidate_and_applies_the_fastest(gpu, restore_classe
           f"{res_m['timings_ms']} -> {res_m['best']}", flush=True)
     reset_block_class(12, "fp64")
     assert block_class(12, "fp64") == 16 and kernel_family(12, "fp64") == "workgroup"  # default
+    assert product_form(12) == "matrix_core"
     assert 96 not in candidate_classes(40, "fp64") and WAVE not in candidate_classes(40, "fp64")
 
 
@@ -72,6 +80,15 @@ def test_results_do_not_depend_on_the_class(gpu, restore_classes):
         assert_parity(got[c], refs)
     for c in (24, 32, 48):
         np.testing.assert_allclose(got[c], got[16], rtol=1e-11, atol=1e-14)
+    # the two product forms of each class
+    for c in (16, 24, 32, 48):
+        assert lib.negf_set_block_class(0, 12, c) == 0
+        assert lib.negf_set_product_form(12, _lib.PRODUCTS_VECTOR) == 0
+        t_vec = transmission_spectrum(w, dev, lead, lead, eta=1e-6)
+        assert lib.negf_set_product_form(12, 0) == 0
+        assert_parity(t_vec, refs)
+        # other rounding (another product order), same T to the oracle's bar
+        np.testing.assert_allclose(t_vec, got[c], rtol=1e-10, atol=1e-14)
     # the two kernel families
     assert lib.negf_set_block_class(0, 12, 0) == 0
     for fam in (_lib.FAMILY_WORKGROUP, _lib.FAMILY_WAVE):

@@ -157,17 +157,21 @@ def test_heat_current_kspace_drivers_match_oracle_composition(gpu, dim):
 def test_autotuned_class_matches_oracle(gpu, n):
     """Run the tuner at n = 24 and 48 (FP64), then a ribbon of that size with the class
     it recorded, against the oracle; the table is restored afterwards."""
-    from negfpy_amd.core.autotune import (WAVE, block_class, kernel_family, reset_block_class,
-                                          tune_block_class)
+    from negfpy_amd.core.autotune import (VECTOR, WAVE, block_class, kernel_family,
+                                          product_form, reset_block_class, tune_block_class)
     from negfpy_amd.workloads import ribbon_params, omega_max
     from negfpy_amd.models.multidof import multidof_device, multidof_lead_blocks
 
     try:
         res = tune_block_class(n, "fp64", n_layers=3, n_inst=256, repeats=1)
-        if res["best"] == WAVE:
+        best = res["best"]
+        if best == WAVE:
             assert kernel_family(n, "fp64") == WAVE
+        elif isinstance(best, tuple):
+            assert block_class(n, "fp64") == best[0] and product_form(n) == VECTOR
         else:
-            assert block_class(n, "fp64") == res["best"]
+            assert block_class(n, "fp64") == best and product_form(n) == "matrix_core"
+        print(f"n={n}: {res['timings_ms']} -> {best}", flush=True)
         params = ribbon_params(9, n // 3)
         lead = multidof_lead_blocks(params)
         rng = np.random.default_rng(n)
