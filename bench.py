@@ -230,7 +230,8 @@ def uneven_batch_3d(steps: int, warmup: int) -> dict:
 def adjoint_batch(steps: int, warmup: int) -> dict:
     """The adjoint workload (48-DOF mass-mismatch interface between two different
     leads, 64 layers): forward transmission solves and adjoint solves (T and
-    dT/d(every device block)) per second, one k point."""
+    dT/d(every device block)) per second, one k point; also with the gradients of both
+    leads (their three blocks and the device-to-lead couplings)."""
     import torch
 
     from negfpy_amd.core import _engine
@@ -251,7 +252,9 @@ def adjoint_batch(steps: int, warmup: int) -> dict:
     solver = _engine.TransmissionSolver(sets)
     times, res = {}, {}
     for name, fn in (("forward", lambda: solver.run(set_of, omega, eta)),
-                     ("adjoint", lambda: _engine.adjoint(sets, set_of, omega, eta, eta))):
+                     ("adjoint", lambda: _engine.adjoint(sets, set_of, omega, eta, eta)),
+                     ("adjoint_leads", lambda: _engine.adjoint(sets, set_of, omega, eta, eta,
+                                                               lead_grads=True))):
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
@@ -267,6 +270,7 @@ def adjoint_batch(steps: int, warmup: int) -> dict:
             "instances": m, "forward_solves_per_s": round(m / times["forward"], 1),
             "adjoint_solves_per_s": round(m / times["adjoint"], 1),
             "adjoint_over_forward_time": round(times["adjoint"] / times["forward"], 3),
+            "adjoint_with_lead_gradients_solves_per_s": round(m / times["adjoint_leads"], 1),
             "t_adjoint_vs_forward_max_rel": float(np.max(np.abs(ta[ok] - tf[ok])
                                                          / np.maximum(np.abs(tf[ok]), 1e-3)))}
 

@@ -235,7 +235,8 @@ int negf_device_green_batched(void* stream, int n, int n_layers, int n_inst, con
  * to the device blocks it assembles at :53-69.
  *   weight [n_inst] (dL/dT, null: 1); workspace >= negf_adjoint_workspace_bytes;
  *   t_out [n_inst]; grad_onsite [n_inst][L][n][n]; grad_coupling [n_inst][L-1][n][n];
- *   status [n_inst] (lead status passed through, 12/13 on device failure)
+ *   status [n_inst] (lead status passed through, 12/13 on device failure);
+ *   g1n_out [n_inst][n][n] or NULL: G_1N, for negf_lead_adjoint_batched
  */
 size_t negf_adjoint_workspace_bytes(int n, int n_layers, int n_inst);
 int negf_adjoint_batched(void* stream, int n, int n_layers, int n_inst, const double* onsite,
@@ -243,7 +244,33 @@ int negf_adjoint_batched(void* stream, int n, int n_layers, int n_inst, const do
                          const double* eta_dev, const double* sigma_l, const double* sigma_r,
                          const int32_t* status_in, const double* weight, void* workspace,
                          size_t workspace_bytes, double* t_out, double* grad_onsite,
-                         double* grad_coupling, int32_t* status);
+                         double* grad_coupling, int32_t* status, double* g1n_out);
+
+/*
+ * Lead side of the adjoint: weight * dT/d(lead blocks) and weight * dT/dV for one lead
+ * (side 0: left, contact layer 1; 1: right, layer N), from the device adjoint's outputs.
+ * No counterpart in the reference (no gradients); differentiates Sigma = V g V^H
+ * (src/negfpy/core/negf.py:72-97) and the surface GF g^-1 = z - d00 - d01 g d10
+ * (src/negfpy/core/surface_gf.py:18-81) by the implicit-function theorem: the adjoint
+ * Stein equation Z - (g d01)^H Z (d10 g)^H = dT/dg is solved by doubling (at most
+ * maxiter doublings) on the matrix cores.  Blocks of up to 48 DOF (max(n, n_lead)).
+ *   lead  [n_sets][3][n_lead][n_lead]  d00, d01, d10 of this lead (as decimated)
+ *   v     [n_sets][n_c][n_lead], idx [n_c]   its coupling and contact positions
+ *   g     [n_inst][n_lead][n_lead]      its surface GF (the forward's)
+ *   g1n   [n_inst][n][n]                G_1N (negf_adjoint_batched's g1n_out)
+ *   sigma_l, sigma_r [n_inst][n][n]     the embedded self-energies of the forward
+ *   grad_onsite [n_inst][L][n][n]       negf_adjoint_batched's output (same weight)
+ *   status_in [n_inst]                  the adjoint's status; != 0 gives zero gradients
+ *   grad_lead [n_inst][3][n_lead][n_lead], grad_v [n_inst][n_c][n_lead]   written
+ *   status [n_inst]                     0, status_in, 1 (not converged), 2 (non-finite)
+ */
+int negf_lead_adjoint_batched(void* stream, int side, int n, int n_layers, int n_inst, int n_lead,
+                              int n_c, const double* lead, const double* v, const int32_t* idx,
+                              const int32_t* set_of, const double* g, const double* g1n,
+                              const double* sigma_l, const double* sigma_r,
+                              const double* grad_onsite, const double* weight,
+                              const int32_t* status_in, int maxiter, double* grad_lead,
+                              double* grad_v, int32_t* status);
 
 /*
  * T = Re Tr[Gamma_L G_1N Gamma_R G_1N^H], Gamma = i (Sigma - Sigma^H), in FP64.

@@ -533,10 +533,11 @@ class TransmissionGraph:
 
 
 def lead_self_energies(sets: TransportSets, set_of, omega, eta_lead, *, tol=1e-12, maxiter=200,
-                       route=0):
+                       route=0, return_g=False):
     """FP64 decimation of both leads and Sigma = V g V^H embedded in the end layers.
 
-    Returns (Sigma_L, Sigma_R [n_inst, n, n], lead status [n_inst]) on the GPU.
+    Returns (Sigma_L, Sigma_R [n_inst, n, n], lead status [n_inst]) on the GPU, and with
+    ``return_g`` also the surface GFs (g_L, g_R).
     """
     lib = _lib.load()
     n_inst = int(omega.numel())
@@ -546,6 +547,7 @@ def lead_self_energies(sets: TransportSets, set_of, omega, eta_lead, *, tol=1e-1
     st = _lib.stream()
     sig = []
     stats = []
+    gs = []
     for lead, v, idx in ((sets.lead_l, sets.v_l, sets.idx_l), (sets.lead_r, sets.v_r, sets.idx_r)):
         nld = int(lead.shape[2])
         g = torch.empty((n_inst, nld, nld), dtype=c128, device=dev)
@@ -566,7 +568,10 @@ def lead_self_energies(sets: TransportSets, set_of, omega, eta_lead, *, tol=1e-1
             idx.data_ptr(), stt.data_ptr(), sg.data_ptr()))
         sig.append(sg)
         stats.append(stt)
+        gs.append(g)
     lead_status = torch.where(stats[0] != 0, stats[0], stats[1])
+    if return_g:
+        return sig[0], sig[1], lead_status, (gs[0], gs[1])
     return sig[0], sig[1], lead_status
 
 
@@ -602,11 +607,24 @@ class AdjointResult:
     grad_onsite: torch.Tensor    # (n_inst, L, n, n) complex128, weight * dT/dD_ii
     grad_coupling: torch.Tensor  # (n_inst, max(L-1, 1), n, n) complex128, weight * dT/dC_i
     status: torch.Tensor         # (n_inst,) int32
+    # with lead_grads: weight * dT/d(d00, d01, d10) (n_inst, 3, nl, nl) and weight * dT/dV
+    # (n_inst, n_c, nl) of each lead
+    grad_lead_l: torch.Tensor | None = None
+    grad_lead_r: torch.Tensor | None = None
+    grad_v_l: torch.Tensor | None = None
+    grad_v_r: torch.Tensor | None = None
+
+
+# doublings of the lead adjoint's Stein solve: the increments shrink like |lambda|^(2^k),
+# |lambda| the slowest lead mode, which at eta = 1e-8 still leaves ~30 doublings
+LEAD_ADJOINT_MAXITER = 64
 
 
 def adjoint(sets: TransportSets, set_of, omega, eta_lead, eta_dev, weight=None, *, tol=1e-12,
-            maxiter=200, route=0) -> AdjointResult:
-    """T and weight * dT/d(device blocks) of every instance (FP64 adjoint kernel)."""
+            maxiter=200, route=0, lead_grads=False) -> AdjointResult:
+    """T and weight * dT/d(device blocks) of every instance (FP64 adjoint kernel); with
+    ``lead_grads`` also weight * dT/d(lead blocks) and weight * dT/dV of both leads
+    (negf_lead_adjoint_batched).  A lead gradient whose Stein solve fails raises."""
     lib = _lib.load()
     set_of = to_dev_i32(set_of)
     omega = to_dev_f64(omega)
@@ -614,10 +632,12 @@ def adjoint(sets: TransportSets, set_of, omega, eta_lead, eta_dev, weight=None, 
     eta_dev = to_dev_f64(eta_dev)
     n_inst = int(omega.numel())
     n, nl = sets.n, sets.n_layers
-    sig_l, sig_r, lead_status = lead_self_energies(sets, set_of, omega, eta_lead, tol=tol,
-                                                   maxiter=maxiter, route=route)
+    out = lead_self_energies(sets, set_of, omega, eta_lead, tol=tol, maxiter=maxiter, route=route,
+                             return_g=lead_grads)
+    sig_l, sig_r, lead_status = out[:3]
     dev = sets.onsite.device
     w = None if weight is None else to_dev_f64(weight)
+    g1n = torch.empty((n_inst, n, n), dtype=torch.complex128, device=dev) if lead_grads else None
     ws_bytes = int(lib.negf_adjoint_workspace_bytes(n, nl, n_inst))
     ws = torch.empty((max(ws_bytes, 1),), dtype=torch.uint8, device=dev)
     t = torch.empty((n_inst,), dtype=torch.float64, device=dev)
@@ -629,8 +649,35 @@ def adjoint(sets: TransportSets, set_of, omega, eta_lead, eta_dev, weight=None, 
         set_of.data_ptr(), omega.data_ptr(), eta_dev.data_ptr(), sig_l.data_ptr(),
         sig_r.data_ptr(), lead_status.data_ptr(), None if w is None else w.data_ptr(),
         ws.data_ptr(), ws_bytes, t.data_ptr(), g_ons.data_ptr(), g_cpl.data_ptr(),
-        status.data_ptr()))
-    return AdjointResult(t=t, grad_onsite=g_ons, grad_coupling=g_cpl, status=status)
+        status.data_ptr(), None if g1n is None else g1n.data_ptr()))
+    res = AdjointResult(t=t, grad_onsite=g_ons, grad_coupling=g_cpl, status=status)
+    if not lead_grads:
+        return res
+    del ws
+    grads = []
+    for side, lead, v, idx, g in ((0, sets.lead_l, sets.v_l, sets.idx_l, out[3][0]),
+                                  (1, sets.lead_r, sets.v_r, sets.idx_r, out[3][1])):
+        nld, nc = int(lead.shape[2]), int(v.shape[1])
+        gl = torch.empty((n_inst, 3, nld, nld), dtype=torch.complex128, device=dev)
+        gv = torch.empty((n_inst, nc, nld), dtype=torch.complex128, device=dev)
+        st = torch.empty((n_inst,), dtype=torch.int32, device=dev)
+        _lib.check(lib.negf_lead_adjoint_batched(
+            _lib.stream(), side, n, nl, n_inst, nld, nc, lead.data_ptr(), v.data_ptr(),
+            idx.data_ptr(), set_of.data_ptr(), g.data_ptr(), g1n.data_ptr(), sig_l.data_ptr(),
+            sig_r.data_ptr(), g_ons.data_ptr(), None if w is None else w.data_ptr(),
+            status.data_ptr(), LEAD_ADJOINT_MAXITER, gl.data_ptr(), gv.data_ptr(),
+            st.data_ptr()))
+        bad = (st != 0) & (status == 0)
+        if bool(bad.any()):
+            i = int(torch.nonzero(bad)[0])
+            raise RuntimeError(
+                f"lead adjoint ({'left' if side == 0 else 'right'} lead) failed at instance {i} "
+                f"(status {int(st[i])}: the Stein solve for dT/dg did not converge in "
+                f"{LEAD_ADJOINT_MAXITER} doublings or met non-finite values)")
+        grads.append((gl, gv))
+    res.grad_lead_l, res.grad_v_l = grads[0]
+    res.grad_lead_r, res.grad_v_r = grads[1]
+    return res
 
 
 # ---------------------------------------------------------------------------

@@ -8,7 +8,9 @@ Here both are ``torch.autograd.Function``s:
   and coupling blocks.  Forward: the batched FP64 transmission kernels.  Backward:
   the adjoint kernel ``k_adjoint`` (one right- and one left-connected sweep per
   (omega) instance), weighted by the incoming dL/dT and summed over omega on the
-  GPU.  Leads and device-to-lead couplings are constants.
+  GPU.  Leads and device-to-lead couplings are constants here; the k-resolved
+  functions (``transmission_sets_torch``, ``transmission_sum_torch``) also
+  differentiate them (the lead adjoint kernel).
 * ``thermal_conductance_torch`` -- kappa(T) of a GPU spectrum (the k_conductance
   kernel); backward multiplies by the trapezoid/Bose weights of
   ``k_conductance_weights``.
@@ -160,13 +162,18 @@ def transmission_torch(omega, onsite, coupling, lead_left, lead_right, *, eta: f
     return _TransmissionFn.apply(ons, cpl, _Static(kp, w, eta, eta_dev))
 
 
-def _adjoint_bytes_per_instance(sets, nl: int, n: int) -> int:
+def _adjoint_bytes_per_instance(sets, nl: int, n: int, leads: bool = False) -> int:
     """Device bytes one instance of an adjoint chunk holds: the kernel's (2L - 1) stored
     sweep blocks, as many per-instance gradient blocks, and the lead surface GFs and
-    self-energies of _engine.adjoint (complex128)."""
+    self-energies of _engine.adjoint (complex128); with lead gradients also G_1N and the
+    per-instance lead and coupling gradients."""
     n_l = int(sets.lead_l.shape[2])
     n_r = int(sets.lead_r.shape[2])
-    return (2 * (2 * nl - 1) * n * n + n_l * n_l + n_r * n_r + 2 * n * n) * 16
+    b = 2 * (2 * nl - 1) * n * n + n_l * n_l + n_r * n_r + 2 * n * n
+    if leads:
+        b += n * n + 3 * (n_l * n_l + n_r * n_r) + int(sets.v_l.shape[1]) * n_l \
+            + int(sets.v_r.shape[1]) * n_r
+    return b * 16
 
 
 def _chunk_instances(per_inst: int, n_inst: int, dev) -> int:
@@ -179,10 +186,20 @@ def _chunk_instances(per_inst: int, n_inst: int, dev) -> int:
     return max(1, min(n_inst, budget // per_inst))
 
 
+def _leads_wanted(ctx) -> bool:
+    return any(ctx.needs_input_grad[2:6])
+
+
+def _add_lead_grads(acc, k, res):
+    """Per-set sums of the per-instance lead gradients (real view: float64 atomics)."""
+    for a, g in zip(acc, (res.grad_lead_l, res.grad_lead_r, res.grad_v_l, res.grad_v_r)):
+        torch.view_as_real(a).index_add_(0, k, torch.view_as_real(g))
+
+
 class _SetsTransmissionFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, ons, cpl, static):
-        sets = static.sets(ons, cpl)
+    def forward(ctx, ons, cpl, lead_l, lead_r, v_l, v_r, static):
+        sets = static.sets(ons, cpl, lead_l, lead_r, v_l, v_r)
         res = _engine.TransmissionSolver(sets, precision="fp64").run(
             static.set_of, static.omega, static.eta, static.eta_dev)
         if bool((res.status != 0).any()):
@@ -191,45 +208,50 @@ class _SetsTransmissionFn(torch.autograd.Function):
             for c, v in zip(st, tc):
                 transmission_value(int(c), float(v))  # raises the reference's exception
         ctx.static = static
-        ctx.save_for_backward(ons, cpl)
+        ctx.save_for_backward(ons, cpl, lead_l, lead_r, v_l, v_r)
         return res.t
 
     @staticmethod
     def backward(ctx, grad_t):
-        ons, cpl = ctx.saved_tensors
+        ons, cpl, lead_l, lead_r, v_l, v_r = ctx.saved_tensors
         static = ctx.static
-        sets = static.sets(ons, cpl)
+        sets = static.sets(ons, cpl, lead_l, lead_r, v_l, v_r)
+        leads = _leads_wanted(ctx)
         weight = grad_t.detach().to(torch.float64).contiguous()
         nl, n = int(ons.shape[1]), int(ons.shape[2])
         n_inst = int(static.omega.numel())
-        per_inst = _adjoint_bytes_per_instance(sets, nl, n)
+        per_inst = _adjoint_bytes_per_instance(sets, nl, n, leads)
         chunk = _chunk_instances(per_inst, n_inst, ons.device)
         g_ons = torch.zeros_like(ons)
         g_cpl = torch.zeros((ons.shape[0], max(nl - 1, 1), n, n), dtype=ons.dtype,
                             device=ons.device)
+        g_lead = [torch.zeros_like(t) for t in (lead_l, lead_r, v_l, v_r)] if leads else None
         for a in range(0, n_inst, chunk):
             b = min(n_inst, a + chunk)
             k = static.set_of[a:b].long()
             res = _engine.adjoint(sets, static.set_of[a:b], static.omega[a:b], static.eta[a:b],
-                                  static.eta_dev[a:b], weight=weight[a:b])
+                                  static.eta_dev[a:b], weight=weight[a:b], lead_grads=leads)
             # per-set sums of the per-instance gradients (real view: float64 atomics)
             torch.view_as_real(g_ons).index_add_(0, k, torch.view_as_real(res.grad_onsite))
             torch.view_as_real(g_cpl).index_add_(0, k, torch.view_as_real(res.grad_coupling))
+            if leads:
+                _add_lead_grads(g_lead, k, res)
             del res
-        return g_ons, (g_cpl[:, :nl - 1] if nl > 1 else None), None
+        gl = g_lead if leads else [None] * 4
+        return (g_ons, (g_cpl[:, :nl - 1] if nl > 1 else None), *gl, None)
+
+
+def _lead_input(t):
+    """A lead block or coupling tensor on the GPU as complex128, keeping its autograd
+    history (numpy arrays become constants)."""
+    return torch.as_tensor(t).to(device=_engine.device(), dtype=torch.complex128)
 
 
 class _SetsStatic:
-    """Leads, couplings and instances of a k-resolved transmission problem."""
+    """Instances of a k-resolved transmission problem (the leads and couplings are
+    autograd inputs of the functions)."""
 
-    def __init__(self, lead_l, lead_r, v_l, v_r, set_of, omega, eta, eta_dev):
-        dev = _engine.device()
-        c128 = torch.complex128
-        self.lead_l = torch.as_tensor(lead_l).to(device=dev, dtype=c128)
-        self.lead_r = torch.as_tensor(lead_r).to(device=dev, dtype=c128)
-        self.v_l = torch.as_tensor(v_l).to(device=dev, dtype=c128)
-        self.v_r = torch.as_tensor(v_r).to(device=dev, dtype=c128)
-        self.share = bool(torch.equal(self.lead_l, self.lead_r))
+    def __init__(self, set_of, omega, eta, eta_dev):
         self.set_of = _engine.to_dev_i32(set_of)
         self.omega = _engine.to_dev_f64(omega)
         m = int(self.omega.numel())
@@ -237,12 +259,15 @@ class _SetsStatic:
         self.eta_dev = _engine.to_dev_f64(
             np.broadcast_to(np.asarray(eta_dev, dtype=float), (m,)).copy())
 
-    def sets(self, ons: torch.Tensor, cpl: torch.Tensor) -> _engine.TransportSets:
+    @staticmethod
+    def sets(ons, cpl, lead_l, lead_r, v_l, v_r) -> _engine.TransportSets:
         n = int(ons.shape[2])
         idx = np.arange(n)
+        lead_l, lead_r = lead_l.detach().contiguous(), lead_r.detach().contiguous()
         return _engine.TransportSets.from_arrays(
-            self.lead_l, self.lead_r, self.v_l, self.v_r, idx, idx, ons.detach(),
-            cpl.detach() if ons.shape[1] > 1 else None, self.share)
+            lead_l, lead_r, v_l.detach().contiguous(), v_r.detach().contiguous(), idx, idx,
+            ons.detach(), cpl.detach() if ons.shape[1] > 1 else None,
+            bool(torch.equal(lead_l, lead_r)))
 
 
 def transmission_sets_torch(omega, set_of, onsite, coupling, lead_left, lead_right, v_left,
@@ -256,9 +281,12 @@ def transmission_sets_torch(omega, set_of, onsite, coupling, lead_left, lead_rig
     per set, each passed to the decimation as given (``surface_gf.py:18-81``), and
     ``v_left`` / ``v_right`` (S, n, n) the device-to-lead couplings of the first and
     last layer (all of its DOFs are contacts); Sigma = V g V^H as in
-    ``negf.py:72-97``.  They are held fixed.  The batched form of ``transmission``
-    (``negf.py:306-357``) over the points of a k mesh, for gradients with respect to
-    force constants that enter every k point's blocks.
+    ``negf.py:72-97``.  Where they are torch tensors that require grad, gradients flow
+    to them too (the lead adjoint: the implicit derivative of the surface GF through
+    its Stein equation, ``negf_lead_adjoint_batched``; blocks up to 48 DOF).  The batched
+    form of ``transmission`` (``negf.py:306-357``) over the points of a k mesh, for
+    gradients with respect to force constants that enter every k point's blocks, the
+    leads' included.
     """
     dev = _engine.device()
     ons = onsite.to(device=dev, dtype=torch.complex128)
@@ -269,19 +297,32 @@ def transmission_sets_torch(omega, set_of, onsite, coupling, lead_left, lead_rig
     if cpl.ndim != 4 or tuple(cpl.shape) != (s, nl - 1, n, n):
         raise ValueError("coupling must have shape (n_sets, n_layers - 1, n, n).")
     eta_dev = float(eta if eta_device is None else eta_device)
-    static = _SetsStatic(lead_left, lead_right, v_left, v_right, set_of, omega, eta, eta_dev)
-    for name, t in (("lead_left", static.lead_l), ("lead_right", static.lead_r)):
+    static = _SetsStatic(set_of, omega, eta, eta_dev)
+    leads = _check_leads(s, n, lead_left, lead_right, v_left, v_right)
+    _check_set_of(static, s)
+    return _SetsTransmissionFn.apply(ons, cpl, *leads, static)
+
+
+def _check_leads(s, n, lead_left, lead_right, v_left, v_right):
+    out = [_lead_input(t) for t in (lead_left, lead_right, v_left, v_right)]
+    for name, t in (("lead_left", out[0]), ("lead_right", out[1])):
         if tuple(t.shape) != (s, 3, n, n):
             raise ValueError(f"{name} must have shape (n_sets, 3, n, n).")
-    for name, t in (("v_left", static.v_l), ("v_right", static.v_r)):
+    for name, t in (("v_left", out[2]), ("v_right", out[3])):
         if tuple(t.shape) != (s, n, n):
             raise ValueError(f"{name} must have shape (n_sets, n, n).")
+    if any(t.requires_grad for t in out) and n > 48:
+        raise NotImplementedError("gradients with respect to the leads take blocks of up to "
+                                  f"48 DOF (got {n}); pass the leads as constants")
+    return out
+
+
+def _check_set_of(static, s):
     k = static.set_of
     if k.numel() != static.omega.numel():
         raise ValueError("set_of needs one entry per omega.")
     if k.numel() and (int(k.min()) < 0 or int(k.max()) >= s):
         raise ValueError("set_of entries must index the sets.")
-    return _SetsTransmissionFn.apply(ons, cpl, static)
 
 
 class _SetsSumFn(torch.autograd.Function):
@@ -289,47 +330,53 @@ class _SetsSumFn(torch.autograd.Function):
     before T is): the forward runs the adjoint kernel and keeps the per-set gradient."""
 
     @staticmethod
-    def forward(ctx, ons, cpl, static, coef):
-        sets = static.sets(ons, cpl)
+    def forward(ctx, ons, cpl, lead_l, lead_r, v_l, v_r, static, coef):
+        sets = static.sets(ons, cpl, lead_l, lead_r, v_l, v_r)
+        leads = _leads_wanted(ctx)
         nl, n = int(ons.shape[1]), int(ons.shape[2])
         n_inst = int(static.omega.numel())
-        per_inst = _adjoint_bytes_per_instance(sets, nl, n)
+        per_inst = _adjoint_bytes_per_instance(sets, nl, n, leads)
         chunk = _chunk_instances(per_inst, n_inst, ons.device)
         g_ons = torch.zeros_like(ons)
         g_cpl = torch.zeros((ons.shape[0], max(nl - 1, 1), n, n), dtype=ons.dtype,
                             device=ons.device)
+        g_lead = [torch.zeros_like(t) for t in (lead_l, lead_r, v_l, v_r)] if leads else []
         t_all = torch.empty((n_inst,), dtype=torch.float64, device=ons.device)
         st_all = torch.empty((n_inst,), dtype=torch.int32, device=ons.device)
         for a in range(0, n_inst, chunk):
             b = min(n_inst, a + chunk)
             k = static.set_of[a:b].long()
             res = _engine.adjoint(sets, static.set_of[a:b], static.omega[a:b], static.eta[a:b],
-                                  static.eta_dev[a:b], weight=coef[a:b])
+                                  static.eta_dev[a:b], weight=coef[a:b], lead_grads=leads)
             t_all[a:b] = res.t
             st_all[a:b] = res.status
             torch.view_as_real(g_ons).index_add_(0, k, torch.view_as_real(res.grad_onsite))
             torch.view_as_real(g_cpl).index_add_(0, k, torch.view_as_real(res.grad_coupling))
+            if leads:
+                _add_lead_grads(g_lead, k, res)
             del res
         if bool((st_all != 0).any()):
             for c, v in zip(st_all.cpu().numpy(), t_all.cpu().numpy()):
                 transmission_value(int(c), float(v))  # raises the reference's exception
-        ctx.save_for_backward(g_ons, g_cpl)
+        ctx.save_for_backward(g_ons, g_cpl, *g_lead)
         ctx.nl = nl
         return (coef * t_all).sum()
 
     @staticmethod
     def backward(ctx, grad):
-        g_ons, g_cpl = ctx.saved_tensors
+        g_ons, g_cpl, *g_lead = ctx.saved_tensors
         nl = ctx.nl
         g = grad.to(g_ons.dtype)
-        return g * g_ons, (g * g_cpl[:, :nl - 1] if nl > 1 else None), None, None
+        gl = [g * t for t in g_lead] if g_lead else [None] * 4
+        return (g * g_ons, (g * g_cpl[:, :nl - 1] if nl > 1 else None), *gl, None, None)
 
 
 def transmission_sum_torch(coef, omega, set_of, onsite, coupling, lead_left, lead_right, v_left,
                            v_right, *, eta: float = 1e-8,
                            eta_device: float | None = None) -> torch.Tensor:
     """sum_i coef[i] T_i over the instances of ``transmission_sets_torch`` (same
-    arguments), differentiable in ``onsite`` and ``coupling`` but not in ``coef``.
+    arguments), differentiable in ``onsite`` and ``coupling`` (and in the leads and
+    couplings where they require grad) but not in ``coef``.
 
     Because coef is known before T, one adjoint pass gives the sum and its gradient
     (the adjoint kernel weighted by coef); there is no separate forward solve.  For
@@ -345,16 +392,13 @@ def transmission_sum_torch(coef, omega, set_of, onsite, coupling, lead_left, lea
     if cpl.ndim != 4 or tuple(cpl.shape) != (s, nl - 1, n, n):
         raise ValueError("coupling must have shape (n_sets, n_layers - 1, n, n).")
     eta_dev = float(eta if eta_device is None else eta_device)
-    static = _SetsStatic(lead_left, lead_right, v_left, v_right, set_of, omega, eta, eta_dev)
+    static = _SetsStatic(set_of, omega, eta, eta_dev)
     c = torch.as_tensor(coef, dtype=torch.float64, device=dev).detach().contiguous()
     if c.shape != static.omega.shape:
         raise ValueError("coef needs one entry per instance.")
-    k = static.set_of
-    if k.numel() != static.omega.numel():
-        raise ValueError("set_of needs one entry per omega.")
-    if k.numel() and (int(k.min()) < 0 or int(k.max()) >= s):
-        raise ValueError("set_of entries must index the sets.")
-    return _SetsSumFn.apply(ons, cpl, static, c)
+    leads = _check_leads(s, n, lead_left, lead_right, v_left, v_right)
+    _check_set_of(static, s)
+    return _SetsSumFn.apply(ons, cpl, *leads, static, c)
 
 
 class _ConductanceFn(torch.autograd.Function):

@@ -1694,6 +1694,7 @@ struct AdjArgs {
   double* grad_onsite;       // [n_inst][L][n][n] complex
   double* grad_coupling;     // [n_inst][L-1][n][n] complex
   int32_t* status;
+  double* g1n_out;           // [n_inst][n][n] G_1N; may be null
 };
 
 template <int NMAX, class L>
@@ -1813,6 +1814,7 @@ __device__ void adj_body(const AdjArgs& a, int inst, L& s, int tid) {
     __threadfence_block();
     __syncthreads();
   }
+  if (a.g1n_out) gstore<T, NT>(a.g1n_out + (size_t)inst * nn * 2, n, s.re[0], s.im[0], LD, tid);
   // 3. Gamma_L (matrix 1), Gamma_R (matrix 2); W = Gamma_R G_1N^H Gamma_L; T = Re Tr[G_1N W]
   for (int e = tid; e < (int)nn; e += NT) {
     const int i = e / n, j = e - (e / n) * n;
@@ -2024,6 +2026,258 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_trace(TraceArgs a) {
   }
   const double tot = block_sum<NT>(acc, s.sred, tid);
   if (tid == 0) a.t_out[inst] = tot;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of the lead side: dT/d(lead blocks d00, d01, d10) and dT/dV (FP64)
+// ---------------------------------------------------------------------------
+//
+// For one lead (left: contact layer 1, right: layer N), with the device adjoint's
+// weighted gradient E of that layer's onsite block, G = G_1N and w the instance weight:
+//   dT/dSigma  = P^T (E - 2 i w X) P,  X = G Gamma_R G^H (left) or G^H Gamma_L G (right),
+// P the contact selection (A = z - D - Sigma: Sigma enters G like the onsite block; the
+// -2 i w X term is T's direct dependence through Gamma = i (Sigma - Sigma^H), X Hermitian).
+// With S = dT/dSigma and Sigma = V g V^H:
+//   dT/dg = V^H S V,   dT/dV = S V g^H + S^H V g.
+// The surface GF solves g^-1 = z - d00 - d01 g d10, so dg - (g d01) dg (d10 g) =
+// g (dd00 + dd01 g d10 + d01 g dd10) g; with Z the solution of the adjoint Stein equation
+//   Z - (g d01)^H Z (d10 g)^H = dT/dg
+// (by doubling: Z += P^H Z Q^H, P <- P^2, Q <- Q^2, P = g d01, Q = d10 g, until the
+// increment is below rounding) and K = g^H Z g^H:
+//   dT/dd00 = K,   dT/dd01 = K d10^H g^H,   dT/dd10 = g^H d01^H K.
+// Gradients in torch's convention (d/dRe + i d/dIm), already multiplied by w like E.
+// The reference has no gradients; this differentiates its Sigma = V g V^H
+// (src/negfpy/core/negf.py:72-97) and surface GF (src/negfpy/core/surface_gf.py:18-81).
+
+struct LeadAdjArgs {
+  int n, n_inst, side;  // side 0: left lead (device layer 1), 1: right lead (layer N)
+  int n_lead, n_c;
+  const double* lead;      // [n_sets][3][n_lead][n_lead]  d00, d01, d10
+  const double* v;         // [n_sets][n_c][n_lead]
+  const int32_t* idx;      // [n_c] contact positions in the end layer
+  const int32_t* set_of;
+  const double* g;         // [n_inst][n_lead][n_lead] surface GF of this lead
+  const double* g1n;       // [n_inst][n][n]
+  const double* sigma_l;   // [n_inst][n][n] embedded self-energies
+  const double* sigma_r;
+  const double* grad_end;  // weighted dT/dD of the contact layer's onsite block (n x n) ..
+  size_t grad_stride;      // .. of instance i at grad_end + i * grad_stride (doubles)
+  const double* weight;    // [n_inst] or null (1)
+  const int32_t* status_in;  // instances with status != 0 get zero gradients
+  int maxiter;             // doublings of the Stein solve
+  double* grad_lead;       // [n_inst][3][n_lead][n_lead]
+  double* grad_v;          // [n_inst][n_c][n_lead]
+  int32_t* status;         // 0, status_in, 1 (Stein doubling not converged), 2 (non-finite)
+};
+
+template <int NMAX>
+__global__ void __launch_bounds__(Cfg<NMAX>::NT) k_lead_adjoint(LeadAdjArgs a) {
+  using T = double;
+  constexpr int NT = Cfg<NMAX>::NT;
+  using L = Lds<T, NMAX, NT, 4>;
+  using M = MOwn<NMAX, NT>;
+  constexpr int LD = L::LD, TPW = M::TPW;
+  extern __shared__ __align__(16) unsigned char smem[];
+  L s(smem);
+  const int tid = threadIdx.x;
+  const int inst = blockIdx.x;
+  const int n = a.n, nl = a.n_lead, nc = a.n_c;
+  const size_t nn = (size_t)n * n, ll = (size_t)nl * nl;
+  double* gl_out = a.grad_lead + (size_t)inst * 3 * ll * 2;
+  double* gv_out = a.grad_v + (size_t)inst * nc * nl * 2;
+  if (a.status_in && a.status_in[inst] != ST_OK) {
+    for (size_t e = tid; e < 3 * ll * 2; e += NT) gl_out[e] = 0.0;
+    for (size_t e = tid; e < (size_t)nc * nl * 2; e += NT) gv_out[e] = 0.0;
+    if (tid == 0) a.status[inst] = a.status_in[inst];
+    return;
+  }
+  const M o(tid);
+  const int ns = (n + 3) / 4, nsl = (nl + 3) / 4, nsc = (nc + 3) / 4;
+  const size_t set = (size_t)a.set_of[inst];
+  const double* d00 = a.lead + set * 3 * ll * 2;
+  const double* d01 = d00 + ll * 2;
+  const double* d10 = d01 + ll * 2;
+  const double* vb = a.v + set * nc * nl * 2;
+  const double* gb = a.g + (size_t)inst * ll * 2;
+  const double w = a.weight ? a.weight[inst] : 1.0;
+  const int ncl = nc > nl ? nc : nl;
+  auto clear = [&]() {
+    __syncthreads();
+    s.zero(tid);
+    __syncthreads();
+  };
+  // owned elements of a matrix-core result, to global memory ([rows][cols], row-major)
+  auto put = [&](double* dst, int rows, int cols, const T (&xr)[TPW], const T (&xi)[TPW]) {
+    double2* d = reinterpret_cast<double2*>(dst);
+#pragma unroll
+    for (int q = 0; q < TPW; ++q) {
+      const int r = o.row(q), c = o.col(q);
+      if (r < rows && c < cols) d[r * cols + c] = make_double2(xr[q], xi[q]);
+    }
+  };
+  T xr[TPW], xi[TPW], yr[TPW], yi[TPW];
+
+  // 1. X = G Gamma_R G^H (left) or G^H Gamma_L G (right): G in [0], Gamma in [1]
+  clear();
+  gload<T, NT>(a.g1n + (size_t)inst * nn * 2, n, n, s.re[0], s.im[0], LD, tid);
+  {
+    const double2* so =
+        reinterpret_cast<const double2*>((a.side == 0 ? a.sigma_r : a.sigma_l) + (size_t)inst * nn * 2);
+    for (int e = tid; e < (int)nn; e += NT) {
+      const int i = e / n, j = e - (e / n) * n;
+      const double2 s1 = so[e], s2 = so[j * n + i];
+      s.re[1][i * LD + j] = -(s1.y + s2.y);
+      s.im[1][i * LD + j] = s1.x - s2.x;
+    }
+  }
+  __syncthreads();
+  mzero<TPW>(xr, xi);
+  if (a.side == 0)
+    mm<NMAX, NT, OPN, OPN>(s.re[0], s.im[0], s.re[1], s.im[1], LD, ns, o, xr, xi);
+  else
+    mm<NMAX, NT, OPH, OPN>(s.re[0], s.im[0], s.re[1], s.im[1], LD, ns, o, xr, xi);
+  mstore<NMAX, NT>(s.re[2], s.im[2], LD, n, o, xr, xi);
+  __syncthreads();
+  mzero<TPW>(xr, xi);
+  if (a.side == 0)
+    mm<NMAX, NT, OPN, OPH>(s.re[2], s.im[2], s.re[0], s.im[0], LD, ns, o, xr, xi);
+  else
+    mm<NMAX, NT, OPN, OPN>(s.re[2], s.im[2], s.re[0], s.im[0], LD, ns, o, xr, xi);
+  // 2. dT/dSigma in the end layer: E - 2 i w X, into [3]; its contact block S into [2]
+  {
+    const double2* ge = reinterpret_cast<const double2*>(a.grad_end + (size_t)inst * a.grad_stride);
+#pragma unroll
+    for (int q = 0; q < TPW; ++q) {
+      const int r = o.row(q), c = o.col(q);
+      if (r < n && c < n) {
+        const double2 e = ge[r * n + c];
+        s.re[3][r * LD + c] = e.x + 2.0 * w * xi[q];
+        s.im[3][r * LD + c] = e.y - 2.0 * w * xr[q];
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < L::MSZ; e += NT) {
+    s.re[0][e] = s.im[0][e] = 0.0;
+    s.re[1][e] = s.im[1][e] = 0.0;
+    s.re[2][e] = s.im[2][e] = 0.0;
+  }
+  __syncthreads();
+  for (int e = tid; e < nc * nc; e += NT) {
+    const int i = e / nc, j = e - (e / nc) * nc;
+    const int pi = a.idx[i], pj = a.idx[j];
+    s.re[2][i * LD + j] = s.re[3][pi * LD + pj];
+    s.im[2][i * LD + j] = s.im[3][pi * LD + pj];
+  }
+  __syncthreads();
+  for (int e = tid; e < L::MSZ; e += NT) s.re[3][e] = s.im[3][e] = 0.0;
+  gload<T, NT>(vb, nc, nl, s.re[0], s.im[0], LD, tid);
+  gload<T, NT>(gb, nl, nl, s.re[1], s.im[1], LD, tid);
+  __syncthreads();
+  // 3. U1 = S V into [3]; U2 = S^H V; dT/dg = V^H U1; dT/dV = U1 g^H + U2 g
+  mzero<TPW>(xr, xi);
+  mm<NMAX, NT, OPN, OPN>(s.re[2], s.im[2], s.re[0], s.im[0], LD, nsc, o, xr, xi);
+  mstore<NMAX, NT>(s.re[3], s.im[3], LD, ncl, o, xr, xi);  // rows >= nc, cols >= nl: zeros
+  mzero<TPW>(yr, yi);
+  mm<NMAX, NT, OPH, OPN>(s.re[2], s.im[2], s.re[0], s.im[0], LD, nsc, o, yr, yi);
+  __syncthreads();
+  T gbr[TPW], gbi[TPW];  // dT/dg, the right side of the Stein equation
+  mzero<TPW>(gbr, gbi);
+  mm<NMAX, NT, OPH, OPN>(s.re[0], s.im[0], s.re[3], s.im[3], LD, nsc, o, gbr, gbi);
+  mzero<TPW>(xr, xi);
+  mm<NMAX, NT, OPN, OPH>(s.re[3], s.im[3], s.re[1], s.im[1], LD, nsl, o, xr, xi);
+  __syncthreads();
+  mstore<NMAX, NT>(s.re[2], s.im[2], LD, ncl, o, yr, yi);  // U2 over S
+  __syncthreads();
+  mm<NMAX, NT, OPN, OPN>(s.re[2], s.im[2], s.re[1], s.im[1], LD, nsl, o, xr, xi);
+  put(gv_out, nc, nl, xr, xi);
+  // 4. Stein equation by doubling: [0] P, [1] Q, [2] Z, [3] P^H Z
+  clear();
+  gload<T, NT>(gb, nl, nl, s.re[3], s.im[3], LD, tid);
+  gload<T, NT>(d01, nl, nl, s.re[0], s.im[0], LD, tid);
+  gload<T, NT>(d10, nl, nl, s.re[1], s.im[1], LD, tid);
+  __syncthreads();
+  mzero<TPW>(xr, xi);
+  mm<NMAX, NT, OPN, OPN>(s.re[3], s.im[3], s.re[0], s.im[0], LD, nsl, o, xr, xi);  // g d01
+  mzero<TPW>(yr, yi);
+  mm<NMAX, NT, OPN, OPN>(s.re[1], s.im[1], s.re[3], s.im[3], LD, nsl, o, yr, yi);  // d10 g
+  __syncthreads();
+  mstore<NMAX, NT>(s.re[0], s.im[0], LD, nl, o, xr, xi);
+  mstore<NMAX, NT>(s.re[1], s.im[1], LD, nl, o, yr, yi);
+  mstore<NMAX, NT>(s.re[2], s.im[2], LD, nl, o, gbr, gbi);
+  __syncthreads();
+  int status = ST_OK;
+  bool conv = false;
+  for (int it = 0; it < a.maxiter && !conv; ++it) {
+    mzero<TPW>(xr, xi);
+    mm<NMAX, NT, OPH, OPN>(s.re[0], s.im[0], s.re[2], s.im[2], LD, nsl, o, xr, xi);  // P^H Z
+    mstore<NMAX, NT>(s.re[3], s.im[3], LD, nl, o, xr, xi);
+    __syncthreads();
+    mzero<TPW>(xr, xi);
+    mm<NMAX, NT, OPN, OPH>(s.re[3], s.im[3], s.re[1], s.im[1], LD, nsl, o, xr, xi);  // .. Q^H
+    T pr[TPW], pi[TPW], qr[TPW], qi[TPW];
+    mzero<TPW>(pr, pi);
+    mm<NMAX, NT, OPN, OPN>(s.re[0], s.im[0], s.re[0], s.im[0], LD, nsl, o, pr, pi);  // P^2
+    mzero<TPW>(qr, qi);
+    mm<NMAX, NT, OPN, OPN>(s.re[1], s.im[1], s.re[1], s.im[1], LD, nsl, o, qr, qi);  // Q^2
+    double dmax = 0.0, zmax = 0.0;
+    bool bad = false;
+#pragma unroll
+    for (int q = 0; q < TPW; ++q) {
+      gbr[q] += xr[q];
+      gbi[q] += xi[q];
+      dmax = fmax(dmax, fabs(xr[q]) + fabs(xi[q]));
+      zmax = fmax(zmax, fabs(gbr[q]) + fabs(gbi[q]));
+      bad = bad || !isfinite(gbr[q]) || !isfinite(gbi[q]);
+    }
+    double dm, zm;
+    bool any_bad, unused;
+    block_max_any<NT>(dmax, bad, s.sred, s.sflag, tid, dm, any_bad);
+    block_max_any<NT>(zmax, false, s.sred, s.sflag, tid, zm, unused);
+    if (any_bad) {
+      status = ST_NONFINITE;
+      break;
+    }
+    conv = dm <= 1e-17 * zm;
+    mstore<NMAX, NT>(s.re[0], s.im[0], LD, nl, o, pr, pi);
+    mstore<NMAX, NT>(s.re[1], s.im[1], LD, nl, o, qr, qi);
+    mstore<NMAX, NT>(s.re[2], s.im[2], LD, nl, o, gbr, gbi);
+    __syncthreads();
+  }
+  if (status == ST_OK && !conv) status = ST_NOCONV;
+  // 5. K = g^H Z g^H; dT/dd00 = K, dT/dd01 = K d10^H g^H, dT/dd10 = g^H d01^H K
+  __syncthreads();
+  gload<T, NT>(gb, nl, nl, s.re[0], s.im[0], LD, tid);
+  __syncthreads();
+  mzero<TPW>(xr, xi);
+  mm<NMAX, NT, OPH, OPN>(s.re[0], s.im[0], s.re[2], s.im[2], LD, nsl, o, xr, xi);  // g^H Z
+  mstore<NMAX, NT>(s.re[3], s.im[3], LD, nl, o, xr, xi);
+  __syncthreads();
+  mzero<TPW>(xr, xi);
+  mm<NMAX, NT, OPN, OPH>(s.re[3], s.im[3], s.re[0], s.im[0], LD, nsl, o, xr, xi);  // K
+  put(gl_out, nl, nl, xr, xi);
+  mstore<NMAX, NT>(s.re[1], s.im[1], LD, nl, o, xr, xi);
+  __syncthreads();
+  gload<T, NT>(d10, nl, nl, s.re[3], s.im[3], LD, tid);
+  __syncthreads();
+  mzero<TPW>(yr, yi);
+  mm<NMAX, NT, OPN, OPH>(s.re[1], s.im[1], s.re[3], s.im[3], LD, nsl, o, yr, yi);  // K d10^H
+  mstore<NMAX, NT>(s.re[2], s.im[2], LD, nl, o, yr, yi);
+  __syncthreads();
+  mzero<TPW>(xr, xi);
+  mm<NMAX, NT, OPN, OPH>(s.re[2], s.im[2], s.re[0], s.im[0], LD, nsl, o, xr, xi);  // .. g^H
+  put(gl_out + ll * 2, nl, nl, xr, xi);
+  __syncthreads();
+  gload<T, NT>(d01, nl, nl, s.re[3], s.im[3], LD, tid);
+  __syncthreads();
+  mzero<TPW>(yr, yi);
+  mm<NMAX, NT, OPH, OPH>(s.re[0], s.im[0], s.re[3], s.im[3], LD, nsl, o, yr, yi);  // g^H d01^H
+  mstore<NMAX, NT>(s.re[2], s.im[2], LD, nl, o, yr, yi);
+  __syncthreads();
+  mzero<TPW>(xr, xi);
+  mm<NMAX, NT, OPN, OPN>(s.re[2], s.im[2], s.re[1], s.im[1], LD, nsl, o, xr, xi);  // .. K
+  put(gl_out + 2 * ll * 2, nl, nl, xr, xi);
+  if (tid == 0) a.status[inst] = status;
 }
 
 // ---------------------------------------------------------------------------
@@ -2956,7 +3210,7 @@ int negf_adjoint_batched(void* stream, int n, int n_layers, int n_inst, const do
                          const double* eta_dev, const double* sigma_l, const double* sigma_r,
                          const int32_t* status_in, const double* weight, void* workspace,
                          size_t workspace_bytes, double* t_out, double* grad_onsite,
-                         double* grad_coupling, int32_t* status) {
+                         double* grad_coupling, int32_t* status, double* g1n_out) {
   if (n <= 0 || n_layers <= 0 || n_inst < 0)
     return fail(NEGF_ERR_ARG, "invalid sizes (n=%d layers=%d)", n, n_layers);
   if (!onsite || (n_layers > 1 && (!coupling || !grad_coupling)) || !set_of || !omega ||
@@ -2972,8 +3226,51 @@ int negf_adjoint_batched(void* stream, int n, int n_layers, int n_inst, const do
     return fail(NEGF_ERR_ARG, "block size %d exceeds the adjoint limit of %d", n, GREEN_MAX);
   AdjArgs a{n, n_layers, n_inst, onsite, coupling, set_of, omega, eta_dev, sigma_l, sigma_r,
             status_in, weight, reinterpret_cast<double*>(workspace), t_out, grad_onsite,
-            grad_coupling, status};
+            grad_coupling, status, g1n_out};
   return run_adj(reinterpret_cast<hipStream_t>(stream), nmax, a);
+}
+
+int negf_lead_adjoint_batched(void* stream, int side, int n, int n_layers, int n_inst, int n_lead,
+                              int n_c, const double* lead, const double* v, const int32_t* idx,
+                              const int32_t* set_of, const double* g, const double* g1n,
+                              const double* sigma_l, const double* sigma_r,
+                              const double* grad_onsite, const double* weight,
+                              const int32_t* status_in, int maxiter, double* grad_lead,
+                              double* grad_v, int32_t* status) {
+  if (side != 0 && side != 1) return fail(NEGF_ERR_ARG, "lead side %d is not 0 or 1", side);
+  if (n <= 0 || n_layers <= 0 || n_inst < 0 || n_lead <= 0 || n_c <= 0 || n_c > n)
+    return fail(NEGF_ERR_ARG, "invalid lead adjoint sizes (n=%d layers=%d lead=%d contact=%d)", n,
+                n_layers, n_lead, n_c);
+  if (maxiter <= 0) return fail(NEGF_ERR_ARG, "maxiter must be positive (got %d)", maxiter);
+  if (!lead || !v || !idx || !set_of || !g || !g1n || !sigma_l || !sigma_r || !grad_onsite ||
+      !grad_lead || !grad_v || !status)
+    return fail(NEGF_ERR_ARG, "null pointer argument");
+  if (n_inst == 0) return NEGF_OK;
+  const int m = n > n_lead ? n : n_lead;
+  const int nmax = default_nmax(m, true);
+  if (!nmax || nmax > 48)
+    return fail(NEGF_ERR_ARG, "lead gradients take blocks of up to 48 DOF (got %d)", m);
+  // the contact layer's onsite gradient: layer 0 (left) or L-1 (right) of [n_inst][L][n][n]
+  const size_t nn = (size_t)n * n;
+  LeadAdjArgs a{n, n_inst, side, n_lead, n_c, lead, v, idx, set_of, g, g1n, sigma_l, sigma_r,
+                grad_onsite + (side ? (size_t)(n_layers - 1) * nn * 2 : 0),
+                (size_t)n_layers * nn * 2, weight, status_in, maxiter, grad_lead, grad_v, status};
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+#define LA_CASE(NM)                                                                          \
+  case NM: {                                                                                 \
+    constexpr int NT = Cfg<NM>::NT;                                                          \
+    NEGF_LAUNCH((k_lead_adjoint<NM>), n_inst, NT, (Lds<double, NM, NT, 4>::bytes()), st, a); \
+    return NEGF_OK;                                                                          \
+  }
+  switch (nmax) {
+    LA_CASE(8)
+    LA_CASE(16)
+    LA_CASE(24)
+    LA_CASE(32)
+    LA_CASE(48)
+  }
+#undef LA_CASE
+  return fail(NEGF_ERR_ARG, "lead gradients take blocks of up to 48 DOF (got %d)", m);
 }
 
 int negf_fcs_cumulants(void* stream, int n_w, const double* omega_rad_s, const double* t_mean,

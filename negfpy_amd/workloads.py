@@ -453,6 +453,28 @@ def interface3d_blocks(p: Interface3DProblem, fc00, fc01):
     return ons, cpl
 
 
+def interface3d_lead_blocks(p: Interface3DProblem, f00, f01):
+    """Lead blocks and device-to-lead couplings of every k point from the bulk force
+    constants ``f00`` / ``f01`` (n_terms, n, n) as a differentiable torch expression,
+    the same construction as ``interface3d_problem``'s (interface contacts: the left
+    lead faces left, V_L = M_0^-1/2 Phi_10 M_L^-1/2, V_R = M_{L-1}^-1/2 Phi_01 M_R^-1/2).
+    Returns (lead_left, lead_right (S, 3, n, n), v_left, v_right (S, n, n))."""
+    import torch
+
+    dev = f00.device
+    c128 = torch.complex128
+    ph = torch.as_tensor(p.phases, dtype=c128, device=dev)
+    m_l, m_r = float(p.masses[0, 0]), float(p.masses[-1, 0])
+    p00 = torch.einsum("st,tij->sij", ph, f00.to(c128))
+    p01 = torch.einsum("st,tij->sij", ph, f01.to(c128))
+    p10 = p01.conj().transpose(1, 2)
+    lead_left = torch.stack([p00 / m_l, p10 / m_l, p01 / m_l], dim=1)
+    lead_right = torch.stack([p00 / m_r, p01 / m_r, p10 / m_r], dim=1)
+    v_left = p10 / np.sqrt(m_l * m_l)
+    v_right = p01 / np.sqrt(m_r * m_r)
+    return lead_left, lead_right, v_left, v_right
+
+
 # (cell, transverse k mesh) of the 3-D uneven batch: 24-, 48- and 96-DOF layers
 # (conventional diamond cubes along x, y, z; k mesh per transverse direction)
 UNEVEN3D_PARTS = (((1, 1, 1), 32), ((2, 1, 1), 8), ((2, 2, 1), 4))
