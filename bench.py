@@ -227,6 +227,12 @@ def uneven_batch_3d(steps: int, warmup: int) -> dict:
             "identical_outputs": bool(same)}
 
 
+def _lib_ws(n: int, n_layers: int, checkpoint: int) -> int:
+    from negfpy_amd import _lib
+
+    return _lib.load().negf_adjoint_workspace_bytes(n, n_layers, 1, checkpoint)
+
+
 def adjoint_batch(steps: int, warmup: int) -> dict:
     """The adjoint workload (48-DOF mass-mismatch interface between two different
     leads, 64 layers): forward transmission solves and adjoint solves (T and
@@ -254,7 +260,9 @@ def adjoint_batch(steps: int, warmup: int) -> dict:
     for name, fn in (("forward", lambda: solver.run(set_of, omega, eta)),
                      ("adjoint", lambda: _engine.adjoint(sets, set_of, omega, eta, eta)),
                      ("adjoint_leads", lambda: _engine.adjoint(sets, set_of, omega, eta, eta,
-                                                               lead_grads=True))):
+                                                               lead_grads=True)),
+                     ("adjoint_ckpt", lambda: _engine.adjoint(sets, set_of, omega, eta, eta,
+                                                              checkpoint=8))):
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
@@ -271,6 +279,10 @@ def adjoint_batch(steps: int, warmup: int) -> dict:
             "adjoint_solves_per_s": round(m / times["adjoint"], 1),
             "adjoint_over_forward_time": round(times["adjoint"] / times["forward"], 3),
             "adjoint_with_lead_gradients_solves_per_s": round(m / times["adjoint_leads"], 1),
+            "adjoint_checkpoint_every_8_layers_solves_per_s": round(m / times["adjoint_ckpt"], 1),
+            "adjoint_workspace_bytes_per_instance": {
+                "stored": int(_lib_ws(n, dev.n_layers, 0)),
+                "checkpoint_every_8": int(_lib_ws(n, dev.n_layers, 8))},
             "t_adjoint_vs_forward_max_rel": float(np.max(np.abs(ta[ok] - tf[ok])
                                                          / np.maximum(np.abs(tf[ok]), 1e-3)))}
 

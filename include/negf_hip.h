@@ -233,18 +233,30 @@ int negf_device_green_batched(void* stream, int n, int n_layers, int n_inst, con
  * left-connected sweep (column N).  The reference has no gradients; this is the
  * reverse mode of its transmission, src/negfpy/core/negf.py:306-357, with respect
  * to the device blocks it assembles at :53-69.
- *   weight [n_inst] (dL/dT, null: 1); workspace >= negf_adjoint_workspace_bytes;
+ *   weight [n_inst] (dL/dT, null: 1); workspace >= negf_adjoint_workspace_bytes (same
+ *   checkpoint);
  *   t_out [n_inst]; grad_onsite [n_inst][L][n][n]; grad_coupling [n_inst][L-1][n][n];
  *   status [n_inst] (lead status passed through, 12/13 on device failure);
- *   g1n_out [n_inst][n][n] or NULL: G_1N, for negf_lead_adjoint_batched
+ *   g1n_out [n_inst][n][n] or NULL: G_1N, for negf_lead_adjoint_batched;
+ *   checkpoint: 0 stores every block of both sweeps ((2L - 1) n^2 complex per instance);
+ *     K > 0 keeps only the left-connected sweep's carried state every K layers and
+ *     recomputes each segment's blocks in the gradient pass (one more left sweep of
+ *     GJ inversions and products; (L + ceil(L/K) + K + 1) n^2 per instance, least near
+ *     K = sqrt(L)), same results;
+ *   per_set: 0 gives per-instance gradients; 1 sums them into grad_onsite
+ *     [n_sets][L][n][n] / grad_coupling [n_sets][L-1][n][n] with atomic adds (the caller
+ *     zeroes them; the order of the sums varies, so results vary at rounding level);
+ *   grad_ends_out [n_inst][2][n][n] or NULL: the onsite gradients of layers 1 and N of
+ *     every instance (also with per_set), for negf_lead_adjoint_batched.
  */
-size_t negf_adjoint_workspace_bytes(int n, int n_layers, int n_inst);
+size_t negf_adjoint_workspace_bytes(int n, int n_layers, int n_inst, int checkpoint);
 int negf_adjoint_batched(void* stream, int n, int n_layers, int n_inst, const double* onsite,
                          const double* coupling, const int32_t* set_of, const double* omega,
                          const double* eta_dev, const double* sigma_l, const double* sigma_r,
                          const int32_t* status_in, const double* weight, void* workspace,
                          size_t workspace_bytes, double* t_out, double* grad_onsite,
-                         double* grad_coupling, int32_t* status, double* g1n_out);
+                         double* grad_coupling, int32_t* status, double* g1n_out,
+                         int checkpoint, int per_set, double* grad_ends_out);
 
 /*
  * Lead side of the adjoint: weight * dT/d(lead blocks) and weight * dT/dV for one lead
@@ -259,16 +271,16 @@ int negf_adjoint_batched(void* stream, int n, int n_layers, int n_inst, const do
  *   g     [n_inst][n_lead][n_lead]      its surface GF (the forward's)
  *   g1n   [n_inst][n][n]                G_1N (negf_adjoint_batched's g1n_out)
  *   sigma_l, sigma_r [n_inst][n][n]     the embedded self-energies of the forward
- *   grad_onsite [n_inst][L][n][n]       negf_adjoint_batched's output (same weight)
+ *   grad_ends [n_inst][2][n][n]         negf_adjoint_batched's grad_ends_out (same weight)
  *   status_in [n_inst]                  the adjoint's status; != 0 gives zero gradients
  *   grad_lead [n_inst][3][n_lead][n_lead], grad_v [n_inst][n_c][n_lead]   written
  *   status [n_inst]                     0, status_in, 1 (not converged), 2 (non-finite)
  */
-int negf_lead_adjoint_batched(void* stream, int side, int n, int n_layers, int n_inst, int n_lead,
-                              int n_c, const double* lead, const double* v, const int32_t* idx,
+int negf_lead_adjoint_batched(void* stream, int side, int n, int n_inst, int n_lead, int n_c,
+                              const double* lead, const double* v, const int32_t* idx,
                               const int32_t* set_of, const double* g, const double* g1n,
                               const double* sigma_l, const double* sigma_r,
-                              const double* grad_onsite, const double* weight,
+                              const double* grad_ends, const double* weight,
                               const int32_t* status_in, int maxiter, double* grad_lead,
                               double* grad_v, int32_t* status);
 

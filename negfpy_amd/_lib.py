@@ -103,13 +103,13 @@ _SIGS = {
         _c_int),
     "negf_device_green_batched": (
         [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
-    "negf_adjoint_workspace_bytes": ([_c_int, _c_int, _c_int], ctypes.c_size_t),
+    "negf_adjoint_workspace_bytes": ([_c_int, _c_int, _c_int, _c_int], ctypes.c_size_t),
     "negf_adjoint_batched": (
         [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-         ctypes.c_size_t, _vp, _vp, _vp, _vp, _vp], _c_int),
+         ctypes.c_size_t, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp], _c_int),
     "negf_lead_adjoint_batched": (
-        [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-         _vp, _vp, _vp, _vp, _c_int, _vp, _vp, _vp], _c_int),
+        [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+         _vp, _vp, _vp, _c_int, _vp, _vp, _vp], _c_int),
     "negf_transmission_trace_batched": ([_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp], _c_int),
     "negf_merge_status": ([_vp, _c_int, _vp, _vp], _c_int),
     "negf_transmission_sweep_batched": (

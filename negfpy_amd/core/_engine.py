@@ -604,8 +604,8 @@ def device_green(sets: TransportSets, set_of, omega, eta_lead, eta_dev, *, tol=1
 @dataclass
 class AdjointResult:
     t: torch.Tensor              # (n_inst,) float64
-    grad_onsite: torch.Tensor    # (n_inst, L, n, n) complex128, weight * dT/dD_ii
-    grad_coupling: torch.Tensor  # (n_inst, max(L-1, 1), n, n) complex128, weight * dT/dC_i
+    grad_onsite: torch.Tensor    # (n_inst or n_sets, L, n, n) complex128, weight * dT/dD_ii
+    grad_coupling: torch.Tensor  # (n_inst or n_sets, max(L-1, 1), n, n), weight * dT/dC_i
     status: torch.Tensor         # (n_inst,) int32
     # with lead_grads: weight * dT/d(d00, d01, d10) (n_inst, 3, nl, nl) and weight * dT/dV
     # (n_inst, n_c, nl) of each lead
@@ -621,10 +621,16 @@ LEAD_ADJOINT_MAXITER = 64
 
 
 def adjoint(sets: TransportSets, set_of, omega, eta_lead, eta_dev, weight=None, *, tol=1e-12,
-            maxiter=200, route=0, lead_grads=False) -> AdjointResult:
+            maxiter=200, route=0, lead_grads=False, checkpoint=0,
+            per_set=False) -> AdjointResult:
     """T and weight * dT/d(device blocks) of every instance (FP64 adjoint kernel); with
     ``lead_grads`` also weight * dT/d(lead blocks) and weight * dT/dV of both leads
-    (negf_lead_adjoint_batched).  A lead gradient whose Stein solve fails raises."""
+    (negf_lead_adjoint_batched).  A lead gradient whose Stein solve fails raises.
+
+    ``checkpoint`` K > 0 keeps the left-connected sweep only every K layers and
+    recomputes the rest per segment (less workspace, one more left sweep);
+    ``per_set`` sums the device-block gradients into their sets on the GPU
+    (grad_onsite / grad_coupling then have n_sets rows)."""
     lib = _lib.load()
     set_of = to_dev_i32(set_of)
     omega = to_dev_f64(omega)
@@ -636,20 +642,24 @@ def adjoint(sets: TransportSets, set_of, omega, eta_lead, eta_dev, weight=None, 
                              return_g=lead_grads)
     sig_l, sig_r, lead_status = out[:3]
     dev = sets.onsite.device
+    c128 = torch.complex128
     w = None if weight is None else to_dev_f64(weight)
-    g1n = torch.empty((n_inst, n, n), dtype=torch.complex128, device=dev) if lead_grads else None
-    ws_bytes = int(lib.negf_adjoint_workspace_bytes(n, nl, n_inst))
+    g1n = torch.empty((n_inst, n, n), dtype=c128, device=dev) if lead_grads else None
+    ends = torch.empty((n_inst, 2, n, n), dtype=c128, device=dev) if lead_grads else None
+    ws_bytes = int(lib.negf_adjoint_workspace_bytes(n, nl, n_inst, int(checkpoint)))
     ws = torch.empty((max(ws_bytes, 1),), dtype=torch.uint8, device=dev)
     t = torch.empty((n_inst,), dtype=torch.float64, device=dev)
-    g_ons = torch.zeros((n_inst, nl, n, n), dtype=torch.complex128, device=dev)
-    g_cpl = torch.zeros((n_inst, max(nl - 1, 1), n, n), dtype=torch.complex128, device=dev)
+    rows = sets.n_sets if per_set else n_inst
+    g_ons = torch.zeros((rows, nl, n, n), dtype=c128, device=dev)
+    g_cpl = torch.zeros((rows, max(nl - 1, 1), n, n), dtype=c128, device=dev)
     status = torch.zeros((n_inst,), dtype=torch.int32, device=dev)
     _lib.check(lib.negf_adjoint_batched(
         _lib.stream(), n, nl, n_inst, sets.onsite.data_ptr(), sets.coupling.data_ptr(),
         set_of.data_ptr(), omega.data_ptr(), eta_dev.data_ptr(), sig_l.data_ptr(),
         sig_r.data_ptr(), lead_status.data_ptr(), None if w is None else w.data_ptr(),
         ws.data_ptr(), ws_bytes, t.data_ptr(), g_ons.data_ptr(), g_cpl.data_ptr(),
-        status.data_ptr(), None if g1n is None else g1n.data_ptr()))
+        status.data_ptr(), None if g1n is None else g1n.data_ptr(), int(checkpoint),
+        1 if per_set else 0, None if ends is None else ends.data_ptr()))
     res = AdjointResult(t=t, grad_onsite=g_ons, grad_coupling=g_cpl, status=status)
     if not lead_grads:
         return res
@@ -658,13 +668,13 @@ def adjoint(sets: TransportSets, set_of, omega, eta_lead, eta_dev, weight=None, 
     for side, lead, v, idx, g in ((0, sets.lead_l, sets.v_l, sets.idx_l, out[3][0]),
                                   (1, sets.lead_r, sets.v_r, sets.idx_r, out[3][1])):
         nld, nc = int(lead.shape[2]), int(v.shape[1])
-        gl = torch.empty((n_inst, 3, nld, nld), dtype=torch.complex128, device=dev)
-        gv = torch.empty((n_inst, nc, nld), dtype=torch.complex128, device=dev)
+        gl = torch.empty((n_inst, 3, nld, nld), dtype=c128, device=dev)
+        gv = torch.empty((n_inst, nc, nld), dtype=c128, device=dev)
         st = torch.empty((n_inst,), dtype=torch.int32, device=dev)
         _lib.check(lib.negf_lead_adjoint_batched(
-            _lib.stream(), side, n, nl, n_inst, nld, nc, lead.data_ptr(), v.data_ptr(),
+            _lib.stream(), side, n, n_inst, nld, nc, lead.data_ptr(), v.data_ptr(),
             idx.data_ptr(), set_of.data_ptr(), g.data_ptr(), g1n.data_ptr(), sig_l.data_ptr(),
-            sig_r.data_ptr(), g_ons.data_ptr(), None if w is None else w.data_ptr(),
+            sig_r.data_ptr(), ends.data_ptr(), None if w is None else w.data_ptr(),
             status.data_ptr(), LEAD_ADJOINT_MAXITER, gl.data_ptr(), gv.data_ptr(),
             st.data_ptr()))
         bad = (st != 0) & (status == 0)

@@ -38,6 +38,12 @@ __all__ = ["transmission_torch", "transmission_sets_torch", "transmission_sum_to
 # when that is more.
 ADJOINT_CHUNK_BYTES = 2 << 30
 ADJOINT_FREE_FRACTION = 0.25
+# Layer checkpointing of the k-resolved backward: None recomputes the left-connected
+# sweep from checkpoints every ~sqrt(L) layers only when the stored blocks would leave
+# fewer than ADJOINT_MIN_CHUNK instances per launch (too few to fill the GPU); 0 never;
+# K > 0 always, every K layers.
+ADJOINT_CHECKPOINT = None
+ADJOINT_MIN_CHUNK = 2048
 
 
 def _merge_layers_torch(onsite: torch.Tensor, coupling: torch.Tensor, m: int):
@@ -162,18 +168,40 @@ def transmission_torch(omega, onsite, coupling, lead_left, lead_right, *, eta: f
     return _TransmissionFn.apply(ons, cpl, _Static(kp, w, eta, eta_dev))
 
 
-def _adjoint_bytes_per_instance(sets, nl: int, n: int, leads: bool = False) -> int:
-    """Device bytes one instance of an adjoint chunk holds: the kernel's (2L - 1) stored
-    sweep blocks, as many per-instance gradient blocks, and the lead surface GFs and
-    self-energies of _engine.adjoint (complex128); with lead gradients also G_1N and the
-    per-instance lead and coupling gradients."""
+def _adjoint_blocks(nl: int, ckpt: int) -> int:
+    """Workspace blocks of one adjoint instance (negf_adjoint_workspace_bytes)."""
+    if ckpt <= 0:
+        return 2 * nl - 1
+    k = min(ckpt, nl)
+    return nl + -(-nl // k) + k + 1
+
+
+def _adjoint_bytes_per_instance(sets, nl: int, n: int, leads: bool = False, ckpt: int = 0,
+                                per_set: bool = False) -> int:
+    """Device bytes one instance of an adjoint chunk holds: the kernel's workspace
+    blocks, the per-instance gradient blocks (none when summed per set on the GPU), and
+    the lead surface GFs and self-energies of _engine.adjoint (complex128); with lead
+    gradients also G_1N, the end-layer gradients and the lead and coupling gradients."""
     n_l = int(sets.lead_l.shape[2])
     n_r = int(sets.lead_r.shape[2])
-    b = 2 * (2 * nl - 1) * n * n + n_l * n_l + n_r * n_r + 2 * n * n
+    b = (_adjoint_blocks(nl, ckpt) + (0 if per_set else 2 * nl - 1) + 2) * n * n \
+        + n_l * n_l + n_r * n_r
     if leads:
-        b += n * n + 3 * (n_l * n_l + n_r * n_r) + int(sets.v_l.shape[1]) * n_l \
+        b += 3 * n * n + 3 * (n_l * n_l + n_r * n_r) + int(sets.v_l.shape[1]) * n_l \
             + int(sets.v_r.shape[1]) * n_r
     return b * 16
+
+
+def _plan(sets, nl: int, n: int, leads: bool, n_inst: int, dev):
+    """(checkpoint interval, instances per launch) of a per-set backward."""
+    stored = _chunk_instances(_adjoint_bytes_per_instance(sets, nl, n, leads, 0, True), n_inst,
+                              dev)
+    ck = ADJOINT_CHECKPOINT
+    if ck == 0 or (ck is None and stored >= min(n_inst, ADJOINT_MIN_CHUNK)):
+        return 0, stored
+    k = int(ck) if ck else max(1, int(round(np.sqrt(nl))))
+    return k, _chunk_instances(_adjoint_bytes_per_instance(sets, nl, n, leads, k, True), n_inst,
+                               dev)
 
 
 def _chunk_instances(per_inst: int, n_inst: int, dev) -> int:
@@ -220,8 +248,7 @@ class _SetsTransmissionFn(torch.autograd.Function):
         weight = grad_t.detach().to(torch.float64).contiguous()
         nl, n = int(ons.shape[1]), int(ons.shape[2])
         n_inst = int(static.omega.numel())
-        per_inst = _adjoint_bytes_per_instance(sets, nl, n, leads)
-        chunk = _chunk_instances(per_inst, n_inst, ons.device)
+        ckpt, chunk = _plan(sets, nl, n, leads, n_inst, ons.device)
         g_ons = torch.zeros_like(ons)
         g_cpl = torch.zeros((ons.shape[0], max(nl - 1, 1), n, n), dtype=ons.dtype,
                             device=ons.device)
@@ -229,11 +256,12 @@ class _SetsTransmissionFn(torch.autograd.Function):
         for a in range(0, n_inst, chunk):
             b = min(n_inst, a + chunk)
             k = static.set_of[a:b].long()
+            # the device-block gradients come back summed per set (GPU atomics)
             res = _engine.adjoint(sets, static.set_of[a:b], static.omega[a:b], static.eta[a:b],
-                                  static.eta_dev[a:b], weight=weight[a:b], lead_grads=leads)
-            # per-set sums of the per-instance gradients (real view: float64 atomics)
-            torch.view_as_real(g_ons).index_add_(0, k, torch.view_as_real(res.grad_onsite))
-            torch.view_as_real(g_cpl).index_add_(0, k, torch.view_as_real(res.grad_coupling))
+                                  static.eta_dev[a:b], weight=weight[a:b], lead_grads=leads,
+                                  checkpoint=ckpt, per_set=True)
+            g_ons += res.grad_onsite
+            g_cpl += res.grad_coupling
             if leads:
                 _add_lead_grads(g_lead, k, res)
             del res
@@ -335,8 +363,7 @@ class _SetsSumFn(torch.autograd.Function):
         leads = _leads_wanted(ctx)
         nl, n = int(ons.shape[1]), int(ons.shape[2])
         n_inst = int(static.omega.numel())
-        per_inst = _adjoint_bytes_per_instance(sets, nl, n, leads)
-        chunk = _chunk_instances(per_inst, n_inst, ons.device)
+        ckpt, chunk = _plan(sets, nl, n, leads, n_inst, ons.device)
         g_ons = torch.zeros_like(ons)
         g_cpl = torch.zeros((ons.shape[0], max(nl - 1, 1), n, n), dtype=ons.dtype,
                             device=ons.device)
@@ -347,11 +374,12 @@ class _SetsSumFn(torch.autograd.Function):
             b = min(n_inst, a + chunk)
             k = static.set_of[a:b].long()
             res = _engine.adjoint(sets, static.set_of[a:b], static.omega[a:b], static.eta[a:b],
-                                  static.eta_dev[a:b], weight=coef[a:b], lead_grads=leads)
+                                  static.eta_dev[a:b], weight=coef[a:b], lead_grads=leads,
+                                  checkpoint=ckpt, per_set=True)
             t_all[a:b] = res.t
             st_all[a:b] = res.status
-            torch.view_as_real(g_ons).index_add_(0, k, torch.view_as_real(res.grad_onsite))
-            torch.view_as_real(g_cpl).index_add_(0, k, torch.view_as_real(res.grad_coupling))
+            g_ons += res.grad_onsite
+            g_cpl += res.grad_coupling
             if leads:
                 _add_lead_grads(g_lead, k, res)
             del res

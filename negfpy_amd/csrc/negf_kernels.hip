@@ -1695,7 +1695,19 @@ struct AdjArgs {
   double* grad_coupling;     // [n_inst][L-1][n][n] complex
   int32_t* status;
   double* g1n_out;           // [n_inst][n][n] G_1N; may be null
+  int ckpt;                  // 0: store every Y_i; K > 0: keep the left sweep's state every K
+                             // layers and recompute Y_i per segment in the gradient pass
+  int per_set;               // 1: gradients summed into [n_sets][..] blocks (atomic adds)
+  double* grad_ends_out;     // [n_inst][2][n][n] onsite gradients of layers 1 and N; may be null
 };
+
+// Workspace blocks (n x n complex) of one adjoint instance: the L row-1 blocks (R_i, then
+// v_i, then P_i) and either the L - 1 left-sweep blocks Y_i or, with checkpoints every K
+// layers, ceil(L / K) carried states, a K-block segment buffer and one block for u.
+__host__ __device__ inline size_t adj_blocks(int n_layers, int ckpt) {
+  if (ckpt <= 0) return (size_t)(2 * n_layers - 1);
+  return (size_t)n_layers + (size_t)((n_layers + ckpt - 1) / ckpt) + (size_t)ckpt + 1;
+}
 
 template <int NMAX, class L>
 __device__ void adj_body(const AdjArgs& a, int inst, L& s, int tid) {
@@ -1716,15 +1728,39 @@ __device__ void adj_body(const AdjArgs& a, int inst, L& s, int tid) {
   const double* cpl = a.coupling + set * (nl > 1 ? nl - 1 : 0) * nn * 2;
   const double2* sgl = reinterpret_cast<const double2*>(a.sigma_l + (size_t)inst * nn * 2);
   const double2* sgr = reinterpret_cast<const double2*>(a.sigma_r + (size_t)inst * nn * 2);
-  double* vw = a.work + (size_t)inst * (2 * nl - 1) * nn * 2;  // R_i, then v_i, then P_i
-  double* yw = vw + (size_t)nl * nn * 2;                         // Y_i
-  double* go = a.grad_onsite + (size_t)inst * nl * nn * 2;
-  double* gc = a.grad_coupling + (size_t)inst * (nl > 1 ? nl - 1 : 0) * nn * 2;
+  const int kc = a.ckpt;
+  double* vw = a.work + (size_t)inst * adj_blocks(nl, kc) * nn * 2;  // R_i, then v_i, then P_i
+  double* yw = vw + (size_t)nl * nn * 2;  // Y_i, or the left sweep's checkpoints
+  double* ybuf = yw + (size_t)(kc > 0 ? (nl + kc - 1) / kc : 0) * nn * 2;  // Y of a segment
+  double* usave = ybuf + (size_t)kc * nn * 2;                               // u across it
+  const size_t gi = a.per_set ? (size_t)a.set_of[inst] : (size_t)inst;
+  double* go = a.grad_onsite + gi * nl * nn * 2;
+  double* gc = a.grad_coupling + gi * (nl > 1 ? nl - 1 : 0) * nn * 2;
   auto blk = [&](double* base, int i) { return base + (size_t)i * nn * 2; };
+
   const double w2 = 2.0 * (a.weight ? a.weight[inst] : 1.0);
   s.zero(tid);
   __syncthreads();
   const Tile<TM> t(n, n, tid);
+  // a gradient block: stored per instance, or added into its set's (instances of one set
+  // run concurrently: atomic adds)
+  auto gput = [&](double* g, const T (&cr)[TM][TM], const T (&ci)[TM][TM]) {
+    if (!a.per_set) {
+      tstore_glb<T, TM>(g, n, n, t, cr, ci);
+      return;
+    }
+    if (!t.own) return;
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+      for (int y = 0; y < TM; ++y) {
+        const int i = t.i0 + x, j = t.j0 + y;
+        if (i < n && j < n) {
+          atomicAdd(g + ((size_t)i * n + j) * 2, (double)cr[x][y]);
+          atomicAdd(g + ((size_t)i * n + j) * 2 + 1, (double)ci[x][y]);
+        }
+      }
+  };
   double zr, zi;
   zsq(a.omega[inst], a.eta_dev[inst], zr, zi);
   // A_ii - S into LDS matrix 0
@@ -1857,27 +1893,34 @@ __device__ void adj_body(const AdjArgs& a, int inst, L& s, int tid) {
   }
   __threadfence_block();
   __syncthreads();
-  // 5. left-connected sweep: gL_i, Y_i = gL_i C_i (stored), ends with gL_{N-1} = G_NN
-  tzero<T, TM>(sr, si);
-  for (int i = 0; i < nl; ++i) {
+  // 5. left-connected sweep: gL_i, Y_i = gL_i C_i (stored, or with checkpoints only the
+  //    carried state S entering every K-th layer), ends with gL_{N-1} = G_NN
+  // one step of it: layer i from the carried S (sr, si); Y_i to ydst (if set); the next S
+  auto left_step = [&](int i, double* ydst) -> int {
     build(i, sr, si);
     __syncthreads();
     const int st = gj_inverse<T, NT, NMAX, L::RG>(s.re[0], s.im[0], LD, n, *s.gj, tid);
-    if (st != ST_OK) {
-      status = 10 + st;
-      break;
-    }
-    if (i == nl - 1) break;
+    if (st != ST_OK || i == nl - 1) return st;
     gload<T, NT>(cpl + (size_t)i * nn * 2, n, n, s.re[2], s.im[2], LD, tid);
     __syncthreads();
     tzero<T, TM>(cr, ci);
     if (t.own) mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[2], s.im[2], LD, n, t.i0, t.j0, cr, ci);
     tstore<T, TM>(s.re[3], s.im[3], LD, n, n, t, cr, ci);
-    tstore_glb<T, TM>(blk(yw, i), n, n, t, cr, ci);
+    if (ydst) tstore_glb<T, TM>(ydst, n, n, t, cr, ci);
     __syncthreads();
     tzero<T, TM>(sr, si);
     if (t.own) mac<T, TM, OPH, OPN>(s.re[2], s.im[2], s.re[3], s.im[3], LD, n, t.i0, t.j0, sr, si);
     __syncthreads();
+    return st;
+  };
+  tzero<T, TM>(sr, si);
+  for (int i = 0; i < nl; ++i) {
+    if (kc > 0 && i % kc == 0) tstore_glb<T, TM>(blk(yw, i / kc), n, n, t, sr, si);
+    const int st = left_step(i, kc > 0 ? nullptr : blk(yw, i));
+    if (st != ST_OK) {
+      status = 10 + st;
+      break;
+    }
   }
   if (tid == 0) a.status[inst] = status;
   if (status != ST_OK) return;
@@ -1903,9 +1946,54 @@ __device__ void adj_body(const AdjArgs& a, int inst, L& s, int tid) {
       cr[x][y] *= w2;
       ci[x][y] *= w2;
     }
-  tstore_glb<T, TM>(blk(go, nl - 1), n, n, t, cr, ci);
+  auto gput_onsite = [&](int i, const T (&cr)[TM][TM], const T (&ci)[TM][TM]) {
+    gput(blk(go, i), cr, ci);
+    if (a.grad_ends_out) {
+      double* e = a.grad_ends_out + (size_t)inst * 2 * nn * 2;
+      if (i == 0) tstore_glb<T, TM>(e, n, n, t, cr, ci);
+      if (i == nl - 1) tstore_glb<T, TM>(e + nn * 2, n, n, t, cr, ci);
+    }
+  };
+  gput_onsite(nl - 1, cr, ci);
+  int seg_a = nl;  // first layer of the segment whose Y are in ybuf (checkpoints)
   for (int i = nl - 2; i >= 0; --i) {
-    bload<T, NT>(blk(yw, i), n, n, f0r, f0i, LD, tid);  // Y_i
+    if (kc > 0 && i < seg_a) {
+      // recompute Y_a..Y_i of this segment from its checkpoint; u_{i+1} waits in usave
+      // and P_{i+1} is reloaded from the row-1 blocks
+      seg_a = (i / kc) * kc;
+      __syncthreads();
+      gstore<T, NT>(usave, n, u1r, u1i, LD, tid);
+      __threadfence_block();
+      __syncthreads();
+      const Tile<TM>& tt = t;
+      if (tt.own) {
+        const double2* ck = reinterpret_cast<const double2*>(blk(yw, i / kc));
+#pragma unroll
+        for (int x = 0; x < TM; ++x)
+#pragma unroll
+          for (int y = 0; y < TM; ++y) {
+            const int r = tt.i0 + x, c = tt.j0 + y;
+            const double2 v = (r < n && c < n) ? ck[r * n + c] : make_double2(0.0, 0.0);
+            sr[x][y] = v.x;
+            si[x][y] = v.y;
+          }
+      }
+      for (int l = seg_a; l <= i; ++l) left_step(l, blk(ybuf, l - seg_a));
+      __threadfence_block();
+      __syncthreads();
+      u1r = s.re[0];
+      u1i = s.im[0];
+      p1r = s.re[1];
+      p1i = s.im[1];
+      f0r = s.re[2];
+      f0i = s.im[2];
+      f1r = s.re[3];
+      f1i = s.im[3];
+      bload<T, NT>(usave, n, n, u1r, u1i, LD, tid);
+      bload<T, NT>(blk(vw, i + 1), n, n, p1r, p1i, LD, tid);
+      __syncthreads();
+    }
+    bload<T, NT>(kc > 0 ? blk(ybuf, i - seg_a) : blk(yw, i), n, n, f0r, f0i, LD, tid);  // Y_i
     bload<T, NT>(blk(vw, i), n, n, f1r, f1i, LD, tid);  // P_i
     __syncthreads();
     tzero<T, TM>(sr, si);  // u_i = Y_i u_{i+1}
@@ -1926,7 +2014,7 @@ __device__ void adj_body(const AdjArgs& a, int inst, L& s, int tid) {
         cr[x][y] *= w2;
         ci[x][y] *= w2;
       }
-    tstore_glb<T, TM>(blk(gc, i), n, n, t, cr, ci);
+    gput(blk(gc, i), cr, ci);
     // onsite i: (u_i P_i)^H = P_i^H u_i^H
     tzero<T, TM>(cr, ci);
     if (t.own) mac<T, TM, OPH, OPH>(f1r, f1i, f0r, f0i, LD, n, t.i0, t.j0, cr, ci);
@@ -1937,7 +2025,7 @@ __device__ void adj_body(const AdjArgs& a, int inst, L& s, int tid) {
         cr[x][y] *= w2;
         ci[x][y] *= w2;
       }
-    tstore_glb<T, TM>(blk(go, i), n, n, t, cr, ci);
+    gput_onsite(i, cr, ci);
     __syncthreads();
     // u_{i+1} <- u_i (f0), P_{i+1} <- P_i (f1); the old ones become free
     T* xr = u1r;
@@ -3200,9 +3288,10 @@ int negf_device_green_batched(void* stream, int n, int n_layers, int n_inst, con
   return run_dgf(reinterpret_cast<hipStream_t>(stream), nmax, a);
 }
 
-size_t negf_adjoint_workspace_bytes(int n, int n_layers, int n_inst) {
-  if (n <= 0 || n_layers <= 0 || n_inst <= 0) return 0;
-  return (size_t)n_inst * (size_t)(2 * n_layers - 1) * (size_t)n * (size_t)n * 16u;
+size_t negf_adjoint_workspace_bytes(int n, int n_layers, int n_inst, int checkpoint) {
+  if (n <= 0 || n_layers <= 0 || n_inst <= 0 || checkpoint < 0) return 0;
+  const int k = checkpoint > n_layers ? n_layers : checkpoint;
+  return (size_t)n_inst * adj_blocks(n_layers, k) * (size_t)n * (size_t)n * 16u;
 }
 
 int negf_adjoint_batched(void* stream, int n, int n_layers, int n_inst, const double* onsite,
@@ -3210,14 +3299,18 @@ int negf_adjoint_batched(void* stream, int n, int n_layers, int n_inst, const do
                          const double* eta_dev, const double* sigma_l, const double* sigma_r,
                          const int32_t* status_in, const double* weight, void* workspace,
                          size_t workspace_bytes, double* t_out, double* grad_onsite,
-                         double* grad_coupling, int32_t* status, double* g1n_out) {
+                         double* grad_coupling, int32_t* status, double* g1n_out,
+                         int checkpoint, int per_set, double* grad_ends_out) {
   if (n <= 0 || n_layers <= 0 || n_inst < 0)
     return fail(NEGF_ERR_ARG, "invalid sizes (n=%d layers=%d)", n, n_layers);
+  if (checkpoint < 0) return fail(NEGF_ERR_ARG, "checkpoint interval %d is negative", checkpoint);
+  if (per_set != 0 && per_set != 1) return fail(NEGF_ERR_ARG, "per_set %d is not 0 or 1", per_set);
+  if (checkpoint > n_layers) checkpoint = n_layers;
   if (!onsite || (n_layers > 1 && (!coupling || !grad_coupling)) || !set_of || !omega ||
       !eta_dev || !sigma_l || !sigma_r || !t_out || !grad_onsite || !status)
     return fail(NEGF_ERR_ARG, "null pointer argument");
   if (n_inst == 0) return NEGF_OK;
-  const size_t need = negf_adjoint_workspace_bytes(n, n_layers, n_inst);
+  const size_t need = negf_adjoint_workspace_bytes(n, n_layers, n_inst, checkpoint);
   if (!workspace || workspace_bytes < need)
     return fail(NEGF_ERR_ARG, "workspace of %zu bytes is smaller than the %zu needed",
                 workspace_bytes, need);
@@ -3226,23 +3319,23 @@ int negf_adjoint_batched(void* stream, int n, int n_layers, int n_inst, const do
     return fail(NEGF_ERR_ARG, "block size %d exceeds the adjoint limit of %d", n, GREEN_MAX);
   AdjArgs a{n, n_layers, n_inst, onsite, coupling, set_of, omega, eta_dev, sigma_l, sigma_r,
             status_in, weight, reinterpret_cast<double*>(workspace), t_out, grad_onsite,
-            grad_coupling, status, g1n_out};
+            grad_coupling, status, g1n_out, checkpoint, per_set, grad_ends_out};
   return run_adj(reinterpret_cast<hipStream_t>(stream), nmax, a);
 }
 
-int negf_lead_adjoint_batched(void* stream, int side, int n, int n_layers, int n_inst, int n_lead,
-                              int n_c, const double* lead, const double* v, const int32_t* idx,
+int negf_lead_adjoint_batched(void* stream, int side, int n, int n_inst, int n_lead, int n_c,
+                              const double* lead, const double* v, const int32_t* idx,
                               const int32_t* set_of, const double* g, const double* g1n,
                               const double* sigma_l, const double* sigma_r,
-                              const double* grad_onsite, const double* weight,
+                              const double* grad_ends, const double* weight,
                               const int32_t* status_in, int maxiter, double* grad_lead,
                               double* grad_v, int32_t* status) {
   if (side != 0 && side != 1) return fail(NEGF_ERR_ARG, "lead side %d is not 0 or 1", side);
-  if (n <= 0 || n_layers <= 0 || n_inst < 0 || n_lead <= 0 || n_c <= 0 || n_c > n)
-    return fail(NEGF_ERR_ARG, "invalid lead adjoint sizes (n=%d layers=%d lead=%d contact=%d)", n,
-                n_layers, n_lead, n_c);
+  if (n <= 0 || n_inst < 0 || n_lead <= 0 || n_c <= 0 || n_c > n)
+    return fail(NEGF_ERR_ARG, "invalid lead adjoint sizes (n=%d lead=%d contact=%d)", n, n_lead,
+                n_c);
   if (maxiter <= 0) return fail(NEGF_ERR_ARG, "maxiter must be positive (got %d)", maxiter);
-  if (!lead || !v || !idx || !set_of || !g || !g1n || !sigma_l || !sigma_r || !grad_onsite ||
+  if (!lead || !v || !idx || !set_of || !g || !g1n || !sigma_l || !sigma_r || !grad_ends ||
       !grad_lead || !grad_v || !status)
     return fail(NEGF_ERR_ARG, "null pointer argument");
   if (n_inst == 0) return NEGF_OK;
@@ -3250,11 +3343,11 @@ int negf_lead_adjoint_batched(void* stream, int side, int n, int n_layers, int n
   const int nmax = default_nmax(m, true);
   if (!nmax || nmax > 48)
     return fail(NEGF_ERR_ARG, "lead gradients take blocks of up to 48 DOF (got %d)", m);
-  // the contact layer's onsite gradient: layer 0 (left) or L-1 (right) of [n_inst][L][n][n]
+  // the contact layer's onsite gradient: [0] (left) or [1] (right) of [n_inst][2][n][n]
   const size_t nn = (size_t)n * n;
   LeadAdjArgs a{n, n_inst, side, n_lead, n_c, lead, v, idx, set_of, g, g1n, sigma_l, sigma_r,
-                grad_onsite + (side ? (size_t)(n_layers - 1) * nn * 2 : 0),
-                (size_t)n_layers * nn * 2, weight, status_in, maxiter, grad_lead, grad_v, status};
+                grad_ends + (side ? nn * 2 : 0), 2 * nn * 2, weight, status_in, maxiter,
+                grad_lead, grad_v, status};
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
 #define LA_CASE(NM)                                                                          \
   case NM: {                                                                                 \

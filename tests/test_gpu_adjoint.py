@@ -261,3 +261,56 @@ def test_transmission_torch_argument_errors(gpu):
         transmission_torch([1.0], ons, cps[:2], lead, lead)
     with pytest.raises(ValueError, match="thermal|tvals"):
         thermal_conductance_torch([1.0, 2.0], torch.zeros(3, dtype=torch.float64), [300.0])
+
+
+@pytest.mark.parametrize("n,nl", [(6, 11), (24, 9), (60, 5)])
+def test_checkpointed_adjoint_equals_stored(gpu, n, nl):
+    """Layer checkpoints (the left-connected sweep kept every K layers and recomputed per
+    segment) give the stored form's T and gradients bit for bit, for K = 1, 2, 3, sqrt(L)
+    and L (one segment), in the workgroup classes and the pooled 96 class; per-set sums on
+    the GPU (atomic adds) equal the per-instance gradients summed on the host."""
+    from negfpy_amd.core import _engine
+
+    d00, d01 = random_lead(70 + n, n)
+    rng = np.random.default_rng(n + nl)
+    sets_n = 2
+    ons = np.stack([[d00 + 0.05 * np.diag(rng.uniform(-1, 1, n)) for _ in range(nl)]
+                    for _ in range(sets_n)])
+    cps = np.stack([[d01 * (1.0 + 0.03 * rng.uniform(-1, 1)) for _ in range(nl - 1)]
+                    for _ in range(sets_n)])
+    lb = np.stack([d00, d01, d01.conj().T])[None].repeat(sets_n, axis=0)
+    sets = _engine.TransportSets.from_arrays(lb, lb, lb[:, 2], lb[:, 1], np.arange(n),
+                                             np.arange(n), ons, cps, True)
+    m = 12
+    w = np.linspace(0.1, 1.0, m) * omega_max(LeadBlocks(d00=d00, d01=d01))
+    set_of = np.arange(m, dtype=np.int32) % sets_n
+    eta = np.full(m, 1e-6)
+    weight = rng.uniform(0.5, 1.5, m)
+    ref = _engine.adjoint(sets, set_of, w, eta, eta, weight)
+    assert int((ref.status != 0).sum()) == 0
+    for k in sorted({1, 2, 3, int(round(np.sqrt(nl))), nl}):
+        res = _engine.adjoint(sets, set_of, w, eta, eta, weight, checkpoint=k)
+        assert torch.equal(res.t, ref.t) and torch.equal(res.status, ref.status), k
+        assert torch.equal(res.grad_onsite, ref.grad_onsite), k
+        assert torch.equal(res.grad_coupling, ref.grad_coupling), k
+    k_of = torch.as_tensor(set_of, device="cuda").long()
+    per = _engine.adjoint(sets, set_of, w, eta, eta, weight, per_set=True, checkpoint=3)
+    for got, full in ((per.grad_onsite, ref.grad_onsite), (per.grad_coupling, ref.grad_coupling)):
+        want = torch.zeros_like(got)
+        want.index_add_(0, k_of, full)
+        err = (got - want).abs().max() / want.abs().max()
+        assert float(err) < 1e-13, float(err)
+
+
+def test_workspace_shrinks_with_checkpoints(gpu):
+    from negfpy_amd import _lib
+
+    lib = _lib.load()
+    n, nl = 48, 64
+    full = lib.negf_adjoint_workspace_bytes(n, nl, 1, 0)
+    assert full == (2 * nl - 1) * n * n * 16
+    k8 = lib.negf_adjoint_workspace_bytes(n, nl, 1, 8)
+    assert k8 == (nl + 8 + 8 + 1) * n * n * 16 and k8 < 0.65 * full
+    assert lib.negf_adjoint_workspace_bytes(n, nl, 1, 1000) == \
+        lib.negf_adjoint_workspace_bytes(n, nl, 1, nl)
+    assert lib.negf_adjoint_workspace_bytes(n, nl, 1, -1) == 0
