@@ -46,6 +46,13 @@ extern "C" {
  * with |lambda| above about 1 - 4e-4 (tol 1e-12), which FP32 cannot resolve to the certificate: it goes
  * to the FP64 decimation (with the caller's maxiter) without further FP32 work. */
 #define NEGF_MIXED_FP32_MAXITER 16
+/* Mixed precision: the FP32 decimation also stops (status 1, FP64 redo) from doubling
+ * STALL_FROM on when one doubling shrinks max(|alpha|, |beta|) by less than STALL_RATIO.
+ * The ratio of doubling k is about |lambda|^(2^(k-1)) for the slowest mode, so a ratio
+ * above 0.9 at doubling 9 means 1 - |lambda| < 4e-4: the decimation would need more
+ * than 17 doublings to reach tol 1e-12, beyond NEGF_MIXED_FP32_MAXITER anyway. */
+#define NEGF_MIXED_FP32_STALL_FROM 9
+#define NEGF_MIXED_FP32_STALL_RATIO 0.9
 
 enum { NEGF_FP64 = 0, NEGF_MIXED = 1 };
 enum { NEGF_SURFACE_DECIMATION = 0, NEGF_SURFACE_EIGEN = 1 };

@@ -358,6 +358,7 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
   int status = ST_OK;
   int it = 0;
   bool conv = false;
+  double gprev = 0.0;  // max |alpha|^2, |beta|^2 of the previous doubling (FP32 stall test)
   while (it < a.maxiter) {
     ++it;
     // G = z - e
@@ -455,6 +456,11 @@ __global__ void __launch_bounds__(Cfg<NMAX>::NT)
     if ((T)sqrt(gmax) < tol) {
       conv = true;
       break;
+    }
+    if constexpr (sizeof(T) == 4) {  // the mixed path's FP32 pass: give up on a stalled mode
+      constexpr double r2 = NEGF_MIXED_FP32_STALL_RATIO * NEGF_MIXED_FP32_STALL_RATIO;
+      if (it >= NEGF_MIXED_FP32_STALL_FROM && gmax > r2 * gprev) break;
+      gprev = gmax;
     }
   }
   if (status == ST_OK && !conv) status = ST_NOCONV;

@@ -46,6 +46,7 @@ __global__ void __launch_bounds__(WAVE) k_sr_wave(SrArgs a) {
   acc_zero(d);
   int status = ST_OK, it = 0;
   bool conv = false;
+  T gprev = 0;  // max |a|^2, |b|^2 of the previous doubling (FP32 stall test)
   while (it < a.maxiter) {
     ++it;
     acc_zminus2(es, d, G, n, zr, zi, lane);
@@ -84,6 +85,11 @@ __global__ void __launch_bounds__(WAVE) k_sr_wave(SrArgs a) {
     if (sqrt(gmax) < tol) {
       conv = true;
       break;
+    }
+    if constexpr (sizeof(T) == 4) {  // the mixed path's FP32 pass: give up on a stalled mode
+      constexpr T r2 = (T)(NEGF_MIXED_FP32_STALL_RATIO * NEGF_MIXED_FP32_STALL_RATIO);
+      if (it >= NEGF_MIXED_FP32_STALL_FROM && gmax > r2 * gprev) break;
+      gprev = gmax;
     }
   }
   if (status == ST_OK && !conv) status = ST_NOCONV;
