@@ -550,10 +550,10 @@ def main():
     fl_kernel = dict(fl_survey, sweep=8.0 * n3 + 32.0 * (n_layers - 1) * n3 + 3 * 16.0 * n3)
     fp64_fl_total = sum(fl_survey.values())
 
-    def roofline(path):
+    def roofline(path, dom=None):
         st = results[path]["stage_ms"]
         cand = [k for k in ("sancho_rubio_l", "sweep") if k in st]
-        dom = max(cand, key=lambda k: st[k])
+        dom = dom or max(cand, key=lambda k: st[k])
         fs, fk = fl_survey[dom], fl_kernel[dom]
         peak = PEAK_TFLOPS["fp64"]
         if path == "mixed" and dom == "sancho_rubio_l":  # the FP32 decimation pass
@@ -624,6 +624,10 @@ def main():
         "fp64_flops_per_instance": fp64_fl_total,
         "roofline": roofline(head),
         "roofline_fp64_path": roofline("fp64") if head != "fp64" else None,
+        # both FP64 kernels of the step (the decimation and the fused sweep), each
+        # against the FP64 matrix peak
+        "roofline_fp64_kernels": {k: roofline("fp64", s) for k, s in
+                                  (("sancho_rubio", "sancho_rubio_l"), ("sweep", "sweep"))},
         "stage_ms": {"mixed": mix["stage_ms"], "fp64": f64["stage_ms"]},
     }
     line["uneven_batch"] = uneven_batch(max(1, min(args.steps, 5)), 1)

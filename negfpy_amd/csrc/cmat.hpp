@@ -340,7 +340,11 @@ __device__ __forceinline__ void crecip1(T ar, T ai, T& rr, T& ri) {
 //  - the column and the pivot are read with one LDS wait;
 //  - the permutation stays in registers: the owner lanes of column k keep its pivot
 //    row, and the lane holding row p keeps the step at which it was the pivot, so the
-//    final scatter needs no LDS tables.
+//    final scatter needs no LDS tables;
+//  - a failed pivot does not leave the loop: the step goes on with row 0, the status
+//    of the first failure is returned and nothing is scattered, so every call passes
+//    n + 2 barriers (what the lockstep form needs) and the loop keeps its registers in
+//    place (an exit from it made the compiler copy the whole state every step).
 // Partial pivoting within that tolerance is as stable as the exact maximum; results
 // differ from gj_core's only where two candidates agree to 2^-14.
 template <typename T, int NMAX>
@@ -430,6 +434,7 @@ __device__ __forceinline__ int gj_lean(PT __restrict__ re, PT __restrict__ im, i
   };
   publish(0, 0);
   __syncthreads();
+  int fail = 0;  // first failed pivot's status (wave-uniform)
   for (int k = 0; k < n; ++k) {
     const int par = k & 1;
     T cr[MAXR], ci[MAXR];
@@ -444,15 +449,11 @@ __device__ __forceinline__ int gj_lean(PT __restrict__ re, PT __restrict__ im, i
 #pragma unroll
     for (int m = 0; m < MAXR; ++m) asm volatile("" : "+v"(cr[m]), "+v"(ci[m]));
     asm volatile("" : "+v"(pir), "+v"(pii), "+v"(pp));
-    const int p = __builtin_amdgcn_readfirstlane(pp);
-    if (p < 0) {
-      if constexpr (LS) {
-        for (int r = k; r <= n; ++r) __syncthreads();
-      } else {
-        __syncthreads();
-      }
-      return -p;
-    }
+    // a failed pivot is remembered and the elimination steps on with row 0 (its values
+    // are discarded): no exit from the loop, whose state then stays in place
+    const int pw = __builtin_amdgcn_readfirstlane(pp);
+    fail = (fail == 0 && pw < 0) ? -pw : fail;
+    const int p = pw < 0 ? 0 : pw;
     const int mp = p / RG, gp = p - (p / RG) * RG;
     T vr = xr[0], vi = xi[0];
 #pragma unroll
@@ -492,7 +493,7 @@ __device__ __forceinline__ int gj_lean(PT __restrict__ re, PT __restrict__ im, i
     if (k + 1 < n) publish(k + 1, par ^ 1);
     __syncthreads();
   }
-  if (on && jc < n) {
+  if (on && jc < n && fail == 0) {
 #pragma unroll
     for (int m = 0; m < MAXR; ++m) {
       const int i = g + RG * m;
@@ -503,7 +504,7 @@ __device__ __forceinline__ int gj_lean(PT __restrict__ re, PT __restrict__ im, i
     }
   }
   __syncthreads();
-  return ST_OK;
+  return fail ? fail : ST_OK;
 }
 
 template <typename T, int NT, int NMAX, int RG, bool LS, typename PT, bool SF = true>
