@@ -588,6 +588,7 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
   };
   publish(0, 0);
   __syncthreads();
+  int fail = 0;  // first failed pivot's status (wave-uniform)
   for (int k = 0; k < n; ++k) {
     const int par = k & 1;
     T cr[MAXR], ci[MAXR];
@@ -597,16 +598,11 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
       ci[m] = b.coli[par][g + RG * m];
     }
     const Pivot<T> pk = b.piv[par];
+    // a failed pivot (published with p = 0) is remembered and the step goes on; the
+    // values are discarded (no exit from the loop, see gj_lean)
     const int flag = __builtin_amdgcn_readfirstlane(pk.flag);
-    if (flag != ST_OK) {
-      if constexpr (LS) {
-        for (int r = k; r <= n; ++r) __syncthreads();
-      } else {
-        __syncthreads();
-      }
-      return flag;
-    }
-    const int p = __builtin_amdgcn_readfirstlane(pk.p);
+    fail = (fail == ST_OK && flag != ST_OK) ? flag : fail;
+    const int p = flag != ST_OK ? 0 : __builtin_amdgcn_readfirstlane(pk.p);
     if constexpr (SF) {
       // row p sits in register slot mp of the group lane gp: both are wave-uniform, so
       // the slot is chosen by scalar branches (uniform_branch keeps them branches
@@ -682,7 +678,7 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
     if (k + 1 < n) publish(k + 1, par ^ 1);
     __syncthreads();
   }
-  if (on && j < n) {
+  if (on && j < n && fail == ST_OK) {
     const int pj = b.pv[j];
 #pragma unroll
     for (int m = 0; m < MAXR; ++m) {
@@ -695,7 +691,7 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
     }
   }
   __syncthreads();
-  return ST_OK;
+  return fail;
 }
 
 template <typename T, int NT, int NMAX, int RG, typename PT>
