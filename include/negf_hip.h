@@ -49,9 +49,10 @@ extern "C" {
 /* Mixed precision: the FP32 decimation also stops (status 1, FP64 redo) from doubling
  * STALL_FROM on when one doubling shrinks max(|alpha|, |beta|) by less than STALL_RATIO.
  * The ratio of doubling k is about |lambda|^(2^(k-1)) for the slowest mode, so a ratio
- * above 0.9 at doubling 9 means 1 - |lambda| < 4e-4: the decimation would need more
- * than 17 doublings to reach tol 1e-12, beyond NEGF_MIXED_FP32_MAXITER anyway. */
-#define NEGF_MIXED_FP32_STALL_FROM 9
+ * above 0.9 at doubling 7 means 1 - |lambda| < 1.6e-3: the decimation would need at
+ * least 15 of its NEGF_MIXED_FP32_MAXITER doublings to reach tol 1e-12, and such
+ * instances (in-band modes at small eta) are the ones the FP64 certificate rejects. */
+#define NEGF_MIXED_FP32_STALL_FROM 7
 #define NEGF_MIXED_FP32_STALL_RATIO 0.9
 
 enum { NEGF_FP64 = 0, NEGF_MIXED = 1 };
