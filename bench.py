@@ -38,7 +38,7 @@ METRIC = "(ω,k) transmission solves/s, mixed vs FP64 GPU path at ≤1e-6 rel er
 PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}  # MI355X dense matrix peaks (FP64, FP32)
 # HBM bytes per launch of the FP64 kernels on this workload, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE scaled by a copy-kernel calibration (profiles/ribbon24_hbm_counters.md)
-TRAFFIC: dict = {"sweep": 379.6e6, "sancho_rubio": 93.4e6}
+TRAFFIC: dict = {"sweep": 347.0e6, "sancho_rubio": 85.3e6}
 
 
 def _cpu_worker(args):
