@@ -328,8 +328,8 @@ __device__ __forceinline__ void crecip1(T ar, T ai, T& rr, T& ri) {
 // of the same n side by side, so every sub-block must pass the same n + 2
 // barriers: a failed pivot then keeps stepping through the remaining barriers
 // instead of returning early (gj_inverse_ls).
-// The same elimination with a shorter pivot step, for the workgroup classes (at most
-// four rows per lane, column groups of 4, 8 or 16 lanes).  A step's cost is the
+// The same elimination with a shorter pivot step, for the workgroup classes and the
+// pooled class (at most twelve rows per lane, column groups of 4, 8 or 16 lanes).  A step's cost is the
 // instruction stream of the wave that publishes the next column, so that stream is
 // cut to the essentials:
 //  - the pivot is the largest |re| + |im| compared to 2^-14 relative (2^-13 above 64
@@ -515,9 +515,9 @@ __device__ __forceinline__ int gj_core(PT __restrict__ re, PT __restrict__ im, i
   constexpr int MAXR = NMAX / RG;
   static_assert(NMAX * RG <= NT, "not enough threads for the inversion layout");
   static_assert(MAXR <= 32, "row mask is 32 bits");
-  // (the pooled class, twelve rows per lane, keeps this step: in the lean one its rows
-  // and the published column do not fit the registers together and spill)
-  if constexpr (SF && MAXR <= 4 && (RG == 4 || RG == 8 || RG == 16) && NMAX <= 128 &&
+  // (the pooled class, twelve rows per lane, takes the short step too since its loop
+  // has no exit: its decimation spills 44 VGPRs there against 73 in this step)
+  if constexpr (SF && MAXR <= 12 && (RG == 4 || RG == 8 || RG == 16) && NMAX <= 128 &&
                 sizeof(GjLeanBuf<T, NMAX>) <= sizeof(GjBuf<T, NMAX>)) {
     return gj_lean<T, NT, NMAX, RG, LS>(re, im, ld, n,
                                        *reinterpret_cast<GjLeanBuf<T, NMAX>*>(&b), tid);
