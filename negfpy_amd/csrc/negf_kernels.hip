@@ -1715,11 +1715,61 @@ __host__ __device__ inline size_t adj_blocks(int n_layers, int ckpt) {
   return (size_t)n_layers + (size_t)((n_layers + ckpt - 1) / ckpt) + (size_t)ckpt + 1;
 }
 
-template <int NMAX, class L>
+// Ownership of the n x n results of the adjoint's block products: a TM x TM register
+// tile per thread with FMA loops on the vector ALUs (Tile, mac), or the matrix-core
+// tile-groups of mm.hpp (MOwn, mm: every lane of the workgroup takes part, and the LDS
+// operands must be zero outside [0, n)^2).  R owned elements per thread, element q at
+// (i, j) when at(q, i, j).
+template <int NMAX, bool MM>
+struct AdjOwn;
+template <int NMAX>
+struct AdjOwn<NMAX, false> {
+  static constexpr int TM = Cfg<NMAX>::TM, R = TM * TM;
+  Tile<TM> t;
+  int n;
+  __device__ AdjOwn(int n_, int tid) : t(n_, n_, tid), n(n_) {}
+  __device__ __forceinline__ bool at(int q, int& i, int& j) const {
+    i = t.i0 + q / TM;
+    j = t.j0 + q % TM;
+    return t.own && i < n && j < n;
+  }
+  template <int OPA, int OPB>
+  __device__ __forceinline__ void mul(const double* ar, const double* ai, const double* br,
+                                      const double* bi, int ld, double (&cr)[R],
+                                      double (&ci)[R]) const {
+    if (t.own)
+      mac<double, TM, OPA, OPB>(ar, ai, br, bi, ld, n, t.i0, t.j0,
+                                reinterpret_cast<double (&)[TM][TM]>(cr),
+                                reinterpret_cast<double (&)[TM][TM]>(ci));
+  }
+};
+template <int NMAX>
+struct AdjOwn<NMAX, true> {
+  using M = MOwn<NMAX, Cfg<NMAX>::NT>;
+  static constexpr int R = M::TPW;
+  M o;
+  int n, ns;
+  __device__ AdjOwn(int n_, int tid) : o(tid), n(n_), ns((n_ + 3) / 4) {}
+  __device__ __forceinline__ bool at(int q, int& i, int& j) const {
+    i = o.row(q);
+    j = o.col(q);
+    return i < n && j < n;
+  }
+  template <int OPA, int OPB>
+  __device__ __forceinline__ void mul(const double* ar, const double* ai, const double* br,
+                                      const double* bi, int ld, double (&cr)[R],
+                                      double (&ci)[R]) const {
+    mm<NMAX, Cfg<NMAX>::NT, OPA, OPB>(ar, ai, br, bi, ld, ns, o, cr, ci);
+  }
+};
+
+template <int NMAX, class L, bool MM>
 __device__ void adj_body(const AdjArgs& a, int inst, L& s, int tid) {
   using T = double;
-  constexpr int NT = Cfg<NMAX>::NT, TM = Cfg<NMAX>::TM;
+  constexpr int NT = Cfg<NMAX>::NT;
   constexpr int LD = L::LD;
+  using O = AdjOwn<NMAX, MM>;
+  constexpr int R = O::R;
   if (a.status_in && a.status_in[inst] != ST_OK) {
     if (tid == 0) {
       a.status[inst] = a.status_in[inst];
@@ -1747,60 +1797,76 @@ __device__ void adj_body(const AdjArgs& a, int inst, L& s, int tid) {
   const double w2 = 2.0 * (a.weight ? a.weight[inst] : 1.0);
   s.zero(tid);
   __syncthreads();
-  const Tile<TM> t(n, n, tid);
+  const O t(n, tid);
+  auto ozero = [](T (&xr)[R], T (&xi)[R]) {
+#pragma unroll
+    for (int q = 0; q < R; ++q) xr[q] = xi[q] = T(0);
+  };
+  auto put_lds = [&](T* re, T* im, const T (&xr)[R], const T (&xi)[R]) {
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      int i, j;
+      if (t.at(q, i, j)) {
+        re[i * LD + j] = xr[q];
+        im[i * LD + j] = xi[q];
+      }
+    }
+  };
+  auto put_glb = [&](double* g, const T (&xr)[R], const T (&xi)[R]) {
+    double2* d = reinterpret_cast<double2*>(g);
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      int i, j;
+      if (t.at(q, i, j)) d[(size_t)i * n + j] = make_double2(xr[q], xi[q]);
+    }
+  };
   // a gradient block: stored per instance, or added into its set's (instances of one set
   // run concurrently: atomic adds)
-  auto gput = [&](double* g, const T (&cr)[TM][TM], const T (&ci)[TM][TM]) {
+  auto gput = [&](double* g, const T (&cr)[R], const T (&ci)[R]) {
     if (!a.per_set) {
-      tstore_glb<T, TM>(g, n, n, t, cr, ci);
+      put_glb(g, cr, ci);
       return;
     }
-    if (!t.own) return;
 #pragma unroll
-    for (int x = 0; x < TM; ++x)
-#pragma unroll
-      for (int y = 0; y < TM; ++y) {
-        const int i = t.i0 + x, j = t.j0 + y;
-        if (i < n && j < n) {
-          atomicAdd(g + ((size_t)i * n + j) * 2, (double)cr[x][y]);
-          atomicAdd(g + ((size_t)i * n + j) * 2 + 1, (double)ci[x][y]);
-        }
+    for (int q = 0; q < R; ++q) {
+      int i, j;
+      if (t.at(q, i, j)) {
+        atomicAdd(g + ((size_t)i * n + j) * 2, cr[q]);
+        atomicAdd(g + ((size_t)i * n + j) * 2 + 1, ci[q]);
       }
+    }
   };
   double zr, zi;
   zsq(a.omega[inst], a.eta_dev[inst], zr, zi);
   // A_ii - S into LDS matrix 0
-  auto build = [&](int layer, const T (&sr)[TM][TM], const T (&si)[TM][TM]) {
-    if (!t.own) return;
+  auto build = [&](int layer, const T (&sr)[R], const T (&si)[R]) {
     const double2* d = ons + (size_t)layer * nn;
 #pragma unroll
-    for (int x = 0; x < TM; ++x)
-#pragma unroll
-      for (int y = 0; y < TM; ++y) {
-        const int i = t.i0 + x, j = t.j0 + y;
-        if (i < n && j < n) {
-          const double2 dv = d[i * n + j];
-          T vr = (i == j ? zr : 0.0) - dv.x - sr[x][y];
-          T vi = (i == j ? zi : 0.0) - dv.y - si[x][y];
-          if (layer == 0) {
-            vr -= sgl[i * n + j].x;
-            vi -= sgl[i * n + j].y;
-          }
-          if (layer == nl - 1) {
-            vr -= sgr[i * n + j].x;
-            vi -= sgr[i * n + j].y;
-          }
-          s.re[0][i * LD + j] = vr;
-          s.im[0][i * LD + j] = vi;
+    for (int q = 0; q < R; ++q) {
+      int i, j;
+      if (t.at(q, i, j)) {
+        const double2 dv = d[i * n + j];
+        T vr = (i == j ? zr : 0.0) - dv.x - sr[q];
+        T vi = (i == j ? zi : 0.0) - dv.y - si[q];
+        if (layer == 0) {
+          vr -= sgl[i * n + j].x;
+          vi -= sgl[i * n + j].y;
         }
+        if (layer == nl - 1) {
+          vr -= sgr[i * n + j].x;
+          vi -= sgr[i * n + j].y;
+        }
+        s.re[0][i * LD + j] = vr;
+        s.im[0][i * LD + j] = vi;
       }
+    }
   };
   int status = ST_OK;
-  T sr[TM][TM], si[TM][TM];
-  T cr[TM][TM], ci[TM][TM];
+  T sr[R], si[R];
+  T cr[R], ci[R];
 
   // 1. right-connected sweep: gR_i, R_i = C_{i-1} gR_i (stored), ends with gR_0 = G_11
-  tzero<T, TM>(sr, si);
+  ozero(sr, si);
   for (int i = nl - 1; i >= 0; --i) {
     build(i, sr, si);
     __syncthreads();
@@ -1812,13 +1878,13 @@ __device__ void adj_body(const AdjArgs& a, int inst, L& s, int tid) {
     if (i == 0) break;
     gload<T, NT>(cpl + (size_t)(i - 1) * nn * 2, n, n, s.re[2], s.im[2], LD, tid);
     __syncthreads();
-    tzero<T, TM>(cr, ci);
-    if (t.own) mac<T, TM, OPN, OPN>(s.re[2], s.im[2], s.re[0], s.im[0], LD, n, t.i0, t.j0, cr, ci);
-    tstore<T, TM>(s.re[3], s.im[3], LD, n, n, t, cr, ci);
-    tstore_glb<T, TM>(blk(vw, i), n, n, t, cr, ci);
+    ozero(cr, ci);
+    t.template mul<OPN, OPN>(s.re[2], s.im[2], s.re[0], s.im[0], LD, cr, ci);
+    put_lds(s.re[3], s.im[3], cr, ci);
+    put_glb(blk(vw, i), cr, ci);
     __syncthreads();
-    tzero<T, TM>(sr, si);
-    if (t.own) mac<T, TM, OPN, OPH>(s.re[3], s.im[3], s.re[2], s.im[2], LD, n, t.i0, t.j0, sr, si);
+    ozero(sr, si);
+    t.template mul<OPN, OPH>(s.re[3], s.im[3], s.re[2], s.im[2], LD, sr, si);
     __syncthreads();
   }
   if (status != ST_OK) {
@@ -1829,30 +1895,26 @@ __device__ void adj_body(const AdjArgs& a, int inst, L& s, int tid) {
     return;
   }
   // 2. row 1 of G: v_0 = G_11 (matrix 0), v_i = v_{i-1} R_i
-  tzero<T, TM>(cr, ci);
-  if (t.own) {
+  ozero(cr, ci);
 #pragma unroll
-    for (int x = 0; x < TM; ++x)
-#pragma unroll
-      for (int y = 0; y < TM; ++y) {
-        const int r = t.i0 + x, c = t.j0 + y;
-        if (r < n && c < n) {
-          cr[x][y] = s.re[0][r * LD + c];
-          ci[x][y] = s.im[0][r * LD + c];
-        }
-      }
+  for (int q = 0; q < R; ++q) {
+    int r, c;
+    if (t.at(q, r, c)) {
+      cr[q] = s.re[0][r * LD + c];
+      ci[q] = s.im[0][r * LD + c];
+    }
   }
-  tstore_glb<T, TM>(blk(vw, 0), n, n, t, cr, ci);
+  put_glb(blk(vw, 0), cr, ci);
   __threadfence_block();
   __syncthreads();
   for (int i = 1; i < nl; ++i) {
     bload<T, NT>(blk(vw, i), n, n, s.re[1], s.im[1], LD, tid);
     __syncthreads();
-    tzero<T, TM>(cr, ci);
-    if (t.own) mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[1], s.im[1], LD, n, t.i0, t.j0, cr, ci);
+    ozero(cr, ci);
+    t.template mul<OPN, OPN>(s.re[0], s.im[0], s.re[1], s.im[1], LD, cr, ci);
     __syncthreads();
-    tstore<T, TM>(s.re[0], s.im[0], LD, n, n, t, cr, ci);
-    tstore_glb<T, TM>(blk(vw, i), n, n, t, cr, ci);
+    put_lds(s.re[0], s.im[0], cr, ci);
+    put_glb(blk(vw, i), cr, ci);
     __threadfence_block();
     __syncthreads();
   }
@@ -1868,34 +1930,30 @@ __device__ void adj_body(const AdjArgs& a, int inst, L& s, int tid) {
     s.im[2][i * LD + j] = b1.x - b2.x;
   }
   __syncthreads();
-  tzero<T, TM>(cr, ci);
-  if (t.own) mac<T, TM, OPH, OPN>(s.re[0], s.im[0], s.re[1], s.im[1], LD, n, t.i0, t.j0, cr, ci);
-  tstore<T, TM>(s.re[3], s.im[3], LD, n, n, t, cr, ci);
+  ozero(cr, ci);
+  t.template mul<OPH, OPN>(s.re[0], s.im[0], s.re[1], s.im[1], LD, cr, ci);
+  put_lds(s.re[3], s.im[3], cr, ci);
   __syncthreads();
-  tzero<T, TM>(cr, ci);
-  if (t.own) mac<T, TM, OPN, OPN>(s.re[2], s.im[2], s.re[3], s.im[3], LD, n, t.i0, t.j0, cr, ci);
+  ozero(cr, ci);
+  t.template mul<OPN, OPN>(s.re[2], s.im[2], s.re[3], s.im[3], LD, cr, ci);
   double acc = 0.0;
-  if (t.own) {
 #pragma unroll
-    for (int x = 0; x < TM; ++x)
-#pragma unroll
-      for (int y = 0; y < TM; ++y) {
-        const int r = t.i0 + x, c = t.j0 + y;
-        if (r < n && c < n) acc += cr[x][y] * s.re[0][c * LD + r] - ci[x][y] * s.im[0][c * LD + r];
-      }
+  for (int q = 0; q < R; ++q) {
+    int r, c;
+    if (t.at(q, r, c)) acc += cr[q] * s.re[0][c * LD + r] - ci[q] * s.im[0][c * LD + r];
   }
   const double tval = block_sum<NT>(acc, s.sred, tid);
   if (tid == 0) a.t_out[inst] = tval;
-  tstore<T, TM>(s.re[1], s.im[1], LD, n, n, t, cr, ci);  // W
+  put_lds(s.re[1], s.im[1], cr, ci);  // W
   __syncthreads();
   // 4. P_i = W v_i, in place
   for (int i = 0; i < nl; ++i) {
     bload<T, NT>(blk(vw, i), n, n, s.re[2], s.im[2], LD, tid);
     __syncthreads();
-    tzero<T, TM>(cr, ci);
-    if (t.own) mac<T, TM, OPN, OPN>(s.re[1], s.im[1], s.re[2], s.im[2], LD, n, t.i0, t.j0, cr, ci);
+    ozero(cr, ci);
+    t.template mul<OPN, OPN>(s.re[1], s.im[1], s.re[2], s.im[2], LD, cr, ci);
     __syncthreads();
-    tstore_glb<T, TM>(blk(vw, i), n, n, t, cr, ci);
+    put_glb(blk(vw, i), cr, ci);
   }
   __threadfence_block();
   __syncthreads();
@@ -1909,19 +1967,19 @@ __device__ void adj_body(const AdjArgs& a, int inst, L& s, int tid) {
     if (st != ST_OK || i == nl - 1) return st;
     gload<T, NT>(cpl + (size_t)i * nn * 2, n, n, s.re[2], s.im[2], LD, tid);
     __syncthreads();
-    tzero<T, TM>(cr, ci);
-    if (t.own) mac<T, TM, OPN, OPN>(s.re[0], s.im[0], s.re[2], s.im[2], LD, n, t.i0, t.j0, cr, ci);
-    tstore<T, TM>(s.re[3], s.im[3], LD, n, n, t, cr, ci);
-    if (ydst) tstore_glb<T, TM>(ydst, n, n, t, cr, ci);
+    ozero(cr, ci);
+    t.template mul<OPN, OPN>(s.re[0], s.im[0], s.re[2], s.im[2], LD, cr, ci);
+    put_lds(s.re[3], s.im[3], cr, ci);
+    if (ydst) put_glb(ydst, cr, ci);
     __syncthreads();
-    tzero<T, TM>(sr, si);
-    if (t.own) mac<T, TM, OPH, OPN>(s.re[2], s.im[2], s.re[3], s.im[3], LD, n, t.i0, t.j0, sr, si);
+    ozero(sr, si);
+    t.template mul<OPH, OPN>(s.re[2], s.im[2], s.re[3], s.im[3], LD, sr, si);
     __syncthreads();
     return st;
   };
-  tzero<T, TM>(sr, si);
+  ozero(sr, si);
   for (int i = 0; i < nl; ++i) {
-    if (kc > 0 && i % kc == 0) tstore_glb<T, TM>(blk(yw, i / kc), n, n, t, sr, si);
+    if (kc > 0 && i % kc == 0) put_glb(blk(yw, i / kc), sr, si);
     const int st = left_step(i, kc > 0 ? nullptr : blk(yw, i));
     if (st != ST_OK) {
       status = 10 + st;
@@ -1943,21 +2001,19 @@ __device__ void adj_body(const AdjArgs& a, int inst, L& s, int tid) {
   __syncthreads();
   bload<T, NT>(blk(vw, nl - 1), n, n, p1r, p1i, LD, tid);
   __syncthreads();
-  tzero<T, TM>(cr, ci);
-  if (t.own) mac<T, TM, OPH, OPH>(p1r, p1i, u1r, u1i, LD, n, t.i0, t.j0, cr, ci);
+  ozero(cr, ci);
+  t.template mul<OPH, OPH>(p1r, p1i, u1r, u1i, LD, cr, ci);
 #pragma unroll
-  for (int x = 0; x < TM; ++x)
-#pragma unroll
-    for (int y = 0; y < TM; ++y) {
-      cr[x][y] *= w2;
-      ci[x][y] *= w2;
-    }
-  auto gput_onsite = [&](int i, const T (&cr)[TM][TM], const T (&ci)[TM][TM]) {
+  for (int q = 0; q < R; ++q) {
+    cr[q] *= w2;
+    ci[q] *= w2;
+  }
+  auto gput_onsite = [&](int i, const T (&cr)[R], const T (&ci)[R]) {
     gput(blk(go, i), cr, ci);
     if (a.grad_ends_out) {
       double* e = a.grad_ends_out + (size_t)inst * 2 * nn * 2;
-      if (i == 0) tstore_glb<T, TM>(e, n, n, t, cr, ci);
-      if (i == nl - 1) tstore_glb<T, TM>(e + nn * 2, n, n, t, cr, ci);
+      if (i == 0) put_glb(e, cr, ci);
+      if (i == nl - 1) put_glb(e + nn * 2, cr, ci);
     }
   };
   gput_onsite(nl - 1, cr, ci);
@@ -1971,18 +2027,15 @@ __device__ void adj_body(const AdjArgs& a, int inst, L& s, int tid) {
       gstore<T, NT>(usave, n, u1r, u1i, LD, tid);
       __threadfence_block();
       __syncthreads();
-      const Tile<TM>& tt = t;
-      if (tt.own) {
+      {
         const double2* ck = reinterpret_cast<const double2*>(blk(yw, i / kc));
 #pragma unroll
-        for (int x = 0; x < TM; ++x)
-#pragma unroll
-          for (int y = 0; y < TM; ++y) {
-            const int r = tt.i0 + x, c = tt.j0 + y;
-            const double2 v = (r < n && c < n) ? ck[r * n + c] : make_double2(0.0, 0.0);
-            sr[x][y] = v.x;
-            si[x][y] = v.y;
-          }
+        for (int q = 0; q < R; ++q) {
+          int r, c;
+          const double2 v = t.at(q, r, c) ? ck[r * n + c] : make_double2(0.0, 0.0);
+          sr[q] = v.x;
+          si[q] = v.y;
+        }
       }
       for (int l = seg_a; l <= i; ++l) left_step(l, blk(ybuf, l - seg_a));
       __threadfence_block();
@@ -2002,35 +2055,29 @@ __device__ void adj_body(const AdjArgs& a, int inst, L& s, int tid) {
     bload<T, NT>(kc > 0 ? blk(ybuf, i - seg_a) : blk(yw, i), n, n, f0r, f0i, LD, tid);  // Y_i
     bload<T, NT>(blk(vw, i), n, n, f1r, f1i, LD, tid);  // P_i
     __syncthreads();
-    tzero<T, TM>(sr, si);  // u_i = Y_i u_{i+1}
-    if (t.own) mac<T, TM, OPN, OPN>(f0r, f0i, u1r, u1i, LD, n, t.i0, t.j0, sr, si);
+    ozero(sr, si);  // u_i = Y_i u_{i+1}
+    t.template mul<OPN, OPN>(f0r, f0i, u1r, u1i, LD, sr, si);
     __syncthreads();
-    tstore<T, TM>(f0r, f0i, LD, n, n, t, sr, si);  // f0 = u_i
+    put_lds(f0r, f0i, sr, si);  // f0 = u_i
     __syncthreads();
     // coupling i: (u_{i+1} P_i)^H + u_i P_{i+1} = P_i^H u_{i+1}^H + u_i P_{i+1}
-    tzero<T, TM>(cr, ci);
-    if (t.own) {
-      mac<T, TM, OPH, OPH>(f1r, f1i, u1r, u1i, LD, n, t.i0, t.j0, cr, ci);
-      mac<T, TM, OPN, OPN>(f0r, f0i, p1r, p1i, LD, n, t.i0, t.j0, cr, ci);
+    ozero(cr, ci);
+    t.template mul<OPH, OPH>(f1r, f1i, u1r, u1i, LD, cr, ci);
+    t.template mul<OPN, OPN>(f0r, f0i, p1r, p1i, LD, cr, ci);
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      cr[q] *= w2;
+      ci[q] *= w2;
     }
-#pragma unroll
-    for (int x = 0; x < TM; ++x)
-#pragma unroll
-      for (int y = 0; y < TM; ++y) {
-        cr[x][y] *= w2;
-        ci[x][y] *= w2;
-      }
     gput(blk(gc, i), cr, ci);
     // onsite i: (u_i P_i)^H = P_i^H u_i^H
-    tzero<T, TM>(cr, ci);
-    if (t.own) mac<T, TM, OPH, OPH>(f1r, f1i, f0r, f0i, LD, n, t.i0, t.j0, cr, ci);
+    ozero(cr, ci);
+    t.template mul<OPH, OPH>(f1r, f1i, f0r, f0i, LD, cr, ci);
 #pragma unroll
-    for (int x = 0; x < TM; ++x)
-#pragma unroll
-      for (int y = 0; y < TM; ++y) {
-        cr[x][y] *= w2;
-        ci[x][y] *= w2;
-      }
+    for (int q = 0; q < R; ++q) {
+      cr[q] *= w2;
+      ci[q] *= w2;
+    }
     gput_onsite(i, cr, ci);
     __syncthreads();
     // u_{i+1} <- u_i (f0), P_{i+1} <- P_i (f1); the old ones become free
@@ -2052,8 +2099,9 @@ __device__ void adj_body(const AdjArgs& a, int inst, L& s, int tid) {
 template <int NMAX>
 __global__ void __launch_bounds__(Cfg<NMAX>::NT) k_adjoint(AdjArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  Lds<double, NMAX, Cfg<NMAX>::NT, 4> s(smem);
-  adj_body<NMAX>(a, blockIdx.x, s, threadIdx.x);
+  using L = Lds<double, NMAX, Cfg<NMAX>::NT, 4>;
+  L s(smem);
+  adj_body<NMAX, L, true>(a, blockIdx.x, s, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -2661,7 +2709,7 @@ __global__ void __launch_bounds__(BIG_NT) k_adjoint_big(AdjArgs a, double* pool)
   __shared__ GmShared<double, BIG, BIG_NT, 4> sh;
   GreenBig s(pool + (size_t)blockIdx.x * BIG_SLOT, &sh);
   for (int inst = blockIdx.x; inst < a.n_inst; inst += gridDim.x) {
-    adj_body<BIG>(a, inst, s, threadIdx.x);
+    adj_body<BIG, GreenBig, false>(a, inst, s, threadIdx.x);
     __syncthreads();
   }
 }

@@ -106,7 +106,7 @@ def _dense_grads(p):
                                               x["lead_l"][k], x["lead_r"][k], x["v_l"][k],
                                               x["v_r"][k])
     tot.backward()
-    return float(tot), {k: x[k].grad.numpy() for k in NAMES if x[k].grad is not None}
+    return float(tot.detach()), {k: x[k].grad.numpy() for k in NAMES if x[k].grad is not None}
 
 
 def _gpu_grads(p, one_pass=False):
