@@ -119,7 +119,7 @@ def _gpu_grads(p, one_pass=False):
         t = transmission_sets_torch(*args, eta=p["eta"])
         tot = (torch.as_tensor(p["weight"], device="cuda") * t).sum()
     tot.backward()
-    return float(tot), {k: x[k].grad.cpu().numpy() for k in NAMES if x[k].grad is not None}
+    return float(tot.detach()), {k: x[k].grad.cpu().numpy() for k in NAMES if x[k].grad is not None}
 
 
 def _rel(a, b):
