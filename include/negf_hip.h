@@ -311,9 +311,11 @@ int negf_merge_status(void* stream, int n_inst, int32_t* st_l, const int32_t* st
  * and T (as negf_transmission_trace_batched), with every block product on the matrix
  * cores.  Replaces the self-energies, the solve and the trace of
  * src/negfpy/core/negf.py:306-357 (with :115-116) in one kernel.  Fused for FP64
- * workgroup classes 8..48 with matrix-core products (negf_set_matrix_core_products 1)
- * and leads no larger than the device block's class; otherwise the four kernels run
- * in turn (and status_l then holds the merged lead status).
+ * workgroup classes 8..48 with matrix-core products (negf_set_matrix_core_products 1):
+ * the launch takes the class of the largest of the device block and the two lead
+ * blocks.  Otherwise (a block above 48 DOF, vector-ALU products, the one-wavefront
+ * family) the four kernels run in turn, and status_l then holds the merged lead
+ * status.
  *   g_l, g_r [n_inst][n_lead][n_lead]  surface GFs; v_*, idx_*: as negf_self_energy_batched
  *   status_l, status_r [n_inst]        lead statuses, merged left first into status
  *   sigma_l, sigma_r, g1n_out [n_inst][n][n]   written

@@ -1,25 +1,33 @@
 // Batched phonon-NEGF transport kernels for MI355X (gfx950) and their C ABI.
 //
 // One workgroup solves one (omega, k_par) instance; all per-instance matrices
-// are held in LDS in planar complex form (cmat.hpp).  Kernels:
+// are held in LDS in planar complex form (cmat.hpp), except for the pooled 96-DOF
+// class (big.hpp), whose matrices sit in a device-memory slot per workgroup.  Kernels:
 //
-//   k_sancho_rubio<T>  Lopez-Sancho decimation of a semi-infinite lead
+//   k_sancho_rubio<T> / k_sr_mm   Lopez-Sancho decimation of a semi-infinite lead
 //                      (reference src/negfpy/core/surface_gf.py:18-81).  In FP64
 //                      mode it ends with g_s = (z - e_s)^-1 and the surface DOS;
 //                      in FP32 mode it hands e_s to k_refine.
-//   k_refine           FP64 g_s = (z - e_s)^-1 followed by fixed-point residual
-//                      steps g <- (z - d00 - d01 g d10)^-1 and the surface DOS.
+//   k_refine           FP64 g_s = (z - e_s)^-1, then Newton steps on
+//                      g = (z - d00 - d01 g d10)^-1 (correction from a Stein equation
+//                      solved in FP32 by doubling), certified by an FP64 residual;
+//                      uncertified instances are marked for the FP64 decimation.
 //   k_self_energy      Sigma = V g V^H embedded in the device end layer, FP64
 //                      (negf.py:72-97, :245-248).
-//   k_rgf<T>           block recursive Green's function sweep for G_{1N} of
+//   k_rgf<T> / k_rgf_mm  block recursive Green's function sweep for G_{1N} of
 //                      A = z - D - Sigma_L - Sigma_R (replaces the sparse LU of
-//                      negf.py:350-354; same matrix, same G_{1N}).
+//                      negf.py:350-354; same matrix, same G_{1N}); k_rgf_mm<NMAX, true>
+//                      forms Sigma_L, Sigma_R and the trace in the same launch.
 //   k_trace            T = Re Tr[Gamma_L G_1N Gamma_R G_1N^H], Gamma = i(S - S^H), FP64
 //                      (negf.py:115-116, :356-357).
+//   k_adjoint          T and dT/d(every device block) by one forward and one adjoint
+//                      sweep; k_lead_adjoint carries dT/dSigma into the lead blocks.
 //   k_conductance      Landauer thermal conductance per temperature, FP64
 //                      (workflows/ifc_bulk.py:408-438).
 //   k_heat_current     finite-bias Landauer heat current, FP64
 //                      (core/thermal_current.py:30-67).
+// Block products run on the FP64 matrix cores (mm.hpp, v_mfma_f64_4x4x4f64; big.hpp,
+// v_mfma_f64_16x16x4f64) or as register-tiled FMA loops (cmat.hpp mac).
 
 #include <hip/hip_runtime.h>
 #include <math.h>

@@ -3,7 +3,8 @@ trace in one launch (k_rgf_mm<NMAX, true>) against the four separate kernels
 (negf_self_energy_batched x 2, negf_merge_status, negf_rgf_g1n_batched,
 negf_transmission_trace_batched) on the same synthetic inputs, for every FP64 workgroup
 class, contact subsets, leads larger and smaller than the device block, one layer, lead
-failures, and the fallback forms (vector-ALU products, a lead that exceeds the class).
+failures, the fallback form (vector-ALU products) and a lead larger than the device
+block (the launch takes the lead's class).
 The physics of the composition is pinned against the oracle by the transmission parity
 tests, which run through this entry."""
 
@@ -147,14 +148,15 @@ def test_fused_equals_separate_kernels(lib, case):
 
 
 def test_fallback_forms(lib):
-    """Vector-ALU products and a lead larger than the device block's class run the four
-    kernels; their results equal the fused form's to rounding."""
+    """Vector-ALU products run the four kernels, whose results equal the fused form's to
+    rounding; a lead larger than the device block's class runs fused in the lead's class
+    and equals the four kernels."""
     c = _case(7, 24, 6, 24, 24, 24, 24)
     fused = _fused(lib, c)
     assert lib.negf_set_matrix_core_products(0) == 1
     _compare(_fused(lib, c), fused, tol=1e-10)
     assert lib.negf_set_matrix_core_products(1) == 0
-    big = _case(8, 24, 4, 30, 24, 24, 24)   # 30-DOF lead: class 32 > the block's 24
+    big = _case(8, 24, 4, 30, 24, 24, 24)   # 30-DOF lead: class 32, the block's is 24
     _compare(_fused(lib, big), _separate(lib, big))
 
 
