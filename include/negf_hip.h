@@ -85,9 +85,12 @@ int negf_max_block_green(void);
  * workgroup size, rows per lane, tile size and LDS footprint of every kernel) used
  * for blocks of n DOF in the given precision.  Class 96 keeps the matrices of an
  * instance in a device-memory pool the library allocates on first use on a device
- * (about 1.3 MB per resident workgroup) and keeps until the process ends; calls on
- * different streams are ordered so they never share it at once, and it cannot be
- * used under stream capture.  Any class >= n is valid; nmax = 0 restores the default (the
+ * (about 1.3 MB per resident workgroup) and keeps until the process ends; eager calls
+ * on different streams are ordered so they never share it at once.  Under stream
+ * capture the pool must already exist (one eager call on the device first, else
+ * NEGF_ERR_ARG); a captured call is ordered only by its own stream, so a graph that
+ * holds class-96 work must not be replayed while class-96 work runs on another
+ * stream.  Any class >= n is valid; nmax = 0 restores the default (the
  * smallest class >= n).  Set by the autotuner (negfpy_amd.core.autotune); the
  * table is process-wide and thread-safe.  No counterpart in the reference, whose
  * dense/sparse solves have no launch configuration.

@@ -2801,8 +2801,11 @@ static int pick_nmax(int n, bool fp64) {
 }
 
 // Device-memory pool of the 96 class: one slot per resident workgroup, allocated on
-// first use on a device and kept for the life of the process.  Calls on different
-// streams are ordered through an event, so two calls never use the slots at once.
+// first use on a device and kept for the life of the process.  Eager calls on
+// different streams are ordered through an event, so two calls never use the slots at
+// once.  A call on a capturing stream neither waits on nor records that event (an
+// event of eager work cannot order a graph node): it only needs the pool to exist,
+// and the graph's replays are ordered by the stream they run on.
 struct BigPool {
   double* ptr = nullptr;
   int slots = 0;
@@ -2824,6 +2827,16 @@ class BigUse {
       return;
     }
     p_ = &table[dev];
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    rc = hip_check(hipStreamIsCapturing(st_, &cs), "stream capture status");
+    if (rc) return;
+    capturing_ = cs != hipStreamCaptureStatusNone;
+    if (!p_->ptr && capturing_) {
+      rc = fail(NEGF_ERR_ARG,
+                "the 49..96 DOF class allocates its device pool on its first call on a "
+                "device, which cannot be captured: make one call outside capture first");
+      return;
+    }
     if (!p_->ptr) {
       int cus = 0, per = 0;
       rc = hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
@@ -2844,12 +2857,14 @@ class BigUse {
       if (rc) return;
       p_->ptr = ptr;
       p_->slots = slots;
-    } else {
+    } else if (!capturing_) {
       rc = hip_check(hipStreamWaitEvent(st_, p_->done, 0), "pool order");
     }
   }
   // marks the end of this call's use of the pool on its stream
-  int finish() { return hip_check(hipEventRecord(p_->done, st_), "pool event record"); }
+  int finish() {
+    return capturing_ ? NEGF_OK : hip_check(hipEventRecord(p_->done, st_), "pool event record");
+  }
   double* pool() const { return p_->ptr; }
   int grid(int n_inst) const { return n_inst < p_->slots ? n_inst : p_->slots; }
   int rc = NEGF_OK;
@@ -2858,6 +2873,7 @@ class BigUse {
   hipStream_t st_;
   std::lock_guard<std::mutex> lk_;
   BigPool* p_ = nullptr;
+  bool capturing_ = false;
 };
 
 #define NEGF_POOL_LAUNCH(KERNEL, STREAM, ARGS, TABLE, SLOT)                                 \

@@ -65,3 +65,35 @@ def test_graph_replays_match_eager(gpu, precision):
             tr = o.transmission_sparse(w2[k], dev.onsite_blocks, dev.coupling_blocks, tup, tup,
                                        eta=1e-6)
             assert abs(t2[k] - tr) <= max(T_TOL * max(abs(tr), 1e-3), 1e-8 * max(abs(tr), 1e-3))
+
+
+def test_graph_pooled_class(gpu):
+    """Blocks of 49..96 DOF (the class whose matrices sit in a device-memory pool) in a
+    captured solve: the eager run before capture creates the pool, and every replay
+    equals the eager result, with eager solves of the same class on the current stream
+    between the replays."""
+    sets, lead, dev = _sets(n_atoms=20, nl=3)
+    assert sets.n == 60
+    solver = _engine.TransmissionSolver(sets)
+    m = 48
+    w = np.linspace(0.05, 1.05, m) * omega_max(lead)
+    set_of = torch.zeros(m, dtype=torch.int32, device="cuda")
+    eta = torch.full((m,), 1e-6, dtype=torch.float64, device="cuda")
+    omega = torch.as_tensor(w, device="cuda")
+    eager = solver.run(set_of, omega, eta)
+    t_eager = eager.t.cpu().numpy().copy()
+    st_eager = eager.status.cpu().numpy().copy()
+    g = _engine.TransmissionGraph(solver, set_of, omega, eta)
+    for r in range(3):
+        other = solver.run(set_of, omega * 0.7, eta)
+        _ = (other.t * 2.0).cpu()
+        out = g.replay()
+        torch.cuda.synchronize()
+        print(f"replay {r}: T[:3] {out.t[:3].tolist()}", flush=True)
+        assert np.array_equal(out.status.cpu().numpy(), st_eager), r
+        np.testing.assert_array_equal(out.t.cpu().numpy(), t_eager)
+    tup = (lead.d00, lead.d01, None)
+    for k in (5, 30):
+        tr = o.transmission_sparse(w[k], dev.onsite_blocks, dev.coupling_blocks, tup, tup,
+                                   eta=1e-6)
+        assert abs(t_eager[k] - tr) <= max(T_TOL * max(abs(tr), 1e-3), 1e-8 * max(abs(tr), 1e-3))
