@@ -1,6 +1,6 @@
 """Mixed precision (FP32 decimation, FP64 Newton completion with FP32 Stein
 corrections and a certificate, FP64 decimation of what it does not certify; FP64
-sweep, self-energies, Gamma and trace) against the FP64 oracle at the north star's
+sweep, self-energies, Gamma and trace) against the FP64 oracle at the mixed path's
 bar: <= 1e-6 relative error in T (relative to max(|T|, 1e-3)), DOS and kappa(300 K)."""
 
 import os
